@@ -1,0 +1,178 @@
+"""Benchmark: FP32 FDTD time-stepping throughput on one MI355X.
+
+Workload (default): plain-FP32 stress grid, 256x256x256 cells of 1 mm, 8-cell
+UPML on all six faces, 40 seeded dielectric boxes and 10 PEC plates, a 50-ohm
+lumped port driven by a Gaussian pulse (f0 5 GHz, fc 4 GHz), field energy
+every 100 steps and a six-face NF2FF box (339,864 surface points) at 16
+frequencies in 1-9 GHz.  A "step" is one complete FDTD timestep of that
+workload (E sweep, H sweep, UPML, excitation, NF2FF sampling and its DFT fold,
+and the periodic energy reduction), with all inputs resident in HBM.
+
+Prints one JSON line.  MCells/s = Nx*Ny*Nz*steps / wall seconds / 1e6.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+import numpy as np  # noqa: E402
+
+METRIC = "MCells/s & %HBM roofline: plain FP32 vs batched/mixed at fixed S11 error"
+HBM_PEAK_GBS = 8000.0
+
+
+def sweep_bytes(n_cells, n_pml):
+    """Algorithmic HBM bytes of one E (or H) sweep launch.
+    Per cell: read V, vv, vi, H (3 components each) and write V = 15 floats
+    = 60 B.  Per UPML cell in addition: flux read + write and three UPML
+    coefficients for 3 components = 60 B."""
+    return 60.0 * n_cells + 60.0 * n_pml
+
+
+def cpu_baseline(w, seconds_target=15.0):
+    """Time the CPU oracle (all host threads, f32 variant) on a bounded
+    sample of the same workload: the same grid, coefficients, UPML and
+    excitation; NF2FF and energy are not part of the sample."""
+    from oracle import oracle as orc
+    threads = os.cpu_count() or 1
+    try:
+        threads = len(os.sched_getaffinity(0))
+    except AttributeError:
+        pass
+    threads = min(threads, 16)
+    orc.Oracle.set_threads(threads)
+    o = orc.Oracle(w["gridSize"], w["coefficients"])
+    o.configurePML(w["pmlRegions"])
+    o.configureExcitation(w["excitation"])
+    o.iterate(1)  # first touch
+    n = int(np.prod(w["gridSize"]))
+    steps, elapsed = 0, 0.0
+    chunk = 2
+    while elapsed < seconds_target:
+        t0 = time.perf_counter()
+        o.iterate(chunk)
+        elapsed += time.perf_counter() - t0
+        steps += chunk
+        chunk = min(chunk * 2, 64)
+    del o
+    return {"value": n * steps / elapsed / 1e6, "unit": "MCells/s", "cores": threads, "kind": "port",
+            "sample": f"{steps} steps of the same {w['gridSize'][0]}^3 UPML+excitation workload "
+                      f"({elapsed:.1f} s) with the C oracle (float storage, double arithmetic, OpenMP); "
+                      f"NF2FF and energy excluded"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=2000)
+    ap.add_argument("--warmup", type=int, default=100)
+    ap.add_argument("--size", type=int, default=256)
+    ap.add_argument("--energy-interval", type=int, default=100)
+    ap.add_argument("--profile-steps", type=int, default=200)
+    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+
+    if args.gpus != 1:
+        print(json.dumps({"error": "this engine runs on one GPU per process; multi-GPU runs are not part "
+                                   "of this build (use --gpus 1)", "n_gpus": args.gpus}))
+        sys.exit(2)
+
+    from efs_amd import workloads
+    from efs_amd.engine import PROF_STEP, PROF_SWEEP_E, PROF_SWEEP_H, HIPEngine
+
+    t_build = time.perf_counter()
+    w = workloads.stress_grid(args.size)
+    t_build = time.perf_counter() - t_build
+    n = int(np.prod(w["gridSize"]))
+
+    eng = HIPEngine()
+    if not eng.initGPU():
+        raise RuntimeError("no HIP device")
+    workloads.configure_engine(eng, w)
+
+    eng.iterate(args.warmup)
+    eng.energy()
+    eng.sync()
+
+    K, E = args.steps, args.energy_interval
+    energies = []
+    t0 = time.perf_counter()
+    done = 0
+    while done < K:
+        k = min(E, K - done)
+        eng.iterate(k)
+        done += k
+        if done % E == 0:
+            energies.append(eng.energy())
+    eng.sync()
+    elapsed = time.perf_counter() - t0
+    mcells = n * K / elapsed / 1e6
+
+    # kernel durations from HIP events on the engine's stream
+    eng.profileEnable(True)
+    eng.iterate(args.profile_steps)
+    eng.sync()
+    ms_e, ce = eng.profileRead(PROF_SWEEP_E)
+    ms_h, ch = eng.profileRead(PROF_SWEEP_H)
+    ms_s, cs = eng.profileRead(PROF_STEP)
+    eng.profileEnable(False)
+    avg_e, avg_h, avg_s = ms_e / ce, ms_h / ch, ms_s / cs
+    b = sweep_bytes(n, w["n_pml_cells"])
+    gbs_e = b / (avg_e * 1e-3) / 1e9
+    gbs_h = b / (avg_h * 1e-3) / 1e9
+    dom_name, dom_ms = ("sweep_h", avg_h) if avg_h >= avg_e else ("sweep_e", avg_e)
+    dom_gbs = b / (dom_ms * 1e-3) / 1e9
+    acc = eng.readNF2FFAccumulation()
+    finite = bool(np.all(np.isfinite(acc["accumE"])) and np.all(np.isfinite(energies)))
+
+    base = None if args.no_cpu_baseline else cpu_baseline(w, args.cpu_seconds)
+    step_bytes = 2 * b
+    result = {
+        "metric": METRIC,
+        "value": round(mcells, 1),
+        "unit": "MCells/s",
+        "n_gpus": 1,
+        "steps": K,
+        "warmup": args.warmup,
+        "ms_per_step": round(elapsed / K * 1e3, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {
+            "workload": f"plain FP32 stress grid {args.size}^3, 1 mm cells, UPML-8 on 6 faces, 40 dielectric boxes, "
+                        f"10 PEC plates, 50-ohm lumped port, Gaussian f0=5 GHz fc=4 GHz, energy every {E} steps, "
+                        f"NF2FF 6-face box {w['nf2ff']['numPoints']} points x 16 frequencies (1-9 GHz)",
+            "grid": w["gridSize"], "pml_cells": w["n_pml_cells"], "dT": w["dT"], "seed": w["seed"],
+            "nf2ff_k_block": w["nf2ff"]["kBlock"], "parallelism": "single GPU",
+        },
+        "roofline": {
+            "bound": "hbm",
+            "kernel": dom_name,
+            "achieved": round(dom_gbs, 1),
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": round(dom_gbs / HBM_PEAK_GBS, 4),
+            "traffic": None,
+            "algorithmic_bytes_per_launch": b,
+            "avg_launch_ms": round(dom_ms, 5),
+            "kernels": {"sweep_e": {"avg_ms": round(avg_e, 5), "GB/s": round(gbs_e, 1)},
+                        "sweep_h": {"avg_ms": round(avg_h, 5), "GB/s": round(gbs_h, 1)},
+                        "step": {"avg_ms": round(avg_s, 5),
+                                 "sweeps_GB/s": round(step_bytes / (avg_s * 1e-3) / 1e9, 1)}},
+        },
+        "cpu_baseline": base,
+        "checks": {"finite": finite, "energy_last": energies[-1] if energies else None,
+                   "host_build_s": round(t_build, 1)},
+    }
+    print(json.dumps(result))
+
+
+if __name__ == "__main__":
+    main()

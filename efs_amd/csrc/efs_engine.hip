@@ -1,0 +1,998 @@
+// efs_engine.hip -- MI355X FDTD engine: context, step driver and C ABI.
+//
+// Implements include/efs_engine.h.  One context = one HIP stream on one
+// device; all buffers are device-resident in the reference's NIJK layout.
+// The step order is the reference CPU engine's (src/webgpu-fdtd.mjs:
+// 1094-1172) with the NF2FF/gather hooks of WebGPUEngine.iterate
+// (src/webgpu-engine.mjs:2407-2504):
+//
+//   ADE(V) update, Mur pre, RLC pre           (read V; UPML cells via flux alias)
+//   sweep_e  = UPML pre + core E + UPML post   (one pass)
+//   TFSF(V), Mur post, [ADE(V) apply], excitation, Mur apply, RLC apply
+//   ADE(I) update
+//   sweep_h  = UPML pre + core H + UPML post   (one pass)
+//   TFSF(I), [ADE(I) apply]
+//   NF2FF sample (+ fold every K), buffered gather, numTS++
+//
+// Every index the kernels dereference is validated on the host at
+// configure time; nothing reaches the GPU unchecked.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/efs_engine.h"
+#include "nf2ff.hpp"
+#include "reduce.hpp"
+#include "sparse.hpp"
+#include "sweep.hpp"
+
+using namespace efs;
+
+namespace {
+
+std::string g_create_error;
+
+struct Timed {
+    int which;
+    hipEvent_t a, b;
+};
+
+template <class T>
+struct DevArr {
+    T* p = nullptr;
+    size_t n = 0;
+};
+
+}  // namespace
+
+struct efs_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    std::string err;
+    bool inited = false;
+
+    uint32_t nx = 0, ny = 0, nz = 0;
+    size_t N = 0;
+    float *volt = nullptr, *curr = nullptr, *vv = nullptr, *vi = nullptr, *ii = nullptr, *iv = nullptr;
+    uint32_t numTS = 0;
+    SweepGeom geom{};
+
+    // excitation
+    uint32_t exc_n = 0, exc_len = 0, exc_period = 0;
+    float *exc_signal = nullptr, *exc_amp = nullptr;
+    uint32_t *exc_delay = nullptr, *exc_gidx = nullptr;
+
+    // UPML
+    PmlTable pml{};
+    float *pml_coef = nullptr, *pml_vflux = nullptr, *pml_cflux = nullptr;
+
+    // Mur
+    uint32_t mur_n = 0;
+    std::vector<uint32_t> h_mur_normal, h_mur_shifted;
+    float *mur_coeff = nullptr, *mur_saved = nullptr;
+    uint32_t *mur_normal = nullptr, *mur_shifted = nullptr, *mur_normal_src = nullptr, *mur_shifted_src = nullptr;
+
+    // ADE (all orders/directions flattened)
+    std::vector<uint32_t> h_ade_gidx;
+    std::vector<uint8_t> h_ade_lor;
+    std::vector<float> h_ade_c[6];  // v_int v_ext v_lor i_int i_ext i_lor
+    bool ade_apply = false;
+    bool ade_dirty = false;
+    uint32_t ade_n = 0;
+    uint32_t *ade_gidx = nullptr, *ade_src_v = nullptr, *ade_src_i = nullptr;
+    uint8_t* ade_lor = nullptr;
+    float* ade_c[6] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
+    float *ade_va = nullptr, *ade_vl = nullptr, *ade_ia = nullptr, *ade_il = nullptr;
+
+    bool alias_dirty = false;
+
+    // TFSF
+    uint32_t tfsf_len = 0, tfsf_period = 0, tfsf_nv = 0, tfsf_ni = 0;
+    float* tfsf_signal = nullptr;
+    uint32_t *tv_idx = nullptr, *tv_dint = nullptr, *ti_idx = nullptr, *ti_dint = nullptr;
+    float *tv_dfrac = nullptr, *tv_amp = nullptr, *ti_dfrac = nullptr, *ti_amp = nullptr;
+
+    // RLC
+    uint32_t rlc_n = 0;
+    uint32_t *rlc_gidx = nullptr, *rlc_type = nullptr;
+    float *rlc_c = nullptr, *rlc_vdn = nullptr, *rlc_jn = nullptr, *rlc_vil = nullptr;
+
+    // probe gather
+    uint32_t g_n = 0;
+    uint32_t* g_idx = nullptr;
+    float* g_out = nullptr;
+    uint32_t gb_n = 0, gb_max = 0, gb_interval = 1, gb_count = 0;
+    uint32_t* gb_idx = nullptr;
+    float* gb_ring = nullptr;
+
+    // NF2FF
+    uint32_t nf_P = 0, nf_F = 0, nf_K = 0, nf_maxTS = 0, nf_window = 0;
+    uint32_t nf_slot = 0, nf_ts0 = 0;
+    double nf_dT = 0, nf_wnorm = 1;
+    Nf2ffGeom nf_geom{};
+    uint32_t* nf_pts = nullptr;
+    float *nf_el = nullptr, *nf_buf = nullptr, *nf_acc = nullptr;
+    double* nf_omega = nullptr;
+    float2* nf_tw = nullptr;
+
+    // energy
+    double* en_partial = nullptr;
+
+    // profiling
+    bool prof = false;
+    std::vector<Timed> pending;
+    std::vector<hipEvent_t> event_pool;
+    double prof_ms[EFS_PROF_COUNT] = {0, 0, 0};
+    uint64_t prof_n[EFS_PROF_COUNT] = {0, 0, 0};
+};
+
+namespace {
+
+int fail(efs_ctx* c, int code, const char* fmt, ...) {
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    if (c) c->err = buf;
+    return code;
+}
+
+#define HIP_TRY(ctx, expr)                                                                      \
+    do {                                                                                        \
+        hipError_t e_ = (expr);                                                                 \
+        if (e_ != hipSuccess)                                                                   \
+            return fail((ctx), EFS_ERR_HIP, "%s: %s (%s:%d)", #expr, hipGetErrorString(e_), __FILE__, \
+                        __LINE__);                                                              \
+    } while (0)
+
+#define CHECK_LAUNCH(ctx) HIP_TRY(ctx, hipGetLastError())
+
+template <class T>
+void dfree(T*& p) {
+    if (p) hipFree((void*)p);
+    p = nullptr;
+}
+
+template <class T>
+int dalloc(efs_ctx* c, T*& p, size_t n) {
+    dfree(p);
+    if (n == 0) n = 1;
+    hipError_t e = hipMalloc((void**)&p, n * sizeof(T));
+    if (e != hipSuccess) {
+        p = nullptr;
+        return fail(c, e == hipErrorOutOfMemory ? EFS_ERR_NOMEM : EFS_ERR_HIP, "hipMalloc(%zu bytes): %s",
+                    n * sizeof(T), hipGetErrorString(e));
+    }
+    return EFS_OK;
+}
+
+template <class T>
+int dupload(efs_ctx* c, T*& p, const T* src, size_t n) {
+    int rc = dalloc(c, p, n);
+    if (rc) return rc;
+    if (n) HIP_TRY(c, hipMemcpy(p, src, n * sizeof(T), hipMemcpyHostToDevice));
+    return EFS_OK;
+}
+
+template <class T>
+int dzero(efs_ctx* c, T*& p, size_t n) {
+    int rc = dalloc(c, p, n);
+    if (rc) return rc;
+    HIP_TRY(c, hipMemset(p, 0, (n ? n : 1) * sizeof(T)));
+    return EFS_OK;
+}
+
+#define TRY(expr)              \
+    do {                       \
+        int rc_ = (expr);      \
+        if (rc_) return rc_;   \
+    } while (0)
+
+inline uint32_t nblk(size_t n, int t = kPointThreads) { return (uint32_t)((n + t - 1) / t); }
+
+int need_init(efs_ctx* c) {
+    if (!c) return EFS_ERR_STATE;
+    if (!c->inited) return fail(c, EFS_ERR_STATE, "Engine not initialized. Call init() first.");
+    return EFS_OK;
+}
+
+// ---- alias resolution: field index -> UPML flux index when in a region ----
+uint32_t alias_of(const efs_ctx* c, uint32_t gidx) {
+    const uint32_t n = (uint32_t)(gidx / c->N);
+    const size_t lin = gidx - (size_t)n * c->N;
+    const uint32_t x = (uint32_t)(lin / ((size_t)c->ny * c->nz));
+    const uint32_t rem = (uint32_t)(lin - (size_t)x * c->ny * c->nz);
+    const uint32_t y = rem / c->nz, z = rem - y * c->nz;
+    for (uint32_t k = 0; k < c->pml.n; ++k) {
+        const PmlRegion& R = c->pml.r[k];
+        if (x >= R.x0 && x < R.x1 && y >= R.y0 && y < R.y1 && z >= R.z0 && z < R.z1) {
+            const uint64_t p = R.flux + (uint64_t)n * R.cs + ((uint64_t)(x - R.x0) * R.ny + (y - R.y0)) * R.nz +
+                               (z - R.z0);
+            return kAliasFlux | (uint32_t)p;
+        }
+    }
+    return gidx;
+}
+
+int rebuild_aliases(efs_ctx* c) {
+    if (c->mur_n) {
+        std::vector<uint32_t> ns(c->mur_n), ss(c->mur_n);
+        for (uint32_t k = 0; k < c->mur_n; ++k) {
+            ns[k] = alias_of(c, c->h_mur_normal[k]);
+            ss[k] = alias_of(c, c->h_mur_shifted[k]);
+        }
+        TRY(dupload(c, c->mur_normal_src, ns.data(), ns.size()));
+        TRY(dupload(c, c->mur_shifted_src, ss.data(), ss.size()));
+    }
+    if (c->ade_n) {
+        std::vector<uint32_t> sv(c->ade_n);
+        for (uint32_t k = 0; k < c->ade_n; ++k) sv[k] = alias_of(c, c->h_ade_gidx[k]);
+        TRY(dupload(c, c->ade_src_v, sv.data(), sv.size()));
+        TRY(dupload(c, c->ade_src_i, sv.data(), sv.size()));  // same offsets into the current flux
+    }
+    c->alias_dirty = false;
+    return EFS_OK;
+}
+
+int rebuild_ade(efs_ctx* c) {
+    c->ade_n = (uint32_t)c->h_ade_gidx.size();
+    if (c->ade_n) {
+        TRY(dupload(c, c->ade_gidx, c->h_ade_gidx.data(), c->ade_n));
+        TRY(dupload(c, c->ade_lor, c->h_ade_lor.data(), c->ade_n));
+        for (int k = 0; k < 6; ++k) TRY(dupload(c, c->ade_c[k], c->h_ade_c[k].data(), c->ade_n));
+        TRY(dzero(c, c->ade_va, c->ade_n));
+        TRY(dzero(c, c->ade_vl, c->ade_n));
+        TRY(dzero(c, c->ade_ia, c->ade_n));
+        TRY(dzero(c, c->ade_il, c->ade_n));
+    }
+    c->ade_dirty = false;
+    c->alias_dirty = true;
+    return EFS_OK;
+}
+
+// ---- profiling ----
+hipEvent_t take_event(efs_ctx* c) {
+    if (!c->event_pool.empty()) {
+        hipEvent_t e = c->event_pool.back();
+        c->event_pool.pop_back();
+        return e;
+    }
+    hipEvent_t e = nullptr;
+    if (hipEventCreate(&e) != hipSuccess) return nullptr;
+    return e;
+}
+
+struct ProfScope {
+    efs_ctx* c;
+    int which;
+    hipEvent_t a = nullptr;
+    ProfScope(efs_ctx* ctx, int w) : c(ctx), which(w) {
+        if (c->prof) {
+            a = take_event(c);
+            if (a) hipEventRecord(a, c->stream);
+        }
+    }
+    ~ProfScope() {
+        if (c->prof && a) {
+            hipEvent_t b = take_event(c);
+            if (b) {
+                hipEventRecord(b, c->stream);
+                c->pending.push_back({which, a, b});
+            } else {
+                c->event_pool.push_back(a);
+            }
+        }
+    }
+};
+
+int drain_profile(efs_ctx* c) {
+    if (c->pending.empty()) return EFS_OK;
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    for (auto& t : c->pending) {
+        float ms = 0.f;
+        if (hipEventElapsedTime(&ms, t.a, t.b) == hipSuccess) {
+            c->prof_ms[t.which] += ms;
+            c->prof_n[t.which] += 1;
+        }
+        c->event_pool.push_back(t.a);
+        c->event_pool.push_back(t.b);
+    }
+    c->pending.clear();
+    return EFS_OK;
+}
+
+// ---- NF2FF fold ----
+int nf2ff_fold(efs_ctx* c) {
+    if (!c->nf_P || c->nf_slot == 0) return EFS_OK;
+    const uint32_t kused = c->nf_slot;
+    const uint32_t nt = 2 * kused * c->nf_F;
+    hipLaunchKernelGGL(k_nf2ff_twiddle, dim3(nblk(nt)), dim3(kPointThreads), 0, c->stream, c->nf_tw, c->nf_K,
+                       kused, c->nf_F, c->nf_ts0, c->nf_omega, c->nf_dT, c->nf_maxTS, c->nf_window,
+                       c->nf_wnorm);
+    CHECK_LAUNCH(c);
+    constexpr int FB = 8;
+    const uint32_t rows = 6 * c->nf_P;
+    const size_t lds = (size_t)2 * kused * FB * sizeof(float2);
+    dim3 grid(nblk(rows), (c->nf_F + FB - 1) / FB);
+    hipLaunchKernelGGL(k_nf2ff_fold<FB>, grid, dim3(256), lds, c->stream, c->nf_buf, rows, 3 * c->nf_P,
+                       c->nf_K, kused, c->nf_tw, c->nf_F, c->nf_acc);
+    CHECK_LAUNCH(c);
+    c->nf_slot = 0;
+    return EFS_OK;
+}
+
+void free_all(efs_ctx* c) {
+    dfree(c->volt); dfree(c->curr); dfree(c->vv); dfree(c->vi); dfree(c->ii); dfree(c->iv);
+    dfree(c->exc_signal); dfree(c->exc_amp); dfree(c->exc_delay); dfree(c->exc_gidx);
+    dfree(c->pml_coef); dfree(c->pml_vflux); dfree(c->pml_cflux);
+    dfree(c->mur_coeff); dfree(c->mur_saved); dfree(c->mur_normal); dfree(c->mur_shifted);
+    dfree(c->mur_normal_src); dfree(c->mur_shifted_src);
+    dfree(c->ade_gidx); dfree(c->ade_src_v); dfree(c->ade_src_i); dfree(c->ade_lor);
+    for (auto& p : c->ade_c) dfree(p);
+    dfree(c->ade_va); dfree(c->ade_vl); dfree(c->ade_ia); dfree(c->ade_il);
+    dfree(c->tfsf_signal); dfree(c->tv_idx); dfree(c->tv_dint); dfree(c->tv_dfrac); dfree(c->tv_amp);
+    dfree(c->ti_idx); dfree(c->ti_dint); dfree(c->ti_dfrac); dfree(c->ti_amp);
+    dfree(c->rlc_gidx); dfree(c->rlc_type); dfree(c->rlc_c); dfree(c->rlc_vdn); dfree(c->rlc_jn); dfree(c->rlc_vil);
+    dfree(c->g_idx); dfree(c->g_out); dfree(c->gb_idx); dfree(c->gb_ring);
+    dfree(c->nf_pts); dfree(c->nf_el); dfree(c->nf_buf); dfree(c->nf_acc); dfree(c->nf_omega); dfree(c->nf_tw);
+    dfree(c->en_partial);
+}
+
+void reset_extensions(efs_ctx* c) {
+    c->exc_n = 0;
+    c->pml = PmlTable{};
+    c->mur_n = 0;
+    c->h_mur_normal.clear();
+    c->h_mur_shifted.clear();
+    c->h_ade_gidx.clear();
+    c->h_ade_lor.clear();
+    for (auto& v : c->h_ade_c) v.clear();
+    c->ade_n = 0;
+    c->ade_dirty = false;
+    c->alias_dirty = false;
+    c->tfsf_nv = c->tfsf_ni = 0;
+    c->rlc_n = 0;
+    c->g_n = 0;
+    c->gb_n = 0;
+    c->gb_count = 0;
+    c->nf_P = 0;
+    c->nf_slot = 0;
+}
+
+bool box_overlap(const PmlRegion& a, const PmlRegion& b) {
+    return a.x0 < b.x1 && b.x0 < a.x1 && a.y0 < b.y1 && b.y0 < a.y1 && a.z0 < b.z1 && b.z0 < a.z1;
+}
+
+// ---- one timestep ----
+int launch_step(efs_ctx* c) {
+    const uint32_t t = c->numTS;
+    ProfScope step_scope(c, EFS_PROF_STEP);
+    const bool has_pml = c->pml.n > 0;
+    // pre-voltage
+    if (c->ade_n) {
+        hipLaunchKernelGGL(k_ade_update, dim3(nblk(c->ade_n)), dim3(kPointThreads), 0, c->stream, c->volt,
+                           c->pml_vflux, c->ade_src_v, c->ade_lor, c->ade_c[0], c->ade_c[1], c->ade_c[2],
+                           c->ade_va, c->ade_vl, c->ade_n);
+    }
+    if (c->mur_n) {
+        hipLaunchKernelGGL(k_mur_pre, dim3(nblk(c->mur_n)), dim3(kPointThreads), 0, c->stream, c->volt,
+                           c->pml_vflux, c->mur_coeff, c->mur_normal_src, c->mur_shifted_src, c->mur_saved,
+                           c->mur_n);
+    }
+    if (c->rlc_n) {
+        hipLaunchKernelGGL(k_rlc_pre, dim3(nblk(c->rlc_n)), dim3(kPointThreads), 0, c->stream, c->rlc_type,
+                           c->rlc_c, c->rlc_vdn, c->rlc_vil, c->rlc_n);
+    }
+    {
+        ProfScope s(c, EFS_PROF_SWEEP_E);
+        if (has_pml)
+            hipLaunchKernelGGL(sweep_e<true>, dim3(c->geom.blocks), dim3(kSweepThreads), 0, c->stream, c->volt,
+                               c->curr, c->vv, c->vi, c->geom, c->pml, c->pml_coef, c->pml_vflux);
+        else
+            hipLaunchKernelGGL(sweep_e<false>, dim3(c->geom.blocks), dim3(kSweepThreads), 0, c->stream, c->volt,
+                               c->curr, c->vv, c->vi, c->geom, c->pml, c->pml_coef, c->pml_vflux);
+    }
+    // post-voltage / apply-to-voltages
+    if (c->tfsf_nv) {
+        hipLaunchKernelGGL(k_tfsf, dim3(nblk(c->tfsf_nv)), dim3(kPointThreads), 0, c->stream, c->volt,
+                           c->tfsf_signal, c->tfsf_len, c->tfsf_period, c->tv_idx, c->tv_dint, c->tv_dfrac,
+                           c->tv_amp, c->tfsf_nv, t);
+    }
+    if (c->mur_n) {
+        hipLaunchKernelGGL(k_mur_post, dim3(nblk(c->mur_n)), dim3(kPointThreads), 0, c->stream, c->volt,
+                           c->mur_coeff, c->mur_shifted, c->mur_saved, c->mur_n);
+    }
+    if (c->ade_n && c->ade_apply) {
+        hipLaunchKernelGGL(k_ade_apply, dim3(nblk(c->ade_n)), dim3(kPointThreads), 0, c->stream, c->volt,
+                           c->ade_gidx, c->ade_va, c->ade_n);
+    }
+    if (c->exc_n) {
+        hipLaunchKernelGGL(k_excitation, dim3(nblk(c->exc_n)), dim3(kPointThreads), 0, c->stream, c->volt,
+                           c->exc_signal, c->exc_len, c->exc_period, c->exc_delay, c->exc_amp, c->exc_gidx,
+                           c->exc_n, t);
+    }
+    if (c->mur_n) {
+        hipLaunchKernelGGL(k_mur_apply, dim3(nblk(c->mur_n)), dim3(kPointThreads), 0, c->stream, c->volt,
+                           c->mur_normal, c->mur_saved, c->mur_n);
+    }
+    if (c->rlc_n) {
+        hipLaunchKernelGGL(k_rlc_apply, dim3(nblk(c->rlc_n)), dim3(kPointThreads), 0, c->stream, c->volt,
+                           c->rlc_gidx, c->rlc_type, c->rlc_c, c->rlc_vdn, c->rlc_jn, c->rlc_vil, c->rlc_n);
+    }
+    // pre-current
+    if (c->ade_n) {
+        hipLaunchKernelGGL(k_ade_update, dim3(nblk(c->ade_n)), dim3(kPointThreads), 0, c->stream, c->curr,
+                           c->pml_cflux, c->ade_src_i, c->ade_lor, c->ade_c[3], c->ade_c[4], c->ade_c[5],
+                           c->ade_ia, c->ade_il, c->ade_n);
+    }
+    {
+        ProfScope s(c, EFS_PROF_SWEEP_H);
+        if (has_pml)
+            hipLaunchKernelGGL(sweep_h<true>, dim3(c->geom.blocks), dim3(kSweepThreads), 0, c->stream, c->curr,
+                               c->volt, c->ii, c->iv, c->geom, c->pml, c->pml_coef, c->pml_cflux);
+        else
+            hipLaunchKernelGGL(sweep_h<false>, dim3(c->geom.blocks), dim3(kSweepThreads), 0, c->stream, c->curr,
+                               c->volt, c->ii, c->iv, c->geom, c->pml, c->pml_coef, c->pml_cflux);
+    }
+    // post-current
+    if (c->tfsf_ni) {
+        hipLaunchKernelGGL(k_tfsf, dim3(nblk(c->tfsf_ni)), dim3(kPointThreads), 0, c->stream, c->curr,
+                           c->tfsf_signal, c->tfsf_len, c->tfsf_period, c->ti_idx, c->ti_dint, c->ti_dfrac,
+                           c->ti_amp, c->tfsf_ni, t);
+    }
+    if (c->ade_n && c->ade_apply) {
+        hipLaunchKernelGGL(k_ade_apply, dim3(nblk(c->ade_n)), dim3(kPointThreads), 0, c->stream, c->curr,
+                           c->ade_gidx, c->ade_ia, c->ade_n);
+    }
+    // engine hooks
+    if (c->nf_P) {
+        if (c->nf_slot == 0) c->nf_ts0 = t + 1;
+        hipLaunchKernelGGL(k_nf2ff_sample, dim3(nblk(c->nf_P)), dim3(kPointThreads), 0, c->stream, c->volt,
+                           c->curr, c->nf_pts, c->nf_el, c->nf_geom,
+                           c->nf_buf + (size_t)c->nf_slot * 6 * c->nf_P);
+        c->nf_slot++;
+        if (c->nf_slot == c->nf_K) TRY(nf2ff_fold(c));
+    }
+    if (c->gb_n && (t + 1) % c->gb_interval == 0 && c->gb_count < c->gb_max) {
+        hipLaunchKernelGGL(k_gather, dim3(nblk(c->gb_n)), dim3(kPointThreads), 0, c->stream, c->volt, c->curr,
+                           c->gb_idx, c->gb_ring + (size_t)c->gb_count * 2 * c->gb_n, c->gb_n);
+        c->gb_count++;
+    }
+    CHECK_LAUNCH(c);
+    c->numTS++;
+    return EFS_OK;
+}
+
+}  // namespace
+
+// ===================================================================== C ABI
+
+extern "C" {
+
+const char* efs_version(void) { return "efs-mi355x 0.1"; }
+
+int efs_device_count(void) {
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess) return 0;
+    return n;
+}
+
+int efs_create(efs_ctx** out, int device) {
+    if (!out) return EFS_ERR_ARG;
+    *out = nullptr;
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || n <= 0) {
+        g_create_error = "no HIP device available";
+        return EFS_ERR_NO_DEVICE;
+    }
+    if (device < 0 || device >= n) {
+        g_create_error = "device index out of range";
+        return EFS_ERR_ARG;
+    }
+    efs_ctx* c = new efs_ctx();
+    c->device = device;
+    hipError_t e = hipSetDevice(device);
+    if (e == hipSuccess) e = hipStreamCreate(&c->stream);
+    if (e != hipSuccess) {
+        g_create_error = std::string("HIP setup failed: ") + hipGetErrorString(e);
+        delete c;
+        return EFS_ERR_HIP;
+    }
+    *out = c;
+    return EFS_OK;
+}
+
+void efs_destroy(efs_ctx* c) {
+    if (!c) return;
+    hipSetDevice(c->device);
+    if (c->stream) hipStreamSynchronize(c->stream);
+    free_all(c);
+    for (auto& t : c->pending) {
+        hipEventDestroy(t.a);
+        hipEventDestroy(t.b);
+    }
+    for (auto e : c->event_pool) hipEventDestroy(e);
+    if (c->stream) hipStreamDestroy(c->stream);
+    delete c;
+}
+
+const char* efs_last_error(const efs_ctx* c) { return c ? c->err.c_str() : g_create_error.c_str(); }
+
+int efs_init(efs_ctx* c, uint32_t nx, uint32_t ny, uint32_t nz, const float* vv, const float* vi,
+             const float* ii, const float* iv) {
+    if (!c) return EFS_ERR_STATE;
+    if (!nx || !ny || !nz) return fail(c, EFS_ERR_ARG, "grid size must be >= 1 in every direction");
+    const uint64_t N = (uint64_t)nx * ny * nz;
+    if (3 * N >= (1ull << 31)) return fail(c, EFS_ERR_ARG, "grid too large: 3*Nx*Ny*Nz must be < 2^31");
+    if (!vv || !vi || !ii || !iv) return fail(c, EFS_ERR_ARG, "coefficient arrays vv, vi, ii, iv are required");
+    HIP_TRY(c, hipSetDevice(c->device));
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    free_all(c);
+    reset_extensions(c);
+    c->inited = false;
+    c->nx = nx; c->ny = ny; c->nz = nz; c->N = N;
+    const size_t F = 3 * N;
+    TRY(dzero(c, c->volt, F));
+    TRY(dzero(c, c->curr, F));
+    TRY(dupload(c, c->vv, vv, F));
+    TRY(dupload(c, c->vi, vi, F));
+    TRY(dupload(c, c->ii, ii, F));
+    TRY(dupload(c, c->iv, iv, F));
+    TRY(dalloc(c, c->en_partial, kEnergyMaxBlocks + 1));
+    // sweep geometry: 256 contiguous (y,z) columns per workgroup, x split in
+    // chunks so the launch has ~8 workgroups per CU (256 CUs)
+    SweepGeom g{};
+    g.nx = nx; g.ny = ny; g.nz = nz;
+    g.cols = ny * nz;
+    g.col_tiles = (g.cols + kSweepThreads - 1) / kSweepThreads;
+    const uint32_t want = 2048;
+    uint32_t chunks = (want + g.col_tiles - 1) / g.col_tiles;
+    chunks = std::max(1u, std::min(chunks, std::max(1u, nx / 8)));
+    g.chunk_len = (nx + chunks - 1) / chunks;
+    g.chunks = (nx + g.chunk_len - 1) / g.chunk_len;
+    g.blocks = g.col_tiles * g.chunks;
+    c->geom = g;
+    c->numTS = 0;
+    c->inited = true;
+    return EFS_OK;
+}
+
+int efs_cfg_excitation(efs_ctx* c, const float* signal, uint32_t len, const uint32_t* delay, const float* amp,
+                       const uint32_t* dir, const uint32_t* pos, uint32_t n, uint32_t period) {
+    TRY(need_init(c));
+    if (n == 0) {
+        c->exc_n = 0;
+        return EFS_OK;
+    }
+    if (!signal || len == 0) return fail(c, EFS_ERR_ARG, "excitation signal must not be empty");
+    if (!delay || !amp || !dir || !pos) return fail(c, EFS_ERR_ARG, "excitation arrays are required");
+    std::vector<uint32_t> g(n);
+    for (uint32_t k = 0; k < n; ++k) {
+        if (dir[k] > 2) return fail(c, EFS_ERR_ARG, "excitation %u: dir %u out of range", k, dir[k]);
+        if (pos[k] >= c->N) return fail(c, EFS_ERR_ARG, "excitation %u: pos %u out of range", k, pos[k]);
+        g[k] = dir[k] * (uint32_t)c->N + pos[k];
+    }
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    TRY(dupload(c, c->exc_signal, signal, len));
+    TRY(dupload(c, c->exc_delay, delay, n));
+    TRY(dupload(c, c->exc_amp, amp, n));
+    TRY(dupload(c, c->exc_gidx, g.data(), n));
+    c->exc_n = n;
+    c->exc_len = len;
+    c->exc_period = period;
+    return EFS_OK;
+}
+
+int efs_cfg_pml(efs_ctx* c, uint32_t nreg, const uint32_t* start, const uint32_t* num, const float* vv,
+                const float* vvfo, const float* vvfn, const float* ii, const float* iifo, const float* iifn) {
+    TRY(need_init(c));
+    if (nreg > (uint32_t)kMaxPml) return fail(c, EFS_ERR_ARG, "at most %d UPML regions are supported", kMaxPml);
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    PmlTable t{};
+    uint64_t total = 0;
+    for (uint32_t k = 0; k < nreg; ++k) {
+        const uint32_t* s = start + 3 * k;
+        const uint32_t* m = num + 3 * k;
+        const uint32_t dims[3] = {c->nx, c->ny, c->nz};
+        for (int d = 0; d < 3; ++d)
+            if (m[d] == 0 || (uint64_t)s[d] + m[d] > dims[d])
+                return fail(c, EFS_ERR_ARG, "UPML region %u exceeds the grid in direction %d", k, d);
+        PmlRegion& R = t.r[k];
+        R.x0 = s[0]; R.x1 = s[0] + m[0];
+        R.y0 = s[1]; R.y1 = s[1] + m[1];
+        R.z0 = s[2]; R.z1 = s[2] + m[2];
+        R.ny = m[1]; R.nz = m[2];
+        R.cs = m[0] * m[1] * m[2];
+        R.flux = total;
+        R.coef = 6 * total;
+        total += 3 * (uint64_t)R.cs;
+        for (uint32_t j = 0; j < k; ++j)
+            if (box_overlap(t.r[j], R)) return fail(c, EFS_ERR_ARG, "UPML regions %u and %u overlap", j, k);
+    }
+    if (total >= (1ull << 31)) return fail(c, EFS_ERR_ARG, "UPML volume too large");
+    t.n = nreg;
+    if (nreg) {
+        if (!vv || !vvfo || !vvfn || !ii || !iifo || !iifn)
+            return fail(c, EFS_ERR_ARG, "UPML coefficient arrays are required");
+        std::vector<float> coef(6 * total);
+        const float* src[6] = {vv, vvfo, vvfn, ii, iifo, iifn};
+        uint64_t off_in = 0;
+        for (uint32_t k = 0; k < nreg; ++k) {
+            const uint64_t T = 3 * (uint64_t)t.r[k].cs;
+            for (int a = 0; a < 6; ++a)
+                std::memcpy(coef.data() + t.r[k].coef + a * T, src[a] + off_in, T * sizeof(float));
+            off_in += T;
+        }
+        TRY(dupload(c, c->pml_coef, coef.data(), coef.size()));
+        TRY(dzero(c, c->pml_vflux, total));
+        TRY(dzero(c, c->pml_cflux, total));
+    } else {
+        dfree(c->pml_coef); dfree(c->pml_vflux); dfree(c->pml_cflux);
+    }
+    c->pml = t;
+    c->alias_dirty = true;
+    return EFS_OK;
+}
+
+int efs_cfg_mur(efs_ctx* c, uint32_t n, const float* coeff, const uint32_t* normal_idx,
+                const uint32_t* shifted_idx) {
+    TRY(need_init(c));
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    if (n == 0) {
+        c->mur_n = 0;
+        return EFS_OK;
+    }
+    if (!coeff || !normal_idx || !shifted_idx) return fail(c, EFS_ERR_ARG, "Mur arrays are required");
+    for (uint32_t k = 0; k < n; ++k)
+        if (normal_idx[k] >= 3 * c->N || shifted_idx[k] >= 3 * c->N)
+            return fail(c, EFS_ERR_ARG, "Mur point %u: index out of range", k);
+    c->h_mur_normal.assign(normal_idx, normal_idx + n);
+    c->h_mur_shifted.assign(shifted_idx, shifted_idx + n);
+    TRY(dupload(c, c->mur_coeff, coeff, n));
+    TRY(dupload(c, c->mur_normal, normal_idx, n));
+    TRY(dupload(c, c->mur_shifted, shifted_idx, n));
+    TRY(dzero(c, c->mur_saved, n));
+    c->mur_n = n;
+    c->alias_dirty = true;
+    return EFS_OK;
+}
+
+int efs_cfg_ade_clear(efs_ctx* c) {
+    TRY(need_init(c));
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    c->h_ade_gidx.clear();
+    c->h_ade_lor.clear();
+    for (auto& v : c->h_ade_c) v.clear();
+    c->ade_n = 0;
+    c->ade_dirty = false;
+    return EFS_OK;
+}
+
+int efs_cfg_ade_add(efs_ctx* c, uint32_t has_lorentz, uint32_t dir, uint32_t ncells, const uint32_t* pos_idx,
+                    const float* v_int, const float* v_ext, const float* v_lor, const float* i_int,
+                    const float* i_ext, const float* i_lor) {
+    TRY(need_init(c));
+    if (dir > 2) return fail(c, EFS_ERR_ARG, "ADE direction %u out of range", dir);
+    if (ncells && !pos_idx) return fail(c, EFS_ERR_ARG, "ADE pos_idx is required");
+    for (uint32_t k = 0; k < ncells; ++k)
+        if (pos_idx[k] >= c->N) return fail(c, EFS_ERR_ARG, "ADE cell %u: pos_idx out of range", k);
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    const float* src[6] = {v_int, v_ext, v_lor, i_int, i_ext, i_lor};
+    for (uint32_t k = 0; k < ncells; ++k) {
+        c->h_ade_gidx.push_back(dir * (uint32_t)c->N + pos_idx[k]);
+        c->h_ade_lor.push_back(has_lorentz ? 1 : 0);
+        for (int a = 0; a < 6; ++a) c->h_ade_c[a].push_back(src[a] ? src[a][k] : 0.f);
+    }
+    c->ade_dirty = true;
+    return EFS_OK;
+}
+
+int efs_set_ade_apply(efs_ctx* c, int apply) {
+    TRY(need_init(c));
+    c->ade_apply = apply != 0;
+    return EFS_OK;
+}
+
+int efs_cfg_tfsf(efs_ctx* c, const float* signal, uint32_t len, uint32_t period, uint32_t nv,
+                 const uint32_t* v_idx, const uint32_t* v_dint, const float* v_dfrac, const float* v_amp,
+                 uint32_t ni, const uint32_t* i_idx, const uint32_t* i_dint, const float* i_dfrac,
+                 const float* i_amp) {
+    TRY(need_init(c));
+    if ((nv || ni) && (!signal || len == 0)) return fail(c, EFS_ERR_ARG, "TFSF signal must not be empty");
+    for (uint32_t k = 0; k < nv; ++k)
+        if (v_idx[k] >= 3 * c->N) return fail(c, EFS_ERR_ARG, "TFSF voltage point %u out of range", k);
+    for (uint32_t k = 0; k < ni; ++k)
+        if (i_idx[k] >= 3 * c->N) return fail(c, EFS_ERR_ARG, "TFSF current point %u out of range", k);
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    if (nv || ni) TRY(dupload(c, c->tfsf_signal, signal, len));
+    if (nv) {
+        TRY(dupload(c, c->tv_idx, v_idx, nv));
+        TRY(dupload(c, c->tv_dint, v_dint, nv));
+        TRY(dupload(c, c->tv_dfrac, v_dfrac, nv));
+        TRY(dupload(c, c->tv_amp, v_amp, nv));
+    }
+    if (ni) {
+        TRY(dupload(c, c->ti_idx, i_idx, ni));
+        TRY(dupload(c, c->ti_dint, i_dint, ni));
+        TRY(dupload(c, c->ti_dfrac, i_dfrac, ni));
+        TRY(dupload(c, c->ti_amp, i_amp, ni));
+    }
+    c->tfsf_len = len;
+    c->tfsf_period = period;
+    c->tfsf_nv = nv;
+    c->tfsf_ni = ni;
+    return EFS_OK;
+}
+
+int efs_cfg_rlc(efs_ctx* c, uint32_t n, const uint32_t* field_idx, const uint32_t* direction,
+                const uint32_t* type_flag, const float* coeffs) {
+    TRY(need_init(c));
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    if (n == 0) {
+        c->rlc_n = 0;
+        return EFS_OK;
+    }
+    std::vector<uint32_t> g(n);
+    for (uint32_t k = 0; k < n; ++k) {
+        if (field_idx[k] >= c->N || direction[k] > 2 || type_flag[k] > 1)
+            return fail(c, EFS_ERR_ARG, "RLC element %u: field_idx/direction/type_flag out of range", k);
+        g[k] = direction[k] * (uint32_t)c->N + field_idx[k];
+    }
+    TRY(dupload(c, c->rlc_gidx, g.data(), n));
+    TRY(dupload(c, c->rlc_type, type_flag, n));
+    TRY(dupload(c, c->rlc_c, coeffs, 9 * (size_t)n));
+    TRY(dzero(c, c->rlc_vdn, 3 * (size_t)n));
+    TRY(dzero(c, c->rlc_jn, 3 * (size_t)n));
+    TRY(dzero(c, c->rlc_vil, n));
+    c->rlc_n = n;
+    return EFS_OK;
+}
+
+int efs_iterate(efs_ctx* c, uint32_t n) {
+    TRY(need_init(c));
+    HIP_TRY(c, hipSetDevice(c->device));
+    if (c->ade_dirty) TRY(rebuild_ade(c));
+    if (c->alias_dirty) TRY(rebuild_aliases(c));
+    for (uint32_t s = 0; s < n; ++s) TRY(launch_step(c));
+    return EFS_OK;
+}
+
+int efs_sync(efs_ctx* c) {
+    if (!c) return EFS_ERR_STATE;
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    return EFS_OK;
+}
+
+uint32_t efs_num_ts(const efs_ctx* c) { return c ? c->numTS : 0; }
+
+int efs_set_num_ts(efs_ctx* c, uint32_t ts) {
+    TRY(need_init(c));
+    c->numTS = ts;
+    return EFS_OK;
+}
+
+int efs_get_fields(efs_ctx* c, float* volt, float* curr) {
+    TRY(need_init(c));
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    if (volt) HIP_TRY(c, hipMemcpy(volt, c->volt, 3 * c->N * sizeof(float), hipMemcpyDeviceToHost));
+    if (curr) HIP_TRY(c, hipMemcpy(curr, c->curr, 3 * c->N * sizeof(float), hipMemcpyDeviceToHost));
+    return EFS_OK;
+}
+
+int efs_upload_fields(efs_ctx* c, const float* volt, const float* curr) {
+    TRY(need_init(c));
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    if (volt) HIP_TRY(c, hipMemcpy(c->volt, volt, 3 * c->N * sizeof(float), hipMemcpyHostToDevice));
+    if (curr) HIP_TRY(c, hipMemcpy(c->curr, curr, 3 * c->N * sizeof(float), hipMemcpyHostToDevice));
+    return EFS_OK;
+}
+
+int efs_cfg_gather(efs_ctx* c, const uint32_t* idx, uint32_t n) {
+    TRY(need_init(c));
+    if (n == 0) return EFS_OK;  // the reference ignores an empty list
+    for (uint32_t k = 0; k < n; ++k)
+        if (idx[k] >= 3 * c->N) return fail(c, EFS_ERR_ARG, "gather index %u out of range", k);
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    TRY(dupload(c, c->g_idx, idx, n));
+    TRY(dalloc(c, c->g_out, 2 * (size_t)n));
+    c->g_n = n;
+    return EFS_OK;
+}
+
+int efs_read_gather(efs_ctx* c, float* out) {
+    TRY(need_init(c));
+    if (!c->g_n) return fail(c, EFS_ERR_STATE, "probe gather not configured");
+    hipLaunchKernelGGL(k_gather, dim3(nblk(c->g_n)), dim3(kPointThreads), 0, c->stream, c->volt, c->curr, c->g_idx,
+                       c->g_out, c->g_n);
+    CHECK_LAUNCH(c);
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    HIP_TRY(c, hipMemcpy(out, c->g_out, 2 * (size_t)c->g_n * sizeof(float), hipMemcpyDeviceToHost));
+    return EFS_OK;
+}
+
+int efs_cfg_gather_buffered(efs_ctx* c, const uint32_t* idx, uint32_t n, uint32_t max_samples, uint32_t interval) {
+    TRY(need_init(c));
+    if (n == 0) return EFS_OK;
+    if (interval == 0) return fail(c, EFS_ERR_ARG, "sampleInterval must be >= 1");
+    for (uint32_t k = 0; k < n; ++k)
+        if (idx[k] >= 3 * c->N) return fail(c, EFS_ERR_ARG, "gather index %u out of range", k);
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    TRY(dupload(c, c->gb_idx, idx, n));
+    TRY(dzero(c, c->gb_ring, (size_t)max_samples * 2 * n));
+    c->gb_n = n;
+    c->gb_max = max_samples;
+    c->gb_interval = interval;
+    c->gb_count = 0;
+    return EFS_OK;
+}
+
+int efs_read_gather_buffered(efs_ctx* c, float* out, uint32_t* num_samples) {
+    TRY(need_init(c));
+    if (num_samples) *num_samples = c->gb_n ? c->gb_count : 0;
+    if (!out || !c->gb_n || !c->gb_count) return EFS_OK;
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    HIP_TRY(c, hipMemcpy(out, c->gb_ring, (size_t)c->gb_count * 2 * c->gb_n * sizeof(float),
+                         hipMemcpyDeviceToHost));
+    return EFS_OK;
+}
+
+int efs_cfg_nf2ff(efs_ctx* c, const uint32_t* points, uint32_t P, const double* omega, uint32_t F, double dT,
+                  const float* edge_lens, const uint32_t* el_len, uint32_t max_ts, uint32_t window_type,
+                  uint32_t k_block) {
+    TRY(need_init(c));
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    if (P == 0) {
+        c->nf_P = 0;
+        return EFS_OK;
+    }
+    if (!points || !omega || F == 0 || !edge_lens || !el_len)
+        return fail(c, EFS_ERR_ARG, "NF2FF points, omega and edge lengths are required");
+    const uint32_t dims[3] = {c->nx, c->ny, c->nz};
+    for (int a = 0; a < 3; ++a) {
+        if (el_len[a] + 1 < dims[a]) return fail(c, EFS_ERR_ARG, "primal edge lengths %d too short", a);
+        if (el_len[3 + a] < dims[a]) return fail(c, EFS_ERR_ARG, "dual edge lengths %d too short", a);
+    }
+    for (uint32_t p = 0; p < P; ++p)
+        for (int a = 0; a < 3; ++a)
+            if (points[3 * p + a] >= dims[a]) return fail(c, EFS_ERR_ARG, "NF2FF point %u outside the grid", p);
+    const uint32_t K = k_block ? k_block : 64;
+    if (K > 256) return fail(c, EFS_ERR_ARG, "NF2FF k_block must be <= 256");
+    uint64_t el_total = 0;
+    Nf2ffGeom g{};
+    g.nx = c->nx; g.ny = c->ny; g.nz = c->nz; g.P = P;
+    for (int a = 0; a < 6; ++a) {
+        g.el_off[a] = (uint32_t)el_total;
+        el_total += el_len[a];
+    }
+    TRY(dupload(c, c->nf_pts, points, 3 * (size_t)P));
+    TRY(dupload(c, c->nf_el, edge_lens, el_total));
+    TRY(dupload(c, c->nf_omega, omega, F));
+    TRY(dalloc(c, c->nf_buf, (size_t)K * 6 * P));
+    TRY(dzero(c, c->nf_acc, (size_t)F * 2 * 6 * P));
+    TRY(dalloc(c, c->nf_tw, (size_t)2 * K * F));
+    c->nf_geom = g;
+    c->nf_P = P; c->nf_F = F; c->nf_K = K;
+    c->nf_dT = dT; c->nf_maxTS = max_ts; c->nf_window = window_type;
+    c->nf_wnorm = 1.0;
+    if (window_type == 1 && max_ts > 1) {
+        double s = 0;
+        for (uint32_t n = 0; n < max_ts; ++n) s += 0.5 * (1 - std::cos(2 * M_PI * n / (max_ts - 1)));
+        c->nf_wnorm = 1.0 / s;
+    }
+    c->nf_slot = 0;
+    return EFS_OK;
+}
+
+int efs_read_nf2ff(efs_ctx* c, float* accE, float* accH) {
+    TRY(need_init(c));
+    if (!c->nf_P) return fail(c, EFS_ERR_STATE, "NF2FF accumulation not configured");
+    TRY(nf2ff_fold(c));
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    const size_t rows = 6 * (size_t)c->nf_P, P = c->nf_P;
+    std::vector<float> acc((size_t)c->nf_F * 2 * rows);
+    HIP_TRY(c, hipMemcpy(acc.data(), c->nf_acc, acc.size() * sizeof(float), hipMemcpyDeviceToHost));
+    for (uint32_t f = 0; f < c->nf_F; ++f) {
+        const float* re = acc.data() + (size_t)f * 2 * rows;
+        const float* im = re + rows;
+        float* oE = accE ? accE + (size_t)f * rows : nullptr;
+        float* oH = accH ? accH + (size_t)f * rows : nullptr;
+        for (size_t p = 0; p < P; ++p)
+            for (int k = 0; k < 3; ++k) {
+                if (oE) {
+                    oE[6 * p + 2 * k] = re[k * P + p];
+                    oE[6 * p + 2 * k + 1] = im[k * P + p];
+                }
+                if (oH) {
+                    oH[6 * p + 2 * k] = re[(3 + k) * P + p];
+                    oH[6 * p + 2 * k + 1] = im[(3 + k) * P + p];
+                }
+            }
+    }
+    return EFS_OK;
+}
+
+int efs_farfield(efs_ctx* c, uint32_t f, const float* meta, const float* theta, uint32_t nT, const float* phi,
+                 uint32_t nP, const double* center, double frequency, double radius, float* out) {
+    TRY(need_init(c));
+    if (!c->nf_P) return fail(c, EFS_ERR_STATE, "NF2FF accumulation not configured");
+    if (f >= c->nf_F) return fail(c, EFS_ERR_ARG, "frequency index out of range");
+    if (!meta || !theta || !phi || !center || !out || !nT || !nP)
+        return fail(c, EFS_ERR_ARG, "far-field arrays are required");
+    if (!(radius > 0)) return fail(c, EFS_ERR_ARG, "radius must be > 0");
+    TRY(nf2ff_fold(c));
+    float *d_meta = nullptr, *d_th = nullptr, *d_ph = nullptr, *d_out = nullptr;
+    int rc = EFS_OK;
+    const size_t na = (size_t)nT * nP;
+    do {
+        if ((rc = dupload(c, d_meta, meta, 8 * (size_t)c->nf_P))) break;
+        if ((rc = dupload(c, d_th, theta, nT))) break;
+        if ((rc = dupload(c, d_ph, phi, nP))) break;
+        if ((rc = dalloc(c, d_out, 5 * na))) break;
+        const double C0 = 299792458.0, Z0 = 376.73031346177066;
+        const double k = 2 * M_PI * frequency / C0;
+        const double fmag = k / (4 * M_PI * radius), fph = -k * radius;
+        FarParams q{};
+        q.P = c->nf_P; q.nT = nT; q.nP = nP;
+        q.k = (float)k; q.Z0 = (float)Z0;
+        q.cx = (float)center[0]; q.cy = (float)center[1]; q.cz = (float)center[2];
+        q.fac_re = (float)(fmag * -std::sin(fph));
+        q.fac_im = (float)(fmag * std::cos(fph));
+        hipLaunchKernelGGL(k_farfield, dim3(nblk(na)), dim3(256), 0, c->stream,
+                           c->nf_acc + (size_t)f * 2 * 6 * c->nf_P, d_meta, d_th, d_ph, q, d_out);
+        hipError_t e = hipGetLastError();
+        if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+        if (e == hipSuccess) e = hipMemcpy(out, d_out, 5 * na * sizeof(float), hipMemcpyDeviceToHost);
+        if (e != hipSuccess) rc = fail(c, EFS_ERR_HIP, "far field: %s", hipGetErrorString(e));
+    } while (0);
+    hipStreamSynchronize(c->stream);
+    dfree(d_meta); dfree(d_th); dfree(d_ph); dfree(d_out);
+    return rc;
+}
+
+int efs_energy(efs_ctx* c, double* out) {
+    TRY(need_init(c));
+    if (!out) return fail(c, EFS_ERR_ARG, "out is required");
+    if (c->nx < 2 || c->ny < 2 || c->nz < 2) {
+        *out = 0.0;
+        return EFS_OK;
+    }
+    const uint32_t rows = (c->nx - 1) * (c->ny - 1);
+    const uint32_t blocks = std::max(1u, std::min((uint32_t)kEnergyMaxBlocks, (rows + 3) / 4));
+    hipLaunchKernelGGL(k_energy_partial, dim3(blocks), dim3(kEnergyThreads), 0, c->stream, c->volt, c->curr, c->nx,
+                       c->ny, c->nz, c->en_partial);
+    hipLaunchKernelGGL(k_energy_final, dim3(1), dim3(kEnergyThreads), 0, c->stream, c->en_partial, blocks,
+                       c->en_partial + kEnergyMaxBlocks);
+    CHECK_LAUNCH(c);
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    HIP_TRY(c, hipMemcpy(out, c->en_partial + kEnergyMaxBlocks, sizeof(double), hipMemcpyDeviceToHost));
+    return EFS_OK;
+}
+
+int efs_profile_enable(efs_ctx* c, int on) {
+    if (!c) return EFS_ERR_STATE;
+    TRY(drain_profile(c));
+    c->prof = on != 0;
+    for (int k = 0; k < EFS_PROF_COUNT; ++k) {
+        c->prof_ms[k] = 0;
+        c->prof_n[k] = 0;
+    }
+    return EFS_OK;
+}
+
+int efs_profile_read(efs_ctx* c, int which, double* total_ms, uint64_t* count) {
+    if (!c) return EFS_ERR_STATE;
+    if (which < 0 || which >= EFS_PROF_COUNT) return fail(c, EFS_ERR_ARG, "unknown profile slot");
+    TRY(drain_profile(c));
+    if (total_ms) *total_ms = c->prof_ms[which];
+    if (count) *count = c->prof_n[which];
+    return EFS_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,150 @@
+// sparse.hpp -- point-list extensions applied around the fused sweeps.
+//
+// Each kernel is one thread per point.  Reads that the reference performs
+// between its UPML pre- and post-passes (ADE update, Mur pre) see the UPML
+// flux, not the field, inside PML regions (the pre-pass has swapped them,
+// src/webgpu-fdtd.mjs:1094-1172).  Because the sweeps do pre+core+post in
+// one pass, those reads go through an alias index resolved on the host:
+// bit 31 set = read flux[idx & 0x7fffffff], clear = read field[idx].
+// Additions use float atomics so repeated indices accumulate correctly.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace efs {
+
+constexpr uint32_t kAliasFlux = 0x80000000u;
+constexpr int kPointThreads = 256;
+
+__device__ __forceinline__ float read_alias(const float* __restrict__ field,
+                                            const float* __restrict__ flux, uint32_t a) {
+    return (a & kAliasFlux) ? flux[a & ~kAliasFlux] : field[a];
+}
+
+// apply_excitation (src/webgpu-fdtd.mjs:161-183 semantics: clamp at 0,
+// optional period, past-the-end reads signal[0]).
+__global__ void k_excitation(float* __restrict__ volt, const float* __restrict__ signal, uint32_t len,
+                             uint32_t period, const uint32_t* __restrict__ delay,
+                             const float* __restrict__ amp, const uint32_t* __restrict__ gidx, uint32_t n,
+                             uint32_t numTS) {
+    const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    int64_t e = (int64_t)numTS - (int64_t)delay[k];
+    if (e < 0) e = 0;
+    if (period > 0) e %= (int64_t)period;
+    if (e >= (int64_t)len) e = 0;
+    atomicAdd(&volt[gidx[k]], amp[k] * signal[e]);
+}
+
+// Mur pre / post / apply (src/webgpu-fdtd.mjs:992-1023).
+__global__ void k_mur_pre(const float* __restrict__ volt, const float* __restrict__ vflux,
+                          const float* __restrict__ coeff, const uint32_t* __restrict__ normal_src,
+                          const uint32_t* __restrict__ shifted_src, float* __restrict__ saved, uint32_t n) {
+    const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    saved[k] = read_alias(volt, vflux, shifted_src[k]) - coeff[k] * read_alias(volt, vflux, normal_src[k]);
+}
+
+__global__ void k_mur_post(const float* __restrict__ volt, const float* __restrict__ coeff,
+                           const uint32_t* __restrict__ shifted, float* __restrict__ saved, uint32_t n) {
+    const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    saved[k] += coeff[k] * volt[shifted[k]];
+}
+
+__global__ void k_mur_apply(float* __restrict__ volt, const uint32_t* __restrict__ normal,
+                            const float* __restrict__ saved, uint32_t n) {
+    const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    volt[normal[k]] = saved[k];
+}
+
+// Lorentz / Drude ADE state update for all (order, direction) entries at
+// once (src/webgpu-fdtd.mjs:405-454).  lorentz[k] selects the Lorentz form.
+__global__ void k_ade_update(const float* __restrict__ field, const float* __restrict__ flux,
+                             const uint32_t* __restrict__ src, const uint8_t* __restrict__ lorentz,
+                             const float* __restrict__ c_int, const float* __restrict__ c_ext,
+                             const float* __restrict__ c_lor, float* __restrict__ A, float* __restrict__ Lr,
+                             uint32_t n) {
+    const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    const float V = read_alias(field, flux, src[k]);
+    if (lorentz[k]) {
+        const float l = Lr[k] + c_lor[k] * A[k];
+        Lr[k] = l;
+        A[k] = c_int[k] * A[k] + c_ext[k] * (V - l);
+    } else {
+        A[k] = c_int[k] * A[k] + c_ext[k] * V;
+    }
+}
+
+// openEMS write-back of the ADE state (optional): field -= A.
+__global__ void k_ade_apply(float* __restrict__ field, const uint32_t* __restrict__ gidx,
+                            const float* __restrict__ A, uint32_t n) {
+    const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    atomicAdd(&field[gidx[k]], -A[k]);
+}
+
+// TFSF injection (src/webgpu-fdtd.mjs:683-739): linear interpolation
+// between signal[d] and signal[d+1]; skipped before the delay and at or
+// past the last sample.
+__global__ void k_tfsf(float* __restrict__ field, const float* __restrict__ signal, uint32_t len,
+                       uint32_t period, const uint32_t* __restrict__ gidx, const uint32_t* __restrict__ dint,
+                       const float* __restrict__ dfrac, const float* __restrict__ amp, uint32_t n,
+                       uint32_t numTS) {
+    const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    int64_t d = (int64_t)numTS - (int64_t)dint[k];
+    if (d < 0) return;
+    if (period > 0) d %= (int64_t)period;
+    if (d >= (int64_t)len - 1) return;
+    const float delta = dfrac[k];
+    const float sig = (1.0f - delta) * signal[d] + delta * signal[d + 1];
+    atomicAdd(&field[gidx[k]], amp[k] * sig);
+}
+
+// Lumped RLC, CPU-engine order: history shift + parallel inductor current
+// before the E sweep, series update + write-back after excitation
+// (src/webgpu-fdtd.mjs:775-846).  c: 9 coefficients per element.
+__global__ void k_rlc_pre(const uint32_t* __restrict__ type_flag, const float* __restrict__ c,
+                          float* __restrict__ vdn, float* __restrict__ vil, uint32_t n) {
+    const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    float* V = vdn + 3 * k;
+    V[2] = V[1];
+    V[1] = V[0];
+    if (type_flag[k] == 0) vil[k] += c[9 * k + 0] * c[9 * k + 1] * V[1];
+}
+
+__global__ void k_rlc_apply(float* __restrict__ volt, const uint32_t* __restrict__ gidx,
+                            const uint32_t* __restrict__ type_flag, const float* __restrict__ c,
+                            float* __restrict__ vdn, float* __restrict__ jn, const float* __restrict__ vil,
+                            uint32_t n) {
+    const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    float* V = vdn + 3 * k;
+    float* J = jn + 3 * k;
+    const float* q = c + 9 * k;  // i2v ilv vvd vv2 vj1 vj2 ib0 b1 b2
+    V[0] = volt[gidx[k]];
+    J[2] = J[1];
+    J[1] = J[0];
+    if (type_flag[k] == 1) {
+        const float v0 = q[2] * (V[0] - vil[k] + q[3] * V[2] + q[4] * J[1] + q[5] * J[2]);
+        V[0] = v0;
+        J[0] = q[6] * (v0 - V[2]) - q[7] * q[6] * J[1] - q[8] * q[6] * J[2];
+        volt[gidx[k]] = v0;
+    }
+}
+
+// Probe gather into out[2*k], out[2*k+1] (src/webgpu-engine.mjs:2593-2665).
+__global__ void k_gather(const float* __restrict__ volt, const float* __restrict__ curr,
+                         const uint32_t* __restrict__ idx, float* __restrict__ out, uint32_t n) {
+    const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    const uint32_t j = idx[k];
+    out[2 * k] = volt[j];
+    out[2 * k + 1] = curr[j];
+}
+
+}  // namespace efs

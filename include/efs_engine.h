@@ -1,0 +1,182 @@
+/*
+ * efs_engine.h -- C ABI of the MI355X FDTD time-stepping engine.
+ *
+ * Drop-in boundary for the engine interface of the reference's WebGPU engine
+ * (class WebGPUEngine, src/webgpu-engine.mjs:994-3498).  Each entry point
+ * names the method it replaces.  Conventions:
+ *   - every array is a plain host pointer plus an explicit count; the engine
+ *     copies what it needs at configure time and never keeps the caller's
+ *     pointer (src/webgpu-engine.mjs:3231-3247);
+ *   - every call returns 0 (EFS_OK) or an EFS_ERR_* code; the message of the
+ *     last failure is returned by efs_last_error();
+ *   - field and coefficient arrays use the reference's NIJK layout:
+ *     addr(n,x,y,z) = n*Nx*Ny*Nz + x*Ny*Nz + y*Nz + z, 3 components;
+ *   - "linear position" (pos, field_idx of RLC/ADE) is x*Ny*Nz + y*Nz + z
+ *     without the component offset, as in the reference;
+ *   - work is enqueued on the context's HIP stream; only the read/energy
+ *     calls and efs_sync() wait for it.
+ */
+#ifndef EFS_ENGINE_H
+#define EFS_ENGINE_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct efs_ctx efs_ctx;
+
+enum {
+    EFS_OK = 0,
+    EFS_ERR_NO_DEVICE = 1, /* no GPU: initGPU() resolves false */
+    EFS_ERR_STATE = 2,     /* call out of order, e.g. before efs_init */
+    EFS_ERR_ARG = 3,       /* invalid size, index or region */
+    EFS_ERR_HIP = 4,       /* HIP runtime failure */
+    EFS_ERR_NOMEM = 5
+};
+
+/* Library version string, e.g. "efs-mi355x 0.1". */
+const char *efs_version(void);
+/* Number of visible HIP devices (0 when there is none). */
+int efs_device_count(void);
+
+/* initGPU() (src/webgpu-engine.mjs:1107-1135): create a context on HIP
+ * device `device`.  Returns EFS_ERR_NO_DEVICE when none is present. */
+int efs_create(efs_ctx **out, int device);
+/* destroy() (src/webgpu-engine.mjs:3163-3210). */
+void efs_destroy(efs_ctx *ctx);
+/* Message of the last failing call on ctx ("" if none).  With ctx == NULL,
+ * the message of the last failing efs_create. */
+const char *efs_last_error(const efs_ctx *ctx);
+
+/* init(gridSize, {vv,vi,ii,iv}) (src/webgpu-engine.mjs:1143-1185): each
+ * coefficient array holds 3*nx*ny*nz floats.  Fields start at zero and
+ * numTS at 0; every extension is cleared. */
+int efs_init(efs_ctx *ctx, uint32_t nx, uint32_t ny, uint32_t nz, const float *vv, const float *vi,
+             const float *ii, const float *iv);
+
+/* configureExcitation (src/webgpu-engine.mjs:1199-1227).  signal[len];
+ * per point n: delay, amp, dir (0..2), pos (linear position).  period 0 =
+ * not periodic.  A signal position past the end reads signal[0] (openEMS
+ * Apply2Voltages; src/webgpu-fdtd.mjs:169-183). */
+int efs_cfg_excitation(efs_ctx *ctx, const float *signal, uint32_t len, const uint32_t *delay,
+                       const float *amp, const uint32_t *dir, const uint32_t *pos, uint32_t n,
+                       uint32_t period);
+
+/* configurePML(regions) (src/webgpu-engine.mjs:1235-1323).  Replaces all
+ * regions.  start[3*r], num[3*r] per region; the six coefficient arrays are
+ * the regions' 3*nx*ny*nz-float NIJK arrays concatenated in region order.
+ * Regions must lie inside the grid and must not overlap.  nregions 0 clears
+ * the UPML. */
+int efs_cfg_pml(efs_ctx *ctx, uint32_t nregions, const uint32_t *start, const uint32_t *num,
+                const float *vv, const float *vvfo, const float *vvfn, const float *ii,
+                const float *iifo, const float *iifn);
+
+/* configureMur (src/webgpu-engine.mjs:1822-1918), single-list form: n points
+ * with coeff, normal_idx and shifted_idx (full field indices incl. the
+ * component offset).  The dual nyP/nyPP form is the concatenation of both
+ * lists (done by the host wrappers).  n 0 clears. */
+int efs_cfg_mur(efs_ctx *ctx, uint32_t n, const float *coeff, const uint32_t *normal_idx,
+                const uint32_t *shifted_idx);
+
+/* configureLorentzADE (src/webgpu-engine.mjs:1358-1475) is expressed as a
+ * clear followed by one efs_cfg_ade_add per (order, direction).  NULL
+ * coefficient pointers mean zeros, as in the reference. */
+int efs_cfg_ade_clear(efs_ctx *ctx);
+int efs_cfg_ade_add(efs_ctx *ctx, uint32_t has_lorentz, uint32_t dir, uint32_t ncells,
+                    const uint32_t *pos_idx, const float *v_int, const float *v_ext, const float *v_lor,
+                    const float *i_int, const float *i_ext, const float *i_lor);
+/* 0 (default): reference semantics, the ADE state is never written back to
+ * the fields (src/webgpu-fdtd.mjs:405-454).  1: openEMS write-back
+ * (field -= ADE state after the E and after the H half step). */
+int efs_set_ade_apply(efs_ctx *ctx, int apply);
+
+/* configureTFSF (src/webgpu-engine.mjs:1527-1645): voltage points (nv) and
+ * current points (ni), each {field_idx (full index), delay_int, delay_frac,
+ * amp}. */
+int efs_cfg_tfsf(efs_ctx *ctx, const float *signal, uint32_t len, uint32_t period, uint32_t nv,
+                 const uint32_t *v_field_idx, const uint32_t *v_delay_int, const float *v_delay_frac,
+                 const float *v_amp, uint32_t ni, const uint32_t *i_field_idx,
+                 const uint32_t *i_delay_int, const float *i_delay_frac, const float *i_amp);
+
+/* configureRLC (src/webgpu-engine.mjs:1722-1794): n elements; coeffs holds 9
+ * floats per element in the order i2v, ilv, vvd, vv2, vj1, vj2, ib0, b1, b2.
+ * type_flag 0 = parallel, 1 = series.  Pre-voltage and apply phases follow
+ * the CPU engine's order (src/webgpu-fdtd.mjs:1135,1155). */
+int efs_cfg_rlc(efs_ctx *ctx, uint32_t n, const uint32_t *field_idx, const uint32_t *direction,
+                const uint32_t *type_flag, const float *coeffs);
+
+/* iterate(numSteps) (src/webgpu-engine.mjs:2407-2504): enqueue n complete
+ * timesteps; does not wait. */
+int efs_iterate(efs_ctx *ctx, uint32_t n);
+/* device.queue.onSubmittedWorkDone(): wait for all enqueued work. */
+int efs_sync(efs_ctx *ctx);
+/* numTS field. */
+uint32_t efs_num_ts(const efs_ctx *ctx);
+int efs_set_num_ts(efs_ctx *ctx, uint32_t ts);
+
+/* getFields() / uploadFields(v, c) (src/webgpu-engine.mjs:2533-2571):
+ * 3*nx*ny*nz floats each; either pointer may be NULL. */
+int efs_get_fields(efs_ctx *ctx, float *volt, float *curr);
+int efs_upload_fields(efs_ctx *ctx, const float *volt, const float *curr);
+
+/* configureProbeGather / readProbeGather (src/webgpu-engine.mjs:2577-2613,
+ * 2697-2715): out receives 2*n floats [volt0, curr0, volt1, ...]. */
+int efs_cfg_gather(efs_ctx *ctx, const uint32_t *indices, uint32_t n);
+int efs_read_gather(efs_ctx *ctx, float *out);
+
+/* configureProbeGatherBuffered / readProbeGatherBuffered
+ * (src/webgpu-engine.mjs:2615-2694): one sample every `interval` steps,
+ * taken after step s when (s+1) % interval == 0, at most max_samples.
+ * out receives num_samples*2*n floats (pass NULL to query the count). */
+int efs_cfg_gather_buffered(efs_ctx *ctx, const uint32_t *indices, uint32_t n, uint32_t max_samples,
+                            uint32_t interval);
+int efs_read_gather_buffered(efs_ctx *ctx, float *out, uint32_t *num_samples);
+
+/* configureNF2FFAccumulation (src/webgpu-engine.mjs:2729-2827), generalised
+ * from one omega to F frequencies.  points: 3*P grid indices (ix,iy,iz);
+ * omega: F angular frequencies (FP64); edge_lens: the six edge-length
+ * arrays (primal x, y, z, dual x, y, z) concatenated, el_len[6] their
+ * lengths (each at least the grid size along its axis); window_type 1 =
+ * Hann over max_ts steps.  Surface E/H are buffered for k_block steps
+ * (0 = default 64) and folded into the F bins as one complex GEMM with
+ * twiddles generated in FP64. */
+int efs_cfg_nf2ff(efs_ctx *ctx, const uint32_t *points, uint32_t P, const double *omega, uint32_t F,
+                  double dT, const float *edge_lens, const uint32_t *el_len, uint32_t max_ts,
+                  uint32_t window_type, uint32_t k_block);
+/* readNF2FFAccumulation (src/webgpu-engine.mjs:2844-2855): accE/accH each
+ * receive F*6*P floats, frequency-major, each frequency in the reference
+ * layout [6*p + 2*c + {0: re, 1: im}]. */
+int efs_read_nf2ff(efs_ctx *ctx, float *accE, float *accH);
+
+/* computeNF2FFfarField (src/webgpu-engine.mjs:2871-2967) for frequency bin
+ * freq_idx: point_meta 8 floats per point (pos x,y,z [m], normalDir,
+ * normSign, area, pad, pad); out receives 5 floats per (theta, phi) pair,
+ * theta-major: P_rad, Et_re, Et_im, Ep_re, Ep_im. */
+int efs_farfield(efs_ctx *ctx, uint32_t freq_idx, const float *point_meta, const float *theta,
+                 uint32_t n_theta, const float *phi, uint32_t n_phi, const double *center,
+                 double frequency, double radius, float *out);
+
+/* computeEnergy() (src/webgpu-engine.mjs:2974-3158): sum of volt^2+curr^2
+ * over the (nx-1)(ny-1)(nz-1) interior, all components; FP32 squares,
+ * FP64 reduction.  Synchronous (returns the current value, not the
+ * previous call's); the lagging form is built on top by the host wrappers. */
+int efs_energy(efs_ctx *ctx, double *out);
+
+/* Kernel timing with HIP events on the engine's stream. */
+enum {
+    EFS_PROF_SWEEP_E = 0, /* fused E sweep (core + UPML) */
+    EFS_PROF_SWEEP_H = 1, /* fused H sweep (core + UPML) */
+    EFS_PROF_STEP = 2,    /* one complete timestep, all kernels */
+    EFS_PROF_COUNT = 3
+};
+int efs_profile_enable(efs_ctx *ctx, int on);
+/* Total milliseconds and number of timed launches since enabling. */
+int efs_profile_read(efs_ctx *ctx, int which, double *total_ms, uint64_t *count);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* EFS_ENGINE_H */
