@@ -75,12 +75,22 @@ def main():
     ap.add_argument("--profile-steps", type=int, default=200)
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="run one elementwise copy of 1 GiB -> 1 GiB first (reference kernel for HBM counters)")
     args = ap.parse_args()
 
     if args.gpus != 1:
         print(json.dumps({"error": "this engine runs on one GPU per process; multi-GPU runs are not part "
                                    "of this build (use --gpus 1)", "n_gpus": args.gpus}))
         sys.exit(2)
+
+    if args.calibrate_copy:
+        import torch
+        a = torch.ones(256 * 1024 * 1024, dtype=torch.float32, device="cuda")
+        b = torch.empty_like(a)
+        torch.mul(a, 1.0, out=b)
+        torch.cuda.synchronize()
+        del a, b
 
     from efs_amd import workloads
     from efs_amd.engine import PROF_STEP, PROF_SWEEP_E, PROF_SWEEP_H, HIPEngine
