@@ -22,6 +22,7 @@
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <string>
 #include <vector>
@@ -58,19 +59,30 @@ struct efs_ctx {
     bool inited = false;
 
     uint32_t nx = 0, ny = 0, nz = 0;
-    size_t N = 0;
+    size_t N = 0;      // cells per component (interface layout)
+    size_t S = 0;      // component stride of the device arrays (>= N, padded)
+    bool vec4 = false; // 4 z cells per lane (Nz % 4 == 0)
+    float* pool = nullptr;  // the six field/coefficient arrays, staggered
+    // compressed coefficients: byte index per cell and component into a
+    // table of <= 256 distinct (a, b) pairs per component (E: vv/vi, H: ii/iv)
+    bool comp = false;
+    uint8_t *cidx_e = nullptr, *cidx_h = nullptr;
+    float2 *ctab_e = nullptr, *ctab_h = nullptr;
     float *volt = nullptr, *curr = nullptr, *vv = nullptr, *vi = nullptr, *ii = nullptr, *iv = nullptr;
     uint32_t numTS = 0;
-    SweepGeom geom{};
+    SweepGeom geom{};   // 4-wide sweeps
+    SweepGeom geom1{};  // 1-wide sweeps
 
     // excitation
     uint32_t exc_n = 0, exc_len = 0, exc_period = 0;
     float *exc_signal = nullptr, *exc_amp = nullptr;
     uint32_t *exc_delay = nullptr, *exc_gidx = nullptr;
 
-    // UPML
+    // UPML: regions with compact region-local arrays (6 coefficient arrays
+    // and 2 flux arrays per region, 16-byte aligned region bases)
     PmlTable pml{};
     float *pml_coef = nullptr, *pml_vflux = nullptr, *pml_cflux = nullptr;
+    bool pml_split = false;  // 4-wide interior + UPML launches (region z-bounds multiples of 4)
 
     // Mur
     uint32_t mur_n = 0;
@@ -203,6 +215,60 @@ int need_init(efs_ctx* c) {
     return EFS_OK;
 }
 
+// Interface field index (component*N + linear position) -> device index
+// (component*S + linear position).
+inline uint32_t tr(const efs_ctx* c, uint64_t gidx) {
+    const uint64_t n = gidx / c->N;
+    return (uint32_t)(n * c->S + (gidx - n * c->N));
+}
+
+// Distinct (a, b) pairs of one component: byte index per cell and a
+// 256-entry table.  Returns false when there are more than 256 pairs.
+bool pair_table(const float* a, const float* b, size_t n, uint8_t* idx, float2* tab) {
+    constexpr uint32_t H = 4096;  // open-addressing slots (power of two)
+    std::vector<uint64_t> key(H);
+    std::vector<int16_t> val(H, -1);
+    uint32_t count = 0;
+    uint64_t last_k = ~0ull;
+    uint8_t last_v = 0;
+    for (size_t i = 0; i < n; ++i) {
+        uint32_t ua, ub;
+        std::memcpy(&ua, a + i, 4);
+        std::memcpy(&ub, b + i, 4);
+        const uint64_t k = ((uint64_t)ua << 32) | ub;
+        if (k == last_k) {
+            idx[i] = last_v;
+            continue;
+        }
+        uint32_t h = (uint32_t)((k * 0x9E3779B97F4A7C15ull) >> 52) & (H - 1);
+        while (val[h] >= 0 && key[h] != k) h = (h + 1) & (H - 1);
+        if (val[h] < 0) {
+            if (count == 256) return false;
+            key[h] = k;
+            val[h] = (int16_t)count;
+            tab[count] = make_float2(a[i], b[i]);
+            ++count;
+        }
+        last_k = k;
+        last_v = (uint8_t)val[h];
+        idx[i] = last_v;
+    }
+    for (uint32_t t = count; t < 256; ++t) tab[t] = make_float2(0.f, 0.f);
+    return true;
+}
+
+std::vector<uint32_t> tr_all(const efs_ctx* c, const uint32_t* gidx, size_t n) {
+    std::vector<uint32_t> out(n);
+    for (size_t k = 0; k < n; ++k) out[k] = tr(c, gidx[k]);
+    return out;
+}
+
+uint64_t env_u64(const char* name, uint64_t dflt) {
+    const char* v = std::getenv(name);
+    if (!v || !*v) return dflt;
+    return std::strtoull(v, nullptr, 10);
+}
+
 // ---- alias resolution: field index -> UPML flux index when in a region ----
 uint32_t alias_of(const efs_ctx* c, uint32_t gidx) {
     const uint32_t n = (uint32_t)(gidx / c->N);
@@ -218,7 +284,7 @@ uint32_t alias_of(const efs_ctx* c, uint32_t gidx) {
             return kAliasFlux | (uint32_t)p;
         }
     }
-    return gidx;
+    return tr(c, gidx);
 }
 
 int rebuild_aliases(efs_ctx* c) {
@@ -244,7 +310,8 @@ int rebuild_aliases(efs_ctx* c) {
 int rebuild_ade(efs_ctx* c) {
     c->ade_n = (uint32_t)c->h_ade_gidx.size();
     if (c->ade_n) {
-        TRY(dupload(c, c->ade_gidx, c->h_ade_gidx.data(), c->ade_n));
+        const std::vector<uint32_t> g = tr_all(c, c->h_ade_gidx.data(), c->ade_n);
+        TRY(dupload(c, c->ade_gidx, g.data(), c->ade_n));
         TRY(dupload(c, c->ade_lor, c->h_ade_lor.data(), c->ade_n));
         for (int k = 0; k < 6; ++k) TRY(dupload(c, c->ade_c[k], c->h_ade_c[k].data(), c->ade_n));
         TRY(dzero(c, c->ade_va, c->ade_n));
@@ -329,7 +396,10 @@ int nf2ff_fold(efs_ctx* c) {
 }
 
 void free_all(efs_ctx* c) {
-    dfree(c->volt); dfree(c->curr); dfree(c->vv); dfree(c->vi); dfree(c->ii); dfree(c->iv);
+    dfree(c->pool);
+    c->volt = c->curr = c->vv = c->vi = c->ii = c->iv = nullptr;
+    dfree(c->cidx_e); dfree(c->cidx_h); dfree(c->ctab_e); dfree(c->ctab_h);
+    c->comp = false;
     dfree(c->exc_signal); dfree(c->exc_amp); dfree(c->exc_delay); dfree(c->exc_gidx);
     dfree(c->pml_coef); dfree(c->pml_vflux); dfree(c->pml_cflux);
     dfree(c->mur_coeff); dfree(c->mur_saved); dfree(c->mur_normal); dfree(c->mur_shifted);
@@ -392,12 +462,19 @@ int launch_step(efs_ctx* c) {
     }
     {
         ProfScope s(c, EFS_PROF_SWEEP_E);
-        if (has_pml)
-            hipLaunchKernelGGL(sweep_e<true>, dim3(c->geom.blocks), dim3(kSweepThreads), 0, c->stream, c->volt,
-                               c->curr, c->vv, c->vi, c->geom, c->pml, c->pml_coef, c->pml_vflux);
-        else
-            hipLaunchKernelGGL(sweep_e<false>, dim3(c->geom.blocks), dim3(kSweepThreads), 0, c->stream, c->volt,
-                               c->curr, c->vv, c->vi, c->geom, c->pml, c->pml_coef, c->pml_vflux);
+        if (c->vec4 && (!has_pml || c->pml_split)) {
+            const Coef cf{c->vv, c->vi, c->cidx_e, c->ctab_e};
+            auto kern = has_pml ? (c->comp ? sweep_e4mc : sweep_e4m) : (c->comp ? sweep_e4c : sweep_e4);
+            hipLaunchKernelGGL(kern, dim3(c->geom.blocks), dim3(kSweepThreads), 0, c->stream, c->volt, c->curr, cf,
+                               c->geom, c->pml, c->pml_coef, c->pml_vflux);
+            if (has_pml)
+                hipLaunchKernelGGL(c->comp ? sweep_e4uc : sweep_e4u, dim3(c->pml.blocks), dim3(kSweepThreads), 0,
+                                   c->stream, c->volt, c->curr, cf, c->geom, c->pml, c->pml_coef, c->pml_vflux);
+        } else {
+            auto kern = has_pml ? sweep_e1<true> : sweep_e1<false>;
+            hipLaunchKernelGGL(kern, dim3(c->geom1.blocks), dim3(kSweepThreads), 0, c->stream, c->volt, c->curr,
+                               c->vv, c->vi, c->geom1, c->pml, c->pml_coef, c->pml_vflux);
+        }
     }
     // post-voltage / apply-to-voltages
     if (c->tfsf_nv) {
@@ -434,12 +511,19 @@ int launch_step(efs_ctx* c) {
     }
     {
         ProfScope s(c, EFS_PROF_SWEEP_H);
-        if (has_pml)
-            hipLaunchKernelGGL(sweep_h<true>, dim3(c->geom.blocks), dim3(kSweepThreads), 0, c->stream, c->curr,
-                               c->volt, c->ii, c->iv, c->geom, c->pml, c->pml_coef, c->pml_cflux);
-        else
-            hipLaunchKernelGGL(sweep_h<false>, dim3(c->geom.blocks), dim3(kSweepThreads), 0, c->stream, c->curr,
-                               c->volt, c->ii, c->iv, c->geom, c->pml, c->pml_coef, c->pml_cflux);
+        if (c->vec4 && (!has_pml || c->pml_split)) {
+            const Coef cf{c->ii, c->iv, c->cidx_h, c->ctab_h};
+            auto kern = has_pml ? (c->comp ? sweep_h4mc : sweep_h4m) : (c->comp ? sweep_h4c : sweep_h4);
+            hipLaunchKernelGGL(kern, dim3(c->geom.blocks), dim3(kSweepThreads), 0, c->stream, c->curr, c->volt, cf,
+                               c->geom, c->pml, c->pml_coef, c->pml_cflux);
+            if (has_pml)
+                hipLaunchKernelGGL(c->comp ? sweep_h4uc : sweep_h4u, dim3(c->pml.blocks), dim3(kSweepThreads), 0,
+                                   c->stream, c->curr, c->volt, cf, c->geom, c->pml, c->pml_coef, c->pml_cflux);
+        } else {
+            auto kern = has_pml ? sweep_h1<true> : sweep_h1<false>;
+            hipLaunchKernelGGL(kern, dim3(c->geom1.blocks), dim3(kSweepThreads), 0, c->stream, c->curr, c->volt,
+                               c->ii, c->iv, c->geom1, c->pml, c->pml_coef, c->pml_cflux);
+        }
     }
     // post-current
     if (c->tfsf_ni) {
@@ -538,27 +622,65 @@ int efs_init(efs_ctx* c, uint32_t nx, uint32_t ny, uint32_t nz, const float* vv,
     reset_extensions(c);
     c->inited = false;
     c->nx = nx; c->ny = ny; c->nz = nz; c->N = N;
-    const size_t F = 3 * N;
-    TRY(dzero(c, c->volt, F));
-    TRY(dzero(c, c->curr, F));
-    TRY(dupload(c, c->vv, vv, F));
-    TRY(dupload(c, c->vi, vi, F));
-    TRY(dupload(c, c->ii, ii, F));
-    TRY(dupload(c, c->iv, iv, F));
+    // Device layout: component stride S = N rounded up to 256 floats plus a
+    // pad, the six arrays in one pool with a stagger between them, so the
+    // concurrent streams of a sweep are not spaced at powers of two.
+    const uint64_t pad = env_u64("EFS_PAD", 1088);
+    const uint64_t stagger = env_u64("EFS_STAGGER", 4160);
+    c->S = ((N + 255) / 256) * 256 + (pad / 4) * 4;
+    if (3 * c->S >= (1ull << 31)) return fail(c, EFS_ERR_ARG, "grid too large for 32-bit device indices");
+    const uint64_t arr = 3 * c->S + (stagger / 64) * 64;
+    TRY(dalloc(c, c->pool, 6 * arr));
+    HIP_TRY(c, hipMemset(c->pool, 0, 6 * arr * sizeof(float)));
+    c->volt = c->pool;
+    c->curr = c->pool + arr;
+    c->vv = c->pool + 2 * arr;
+    c->vi = c->pool + 3 * arr;
+    c->ii = c->pool + 4 * arr;
+    c->iv = c->pool + 5 * arr;
+    const float* src[4] = {vv, vi, ii, iv};
+    float* dst[4] = {c->vv, c->vi, c->ii, c->iv};
+    for (int a = 0; a < 4; ++a)
+        for (int n = 0; n < 3; ++n)
+            HIP_TRY(c, hipMemcpy(dst[a] + n * c->S, src[a] + n * N, N * sizeof(float), hipMemcpyHostToDevice));
+    c->vec4 = (nz % 4 == 0) && env_u64("EFS_VEC", 4) == 4;
+    // coefficient compression (4-wide sweeps): both fields, all components
+    c->comp = false;
+    if (c->vec4 && env_u64("EFS_COMPRESS", 1) != 0) {
+        std::vector<uint8_t> ie(3 * c->S, 0), ih(3 * c->S, 0);
+        std::vector<float2> te(3 * 256), th(3 * 256);
+        bool ok = true;
+        for (int n = 0; n < 3 && ok; ++n) {
+            ok = pair_table(vv + n * N, vi + n * N, N, ie.data() + n * c->S, te.data() + n * 256) &&
+                 pair_table(ii + n * N, iv + n * N, N, ih.data() + n * c->S, th.data() + n * 256);
+        }
+        if (ok) {
+            TRY(dupload(c, c->cidx_e, ie.data(), ie.size()));
+            TRY(dupload(c, c->cidx_h, ih.data(), ih.size()));
+            TRY(dupload(c, c->ctab_e, te.data(), te.size()));
+            TRY(dupload(c, c->ctab_h, th.data(), th.size()));
+            c->comp = true;
+        }
+    }
     TRY(dalloc(c, c->en_partial, kEnergyMaxBlocks + 1));
-    // sweep geometry: 256 contiguous (y,z) columns per workgroup, x split in
-    // chunks so the launch has ~8 workgroups per CU (256 CUs)
-    SweepGeom g{};
-    g.nx = nx; g.ny = ny; g.nz = nz;
-    g.cols = ny * nz;
-    g.col_tiles = (g.cols + kSweepThreads - 1) / kSweepThreads;
-    const uint32_t want = 2048;
-    uint32_t chunks = (want + g.col_tiles - 1) / g.col_tiles;
-    chunks = std::max(1u, std::min(chunks, std::max(1u, nx / 8)));
-    g.chunk_len = (nx + chunks - 1) / chunks;
-    g.chunks = (nx + g.chunk_len - 1) / g.chunk_len;
-    g.blocks = g.col_tiles * g.chunks;
-    c->geom = g;
+    // sweep geometries: 256 lanes per workgroup over contiguous (y,z)
+    // columns (1 or 4 cells per lane), x split in chunks so the launch has
+    // ~8 workgroups per CU (256 CUs)
+    const uint32_t want = (uint32_t)env_u64("EFS_SWEEP_BLOCKS", 2048);
+    for (int v = 0; v < 2; ++v) {
+        SweepGeom g{};
+        g.nx = nx; g.ny = ny; g.nz = nz;
+        g.cols = ny * nz;
+        const uint32_t per_block = kSweepThreads * (v == 0 ? 4u : 1u);
+        g.col_tiles = (g.cols + per_block - 1) / per_block;
+        g.S = c->S;
+        uint32_t chunks = (want + g.col_tiles - 1) / g.col_tiles;
+        chunks = std::max(1u, std::min(chunks, std::max(1u, nx / 8)));
+        g.chunk_len = (nx + chunks - 1) / chunks;
+        g.chunks = (nx + g.chunk_len - 1) / g.chunk_len;
+        g.blocks = g.col_tiles * g.chunks;
+        (v == 0 ? c->geom : c->geom1) = g;
+    }
     c->numTS = 0;
     c->inited = true;
     return EFS_OK;
@@ -577,7 +699,7 @@ int efs_cfg_excitation(efs_ctx* c, const float* signal, uint32_t len, const uint
     for (uint32_t k = 0; k < n; ++k) {
         if (dir[k] > 2) return fail(c, EFS_ERR_ARG, "excitation %u: dir %u out of range", k, dir[k]);
         if (pos[k] >= c->N) return fail(c, EFS_ERR_ARG, "excitation %u: pos %u out of range", k, pos[k]);
-        g[k] = dir[k] * (uint32_t)c->N + pos[k];
+        g[k] = tr(c, (uint64_t)dir[k] * c->N + pos[k]);
     }
     HIP_TRY(c, hipStreamSynchronize(c->stream));
     TRY(dupload(c, c->exc_signal, signal, len));
@@ -597,6 +719,7 @@ int efs_cfg_pml(efs_ctx* c, uint32_t nreg, const uint32_t* start, const uint32_t
     HIP_TRY(c, hipStreamSynchronize(c->stream));
     PmlTable t{};
     uint64_t total = 0;
+    bool aligned = true;
     for (uint32_t k = 0; k < nreg; ++k) {
         const uint32_t* s = start + 3 * k;
         const uint32_t* m = num + 3 * k;
@@ -610,18 +733,36 @@ int efs_cfg_pml(efs_ctx* c, uint32_t nreg, const uint32_t* start, const uint32_t
         R.z0 = s[2]; R.z1 = s[2] + m[2];
         R.ny = m[1]; R.nz = m[2];
         R.cs = m[0] * m[1] * m[2];
+        total = (total + 3) / 4 * 4;  // 16-byte aligned region bases
         R.flux = total;
         R.coef = 6 * total;
         total += 3 * (uint64_t)R.cs;
+        aligned = aligned && (R.z0 % 4 == 0) && (R.nz % 4 == 0);
         for (uint32_t j = 0; j < k; ++j)
             if (box_overlap(t.r[j], R)) return fail(c, EFS_ERR_ARG, "UPML regions %u and %u overlap", j, k);
     }
     if (total >= (1ull << 31)) return fail(c, EFS_ERR_ARG, "UPML volume too large");
+    // UPML launch geometry: per region, 256-lane tiles over its (y, z/4)
+    // plane and x chunks sized to give the launch ~2 workgroups per CU
+    uint32_t blocks = 0;
+    for (uint32_t k = 0; k < nreg; ++k) {
+        PmlRegion& R = t.r[k];
+        const uint32_t lanes = R.ny * ((R.nz + 3) / 4);
+        R.tiles = (lanes + kSweepThreads - 1) / kSweepThreads;
+        const uint32_t nxr = R.x1 - R.x0;
+        uint32_t chunks = std::max(1u, std::min(nxr, (uint32_t)((uint64_t)512 * R.cs / std::max<uint64_t>(total / 3, 1) / R.tiles)));
+        R.chunk_len = (nxr + chunks - 1) / chunks;
+        chunks = (nxr + R.chunk_len - 1) / R.chunk_len;
+        R.blocks = R.tiles * chunks;
+        R.blk0 = blocks;
+        blocks += R.blocks;
+    }
     t.n = nreg;
+    t.blocks = blocks;
     if (nreg) {
         if (!vv || !vvfo || !vvfn || !ii || !iifo || !iifn)
             return fail(c, EFS_ERR_ARG, "UPML coefficient arrays are required");
-        std::vector<float> coef(6 * total);
+        std::vector<float> coef(6 * total, 0.f);
         const float* src[6] = {vv, vvfo, vvfn, ii, iifo, iifn};
         uint64_t off_in = 0;
         for (uint32_t k = 0; k < nreg; ++k) {
@@ -637,6 +778,7 @@ int efs_cfg_pml(efs_ctx* c, uint32_t nreg, const uint32_t* start, const uint32_t
         dfree(c->pml_coef); dfree(c->pml_vflux); dfree(c->pml_cflux);
     }
     c->pml = t;
+    c->pml_split = aligned && env_u64("EFS_PML_SPLIT", 1) != 0;
     c->alias_dirty = true;
     return EFS_OK;
 }
@@ -656,8 +798,9 @@ int efs_cfg_mur(efs_ctx* c, uint32_t n, const float* coeff, const uint32_t* norm
     c->h_mur_normal.assign(normal_idx, normal_idx + n);
     c->h_mur_shifted.assign(shifted_idx, shifted_idx + n);
     TRY(dupload(c, c->mur_coeff, coeff, n));
-    TRY(dupload(c, c->mur_normal, normal_idx, n));
-    TRY(dupload(c, c->mur_shifted, shifted_idx, n));
+    const std::vector<uint32_t> tn = tr_all(c, normal_idx, n), ts = tr_all(c, shifted_idx, n);
+    TRY(dupload(c, c->mur_normal, tn.data(), n));
+    TRY(dupload(c, c->mur_shifted, ts.data(), n));
     TRY(dzero(c, c->mur_saved, n));
     c->mur_n = n;
     c->alias_dirty = true;
@@ -713,13 +856,15 @@ int efs_cfg_tfsf(efs_ctx* c, const float* signal, uint32_t len, uint32_t period,
     HIP_TRY(c, hipStreamSynchronize(c->stream));
     if (nv || ni) TRY(dupload(c, c->tfsf_signal, signal, len));
     if (nv) {
-        TRY(dupload(c, c->tv_idx, v_idx, nv));
+        const std::vector<uint32_t> t = tr_all(c, v_idx, nv);
+        TRY(dupload(c, c->tv_idx, t.data(), nv));
         TRY(dupload(c, c->tv_dint, v_dint, nv));
         TRY(dupload(c, c->tv_dfrac, v_dfrac, nv));
         TRY(dupload(c, c->tv_amp, v_amp, nv));
     }
     if (ni) {
-        TRY(dupload(c, c->ti_idx, i_idx, ni));
+        const std::vector<uint32_t> t = tr_all(c, i_idx, ni);
+        TRY(dupload(c, c->ti_idx, t.data(), ni));
         TRY(dupload(c, c->ti_dint, i_dint, ni));
         TRY(dupload(c, c->ti_dfrac, i_dfrac, ni));
         TRY(dupload(c, c->ti_amp, i_amp, ni));
@@ -743,7 +888,7 @@ int efs_cfg_rlc(efs_ctx* c, uint32_t n, const uint32_t* field_idx, const uint32_
     for (uint32_t k = 0; k < n; ++k) {
         if (field_idx[k] >= c->N || direction[k] > 2 || type_flag[k] > 1)
             return fail(c, EFS_ERR_ARG, "RLC element %u: field_idx/direction/type_flag out of range", k);
-        g[k] = direction[k] * (uint32_t)c->N + field_idx[k];
+        g[k] = tr(c, (uint64_t)direction[k] * c->N + field_idx[k]);
     }
     TRY(dupload(c, c->rlc_gidx, g.data(), n));
     TRY(dupload(c, c->rlc_type, type_flag, n));
@@ -781,16 +926,24 @@ int efs_set_num_ts(efs_ctx* c, uint32_t ts) {
 int efs_get_fields(efs_ctx* c, float* volt, float* curr) {
     TRY(need_init(c));
     HIP_TRY(c, hipStreamSynchronize(c->stream));
-    if (volt) HIP_TRY(c, hipMemcpy(volt, c->volt, 3 * c->N * sizeof(float), hipMemcpyDeviceToHost));
-    if (curr) HIP_TRY(c, hipMemcpy(curr, c->curr, 3 * c->N * sizeof(float), hipMemcpyDeviceToHost));
+    for (int n = 0; n < 3; ++n) {
+        if (volt)
+            HIP_TRY(c, hipMemcpy(volt + n * c->N, c->volt + n * c->S, c->N * sizeof(float), hipMemcpyDeviceToHost));
+        if (curr)
+            HIP_TRY(c, hipMemcpy(curr + n * c->N, c->curr + n * c->S, c->N * sizeof(float), hipMemcpyDeviceToHost));
+    }
     return EFS_OK;
 }
 
 int efs_upload_fields(efs_ctx* c, const float* volt, const float* curr) {
     TRY(need_init(c));
     HIP_TRY(c, hipStreamSynchronize(c->stream));
-    if (volt) HIP_TRY(c, hipMemcpy(c->volt, volt, 3 * c->N * sizeof(float), hipMemcpyHostToDevice));
-    if (curr) HIP_TRY(c, hipMemcpy(c->curr, curr, 3 * c->N * sizeof(float), hipMemcpyHostToDevice));
+    for (int n = 0; n < 3; ++n) {
+        if (volt)
+            HIP_TRY(c, hipMemcpy(c->volt + n * c->S, volt + n * c->N, c->N * sizeof(float), hipMemcpyHostToDevice));
+        if (curr)
+            HIP_TRY(c, hipMemcpy(c->curr + n * c->S, curr + n * c->N, c->N * sizeof(float), hipMemcpyHostToDevice));
+    }
     return EFS_OK;
 }
 
@@ -800,7 +953,8 @@ int efs_cfg_gather(efs_ctx* c, const uint32_t* idx, uint32_t n) {
     for (uint32_t k = 0; k < n; ++k)
         if (idx[k] >= 3 * c->N) return fail(c, EFS_ERR_ARG, "gather index %u out of range", k);
     HIP_TRY(c, hipStreamSynchronize(c->stream));
-    TRY(dupload(c, c->g_idx, idx, n));
+    const std::vector<uint32_t> t = tr_all(c, idx, n);
+    TRY(dupload(c, c->g_idx, t.data(), n));
     TRY(dalloc(c, c->g_out, 2 * (size_t)n));
     c->g_n = n;
     return EFS_OK;
@@ -824,7 +978,8 @@ int efs_cfg_gather_buffered(efs_ctx* c, const uint32_t* idx, uint32_t n, uint32_
     for (uint32_t k = 0; k < n; ++k)
         if (idx[k] >= 3 * c->N) return fail(c, EFS_ERR_ARG, "gather index %u out of range", k);
     HIP_TRY(c, hipStreamSynchronize(c->stream));
-    TRY(dupload(c, c->gb_idx, idx, n));
+    const std::vector<uint32_t> t = tr_all(c, idx, n);
+    TRY(dupload(c, c->gb_idx, t.data(), n));
     TRY(dzero(c, c->gb_ring, (size_t)max_samples * 2 * n));
     c->gb_n = n;
     c->gb_max = max_samples;
@@ -867,6 +1022,7 @@ int efs_cfg_nf2ff(efs_ctx* c, const uint32_t* points, uint32_t P, const double* 
     uint64_t el_total = 0;
     Nf2ffGeom g{};
     g.nx = c->nx; g.ny = c->ny; g.nz = c->nz; g.P = P;
+    g.S = c->S;
     for (int a = 0; a < 6; ++a) {
         g.el_off[a] = (uint32_t)el_total;
         el_total += el_len[a];
@@ -966,7 +1122,7 @@ int efs_energy(efs_ctx* c, double* out) {
     const uint32_t rows = (c->nx - 1) * (c->ny - 1);
     const uint32_t blocks = std::max(1u, std::min((uint32_t)kEnergyMaxBlocks, (rows + 3) / 4));
     hipLaunchKernelGGL(k_energy_partial, dim3(blocks), dim3(kEnergyThreads), 0, c->stream, c->volt, c->curr, c->nx,
-                       c->ny, c->nz, c->en_partial);
+                       c->ny, c->nz, (uint64_t)c->S, c->en_partial);
     hipLaunchKernelGGL(k_energy_final, dim3(1), dim3(kEnergyThreads), 0, c->stream, c->en_partial, blocks,
                        c->en_partial + kEnergyMaxBlocks);
     CHECK_LAUNCH(c);

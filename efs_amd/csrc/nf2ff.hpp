@@ -18,6 +18,7 @@ namespace efs {
 struct Nf2ffGeom {
     uint32_t nx, ny, nz, P;
     uint32_t el_off[6];  // offsets of primal x,y,z / dual x,y,z edge lengths
+    uint64_t S;          // component stride of the field arrays
 };
 
 // Cell interpolation of E (mean of 4 edges / primal length; 0 on the last
@@ -30,7 +31,7 @@ __global__ void k_nf2ff_sample(const float* __restrict__ volt, const float* __re
     if (p >= g.P) return;
     const uint32_t ix = pts[3 * p], iy = pts[3 * p + 1], iz = pts[3 * p + 2];
     const size_t Ny = g.ny, Nz = g.nz;
-    const size_t N = (size_t)g.nx * Ny * Nz;
+    const size_t N = g.S;
     const size_t b = ix * Ny * Nz + iy * Nz + iz;
     const size_t sx = Ny * Nz, sy = Nz;
     float E0 = 0.f, E1 = 0.f, E2 = 0.f;

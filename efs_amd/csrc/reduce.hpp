@@ -24,11 +24,11 @@ __device__ __forceinline__ double wave_sum(double v) {
 __global__ __launch_bounds__(kEnergyThreads) void k_energy_partial(const float* __restrict__ volt,
                                                                    const float* __restrict__ curr,
                                                                    uint32_t nx, uint32_t ny, uint32_t nz,
-                                                                   double* __restrict__ partial) {
+                                                                   uint64_t S, double* __restrict__ partial) {
     __shared__ double s_w[kEnergyThreads / 64];
     const uint32_t lane = threadIdx.x & 63u, wv = threadIdx.x >> 6;
     const uint32_t nwaves = gridDim.x * (kEnergyThreads / 64);
-    const size_t N = (size_t)nx * ny * nz;
+    const size_t N = S;  // component stride
     const uint32_t rows = (nx - 1) * (ny - 1);
     double acc = 0.0;
     for (uint32_t row = blockIdx.x * (kEnergyThreads / 64) + wv; row < rows; row += nwaves) {

@@ -11,6 +11,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include "sweep.hpp"
+
 namespace efs {
 
 constexpr uint32_t kAliasFlux = 0x80000000u;

@@ -4,20 +4,37 @@
 // CPU form src/webgpu-fdtd.mjs:64-155) with the UPML pre/post passes
 // (update_pml modes 0-3, src/webgpu-engine.mjs:124-204) folded in.
 //
-// Mapping.  Fields keep the reference's NIJK layout, where for one x plane
-// and one component the (y, z) cells form a contiguous run of Ny*Nz floats.
-// A workgroup owns 256 consecutive cells of that run ("columns" c = y*Nz+z,
-// fully coalesced whatever Nz is) and marches along x, the slowest axis,
-// over a chunk of planes.  The x-neighbour needed by the stencil stays in
-// registers from the previous plane, so each H value (E sweep) or E value
-// (H sweep) is fetched from HBM once; the y and z neighbours belong to
-// adjacent lanes/waves of the same workgroup and are served by L1/L2.
-// Workgroup ids are remapped so each XCD (own L2) gets a contiguous range
-// of tiles: neighbouring column tiles share an L2.
+// Mapping.  Fields keep the reference's NIJK order; for one x plane and one
+// component the (y, z) cells form a contiguous run of Ny*Nz floats.  A lane
+// owns 4 consecutive z cells (16-byte accesses), a workgroup a run of 1024
+// cells of the plane, and marches along x, the slowest axis, over a chunk
+// of planes.  The x-neighbour needed by the stencil stays in registers from
+// the previous plane, so each H value (E sweep) or E value (H sweep) comes
+// from HBM once; y and z neighbours belong to adjacent lanes or waves of
+// the same or the neighbouring workgroup and are served by L1/L2.
+// Workgroup ids are remapped so each XCD (own L2) gets a contiguous range of
+// tiles.  Components are S floats apart (S >= Nx*Ny*Nz, padded).
+//
+// Each sweep is two launches over disjoint cells:
+//   sweep_*4  interior: every lane whose 4 cells lie outside all UPML
+//             regions (lanes inside a region are predicated off);
+//   sweep_*4u UPML: one grid over the region boxes, so its waves are
+//             uniformly UPML and carry the UPML flux and coefficients,
+//             stored compactly per region (region-local NIJK, as the
+//             reference hands them over).
+// This keeps the interior code lean (high occupancy, no divergent UPML path
+// in every wave) and needs region z-bounds that are multiples of 4; other
+// grids use the one-cell-per-lane sweeps (sweep_*1).
+//
+// Coefficients: two FP32 arrays (a, b) per component, or -- when every
+// component has at most 256 distinct (a, b) pairs -- one byte per cell
+// indexing a 256-entry table held in LDS.  Values and arithmetic are
+// identical; only the bytes moved differ (3 B instead of 24 B per cell and
+// sweep).
 //
 // UPML fold (per component, per PML cell; F = flux, exact algebra of
 // pre -> core -> post):
-//   F' = vv_c*F + vi*curl                      (the core update acts on F)
+//   F' = a*F + b*curl                          (the core update acts on F)
 //   V' = (vv_p*V - vvfo*F) + vvfn*F'
 // and for H the same with ii/iv, iifo/iifn.  H cells outside the core
 // bounds (last x/y/z plane) get the pre/post pair without the core update:
@@ -30,29 +47,43 @@ namespace efs {
 
 constexpr int kMaxPml = 16;
 constexpr int kSweepThreads = 256;
+constexpr int kTab = 256;
+#ifndef EFS_SWEEP_WAVES
+#define EFS_SWEEP_WAVES 4  // waves per SIMD the interior sweeps are compiled for (128 registers)
+#endif
+#ifndef EFS_SWEEP_WAVES_PML
+#define EFS_SWEEP_WAVES_PML 3  // UPML sweeps (168 registers)
+#endif
+
+typedef float v4 __attribute__((ext_vector_type(4)));
 
 struct PmlRegion {
     uint32_t x0, x1, y0, y1, z0, z1;  // [x0,x1) x [y0,y1) x [z0,z1)
     uint32_t ny, nz;                  // local extents in y, z
-    uint32_t cs;                      // nx*ny*nz (component stride)
+    uint32_t cs;                      // nx*ny*nz (component stride of the region arrays)
+    uint32_t blk0;                    // first workgroup of this region in the UPML launch
+    uint32_t tiles;                   // lane tiles per plane (ceil(ny*nz/4 / 256))
+    uint32_t chunk_len;               // x planes per workgroup
+    uint32_t blocks;                  // tiles * x chunks
     uint32_t pad;
-    uint64_t coef;                    // element offset of this region's 6 coefficient arrays
-    uint64_t flux;                    // element offset of this region's flux arrays (3*cs each)
+    uint64_t coef;                    // element offset of the region's 6 coefficient arrays
+    uint64_t flux;                    // element offset of the region's flux arrays (3*cs each)
 };
 
 struct PmlTable {
     uint32_t n;
-    uint32_t pad;
+    uint32_t blocks;  // total workgroups of the UPML launch
     PmlRegion r[kMaxPml];
 };
 
 struct SweepGeom {
     uint32_t nx, ny, nz;
     uint32_t cols;        // ny*nz
-    uint32_t col_tiles;   // ceil(cols / 256)
+    uint32_t col_tiles;   // ceil(cols / (256*vec))
     uint32_t chunks;      // x chunks
     uint32_t chunk_len;   // planes per chunk
     uint32_t blocks;      // col_tiles * chunks
+    uint64_t S;           // component stride in floats
 };
 
 // XCD-contiguous remap of the hardware workgroup id: dispatch hands out
@@ -81,88 +112,99 @@ __device__ __forceinline__ int pml_find(const PmlTable& t, uint32_t mask, uint32
     return -1;
 }
 
-// E sweep: V = vv*V + vi*curl(H), backward neighbour clamped to self at 0.
-template <bool HAS_PML>
-__global__ __launch_bounds__(kSweepThreads) void sweep_e(float* __restrict__ volt,
-                                                         const float* __restrict__ curr,
-                                                         const float* __restrict__ vv,
-                                                         const float* __restrict__ vi, SweepGeom g,
-                                                         PmlTable pml,
-                                                         const float* __restrict__ pml_coef,
-                                                         float* __restrict__ pml_vflux) {
-    const uint32_t L = xcd_remap(blockIdx.x, g.blocks);
-    const uint32_t tile = L % g.col_tiles, chunk = L / g.col_tiles;
-    const uint32_t c = tile * kSweepThreads + threadIdx.x;
-    if (c >= g.cols) return;
-    const uint32_t y = c / g.nz, z = c - y * g.nz;
-    const uint32_t x0 = chunk * g.chunk_len;
-    const uint32_t x1 = min(g.nx, x0 + g.chunk_len);
-    if (x0 >= x1) return;
+__device__ __forceinline__ v4 ld4(const float* p) { return *reinterpret_cast<const v4*>(p); }
+__device__ __forceinline__ v4 ld4nt(const float* p) {
+    return __builtin_nontemporal_load(reinterpret_cast<const v4*>(p));
+}
+__device__ __forceinline__ float ldnt(const float* p) { return __builtin_nontemporal_load(p); }
+__device__ __forceinline__ void st4(float* p, v4 v) { *reinterpret_cast<v4*>(p) = v; }
 
-    const size_t plane = g.cols;
-    const size_t N = (size_t)g.nx * plane;
-    const size_t sy = y > 0 ? g.nz : 0;
-    const size_t sz = z > 0 ? 1 : 0;
-    uint32_t mask = 0;
-    if (HAS_PML) mask = pml_column_mask(pml, y, z);
+// Explicit fused multiply-adds fix the rounding of every update, so the
+// 1-wide and 4-wide sweeps (and the interior/UPML split) give identical bits:
+//   core  V' = fma(a, V, b*curl)
+//   UPML  F' = fma(a, F, b*curl),  V' = fma(vvfn, F', fma(vv_p, V, -(vvfo*F)))
+__device__ __forceinline__ float fma1(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
+__device__ __forceinline__ v4 fma4(v4 a, v4 b, v4 c) { return __builtin_elementwise_fma(a, b, c); }
 
-    size_t i = (size_t)x0 * plane + c;
-    float hy_m, hz_m;  // Hy, Hz at x-1
-    if (x0 > 0) {
-        hy_m = curr[N + i - plane];
-        hz_m = curr[2 * N + i - plane];
-    } else {
-        hy_m = curr[N + i];
-        hz_m = curr[2 * N + i];
+__device__ __forceinline__ v4 sel4(uint32_t bits, v4 t, v4 f) {
+    return (v4){(bits & 1u) ? t.x : f.x, (bits & 2u) ? t.y : f.y, (bits & 4u) ? t.z : f.z,
+                (bits & 8u) ? t.w : f.w};
+}
+
+// ---------------------------------------------------------------- coefficients
+
+struct Coef {
+    const float* a;      // FP32 arrays (3 components, stride S)
+    const float* b;
+    const uint8_t* idx;  // byte indices (3 components, stride S)
+    const float2* tab;   // [3][256] pairs (global; staged into LDS)
+};
+
+template <bool COMP>
+__device__ __forceinline__ void coef_stage(const Coef& cf, float2* s_tab) {
+    if (COMP) {
+        for (int t = threadIdx.x; t < 3 * kTab; t += kSweepThreads) s_tab[t] = cf.tab[t];
+        __syncthreads();
     }
-    for (uint32_t x = x0; x < x1; ++x, i += plane) {
-        const float hx = curr[i], hy = curr[N + i], hz = curr[2 * N + i];
-        const float hz_ym = curr[2 * N + i - sy], hx_ym = curr[i - sy];
-        const float hy_zm = curr[N + i - sz], hx_zm = curr[i - sz];
-        const float cx = hz - hz_ym - hy + hy_zm;
-        const float cy = hx - hx_zm - hz + hz_m;
-        const float cz = hy - hy_m - hx + hx_ym;
-        hy_m = hy;
-        hz_m = hz;
-        const float curl[3] = {cx, cy, cz};
-        int k = -1;
-        if (HAS_PML && mask) k = pml_find(pml, mask, x);
-        if (!HAS_PML || k < 0) {
+}
+
+template <bool COMP>
+__device__ __forceinline__ void coef_load(const Coef& cf, const float2* s_tab, size_t S, uint32_t i, v4 a[3],
+                                          v4 b[3]) {
+    if (COMP) {
+        uint32_t q[3];
 #pragma unroll
-            for (int n = 0; n < 3; ++n) {
-                const size_t j = n * N + i;
-                volt[j] = vv[j] * volt[j] + vi[j] * curl[n];
-            }
-        } else {
-            const PmlRegion& R = pml.r[k];
-            const uint32_t p0 = ((x - R.x0) * R.ny + (y - R.y0)) * R.nz + (z - R.z0);
-            const size_t T = 3 * (size_t)R.cs;
-            const float* cf = pml_coef + R.coef;
-            float* fl = pml_vflux + R.flux;
+        for (int n = 0; n < 3; ++n)
+            q[n] = __builtin_nontemporal_load(reinterpret_cast<const uint32_t*>(cf.idx + n * S + i));
 #pragma unroll
-            for (int n = 0; n < 3; ++n) {
-                const size_t j = n * N + i;
-                const size_t p = (size_t)n * R.cs + p0;
-                const float F = fl[p];
-                const float Fn = vv[j] * F + vi[j] * curl[n];
-                const float V = volt[j];
-                const float f = cf[p] * V - cf[T + p] * F;  // vv_p*V - vvfo*F
-                volt[j] = f + cf[2 * T + p] * Fn;           // + vvfn*F'
-                fl[p] = Fn;
-            }
+        for (int n = 0; n < 3; ++n) {
+            const float2* t = s_tab + n * kTab;
+            const float2 p0 = t[q[n] & 255u], p1 = t[(q[n] >> 8) & 255u], p2 = t[(q[n] >> 16) & 255u],
+                         p3 = t[q[n] >> 24];
+            a[n] = (v4){p0.x, p1.x, p2.x, p3.x};
+            b[n] = (v4){p0.y, p1.y, p2.y, p3.y};
+        }
+    } else {
+#pragma unroll
+        for (int n = 0; n < 3; ++n) {
+            a[n] = ld4nt(cf.a + n * S + i);
+            b[n] = ld4nt(cf.b + n * S + i);
         }
     }
 }
 
-// H sweep: I = ii*I + iv*curl(V) on [0,Nx-1) x [0,Ny-1) x [0,Nz-1).
+// ---------------------------------------------------------------- 1 cell/lane
+
+// One cell of component n with curl formed; k = UPML region or -1; core =
+// whether the core update applies (always for E); HF selects the H-side
+// UPML coefficients (ii, iifo, iifn follow vv, vvfo, vvfn in each region).
+template <bool HF>
+__device__ __forceinline__ void cell1(float* __restrict__ fld, const float* __restrict__ a,
+                                      const float* __restrict__ b, size_t j, bool core, float curl, int k,
+                                      const PmlTable& pml, const float* __restrict__ pcoef,
+                                      float* __restrict__ pflux, uint32_t n, uint32_t x, uint32_t y, uint32_t z) {
+    if (k < 0) {
+        if (core) fld[j] = fma1(ldnt(a + j), fld[j], ldnt(b + j) * curl);
+        return;
+    }
+    const PmlRegion& R = pml.r[k];
+    const size_t p = (size_t)n * R.cs + ((size_t)(x - R.x0) * R.ny + (y - R.y0)) * R.nz + (z - R.z0);
+    const size_t T = 3 * (size_t)R.cs;
+    const float* cf = pcoef + R.coef + (HF ? 3 * T : 0);
+    float* fl = pflux + R.flux;
+    const float F = fl[p];
+    const float Fn = core ? fma1(ldnt(a + j), F, ldnt(b + j) * curl) : F;
+    fld[j] = fma1(ldnt(cf + 2 * T + p), Fn, fma1(ldnt(cf + p), fld[j], -(ldnt(cf + T + p) * F)));
+    fl[p] = Fn;
+}
+
 template <bool HAS_PML>
-__global__ __launch_bounds__(kSweepThreads) void sweep_h(float* __restrict__ curr,
-                                                         const float* __restrict__ volt,
-                                                         const float* __restrict__ ii,
-                                                         const float* __restrict__ iv, SweepGeom g,
-                                                         PmlTable pml,
-                                                         const float* __restrict__ pml_coef,
-                                                         float* __restrict__ pml_cflux) {
+__global__ __launch_bounds__(kSweepThreads) void sweep_e1(float* __restrict__ volt,
+                                                          const float* __restrict__ curr,
+                                                          const float* __restrict__ vv,
+                                                          const float* __restrict__ vi, SweepGeom g,
+                                                          PmlTable pml, const float* __restrict__ pcoef,
+                                                          float* __restrict__ pflux) {
     const uint32_t L = xcd_remap(blockIdx.x, g.blocks);
     const uint32_t tile = L % g.col_tiles, chunk = L / g.col_tiles;
     const uint32_t c = tile * kSweepThreads + threadIdx.x;
@@ -171,73 +213,317 @@ __global__ __launch_bounds__(kSweepThreads) void sweep_h(float* __restrict__ cur
     const uint32_t x0 = chunk * g.chunk_len;
     const uint32_t x1 = min(g.nx, x0 + g.chunk_len);
     if (x0 >= x1) return;
+    const size_t plane = g.cols, S = g.S;
+    const size_t sy = y > 0 ? g.nz : 0;
+    const size_t sz = z > 0 ? 1 : 0;
+    const uint32_t mask = HAS_PML ? pml_column_mask(pml, y, z) : 0u;
 
-    const size_t plane = g.cols;
-    const size_t N = (size_t)g.nx * plane;
+    size_t i = (size_t)x0 * plane + c;
+    const size_t ib = x0 > 0 ? i - plane : i;
+    float hy_m = curr[S + ib], hz_m = curr[2 * S + ib];  // Hy, Hz at x-1 (self at x = 0)
+    for (uint32_t x = x0; x < x1; ++x, i += plane) {
+        const float hx = curr[i], hy = curr[S + i], hz = curr[2 * S + i];
+        const float hz_ym = curr[2 * S + i - sy], hx_ym = curr[i - sy];
+        const float hy_zm = curr[S + i - sz], hx_zm = curr[i - sz];
+        const float curl[3] = {hz - hz_ym - hy + hy_zm, hx - hx_zm - hz + hz_m, hy - hy_m - hx + hx_ym};
+        hy_m = hy;
+        hz_m = hz;
+        const int k = (HAS_PML && mask) ? pml_find(pml, mask, x) : -1;
+#pragma unroll
+        for (int n = 0; n < 3; ++n)
+            cell1<false>(volt, vv, vi, n * S + i, true, curl[n], k, pml, pcoef, pflux, n, x, y, z);
+    }
+}
+
+template <bool HAS_PML>
+__global__ __launch_bounds__(kSweepThreads) void sweep_h1(float* __restrict__ curr,
+                                                          const float* __restrict__ volt,
+                                                          const float* __restrict__ ii,
+                                                          const float* __restrict__ iv, SweepGeom g,
+                                                          PmlTable pml, const float* __restrict__ pcoef,
+                                                          float* __restrict__ pflux) {
+    const uint32_t L = xcd_remap(blockIdx.x, g.blocks);
+    const uint32_t tile = L % g.col_tiles, chunk = L / g.col_tiles;
+    const uint32_t c = tile * kSweepThreads + threadIdx.x;
+    if (c >= g.cols) return;
+    const uint32_t y = c / g.nz, z = c - y * g.nz;
+    const uint32_t x0 = chunk * g.chunk_len;
+    const uint32_t x1 = min(g.nx, x0 + g.chunk_len);
+    if (x0 >= x1) return;
+    const size_t plane = g.cols, S = g.S;
     const bool col_core = (y + 1 < g.ny) && (z + 1 < g.nz);
-    uint32_t mask = 0;
-    if (HAS_PML) mask = pml_column_mask(pml, y, z);
+    const uint32_t mask = HAS_PML ? pml_column_mask(pml, y, z) : 0u;
     if (!col_core && !mask) return;
 
     size_t i = (size_t)x0 * plane + c;
-    float ex = 0.f, ey = 0.f, ez = 0.f;  // E at x (carried from the previous plane)
+    float ex = 0.f, ey = 0.f, ez = 0.f;  // E at x, carried from the previous plane
     if (col_core) {
         ex = volt[i];
-        ey = volt[N + i];
-        ez = volt[2 * N + i];
+        ey = volt[S + i];
+        ez = volt[2 * S + i];
     }
     for (uint32_t x = x0; x < x1; ++x, i += plane) {
         const bool core = col_core && (x + 1 < g.nx);
-        int k = -1;
-        if (HAS_PML && mask) k = pml_find(pml, mask, x);
+        const int k = (HAS_PML && mask) ? pml_find(pml, mask, x) : -1;
+        if (!core && k < 0) continue;
+        float curl[3] = {0.f, 0.f, 0.f};
         if (core) {
-            const float ex_p = volt[i + plane], ey_p = volt[N + i + plane], ez_p = volt[2 * N + i + plane];
-            const float ez_yp = volt[2 * N + i + g.nz], ey_zp = volt[N + i + 1];
+            const float ex_p = volt[i + plane], ey_p = volt[S + i + plane], ez_p = volt[2 * S + i + plane];
+            const float ez_yp = volt[2 * S + i + g.nz], ey_zp = volt[S + i + 1];
             const float ex_zp = volt[i + 1], ex_yp = volt[i + g.nz];
-            const float curl[3] = {ez - ez_yp - ey + ey_zp, ex - ex_zp - ez + ez_p, ey - ey_p - ex + ex_yp};
+            curl[0] = ez - ez_yp - ey + ey_zp;
+            curl[1] = ex - ex_zp - ez + ez_p;
+            curl[2] = ey - ey_p - ex + ex_yp;
             ex = ex_p;
             ey = ey_p;
             ez = ez_p;
-            if (!HAS_PML || k < 0) {
+        }
 #pragma unroll
-                for (int n = 0; n < 3; ++n) {
-                    const size_t j = n * N + i;
-                    curr[j] = ii[j] * curr[j] + iv[j] * curl[n];
-                }
-            } else {
-                const PmlRegion& R = pml.r[k];
-                const uint32_t p0 = ((x - R.x0) * R.ny + (y - R.y0)) * R.nz + (z - R.z0);
-                const size_t T = 3 * (size_t)R.cs;
-                const float* cf = pml_coef + R.coef + 3 * T;  // ii, iifo, iifn
-                float* fl = pml_cflux + R.flux;
+        for (int n = 0; n < 3; ++n)
+            cell1<true>(curr, ii, iv, n * S + i, core, curl[n], k, pml, pcoef, pflux, n, x, y, z);
+    }
+}
+
+// ---------------------------------------------------------------- 4 cells/lane
+
+// Per-lane work of the 4-wide sweeps: the lane's (y, z0) column and its x
+// range.  The interior launch derives it from the grid tiling, the UPML
+// launch from the region boxes.
+struct Lane4 {
+    uint32_t y, z0, x0, x1;
+    int region;  // UPML region of the lane (UPML launch), -1 in the interior launch
+};
+
+// E update of 4 cells over x in [x0, x1).  INTERIOR: skip planes where the
+// lane is inside a UPML region (column mask m); otherwise every plane is in
+// region w.region (compact region arrays).
+template <bool INTERIOR, bool COMP>
+__device__ __forceinline__ void e4_march(float* __restrict__ volt, const float* __restrict__ curr, const Coef& cf,
+                                         const float2* s_tab, const SweepGeom& g, const PmlTable& pml,
+                                         const float* __restrict__ pcoef, float* __restrict__ pflux, Lane4 w,
+                                         uint32_t m) {
+    const size_t S = g.S;
+    float* const E[3] = {volt, volt + S, volt + 2 * S};
+    const float* const H0 = curr;
+    const float* const H1 = curr + S;
+    const float* const H2 = curr + 2 * S;
+    const uint32_t plane = g.cols;
+    const uint32_t sy = w.y > 0 ? g.nz : 0u;  // backward neighbours clamp to self at index 0
+    const uint32_t sz = w.z0 > 0 ? 1u : 0u;
+    uint32_t i = w.x0 * plane + w.y * g.nz + w.z0;
+    bool carry = false;  // hy_m/hz_m hold H at x-1
+    v4 hy_m = 0.f, hz_m = 0.f;
+    const PmlRegion* R = INTERIOR ? nullptr : &pml.r[w.region];
+    for (uint32_t x = w.x0; x < w.x1; ++x, i += plane) {
+        if (INTERIOR && m && pml_find(pml, m, x) >= 0) {
+            carry = false;
+            continue;
+        }
+        if (!carry) {
+            const uint32_t ib = x > 0 ? i - plane : i;
+            hy_m = ld4(H1 + ib);
+            hz_m = ld4(H2 + ib);
+        }
+        const v4 hx = ld4(H0 + i), hy = ld4(H1 + i), hz = ld4(H2 + i);
+        const v4 hz_ym = ld4(H2 + i - sy), hx_ym = ld4(H0 + i - sy);
+        const float hy_z = H1[i - sz], hx_z = H0[i - sz];  // z-1 of the first cell
+        v4 V[3], a[3], b[3];
 #pragma unroll
-                for (int n = 0; n < 3; ++n) {
-                    const size_t j = n * N + i;
-                    const size_t p = (size_t)n * R.cs + p0;
-                    const float F = fl[p];
-                    const float Fn = ii[j] * F + iv[j] * curl[n];
-                    const float I = curr[j];
-                    const float f = cf[p] * I - cf[T + p] * F;
-                    curr[j] = f + cf[2 * T + p] * Fn;
-                    fl[p] = Fn;
-                }
-            }
-        } else if (HAS_PML && k >= 0) {
-            const PmlRegion& R = pml.r[k];
-            const uint32_t p0 = ((x - R.x0) * R.ny + (y - R.y0)) * R.nz + (z - R.z0);
-            const size_t T = 3 * (size_t)R.cs;
-            const float* cf = pml_coef + R.coef + 3 * T;
-            const float* fl = pml_cflux + R.flux;
+        for (int n = 0; n < 3; ++n) V[n] = ld4(E[n] + i);
+        coef_load<COMP>(cf, s_tab, S, i, a, b);
+        v4 F[3], P0[3], P1[3], P2[3];
+        size_t p0 = 0;
+        if (!INTERIOR) {
+            p0 = ((size_t)(x - R->x0) * R->ny + (w.y - R->y0)) * R->nz + (w.z0 - R->z0);
+            const size_t T = 3 * (size_t)R->cs;
+            const float* cfp = pcoef + R->coef;
+            const float* fl = pflux + R->flux;
 #pragma unroll
             for (int n = 0; n < 3; ++n) {
-                const size_t j = n * N + i;
-                const size_t p = (size_t)n * R.cs + p0;
-                const float F = fl[p];
-                const float f = cf[p] * curr[j] - cf[T + p] * F;
-                curr[j] = f + cf[2 * T + p] * F;
+                const size_t p = (size_t)n * R->cs + p0;
+                F[n] = ld4(fl + p);
+                P0[n] = ld4nt(cfp + p);
+                P1[n] = ld4nt(cfp + T + p);
+                P2[n] = ld4nt(cfp + 2 * T + p);
+            }
+        }
+        const v4 hy_zm = (v4){hy_z, hy.x, hy.y, hy.z};
+        const v4 hx_zm = (v4){hx_z, hx.x, hx.y, hx.z};
+        const v4 cur[3] = {hz - hz_ym - hy + hy_zm, hx - hx_zm - hz + hz_m, hy - hy_m - hx + hx_ym};
+        hy_m = hy;
+        hz_m = hz;
+        carry = true;
+#pragma unroll
+        for (int n = 0; n < 3; ++n) {
+            if (INTERIOR) {
+                st4(E[n] + i, fma4(a[n], V[n], b[n] * cur[n]));
+            } else {
+                const v4 Fn = fma4(a[n], F[n], b[n] * cur[n]);
+                st4(E[n] + i, fma4(P2[n], Fn, fma4(P0[n], V[n], -(P1[n] * F[n]))));
+                st4(pflux + R->flux + (size_t)n * R->cs + p0, Fn);
             }
         }
     }
 }
+
+// H update of 4 cells over x in [x0, x1) (see e4_march).  Cells outside the
+// core bounds (x = Nx-1, y = Ny-1, z = Nz-1) keep I unless in UPML.
+template <bool INTERIOR, bool COMP>
+__device__ __forceinline__ void h4_march(float* __restrict__ curr, const float* __restrict__ volt, const Coef& cf,
+                                         const float2* s_tab, const SweepGeom& g, const PmlTable& pml,
+                                         const float* __restrict__ pcoef, float* __restrict__ pflux, Lane4 w,
+                                         uint32_t m) {
+    const size_t S = g.S;
+    float* const Ic[3] = {curr, curr + S, curr + 2 * S};
+    const float* const E0 = volt;
+    const float* const E1 = volt + S;
+    const float* const E2 = volt + 2 * S;
+    const uint32_t plane = g.cols;
+    const bool row_core = w.y + 1 < g.ny;
+    const bool last = w.z0 + 4 >= g.nz;  // the lane holds the row's last z cell (no core update there)
+    if (INTERIOR && !row_core) return;
+    const uint32_t sy = row_core ? g.nz : 0u;  // clamped forward neighbours stay in bounds
+    const uint32_t sz = last ? 3u : 4u;
+    uint32_t i = w.x0 * plane + w.y * g.nz + w.z0;
+    bool carry = false;  // ex/ey/ez hold E at x
+    v4 ex = 0.f, ey = 0.f, ez = 0.f;
+    const PmlRegion* R = INTERIOR ? nullptr : &pml.r[w.region];
+    for (uint32_t x = w.x0; x < w.x1; ++x, i += plane) {
+        const bool core = row_core && (x + 1 < g.nx);
+        if (INTERIOR && (!core || (m && pml_find(pml, m, x) >= 0))) {
+            carry = false;
+            continue;
+        }
+        if (!carry) {
+            ex = ld4(E0 + i);
+            ey = ld4(E1 + i);
+            ez = ld4(E2 + i);
+        }
+        // per-cell core flags: cells 0..2 follow `core`, cell 3 also needs z+1 < Nz
+        const uint32_t cb = core ? (last ? 7u : 15u) : 0u;
+        const uint32_t sx = core ? plane : 0u;
+        const v4 ex_p = ld4(E0 + i + sx), ey_p = ld4(E1 + i + sx), ez_p = ld4(E2 + i + sx);
+        const v4 ez_yp = ld4(E2 + i + sy), ex_yp = ld4(E0 + i + sy);
+        const float ey_z = E1[i + sz], ex_z = E0[i + sz];  // z+1 of the fourth cell
+        v4 I[3], a[3], b[3];
+#pragma unroll
+        for (int n = 0; n < 3; ++n) I[n] = ld4(Ic[n] + i);
+        coef_load<COMP>(cf, s_tab, S, i, a, b);
+        v4 F[3], P0[3], P1[3], P2[3];
+        size_t p0 = 0;
+        if (!INTERIOR) {
+            p0 = ((size_t)(x - R->x0) * R->ny + (w.y - R->y0)) * R->nz + (w.z0 - R->z0);
+            const size_t T = 3 * (size_t)R->cs;
+            const float* cfp = pcoef + R->coef + 3 * T;  // ii, iifo, iifn
+            const float* fl = pflux + R->flux;
+#pragma unroll
+            for (int n = 0; n < 3; ++n) {
+                const size_t p = (size_t)n * R->cs + p0;
+                F[n] = ld4(fl + p);
+                P0[n] = ld4nt(cfp + p);
+                P1[n] = ld4nt(cfp + T + p);
+                P2[n] = ld4nt(cfp + 2 * T + p);
+            }
+        }
+        const v4 ey_zp = (v4){ey.y, ey.z, ey.w, ey_z};
+        const v4 ex_zp = (v4){ex.y, ex.z, ex.w, ex_z};
+        const v4 cur[3] = {ez - ez_yp - ey + ey_zp, ex - ex_zp - ez + ez_p, ey - ey_p - ex + ex_yp};
+        ex = ex_p;
+        ey = ey_p;
+        ez = ez_p;
+        carry = core;
+#pragma unroll
+        for (int n = 0; n < 3; ++n) {
+            if (INTERIOR) {
+                st4(Ic[n] + i, sel4(cb, fma4(a[n], I[n], b[n] * cur[n]), I[n]));
+            } else {
+                const v4 Fn = sel4(cb, fma4(a[n], F[n], b[n] * cur[n]), F[n]);
+                st4(Ic[n] + i, fma4(P2[n], Fn, fma4(P0[n], I[n], -(P1[n] * F[n]))));
+                st4(pflux + R->flux + (size_t)n * R->cs + p0, Fn);
+            }
+        }
+    }
+}
+
+// Interior launch: grid tiling of the (y, z) plane, 1024 cells per workgroup.
+template <bool HAS_PML>
+__device__ __forceinline__ bool interior_lane(const SweepGeom& g, const PmlTable& pml, Lane4& w, uint32_t& m) {
+    const uint32_t L = xcd_remap(blockIdx.x, g.blocks);
+    const uint32_t tile = L % g.col_tiles, chunk = L / g.col_tiles;
+    const uint32_t c0 = (tile * kSweepThreads + threadIdx.x) * 4u;
+    if (c0 >= g.cols) return false;
+    w.y = c0 / g.nz;
+    w.z0 = c0 - w.y * g.nz;
+    w.x0 = chunk * g.chunk_len;
+    w.x1 = min(g.nx, w.x0 + g.chunk_len);
+    w.region = -1;
+    m = HAS_PML ? pml_column_mask(pml, w.y, w.z0) : 0u;  // region z-bounds are multiples of 4
+    return w.x0 < w.x1;
+}
+
+// UPML launch: workgroups [blk0, blk0 + blocks) of region k tile its local
+// (y, z) plane (4 cells per lane) and march a chunk of its x range.
+__device__ __forceinline__ bool pml_lane(const PmlTable& pml, Lane4& w) {
+    const uint32_t b = xcd_remap(blockIdx.x, pml.blocks);
+    int k = 0;
+    while (k + 1 < (int)pml.n && b >= pml.r[k + 1].blk0) ++k;
+    const PmlRegion& R = pml.r[k];
+    const uint32_t lb = b - R.blk0;
+    const uint32_t tile = lb % R.tiles, chunk = lb / R.tiles;
+    const uint32_t gz = R.nz / 4;
+    const uint32_t lane = tile * kSweepThreads + threadIdx.x;
+    if (lane >= R.ny * gz) return false;
+    const uint32_t ly = lane / gz;
+    w.y = R.y0 + ly;
+    w.z0 = R.z0 + 4 * (lane - ly * gz);
+    w.x0 = R.x0 + chunk * R.chunk_len;
+    w.x1 = min(R.x1, w.x0 + R.chunk_len);
+    w.region = k;
+    return w.x0 < w.x1;
+}
+
+#define EFS_SWEEP4_PARAMS                                                                                       \
+    float *__restrict__ dst, const float *__restrict__ src, Coef cf, SweepGeom g, PmlTable pml,                \
+        const float *__restrict__ pcoef, float *__restrict__ pflux
+
+// Interior (4 waves/SIMD) and UPML (3 waves/SIMD) entry points, each with
+// FP32 arrays or table-compressed (suffix c) coefficients; the interior
+// kernels with suffix m skip the UPML regions.
+#define EFS_SWEEP4_INTERIOR(NAME, MARCH, HAS_PML, COMP)                                                           \
+    __global__ __launch_bounds__(kSweepThreads)                                                                    \
+        __attribute__((amdgpu_waves_per_eu(EFS_SWEEP_WAVES, EFS_SWEEP_WAVES))) void NAME(EFS_SWEEP4_PARAMS) {     \
+        __shared__ float2 s_tab[COMP ? 3 * kTab : 1];                                                              \
+        coef_stage<COMP>(cf, s_tab);                                                                               \
+        Lane4 w;                                                                                                   \
+        uint32_t m;                                                                                                \
+        if (!interior_lane<HAS_PML>(g, pml, w, m)) return;                                                         \
+        MARCH<true, COMP>(dst, src, cf, s_tab, g, pml, pcoef, pflux, w, m);                                        \
+    }
+
+#define EFS_SWEEP4_UPML(NAME, MARCH, COMP)                                                                         \
+    __global__ __launch_bounds__(kSweepThreads) __attribute__((                                                    \
+        amdgpu_waves_per_eu(EFS_SWEEP_WAVES_PML, EFS_SWEEP_WAVES_PML))) void NAME(EFS_SWEEP4_PARAMS) {            \
+        __shared__ float2 s_tab[COMP ? 3 * kTab : 1];                                                              \
+        coef_stage<COMP>(cf, s_tab);                                                                               \
+        Lane4 w;                                                                                                   \
+        if (!pml_lane(pml, w)) return;                                                                             \
+        MARCH<false, COMP>(dst, src, cf, s_tab, g, pml, pcoef, pflux, w, 0u);                                      \
+    }
+
+EFS_SWEEP4_INTERIOR(sweep_e4, e4_march, false, false)
+EFS_SWEEP4_INTERIOR(sweep_e4c, e4_march, false, true)
+EFS_SWEEP4_INTERIOR(sweep_e4m, e4_march, true, false)
+EFS_SWEEP4_INTERIOR(sweep_e4mc, e4_march, true, true)
+EFS_SWEEP4_UPML(sweep_e4u, e4_march, false)
+EFS_SWEEP4_UPML(sweep_e4uc, e4_march, true)
+EFS_SWEEP4_INTERIOR(sweep_h4, h4_march, false, false)
+EFS_SWEEP4_INTERIOR(sweep_h4c, h4_march, false, true)
+EFS_SWEEP4_INTERIOR(sweep_h4m, h4_march, true, false)
+EFS_SWEEP4_INTERIOR(sweep_h4mc, h4_march, true, true)
+EFS_SWEEP4_UPML(sweep_h4u, h4_march, false)
+EFS_SWEEP4_UPML(sweep_h4uc, h4_march, true)
+#undef EFS_SWEEP4_INTERIOR
+#undef EFS_SWEEP4_UPML
+#undef EFS_SWEEP4_PARAMS
 
 }  // namespace efs

@@ -13,10 +13,17 @@ def f32(rng, n, lo, hi):
     return rng.uniform(lo, hi, n).astype(np.float32)
 
 
-def random_coeffs(rng, N):
+def random_coeffs(rng, N, quantize=0):
+    """Uniform random coefficients; with quantize=k each array takes only k
+    distinct values (the case the engine's coefficient tables compress)."""
     t = 3 * N
-    return {"vv": f32(rng, t, 0.9, 1.0), "vi": f32(rng, t, 0.05, 0.3),
-            "ii": f32(rng, t, 0.9, 1.0), "iv": f32(rng, t, 0.05, 0.3)}
+
+    def arr(lo, hi):
+        if not quantize:
+            return f32(rng, t, lo, hi)
+        levels = f32(rng, quantize, lo, hi)
+        return levels[rng.integers(0, quantize, t)]
+    return {"vv": arr(0.9, 1.0), "vi": arr(0.05, 0.3), "ii": arr(0.9, 1.0), "iv": arr(0.05, 0.3)}
 
 
 def shell_regions(dims, w):
@@ -51,11 +58,12 @@ def lin(dims, x, y, z):
     return x * dims[1] * dims[2] + y * dims[2] + z
 
 
-def random_case(seed, dims, pml_width=0, exc=True, mur=False, ade=False, tfsf=False, rlc=False, fields=True):
+def random_case(seed, dims, pml_width=0, exc=True, mur=False, ade=False, tfsf=False, rlc=False, fields=True,
+                quantize=0):
     rng = np.random.default_rng(seed)
     nx, ny, nz = dims
     N = nx * ny * nz
-    case = {"dims": list(dims), "coeffs": random_coeffs(rng, N)}
+    case = {"dims": list(dims), "coeffs": random_coeffs(rng, N, quantize)}
     if fields:
         case["volt0"] = f32(rng, 3 * N, -0.5, 0.5)
         case["curr0"] = f32(rng, 3 * N, -0.5, 0.5)

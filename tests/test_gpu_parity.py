@@ -78,6 +78,13 @@ CASES = [
     ("line_8x1x1", 7, (8, 1, 1), dict(pml_width=0)),
     ("single_cell", 8, (1, 1, 1), dict(pml_width=0)),
     ("flat_y1", 9, (11, 1, 9), dict(pml_width=0)),
+    ("v4_pml_aligned", 31, (16, 12, 16), dict(pml_width=4, mur=True)),
+    ("v4_pml_split", 32, (20, 18, 24), dict(pml_width=3, ade=True, tfsf=True)),
+    ("v4_ny1", 34, (9, 1, 12), dict(pml_width=0)),
+    ("v4_nz4", 35, (6, 5, 4), dict(pml_width=1, mur=True)),
+    ("v4_rlc_tfsf", 36, (12, 10, 8), dict(pml_width=2, tfsf=True, rlc=True, ade=True)),
+    ("split_pml8_all", 37, (24, 20, 32), dict(pml_width=8, mur=True, ade=True, tfsf=True, rlc=True)),
+    ("split_pml4_tall", 38, (40, 9, 12), dict(pml_width=4, mur=True)),
 ]
 
 
@@ -93,6 +100,66 @@ def test_random_cases(HIPEngine, oracle_mod, name, seed, dims, kw):
     assert g.numTS == o.numTS == steps
     assert_fields_close(g.getFields(), o.getFields(), what=name)
     g.destroy()
+
+
+@pytest.mark.parametrize("name,seed,dims,kw", [c for c in CASES if c[2][2] % 4 == 0],
+                         ids=[c[0] for c in CASES if c[2][2] % 4 == 0])
+def test_scalar_lane_path(HIPEngine, oracle_mod, monkeypatch, name, seed, dims, kw):
+    """The same cases with the one-cell-per-lane sweeps forced (EFS_VEC=1)
+    and no layout padding (EFS_PAD=0)."""
+    monkeypatch.setenv("EFS_VEC", "1")
+    monkeypatch.setenv("EFS_PAD", "0")
+    monkeypatch.setenv("EFS_STAGGER", "0")
+    case = random_case(seed, dims, **kw)
+    say("case scalar-lane", name, seed, dims)
+    o = make_oracle(oracle_mod, case)
+    g = make_gpu(HIPEngine, case)
+    o.iterate(30)
+    g.iterate(30)
+    assert_fields_close(g.getFields(), o.getFields(), what=name + "/vec1")
+    g.destroy()
+
+
+QCASES = [
+    ("q_core", 41, (16, 14, 32), dict(quantize=23)),
+    ("q_pml_split", 42, (20, 18, 24), dict(pml_width=3, quantize=50, mur=True, ade=True, tfsf=True, rlc=True)),
+    ("q_pml_aligned", 43, (16, 16, 16), dict(pml_width=4, quantize=200, mur=True)),
+    ("q_split_pml8", 44, (26, 22, 36), dict(pml_width=8, quantize=120, mur=True, ade=True, tfsf=True, rlc=True)),
+]
+
+
+@pytest.mark.parametrize("name,seed,dims,kw", QCASES, ids=[c[0] for c in QCASES])
+def test_compressed_coefficients(HIPEngine, oracle_mod, monkeypatch, name, seed, dims, kw):
+    """Few distinct coefficients: the table-compressed sweeps must give the
+    same bits as the FP32-array sweeps (same values, same arithmetic) and
+    match the oracle."""
+    case = random_case(seed, dims, **kw)
+    say("case compressed", name, seed, dims)
+    g = make_gpu(HIPEngine, case)
+    monkeypatch.setenv("EFS_COMPRESS", "0")
+    u = make_gpu(HIPEngine, case)
+    o = make_oracle(oracle_mod, case)
+    for e in (g, u, o):
+        e.iterate(30)
+    fg, fu = g.getFields(), u.getFields()
+    np.testing.assert_array_equal(fg["volt"], fu["volt"])
+    np.testing.assert_array_equal(fg["curr"], fu["curr"])
+    assert_fields_close(fg, o.getFields(), what=name)
+
+
+def test_split_matches_unsplit(HIPEngine, monkeypatch):
+    """Interior + UPML launches vs the one-cell-per-lane fused sweeps on the
+    same aligned case: same arithmetic per cell, so identical bits."""
+    case = random_case(45, (24, 20, 32), pml_width=8, mur=True, tfsf=True)
+    say("case split vs vec1")
+    a = make_gpu(HIPEngine, case)
+    monkeypatch.setenv("EFS_VEC", "1")
+    b = make_gpu(HIPEngine, case)
+    a.iterate(25)
+    b.iterate(25)
+    fa, fb = a.getFields(), b.getFields()
+    np.testing.assert_array_equal(fa["volt"], fb["volt"])
+    np.testing.assert_array_equal(fa["curr"], fb["curr"])
 
 
 def test_ade_write_back(HIPEngine, oracle_mod):
