@@ -72,6 +72,8 @@ struct efs_ctx {
     uint32_t numTS = 0;
     SweepGeom geom{};   // 4-wide sweeps
     SweepGeom geom1{};  // 1-wide sweeps
+    SweepBlock* sweep_tab4 = nullptr;  // their block tables
+    SweepBlock* sweep_tab1 = nullptr;
 
     // excitation
     uint32_t exc_n = 0, exc_len = 0, exc_period = 0;
@@ -82,7 +84,8 @@ struct efs_ctx {
     // and 2 flux arrays per region, 16-byte aligned region bases)
     PmlTable pml{};
     float *pml_coef = nullptr, *pml_vflux = nullptr, *pml_cflux = nullptr;
-    bool pml_split = false;  // 4-wide interior + UPML launches (region z-bounds multiples of 4)
+    bool pml_split = false;  // 4-wide sweeps with UPML (region z-bounds multiples of 4)
+    bool pml_fused = true;   // one grid launch per sweep (else interior + region launches)
 
     // Mur
     uint32_t mur_n = 0;
@@ -397,6 +400,7 @@ int nf2ff_fold(efs_ctx* c) {
 
 void free_all(efs_ctx* c) {
     dfree(c->pool);
+    dfree(c->sweep_tab4); dfree(c->sweep_tab1);
     c->volt = c->curr = c->vv = c->vi = c->ii = c->iv = nullptr;
     dfree(c->cidx_e); dfree(c->cidx_h); dfree(c->ctab_e); dfree(c->ctab_h);
     c->comp = false;
@@ -441,6 +445,90 @@ bool box_overlap(const PmlRegion& a, const PmlRegion& b) {
 }
 
 // ---- one timestep ----
+// Sweep geometries (4-wide and 1-wide lanes): block tables of equal-cost
+// workgroups.  Cost model in bytes per cell: a core cell moves V r+w, H and
+// its coefficients (39 B with byte-indexed coefficients, 60 B without); a
+// UPML cell in the same launch adds flux r+w and three UPML coefficients
+// (60 B); in the interior launch of the split form it costs only the loop.
+// The launch gets ~2048 workgroups (8 per CU on 256 CUs) but chunks of 8
+// planes on average; heavy tiles go down to 2-plane chunks.
+int plan_sweeps(efs_ctx* c) {
+    const uint32_t nx = c->nx, ny = c->ny, nz = c->nz;
+    const uint64_t cols = (uint64_t)ny * nz;
+    const bool upml_in_grid = !(c->vec4 && c->pml_split && !c->pml_fused);
+    const double core_b = c->comp ? 39.0 : 60.0;
+    const double upml_b = upml_in_grid ? core_b + 60.0 : 2.0;
+    const uint64_t want = env_u64("EFS_SWEEP_BLOCKS", 2048);
+    for (int v = 0; v < 2; ++v) {
+        const uint64_t tcells = (uint64_t)kSweepThreads * (v == 0 ? 4u : 1u);
+        const uint32_t T = (uint32_t)((cols + tcells - 1) / tcells);
+        // UPML cells of each region in each tile (per plane of the region)
+        std::vector<std::vector<double>> cnt(c->pml.n, std::vector<double>(T, 0.0));
+        for (uint32_t k = 0; k < c->pml.n; ++k) {
+            const PmlRegion& R = c->pml.r[k];
+            for (uint32_t y = R.y0; y < R.y1; ++y) {
+                uint64_t a = (uint64_t)y * nz + R.z0;
+                const uint64_t e = (uint64_t)y * nz + R.z1;
+                while (a < e) {
+                    const uint64_t t = a / tcells, te = std::min(e, (t + 1) * tcells);
+                    cnt[k][t] += (double)(te - a);
+                    a = te;
+                }
+            }
+        }
+        std::vector<double> cost((size_t)T * nx);  // [tile][x]
+        double total = 0.0;
+        for (uint32_t t = 0; t < T; ++t) {
+            const double cells = (double)(std::min(cols, (uint64_t)(t + 1) * tcells) - (uint64_t)t * tcells);
+            for (uint32_t x = 0; x < nx; ++x) cost[(size_t)t * nx + x] = core_b * cells;
+        }
+        for (uint32_t k = 0; k < c->pml.n; ++k)
+            for (uint32_t t = 0; t < T; ++t)
+                if (cnt[k][t] > 0)
+                    for (uint32_t x = c->pml.r[k].x0; x < c->pml.r[k].x1; ++x)
+                        cost[(size_t)t * nx + x] += (upml_b - core_b) * cnt[k][t];
+        for (double q : cost) total += q;
+        const double nblocks = (double)std::max<uint64_t>(1, std::min<uint64_t>(want, (uint64_t)T * std::max(1u, nx / 8)));
+        const double target = total / nblocks;
+        std::vector<SweepBlock> tab;
+        std::vector<double> pre(nx + 1);
+        for (uint32_t t = 0; t < T; ++t) {
+            const double* ct = &cost[(size_t)t * nx];
+            pre[0] = 0.0;
+            for (uint32_t x = 0; x < nx; ++x) pre[x + 1] = pre[x] + ct[x];
+            const uint32_t n = (uint32_t)std::max(1.0, std::min((double)std::max(1u, nx / 2),
+                                                                std::floor(pre[nx] / target + 0.5)));
+            // boundary j at the plane whose prefix cost is nearest j/n of the tile's total
+            uint32_t x = 0, b0 = 0;
+            for (uint32_t j = 1; j <= n; ++j) {
+                uint32_t b = nx;
+                if (j < n) {
+                    const double goal = pre[nx] * j / n;
+                    while (x < nx && pre[x + 1] <= goal) ++x;
+                    b = (x < nx && goal - pre[x] > pre[x + 1] - goal) ? x + 1 : x;
+                }
+                if (b > b0) {
+                    tab.push_back(SweepBlock{t, b0, b, 0});
+                    b0 = b;
+                }
+            }
+        }
+        std::stable_sort(tab.begin(), tab.end(), [](const SweepBlock& a, const SweepBlock& b) {
+            return a.x0 + a.x1 != b.x0 + b.x1 ? a.x0 + a.x1 < b.x0 + b.x1 : a.tile < b.tile;
+        });
+        SweepGeom g{};
+        g.nx = nx; g.ny = ny; g.nz = nz;
+        g.cols = (uint32_t)cols;
+        g.S = c->S;
+        g.blocks = (uint32_t)tab.size();
+        SweepBlock*& dtab = v == 0 ? c->sweep_tab4 : c->sweep_tab1;
+        TRY(dupload(c, dtab, tab.data(), tab.size()));
+        g.tab = dtab;
+        (v == 0 ? c->geom : c->geom1) = g;
+    }
+    return EFS_OK;
+}
+
 int launch_step(efs_ctx* c) {
     const uint32_t t = c->numTS;
     ProfScope step_scope(c, EFS_PROF_STEP);
@@ -465,9 +553,10 @@ int launch_step(efs_ctx* c) {
         if (c->vec4 && (!has_pml || c->pml_split)) {
             const Coef cf{c->vv, c->vi, c->cidx_e, c->ctab_e};
             auto kern = has_pml ? (c->comp ? sweep_e4mc : sweep_e4m) : (c->comp ? sweep_e4c : sweep_e4);
+            if (has_pml && c->pml_fused) kern = c->comp ? sweep_e4fc : sweep_e4f;
             hipLaunchKernelGGL(kern, dim3(c->geom.blocks), dim3(kSweepThreads), 0, c->stream, c->volt, c->curr, cf,
                                c->geom, c->pml, c->pml_coef, c->pml_vflux);
-            if (has_pml)
+            if (has_pml && !c->pml_fused)
                 hipLaunchKernelGGL(c->comp ? sweep_e4uc : sweep_e4u, dim3(c->pml.blocks), dim3(kSweepThreads), 0,
                                    c->stream, c->volt, c->curr, cf, c->geom, c->pml, c->pml_coef, c->pml_vflux);
         } else {
@@ -514,9 +603,10 @@ int launch_step(efs_ctx* c) {
         if (c->vec4 && (!has_pml || c->pml_split)) {
             const Coef cf{c->ii, c->iv, c->cidx_h, c->ctab_h};
             auto kern = has_pml ? (c->comp ? sweep_h4mc : sweep_h4m) : (c->comp ? sweep_h4c : sweep_h4);
+            if (has_pml && c->pml_fused) kern = c->comp ? sweep_h4fc : sweep_h4f;
             hipLaunchKernelGGL(kern, dim3(c->geom.blocks), dim3(kSweepThreads), 0, c->stream, c->curr, c->volt, cf,
                                c->geom, c->pml, c->pml_coef, c->pml_cflux);
-            if (has_pml)
+            if (has_pml && !c->pml_fused)
                 hipLaunchKernelGGL(c->comp ? sweep_h4uc : sweep_h4u, dim3(c->pml.blocks), dim3(kSweepThreads), 0,
                                    c->stream, c->curr, c->volt, cf, c->geom, c->pml, c->pml_coef, c->pml_cflux);
         } else {
@@ -663,24 +753,7 @@ int efs_init(efs_ctx* c, uint32_t nx, uint32_t ny, uint32_t nz, const float* vv,
         }
     }
     TRY(dalloc(c, c->en_partial, kEnergyMaxBlocks + 1));
-    // sweep geometries: 256 lanes per workgroup over contiguous (y,z)
-    // columns (1 or 4 cells per lane), x split in chunks so the launch has
-    // ~8 workgroups per CU (256 CUs)
-    const uint32_t want = (uint32_t)env_u64("EFS_SWEEP_BLOCKS", 2048);
-    for (int v = 0; v < 2; ++v) {
-        SweepGeom g{};
-        g.nx = nx; g.ny = ny; g.nz = nz;
-        g.cols = ny * nz;
-        const uint32_t per_block = kSweepThreads * (v == 0 ? 4u : 1u);
-        g.col_tiles = (g.cols + per_block - 1) / per_block;
-        g.S = c->S;
-        uint32_t chunks = (want + g.col_tiles - 1) / g.col_tiles;
-        chunks = std::max(1u, std::min(chunks, std::max(1u, nx / 8)));
-        g.chunk_len = (nx + chunks - 1) / chunks;
-        g.chunks = (nx + g.chunk_len - 1) / g.chunk_len;
-        g.blocks = g.col_tiles * g.chunks;
-        (v == 0 ? c->geom : c->geom1) = g;
-    }
+    TRY(plan_sweeps(c));
     c->numTS = 0;
     c->inited = true;
     return EFS_OK;
@@ -743,14 +816,16 @@ int efs_cfg_pml(efs_ctx* c, uint32_t nreg, const uint32_t* start, const uint32_t
     }
     if (total >= (1ull << 31)) return fail(c, EFS_ERR_ARG, "UPML volume too large");
     // UPML launch geometry: per region, 256-lane tiles over its (y, z/4)
-    // plane and x chunks sized to give the launch ~2 workgroups per CU
+    // plane and x chunks; the launch's workgroups are shared out in
+    // proportion to region volume
+    const uint64_t want = env_u64("EFS_PML_BLOCKS", 512);
     uint32_t blocks = 0;
     for (uint32_t k = 0; k < nreg; ++k) {
         PmlRegion& R = t.r[k];
         const uint32_t lanes = R.ny * ((R.nz + 3) / 4);
         R.tiles = (lanes + kSweepThreads - 1) / kSweepThreads;
         const uint32_t nxr = R.x1 - R.x0;
-        uint32_t chunks = std::max(1u, std::min(nxr, (uint32_t)((uint64_t)512 * R.cs / std::max<uint64_t>(total / 3, 1) / R.tiles)));
+        uint32_t chunks = std::max(1u, std::min(nxr, (uint32_t)(want * R.cs / std::max<uint64_t>(total / 3, 1) / R.tiles)));
         R.chunk_len = (nxr + chunks - 1) / chunks;
         chunks = (nxr + R.chunk_len - 1) / R.chunk_len;
         R.blocks = R.tiles * chunks;
@@ -779,6 +854,8 @@ int efs_cfg_pml(efs_ctx* c, uint32_t nreg, const uint32_t* start, const uint32_t
     }
     c->pml = t;
     c->pml_split = aligned && env_u64("EFS_PML_SPLIT", 1) != 0;
+    c->pml_fused = env_u64("EFS_PML_FUSED", 1) != 0;
+    TRY(plan_sweeps(c));
     c->alias_dirty = true;
     return EFS_OK;
 }

@@ -12,19 +12,21 @@
 // the previous plane, so each H value (E sweep) or E value (H sweep) comes
 // from HBM once; y and z neighbours belong to adjacent lanes or waves of
 // the same or the neighbouring workgroup and are served by L1/L2.
-// Workgroup ids are remapped so each XCD (own L2) gets a contiguous range of
-// tiles.  Components are S floats apart (S >= Nx*Ny*Nz, padded).
+// Work is cut into workgroups of one column tile and a chunk of planes by
+// a host-built table of equal-cost blocks (see SweepBlock); workgroup ids
+// are remapped so each XCD (own L2) gets a contiguous run of the table.
+// Components are S floats apart (S >= Nx*Ny*Nz, padded).
 //
-// Each sweep is two launches over disjoint cells:
-//   sweep_*4  interior: every lane whose 4 cells lie outside all UPML
-//             regions (lanes inside a region are predicated off);
-//   sweep_*4u UPML: one grid over the region boxes, so its waves are
-//             uniformly UPML and carry the UPML flux and coefficients,
-//             stored compactly per region (region-local NIJK, as the
-//             reference hands them over).
-// This keeps the interior code lean (high occupancy, no divergent UPML path
-// in every wave) and needs region z-bounds that are multiples of 4; other
-// grids use the one-cell-per-lane sweeps (sweep_*1).
+// UPML cells are stored compactly per region (region-local NIJK, as the
+// reference hands them over).  Default form, one launch per sweep
+// (sweep_*4f): the grid launch updates every cell and branches per lane and
+// plane into the UPML update; the branch is uniform over whole waves in x-
+// and y-slabs and diverges only on the few lanes of a z-slab.  Every row of
+// the main arrays is then read whole, once.  Alternative form (kept for
+// comparison and tests): an interior launch that skips UPML cells
+// (sweep_*4m) plus a launch over the region boxes (sweep_*4u).  Both need
+// region z-bounds that are multiples of 4; other grids use the one-cell-
+// per-lane sweeps (sweep_*1).
 //
 // Coefficients: two FP32 arrays (a, b) per component, or -- when every
 // component has at most 256 distinct (a, b) pairs -- one byte per cell
@@ -76,14 +78,24 @@ struct PmlTable {
     PmlRegion r[kMaxPml];
 };
 
+// One workgroup of a grid launch: a column tile (256 lanes over
+// contiguous (y, z) columns) and the x planes [x0, x1) it marches over.
+struct SweepBlock {
+    uint32_t tile, x0, x1, pad;
+};
+
+// Grid launches read their work from a host-built block table: each column
+// tile's x range is cut into chunks of equal bytes moved (UPML cells move
+// flux and UPML coefficients too, so tiles and planes full of them get
+// shorter chunks), and the table is ordered by x, so the contiguous runs of
+// workgroups each XCD receives form x slabs of equal work.
 struct SweepGeom {
     uint32_t nx, ny, nz;
     uint32_t cols;        // ny*nz
-    uint32_t col_tiles;   // ceil(cols / (256*vec))
-    uint32_t chunks;      // x chunks
-    uint32_t chunk_len;   // planes per chunk
-    uint32_t blocks;      // col_tiles * chunks
+    uint32_t blocks;      // entries of tab
+    uint32_t pad;
     uint64_t S;           // component stride in floats
+    const SweepBlock* tab;
 };
 
 // XCD-contiguous remap of the hardware workgroup id: dispatch hands out
@@ -148,14 +160,27 @@ __device__ __forceinline__ void coef_stage(const Coef& cf, float2* s_tab) {
     }
 }
 
+// Two steps so a caller can issue every global load of an iteration before
+// the first wait: coef_fetch loads the byte indices (or the FP32 arrays),
+// coef_decode turns indices into values through the LDS table.
 template <bool COMP>
-__device__ __forceinline__ void coef_load(const Coef& cf, const float2* s_tab, size_t S, uint32_t i, v4 a[3],
-                                          v4 b[3]) {
+__device__ __forceinline__ void coef_fetch(const Coef& cf, size_t S, uint32_t i, uint32_t q[3], v4 a[3], v4 b[3]) {
     if (COMP) {
-        uint32_t q[3];
 #pragma unroll
         for (int n = 0; n < 3; ++n)
             q[n] = __builtin_nontemporal_load(reinterpret_cast<const uint32_t*>(cf.idx + n * S + i));
+    } else {
+#pragma unroll
+        for (int n = 0; n < 3; ++n) {
+            a[n] = ld4nt(cf.a + n * S + i);
+            b[n] = ld4nt(cf.b + n * S + i);
+        }
+    }
+}
+
+template <bool COMP>
+__device__ __forceinline__ void coef_decode(const float2* s_tab, const uint32_t q[3], v4 a[3], v4 b[3]) {
+    if (COMP) {
 #pragma unroll
         for (int n = 0; n < 3; ++n) {
             const float2* t = s_tab + n * kTab;
@@ -163,12 +188,6 @@ __device__ __forceinline__ void coef_load(const Coef& cf, const float2* s_tab, s
                          p3 = t[q[n] >> 24];
             a[n] = (v4){p0.x, p1.x, p2.x, p3.x};
             b[n] = (v4){p0.y, p1.y, p2.y, p3.y};
-        }
-    } else {
-#pragma unroll
-        for (int n = 0; n < 3; ++n) {
-            a[n] = ld4nt(cf.a + n * S + i);
-            b[n] = ld4nt(cf.b + n * S + i);
         }
     }
 }
@@ -205,13 +224,12 @@ __global__ __launch_bounds__(kSweepThreads) void sweep_e1(float* __restrict__ vo
                                                           const float* __restrict__ vi, SweepGeom g,
                                                           PmlTable pml, const float* __restrict__ pcoef,
                                                           float* __restrict__ pflux) {
-    const uint32_t L = xcd_remap(blockIdx.x, g.blocks);
-    const uint32_t tile = L % g.col_tiles, chunk = L / g.col_tiles;
-    const uint32_t c = tile * kSweepThreads + threadIdx.x;
+    const SweepBlock d = g.tab[xcd_remap(blockIdx.x, g.blocks)];
+    const uint32_t c = d.tile * kSweepThreads + threadIdx.x;
     if (c >= g.cols) return;
     const uint32_t y = c / g.nz, z = c - y * g.nz;
-    const uint32_t x0 = chunk * g.chunk_len;
-    const uint32_t x1 = min(g.nx, x0 + g.chunk_len);
+    const uint32_t x0 = d.x0;
+    const uint32_t x1 = d.x1;
     if (x0 >= x1) return;
     const size_t plane = g.cols, S = g.S;
     const size_t sy = y > 0 ? g.nz : 0;
@@ -242,13 +260,12 @@ __global__ __launch_bounds__(kSweepThreads) void sweep_h1(float* __restrict__ cu
                                                           const float* __restrict__ iv, SweepGeom g,
                                                           PmlTable pml, const float* __restrict__ pcoef,
                                                           float* __restrict__ pflux) {
-    const uint32_t L = xcd_remap(blockIdx.x, g.blocks);
-    const uint32_t tile = L % g.col_tiles, chunk = L / g.col_tiles;
-    const uint32_t c = tile * kSweepThreads + threadIdx.x;
+    const SweepBlock d = g.tab[xcd_remap(blockIdx.x, g.blocks)];
+    const uint32_t c = d.tile * kSweepThreads + threadIdx.x;
     if (c >= g.cols) return;
     const uint32_t y = c / g.nz, z = c - y * g.nz;
-    const uint32_t x0 = chunk * g.chunk_len;
-    const uint32_t x1 = min(g.nx, x0 + g.chunk_len);
+    const uint32_t x0 = d.x0;
+    const uint32_t x1 = d.x1;
     if (x0 >= x1) return;
     const size_t plane = g.cols, S = g.S;
     const bool col_core = (y + 1 < g.ny) && (z + 1 < g.nz);
@@ -287,17 +304,75 @@ __global__ __launch_bounds__(kSweepThreads) void sweep_h1(float* __restrict__ cu
 // ---------------------------------------------------------------- 4 cells/lane
 
 // Per-lane work of the 4-wide sweeps: the lane's (y, z0) column and its x
-// range.  The interior launch derives it from the grid tiling, the UPML
+// range.  The grid launches derive it from the grid tiling, the region
 // launch from the region boxes.
 struct Lane4 {
     uint32_t y, z0, x0, x1;
-    int region;  // UPML region of the lane (UPML launch), -1 in the interior launch
+    int region;  // UPML region of the lane (region launch), -1 otherwise
 };
 
-// E update of 4 cells over x in [x0, x1).  INTERIOR: skip planes where the
-// lane is inside a UPML region (column mask m); otherwise every plane is in
-// region w.region (compact region arrays).
-template <bool INTERIOR, bool COMP>
+// What a march does with the UPML cells it meets.
+enum : int {
+    kMarchInterior = 0,  // grid launch, UPML planes skipped (another launch does them)
+    kMarchRegion = 1,    // region launch: every plane is in region w.region
+    kMarchFused = 2,     // grid launch doing both: per plane, core or UPML update
+};
+
+// A lane's position in the compact UPML arrays as it marches along x.
+// The region lookup (a walk over the column's region mask, reading the
+// region table) runs only when x reaches `until`, i.e. a few times per
+// march; on the other planes the pointers advance by one region plane.
+// This keeps the table reads off the per-plane path, where they would
+// delay the UPML loads by a round trip.
+struct PmlCursor {
+    int k;            // region of the current plane, -1 = none
+    uint32_t until;   // the state holds for planes < until
+    float* flux;      // flux of component 0 at the lane's first cell
+    const float* coef;  // first of the three coefficient arrays (component 0)
+    uint32_t cs;      // component stride of the region arrays
+    uint32_t step;    // floats per region plane (ny*nz of the region)
+};
+
+template <bool HF>
+__device__ __forceinline__ void cursor_seek(PmlCursor& u, const PmlTable& pml, uint32_t m, uint32_t x, uint32_t y,
+                                            uint32_t z0, const float* __restrict__ pcoef,
+                                            float* __restrict__ pflux) {
+    u.k = -1;
+    u.until = 0xffffffffu;
+    while (m) {
+        const int k = __builtin_ctz(m);
+        m &= m - 1u;
+        const PmlRegion& R = pml.r[k];
+        if (x >= R.x0 && x < R.x1) {
+            const size_t p0 = ((size_t)(x - R.x0) * R.ny + (y - R.y0)) * R.nz + (z0 - R.z0);
+            u.k = k;
+            u.until = R.x1;
+            u.cs = R.cs;
+            u.step = R.ny * R.nz;
+            u.flux = pflux + R.flux + p0;
+            u.coef = pcoef + R.coef + (HF ? 9 * (size_t)R.cs : 0) + p0;
+            return;
+        }
+        if (R.x0 > x) u.until = min(u.until, R.x0);
+    }
+}
+
+// Position the cursor on plane x (called for consecutive x).
+template <bool HF>
+__device__ __forceinline__ void cursor_at(PmlCursor& u, const PmlTable& pml, uint32_t m, uint32_t x, uint32_t y,
+                                          uint32_t z0, const float* __restrict__ pcoef,
+                                          float* __restrict__ pflux) {
+    if (x >= u.until) {
+        cursor_seek<HF>(u, pml, m, x, y, z0, pcoef, pflux);
+    } else if (u.k >= 0) {
+        u.flux += u.step;
+        u.coef += u.step;
+    }
+}
+
+// E update of 4 cells over x in [x0, x1); MODE says how UPML cells are
+// treated (column mask m gives the regions the lane's column crosses).
+template <int MODE, bool COMP>
 __device__ __forceinline__ void e4_march(float* __restrict__ volt, const float* __restrict__ curr, const Coef& cf,
                                          const float2* s_tab, const SweepGeom& g, const PmlTable& pml,
                                          const float* __restrict__ pcoef, float* __restrict__ pflux, Lane4 w,
@@ -313,9 +388,11 @@ __device__ __forceinline__ void e4_march(float* __restrict__ volt, const float* 
     uint32_t i = w.x0 * plane + w.y * g.nz + w.z0;
     bool carry = false;  // hy_m/hz_m hold H at x-1
     v4 hy_m = 0.f, hz_m = 0.f;
-    const PmlRegion* R = INTERIOR ? nullptr : &pml.r[w.region];
+    PmlCursor u{-1, m ? 0u : 0xffffffffu, nullptr, nullptr, 0u, 0u};
     for (uint32_t x = w.x0; x < w.x1; ++x, i += plane) {
-        if (INTERIOR && m && pml_find(pml, m, x) >= 0) {
+        cursor_at<false>(u, pml, m, x, w.y, w.z0, pcoef, pflux);
+        const bool upml = MODE == kMarchRegion || (MODE == kMarchFused && u.k >= 0);
+        if (MODE == kMarchInterior && u.k >= 0) {
             carry = false;
             continue;
         }
@@ -328,39 +405,37 @@ __device__ __forceinline__ void e4_march(float* __restrict__ volt, const float* 
         const v4 hz_ym = ld4(H2 + i - sy), hx_ym = ld4(H0 + i - sy);
         const float hy_z = H1[i - sz], hx_z = H0[i - sz];  // z-1 of the first cell
         v4 V[3], a[3], b[3];
+        uint32_t q[3];
 #pragma unroll
         for (int n = 0; n < 3; ++n) V[n] = ld4(E[n] + i);
-        coef_load<COMP>(cf, s_tab, S, i, a, b);
+        coef_fetch<COMP>(cf, S, i, q, a, b);
         v4 F[3], P0[3], P1[3], P2[3];
-        size_t p0 = 0;
-        if (!INTERIOR) {
-            p0 = ((size_t)(x - R->x0) * R->ny + (w.y - R->y0)) * R->nz + (w.z0 - R->z0);
-            const size_t T = 3 * (size_t)R->cs;
-            const float* cfp = pcoef + R->coef;
-            const float* fl = pflux + R->flux;
+        if (upml) {
+            const size_t T = 3 * (size_t)u.cs;
 #pragma unroll
             for (int n = 0; n < 3; ++n) {
-                const size_t p = (size_t)n * R->cs + p0;
-                F[n] = ld4(fl + p);
-                P0[n] = ld4nt(cfp + p);
-                P1[n] = ld4nt(cfp + T + p);
-                P2[n] = ld4nt(cfp + 2 * T + p);
+                F[n] = ld4(u.flux + n * u.cs);
+                P0[n] = ld4nt(u.coef + n * u.cs);
+                P1[n] = ld4nt(u.coef + T + n * u.cs);
+                P2[n] = ld4nt(u.coef + 2 * T + n * u.cs);
             }
         }
+        coef_decode<COMP>(s_tab, q, a, b);
         const v4 hy_zm = (v4){hy_z, hy.x, hy.y, hy.z};
         const v4 hx_zm = (v4){hx_z, hx.x, hx.y, hx.z};
         const v4 cur[3] = {hz - hz_ym - hy + hy_zm, hx - hx_zm - hz + hz_m, hy - hy_m - hx + hx_ym};
         hy_m = hy;
         hz_m = hz;
         carry = true;
+        if (!upml) {
 #pragma unroll
-        for (int n = 0; n < 3; ++n) {
-            if (INTERIOR) {
-                st4(E[n] + i, fma4(a[n], V[n], b[n] * cur[n]));
-            } else {
+            for (int n = 0; n < 3; ++n) st4(E[n] + i, fma4(a[n], V[n], b[n] * cur[n]));
+        } else {
+#pragma unroll
+            for (int n = 0; n < 3; ++n) {
                 const v4 Fn = fma4(a[n], F[n], b[n] * cur[n]);
                 st4(E[n] + i, fma4(P2[n], Fn, fma4(P0[n], V[n], -(P1[n] * F[n]))));
-                st4(pflux + R->flux + (size_t)n * R->cs + p0, Fn);
+                st4(u.flux + n * u.cs, Fn);
             }
         }
     }
@@ -368,7 +443,7 @@ __device__ __forceinline__ void e4_march(float* __restrict__ volt, const float* 
 
 // H update of 4 cells over x in [x0, x1) (see e4_march).  Cells outside the
 // core bounds (x = Nx-1, y = Ny-1, z = Nz-1) keep I unless in UPML.
-template <bool INTERIOR, bool COMP>
+template <int MODE, bool COMP>
 __device__ __forceinline__ void h4_march(float* __restrict__ curr, const float* __restrict__ volt, const Coef& cf,
                                          const float2* s_tab, const SweepGeom& g, const PmlTable& pml,
                                          const float* __restrict__ pcoef, float* __restrict__ pflux, Lane4 w,
@@ -381,16 +456,19 @@ __device__ __forceinline__ void h4_march(float* __restrict__ curr, const float* 
     const uint32_t plane = g.cols;
     const bool row_core = w.y + 1 < g.ny;
     const bool last = w.z0 + 4 >= g.nz;  // the lane holds the row's last z cell (no core update there)
-    if (INTERIOR && !row_core) return;
+    if (MODE == kMarchInterior && !row_core) return;
+    if (MODE == kMarchFused && !row_core && !m) return;
     const uint32_t sy = row_core ? g.nz : 0u;  // clamped forward neighbours stay in bounds
     const uint32_t sz = last ? 3u : 4u;
     uint32_t i = w.x0 * plane + w.y * g.nz + w.z0;
     bool carry = false;  // ex/ey/ez hold E at x
     v4 ex = 0.f, ey = 0.f, ez = 0.f;
-    const PmlRegion* R = INTERIOR ? nullptr : &pml.r[w.region];
+    PmlCursor u{-1, m ? 0u : 0xffffffffu, nullptr, nullptr, 0u, 0u};
     for (uint32_t x = w.x0; x < w.x1; ++x, i += plane) {
         const bool core = row_core && (x + 1 < g.nx);
-        if (INTERIOR && (!core || (m && pml_find(pml, m, x) >= 0))) {
+        cursor_at<true>(u, pml, m, x, w.y, w.z0, pcoef, pflux);
+        const bool upml = MODE == kMarchRegion || (MODE == kMarchFused && u.k >= 0);
+        if (MODE != kMarchRegion && !upml && (!core || u.k >= 0)) {
             carry = false;
             continue;
         }
@@ -406,25 +484,22 @@ __device__ __forceinline__ void h4_march(float* __restrict__ curr, const float* 
         const v4 ez_yp = ld4(E2 + i + sy), ex_yp = ld4(E0 + i + sy);
         const float ey_z = E1[i + sz], ex_z = E0[i + sz];  // z+1 of the fourth cell
         v4 I[3], a[3], b[3];
+        uint32_t q[3];
 #pragma unroll
         for (int n = 0; n < 3; ++n) I[n] = ld4(Ic[n] + i);
-        coef_load<COMP>(cf, s_tab, S, i, a, b);
+        coef_fetch<COMP>(cf, S, i, q, a, b);
         v4 F[3], P0[3], P1[3], P2[3];
-        size_t p0 = 0;
-        if (!INTERIOR) {
-            p0 = ((size_t)(x - R->x0) * R->ny + (w.y - R->y0)) * R->nz + (w.z0 - R->z0);
-            const size_t T = 3 * (size_t)R->cs;
-            const float* cfp = pcoef + R->coef + 3 * T;  // ii, iifo, iifn
-            const float* fl = pflux + R->flux;
+        if (upml) {  // ii, iifo, iifn
+            const size_t T = 3 * (size_t)u.cs;
 #pragma unroll
             for (int n = 0; n < 3; ++n) {
-                const size_t p = (size_t)n * R->cs + p0;
-                F[n] = ld4(fl + p);
-                P0[n] = ld4nt(cfp + p);
-                P1[n] = ld4nt(cfp + T + p);
-                P2[n] = ld4nt(cfp + 2 * T + p);
+                F[n] = ld4(u.flux + n * u.cs);
+                P0[n] = ld4nt(u.coef + n * u.cs);
+                P1[n] = ld4nt(u.coef + T + n * u.cs);
+                P2[n] = ld4nt(u.coef + 2 * T + n * u.cs);
             }
         }
+        coef_decode<COMP>(s_tab, q, a, b);
         const v4 ey_zp = (v4){ey.y, ey.z, ey.w, ey_z};
         const v4 ex_zp = (v4){ex.y, ex.z, ex.w, ex_z};
         const v4 cur[3] = {ez - ez_yp - ey + ey_zp, ex - ex_zp - ez + ez_p, ey - ey_p - ex + ex_yp};
@@ -432,14 +507,15 @@ __device__ __forceinline__ void h4_march(float* __restrict__ curr, const float* 
         ey = ey_p;
         ez = ez_p;
         carry = core;
+        if (!upml) {
 #pragma unroll
-        for (int n = 0; n < 3; ++n) {
-            if (INTERIOR) {
-                st4(Ic[n] + i, sel4(cb, fma4(a[n], I[n], b[n] * cur[n]), I[n]));
-            } else {
+            for (int n = 0; n < 3; ++n) st4(Ic[n] + i, sel4(cb, fma4(a[n], I[n], b[n] * cur[n]), I[n]));
+        } else {
+#pragma unroll
+            for (int n = 0; n < 3; ++n) {
                 const v4 Fn = sel4(cb, fma4(a[n], F[n], b[n] * cur[n]), F[n]);
                 st4(Ic[n] + i, fma4(P2[n], Fn, fma4(P0[n], I[n], -(P1[n] * F[n]))));
-                st4(pflux + R->flux + (size_t)n * R->cs + p0, Fn);
+                st4(u.flux + n * u.cs, Fn);
             }
         }
     }
@@ -448,14 +524,13 @@ __device__ __forceinline__ void h4_march(float* __restrict__ curr, const float* 
 // Interior launch: grid tiling of the (y, z) plane, 1024 cells per workgroup.
 template <bool HAS_PML>
 __device__ __forceinline__ bool interior_lane(const SweepGeom& g, const PmlTable& pml, Lane4& w, uint32_t& m) {
-    const uint32_t L = xcd_remap(blockIdx.x, g.blocks);
-    const uint32_t tile = L % g.col_tiles, chunk = L / g.col_tiles;
-    const uint32_t c0 = (tile * kSweepThreads + threadIdx.x) * 4u;
+    const SweepBlock d = g.tab[xcd_remap(blockIdx.x, g.blocks)];
+    const uint32_t c0 = (d.tile * kSweepThreads + threadIdx.x) * 4u;
     if (c0 >= g.cols) return false;
     w.y = c0 / g.nz;
     w.z0 = c0 - w.y * g.nz;
-    w.x0 = chunk * g.chunk_len;
-    w.x1 = min(g.nx, w.x0 + g.chunk_len);
+    w.x0 = d.x0;
+    w.x1 = d.x1;
     w.region = -1;
     m = HAS_PML ? pml_column_mask(pml, w.y, w.z0) : 0u;  // region z-bounds are multiples of 4
     return w.x0 < w.x1;
@@ -497,7 +572,7 @@ __device__ __forceinline__ bool pml_lane(const PmlTable& pml, Lane4& w) {
         Lane4 w;                                                                                                   \
         uint32_t m;                                                                                                \
         if (!interior_lane<HAS_PML>(g, pml, w, m)) return;                                                         \
-        MARCH<true, COMP>(dst, src, cf, s_tab, g, pml, pcoef, pflux, w, m);                                        \
+        MARCH<kMarchInterior, COMP>(dst, src, cf, s_tab, g, pml, pcoef, pflux, w, m);                              \
     }
 
 #define EFS_SWEEP4_UPML(NAME, MARCH, COMP)                                                                         \
@@ -507,7 +582,20 @@ __device__ __forceinline__ bool pml_lane(const PmlTable& pml, Lane4& w) {
         coef_stage<COMP>(cf, s_tab);                                                                               \
         Lane4 w;                                                                                                   \
         if (!pml_lane(pml, w)) return;                                                                             \
-        MARCH<false, COMP>(dst, src, cf, s_tab, g, pml, pcoef, pflux, w, 0u);                                      \
+        MARCH<kMarchRegion, COMP>(dst, src, cf, s_tab, g, pml, pcoef, pflux, w, 1u << w.region);                   \
+    }
+
+// Fused: the grid launch does the UPML cells too (3 waves/SIMD, the UPML
+// register budget).
+#define EFS_SWEEP4_FUSED(NAME, MARCH, COMP)                                                                        \
+    __global__ __launch_bounds__(kSweepThreads) __attribute__((                                                    \
+        amdgpu_waves_per_eu(EFS_SWEEP_WAVES_PML, EFS_SWEEP_WAVES_PML))) void NAME(EFS_SWEEP4_PARAMS) {            \
+        __shared__ float2 s_tab[COMP ? 3 * kTab : 1];                                                              \
+        coef_stage<COMP>(cf, s_tab);                                                                               \
+        Lane4 w;                                                                                                   \
+        uint32_t m;                                                                                                \
+        if (!interior_lane<true>(g, pml, w, m)) return;                                                            \
+        MARCH<kMarchFused, COMP>(dst, src, cf, s_tab, g, pml, pcoef, pflux, w, m);                                 \
     }
 
 EFS_SWEEP4_INTERIOR(sweep_e4, e4_march, false, false)
@@ -516,14 +604,19 @@ EFS_SWEEP4_INTERIOR(sweep_e4m, e4_march, true, false)
 EFS_SWEEP4_INTERIOR(sweep_e4mc, e4_march, true, true)
 EFS_SWEEP4_UPML(sweep_e4u, e4_march, false)
 EFS_SWEEP4_UPML(sweep_e4uc, e4_march, true)
+EFS_SWEEP4_FUSED(sweep_e4f, e4_march, false)
+EFS_SWEEP4_FUSED(sweep_e4fc, e4_march, true)
 EFS_SWEEP4_INTERIOR(sweep_h4, h4_march, false, false)
 EFS_SWEEP4_INTERIOR(sweep_h4c, h4_march, false, true)
 EFS_SWEEP4_INTERIOR(sweep_h4m, h4_march, true, false)
 EFS_SWEEP4_INTERIOR(sweep_h4mc, h4_march, true, true)
 EFS_SWEEP4_UPML(sweep_h4u, h4_march, false)
 EFS_SWEEP4_UPML(sweep_h4uc, h4_march, true)
+EFS_SWEEP4_FUSED(sweep_h4f, h4_march, false)
+EFS_SWEEP4_FUSED(sweep_h4fc, h4_march, true)
 #undef EFS_SWEEP4_INTERIOR
 #undef EFS_SWEEP4_UPML
+#undef EFS_SWEEP4_FUSED
 #undef EFS_SWEEP4_PARAMS
 
 }  // namespace efs

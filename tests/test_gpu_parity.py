@@ -162,6 +162,30 @@ def test_split_matches_unsplit(HIPEngine, monkeypatch):
     np.testing.assert_array_equal(fa["curr"], fb["curr"])
 
 
+@pytest.mark.parametrize("name,seed,dims,kw", [
+    ("fused_pml8_all", 46, (24, 20, 32), dict(pml_width=8, mur=True, tfsf=True, ade=True, rlc=True)),
+    ("fused_pml4_q", 47, (20, 16, 24), dict(pml_width=4, mur=True, quantize=5)),
+    ("fused_tall", 48, (40, 9, 12), dict(pml_width=4)),
+])
+def test_fused_matches_two_launch(HIPEngine, oracle_mod, monkeypatch, name, seed, dims, kw):
+    """One grid launch doing core and UPML cells vs the interior launch plus
+    the region launch: same arithmetic per cell, so identical bits; both
+    against the oracle too."""
+    case = random_case(seed, dims, **kw)
+    say(f"case {name} fused vs two launches")
+    a = make_gpu(HIPEngine, case)
+    monkeypatch.setenv("EFS_PML_FUSED", "0")
+    b = make_gpu(HIPEngine, case)
+    o = make_oracle(oracle_mod, case)
+    a.iterate(30)
+    b.iterate(30)
+    o.iterate(30)
+    fa, fb = a.getFields(), b.getFields()
+    np.testing.assert_array_equal(fa["volt"], fb["volt"])
+    np.testing.assert_array_equal(fa["curr"], fb["curr"])
+    assert_fields_close(fa, o.getFields(), what=name)
+
+
 def test_ade_write_back(HIPEngine, oracle_mod):
     """Optional openEMS ADE write-back, same switch on both sides."""
     case = random_case(11, (14, 12, 10), pml_width=2, ade=True)
