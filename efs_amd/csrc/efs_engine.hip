@@ -86,6 +86,10 @@ struct efs_ctx {
     float *pml_coef = nullptr, *pml_vflux = nullptr, *pml_cflux = nullptr;
     bool pml_split = false;  // 4-wide sweeps with UPML (region z-bounds multiples of 4)
     bool pml_fused = true;   // one grid launch per sweep (else interior + region launches)
+    bool pml_comp = false;   // UPML coefficients also as byte indices + tables (fused sweeps)
+    uint8_t* pml_idx = nullptr;   // per region: E then H component indices (PmlRegion::pidx)
+    float4* pml_tab_e = nullptr;  // [3][256] (vv, vvfo, vvfn, 0)
+    float4* pml_tab_h = nullptr;  // [3][256] (ii, iifo, iifn, 0)
 
     // Mur
     uint32_t mur_n = 0;
@@ -132,6 +136,10 @@ struct efs_ctx {
     double nf_dT = 0, nf_wnorm = 1;
     Nf2ffGeom nf_geom{};
     uint32_t* nf_pts = nullptr;
+    // NF2FF points are kept in grid order (sorted by linear cell index) so
+    // neighbouring threads of the sampling kernel read neighbouring cells;
+    // nf_perm[q] is the caller's index of internal point q.
+    std::vector<uint32_t> nf_perm;
     float *nf_el = nullptr, *nf_buf = nullptr, *nf_acc = nullptr;
     double* nf_omega = nullptr;
     float2* nf_tw = nullptr;
@@ -259,6 +267,50 @@ bool pair_table(const float* a, const float* b, size_t n, uint8_t* idx, float2* 
     for (uint32_t t = count; t < 256; ++t) tab[t] = make_float2(0.f, 0.f);
     return true;
 }
+
+// Distinct (a, b, c) triples of one component, accumulated over several
+// arrays (the UPML regions): byte index per cell into a 256-entry table of
+// (a, b, c, 0).  add() returns false once there are more than 256 triples.
+struct TripleTable {
+    static constexpr uint32_t H = 4096;  // open-addressing slots (power of two)
+    std::vector<uint64_t> k0 = std::vector<uint64_t>(H);
+    std::vector<uint32_t> k1 = std::vector<uint32_t>(H);
+    std::vector<int16_t> val = std::vector<int16_t>(H, -1);
+    uint32_t count = 0;
+    float4 tab[256] = {};
+
+    bool add(const float* a, const float* b, const float* c, size_t n, uint8_t* idx) {
+        uint64_t last0 = ~0ull;
+        uint32_t last1 = ~0u;
+        uint8_t last_v = 0;
+        for (size_t i = 0; i < n; ++i) {
+            uint32_t ua, ub, uc;
+            std::memcpy(&ua, a + i, 4);
+            std::memcpy(&ub, b + i, 4);
+            std::memcpy(&uc, c + i, 4);
+            const uint64_t q0 = ((uint64_t)ua << 32) | ub;
+            if (q0 == last0 && uc == last1) {
+                idx[i] = last_v;
+                continue;
+            }
+            uint32_t h = (uint32_t)(((q0 ^ ((uint64_t)uc * 0xC2B2AE3D27D4EB4Full)) * 0x9E3779B97F4A7C15ull) >> 52) & (H - 1);
+            while (val[h] >= 0 && (k0[h] != q0 || k1[h] != uc)) h = (h + 1) & (H - 1);
+            if (val[h] < 0) {
+                if (count == 256) return false;
+                k0[h] = q0;
+                k1[h] = uc;
+                val[h] = (int16_t)count;
+                tab[count] = make_float4(a[i], b[i], c[i], 0.f);
+                ++count;
+            }
+            last0 = q0;
+            last1 = uc;
+            last_v = (uint8_t)val[h];
+            idx[i] = last_v;
+        }
+        return true;
+    }
+};
 
 std::vector<uint32_t> tr_all(const efs_ctx* c, const uint32_t* gidx, size_t n) {
     std::vector<uint32_t> out(n);
@@ -406,6 +458,7 @@ void free_all(efs_ctx* c) {
     c->comp = false;
     dfree(c->exc_signal); dfree(c->exc_amp); dfree(c->exc_delay); dfree(c->exc_gidx);
     dfree(c->pml_coef); dfree(c->pml_vflux); dfree(c->pml_cflux);
+    dfree(c->pml_idx); dfree(c->pml_tab_e); dfree(c->pml_tab_h);
     dfree(c->mur_coeff); dfree(c->mur_saved); dfree(c->mur_normal); dfree(c->mur_shifted);
     dfree(c->mur_normal_src); dfree(c->mur_shifted_src);
     dfree(c->ade_gidx); dfree(c->ade_src_v); dfree(c->ade_src_i); dfree(c->ade_lor);
@@ -449,7 +502,9 @@ bool box_overlap(const PmlRegion& a, const PmlRegion& b) {
 // workgroups.  Cost model in bytes per cell: a core cell moves V r+w, H and
 // its coefficients (39 B with byte-indexed coefficients, 60 B without); a
 // UPML cell in the same launch adds flux r+w and three UPML coefficients
-// (60 B); in the interior launch of the split form it costs only the loop.
+// (60 B, or 27 B byte-indexed); in the interior launch of the split form it
+// costs only the loop.  Blocks that contain UPML cells are flagged (they
+// stage the UPML coefficient tables).
 // The launch gets ~2048 workgroups (8 per CU on 256 CUs) but chunks of 8
 // planes on average; heavy tiles go down to 2-plane chunks.
 int plan_sweeps(efs_ctx* c) {
@@ -457,7 +512,7 @@ int plan_sweeps(efs_ctx* c) {
     const uint64_t cols = (uint64_t)ny * nz;
     const bool upml_in_grid = !(c->vec4 && c->pml_split && !c->pml_fused);
     const double core_b = c->comp ? 39.0 : 60.0;
-    const double upml_b = upml_in_grid ? core_b + 60.0 : 2.0;
+    const double upml_b = upml_in_grid ? core_b + (c->pml_comp ? 27.0 : 60.0) : 2.0;
     const uint64_t want = env_u64("EFS_SWEEP_BLOCKS", 2048);
     for (int v = 0; v < 2; ++v) {
         const uint64_t tcells = (uint64_t)kSweepThreads * (v == 0 ? 4u : 1u);
@@ -477,6 +532,7 @@ int plan_sweeps(efs_ctx* c) {
             }
         }
         std::vector<double> cost((size_t)T * nx);  // [tile][x]
+        std::vector<uint8_t> up((size_t)T * nx, 0);  // tile and plane hold UPML cells
         double total = 0.0;
         for (uint32_t t = 0; t < T; ++t) {
             const double cells = (double)(std::min(cols, (uint64_t)(t + 1) * tcells) - (uint64_t)t * tcells);
@@ -485,8 +541,10 @@ int plan_sweeps(efs_ctx* c) {
         for (uint32_t k = 0; k < c->pml.n; ++k)
             for (uint32_t t = 0; t < T; ++t)
                 if (cnt[k][t] > 0)
-                    for (uint32_t x = c->pml.r[k].x0; x < c->pml.r[k].x1; ++x)
+                    for (uint32_t x = c->pml.r[k].x0; x < c->pml.r[k].x1; ++x) {
                         cost[(size_t)t * nx + x] += (upml_b - core_b) * cnt[k][t];
+                        up[(size_t)t * nx + x] = 1;
+                    }
         for (double q : cost) total += q;
         const double nblocks = (double)std::max<uint64_t>(1, std::min<uint64_t>(want, (uint64_t)T * std::max(1u, nx / 8)));
         const double target = total / nblocks;
@@ -508,7 +566,10 @@ int plan_sweeps(efs_ctx* c) {
                     b = (x < nx && goal - pre[x] > pre[x + 1] - goal) ? x + 1 : x;
                 }
                 if (b > b0) {
-                    tab.push_back(SweepBlock{t, b0, b, 0});
+                    uint32_t flags = 0;
+                    for (uint32_t xx = b0; xx < b; ++xx)
+                        if (up[(size_t)t * nx + xx]) flags = kBlockUpml;
+                    tab.push_back(SweepBlock{t, b0, b, flags});
                     b0 = b;
                 }
             }
@@ -551,9 +612,10 @@ int launch_step(efs_ctx* c) {
     {
         ProfScope s(c, EFS_PROF_SWEEP_E);
         if (c->vec4 && (!has_pml || c->pml_split)) {
-            const Coef cf{c->vv, c->vi, c->cidx_e, c->ctab_e};
+            const Coef cf{c->vv, c->vi, c->cidx_e, c->ctab_e, c->pml_idx, c->pml_tab_e};
             auto kern = has_pml ? (c->comp ? sweep_e4mc : sweep_e4m) : (c->comp ? sweep_e4c : sweep_e4);
-            if (has_pml && c->pml_fused) kern = c->comp ? sweep_e4fc : sweep_e4f;
+            if (has_pml && c->pml_fused)
+                kern = c->pml_comp ? (c->comp ? sweep_e4fcp : sweep_e4fp) : (c->comp ? sweep_e4fc : sweep_e4f);
             hipLaunchKernelGGL(kern, dim3(c->geom.blocks), dim3(kSweepThreads), 0, c->stream, c->volt, c->curr, cf,
                                c->geom, c->pml, c->pml_coef, c->pml_vflux);
             if (has_pml && !c->pml_fused)
@@ -601,9 +663,10 @@ int launch_step(efs_ctx* c) {
     {
         ProfScope s(c, EFS_PROF_SWEEP_H);
         if (c->vec4 && (!has_pml || c->pml_split)) {
-            const Coef cf{c->ii, c->iv, c->cidx_h, c->ctab_h};
+            const Coef cf{c->ii, c->iv, c->cidx_h, c->ctab_h, c->pml_idx, c->pml_tab_h};
             auto kern = has_pml ? (c->comp ? sweep_h4mc : sweep_h4m) : (c->comp ? sweep_h4c : sweep_h4);
-            if (has_pml && c->pml_fused) kern = c->comp ? sweep_h4fc : sweep_h4f;
+            if (has_pml && c->pml_fused)
+                kern = c->pml_comp ? (c->comp ? sweep_h4fcp : sweep_h4fp) : (c->comp ? sweep_h4fc : sweep_h4f);
             hipLaunchKernelGGL(kern, dim3(c->geom.blocks), dim3(kSweepThreads), 0, c->stream, c->curr, c->volt, cf,
                                c->geom, c->pml, c->pml_coef, c->pml_cflux);
             if (has_pml && !c->pml_fused)
@@ -851,6 +914,43 @@ int efs_cfg_pml(efs_ctx* c, uint32_t nreg, const uint32_t* start, const uint32_t
         TRY(dzero(c, c->pml_cflux, total));
     } else {
         dfree(c->pml_coef); dfree(c->pml_vflux); dfree(c->pml_cflux);
+    }
+    // UPML coefficient compression: one table per field and component over
+    // all regions, byte indices per region cell (region-local NIJK, like the
+    // FP32 arrays); used by the fused sweeps when every table fits.
+    dfree(c->pml_idx); dfree(c->pml_tab_e); dfree(c->pml_tab_h);
+    c->pml_comp = false;
+    if (nreg && aligned && env_u64("EFS_PML_COMPRESS", 1) != 0) {
+        uint64_t ibytes = 0;
+        for (uint32_t k = 0; k < nreg; ++k) {
+            t.r[k].pidx = ibytes;
+            ibytes += 6 * (uint64_t)t.r[k].cs;  // multiple of 4: region nz % 4 == 0
+        }
+        std::vector<uint8_t> idx(ibytes);
+        TripleTable tabs[2][3];
+        const float* src[2][3] = {{vv, vvfo, vvfn}, {ii, iifo, iifn}};
+        bool ok = true;
+        uint64_t off_in = 0;
+        for (uint32_t k = 0; k < nreg && ok; ++k) {
+            const uint64_t cs = t.r[k].cs;
+            for (int h = 0; h < 2 && ok; ++h)
+                for (int n = 0; n < 3 && ok; ++n)
+                    ok = tabs[h][n].add(src[h][0] + off_in + n * cs, src[h][1] + off_in + n * cs,
+                                        src[h][2] + off_in + n * cs, cs, idx.data() + t.r[k].pidx + (3 * h + n) * cs);
+            off_in += 3 * cs;
+        }
+        if (ok) {
+            std::vector<float4> te(3 * 256), th(3 * 256);
+            for (int n = 0; n < 3; ++n)
+                for (int q = 0; q < 256; ++q) {
+                    te[256 * n + q] = tabs[0][n].tab[q];
+                    th[256 * n + q] = tabs[1][n].tab[q];
+                }
+            TRY(dupload(c, c->pml_idx, idx.data(), idx.size()));
+            TRY(dupload(c, c->pml_tab_e, te.data(), te.size()));
+            TRY(dupload(c, c->pml_tab_h, th.data(), th.size()));
+            c->pml_comp = true;
+        }
     }
     c->pml = t;
     c->pml_split = aligned && env_u64("EFS_PML_SPLIT", 1) != 0;
@@ -1104,7 +1204,16 @@ int efs_cfg_nf2ff(efs_ctx* c, const uint32_t* points, uint32_t P, const double* 
         g.el_off[a] = (uint32_t)el_total;
         el_total += el_len[a];
     }
-    TRY(dupload(c, c->nf_pts, points, 3 * (size_t)P));
+    std::vector<uint32_t> perm(P), sorted(3 * (size_t)P);
+    for (uint32_t p = 0; p < P; ++p) perm[p] = p;
+    auto lin = [&](uint32_t p) {
+        return ((uint64_t)points[3 * p] * c->ny + points[3 * p + 1]) * c->nz + points[3 * p + 2];
+    };
+    std::stable_sort(perm.begin(), perm.end(), [&](uint32_t a, uint32_t b) { return lin(a) < lin(b); });
+    for (uint32_t q = 0; q < P; ++q)
+        for (int a = 0; a < 3; ++a) sorted[3 * (size_t)q + a] = points[3 * (size_t)perm[q] + a];
+    TRY(dupload(c, c->nf_pts, sorted.data(), sorted.size()));
+    c->nf_perm = std::move(perm);
     TRY(dupload(c, c->nf_el, edge_lens, el_total));
     TRY(dupload(c, c->nf_omega, omega, F));
     TRY(dalloc(c, c->nf_buf, (size_t)K * 6 * P));
@@ -1136,17 +1245,19 @@ int efs_read_nf2ff(efs_ctx* c, float* accE, float* accH) {
         const float* im = re + rows;
         float* oE = accE ? accE + (size_t)f * rows : nullptr;
         float* oH = accH ? accH + (size_t)f * rows : nullptr;
-        for (size_t p = 0; p < P; ++p)
+        for (size_t q = 0; q < P; ++q) {
+            const size_t p = c->nf_perm[q];
             for (int k = 0; k < 3; ++k) {
                 if (oE) {
-                    oE[6 * p + 2 * k] = re[k * P + p];
-                    oE[6 * p + 2 * k + 1] = im[k * P + p];
+                    oE[6 * p + 2 * k] = re[k * P + q];
+                    oE[6 * p + 2 * k + 1] = im[k * P + q];
                 }
                 if (oH) {
-                    oH[6 * p + 2 * k] = re[(3 + k) * P + p];
-                    oH[6 * p + 2 * k + 1] = im[(3 + k) * P + p];
+                    oH[6 * p + 2 * k] = re[(3 + k) * P + q];
+                    oH[6 * p + 2 * k + 1] = im[(3 + k) * P + q];
                 }
             }
+        }
     }
     return EFS_OK;
 }
@@ -1163,8 +1274,11 @@ int efs_farfield(efs_ctx* c, uint32_t f, const float* meta, const float* theta, 
     float *d_meta = nullptr, *d_th = nullptr, *d_ph = nullptr, *d_out = nullptr;
     int rc = EFS_OK;
     const size_t na = (size_t)nT * nP;
+    std::vector<float> meta_q(8 * (size_t)c->nf_P);  // caller order -> internal point order
+    for (size_t q = 0; q < c->nf_P; ++q)
+        std::memcpy(&meta_q[8 * q], meta + 8 * (size_t)c->nf_perm[q], 8 * sizeof(float));
     do {
-        if ((rc = dupload(c, d_meta, meta, 8 * (size_t)c->nf_P))) break;
+        if ((rc = dupload(c, d_meta, meta_q.data(), meta_q.size()))) break;
         if ((rc = dupload(c, d_th, theta, nT))) break;
         if ((rc = dupload(c, d_ph, phi, nP))) break;
         if ((rc = dalloc(c, d_out, 5 * na))) break;
@@ -1205,6 +1319,15 @@ int efs_energy(efs_ctx* c, double* out) {
     CHECK_LAUNCH(c);
     HIP_TRY(c, hipStreamSynchronize(c->stream));
     HIP_TRY(c, hipMemcpy(out, c->en_partial + kEnergyMaxBlocks, sizeof(double), hipMemcpyDeviceToHost));
+    return EFS_OK;
+}
+
+int efs_path_flags(const efs_ctx* c, uint32_t* flags) {
+    if (!c || !flags) return EFS_ERR_ARG;
+    const bool upml4 = c->vec4 && c->pml.n && c->pml_split;
+    *flags = (c->vec4 ? EFS_PATH_VEC4 : 0u) | (c->comp ? EFS_PATH_COEF_IDX : 0u) | (upml4 ? EFS_PATH_UPML_4 : 0u) |
+             (upml4 && c->pml_fused ? EFS_PATH_FUSED : 0u) |
+             (upml4 && c->pml_fused && c->pml_comp ? EFS_PATH_UPML_IDX : 0u);
     return EFS_OK;
 }
 

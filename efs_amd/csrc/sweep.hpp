@@ -70,6 +70,8 @@ struct PmlRegion {
     uint32_t pad;
     uint64_t coef;                    // element offset of the region's 6 coefficient arrays
     uint64_t flux;                    // element offset of the region's flux arrays (3*cs each)
+    uint64_t pidx;                    // byte offset of the region's UPML coefficient indices
+                                      // (E components, then H components, cs bytes each)
 };
 
 struct PmlTable {
@@ -81,8 +83,10 @@ struct PmlTable {
 // One workgroup of a grid launch: a column tile (256 lanes over
 // contiguous (y, z) columns) and the x planes [x0, x1) it marches over.
 struct SweepBlock {
-    uint32_t tile, x0, x1, pad;
+    uint32_t tile, x0, x1;
+    uint32_t flags;  // kBlockUpml: the block contains UPML cells
 };
+constexpr uint32_t kBlockUpml = 1u;
 
 // Grid launches read their work from a host-built block table: each column
 // tile's x range is cut into chunks of equal bytes moved (UPML cells move
@@ -150,12 +154,27 @@ struct Coef {
     const float* b;
     const uint8_t* idx;  // byte indices (3 components, stride S)
     const float2* tab;   // [3][256] pairs (global; staged into LDS)
+    // UPML coefficients in the same form: per region byte indices (PmlRegion
+    // pidx) into a [3][256] table of (vv, vvfo, vvfn, 0) -- or ii, iifo,
+    // iifn for the H sweep -- when each component has at most 256 triples.
+    const uint8_t* pidx;
+    const float4* ptab;
 };
 
 template <bool COMP>
 __device__ __forceinline__ void coef_stage(const Coef& cf, float2* s_tab) {
     if (COMP) {
         for (int t = threadIdx.x; t < 3 * kTab; t += kSweepThreads) s_tab[t] = cf.tab[t];
+        __syncthreads();
+    }
+}
+
+// UPML table staging, only in workgroups whose block has UPML cells (the
+// flag is uniform over the workgroup, so the barrier is too).
+template <bool PCOMP>
+__device__ __forceinline__ void upml_stage(const Coef& cf, float4* s_ptab, uint32_t flags) {
+    if (PCOMP && (flags & kBlockUpml)) {
+        for (int t = threadIdx.x; t < 3 * kTab; t += kSweepThreads) s_ptab[t] = cf.ptab[t];
         __syncthreads();
     }
 }
@@ -329,14 +348,15 @@ struct PmlCursor {
     uint32_t until;   // the state holds for planes < until
     float* flux;      // flux of component 0 at the lane's first cell
     const float* coef;  // first of the three coefficient arrays (component 0)
+    const uint8_t* pq;  // coefficient indices of component 0 (compressed UPML)
     uint32_t cs;      // component stride of the region arrays
-    uint32_t step;    // floats per region plane (ny*nz of the region)
+    uint32_t step;    // cells per region plane (ny*nz of the region)
 };
 
 template <bool HF>
 __device__ __forceinline__ void cursor_seek(PmlCursor& u, const PmlTable& pml, uint32_t m, uint32_t x, uint32_t y,
                                             uint32_t z0, const float* __restrict__ pcoef,
-                                            float* __restrict__ pflux) {
+                                            float* __restrict__ pflux, const uint8_t* __restrict__ pidx) {
     u.k = -1;
     u.until = 0xffffffffu;
     while (m) {
@@ -351,6 +371,7 @@ __device__ __forceinline__ void cursor_seek(PmlCursor& u, const PmlTable& pml, u
             u.step = R.ny * R.nz;
             u.flux = pflux + R.flux + p0;
             u.coef = pcoef + R.coef + (HF ? 9 * (size_t)R.cs : 0) + p0;
+            u.pq = pidx + R.pidx + (HF ? 3 * (size_t)R.cs : 0) + p0;
             return;
         }
         if (R.x0 > x) u.until = min(u.until, R.x0);
@@ -361,20 +382,57 @@ __device__ __forceinline__ void cursor_seek(PmlCursor& u, const PmlTable& pml, u
 template <bool HF>
 __device__ __forceinline__ void cursor_at(PmlCursor& u, const PmlTable& pml, uint32_t m, uint32_t x, uint32_t y,
                                           uint32_t z0, const float* __restrict__ pcoef,
-                                          float* __restrict__ pflux) {
+                                          float* __restrict__ pflux, const uint8_t* __restrict__ pidx) {
     if (x >= u.until) {
-        cursor_seek<HF>(u, pml, m, x, y, z0, pcoef, pflux);
+        cursor_seek<HF>(u, pml, m, x, y, z0, pcoef, pflux, pidx);
     } else if (u.k >= 0) {
         u.flux += u.step;
         u.coef += u.step;
+        u.pq += u.step;
+    }
+}
+
+// UPML operands of one lane and plane: flux always; coefficients either
+// from the FP32 region arrays or, compressed, as byte indices decoded later.
+template <bool PCOMP>
+__device__ __forceinline__ void upml_fetch(const PmlCursor& u, v4 F[3], v4 P0[3], v4 P1[3], v4 P2[3],
+                                           uint32_t pq[3]) {
+    const size_t T = 3 * (size_t)u.cs;
+#pragma unroll
+    for (int n = 0; n < 3; ++n) {
+        F[n] = ld4(u.flux + n * u.cs);
+        if (PCOMP) {
+            pq[n] = __builtin_nontemporal_load(reinterpret_cast<const uint32_t*>(u.pq + n * u.cs));
+        } else {
+            P0[n] = ld4nt(u.coef + n * u.cs);
+            P1[n] = ld4nt(u.coef + T + n * u.cs);
+            P2[n] = ld4nt(u.coef + 2 * T + n * u.cs);
+        }
+    }
+}
+
+template <bool PCOMP>
+__device__ __forceinline__ void upml_decode(const float4* s_ptab, const uint32_t pq[3], v4 P0[3], v4 P1[3],
+                                            v4 P2[3]) {
+    if (PCOMP) {
+#pragma unroll
+        for (int n = 0; n < 3; ++n) {
+            const float4* t = s_ptab + n * kTab;
+            const float4 c0 = t[pq[n] & 255u], c1 = t[(pq[n] >> 8) & 255u], c2 = t[(pq[n] >> 16) & 255u],
+                         c3 = t[pq[n] >> 24];
+            P0[n] = (v4){c0.x, c1.x, c2.x, c3.x};
+            P1[n] = (v4){c0.y, c1.y, c2.y, c3.y};
+            P2[n] = (v4){c0.z, c1.z, c2.z, c3.z};
+        }
     }
 }
 
 // E update of 4 cells over x in [x0, x1); MODE says how UPML cells are
 // treated (column mask m gives the regions the lane's column crosses).
-template <int MODE, bool COMP>
+template <int MODE, bool COMP, bool PCOMP>
 __device__ __forceinline__ void e4_march(float* __restrict__ volt, const float* __restrict__ curr, const Coef& cf,
-                                         const float2* s_tab, const SweepGeom& g, const PmlTable& pml,
+                                         const float2* s_tab, const float4* s_ptab, const SweepGeom& g,
+                                         const PmlTable& pml,
                                          const float* __restrict__ pcoef, float* __restrict__ pflux, Lane4 w,
                                          uint32_t m) {
     const size_t S = g.S;
@@ -388,9 +446,9 @@ __device__ __forceinline__ void e4_march(float* __restrict__ volt, const float* 
     uint32_t i = w.x0 * plane + w.y * g.nz + w.z0;
     bool carry = false;  // hy_m/hz_m hold H at x-1
     v4 hy_m = 0.f, hz_m = 0.f;
-    PmlCursor u{-1, m ? 0u : 0xffffffffu, nullptr, nullptr, 0u, 0u};
+    PmlCursor u{-1, m ? 0u : 0xffffffffu, nullptr, nullptr, nullptr, 0u, 0u};
     for (uint32_t x = w.x0; x < w.x1; ++x, i += plane) {
-        cursor_at<false>(u, pml, m, x, w.y, w.z0, pcoef, pflux);
+        cursor_at<false>(u, pml, m, x, w.y, w.z0, pcoef, pflux, cf.pidx);
         const bool upml = MODE == kMarchRegion || (MODE == kMarchFused && u.k >= 0);
         if (MODE == kMarchInterior && u.k >= 0) {
             carry = false;
@@ -410,16 +468,8 @@ __device__ __forceinline__ void e4_march(float* __restrict__ volt, const float* 
         for (int n = 0; n < 3; ++n) V[n] = ld4(E[n] + i);
         coef_fetch<COMP>(cf, S, i, q, a, b);
         v4 F[3], P0[3], P1[3], P2[3];
-        if (upml) {
-            const size_t T = 3 * (size_t)u.cs;
-#pragma unroll
-            for (int n = 0; n < 3; ++n) {
-                F[n] = ld4(u.flux + n * u.cs);
-                P0[n] = ld4nt(u.coef + n * u.cs);
-                P1[n] = ld4nt(u.coef + T + n * u.cs);
-                P2[n] = ld4nt(u.coef + 2 * T + n * u.cs);
-            }
-        }
+        uint32_t pq[3];
+        if (upml) upml_fetch<PCOMP>(u, F, P0, P1, P2, pq);
         coef_decode<COMP>(s_tab, q, a, b);
         const v4 hy_zm = (v4){hy_z, hy.x, hy.y, hy.z};
         const v4 hx_zm = (v4){hx_z, hx.x, hx.y, hx.z};
@@ -431,6 +481,7 @@ __device__ __forceinline__ void e4_march(float* __restrict__ volt, const float* 
 #pragma unroll
             for (int n = 0; n < 3; ++n) st4(E[n] + i, fma4(a[n], V[n], b[n] * cur[n]));
         } else {
+            upml_decode<PCOMP>(s_ptab, pq, P0, P1, P2);
 #pragma unroll
             for (int n = 0; n < 3; ++n) {
                 const v4 Fn = fma4(a[n], F[n], b[n] * cur[n]);
@@ -443,9 +494,10 @@ __device__ __forceinline__ void e4_march(float* __restrict__ volt, const float* 
 
 // H update of 4 cells over x in [x0, x1) (see e4_march).  Cells outside the
 // core bounds (x = Nx-1, y = Ny-1, z = Nz-1) keep I unless in UPML.
-template <int MODE, bool COMP>
+template <int MODE, bool COMP, bool PCOMP>
 __device__ __forceinline__ void h4_march(float* __restrict__ curr, const float* __restrict__ volt, const Coef& cf,
-                                         const float2* s_tab, const SweepGeom& g, const PmlTable& pml,
+                                         const float2* s_tab, const float4* s_ptab, const SweepGeom& g,
+                                         const PmlTable& pml,
                                          const float* __restrict__ pcoef, float* __restrict__ pflux, Lane4 w,
                                          uint32_t m) {
     const size_t S = g.S;
@@ -463,10 +515,10 @@ __device__ __forceinline__ void h4_march(float* __restrict__ curr, const float* 
     uint32_t i = w.x0 * plane + w.y * g.nz + w.z0;
     bool carry = false;  // ex/ey/ez hold E at x
     v4 ex = 0.f, ey = 0.f, ez = 0.f;
-    PmlCursor u{-1, m ? 0u : 0xffffffffu, nullptr, nullptr, 0u, 0u};
+    PmlCursor u{-1, m ? 0u : 0xffffffffu, nullptr, nullptr, nullptr, 0u, 0u};
     for (uint32_t x = w.x0; x < w.x1; ++x, i += plane) {
         const bool core = row_core && (x + 1 < g.nx);
-        cursor_at<true>(u, pml, m, x, w.y, w.z0, pcoef, pflux);
+        cursor_at<true>(u, pml, m, x, w.y, w.z0, pcoef, pflux, cf.pidx);
         const bool upml = MODE == kMarchRegion || (MODE == kMarchFused && u.k >= 0);
         if (MODE != kMarchRegion && !upml && (!core || u.k >= 0)) {
             carry = false;
@@ -488,17 +540,9 @@ __device__ __forceinline__ void h4_march(float* __restrict__ curr, const float* 
 #pragma unroll
         for (int n = 0; n < 3; ++n) I[n] = ld4(Ic[n] + i);
         coef_fetch<COMP>(cf, S, i, q, a, b);
-        v4 F[3], P0[3], P1[3], P2[3];
-        if (upml) {  // ii, iifo, iifn
-            const size_t T = 3 * (size_t)u.cs;
-#pragma unroll
-            for (int n = 0; n < 3; ++n) {
-                F[n] = ld4(u.flux + n * u.cs);
-                P0[n] = ld4nt(u.coef + n * u.cs);
-                P1[n] = ld4nt(u.coef + T + n * u.cs);
-                P2[n] = ld4nt(u.coef + 2 * T + n * u.cs);
-            }
-        }
+        v4 F[3], P0[3], P1[3], P2[3];  // P: ii, iifo, iifn
+        uint32_t pq[3];
+        if (upml) upml_fetch<PCOMP>(u, F, P0, P1, P2, pq);
         coef_decode<COMP>(s_tab, q, a, b);
         const v4 ey_zp = (v4){ey.y, ey.z, ey.w, ey_z};
         const v4 ex_zp = (v4){ex.y, ex.z, ex.w, ex_z};
@@ -511,6 +555,7 @@ __device__ __forceinline__ void h4_march(float* __restrict__ curr, const float* 
 #pragma unroll
             for (int n = 0; n < 3; ++n) st4(Ic[n] + i, sel4(cb, fma4(a[n], I[n], b[n] * cur[n]), I[n]));
         } else {
+            upml_decode<PCOMP>(s_ptab, pq, P0, P1, P2);
 #pragma unroll
             for (int n = 0; n < 3; ++n) {
                 const v4 Fn = sel4(cb, fma4(a[n], F[n], b[n] * cur[n]), F[n]);
@@ -523,8 +568,12 @@ __device__ __forceinline__ void h4_march(float* __restrict__ curr, const float* 
 
 // Interior launch: grid tiling of the (y, z) plane, 1024 cells per workgroup.
 template <bool HAS_PML>
-__device__ __forceinline__ bool interior_lane(const SweepGeom& g, const PmlTable& pml, Lane4& w, uint32_t& m) {
+// flags: the block's flags (uniform over the workgroup, set on every path).
+__device__ __forceinline__ bool interior_lane(const SweepGeom& g, const PmlTable& pml, Lane4& w, uint32_t& m,
+                                              uint32_t& flags) {
     const SweepBlock d = g.tab[xcd_remap(blockIdx.x, g.blocks)];
+    flags = d.flags;
+    m = 0u;
     const uint32_t c0 = (d.tile * kSweepThreads + threadIdx.x) * 4u;
     if (c0 >= g.cols) return false;
     w.y = c0 / g.nz;
@@ -570,9 +619,9 @@ __device__ __forceinline__ bool pml_lane(const PmlTable& pml, Lane4& w) {
         __shared__ float2 s_tab[COMP ? 3 * kTab : 1];                                                              \
         coef_stage<COMP>(cf, s_tab);                                                                               \
         Lane4 w;                                                                                                   \
-        uint32_t m;                                                                                                \
-        if (!interior_lane<HAS_PML>(g, pml, w, m)) return;                                                         \
-        MARCH<kMarchInterior, COMP>(dst, src, cf, s_tab, g, pml, pcoef, pflux, w, m);                              \
+        uint32_t m, flags;                                                                                         \
+        if (!interior_lane<HAS_PML>(g, pml, w, m, flags)) return;                                                  \
+        MARCH<kMarchInterior, COMP, false>(dst, src, cf, s_tab, nullptr, g, pml, pcoef, pflux, w, m);              \
     }
 
 #define EFS_SWEEP4_UPML(NAME, MARCH, COMP)                                                                         \
@@ -582,20 +631,23 @@ __device__ __forceinline__ bool pml_lane(const PmlTable& pml, Lane4& w) {
         coef_stage<COMP>(cf, s_tab);                                                                               \
         Lane4 w;                                                                                                   \
         if (!pml_lane(pml, w)) return;                                                                             \
-        MARCH<kMarchRegion, COMP>(dst, src, cf, s_tab, g, pml, pcoef, pflux, w, 1u << w.region);                   \
+        MARCH<kMarchRegion, COMP, false>(dst, src, cf, s_tab, nullptr, g, pml, pcoef, pflux, w, 1u << w.region);   \
     }
 
 // Fused: the grid launch does the UPML cells too (3 waves/SIMD, the UPML
-// register budget).
-#define EFS_SWEEP4_FUSED(NAME, MARCH, COMP)                                                                        \
+// register budget); PCOMP: UPML coefficients as byte indices + LDS table.
+#define EFS_SWEEP4_FUSED(NAME, MARCH, COMP, PCOMP)                                                                 \
     __global__ __launch_bounds__(kSweepThreads) __attribute__((                                                    \
         amdgpu_waves_per_eu(EFS_SWEEP_WAVES_PML, EFS_SWEEP_WAVES_PML))) void NAME(EFS_SWEEP4_PARAMS) {            \
         __shared__ float2 s_tab[COMP ? 3 * kTab : 1];                                                              \
+        __shared__ float4 s_ptab[PCOMP ? 3 * kTab : 1];                                                            \
         coef_stage<COMP>(cf, s_tab);                                                                               \
         Lane4 w;                                                                                                   \
-        uint32_t m;                                                                                                \
-        if (!interior_lane<true>(g, pml, w, m)) return;                                                            \
-        MARCH<kMarchFused, COMP>(dst, src, cf, s_tab, g, pml, pcoef, pflux, w, m);                                 \
+        uint32_t m, flags;                                                                                         \
+        const bool live = interior_lane<true>(g, pml, w, m, flags);                                                \
+        upml_stage<PCOMP>(cf, s_ptab, flags);                                                                      \
+        if (!live) return;                                                                                         \
+        MARCH<kMarchFused, COMP, PCOMP>(dst, src, cf, s_tab, s_ptab, g, pml, pcoef, pflux, w, m);                  \
     }
 
 EFS_SWEEP4_INTERIOR(sweep_e4, e4_march, false, false)
@@ -604,16 +656,20 @@ EFS_SWEEP4_INTERIOR(sweep_e4m, e4_march, true, false)
 EFS_SWEEP4_INTERIOR(sweep_e4mc, e4_march, true, true)
 EFS_SWEEP4_UPML(sweep_e4u, e4_march, false)
 EFS_SWEEP4_UPML(sweep_e4uc, e4_march, true)
-EFS_SWEEP4_FUSED(sweep_e4f, e4_march, false)
-EFS_SWEEP4_FUSED(sweep_e4fc, e4_march, true)
+EFS_SWEEP4_FUSED(sweep_e4f, e4_march, false, false)
+EFS_SWEEP4_FUSED(sweep_e4fc, e4_march, true, false)
+EFS_SWEEP4_FUSED(sweep_e4fp, e4_march, false, true)
+EFS_SWEEP4_FUSED(sweep_e4fcp, e4_march, true, true)
 EFS_SWEEP4_INTERIOR(sweep_h4, h4_march, false, false)
 EFS_SWEEP4_INTERIOR(sweep_h4c, h4_march, false, true)
 EFS_SWEEP4_INTERIOR(sweep_h4m, h4_march, true, false)
 EFS_SWEEP4_INTERIOR(sweep_h4mc, h4_march, true, true)
 EFS_SWEEP4_UPML(sweep_h4u, h4_march, false)
 EFS_SWEEP4_UPML(sweep_h4uc, h4_march, true)
-EFS_SWEEP4_FUSED(sweep_h4f, h4_march, false)
-EFS_SWEEP4_FUSED(sweep_h4fc, h4_march, true)
+EFS_SWEEP4_FUSED(sweep_h4f, h4_march, false, false)
+EFS_SWEEP4_FUSED(sweep_h4fc, h4_march, true, false)
+EFS_SWEEP4_FUSED(sweep_h4fp, h4_march, false, true)
+EFS_SWEEP4_FUSED(sweep_h4fcp, h4_march, true, true)
 #undef EFS_SWEEP4_INTERIOR
 #undef EFS_SWEEP4_UPML
 #undef EFS_SWEEP4_FUSED

@@ -65,6 +65,7 @@ SIGNATURES = {
     "efs_energy": (C.c_int, [C.c_void_p, C.POINTER(C.c_double)]),
     "efs_profile_enable": (C.c_int, [C.c_void_p, C.c_int]),
     "efs_profile_read": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_uint64)]),
+    "efs_path_flags": (C.c_int, [C.c_void_p, C.POINTER(C.c_uint32)]),
 }
 
 
@@ -393,6 +394,13 @@ class HIPEngine:
         ms, n = C.c_double(0), C.c_uint64(0)
         self._check(self.lib.efs_profile_read(self.ctx, int(which), C.byref(ms), C.byref(n)))
         return ms.value, n.value
+
+    def pathFlags(self):
+        """Sweep forms in use: set of 'vec4', 'coef_idx', 'upml4', 'fused', 'upml_idx'."""
+        f = C.c_uint32(0)
+        self._check(self.lib.efs_path_flags(self.ctx, C.byref(f)))
+        names = ("vec4", "coef_idx", "upml4", "fused", "upml_idx")
+        return {n for b, n in enumerate(names) if f.value & (1 << b)}
 
 
 WebGPUEngine = HIPEngine  # drop-in name for code written against the reference class

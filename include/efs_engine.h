@@ -164,6 +164,17 @@ int efs_farfield(efs_ctx *ctx, uint32_t freq_idx, const float *point_meta, const
  * previous call's); the lagging form is built on top by the host wrappers. */
 int efs_energy(efs_ctx *ctx, double *out);
 
+/* Which sweep forms the context uses (diagnostic; no counterpart in the
+ * reference): bit set = form in use. */
+enum {
+    EFS_PATH_VEC4 = 1,      /* 4 cells per lane (else 1) */
+    EFS_PATH_COEF_IDX = 2,  /* core coefficients as byte indices + tables */
+    EFS_PATH_UPML_4 = 4,    /* UPML handled by the 4-wide sweeps */
+    EFS_PATH_FUSED = 8,     /* one launch per sweep for core and UPML cells */
+    EFS_PATH_UPML_IDX = 16  /* UPML coefficients as byte indices + tables */
+};
+int efs_path_flags(const efs_ctx *ctx, uint32_t *flags);
+
 /* Kernel timing with HIP events on the engine's stream. */
 enum {
     EFS_PROF_SWEEP_E = 0, /* fused E sweep (core + UPML) */

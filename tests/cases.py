@@ -44,13 +44,20 @@ def shell_regions(dims, w):
     return regs
 
 
-def random_pml(rng, regions):
+def random_pml(rng, regions, quantize=0):
+    """Random UPML coefficients; with quantize=k each region array takes
+    only k distinct values (the case the engine's UPML tables compress)."""
     out = []
     for start, num in regions:
         t = 3 * int(np.prod(num))
+
+        def arr(lo, hi):
+            if not quantize:
+                return f32(rng, t, lo, hi)
+            return f32(rng, quantize, lo, hi)[rng.integers(0, quantize, t)]
         out.append({"startPos": list(start), "numLines": list(num),
-                    "vv": f32(rng, t, 0.6, 1.0), "vvfo": f32(rng, t, 0.0, 0.3), "vvfn": f32(rng, t, 0.8, 1.4),
-                    "ii": f32(rng, t, 0.6, 1.0), "iifo": f32(rng, t, 0.0, 0.3), "iifn": f32(rng, t, 0.8, 1.4)})
+                    "vv": arr(0.6, 1.0), "vvfo": arr(0.0, 0.3), "vvfn": arr(0.8, 1.4),
+                    "ii": arr(0.6, 1.0), "iifo": arr(0.0, 0.3), "iifn": arr(0.8, 1.4)})
     return out
 
 
@@ -59,7 +66,7 @@ def lin(dims, x, y, z):
 
 
 def random_case(seed, dims, pml_width=0, exc=True, mur=False, ade=False, tfsf=False, rlc=False, fields=True,
-                quantize=0):
+                quantize=0, pml_quantize=0):
     rng = np.random.default_rng(seed)
     nx, ny, nz = dims
     N = nx * ny * nz
@@ -68,7 +75,7 @@ def random_case(seed, dims, pml_width=0, exc=True, mur=False, ade=False, tfsf=Fa
         case["volt0"] = f32(rng, 3 * N, -0.5, 0.5)
         case["curr0"] = f32(rng, 3 * N, -0.5, 0.5)
     if pml_width:
-        case["pml"] = random_pml(rng, shell_regions(dims, pml_width))
+        case["pml"] = random_pml(rng, shell_regions(dims, pml_width), pml_quantize)
     if exc:
         k = 4
         case["excitation"] = {"signal": f32(rng, 17, -1, 1), "delay": rng.integers(0, 6, k).astype(np.uint32),

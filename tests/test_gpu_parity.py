@@ -155,6 +155,7 @@ def test_split_matches_unsplit(HIPEngine, monkeypatch):
     a = make_gpu(HIPEngine, case)
     monkeypatch.setenv("EFS_VEC", "1")
     b = make_gpu(HIPEngine, case)
+    assert "upml4" in a.pathFlags() and "vec4" not in b.pathFlags()
     a.iterate(25)
     b.iterate(25)
     fa, fb = a.getFields(), b.getFields()
@@ -176,10 +177,41 @@ def test_fused_matches_two_launch(HIPEngine, oracle_mod, monkeypatch, name, seed
     a = make_gpu(HIPEngine, case)
     monkeypatch.setenv("EFS_PML_FUSED", "0")
     b = make_gpu(HIPEngine, case)
+    assert "fused" in a.pathFlags() and {"upml4"} <= b.pathFlags() and "fused" not in b.pathFlags()
     o = make_oracle(oracle_mod, case)
     a.iterate(30)
     b.iterate(30)
     o.iterate(30)
+    fa, fb = a.getFields(), b.getFields()
+    np.testing.assert_array_equal(fa["volt"], fb["volt"])
+    np.testing.assert_array_equal(fa["curr"], fb["curr"])
+    assert_fields_close(fa, o.getFields(), what=name)
+
+
+PQCASES = [
+    # per region arrays of 3 levels each -> at most 27 triples per region and component
+    ("pq_pml8_all", 51, (24, 20, 32), dict(pml_width=8, pml_quantize=3, mur=True, tfsf=True, ade=True, rlc=True)),
+    ("pq_both_tables", 52, (20, 16, 24), dict(pml_width=4, pml_quantize=3, quantize=6, mur=True)),
+    ("pq_tall", 53, (40, 9, 12), dict(pml_width=4, pml_quantize=2)),
+    # 6 regions x 6^3 triples > 256 per component: tables do not fit, FP32 UPML arrays are used
+    ("pq_overflow", 54, (20, 16, 24), dict(pml_width=4, pml_quantize=6)),
+]
+
+
+@pytest.mark.parametrize("name,seed,dims,kw", PQCASES, ids=[c[0] for c in PQCASES])
+def test_compressed_upml_coefficients(HIPEngine, oracle_mod, monkeypatch, name, seed, dims, kw):
+    """UPML coefficients as byte indices into LDS tables vs the FP32 region
+    arrays: same values and arithmetic, so identical bits; and the oracle."""
+    case = random_case(seed, dims, **kw)
+    say(f"case {name} compressed UPML")
+    a = make_gpu(HIPEngine, case)
+    monkeypatch.setenv("EFS_PML_COMPRESS", "0")
+    b = make_gpu(HIPEngine, case)
+    assert ("upml_idx" in a.pathFlags()) == (name != "pq_overflow"), a.pathFlags()
+    assert "upml_idx" not in b.pathFlags() and "fused" in b.pathFlags()
+    o = make_oracle(oracle_mod, case)
+    for e in (a, b, o):
+        e.iterate(30)
     fa, fb = a.getFields(), b.getFields()
     np.testing.assert_array_equal(fa["volt"], fb["volt"])
     np.testing.assert_array_equal(fa["curr"], fb["curr"])
@@ -210,7 +242,7 @@ def test_iterate_split_equals_single_call(HIPEngine):
     np.testing.assert_array_equal(fa["curr"], fb["curr"])
 
 
-def nf2ff_setup(dims, inset=2):
+def nf2ff_setup(dims, inset=2, shuffle=False):
     nx, ny, nz = dims
     pts = []
     lo = [inset] * 3
@@ -224,16 +256,19 @@ def nf2ff_setup(dims, inset=2):
                     p[ax], p[a1], p[a2] = side, i, j
                     pts.append(p)
     rng = np.random.default_rng(99)
+    if shuffle:  # any point order: the engine sorts internally and returns caller order
+        pts = [pts[i] for i in rng.permutation(len(pts))]
     primal = [rng.uniform(0.8e-3, 1.2e-3, d).astype(np.float32) for d in dims]
     dual = [rng.uniform(0.8e-3, 1.2e-3, d).astype(np.float32) for d in dims]
     return np.array(pts, np.uint32).ravel(), primal, dual
 
 
-@pytest.mark.parametrize("kblock,steps,window", [(64, 100, 0), (7, 45, 1), (16, 16, 0)])
-def test_nf2ff_accumulation(HIPEngine, oracle_mod, kblock, steps, window):
+@pytest.mark.parametrize("kblock,steps,window,shuffle", [(64, 100, 0, False), (7, 45, 1, False), (16, 16, 0, False),
+                                                         (16, 40, 0, True)])
+def test_nf2ff_accumulation(HIPEngine, oracle_mod, kblock, steps, window, shuffle):
     dims = (14, 13, 15)
     case = random_case(21, dims, pml_width=2)
-    pts, primal, dual = nf2ff_setup(dims)
+    pts, primal, dual = nf2ff_setup(dims, shuffle=shuffle)
     dT = 1.9e-12
     omegas = 2 * np.pi * np.array([1.0e9, 2.5e9, 4.0e9, 6.3e9, 9.0e9, 11e9, 12e9, 20e9, 31e9])
     say("case nf2ff", kblock, steps, window)
@@ -256,7 +291,7 @@ def test_nf2ff_accumulation(HIPEngine, oracle_mod, kblock, steps, window):
 def test_farfield_matches_fp64_oracle(HIPEngine, oracle_mod):
     dims = (14, 13, 15)
     case = random_case(22, dims, pml_width=2)
-    pts, primal, dual = nf2ff_setup(dims)
+    pts, primal, dual = nf2ff_setup(dims, shuffle=True)
     dT = 1.9e-12
     freq = 3.1e9
     g = make_gpu(HIPEngine, case)
