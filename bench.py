@@ -33,6 +33,33 @@ def sweep_bytes(n_cells, n_pml):
     return 60.0 * n_cells + 60.0 * n_pml
 
 
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "fused_sweeps_256cube_stress_traffic.json")
+
+
+def sweep_kernel_symbol(flags, side):
+    """Name of the kernel that runs the E or H sweep for the path flags."""
+    if "vec4" not in flags:
+        return f"efs::sweep_{side}1<true>"
+    suffix = ""
+    if "fused" in flags:
+        suffix = "f" + ("c" if "coef_idx" in flags else "") + ("p" if "upml_idx" in flags else "")
+    elif "upml4" in flags:
+        suffix = "mc" if "coef_idx" in flags else "m"
+    else:
+        suffix = "c" if "coef_idx" in flags else ""
+    return f"efs::sweep_{side}4{suffix}"
+
+
+def measured_traffic(kernel, size):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3
+    FETCH_SIZE / WRITE_SIZE measurement of this workload (None if absent)."""
+    if size != 256 or not os.path.exists(TRAFFIC_FILE):
+        return None
+    with open(TRAFFIC_FILE) as f:
+        t = json.load(f)
+    return t.get(kernel, {}).get("hbm_bytes_per_launch")
+
+
 def cpu_baseline(w, seconds_target=15.0):
     """Time the CPU oracle (all host threads, f32 variant) on a bounded
     sample of the same workload: the same grid, coefficients, UPML and
@@ -136,6 +163,8 @@ def main():
     gbs_e = b / (avg_e * 1e-3) / 1e9
     gbs_h = b / (avg_h * 1e-3) / 1e9
     dom_name, dom_ms = ("sweep_h", avg_h) if avg_h >= avg_e else ("sweep_e", avg_e)
+    dom_sym = sweep_kernel_symbol(eng.pathFlags(), dom_name[-1])
+    traffic = measured_traffic(dom_sym, args.size)
     dom_gbs = b / (dom_ms * 1e-3) / 1e9
     acc = eng.readNF2FFAccumulation()
     finite = bool(np.all(np.isfinite(acc["accumE"])) and np.all(np.isfinite(energies)))
@@ -169,7 +198,8 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": round(dom_gbs / HBM_PEAK_GBS, 4),
-            "traffic": None,
+            "traffic": round(traffic) if traffic else None,
+            "traffic_source": (os.path.relpath(TRAFFIC_FILE, ROOT) + f" [{dom_sym}]") if traffic else None,
             "algorithmic_bytes_per_launch": b,
             "avg_launch_ms": round(dom_ms, 5),
             "kernels": {"sweep_e": {"avg_ms": round(avg_e, 5), "GB/s": round(gbs_e, 1)},
