@@ -132,6 +132,8 @@ struct efs_ctx {
 
     // NF2FF
     uint32_t nf_P = 0, nf_F = 0, nf_K = 0, nf_maxTS = 0, nf_window = 0;
+    uint32_t nf_rows = 0;  // buffer / accumulator rows: E rows, then H rows from nf_geom.rowsE
+    uint32_t nf_NC = 0;    // accumulator columns: 2F rounded up to kFoldCols
     uint32_t nf_slot = 0, nf_ts0 = 0;
     double nf_dT = 0, nf_wnorm = 1;
     Nf2ffGeom nf_geom{};
@@ -142,7 +144,7 @@ struct efs_ctx {
     std::vector<uint32_t> nf_perm;
     float *nf_el = nullptr, *nf_buf = nullptr, *nf_acc = nullptr;
     double* nf_omega = nullptr;
-    float2* nf_tw = nullptr;
+    float* nf_tw = nullptr;  // [2][K][NC] twiddles
 
     // energy
     double* en_partial = nullptr;
@@ -434,17 +436,15 @@ int drain_profile(efs_ctx* c) {
 int nf2ff_fold(efs_ctx* c) {
     if (!c->nf_P || c->nf_slot == 0) return EFS_OK;
     const uint32_t kused = c->nf_slot;
-    const uint32_t nt = 2 * kused * c->nf_F;
+    const uint32_t nt = 2 * kused * c->nf_NC;
     hipLaunchKernelGGL(k_nf2ff_twiddle, dim3(nblk(nt)), dim3(kPointThreads), 0, c->stream, c->nf_tw, c->nf_K,
-                       kused, c->nf_F, c->nf_ts0, c->nf_omega, c->nf_dT, c->nf_maxTS, c->nf_window,
+                       kused, c->nf_F, c->nf_NC, c->nf_ts0, c->nf_omega, c->nf_dT, c->nf_maxTS, c->nf_window,
                        c->nf_wnorm);
     CHECK_LAUNCH(c);
-    constexpr int FB = 8;
-    const uint32_t rows = 6 * c->nf_P;
-    const size_t lds = (size_t)2 * kused * FB * sizeof(float2);
-    dim3 grid(nblk(rows), (c->nf_F + FB - 1) / FB);
-    hipLaunchKernelGGL(k_nf2ff_fold<FB>, grid, dim3(256), lds, c->stream, c->nf_buf, rows, 3 * c->nf_P,
-                       c->nf_K, kused, c->nf_tw, c->nf_F, c->nf_acc);
+    const size_t lds = (size_t)((kused + 1) & ~1u) * kFoldCols * sizeof(float);
+    const dim3 grid(c->nf_rows / kFoldRows, c->nf_NC / kFoldCols);
+    hipLaunchKernelGGL(k_nf2ff_fold, grid, dim3(256), lds, c->stream, c->nf_buf, c->nf_rows,
+                       c->nf_geom.rowsE, c->nf_K, kused, c->nf_tw, c->nf_NC, c->nf_acc);
     CHECK_LAUNCH(c);
     c->nf_slot = 0;
     return EFS_OK;
@@ -693,7 +693,7 @@ int launch_step(efs_ctx* c) {
         if (c->nf_slot == 0) c->nf_ts0 = t + 1;
         hipLaunchKernelGGL(k_nf2ff_sample, dim3(nblk(c->nf_P)), dim3(kPointThreads), 0, c->stream, c->volt,
                            c->curr, c->nf_pts, c->nf_el, c->nf_geom,
-                           c->nf_buf + (size_t)c->nf_slot * 6 * c->nf_P);
+                           c->nf_buf + (size_t)c->nf_slot * c->nf_rows);
         c->nf_slot++;
         if (c->nf_slot == c->nf_K) TRY(nf2ff_fold(c));
     }
@@ -1200,6 +1200,11 @@ int efs_cfg_nf2ff(efs_ctx* c, const uint32_t* points, uint32_t P, const double* 
     Nf2ffGeom g{};
     g.nx = c->nx; g.ny = c->ny; g.nz = c->nz; g.P = P;
     g.S = c->S;
+    const uint64_t rowsE = (3 * (uint64_t)P + kFoldRows - 1) / kFoldRows * kFoldRows;
+    if (2 * rowsE >= (1ull << 31)) return fail(c, EFS_ERR_ARG, "too many NF2FF points");
+    g.rowsE = (uint32_t)rowsE;
+    const uint32_t rows = 2 * g.rowsE;
+    const uint32_t NC = (2 * F + kFoldCols - 1) / kFoldCols * kFoldCols;
     for (int a = 0; a < 6; ++a) {
         g.el_off[a] = (uint32_t)el_total;
         el_total += el_len[a];
@@ -1216,11 +1221,12 @@ int efs_cfg_nf2ff(efs_ctx* c, const uint32_t* points, uint32_t P, const double* 
     c->nf_perm = std::move(perm);
     TRY(dupload(c, c->nf_el, edge_lens, el_total));
     TRY(dupload(c, c->nf_omega, omega, F));
-    TRY(dalloc(c, c->nf_buf, (size_t)K * 6 * P));
-    TRY(dzero(c, c->nf_acc, (size_t)F * 2 * 6 * P));
-    TRY(dalloc(c, c->nf_tw, (size_t)2 * K * F));
+    TRY(dzero(c, c->nf_buf, (size_t)K * rows));  // padding rows stay zero
+    TRY(dzero(c, c->nf_acc, (size_t)rows * NC));
+    TRY(dalloc(c, c->nf_tw, (size_t)2 * K * NC));
     c->nf_geom = g;
     c->nf_P = P; c->nf_F = F; c->nf_K = K;
+    c->nf_rows = rows; c->nf_NC = NC;
     c->nf_dT = dT; c->nf_maxTS = max_ts; c->nf_window = window_type;
     c->nf_wnorm = 1.0;
     if (window_type == 1 && max_ts > 1) {
@@ -1237,24 +1243,24 @@ int efs_read_nf2ff(efs_ctx* c, float* accE, float* accH) {
     if (!c->nf_P) return fail(c, EFS_ERR_STATE, "NF2FF accumulation not configured");
     TRY(nf2ff_fold(c));
     HIP_TRY(c, hipStreamSynchronize(c->stream));
-    const size_t rows = 6 * (size_t)c->nf_P, P = c->nf_P;
-    std::vector<float> acc((size_t)c->nf_F * 2 * rows);
+    const size_t P = c->nf_P, NC = c->nf_NC, RE = c->nf_geom.rowsE;
+    std::vector<float> acc((size_t)c->nf_rows * NC);
     HIP_TRY(c, hipMemcpy(acc.data(), c->nf_acc, acc.size() * sizeof(float), hipMemcpyDeviceToHost));
-    for (uint32_t f = 0; f < c->nf_F; ++f) {
-        const float* re = acc.data() + (size_t)f * 2 * rows;
-        const float* im = re + rows;
-        float* oE = accE ? accE + (size_t)f * rows : nullptr;
-        float* oH = accH ? accH + (size_t)f * rows : nullptr;
-        for (size_t q = 0; q < P; ++q) {
-            const size_t p = c->nf_perm[q];
-            for (int k = 0; k < 3; ++k) {
-                if (oE) {
-                    oE[6 * p + 2 * k] = re[k * P + q];
-                    oE[6 * p + 2 * k + 1] = im[k * P + q];
+    // device [row][2f + re/im] (grid-ordered points) -> caller [f][6p + 2k + re/im]
+    for (size_t q = 0; q < P; ++q) {
+        const size_t p = c->nf_perm[q];
+        for (int k = 0; k < 3; ++k) {
+            const float* e = acc.data() + (k * P + q) * NC;
+            const float* h = acc.data() + (RE + k * P + q) * NC;
+            for (uint32_t f = 0; f < c->nf_F; ++f) {
+                const size_t o = (size_t)f * 6 * P + 6 * p + 2 * k;
+                if (accE) {
+                    accE[o] = e[2 * f];
+                    accE[o + 1] = e[2 * f + 1];
                 }
-                if (oH) {
-                    oH[6 * p + 2 * k] = re[(3 + k) * P + q];
-                    oH[6 * p + 2 * k + 1] = im[(3 + k) * P + q];
+                if (accH) {
+                    accH[o] = h[2 * f];
+                    accH[o + 1] = h[2 * f + 1];
                 }
             }
         }
@@ -1291,8 +1297,8 @@ int efs_farfield(efs_ctx* c, uint32_t f, const float* meta, const float* theta, 
         q.cx = (float)center[0]; q.cy = (float)center[1]; q.cz = (float)center[2];
         q.fac_re = (float)(fmag * -std::sin(fph));
         q.fac_im = (float)(fmag * std::cos(fph));
-        hipLaunchKernelGGL(k_farfield, dim3(nblk(na)), dim3(256), 0, c->stream,
-                           c->nf_acc + (size_t)f * 2 * 6 * c->nf_P, d_meta, d_th, d_ph, q, d_out);
+        hipLaunchKernelGGL(k_farfield, dim3(nblk(na)), dim3(256), 0, c->stream, c->nf_acc + 2 * (size_t)f,
+                           c->nf_NC, c->nf_geom.rowsE, d_meta, d_th, d_ph, q, d_out);
         hipError_t e = hipGetLastError();
         if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
         if (e == hipSuccess) e = hipMemcpy(out, d_out, 5 * na * sizeof(float), hipMemcpyDeviceToHost);

@@ -3,21 +3,31 @@
 // The reference accumulates one frequency every step with an FP32 phase
 // (NF2FF accumulate WGSL, src/webgpu-engine.mjs:637-800).  Here the
 // cell-interpolated surface E/H of K consecutive steps are written to a
-// buffer B[K][6P] and folded into F frequency bins at once:
-//     acc[f][re|im][row] += sum_k B[k][row] * W[k][f]
-// a complex [6P x K] x [K x F] product whose twiddles
-//     W[k][f] = exp(-j*omega_f*t_k) * w(n_k) * windowNorm * dT
-// (t_k = n_k*dT for E rows, (n_k+1/2)*dT for H rows) are generated in FP64.
-// Row r = c*P + p for E component c, (3+c)*P + p for H component c.
+// buffer B[K][rows] and folded into F frequency bins at once:
+//     acc[row][2f + {0: re, 1: im}] += sum_k B[k][row] * W[h][k][2f + {0, 1}]
+// a real [rows x K] x [K x 2F] product (the complex product with re/im
+// interleaved in the columns) on the matrix cores, whose twiddles
+//     W[h][k][f] = exp(-j*omega_f*t_k) * w(n_k) * windowNorm * dT
+// (t_k = n_k*dT for E rows, h = 0; (n_k+1/2)*dT for H rows, h = 1) are
+// generated in FP64.  Row c*P + p holds E component c of point p, row
+// rowsE + c*P + p H component c; rowsE = 3P rounded up to kFoldRows, so a
+// fold workgroup's rows are all E or all H.
 #pragma once
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include "sweep.hpp"
+
 namespace efs {
+
+constexpr int kFoldRows = 512;  // rows per fold workgroup: 4 waves x 128
+constexpr int kFoldCols = 32;   // columns per fold workgroup (16 frequencies)
 
 struct Nf2ffGeom {
     uint32_t nx, ny, nz, P;
     uint32_t el_off[6];  // offsets of primal x,y,z / dual x,y,z edge lengths
+    uint32_t rowsE;      // first H row
+    uint32_t pad;
     uint64_t S;          // component stride of the field arrays
 };
 
@@ -27,7 +37,9 @@ struct Nf2ffGeom {
 __global__ void k_nf2ff_sample(const float* __restrict__ volt, const float* __restrict__ curr,
                                const uint32_t* __restrict__ pts, const float* __restrict__ el, Nf2ffGeom g,
                                float* __restrict__ row_out /* buf + slot*6P */) {
-    const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+    // points are in grid order; consecutive workgroups share cells, so they
+    // stay on one XCD (own L2)
+    const uint32_t p = xcd_remap(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
     if (p >= g.P) return;
     const uint32_t ix = pts[3 * p], iy = pts[3 * p + 1], iz = pts[3 * p + 2];
     const size_t Ny = g.ny, Nz = g.nz;
@@ -61,68 +73,101 @@ __global__ void k_nf2ff_sample(const float* __restrict__ volt, const float* __re
         }
         H[n] = h;
     }
-    const size_t P = g.P;
+    const size_t P = g.P, RE = g.rowsE;
     row_out[p] = E0;
     row_out[P + p] = E1;
     row_out[2 * P + p] = E2;
-    row_out[3 * P + p] = H[0];
-    row_out[4 * P + p] = H[1];
-    row_out[5 * P + p] = H[2];
+    row_out[RE + p] = H[0];
+    row_out[RE + P + p] = H[1];
+    row_out[RE + 2 * P + p] = H[2];
 }
 
-// FP64 twiddles for k in [0,kused): tw[(h*K + k)*F + f] = {re, im}, h = 0 (E)
-// or 1 (H, half-step later).  The Hann weight uses the integer step index
-// for both, as the reference does.
-__global__ void k_nf2ff_twiddle(float2* __restrict__ tw, uint32_t K, uint32_t kused, uint32_t F, uint32_t ts0,
-                                const double* __restrict__ omega, double dT, uint32_t maxTS,
+// FP64 twiddles for k in [0,kused): tw[(h*K + k)*NC + 2f + {0: re, 1: im}],
+// h = 0 (E) or 1 (H, half-step later), zero for f >= F.  The Hann weight
+// uses the integer step index for both, as the reference does.
+__global__ void k_nf2ff_twiddle(float* __restrict__ tw, uint32_t K, uint32_t kused, uint32_t F, uint32_t NC,
+                                uint32_t ts0, const double* __restrict__ omega, double dT, uint32_t maxTS,
                                 uint32_t window, double wnorm) {
     const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= 2 * kused * F) return;
-    const uint32_t f = t % F, k = (t / F) % kused, h = t / (F * kused);
-    const double n = (double)ts0 + (double)k;
-    double w = 1.0;
-    if (window == 1 && maxTS > 1) w = 0.5 * (1.0 - cos(2.0 * M_PI * n / (double)(maxTS - 1)));
-    const double s = w * wnorm * dT;
-    const double ph = omega[f] * (n + (h ? 0.5 : 0.0)) * dT;
-    double sn, cs;
-    sincos(ph, &sn, &cs);
-    tw[((size_t)h * K + k) * F + f] = make_float2((float)(cs * s), (float)(-sn * s));
+    if (t >= 2 * kused * NC) return;
+    const uint32_t c = t % NC, k = (t / NC) % kused, h = t / (NC * kused);
+    const uint32_t f = c / 2;
+    float v = 0.f;
+    if (f < F) {
+        const double n = (double)ts0 + (double)k;
+        double w = 1.0;
+        if (window == 1 && maxTS > 1) w = 0.5 * (1.0 - cos(2.0 * M_PI * n / (double)(maxTS - 1)));
+        const double s = w * wnorm * dT;
+        const double ph = omega[f] * (n + (h ? 0.5 : 0.0)) * dT;
+        double sn, cs;
+        sincos(ph, &sn, &cs);
+        v = (float)((c & 1u) ? -sn * s : cs * s);
+    }
+    tw[((size_t)h * K + k) * NC + c] = v;
 }
 
-// Fold: one thread per row, FB frequencies per workgroup (grid.y over F).
-template <int FB>
-__global__ __launch_bounds__(256) void k_nf2ff_fold(const float* __restrict__ buf, uint32_t rows,
-                                                    uint32_t rowsE, uint32_t K, uint32_t kused,
-                                                    const float2* __restrict__ tw, uint32_t F,
-                                                    float* __restrict__ acc) {
-    extern __shared__ float2 s_tw[];  // [2][kused][FB]
-    const uint32_t f0 = blockIdx.y * FB;
-    for (uint32_t t = threadIdx.x; t < 2 * kused * FB; t += blockDim.x) {
-        const uint32_t f = t % FB, k = (t / FB) % kused, h = t / (FB * kused);
-        s_tw[t] = (f0 + f < F) ? tw[((size_t)h * K + k) * F + f0 + f] : make_float2(0.f, 0.f);
+typedef float f32x16 __attribute__((ext_vector_type(16)));
+
+// Fold on the matrix cores (v_mfma_f32_32x32x2f32: D[32x32] += A[32x2] B[2x32]).
+// Workgroup: kFoldRows rows x 32 columns (column tile blockIdx.y); wave w
+// owns rows r0 + 4m + j (m < 32, j < 4) of r0 = base + 128w as four 32-row
+// tiles j, so one 16-byte load per lane, buf[k][r0 + 4m .. +3], feeds the
+// four products of a k-pair.  Operand layouts of the instruction: lane l
+// gives A[m = l%32][k = l/32] and B[k = l/32][n = l%32]; D[m][n] is in lane
+// n + 32*((m/4)%2), accumulator register (m%4) + 4*(m/8).  The block's sums
+// start at zero and are added to acc (the running DFT) at the end; every
+// acc element belongs to exactly one lane.
+__global__ __launch_bounds__(256) void k_nf2ff_fold(const float* __restrict__ buf, uint32_t rows, uint32_t rowsE,
+                                                    uint32_t K, uint32_t kused, const float* __restrict__ tw,
+                                                    uint32_t NC, float* __restrict__ acc) {
+    extern __shared__ float s_w[];  // [kpad][32]: this tile's twiddle columns, zero rows past kused
+    const uint32_t base = blockIdx.x * kFoldRows;
+    const uint32_t h = base >= rowsE ? 1u : 0u;
+    const uint32_t c0 = blockIdx.y * kFoldCols;
+    const uint32_t kp = (kused + 1u) & ~1u;
+    for (uint32_t t = threadIdx.x; t < kp * kFoldCols; t += blockDim.x) {
+        const uint32_t k = t / kFoldCols, c = t % kFoldCols;
+        s_w[t] = k < kused ? tw[((size_t)h * K + k) * NC + c0 + c] : 0.f;
     }
     __syncthreads();
-    const uint32_t r = blockIdx.x * blockDim.x + threadIdx.x;
-    if (r >= rows) return;
-    const float2* t = s_tw + (r >= rowsE ? kused * FB : 0);
-    float re[FB], im[FB];
+    const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
+    const uint32_t r0 = base + wave * 128u;
+    const uint32_t col = lane & 31u, kh = lane >> 5;
+    f32x16 D[4];
 #pragma unroll
-    for (int f = 0; f < FB; ++f) re[f] = im[f] = 0.f;
-    for (uint32_t k = 0; k < kused; ++k) {
-        const float v = buf[(size_t)k * rows + r];
+    for (int j = 0; j < 4; ++j) D[j] = (f32x16)0.f;
+    const float* a_ptr = buf + r0 + 4u * col;
+    auto pair = [&](v4 a, float b) {
+        D[0] = __builtin_amdgcn_mfma_f32_32x32x2f32(a.x, b, D[0], 0, 0, 0);
+        D[1] = __builtin_amdgcn_mfma_f32_32x32x2f32(a.y, b, D[1], 0, 0, 0);
+        D[2] = __builtin_amdgcn_mfma_f32_32x32x2f32(a.z, b, D[2], 0, 0, 0);
+        D[3] = __builtin_amdgcn_mfma_f32_32x32x2f32(a.w, b, D[3], 0, 0, 0);
+    };
+    // 8 k-pairs per batch: all loads issued before the first product
+    const uint32_t kfull = kused & ~1u;  // k-pairs with both steps inside [0, kused)
+    uint32_t k = 0;
+    for (; k + 16 <= kfull; k += 16) {
+        v4 a[8];
 #pragma unroll
-        for (int f = 0; f < FB; ++f) {
-            const float2 w = t[k * FB + f];
-            re[f] += v * w.x;
-            im[f] += v * w.y;
-        }
+        for (int q = 0; q < 8; ++q) a[q] = ld4(a_ptr + (size_t)(k + 2 * q + kh) * rows);
+        __builtin_amdgcn_sched_barrier(0);  // keep the 8 loads ahead of the products
+#pragma unroll
+        for (int q = 0; q < 8; ++q) pair(a[q], s_w[(k + 2 * q + kh) * kFoldCols + col]);
     }
+    for (; k < kfull; k += 2) pair(ld4(a_ptr + (size_t)(k + kh) * rows), s_w[(k + kh) * kFoldCols + col]);
+    if (k < kp) {  // odd kused: the second step of the last pair is zero
+        const uint32_t kk = k + kh;
+        pair(kk < kused ? ld4(a_ptr + (size_t)kk * rows) : (v4)0.f, s_w[kk * kFoldCols + col]);
+    }
+    float* const acc_t = acc + c0;  // uniform base; per-lane 32-bit element offsets
+    const uint32_t o0 = (r0 + 16u * kh) * NC + col;
 #pragma unroll
-    for (int f = 0; f < FB; ++f) {
-        if (f0 + f < F) {
-            acc[((size_t)(f0 + f) * 2) * rows + r] += re[f];
-            acc[((size_t)(f0 + f) * 2 + 1) * rows + r] += im[f];
-        }
+    for (int j = 0; j < 4; ++j) {
+        float v[16];
+#pragma unroll
+        for (int i = 0; i < 16; ++i) v[i] = acc_t[o0 + (4u * (8u * (i / 4) + (i % 4)) + j) * NC];
+#pragma unroll
+        for (int i = 0; i < 16; ++i) acc_t[o0 + (4u * (8u * (i / 4) + (i % 4)) + j) * NC] = v[i] + D[j][i];
     }
 }
 
@@ -136,7 +181,8 @@ struct FarParams {
 
 constexpr int kFarTile = 256;
 
-__global__ __launch_bounds__(256) void k_farfield(const float* __restrict__ acc_f /* [2][6P] */,
+// acc_f = acc + 2f: the frequency's (re, im) column pair, NC floats per row.
+__global__ __launch_bounds__(256) void k_farfield(const float* __restrict__ acc_f, uint32_t NC, uint32_t rowsE,
                                                   const float* __restrict__ meta, const float* __restrict__ theta,
                                                   const float* __restrict__ phi, FarParams q,
                                                   float* __restrict__ out) {
@@ -149,7 +195,6 @@ __global__ __launch_bounds__(256) void k_farfield(const float* __restrict__ acc_
         sincosf(phi[a % q.nP], &sP, &cP);
     }
     double Ntr = 0, Nti = 0, Npr = 0, Npi = 0, Ltr = 0, Lti = 0, Lpr = 0, Lpi = 0;
-    const size_t rows = 6 * (size_t)q.P;
     for (uint32_t base = 0; base < q.P; base += kFarTile) {
         __syncthreads();
         const uint32_t p = base + threadIdx.x;
@@ -163,10 +208,12 @@ __global__ __launch_bounds__(256) void k_farfield(const float* __restrict__ acc_
             s[5] = meta[8 * (size_t)p + 5];
 #pragma unroll
             for (int c = 0; c < 3; ++c) {
-                s[6 + 2 * c] = acc_f[(size_t)c * q.P + p];
-                s[7 + 2 * c] = acc_f[rows + (size_t)c * q.P + p];
-                s[12 + 2 * c] = acc_f[(size_t)(3 + c) * q.P + p];
-                s[13 + 2 * c] = acc_f[rows + (size_t)(3 + c) * q.P + p];
+                const float* e = acc_f + ((size_t)c * q.P + p) * NC;
+                const float* hh = acc_f + ((size_t)rowsE + (size_t)c * q.P + p) * NC;
+                s[6 + 2 * c] = e[0];
+                s[7 + 2 * c] = e[1];
+                s[12 + 2 * c] = hh[0];
+                s[13 + 2 * c] = hh[1];
             }
         }
         __syncthreads();
