@@ -122,6 +122,16 @@ struct efs_ctx {
     uint32_t *rlc_gidx = nullptr, *rlc_type = nullptr;
     float *rlc_c = nullptr, *rlc_vdn = nullptr, *rlc_jn = nullptr, *rlc_vil = nullptr;
 
+    // periodic boundaries: one entry per axis, in configuration order
+    std::vector<PbcAxis> pbc_e, pbc_h;
+    float* pbc_imag = nullptr;  // Bloch imaginary parts (4 face-sized buffers per phased axis)
+
+    // steady-state detection
+    uint32_t ss_n = 0, ss_period = 0, ss_current = 0;
+    bool ss_recording = false;
+    uint32_t* ss_gidx = nullptr;
+    float* ss_e = nullptr;  // [2][ss_n]: period 1, period 2
+
     // probe gather
     uint32_t g_n = 0;
     uint32_t* g_idx = nullptr;
@@ -468,6 +478,7 @@ void free_all(efs_ctx* c) {
     dfree(c->ti_idx); dfree(c->ti_dint); dfree(c->ti_dfrac); dfree(c->ti_amp);
     dfree(c->rlc_gidx); dfree(c->rlc_type); dfree(c->rlc_c); dfree(c->rlc_vdn); dfree(c->rlc_jn); dfree(c->rlc_vil);
     dfree(c->g_idx); dfree(c->g_out); dfree(c->gb_idx); dfree(c->gb_ring);
+    dfree(c->pbc_imag); dfree(c->ss_gidx); dfree(c->ss_e);
     dfree(c->nf_pts); dfree(c->nf_el); dfree(c->nf_buf); dfree(c->nf_acc); dfree(c->nf_omega); dfree(c->nf_tw);
     dfree(c->en_partial);
 }
@@ -491,6 +502,11 @@ void reset_extensions(efs_ctx* c) {
     c->gb_count = 0;
     c->nf_P = 0;
     c->nf_slot = 0;
+    c->pbc_e.clear();
+    c->pbc_h.clear();
+    c->ss_n = 0;
+    c->ss_current = 0;
+    c->ss_recording = false;
 }
 
 bool box_overlap(const PmlRegion& a, const PmlRegion& b) {
@@ -594,7 +610,12 @@ int launch_step(efs_ctx* c) {
     const uint32_t t = c->numTS;
     ProfScope step_scope(c, EFS_PROF_STEP);
     const bool has_pml = c->pml.n > 0;
-    // pre-voltage
+    // pre-voltage: steady-state energies first (priority +2M)
+    if (c->ss_n && c->ss_recording) {
+        hipLaunchKernelGGL(k_ss_accumulate, dim3(nblk(c->ss_n)), dim3(kPointThreads), 0, c->stream, c->volt,
+                           c->ss_gidx, c->ss_n, c->ss_period, c->ss_current, c->ss_e, c->ss_e + c->ss_n);
+        c->ss_current++;
+    }
     if (c->ade_n) {
         hipLaunchKernelGGL(k_ade_update, dim3(nblk(c->ade_n)), dim3(kPointThreads), 0, c->stream, c->volt,
                            c->pml_vflux, c->ade_src_v, c->ade_lor, c->ade_c[0], c->ade_c[1], c->ade_c[2],
@@ -633,6 +654,9 @@ int launch_step(efs_ctx* c) {
                            c->tfsf_signal, c->tfsf_len, c->tfsf_period, c->tv_idx, c->tv_dint, c->tv_dfrac,
                            c->tv_amp, c->tfsf_nv, t);
     }
+    for (const PbcAxis& a : c->pbc_e)  // after TFSF-V, before Mur post (src/webgpu-engine.mjs:2461)
+        hipLaunchKernelGGL(k_pbc, dim3(nblk((size_t)a.b0 * a.b1)), dim3(kPointThreads), 0, c->stream, c->volt,
+                           c->pbc_imag, a, c->ny, c->nz, (uint64_t)c->S);
     if (c->mur_n) {
         hipLaunchKernelGGL(k_mur_post, dim3(nblk(c->mur_n)), dim3(kPointThreads), 0, c->stream, c->volt,
                            c->mur_coeff, c->mur_shifted, c->mur_saved, c->mur_n);
@@ -684,6 +708,9 @@ int launch_step(efs_ctx* c) {
                            c->tfsf_signal, c->tfsf_len, c->tfsf_period, c->ti_idx, c->ti_dint, c->ti_dfrac,
                            c->ti_amp, c->tfsf_ni, t);
     }
+    for (const PbcAxis& a : c->pbc_h)  // after TFSF-I (src/webgpu-engine.mjs:2477)
+        hipLaunchKernelGGL(k_pbc, dim3(nblk((size_t)a.b0 * a.b1)), dim3(kPointThreads), 0, c->stream, c->curr,
+                           c->pbc_imag, a, c->ny, c->nz, (uint64_t)c->S);
     if (c->ade_n && c->ade_apply) {
         hipLaunchKernelGGL(k_ade_apply, dim3(nblk(c->ade_n)), dim3(kPointThreads), 0, c->stream, c->curr,
                            c->ade_gidx, c->ade_ia, c->ade_n);
@@ -1074,6 +1101,77 @@ int efs_cfg_rlc(efs_ctx* c, uint32_t n, const uint32_t* field_idx, const uint32_
     TRY(dzero(c, c->rlc_jn, 3 * (size_t)n));
     TRY(dzero(c, c->rlc_vil, n));
     c->rlc_n = n;
+    return EFS_OK;
+}
+
+int efs_cfg_pbc(efs_ctx* c, uint32_t n, const uint32_t* axis, const double* phase) {
+    TRY(need_init(c));
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    if (n > 3) return fail(c, EFS_ERR_ARG, "at most 3 periodic axes");
+    if (n && (!axis || !phase)) return fail(c, EFS_ERR_ARG, "PBC axes and phases are required");
+    const uint32_t dims[3] = {c->nx, c->ny, c->nz};
+    std::vector<PbcAxis> ve, vh;
+    uint32_t imag = 0;
+    for (uint32_t k = 0; k < n; ++k) {
+        if (axis[k] > 2) return fail(c, EFS_ERR_ARG, "PBC axis %u out of range", axis[k]);
+        const uint32_t ax = axis[k];
+        if (dims[ax] < 2) return fail(c, EFS_ERR_ARG, "PBC axis %u needs at least 2 grid lines", ax);
+        PbcAxis a{};
+        a.axis = ax;
+        a.t0 = (ax + 1) % 3;
+        a.t1 = (ax + 2) % 3;
+        const uint32_t f0 = dims[a.t0], f1 = dims[a.t1];
+        a.f1 = f1;
+        a.plane = f0 * f1;
+        a.cs = (float)std::cos(phase[k]);  // the engine's f32 uniform (src/webgpu-engine.mjs:2091-2092)
+        a.sn = (float)std::sin(phase[k]);
+        a.has_imag = a.sn != 0.0f;
+        a.imag_off = imag;
+        if (a.has_imag) imag += 4 * a.plane;
+        PbcAxis e = a, h = a;
+        e.src = dims[ax] - 2; e.dst = 0; e.b0 = f0; e.b1 = f1;                       // E: plane N-2 -> plane 0
+        h.src = 0; h.dst = dims[ax] - 2; h.b0 = f0 - 1; h.b1 = f1 - 1;               // H: plane 0 -> plane N-2
+        h.imag_off += 2 * a.plane;
+        ve.push_back(e);
+        if (h.b0 && h.b1) vh.push_back(h);
+    }
+    TRY(dzero(c, c->pbc_imag, imag));
+    c->pbc_e = ve;
+    c->pbc_h = vh;
+    return EFS_OK;
+}
+
+int efs_cfg_steady_state(efs_ctx* c, const uint32_t* probe_idx, uint32_t n, uint32_t period_samples) {
+    TRY(need_init(c));
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    if (n && !probe_idx) return fail(c, EFS_ERR_ARG, "steady-state probe indices are required");
+    for (uint32_t k = 0; k < n; ++k)
+        if (probe_idx[k] >= 3 * c->N) return fail(c, EFS_ERR_ARG, "steady-state probe %u: index out of range", k);
+    const std::vector<uint32_t> g = tr_all(c, probe_idx, n);
+    TRY(dupload(c, c->ss_gidx, g.data(), n));
+    TRY(dzero(c, c->ss_e, 2 * (size_t)n));
+    c->ss_n = n;
+    c->ss_period = period_samples;
+    c->ss_current = 0;
+    c->ss_recording = false;
+    return EFS_OK;
+}
+
+int efs_ss_set(efs_ctx* c, int recording, uint32_t current_sample) {
+    TRY(need_init(c));
+    c->ss_recording = recording != 0;
+    c->ss_current = current_sample;
+    return EFS_OK;
+}
+
+int efs_ss_read(efs_ctx* c, float* e1, float* e2, uint32_t* current_sample, int* recording) {
+    TRY(need_init(c));
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    if (c->ss_n && e1) HIP_TRY(c, hipMemcpy(e1, c->ss_e, c->ss_n * sizeof(float), hipMemcpyDeviceToHost));
+    if (c->ss_n && e2)
+        HIP_TRY(c, hipMemcpy(e2, c->ss_e + c->ss_n, c->ss_n * sizeof(float), hipMemcpyDeviceToHost));
+    if (current_sample) *current_sample = c->ss_current;
+    if (recording) *recording = c->ss_recording ? 1 : 0;
     return EFS_OK;
 }
 

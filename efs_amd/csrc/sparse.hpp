@@ -139,6 +139,60 @@ __global__ void k_rlc_apply(float* __restrict__ volt, const uint32_t* __restrict
     }
 }
 
+// Periodic boundary copy for one axis (PBC copy WGSL,
+// src/webgpu-engine.mjs:475-584): thread (i0, i1) of the b0 x b1 face copies
+// both tangential components from plane src to plane dst of the axis; with
+// a Bloch phase the pair (field, imaginary part) is rotated: dst = c*re -
+// s*im, im = s*re + c*im, the imaginary part kept per face cell.
+struct PbcAxis {
+    uint32_t axis, t0, t1;   // periodic axis and its tangential axes
+    uint32_t f1;             // face extent along t1 (row length of the imaginary buffers)
+    uint32_t b0, b1;         // face cells handled: f0 x f1 (E) or (f0-1) x (f1-1) (H)
+    uint32_t src, dst;       // source / destination plane index along the axis
+    uint32_t imag_off;       // this face's first imaginary buffer (component t0; t1 at +plane)
+    uint32_t plane;          // f0 * f1
+    float cs, sn;
+    uint32_t has_imag;
+};
+
+__global__ void k_pbc(float* __restrict__ field, float* __restrict__ imag, PbcAxis a, uint32_t ny, uint32_t nz,
+                      uint64_t S) {
+    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= a.b0 * a.b1) return;
+    const uint32_t i0 = t / a.b1, i1 = t - i0 * a.b1;
+    uint32_t ps[3], pd[3];
+    ps[a.axis] = a.src;
+    pd[a.axis] = a.dst;
+    ps[a.t0] = pd[a.t0] = i0;
+    ps[a.t1] = pd[a.t1] = i1;
+    const size_t ls = ((size_t)ps[0] * ny + ps[1]) * nz + ps[2], ld = ((size_t)pd[0] * ny + pd[1]) * nz + pd[2];
+#pragma unroll
+    for (int q = 0; q < 2; ++q) {
+        const uint32_t comp = q ? a.t1 : a.t0;
+        const float re = field[comp * S + ls];
+        if (!a.has_imag) {
+            field[comp * S + ld] = a.cs * re;
+        } else {
+            float* ip = imag + a.imag_off + (size_t)q * a.plane + (size_t)i0 * a.f1 + i1;
+            const float im = *ip;
+            field[comp * S + ld] = a.cs * re - a.sn * im;
+            *ip = a.sn * re + a.cs * im;
+        }
+    }
+}
+
+// Steady-state energy accumulation (src/webgpu-engine.mjs:586-625): v^2 of
+// each probe added to period 1 while current < period, else to period 2.
+__global__ void k_ss_accumulate(const float* __restrict__ volt, const uint32_t* __restrict__ gidx, uint32_t n,
+                                uint32_t period, uint32_t current, float* __restrict__ e1,
+                                float* __restrict__ e2) {
+    const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    const float v = volt[gidx[k]];
+    if (current < period) e1[k] += v * v;
+    else e2[k] += v * v;
+}
+
 // Probe gather into out[2*k], out[2*k+1] (src/webgpu-engine.mjs:2593-2665).
 __global__ void k_gather(const float* __restrict__ volt, const float* __restrict__ curr,
                          const uint32_t* __restrict__ idx, float* __restrict__ out, uint32_t n) {

@@ -66,6 +66,10 @@ SIGNATURES = {
     "efs_profile_enable": (C.c_int, [C.c_void_p, C.c_int]),
     "efs_profile_read": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_uint64)]),
     "efs_path_flags": (C.c_int, [C.c_void_p, C.POINTER(C.c_uint32)]),
+    "efs_cfg_pbc": (C.c_int, [C.c_void_p, C.c_uint32, _u32p, _f64p]),
+    "efs_cfg_steady_state": (C.c_int, [C.c_void_p, _u32p, C.c_uint32, C.c_uint32]),
+    "efs_ss_set": (C.c_int, [C.c_void_p, C.c_int, C.c_uint32]),
+    "efs_ss_read": (C.c_int, [C.c_void_p, _f32p, _f32p, C.POINTER(C.c_uint32), C.POINTER(C.c_int)]),
 }
 
 
@@ -196,6 +200,66 @@ class HIPEngine:
             c = config["coeff"]
             coeff = np.full(len(nidx), c, dtype=np.float32) if np.isscalar(c) else f32(c)
         self._check(self.lib.efs_cfg_mur(self.ctx, len(nidx), f32(coeff), u32(nidx), u32(sidx)))
+
+    def configurePBC(self, config):
+        """configurePBC({axes: [{axis, phase}]}) (src/webgpu-engine.mjs:1976-2155)."""
+        self._need_device()
+        axes = config.get("axes") or []
+        if not axes:
+            return
+        self._check(self.lib.efs_cfg_pbc(self.ctx, len(axes), u32([a["axis"] for a in axes]),
+                                         np.ascontiguousarray([float(a.get("phase") or 0) for a in axes])))
+
+    def configureSteadyState(self, config):
+        """configureSteadyState({probe_idx, periodSamples, threshold})
+        (src/webgpu-engine.mjs:2161-2217)."""
+        self._need_device()
+        idx = u32(config["probe_idx"])
+        self._ss_n = len(idx)
+        self._ss_threshold = float(config.get("threshold") or 1e-6)
+        self._ss_period = int(config["periodSamples"])
+        self._check(self.lib.efs_cfg_steady_state(self.ctx, idx, len(idx), self._ss_period))
+
+    def _ss_state(self):
+        cur, rec = C.c_uint32(0), C.c_int(0)
+        n = max(getattr(self, "_ss_n", 0), 1)
+        e1, e2 = np.zeros(n, np.float32), np.zeros(n, np.float32)
+        self._check(self.lib.efs_ss_read(self.ctx, e1, e2, C.byref(cur), C.byref(rec)))
+        k = getattr(self, "_ss_n", 0)
+        return e1[:k], e2[:k], cur.value, bool(rec.value)
+
+    @property
+    def ssRecording(self):
+        return self._ss_state()[3]
+
+    @ssRecording.setter
+    def ssRecording(self, on):
+        self._check(self.lib.efs_ss_set(self.ctx, 1 if on else 0, self._ss_state()[2]))
+
+    @property
+    def ssCurrentSample(self):
+        return self._ss_state()[2]
+
+    @ssCurrentSample.setter
+    def ssCurrentSample(self, v):
+        self._check(self.lib.efs_ss_set(self.ctx, 1 if self._ss_state()[3] else 0, int(v)))
+
+    def readSteadyState(self):
+        """The two period energies and the sample counter."""
+        e1, e2, cur, _ = self._ss_state()
+        return {"energy_period1": e1, "energy_period2": e2, "currentSample": cur}
+
+    def checkConvergence(self):
+        """Both periods recorded and max |e2-e1|/|e1| < threshold (the CPU
+        engine's rule, src/webgpu-fdtd.mjs:1077-1095)."""
+        e1, e2, cur, _ = self._ss_state()
+        if not hasattr(self, "_ss_period") or cur < 2 * self._ss_period:
+            return False
+        e1 = e1.astype(np.float64)
+        e2 = e2.astype(np.float64)
+        ok = np.abs(e1) >= 1e-30
+        ratio = np.abs(e2[ok] - e1[ok]) / np.abs(e1[ok])
+        return bool((ratio.max() if ratio.size else 0.0) < self._ss_threshold)
 
     def configureLorentzADE(self, config, apply=False):
         """configureLorentzADE (src/webgpu-engine.mjs:1358-1475).  apply=True

@@ -135,6 +135,50 @@ export class HIPEngine {
         addon.cfgMur(this._h, coeff, nidx, sidx);
     }
 
+    // configurePBC({axes: [{axis, phase}]}) (src/webgpu-engine.mjs:1976-2155).
+    configurePBC(config) {
+        this._need();
+        const axes = (config && config.axes) || [];
+        if (axes.length === 0) return;
+        addon.cfgPBC(this._h, u32(axes.map((a) => a.axis)), f64(axes.map((a) => a.phase || 0)));
+    }
+
+    // configureSteadyState({probe_idx, periodSamples, threshold})
+    // (src/webgpu-engine.mjs:2161-2217); recording starts off.
+    configureSteadyState(config) {
+        this._need();
+        const idx = u32(config.probe_idx);
+        addon.cfgSteadyState(this._h, idx, config.periodSamples >>> 0);
+        this._ss = { n: idx.length, period: config.periodSamples >>> 0, threshold: config.threshold || 1e-6 };
+    }
+
+    get ssRecording() { return this._h && this._ss ? addon.ssRead(this._h, null, null)[1] === 1 : false; }
+    set ssRecording(on) { this._need(); addon.ssSet(this._h, on ? 1 : 0, addon.ssRead(this._h, null, null)[0]); }
+    get ssCurrentSample() { return this._h && this._ss ? addon.ssRead(this._h, null, null)[0] : 0; }
+    set ssCurrentSample(v) { this._need(); addon.ssSet(this._h, addon.ssRead(this._h, null, null)[1], v >>> 0); }
+
+    // Period energies and counter (host copy).
+    readSteadyState() {
+        if (!this._ss) return null;
+        const e1 = new Float32Array(this._ss.n), e2 = new Float32Array(this._ss.n);
+        const [cur] = addon.ssRead(this._h, e1, e2);
+        return { energy_period1: e1, energy_period2: e2, currentSample: cur };
+    }
+
+    // CPU engine's convergence rule (src/webgpu-fdtd.mjs:1077-1095).
+    checkConvergence() {
+        const s = this.readSteadyState();
+        if (!s || s.currentSample < 2 * this._ss.period) return false;
+        let maxRatio = 0;
+        for (let n = 0; n < this._ss.n; n++) {
+            const e1 = s.energy_period1[n], e2 = s.energy_period2[n];
+            if (Math.abs(e1) < 1e-30) continue;
+            const r = Math.abs(e2 - e1) / Math.abs(e1);
+            if (r > maxRatio) maxRatio = r;
+        }
+        return maxRatio < this._ss.threshold;
+    }
+
     iterate(numSteps) {
         this._need();
         addon.iterate(this._h, numSteps >>> 0);

@@ -317,6 +317,58 @@ static napi_value js_cfg_rlc(napi_env env, napi_callback_info info) {
     return status(&c, ctx, efs_cfg_rlc(ctx, (uint32_t)a, fi, dr, tf, co));
 }
 
+/* cfgPBC(h, axis Uint32Array, phase Float64Array) */
+static napi_value js_cfg_pbc(napi_env env, napi_callback_info info) {
+    BEGIN();
+    CTX(0);
+    size_t a, b;
+    uint32_t *ax = U32(1, a);
+    double *ph = F64(2, b);
+    BAIL();
+    if (a != b) {
+        throwf(&c, "efs: PBC axis and phase arrays must have equal length");
+        return NULL;
+    }
+    return status(&c, ctx, efs_cfg_pbc(ctx, (uint32_t)a, ax, ph));
+}
+
+/* cfgSteadyState(h, probeIdx, periodSamples) */
+static napi_value js_cfg_ss(napi_env env, napi_callback_info info) {
+    BEGIN();
+    CTX(0);
+    size_t n;
+    uint32_t *idx = U32(1, n);
+    uint32_t period = u32_arg(&c, 2);
+    BAIL();
+    return status(&c, ctx, efs_cfg_steady_state(ctx, idx, (uint32_t)n, period));
+}
+
+/* ssSet(h, recording, currentSample) */
+static napi_value js_ss_set(napi_env env, napi_callback_info info) {
+    BEGIN();
+    CTX(0);
+    uint32_t rec = u32_arg(&c, 1), cur = u32_arg(&c, 2);
+    BAIL();
+    return status(&c, ctx, efs_ss_set(ctx, (int)rec, cur));
+}
+
+/* ssRead(h, e1 Float32Array | null, e2 | null) -> [currentSample, recording] */
+static napi_value js_ss_read(napi_env env, napi_callback_info info) {
+    BEGIN();
+    CTX(0);
+    size_t a, b;
+    float *e1 = F32OPT(1, a), *e2 = F32OPT(2, b);
+    BAIL();
+    uint32_t cur = 0;
+    int rec = 0;
+    if (!status(&c, ctx, efs_ss_read(ctx, e1, e2, &cur, &rec))) return NULL;
+    napi_value arr;
+    napi_create_array_with_length(env, 2, &arr);
+    napi_set_element(env, arr, 0, num(env, cur));
+    napi_set_element(env, arr, 1, num(env, rec));
+    return arr;
+}
+
 static napi_value js_iterate(napi_env env, napi_callback_info info) {
     BEGIN();
     CTX(0);
@@ -482,6 +534,10 @@ static napi_value Init(napi_env env, napi_value exports) {
         FN("adeApply", js_ade_apply),
         FN("cfgTFSF", js_cfg_tfsf),
         FN("cfgRLC", js_cfg_rlc),
+        FN("cfgPBC", js_cfg_pbc),
+        FN("cfgSteadyState", js_cfg_ss),
+        FN("ssSet", js_ss_set),
+        FN("ssRead", js_ss_read),
         FN("iterate", js_iterate),
         FN("sync", js_sync),
         FN("numTS", js_num_ts),

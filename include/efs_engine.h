@@ -107,6 +107,27 @@ int efs_cfg_tfsf(efs_ctx *ctx, const float *signal, uint32_t len, uint32_t perio
 int efs_cfg_rlc(efs_ctx *ctx, uint32_t n, const uint32_t *field_idx, const uint32_t *direction,
                 const uint32_t *type_flag, const float *coeffs);
 
+/* configurePBC({axes: [{axis, phase}]}) (src/webgpu-engine.mjs:1976-2155,
+ * copy kernel 475-584): n_axes (<= 3) periodic axes with Bloch phases in
+ * radians.  After the E update (following TFSF) the tangential E of plane
+ * N-2 is copied to plane 0; after the H update the tangential H of plane 0
+ * to plane N-2 over (N0-1) x (N1-1) face cells.  A phase whose f32 sine is
+ * nonzero rotates (field, imaginary part), the imaginary part kept per face
+ * cell.  Axes apply in the given order; n_axes 0 clears. */
+int efs_cfg_pbc(efs_ctx *ctx, uint32_t n_axes, const uint32_t *axis, const double *phase);
+
+/* configureSteadyState({probe_idx, periodSamples, threshold})
+ * (src/webgpu-engine.mjs:2161-2217; CPU form src/webgpu-fdtd.mjs:1037-1095):
+ * probe_idx are full field indices.  While recording, every step first adds
+ * volt^2 of each probe to period 1 (current sample < period_samples) or
+ * period 2, and advances the current sample.  Recording starts off; the
+ * convergence threshold is applied by the host (checkConvergence). */
+int efs_cfg_steady_state(efs_ctx *ctx, const uint32_t *probe_idx, uint32_t n, uint32_t period_samples);
+/* the ssRecording / ssCurrentSample fields */
+int efs_ss_set(efs_ctx *ctx, int recording, uint32_t current_sample);
+/* the period energies (n floats each; either may be NULL) and the fields */
+int efs_ss_read(efs_ctx *ctx, float *period1, float *period2, uint32_t *current_sample, int *recording);
+
 /* iterate(numSteps) (src/webgpu-engine.mjs:2407-2504): enqueue n complete
  * timesteps; does not wait. */
 int efs_iterate(efs_ctx *ctx, uint32_t n);

@@ -119,6 +119,21 @@ typedef struct {
     uint32_t g_n, g_max, g_interval, g_count;
     uint32_t *g_idx;
     real *g_ring;
+
+    /* steady-state detection (CPU class, src/webgpu-fdtd.mjs:1037-1095) */
+    uint32_t ss_n, ss_period, ss_current;
+    int ss_recording;
+    double ss_threshold;
+    uint32_t *ss_idx;
+    real *ss_e1, *ss_e2;
+
+    /* periodic boundaries (GPU engine only, src/webgpu-engine.mjs:475-584,
+     * 1976-2155): per axis the Bloch phase as the f32 cos/sin the engine
+     * uploads, and 4 face-sized imaginary-part buffers when sin != 0 */
+    uint32_t pbc_n;
+    uint32_t pbc_axis[3];
+    float pbc_cos[3], pbc_sin[3];
+    real *pbc_imag[3];
 } orc;
 
 static void *xcalloc(size_t n, size_t sz) {
@@ -193,6 +208,8 @@ void PFX(destroy)(orc *e) {
     for (int k = 0; k < 6; ++k) free(e->nf_el[k]);
     free(e->nf_accE); free(e->nf_accH);
     free(e->g_idx); free(e->g_ring);
+    free(e->ss_idx); free(e->ss_e1); free(e->ss_e2);
+    for (uint32_t k = 0; k < e->pbc_n; ++k) free(e->pbc_imag[k]);
     free(e);
 }
 
@@ -832,12 +849,115 @@ static void gather_hook(orc *e) {
 }
 
 /* ------------------------------------------------------------------ */
+/* steady-state detection  (src/webgpu-fdtd.mjs:1037-1095)             */
+/* ------------------------------------------------------------------ */
+
+void PFX(cfg_steady_state)(orc *e, const uint32_t *idx, uint32_t n, uint32_t period, double threshold) {
+    free(e->ss_idx); free(e->ss_e1); free(e->ss_e2);
+    e->ss_idx = dup_u32(idx, n);
+    e->ss_n = n; e->ss_period = period; e->ss_threshold = threshold > 0 ? threshold : 1e-6;
+    e->ss_current = 0; e->ss_recording = 0;
+    e->ss_e1 = (real *)xcalloc(n, sizeof(real));
+    e->ss_e2 = (real *)xcalloc(n, sizeof(real));
+}
+
+void PFX(ss_set)(orc *e, int recording, uint32_t current) { e->ss_recording = recording; e->ss_current = current; }
+
+uint32_t PFX(ss_read)(const orc *e, double *e1, double *e2) {
+    for (uint32_t n = 0; n < e->ss_n; ++n) {
+        if (e1) e1[n] = e->ss_e1[n];
+        if (e2) e2[n] = e->ss_e2[n];
+    }
+    return e->ss_current;
+}
+
+void PFX(ss_accumulate)(orc *e) {
+    if (!e->ss_n || !e->ss_recording) return;
+    for (uint32_t n = 0; n < e->ss_n; ++n) {
+        const double v = e->volt[e->ss_idx[n]];
+        const double en = v * v;
+        if (e->ss_current < e->ss_period) e->ss_e1[n] = ST(e->ss_e1[n] + en);
+        else e->ss_e2[n] = ST(e->ss_e2[n] + en);
+    }
+    e->ss_current++;
+}
+
+/* checkConvergence: both periods recorded and max |e2-e1|/|e1| < threshold */
+int PFX(ss_converged)(const orc *e) {
+    if (!e->ss_n && !e->ss_period) return 0;
+    if (e->ss_current < 2 * e->ss_period) return 0;
+    double mx = 0;
+    for (uint32_t n = 0; n < e->ss_n; ++n) {
+        const double e1 = e->ss_e1[n], e2 = e->ss_e2[n];
+        if (fabs(e1) < 1e-30) continue;
+        const double r = fabs(e2 - e1) / fabs(e1);
+        if (r > mx) mx = r;
+    }
+    return mx < e->ss_threshold;
+}
+
+/* ------------------------------------------------------------------ */
+/* periodic boundaries  (src/webgpu-engine.mjs:475-584, 1976-2155)      */
+/* voltage: tangential E of face N-2 -> face 0; current: tangential H  */
+/* of face 0 -> face N-2 over (face-1) x (face-1); Bloch phase rotates */
+/* (field, imag) by the phase with the imaginary part kept per face.   */
+/* ------------------------------------------------------------------ */
+
+void PFX(cfg_pbc)(orc *e, uint32_t n, const uint32_t *axis, const double *phase) {
+    for (uint32_t k = 0; k < e->pbc_n; ++k) { free(e->pbc_imag[k]); e->pbc_imag[k] = NULL; }
+    e->pbc_n = n > 3 ? 3 : n;
+    const uint32_t dims[3] = {e->nx, e->ny, e->nz};
+    for (uint32_t k = 0; k < e->pbc_n; ++k) {
+        e->pbc_axis[k] = axis[k];
+        e->pbc_cos[k] = (float)cos(phase[k]);
+        e->pbc_sin[k] = (float)sin(phase[k]);
+        const size_t F = (size_t)dims[(axis[k] + 1) % 3] * dims[(axis[k] + 2) % 3];
+        e->pbc_imag[k] = e->pbc_sin[k] != 0.0f ? (real *)xcalloc(4 * F, sizeof(real)) : NULL;
+    }
+}
+
+static void pbc_apply(orc *e, int current) {
+    const uint32_t dims[3] = {e->nx, e->ny, e->nz};
+    real *fld = current ? e->curr : e->volt;
+    for (uint32_t k = 0; k < e->pbc_n; ++k) {
+        const uint32_t ax = e->pbc_axis[k], t0 = (ax + 1) % 3, t1 = (ax + 2) % 3;
+        const uint32_t f0 = dims[t0], f1 = dims[t1];
+        const uint32_t src = current ? 0 : dims[ax] - 2, dst = current ? dims[ax] - 2 : 0;
+        const uint32_t b0 = current ? f0 - 1 : f0, b1 = current ? f1 - 1 : f1;
+        const double c = e->pbc_cos[k], sn = e->pbc_sin[k];
+        real *im = e->pbc_imag[k] ? e->pbc_imag[k] + (current ? 2 : 0) * (size_t)f0 * f1 : NULL;
+        for (uint32_t i0 = 0; i0 < b0; ++i0)
+            for (uint32_t i1 = 0; i1 < b1; ++i1) {
+                uint32_t ps[3], pd[3];
+                ps[ax] = src; pd[ax] = dst;
+                ps[t0] = pd[t0] = i0;
+                ps[t1] = pd[t1] = i1;
+                const size_t fl = (size_t)i0 * f1 + i1;
+                for (int q = 0; q < 2; ++q) {
+                    const uint32_t comp = q ? t1 : t0;
+                    const size_t si = IDX(e, comp, ps[0], ps[1], ps[2]), di = IDX(e, comp, pd[0], pd[1], pd[2]);
+                    if (!im) {
+                        fld[di] = ST(c * fld[si]);
+                    } else {
+                        real *ip = im + (size_t)q * f0 * f1 + fl;
+                        const double re = fld[si], ii = *ip;
+                        fld[di] = ST(c * re - sn * ii);
+                        *ip = ST(sn * re + c * ii);
+                    }
+                }
+            }
+    }
+}
+
+void PFX(pbc_voltage)(orc *e) { pbc_apply(e, 0); }
+void PFX(pbc_current)(orc *e) { pbc_apply(e, 1); }
+
+/* ------------------------------------------------------------------ */
 /* one complete step  (src/webgpu-fdtd.mjs:1094-1172 order)             */
 /* ------------------------------------------------------------------ */
 
 void PFX(step)(orc *e) {
-    /* pre-voltage: steady-state accumulation never records in the CPU
-     * class (its recording flag is never set), so it is omitted here. */
+    PFX(ss_accumulate)(e); /* pre-voltage, priority +2M */
     PFX(pml_step)(e, 0);
     PFX(update_volt_ade)(e);
     PFX(mur_pre)(e);
@@ -845,6 +965,7 @@ void PFX(step)(orc *e) {
     PFX(update_voltages)(e);
     PFX(pml_step)(e, 1);
     PFX(apply_tfsf_voltage)(e);
+    pbc_apply(e, 0); /* after TFSF-V, before Mur post (src/webgpu-engine.mjs:2461) */
     PFX(mur_post)(e);
     if (e->ade_apply) ade_apply(e, 0);
     PFX(apply_excitation)(e);
@@ -855,6 +976,7 @@ void PFX(step)(orc *e) {
     PFX(update_currents)(e);
     PFX(pml_step)(e, 3);
     PFX(apply_tfsf_current)(e);
+    pbc_apply(e, 1); /* after TFSF-I (src/webgpu-engine.mjs:2477) */
     if (e->ade_apply) ade_apply(e, 1);
     /* engine hooks after all field updates (src/webgpu-engine.mjs:2483-2500) */
     if (e->nf_P) PFX(nf2ff_accumulate)(e, e->numTS + 1);

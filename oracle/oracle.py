@@ -79,6 +79,14 @@ def _load():
         f("cfg_gather_buffered", None, vp, _u32p, C.c_uint32, C.c_uint32, C.c_uint32)
         f("read_gather_buffered", C.c_uint32, vp, _f64p)
         f("iterate", None, vp, C.c_uint32)
+        f("cfg_steady_state", None, vp, _u32p, C.c_uint32, C.c_uint32, C.c_double)
+        f("ss_set", None, vp, C.c_int, C.c_uint32)
+        f("ss_read", C.c_uint32, vp, _f64p, _f64p)
+        f("ss_accumulate", None, vp)
+        f("ss_converged", C.c_int, vp)
+        f("cfg_pbc", None, vp, C.c_uint32, _u32p, _f64p)
+        f("pbc_voltage", None, vp)
+        f("pbc_current", None, vp)
     _lib = lib
     return lib
 
@@ -270,6 +278,42 @@ class Oracle:
 
     def energy(self):
         return self._f("energy")(self.h)
+
+    # -- steady state (CPUFDTDEngine.configureSteadyState / accumulateEnergy /
+    #    checkConvergence, src/webgpu-fdtd.mjs:1037-1095) ---------------------
+    def configureSteadyState(self, cfg):
+        idx = _u32(cfg["probe_idx"])
+        self._ss_n = len(idx)
+        self._f("cfg_steady_state")(self.h, idx, len(idx), int(cfg["periodSamples"]),
+                                    float(cfg.get("threshold") or 1e-6))
+
+    def ssSet(self, recording, currentSample=None):
+        if currentSample is None:
+            currentSample = self.ssRead()["currentSample"]
+        self._f("ss_set")(self.h, 1 if recording else 0, int(currentSample))
+
+    def ssRead(self):
+        e1, e2 = np.zeros(max(self._ss_n, 1)), np.zeros(max(self._ss_n, 1))
+        cur = self._f("ss_read")(self.h, e1, e2)
+        return {"energy_period1": e1[: self._ss_n], "energy_period2": e2[: self._ss_n], "currentSample": cur}
+
+    def accumulateEnergy(self):
+        self._f("ss_accumulate")(self.h)
+
+    def checkConvergence(self):
+        return bool(self._f("ss_converged")(self.h))
+
+    # -- periodic boundaries (WebGPUEngine.configurePBC, src/webgpu-engine.mjs:1976-2155)
+    def configurePBC(self, cfg):
+        axes = cfg.get("axes") or []
+        self._f("cfg_pbc")(self.h, len(axes), _u32([a["axis"] for a in axes] or [0]),
+                           _f64([a.get("phase", 0) or 0 for a in axes] or [0]))
+
+    def pbcVoltage(self):
+        self._f("pbc_voltage")(self.h)
+
+    def pbcCurrent(self):
+        self._f("pbc_current")(self.h)
 
     # -- stepping ----------------------------------------------------------
     def __getattr__(self, name):

@@ -66,7 +66,7 @@ def lin(dims, x, y, z):
 
 
 def random_case(seed, dims, pml_width=0, exc=True, mur=False, ade=False, tfsf=False, rlc=False, fields=True,
-                quantize=0, pml_quantize=0):
+                quantize=0, pml_quantize=0, pbc=None):
     rng = np.random.default_rng(seed)
     nx, ny, nz = dims
     N = nx * ny * nz
@@ -116,6 +116,8 @@ def random_case(seed, dims, pml_width=0, exc=True, mur=False, ade=False, tfsf=Fa
             {"field_idx": int(rng.integers(0, N)), "direction": 1, "type_flag": 1, "i2v": 0, "ilv": 0,
              "vvd": 0.5 + 0.4 * r(), "vv2": 0.2 * r(), "vj1": 0.1 * r(), "vj2": 0.1 * r(), "ib0": 0.3 * r(),
              "b1": 0.2 * r(), "b2": 0.1 * r()}]}
+    if pbc:  # [(axis, phase), ...]
+        case["pbc"] = {"axes": [{"axis": a, "phase": p} for a, p in pbc]}
     return case
 
 
@@ -138,6 +140,8 @@ def configure(eng, case, ade_apply=False):
         eng.configureTFSF(case["tfsf"])
     if "rlc" in case:
         eng.configureRLC(case["rlc"])
+    if "pbc" in case:
+        eng.configurePBC(case["pbc"])
 
 
 def golden_cases():

@@ -5,7 +5,8 @@ import { WebGPUEngine, version, deviceCount } from '../../efs_amd/js/webgpu-engi
 const methods = ['initGPU', 'init', 'configureExcitation', 'configurePML', 'configureLorentzADE', 'configureTFSF',
     'configureRLC', 'configureMur', 'iterate', 'getFields', 'uploadFields', 'readProbe', 'configureProbeGather',
     'readProbeGather', 'configureProbeGatherBuffered', 'readProbeGatherBuffered', 'configureNF2FFAccumulation',
-    'readNF2FFAccumulation', 'computeNF2FFfarField', 'computeEnergy', 'flushEnergy', 'destroy'];
+    'readNF2FFAccumulation', 'computeNF2FFfarField', 'computeEnergy', 'flushEnergy', 'destroy',
+    'configurePBC', 'configureSteadyState', 'readSteadyState', 'checkConvergence'];
 const eng = new WebGPUEngine();
 const missing = methods.filter((m) => typeof eng[m] !== 'function');
 eng.initGPU().then((ok) => {

@@ -218,6 +218,61 @@ def test_compressed_upml_coefficients(HIPEngine, oracle_mod, monkeypatch, name, 
     assert_fields_close(fa, o.getFields(), what=name)
 
 
+PBC_CASES = [
+    ("pbc_x", 61, (12, 10, 16), dict(pbc=[(0, 0.0)])),
+    ("pbc_bloch_yz", 62, (10, 12, 16), dict(pbc=[(1, 0.6), (2, -1.1)], tfsf=True)),
+    ("pbc_x_pml_mur", 63, (14, 10, 12), dict(pbc=[(0, 0.3)], pml_width=2, mur=True, ade=True)),
+    ("pbc_xyz_vec1", 64, (9, 8, 7), dict(pbc=[(2, 0.0), (0, 2.5), (1, 0.0)], rlc=True)),
+]
+
+
+@pytest.mark.parametrize("name,seed,dims,kw", PBC_CASES, ids=[c[0] for c in PBC_CASES])
+def test_periodic_boundaries(HIPEngine, oracle_mod, name, seed, dims, kw):
+    """configurePBC (plain and Bloch, several axes) against the oracle."""
+    case = random_case(seed, dims, **kw)
+    say("case", name)
+    o = make_oracle(oracle_mod, case)
+    g = make_gpu(HIPEngine, case)
+    o.iterate(40)
+    g.iterate(40)
+    assert_fields_close(g.getFields(), o.getFields(), what=name)
+
+
+def test_steady_state(HIPEngine, oracle_mod):
+    """configureSteadyState: recording window, period split, counter and
+    convergence rule against the oracle."""
+    case = random_case(65, (12, 11, 16), pml_width=2)
+    N = 12 * 11 * 16
+    probes = np.array([5, N + 700, 2 * N + 1333, 900], np.uint32)
+    say("case steady state")
+    o = make_oracle(oracle_mod, case)
+    g = make_gpu(HIPEngine, case)
+    cfg = {"probe_idx": probes, "periodSamples": 7, "threshold": 0.5}
+    o.configureSteadyState(cfg)
+    g.configureSteadyState(cfg)
+    for e in (o, g):
+        e.iterate(5)            # not recording: nothing accumulates
+    assert g.readSteadyState()["currentSample"] == 0
+    o.ssSet(True, 0)
+    g.ssRecording = True
+    g.ssCurrentSample = 0
+    for e in (o, g):
+        e.iterate(10)
+    mid_o, mid_g = o.ssRead(), g.readSteadyState()
+    assert mid_g["currentSample"] == mid_o["currentSample"] == 10
+    assert not g.checkConvergence() and not o.checkConvergence()  # period 2 not complete
+    for e in (o, g):
+        e.iterate(6)
+    so, sg = o.ssRead(), g.readSteadyState()
+    assert sg["currentSample"] == so["currentSample"] == 16
+    for k in ("energy_period1", "energy_period2"):
+        np.testing.assert_allclose(sg[k], so[k], rtol=1e-5, atol=1e-30)
+    assert g.checkConvergence() == o.checkConvergence()
+    g.ssRecording = False
+    g.iterate(3)
+    assert g.readSteadyState()["currentSample"] == 16
+
+
 def test_ade_write_back(HIPEngine, oracle_mod):
     """Optional openEMS ADE write-back, same switch on both sides."""
     case = random_case(11, (14, 12, 10), pml_width=2, ade=True)
