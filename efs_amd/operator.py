@@ -11,8 +11,10 @@ Gedney's UPML, in the array layouts those getters produce:
   E edge (n; i,j,k): C = eps0*<eps_r>*A/dl, G = <kappa>*A/dl with A the
       dual face and <.> the area-weighted mean of the (up to 4) touching
       cells; vv = (1-g)/(1+g), vi = (dT/C)/(1+g), g = dT*G/(2C).
-  H edge (n; dual): L = mu0*<mu_r>*dl'/A' (series mean over the 2 cells on
-      the dual edge); ii = 1, iv = dT/L.
+  H edge (n; dual, length dl' crossing the primal face A'): the flux
+      through A' is continuous along the dual edge, so the edge inductance
+      is L = mu0*A' / sum_i(w_i/mu_r,i) over the (up to 2) cells the edge
+      crosses (w_i its length in cell i); ii = 1, iv = dT/L.
   PEC edges and edges leaving the grid: vv = vi = 0.
   UPML (component n, nP=(n+1)%3, nPP=(n+2)%3, s_m = sigma_m*dT/(2*eps0)):
       flux   F' = vv_c F + vi_c curl,  vv_c = (1-s_nP)/(1+s_nP), vi_c = vi/(1+s_nP)
@@ -147,19 +149,23 @@ def build_coefficients(mesh, mat, dT):
         ok = (dl > 0) & (eA > 0) & ~mat.pec[n]
         vv[n] = np.where(ok, v1, 0.0)
         vi[n] = np.where(ok, v2, 0.0)
-        # --- H edges: series mean of mu over the 2 cells along the dual edge
-        lA = np.zeros(N)
+        # --- H edges: reluctance of the dual edge, sum of w_i / mu_i over the
+        # cells it crosses (series), L = mu0 * A' / that sum
+        rel = np.zeros(N)
         for s, w in ((-1, mesh.half_lo[n]), (0, mesh.half_hi[n])):
             sl = [None, None, None]
             sl[n] = slice(1 + s, 1 + s + N[n])
             sl[nP] = slice(1, N[nP] + 1)
             sl[nPP] = slice(1, N[nPP] + 1)
-            lA += muP[tuple(sl)] * _bcast(w, n)
+            mu_c = muP[tuple(sl)]
+            wb = _bcast(w, n)
+            with np.errstate(divide="ignore", invalid="ignore"):
+                rel += np.where((wb > 0) & (mu_c > 0), wb / mu_c, 0.0)
         area = _bcast(mesh.primal[nP], nP) * _bcast(mesh.primal[nPP], nPP)
         with np.errstate(divide="ignore", invalid="ignore"):
-            L = MUE0 * lA / area
+            L = MUE0 * area / rel
             v3 = dT / L
-        okh = (area > 0) & (lA > 0)
+        okh = (area > 0) & (rel > 0)
         ii[n] = np.where(okh, 1.0, 0.0)
         iv[n] = np.where(okh, v3, 0.0)
     return {"vv": vv, "vi": vi, "ii": ii, "iv": iv}
@@ -263,11 +269,14 @@ def pec_boundary(mat, faces=range(6)):
 
 def gaussian_pulse(f0, fc, dT):
     """openEMS-style Gaussian: cos(2 pi f0 (t - T0)) exp(-((t - T0) 2 pi fc / 3)^2),
-    T0 = 9 / (2 pi fc), sampled at t = n dT over 2 T0."""
+    T0 = 9 / (2 pi fc), over 2 T0.  Sample 0 is zero and sample k >= 1 is the
+    pulse at t = (k-1) dT: the engine reads sample 0 once the pulse has
+    ended (excitation position past the end), so that must inject nothing."""
     T0 = 9.0 / (2 * np.pi * fc)
     n = int(2 * T0 / dT)
     t = np.arange(n) * dT
-    return (np.cos(2 * np.pi * f0 * (t - T0)) * np.exp(-((t - T0) * 2 * np.pi * fc / 3) ** 2)).astype(np.float32)
+    pulse = np.cos(2 * np.pi * f0 * (t - T0)) * np.exp(-((t - T0) * 2 * np.pi * fc / 3) ** 2)
+    return np.concatenate([[0.0], pulse]).astype(np.float32)
 
 
 def mur_config(mesh, faces, dT):
