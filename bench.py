@@ -25,12 +25,14 @@ METRIC = "MCells/s & %HBM roofline: plain FP32 vs batched/mixed at fixed S11 err
 HBM_PEAK_GBS = 8000.0
 
 
-def sweep_bytes(n_cells, n_pml):
+def sweep_bytes(n_cells, n_pml, storage="fp32"):
     """Algorithmic HBM bytes of one E (or H) sweep launch.
-    Per cell: read V, vv, vi, H (3 components each) and write V = 15 floats
-    = 60 B.  Per UPML cell in addition: flux read + write and three UPML
-    coefficients for 3 components = 60 B."""
-    return 60.0 * n_cells + 60.0 * n_pml
+    Per cell: read V, vv, vi, H (3 components each) and write V = 15 values:
+    60 B with FP32 fields, 42 B with FP16 fields (9 field values x 2 B + 6
+    FP32 coefficients).  Per UPML cell in addition: flux read + write and
+    three UPML coefficients for 3 components = 60 B."""
+    per_cell = 42.0 if storage == "fp16" else 60.0
+    return per_cell * n_cells + 60.0 * n_pml
 
 
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "fused_sweeps_256cube_stress_traffic.json")
@@ -47,7 +49,7 @@ def sweep_kernel_symbol(flags, side):
         suffix = "mc" if "coef_idx" in flags else "m"
     else:
         suffix = "c" if "coef_idx" in flags else ""
-    return f"efs::sweep_{side}4{suffix}"
+    return f"efs::sweep_{side}4{suffix}" + ("_h" if "f16" in flags else "")
 
 
 def measured_traffic(kernel, size):
@@ -58,6 +60,40 @@ def measured_traffic(kernel, size):
     with open(TRAFFIC_FILE) as f:
         t = json.load(f)
     return t.get(kernel, {}).get("hbm_bytes_per_launch")
+
+
+def precision_sample(storage, n=48, steps=150):
+    """Error of the GPU engine against the FP64 oracle on a bounded sample of
+    the same workload (n^3 stress grid, same builder and extensions)."""
+    from efs_amd import workloads
+    from efs_amd.engine import HIPEngine
+    from oracle import oracle as orc
+    w = workloads.stress_grid(n, pml=8, nfreq=3, n_boxes=6, n_plates=2)
+    o = orc.Oracle(w["gridSize"], w["coefficients"], fp64=True)
+    o.configurePML(w["pmlRegions"])
+    o.configureExcitation(w["excitation"])
+    nf = w["nf2ff"]
+    o.configureNF2FF(nf["surfaceIndices"], nf["omega"], nf["dT"], nf["primalEdgeLens"], nf["dualEdgeLens"])
+    o.iterate(steps)
+    ref = o.getFields()
+    refE, refH = o.readNF2FF()
+    g = HIPEngine()
+    g.initGPU()
+    workloads.configure_engine(g, w, storage=storage)
+    g.iterate(steps)
+    f = g.getFields()
+    acc = g.readNF2FFAccumulation()
+
+    def rl2(a, b):
+        a, b = np.asarray(a, np.float64), np.asarray(b, np.float64)
+        return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300))
+    out = {"against": "FP64 oracle (C restatement, double storage)", "grid": [n, n, n], "steps": steps,
+           "volt_rel_l2": rl2(f["volt"], ref["volt"]), "curr_rel_l2": rl2(f["curr"], ref["curr"]),
+           "energy_rel": abs(g.energy() - o.energy()) / o.energy(),
+           "nf2ff_rel_l2": max(rl2(np.concatenate([acc["accumE"][k], acc["accumH"][k]]),
+                                   np.concatenate([refE[k], refH[k]])) for k in range(len(refE)))}
+    g.destroy()
+    return out
 
 
 def cpu_baseline(w, seconds_target=15.0):
@@ -102,6 +138,9 @@ def main():
     ap.add_argument("--profile-steps", type=int, default=200)
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--storage", choices=["fp32", "fp16"], default="fp32",
+                    help="field storage: fp32 (plain engine) or fp16 (per-component power-of-two scales, "
+                         "FP32 updates)")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="run one elementwise copy of 1 GiB -> 1 GiB first (reference kernel for HBM counters)")
     args = ap.parse_args()
@@ -130,7 +169,7 @@ def main():
     eng = HIPEngine()
     if not eng.initGPU():
         raise RuntimeError("no HIP device")
-    workloads.configure_engine(eng, w)
+    workloads.configure_engine(eng, w, storage=args.storage)
 
     eng.iterate(args.warmup)
     eng.energy()
@@ -159,7 +198,7 @@ def main():
     ms_s, cs = eng.profileRead(PROF_STEP)
     eng.profileEnable(False)
     avg_e, avg_h, avg_s = ms_e / ce, ms_h / ch, ms_s / cs
-    b = sweep_bytes(n, w["n_pml_cells"])
+    b = sweep_bytes(n, w["n_pml_cells"], args.storage)
     gbs_e = b / (avg_e * 1e-3) / 1e9
     gbs_h = b / (avg_h * 1e-3) / 1e9
     dom_name, dom_ms = ("sweep_h", avg_h) if avg_h >= avg_e else ("sweep_e", avg_e)
@@ -169,6 +208,28 @@ def main():
     acc = eng.readNF2FFAccumulation()
     finite = bool(np.all(np.isfinite(acc["accumE"])) and np.all(np.isfinite(energies)))
 
+    extra = {}
+    if args.storage == "fp16":
+        # the same steps on the plain FP32 engine, and the error against FP64
+        eng.destroy()
+        e32 = HIPEngine()
+        e32.initGPU()
+        workloads.configure_engine(e32, w)
+        e32.iterate(args.warmup)
+        e32.sync()
+        t1 = time.perf_counter()
+        done = 0
+        while done < K:
+            k = min(E, K - done)
+            e32.iterate(k)
+            done += k
+            if done % E == 0:
+                e32.energy()
+        e32.sync()
+        v32 = n * K / (time.perf_counter() - t1) / 1e6
+        e32.destroy()
+        extra = {"fp32_value": round(v32, 1), "speedup_vs_fp32": round(mcells / v32, 3),
+                 "precision": precision_sample("fp16")}
     base = None if args.no_cpu_baseline else cpu_baseline(w, args.cpu_seconds)
     step_bytes = 2 * b
     result = {
@@ -182,7 +243,7 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f32",
+        "dtype": "f32" if args.storage == "fp32" else "f16 storage, f32 compute",
         "data": "synthetic",
         "config": {
             "workload": f"plain FP32 stress grid {args.size}^3, 1 mm cells, UPML-8 on 6 faces, 40 dielectric boxes, "
@@ -208,6 +269,7 @@ def main():
                                  "sweeps_GB/s": round(step_bytes / (avg_s * 1e-3) / 1e9, 1)}},
         },
         "cpu_baseline": base,
+        **extra,
         "checks": {"finite": finite, "energy_last": energies[-1] if energies else None,
                    "host_build_s": round(t_build, 1)},
     }

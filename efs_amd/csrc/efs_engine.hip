@@ -87,6 +87,19 @@ struct efs_ctx {
     bool pml_split = false;  // 4-wide sweeps with UPML (region z-bounds multiples of 4)
     bool pml_fused = true;   // one grid launch per sweep (else interior + region launches)
     bool pml_comp = false;   // UPML coefficients also as byte indices + tables (fused sweeps)
+    // field storage: FP32, or FP16 with per-component power-of-two scales
+    // (value = stored * fscale[k], k = volt 0..2, curr 0..2) kept on the
+    // device and adjusted every range_interval steps by the range kernels
+    int storage_req = EFS_STORAGE_F32;
+    bool half = false;
+    float* fscale = nullptr;        // [6]
+    unsigned int* f_amax = nullptr; // [6] running max |stored| (float bits)
+    float* f_factor = nullptr;      // [6] last rescale factors
+    float* f_smin = nullptr;        // [6] finest scale the range control may choose
+    uint32_t range_interval = 32;
+    bool fields_zero = true;        // no upload and no step since init: scales are free to choose
+    uint64_t source_end = 0;        // first step index at which no excitation / TFSF sample is injected
+    bool source_periodic = false;
     uint8_t* pml_idx = nullptr;   // per region: E then H component indices (PmlRegion::pidx)
     float4* pml_tab_e = nullptr;  // [3][256] (vv, vvfo, vvfn, 0)
     float4* pml_tab_h = nullptr;  // [3][256] (ii, iifo, iifn, 0)
@@ -469,6 +482,8 @@ void free_all(efs_ctx* c) {
     dfree(c->exc_signal); dfree(c->exc_amp); dfree(c->exc_delay); dfree(c->exc_gidx);
     dfree(c->pml_coef); dfree(c->pml_vflux); dfree(c->pml_cflux);
     dfree(c->pml_idx); dfree(c->pml_tab_e); dfree(c->pml_tab_h);
+    dfree(c->fscale); dfree(c->f_amax); dfree(c->f_factor); dfree(c->f_smin);
+    c->half = false;
     dfree(c->mur_coeff); dfree(c->mur_saved); dfree(c->mur_normal); dfree(c->mur_shifted);
     dfree(c->mur_normal_src); dfree(c->mur_shifted_src);
     dfree(c->ade_gidx); dfree(c->ade_src_v); dfree(c->ade_src_i); dfree(c->ade_lor);
@@ -504,6 +519,8 @@ void reset_extensions(efs_ctx* c) {
     c->nf_slot = 0;
     c->pbc_e.clear();
     c->pbc_h.clear();
+    c->source_end = 0;
+    c->source_periodic = false;
     c->ss_n = 0;
     c->ss_current = 0;
     c->ss_recording = false;
@@ -606,23 +623,42 @@ int plan_sweeps(efs_ctx* c) {
     return EFS_OK;
 }
 
+// A field array as the point kernels see it (storage type and scales).
+FieldView fv(const efs_ctx* c, int curr) {
+    return FieldView{curr ? (void*)c->curr : (void*)c->volt, c->fscale + 3 * curr, (uint64_t)c->S,
+                     c->half ? 1u : 0u};
+}
+
+// FP16 storage: largest |stored| per component, then rescale where needed.
+void launch_range_control(efs_ctx* c, uint32_t t) {
+    const uint32_t N = (uint32_t)c->N;
+    const dim3 grid(std::max(1u, std::min(1024u, (N / 4 + 255) / 256)), 6);
+    hipLaunchKernelGGL(k_absmax16, grid, dim3(256), 0, c->stream, (const f16*)c->volt, (const f16*)c->curr,
+                       (uint64_t)c->S, N, c->f_amax);
+    const uint32_t active = c->source_periodic || (uint64_t)t + 1 < c->source_end;
+    hipLaunchKernelGGL(k_range_plan, dim3(1), dim3(64), 0, c->stream, c->f_amax, c->fscale, c->f_smin, c->f_factor,
+                       active);
+    hipLaunchKernelGGL(k_range_apply, grid, dim3(256), 0, c->stream, (f16*)c->volt, (f16*)c->curr, (uint64_t)c->S,
+                       N, c->f_factor);
+}
+
 int launch_step(efs_ctx* c) {
     const uint32_t t = c->numTS;
     ProfScope step_scope(c, EFS_PROF_STEP);
     const bool has_pml = c->pml.n > 0;
     // pre-voltage: steady-state energies first (priority +2M)
     if (c->ss_n && c->ss_recording) {
-        hipLaunchKernelGGL(k_ss_accumulate, dim3(nblk(c->ss_n)), dim3(kPointThreads), 0, c->stream, c->volt,
+        hipLaunchKernelGGL(k_ss_accumulate, dim3(nblk(c->ss_n)), dim3(kPointThreads), 0, c->stream, fv(c, 0),
                            c->ss_gidx, c->ss_n, c->ss_period, c->ss_current, c->ss_e, c->ss_e + c->ss_n);
         c->ss_current++;
     }
     if (c->ade_n) {
-        hipLaunchKernelGGL(k_ade_update, dim3(nblk(c->ade_n)), dim3(kPointThreads), 0, c->stream, c->volt,
+        hipLaunchKernelGGL(k_ade_update, dim3(nblk(c->ade_n)), dim3(kPointThreads), 0, c->stream, fv(c, 0),
                            c->pml_vflux, c->ade_src_v, c->ade_lor, c->ade_c[0], c->ade_c[1], c->ade_c[2],
                            c->ade_va, c->ade_vl, c->ade_n);
     }
     if (c->mur_n) {
-        hipLaunchKernelGGL(k_mur_pre, dim3(nblk(c->mur_n)), dim3(kPointThreads), 0, c->stream, c->volt,
+        hipLaunchKernelGGL(k_mur_pre, dim3(nblk(c->mur_n)), dim3(kPointThreads), 0, c->stream, fv(c, 0),
                            c->pml_vflux, c->mur_coeff, c->mur_normal_src, c->mur_shifted_src, c->mur_saved,
                            c->mur_n);
     }
@@ -637,11 +673,20 @@ int launch_step(efs_ctx* c) {
             auto kern = has_pml ? (c->comp ? sweep_e4mc : sweep_e4m) : (c->comp ? sweep_e4c : sweep_e4);
             if (has_pml && c->pml_fused)
                 kern = c->pml_comp ? (c->comp ? sweep_e4fcp : sweep_e4fp) : (c->comp ? sweep_e4fc : sweep_e4f);
-            hipLaunchKernelGGL(kern, dim3(c->geom.blocks), dim3(kSweepThreads), 0, c->stream, c->volt, c->curr, cf,
-                               c->geom, c->pml, c->pml_coef, c->pml_vflux);
+            if (c->half) {
+                auto k16 = has_pml ? (c->pml_comp ? (c->comp ? sweep_e4fcp_h : sweep_e4fp_h)
+                                                  : (c->comp ? sweep_e4fc_h : sweep_e4f_h))
+                                   : (c->comp ? sweep_e4c_h : sweep_e4_h);
+                hipLaunchKernelGGL(k16, dim3(c->geom.blocks), dim3(kSweepThreads), 0, c->stream, (f16*)c->volt,
+                                   (const f16*)c->curr, cf, c->geom, c->pml, c->pml_coef, c->pml_vflux, c->fscale);
+            } else {
+                hipLaunchKernelGGL(kern, dim3(c->geom.blocks), dim3(kSweepThreads), 0, c->stream, c->volt, c->curr,
+                                   cf, c->geom, c->pml, c->pml_coef, c->pml_vflux, c->fscale);
+            }
             if (has_pml && !c->pml_fused)
                 hipLaunchKernelGGL(c->comp ? sweep_e4uc : sweep_e4u, dim3(c->pml.blocks), dim3(kSweepThreads), 0,
-                                   c->stream, c->volt, c->curr, cf, c->geom, c->pml, c->pml_coef, c->pml_vflux);
+                                   c->stream, c->volt, c->curr, cf, c->geom, c->pml, c->pml_coef, c->pml_vflux,
+                                   c->fscale);
         } else {
             auto kern = has_pml ? sweep_e1<true> : sweep_e1<false>;
             hipLaunchKernelGGL(kern, dim3(c->geom1.blocks), dim3(kSweepThreads), 0, c->stream, c->volt, c->curr,
@@ -650,37 +695,37 @@ int launch_step(efs_ctx* c) {
     }
     // post-voltage / apply-to-voltages
     if (c->tfsf_nv) {
-        hipLaunchKernelGGL(k_tfsf, dim3(nblk(c->tfsf_nv)), dim3(kPointThreads), 0, c->stream, c->volt,
+        hipLaunchKernelGGL(k_tfsf, dim3(nblk(c->tfsf_nv)), dim3(kPointThreads), 0, c->stream, fv(c, 0),
                            c->tfsf_signal, c->tfsf_len, c->tfsf_period, c->tv_idx, c->tv_dint, c->tv_dfrac,
                            c->tv_amp, c->tfsf_nv, t);
     }
     for (const PbcAxis& a : c->pbc_e)  // after TFSF-V, before Mur post (src/webgpu-engine.mjs:2461)
-        hipLaunchKernelGGL(k_pbc, dim3(nblk((size_t)a.b0 * a.b1)), dim3(kPointThreads), 0, c->stream, c->volt,
+        hipLaunchKernelGGL(k_pbc, dim3(nblk((size_t)a.b0 * a.b1)), dim3(kPointThreads), 0, c->stream, fv(c, 0),
                            c->pbc_imag, a, c->ny, c->nz, (uint64_t)c->S);
     if (c->mur_n) {
-        hipLaunchKernelGGL(k_mur_post, dim3(nblk(c->mur_n)), dim3(kPointThreads), 0, c->stream, c->volt,
+        hipLaunchKernelGGL(k_mur_post, dim3(nblk(c->mur_n)), dim3(kPointThreads), 0, c->stream, fv(c, 0),
                            c->mur_coeff, c->mur_shifted, c->mur_saved, c->mur_n);
     }
     if (c->ade_n && c->ade_apply) {
-        hipLaunchKernelGGL(k_ade_apply, dim3(nblk(c->ade_n)), dim3(kPointThreads), 0, c->stream, c->volt,
+        hipLaunchKernelGGL(k_ade_apply, dim3(nblk(c->ade_n)), dim3(kPointThreads), 0, c->stream, fv(c, 0),
                            c->ade_gidx, c->ade_va, c->ade_n);
     }
     if (c->exc_n) {
-        hipLaunchKernelGGL(k_excitation, dim3(nblk(c->exc_n)), dim3(kPointThreads), 0, c->stream, c->volt,
+        hipLaunchKernelGGL(k_excitation, dim3(nblk(c->exc_n)), dim3(kPointThreads), 0, c->stream, fv(c, 0),
                            c->exc_signal, c->exc_len, c->exc_period, c->exc_delay, c->exc_amp, c->exc_gidx,
                            c->exc_n, t);
     }
     if (c->mur_n) {
-        hipLaunchKernelGGL(k_mur_apply, dim3(nblk(c->mur_n)), dim3(kPointThreads), 0, c->stream, c->volt,
+        hipLaunchKernelGGL(k_mur_apply, dim3(nblk(c->mur_n)), dim3(kPointThreads), 0, c->stream, fv(c, 0),
                            c->mur_normal, c->mur_saved, c->mur_n);
     }
     if (c->rlc_n) {
-        hipLaunchKernelGGL(k_rlc_apply, dim3(nblk(c->rlc_n)), dim3(kPointThreads), 0, c->stream, c->volt,
+        hipLaunchKernelGGL(k_rlc_apply, dim3(nblk(c->rlc_n)), dim3(kPointThreads), 0, c->stream, fv(c, 0),
                            c->rlc_gidx, c->rlc_type, c->rlc_c, c->rlc_vdn, c->rlc_jn, c->rlc_vil, c->rlc_n);
     }
     // pre-current
     if (c->ade_n) {
-        hipLaunchKernelGGL(k_ade_update, dim3(nblk(c->ade_n)), dim3(kPointThreads), 0, c->stream, c->curr,
+        hipLaunchKernelGGL(k_ade_update, dim3(nblk(c->ade_n)), dim3(kPointThreads), 0, c->stream, fv(c, 1),
                            c->pml_cflux, c->ade_src_i, c->ade_lor, c->ade_c[3], c->ade_c[4], c->ade_c[5],
                            c->ade_ia, c->ade_il, c->ade_n);
     }
@@ -691,11 +736,20 @@ int launch_step(efs_ctx* c) {
             auto kern = has_pml ? (c->comp ? sweep_h4mc : sweep_h4m) : (c->comp ? sweep_h4c : sweep_h4);
             if (has_pml && c->pml_fused)
                 kern = c->pml_comp ? (c->comp ? sweep_h4fcp : sweep_h4fp) : (c->comp ? sweep_h4fc : sweep_h4f);
-            hipLaunchKernelGGL(kern, dim3(c->geom.blocks), dim3(kSweepThreads), 0, c->stream, c->curr, c->volt, cf,
-                               c->geom, c->pml, c->pml_coef, c->pml_cflux);
+            if (c->half) {
+                auto k16 = has_pml ? (c->pml_comp ? (c->comp ? sweep_h4fcp_h : sweep_h4fp_h)
+                                                  : (c->comp ? sweep_h4fc_h : sweep_h4f_h))
+                                   : (c->comp ? sweep_h4c_h : sweep_h4_h);
+                hipLaunchKernelGGL(k16, dim3(c->geom.blocks), dim3(kSweepThreads), 0, c->stream, (f16*)c->curr,
+                                   (const f16*)c->volt, cf, c->geom, c->pml, c->pml_coef, c->pml_cflux, c->fscale);
+            } else {
+                hipLaunchKernelGGL(kern, dim3(c->geom.blocks), dim3(kSweepThreads), 0, c->stream, c->curr, c->volt,
+                                   cf, c->geom, c->pml, c->pml_coef, c->pml_cflux, c->fscale);
+            }
             if (has_pml && !c->pml_fused)
                 hipLaunchKernelGGL(c->comp ? sweep_h4uc : sweep_h4u, dim3(c->pml.blocks), dim3(kSweepThreads), 0,
-                                   c->stream, c->curr, c->volt, cf, c->geom, c->pml, c->pml_coef, c->pml_cflux);
+                                   c->stream, c->curr, c->volt, cf, c->geom, c->pml, c->pml_coef, c->pml_cflux,
+                                   c->fscale);
         } else {
             auto kern = has_pml ? sweep_h1<true> : sweep_h1<false>;
             hipLaunchKernelGGL(kern, dim3(c->geom1.blocks), dim3(kSweepThreads), 0, c->stream, c->curr, c->volt,
@@ -704,28 +758,29 @@ int launch_step(efs_ctx* c) {
     }
     // post-current
     if (c->tfsf_ni) {
-        hipLaunchKernelGGL(k_tfsf, dim3(nblk(c->tfsf_ni)), dim3(kPointThreads), 0, c->stream, c->curr,
+        hipLaunchKernelGGL(k_tfsf, dim3(nblk(c->tfsf_ni)), dim3(kPointThreads), 0, c->stream, fv(c, 1),
                            c->tfsf_signal, c->tfsf_len, c->tfsf_period, c->ti_idx, c->ti_dint, c->ti_dfrac,
                            c->ti_amp, c->tfsf_ni, t);
     }
     for (const PbcAxis& a : c->pbc_h)  // after TFSF-I (src/webgpu-engine.mjs:2477)
-        hipLaunchKernelGGL(k_pbc, dim3(nblk((size_t)a.b0 * a.b1)), dim3(kPointThreads), 0, c->stream, c->curr,
+        hipLaunchKernelGGL(k_pbc, dim3(nblk((size_t)a.b0 * a.b1)), dim3(kPointThreads), 0, c->stream, fv(c, 1),
                            c->pbc_imag, a, c->ny, c->nz, (uint64_t)c->S);
     if (c->ade_n && c->ade_apply) {
-        hipLaunchKernelGGL(k_ade_apply, dim3(nblk(c->ade_n)), dim3(kPointThreads), 0, c->stream, c->curr,
+        hipLaunchKernelGGL(k_ade_apply, dim3(nblk(c->ade_n)), dim3(kPointThreads), 0, c->stream, fv(c, 1),
                            c->ade_gidx, c->ade_ia, c->ade_n);
     }
     // engine hooks
     if (c->nf_P) {
         if (c->nf_slot == 0) c->nf_ts0 = t + 1;
-        hipLaunchKernelGGL(k_nf2ff_sample, dim3(nblk(c->nf_P)), dim3(kPointThreads), 0, c->stream, c->volt,
-                           c->curr, c->nf_pts, c->nf_el, c->nf_geom,
+        hipLaunchKernelGGL(k_nf2ff_sample, dim3(nblk(c->nf_P)), dim3(kPointThreads), 0, c->stream, fv(c, 0),
+                           fv(c, 1), c->nf_pts, c->nf_el, c->nf_geom,
                            c->nf_buf + (size_t)c->nf_slot * c->nf_rows);
         c->nf_slot++;
         if (c->nf_slot == c->nf_K) TRY(nf2ff_fold(c));
     }
+    if (c->half && (t + 1) % c->range_interval == 0) launch_range_control(c, t);
     if (c->gb_n && (t + 1) % c->gb_interval == 0 && c->gb_count < c->gb_max) {
-        hipLaunchKernelGGL(k_gather, dim3(nblk(c->gb_n)), dim3(kPointThreads), 0, c->stream, c->volt, c->curr,
+        hipLaunchKernelGGL(k_gather, dim3(nblk(c->gb_n)), dim3(kPointThreads), 0, c->stream, fv(c, 0), fv(c, 1),
                            c->gb_idx, c->gb_ring + (size_t)c->gb_count * 2 * c->gb_n, c->gb_n);
         c->gb_count++;
     }
@@ -824,6 +879,16 @@ int efs_init(efs_ctx* c, uint32_t nx, uint32_t ny, uint32_t nz, const float* vv,
         for (int n = 0; n < 3; ++n)
             HIP_TRY(c, hipMemcpy(dst[a] + n * c->S, src[a] + n * N, N * sizeof(float), hipMemcpyHostToDevice));
     c->vec4 = (nz % 4 == 0) && env_u64("EFS_VEC", 4) == 4;
+    c->half = c->storage_req == EFS_STORAGE_F16;
+    c->range_interval = (uint32_t)std::max<uint64_t>(1, env_u64("EFS_RANGE_INTERVAL", 32));
+    if (c->half && !c->vec4) return fail(c, EFS_ERR_ARG, "FP16 field storage needs Nz %% 4 == 0");
+    {
+        const float one[6] = {1.f, 1.f, 1.f, 1.f, 1.f, 1.f};
+        TRY(dupload(c, c->fscale, one, 6));
+        TRY(dzero(c, c->f_amax, 6));
+        TRY(dupload(c, c->f_factor, one, 6));
+        TRY(dupload(c, c->f_smin, one, 6));
+    }
     // coefficient compression (4-wide sweeps): both fields, all components
     c->comp = false;
     if (c->vec4 && env_u64("EFS_COMPRESS", 1) != 0) {
@@ -845,7 +910,36 @@ int efs_init(efs_ctx* c, uint32_t nx, uint32_t ny, uint32_t nz, const float* vv,
     TRY(dalloc(c, c->en_partial, kEnergyMaxBlocks + 1));
     TRY(plan_sweeps(c));
     c->numTS = 0;
+    c->fields_zero = true;
     c->inited = true;
+    return EFS_OK;
+}
+
+// FP16 storage from zero fields: start the scales where a source puts its
+// largest value near 2^kRangeTarget -- volt from the voltage amplitudes,
+// curr from the current amplitudes or, without those, 2^8 below volt
+// (about 1/Z0).  These are also the floors of the range control.
+int source_scales(efs_ctx* c, const float* signal, uint32_t len, const float* vamp, uint32_t nv, const float* iamp,
+                  uint32_t ni) {
+    if (!c->half || !c->fields_zero) return EFS_OK;
+    float smax = 0.f, va = 0.f, ia = 0.f;
+    for (uint32_t k = 0; k < len; ++k) smax = std::max(smax, std::fabs(signal[k]));
+    for (uint32_t k = 0; k < nv; ++k) va = std::max(va, std::fabs(vamp[k]));
+    for (uint32_t k = 0; k < ni; ++k) ia = std::max(ia, std::fabs(iamp[k]));
+    const float mv = smax * va, mi = smax * ia;
+    if (!(mv > 0.f && std::isfinite(mv)) && !(mi > 0.f && std::isfinite(mi))) return EFS_OK;
+    float sc[6];
+    HIP_TRY(c, hipMemcpy(sc, c->fscale, sizeof sc, hipMemcpyDeviceToHost));
+    const float vs = mv > 0.f && std::isfinite(mv) ? std::ldexp(1.0f, std::ilogb(mv) - kRangeTarget)
+                                                   : std::ldexp(1.0f, std::ilogb(mi) - kRangeTarget + 8);
+    const float is = mi > 0.f && std::isfinite(mi) ? std::ldexp(1.0f, std::ilogb(mi) - kRangeTarget)
+                                                   : std::ldexp(vs, -8);
+    for (int k = 0; k < 3; ++k) {
+        sc[k] = vs;
+        sc[3 + k] = is;
+    }
+    HIP_TRY(c, hipMemcpy(c->fscale, sc, sizeof sc, hipMemcpyHostToDevice));
+    HIP_TRY(c, hipMemcpy(c->f_smin, sc, sizeof sc, hipMemcpyHostToDevice));
     return EFS_OK;
 }
 
@@ -872,6 +966,9 @@ int efs_cfg_excitation(efs_ctx* c, const float* signal, uint32_t len, const uint
     c->exc_n = n;
     c->exc_len = len;
     c->exc_period = period;
+    for (uint32_t k = 0; k < n; ++k) c->source_end = std::max<uint64_t>(c->source_end, (uint64_t)delay[k] + len);
+    if (period) c->source_periodic = true;
+    TRY(source_scales(c, signal, len, amp, n, nullptr, 0));
     return EFS_OK;
 }
 
@@ -982,6 +1079,10 @@ int efs_cfg_pml(efs_ctx* c, uint32_t nreg, const uint32_t* start, const uint32_t
     c->pml = t;
     c->pml_split = aligned && env_u64("EFS_PML_SPLIT", 1) != 0;
     c->pml_fused = env_u64("EFS_PML_FUSED", 1) != 0;
+    if (c->half && nreg && !(c->pml_split && c->pml_fused)) {
+        c->pml = PmlTable{};
+        return fail(c, EFS_ERR_ARG, "FP16 field storage needs UPML z-bounds that are multiples of 4");
+    }
     TRY(plan_sweeps(c));
     c->alias_dirty = true;
     return EFS_OK;
@@ -1075,8 +1176,12 @@ int efs_cfg_tfsf(efs_ctx* c, const float* signal, uint32_t len, uint32_t period,
     }
     c->tfsf_len = len;
     c->tfsf_period = period;
+    for (uint32_t k = 0; k < nv; ++k) c->source_end = std::max<uint64_t>(c->source_end, (uint64_t)v_dint[k] + len);
+    for (uint32_t k = 0; k < ni; ++k) c->source_end = std::max<uint64_t>(c->source_end, (uint64_t)i_dint[k] + len);
+    if (period && (nv || ni)) c->source_periodic = true;
     c->tfsf_nv = nv;
     c->tfsf_ni = ni;
+    if (nv || ni) TRY(source_scales(c, signal, len, v_amp, nv, i_amp, ni));
     return EFS_OK;
 }
 
@@ -1180,6 +1285,7 @@ int efs_iterate(efs_ctx* c, uint32_t n) {
     HIP_TRY(c, hipSetDevice(c->device));
     if (c->ade_dirty) TRY(rebuild_ade(c));
     if (c->alias_dirty) TRY(rebuild_aliases(c));
+    if (n) c->fields_zero = false;
     for (uint32_t s = 0; s < n; ++s) TRY(launch_step(c));
     return EFS_OK;
 }
@@ -1201,6 +1307,22 @@ int efs_set_num_ts(efs_ctx* c, uint32_t ts) {
 int efs_get_fields(efs_ctx* c, float* volt, float* curr) {
     TRY(need_init(c));
     HIP_TRY(c, hipStreamSynchronize(c->stream));
+    if (c->half) {  // widen and scale on the host
+        float sc[6];
+        HIP_TRY(c, hipMemcpy(sc, c->fscale, sizeof sc, hipMemcpyDeviceToHost));
+        std::vector<f16> tmp(c->N);
+        for (int a = 0; a < 2; ++a) {
+            float* out = a ? curr : volt;
+            if (!out) continue;
+            const f16* src = (const f16*)(a ? c->curr : c->volt);
+            for (int n = 0; n < 3; ++n) {
+                HIP_TRY(c, hipMemcpy(tmp.data(), src + n * c->S, c->N * sizeof(f16), hipMemcpyDeviceToHost));
+                const float s = sc[3 * a + n];
+                for (size_t j = 0; j < c->N; ++j) out[n * c->N + j] = (float)tmp[j] * s;
+            }
+        }
+        return EFS_OK;
+    }
     for (int n = 0; n < 3; ++n) {
         if (volt)
             HIP_TRY(c, hipMemcpy(volt + n * c->N, c->volt + n * c->S, c->N * sizeof(float), hipMemcpyDeviceToHost));
@@ -1213,6 +1335,30 @@ int efs_get_fields(efs_ctx* c, float* volt, float* curr) {
 int efs_upload_fields(efs_ctx* c, const float* volt, const float* curr) {
     TRY(need_init(c));
     HIP_TRY(c, hipStreamSynchronize(c->stream));
+    if (c->half) {  // per component: scale 2^e with the largest value near 2^10, then narrow
+        float sc[6];
+        HIP_TRY(c, hipMemcpy(sc, c->fscale, sizeof sc, hipMemcpyDeviceToHost));
+        std::vector<f16> tmp(c->N);
+        for (int a = 0; a < 2; ++a) {
+            const float* in = a ? curr : volt;
+            if (!in) continue;
+            f16* dst = (f16*)(a ? c->curr : c->volt);
+            for (int n = 0; n < 3; ++n) {
+                const float* src = in + n * c->N;
+                float m = 0.f;
+                for (size_t j = 0; j < c->N; ++j) m = std::max(m, std::fabs(src[j]));
+                if (m > 0.f && std::isfinite(m)) sc[3 * a + n] = std::ldexp(1.0f, std::ilogb(m) - kRangeTarget);
+                const float inv = 1.0f / sc[3 * a + n];
+                for (size_t j = 0; j < c->N; ++j) tmp[j] = (f16)(src[j] * inv);
+                HIP_TRY(c, hipMemcpy(dst + n * c->S, tmp.data(), c->N * sizeof(f16), hipMemcpyHostToDevice));
+            }
+        }
+        HIP_TRY(c, hipMemcpy(c->fscale, sc, sizeof sc, hipMemcpyHostToDevice));
+        HIP_TRY(c, hipMemcpy(c->f_smin, sc, sizeof sc, hipMemcpyHostToDevice));
+        c->fields_zero = false;
+        return EFS_OK;
+    }
+    c->fields_zero = false;
     for (int n = 0; n < 3; ++n) {
         if (volt)
             HIP_TRY(c, hipMemcpy(c->volt + n * c->S, volt + n * c->N, c->N * sizeof(float), hipMemcpyHostToDevice));
@@ -1238,7 +1384,7 @@ int efs_cfg_gather(efs_ctx* c, const uint32_t* idx, uint32_t n) {
 int efs_read_gather(efs_ctx* c, float* out) {
     TRY(need_init(c));
     if (!c->g_n) return fail(c, EFS_ERR_STATE, "probe gather not configured");
-    hipLaunchKernelGGL(k_gather, dim3(nblk(c->g_n)), dim3(kPointThreads), 0, c->stream, c->volt, c->curr, c->g_idx,
+    hipLaunchKernelGGL(k_gather, dim3(nblk(c->g_n)), dim3(kPointThreads), 0, c->stream, fv(c, 0), fv(c, 1), c->g_idx,
                        c->g_out, c->g_n);
     CHECK_LAUNCH(c);
     HIP_TRY(c, hipStreamSynchronize(c->stream));
@@ -1416,8 +1562,13 @@ int efs_energy(efs_ctx* c, double* out) {
     }
     const uint32_t rows = (c->nx - 1) * (c->ny - 1);
     const uint32_t blocks = std::max(1u, std::min((uint32_t)kEnergyMaxBlocks, (rows + 3) / 4));
-    hipLaunchKernelGGL(k_energy_partial, dim3(blocks), dim3(kEnergyThreads), 0, c->stream, c->volt, c->curr, c->nx,
-                       c->ny, c->nz, (uint64_t)c->S, c->en_partial);
+    if (c->half)
+        hipLaunchKernelGGL(k_energy_partial<f16>, dim3(blocks), dim3(kEnergyThreads), 0, c->stream,
+                           (const f16*)c->volt, (const f16*)c->curr, c->nx, c->ny, c->nz, (uint64_t)c->S, c->fscale,
+                           c->en_partial);
+    else
+        hipLaunchKernelGGL(k_energy_partial<float>, dim3(blocks), dim3(kEnergyThreads), 0, c->stream, c->volt,
+                           c->curr, c->nx, c->ny, c->nz, (uint64_t)c->S, c->fscale, c->en_partial);
     hipLaunchKernelGGL(k_energy_final, dim3(1), dim3(kEnergyThreads), 0, c->stream, c->en_partial, blocks,
                        c->en_partial + kEnergyMaxBlocks);
     CHECK_LAUNCH(c);
@@ -1431,7 +1582,22 @@ int efs_path_flags(const efs_ctx* c, uint32_t* flags) {
     const bool upml4 = c->vec4 && c->pml.n && c->pml_split;
     *flags = (c->vec4 ? EFS_PATH_VEC4 : 0u) | (c->comp ? EFS_PATH_COEF_IDX : 0u) | (upml4 ? EFS_PATH_UPML_4 : 0u) |
              (upml4 && c->pml_fused ? EFS_PATH_FUSED : 0u) |
-             (upml4 && c->pml_fused && c->pml_comp ? EFS_PATH_UPML_IDX : 0u);
+             (upml4 && c->pml_fused && c->pml_comp ? EFS_PATH_UPML_IDX : 0u) | (c->half ? EFS_PATH_F16 : 0u);
+    return EFS_OK;
+}
+
+int efs_set_storage(efs_ctx* c, int storage) {
+    if (!c) return EFS_ERR_STATE;
+    if (storage != EFS_STORAGE_F32 && storage != EFS_STORAGE_F16) return fail(c, EFS_ERR_ARG, "unknown storage %d", storage);
+    c->storage_req = storage;
+    return EFS_OK;
+}
+
+int efs_field_scales(efs_ctx* c, float* scales) {
+    TRY(need_init(c));
+    if (!scales) return fail(c, EFS_ERR_ARG, "scales is required");
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    HIP_TRY(c, hipMemcpy(scales, c->fscale, 6 * sizeof(float), hipMemcpyDeviceToHost));
     return EFS_OK;
 }
 

@@ -34,7 +34,7 @@ struct Nf2ffGeom {
 // Cell interpolation of E (mean of 4 edges / primal length; 0 on the last
 // grid line) and H (lerp of 2 edges / dual length along its own axis),
 // as in the NF2FF accumulate WGSL.
-__global__ void k_nf2ff_sample(const float* __restrict__ volt, const float* __restrict__ curr,
+__global__ void k_nf2ff_sample(FieldView volt, FieldView curr,
                                const uint32_t* __restrict__ pts, const float* __restrict__ el, Nf2ffGeom g,
                                float* __restrict__ row_out /* buf + slot*6P */) {
     // points are in grid order; consecutive workgroups share cells, so they
@@ -43,15 +43,15 @@ __global__ void k_nf2ff_sample(const float* __restrict__ volt, const float* __re
     if (p >= g.P) return;
     const uint32_t ix = pts[3 * p], iy = pts[3 * p + 1], iz = pts[3 * p + 2];
     const size_t Ny = g.ny, Nz = g.nz;
-    const size_t N = g.S;
     const size_t b = ix * Ny * Nz + iy * Nz + iz;
     const size_t sx = Ny * Nz, sy = Nz;
     float E0 = 0.f, E1 = 0.f, E2 = 0.f;
     if (!(ix >= g.nx - 1 || iy >= g.ny - 1 || iz >= g.nz - 1)) {
-        E0 = (volt[b] + volt[b + sy] + volt[b + sy + 1] + volt[b + 1]) / (4.0f * el[g.el_off[0] + ix]);
-        E1 = (volt[N + b] + volt[N + b + 1] + volt[N + b + sx + 1] + volt[N + b + sx]) /
+        E0 = (volt.ldc(0, b) + volt.ldc(0, b + sy) + volt.ldc(0, b + sy + 1) + volt.ldc(0, b + 1)) /
+             (4.0f * el[g.el_off[0] + ix]);
+        E1 = (volt.ldc(1, b) + volt.ldc(1, b + 1) + volt.ldc(1, b + sx + 1) + volt.ldc(1, b + sx)) /
              (4.0f * el[g.el_off[1] + iy]);
-        E2 = (volt[2 * N + b] + volt[2 * N + b + sx] + volt[2 * N + b + sx + sy] + volt[2 * N + b + sy]) /
+        E2 = (volt.ldc(2, b) + volt.ldc(2, b + sx) + volt.ldc(2, b + sx + sy) + volt.ldc(2, b + sy)) /
              (4.0f * el[g.el_off[2] + iz]);
     }
     const uint32_t pos[3] = {ix, iy, iz};
@@ -63,11 +63,11 @@ __global__ void k_nf2ff_sample(const float* __restrict__ volt, const float* __re
         float h = 0.f;
         const float d0 = el[g.el_off[3 + n] + pos[n]];
         if (d0 > 0.f) {
-            h = curr[n * N + b] / d0;
+            h = curr.ldc(n, b) / d0;
             if (pos[n] < lim[n] - 1) {
                 const float d1 = el[g.el_off[3 + n] + pos[n] + 1];
                 const float dr = d0 / (d0 + d1);
-                const float up = curr[n * N + b + step[n]] / d1;
+                const float up = curr.ldc(n, b + step[n]) / d1;
                 h = h * (1.0f - dr) + up * dr;
             }
         }

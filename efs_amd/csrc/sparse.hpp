@@ -18,14 +18,13 @@ namespace efs {
 constexpr uint32_t kAliasFlux = 0x80000000u;
 constexpr int kPointThreads = 256;
 
-__device__ __forceinline__ float read_alias(const float* __restrict__ field,
-                                            const float* __restrict__ flux, uint32_t a) {
-    return (a & kAliasFlux) ? flux[a & ~kAliasFlux] : field[a];
+__device__ __forceinline__ float read_alias(const FieldView& field, const float* __restrict__ flux, uint32_t a) {
+    return (a & kAliasFlux) ? flux[a & ~kAliasFlux] : field.ld(a);
 }
 
 // apply_excitation (src/webgpu-fdtd.mjs:161-183 semantics: clamp at 0,
 // optional period, past-the-end reads signal[0]).
-__global__ void k_excitation(float* __restrict__ volt, const float* __restrict__ signal, uint32_t len,
+__global__ void k_excitation(FieldView volt, const float* __restrict__ signal, uint32_t len,
                              uint32_t period, const uint32_t* __restrict__ delay,
                              const float* __restrict__ amp, const uint32_t* __restrict__ gidx, uint32_t n,
                              uint32_t numTS) {
@@ -35,11 +34,11 @@ __global__ void k_excitation(float* __restrict__ volt, const float* __restrict__
     if (e < 0) e = 0;
     if (period > 0) e %= (int64_t)period;
     if (e >= (int64_t)len) e = 0;
-    atomicAdd(&volt[gidx[k]], amp[k] * signal[e]);
+    volt.add(gidx[k], amp[k] * signal[e]);
 }
 
 // Mur pre / post / apply (src/webgpu-fdtd.mjs:992-1023).
-__global__ void k_mur_pre(const float* __restrict__ volt, const float* __restrict__ vflux,
+__global__ void k_mur_pre(FieldView volt, const float* __restrict__ vflux,
                           const float* __restrict__ coeff, const uint32_t* __restrict__ normal_src,
                           const uint32_t* __restrict__ shifted_src, float* __restrict__ saved, uint32_t n) {
     const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
@@ -47,23 +46,23 @@ __global__ void k_mur_pre(const float* __restrict__ volt, const float* __restric
     saved[k] = read_alias(volt, vflux, shifted_src[k]) - coeff[k] * read_alias(volt, vflux, normal_src[k]);
 }
 
-__global__ void k_mur_post(const float* __restrict__ volt, const float* __restrict__ coeff,
-                           const uint32_t* __restrict__ shifted, float* __restrict__ saved, uint32_t n) {
+__global__ void k_mur_post(FieldView volt, const float* __restrict__ coeff, const uint32_t* __restrict__ shifted,
+                           float* __restrict__ saved, uint32_t n) {
     const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= n) return;
-    saved[k] += coeff[k] * volt[shifted[k]];
+    saved[k] += coeff[k] * volt.ld(shifted[k]);
 }
 
-__global__ void k_mur_apply(float* __restrict__ volt, const uint32_t* __restrict__ normal,
-                            const float* __restrict__ saved, uint32_t n) {
+__global__ void k_mur_apply(FieldView volt, const uint32_t* __restrict__ normal, const float* __restrict__ saved,
+                            uint32_t n) {
     const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= n) return;
-    volt[normal[k]] = saved[k];
+    volt.st(normal[k], saved[k]);
 }
 
 // Lorentz / Drude ADE state update for all (order, direction) entries at
 // once (src/webgpu-fdtd.mjs:405-454).  lorentz[k] selects the Lorentz form.
-__global__ void k_ade_update(const float* __restrict__ field, const float* __restrict__ flux,
+__global__ void k_ade_update(FieldView field, const float* __restrict__ flux,
                              const uint32_t* __restrict__ src, const uint8_t* __restrict__ lorentz,
                              const float* __restrict__ c_int, const float* __restrict__ c_ext,
                              const float* __restrict__ c_lor, float* __restrict__ A, float* __restrict__ Lr,
@@ -81,17 +80,17 @@ __global__ void k_ade_update(const float* __restrict__ field, const float* __res
 }
 
 // openEMS write-back of the ADE state (optional): field -= A.
-__global__ void k_ade_apply(float* __restrict__ field, const uint32_t* __restrict__ gidx,
-                            const float* __restrict__ A, uint32_t n) {
+__global__ void k_ade_apply(FieldView field, const uint32_t* __restrict__ gidx, const float* __restrict__ A,
+                            uint32_t n) {
     const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= n) return;
-    atomicAdd(&field[gidx[k]], -A[k]);
+    field.add(gidx[k], -A[k]);
 }
 
 // TFSF injection (src/webgpu-fdtd.mjs:683-739): linear interpolation
 // between signal[d] and signal[d+1]; skipped before the delay and at or
 // past the last sample.
-__global__ void k_tfsf(float* __restrict__ field, const float* __restrict__ signal, uint32_t len,
+__global__ void k_tfsf(FieldView field, const float* __restrict__ signal, uint32_t len,
                        uint32_t period, const uint32_t* __restrict__ gidx, const uint32_t* __restrict__ dint,
                        const float* __restrict__ dfrac, const float* __restrict__ amp, uint32_t n,
                        uint32_t numTS) {
@@ -103,7 +102,7 @@ __global__ void k_tfsf(float* __restrict__ field, const float* __restrict__ sign
     if (d >= (int64_t)len - 1) return;
     const float delta = dfrac[k];
     const float sig = (1.0f - delta) * signal[d] + delta * signal[d + 1];
-    atomicAdd(&field[gidx[k]], amp[k] * sig);
+    field.add(gidx[k], amp[k] * sig);
 }
 
 // Lumped RLC, CPU-engine order: history shift + parallel inductor current
@@ -119,7 +118,7 @@ __global__ void k_rlc_pre(const uint32_t* __restrict__ type_flag, const float* _
     if (type_flag[k] == 0) vil[k] += c[9 * k + 0] * c[9 * k + 1] * V[1];
 }
 
-__global__ void k_rlc_apply(float* __restrict__ volt, const uint32_t* __restrict__ gidx,
+__global__ void k_rlc_apply(FieldView volt, const uint32_t* __restrict__ gidx,
                             const uint32_t* __restrict__ type_flag, const float* __restrict__ c,
                             float* __restrict__ vdn, float* __restrict__ jn, const float* __restrict__ vil,
                             uint32_t n) {
@@ -128,14 +127,14 @@ __global__ void k_rlc_apply(float* __restrict__ volt, const uint32_t* __restrict
     float* V = vdn + 3 * k;
     float* J = jn + 3 * k;
     const float* q = c + 9 * k;  // i2v ilv vvd vv2 vj1 vj2 ib0 b1 b2
-    V[0] = volt[gidx[k]];
+    V[0] = volt.ld(gidx[k]);
     J[2] = J[1];
     J[1] = J[0];
     if (type_flag[k] == 1) {
         const float v0 = q[2] * (V[0] - vil[k] + q[3] * V[2] + q[4] * J[1] + q[5] * J[2]);
         V[0] = v0;
         J[0] = q[6] * (v0 - V[2]) - q[7] * q[6] * J[1] - q[8] * q[6] * J[2];
-        volt[gidx[k]] = v0;
+        volt.st(gidx[k], v0);
     }
 }
 
@@ -155,8 +154,7 @@ struct PbcAxis {
     uint32_t has_imag;
 };
 
-__global__ void k_pbc(float* __restrict__ field, float* __restrict__ imag, PbcAxis a, uint32_t ny, uint32_t nz,
-                      uint64_t S) {
+__global__ void k_pbc(FieldView field, float* __restrict__ imag, PbcAxis a, uint32_t ny, uint32_t nz, uint64_t S) {
     const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= a.b0 * a.b1) return;
     const uint32_t i0 = t / a.b1, i1 = t - i0 * a.b1;
@@ -169,13 +167,13 @@ __global__ void k_pbc(float* __restrict__ field, float* __restrict__ imag, PbcAx
 #pragma unroll
     for (int q = 0; q < 2; ++q) {
         const uint32_t comp = q ? a.t1 : a.t0;
-        const float re = field[comp * S + ls];
+        const float re = field.ld(comp * S + ls);
         if (!a.has_imag) {
-            field[comp * S + ld] = a.cs * re;
+            field.st(comp * S + ld, a.cs * re);
         } else {
             float* ip = imag + a.imag_off + (size_t)q * a.plane + (size_t)i0 * a.f1 + i1;
             const float im = *ip;
-            field[comp * S + ld] = a.cs * re - a.sn * im;
+            field.st(comp * S + ld, a.cs * re - a.sn * im);
             *ip = a.sn * re + a.cs * im;
         }
     }
@@ -183,24 +181,24 @@ __global__ void k_pbc(float* __restrict__ field, float* __restrict__ imag, PbcAx
 
 // Steady-state energy accumulation (src/webgpu-engine.mjs:586-625): v^2 of
 // each probe added to period 1 while current < period, else to period 2.
-__global__ void k_ss_accumulate(const float* __restrict__ volt, const uint32_t* __restrict__ gidx, uint32_t n,
+__global__ void k_ss_accumulate(FieldView volt, const uint32_t* __restrict__ gidx, uint32_t n,
                                 uint32_t period, uint32_t current, float* __restrict__ e1,
                                 float* __restrict__ e2) {
     const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= n) return;
-    const float v = volt[gidx[k]];
+    const float v = volt.ld(gidx[k]);
     if (current < period) e1[k] += v * v;
     else e2[k] += v * v;
 }
 
 // Probe gather into out[2*k], out[2*k+1] (src/webgpu-engine.mjs:2593-2665).
-__global__ void k_gather(const float* __restrict__ volt, const float* __restrict__ curr,
-                         const uint32_t* __restrict__ idx, float* __restrict__ out, uint32_t n) {
+__global__ void k_gather(FieldView volt, FieldView curr, const uint32_t* __restrict__ idx, float* __restrict__ out,
+                         uint32_t n) {
     const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= n) return;
     const uint32_t j = idx[k];
-    out[2 * k] = volt[j];
-    out[2 * k + 1] = curr[j];
+    out[2 * k] = volt.ld(j);
+    out[2 * k + 1] = curr.ld(j);
 }
 
 }  // namespace efs

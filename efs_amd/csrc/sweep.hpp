@@ -135,6 +135,117 @@ __device__ __forceinline__ v4 ld4nt(const float* p) {
 __device__ __forceinline__ float ldnt(const float* p) { return __builtin_nontemporal_load(p); }
 __device__ __forceinline__ void st4(float* p, v4 v) { *reinterpret_cast<v4*>(p) = v; }
 
+// Field storage.  FP32, or FP16 holding value * 2^-e per component: the
+// scale s = 2^e is exact, loads widen to FP32 and multiply by s, stores
+// multiply by 1/s and round to nearest.  The update arithmetic is FP32 in
+// both cases.
+typedef _Float16 f16;
+typedef f16 h4 __attribute__((ext_vector_type(4)));
+
+template <typename T>
+struct Fs;
+template <>
+struct Fs<float> {
+    static constexpr bool kHalf = false;
+    __device__ static __forceinline__ v4 ld(const float* p, float) { return ld4(p); }
+    __device__ static __forceinline__ float ld1(const float* p, float) { return *p; }
+    __device__ static __forceinline__ void st(float* p, v4 v, float) { st4(p, v); }
+};
+// FP16 stores saturate at the largest finite half: a value that outgrew its
+// scale between two range checks is clamped (and the next check rescales)
+// instead of becoming infinite.
+constexpr float kHalfMax = 65504.0f;
+__device__ __forceinline__ v4 clamp16(v4 v) {
+    return __builtin_elementwise_min(__builtin_elementwise_max(v, (v4)(-kHalfMax)), (v4)kHalfMax);
+}
+__device__ __forceinline__ float clamp16(float v) { return fminf(fmaxf(v, -kHalfMax), kHalfMax); }
+
+template <>
+struct Fs<f16> {
+    static constexpr bool kHalf = true;
+    __device__ static __forceinline__ v4 ld(const f16* p, float s) {
+        return __builtin_convertvector(*reinterpret_cast<const h4*>(p), v4) * s;
+    }
+    __device__ static __forceinline__ float ld1(const f16* p, float s) { return (float)*p * s; }
+    __device__ static __forceinline__ void st(f16* p, v4 v, float inv) {
+        *reinterpret_cast<h4*>(p) = __builtin_convertvector(clamp16(v * inv), h4);
+    }
+};
+
+// A field array (volt or curr, 3 components S apart) as the point kernels
+// see it: FP32, or FP16 with per-component scales (value = stored *
+// scale[n], n = index / S; powers of two, so the scaling is exact).
+struct FieldView {
+    void* p;
+    const float* scale;  // 3 scales of this array (FP16 storage)
+    uint64_t S;
+    uint32_t half;
+
+    // component c, index j inside the component
+    __device__ __forceinline__ float ldc(uint32_t c, size_t j) const {
+        if (!half) return static_cast<const float*>(p)[c * S + j];
+        return (float)static_cast<const f16*>(p)[c * S + j] * scale[c];
+    }
+    __device__ __forceinline__ float ld(size_t i) const {
+        if (!half) return static_cast<const float*>(p)[i];
+        return (float)static_cast<const f16*>(p)[i] * scale[i / S];
+    }
+    __device__ __forceinline__ void st(size_t i, float v) const {
+        if (!half) {
+            static_cast<float*>(p)[i] = v;
+            return;
+        }
+        static_cast<f16*>(p)[i] = (f16)clamp16(v / scale[i / S]);
+    }
+    // field[i] += v, safe when indices repeat: float atomics, or for FP16 a
+    // compare-and-swap loop on the aligned 32-bit word holding the half.
+    __device__ __forceinline__ void add(size_t i, float v) const {
+        if (!half) {
+            atomicAdd(static_cast<float*>(p) + i, v);
+            return;
+        }
+        const float sc = scale[i / S];
+        f16* hp = static_cast<f16*>(p) + i;
+        unsigned int* w = reinterpret_cast<unsigned int*>(reinterpret_cast<uintptr_t>(hp) & ~(uintptr_t)3);
+        const unsigned int sh = (reinterpret_cast<uintptr_t>(hp) & 2) ? 16u : 0u;
+        unsigned int old = *w, assumed;
+        do {
+            assumed = old;
+            const unsigned short bits = (unsigned short)(assumed >> sh);
+            f16 h;
+            __builtin_memcpy(&h, &bits, 2);
+            const f16 nh = (f16)clamp16(((float)h * sc + v) / sc);
+            unsigned short nb;
+            __builtin_memcpy(&nb, &nh, 2);
+            const unsigned int nw = (assumed & ~(0xffffu << sh)) | ((unsigned int)nb << sh);
+            old = atomicCAS(w, assumed, nw);
+        } while (old != assumed);
+    }
+};
+
+// Per-component scales of the two field arrays (FP16 storage): fscale[n]
+// for volt component n, fscale[3+n] for curr.  own = the array the sweep
+// writes, other = the array it reads.
+struct Scales {
+    float own[3], own_inv[3], other[3];
+};
+
+template <typename T>
+__device__ __forceinline__ Scales load_scales(const float* __restrict__ fscale, bool own_is_curr) {
+    Scales sc;
+#pragma unroll
+    for (int n = 0; n < 3; ++n) {
+        if (Fs<T>::kHalf) {
+            sc.own[n] = fscale[(own_is_curr ? 3 : 0) + n];
+            sc.own_inv[n] = 1.0f / sc.own[n];  // exact: powers of two
+            sc.other[n] = fscale[(own_is_curr ? 0 : 3) + n];
+        } else {
+            sc.own[n] = sc.own_inv[n] = sc.other[n] = 1.0f;
+        }
+    }
+    return sc;
+}
+
 // Explicit fused multiply-adds fix the rounding of every update, so the
 // 1-wide and 4-wide sweeps (and the interior/UPML split) give identical bits:
 //   core  V' = fma(a, V, b*curl)
@@ -429,17 +540,18 @@ __device__ __forceinline__ void upml_decode(const float4* s_ptab, const uint32_t
 
 // E update of 4 cells over x in [x0, x1); MODE says how UPML cells are
 // treated (column mask m gives the regions the lane's column crosses).
-template <int MODE, bool COMP, bool PCOMP>
-__device__ __forceinline__ void e4_march(float* __restrict__ volt, const float* __restrict__ curr, const Coef& cf,
+template <int MODE, bool COMP, bool PCOMP, typename T>
+__device__ __forceinline__ void e4_march(T* __restrict__ volt, const T* __restrict__ curr, const Coef& cf,
                                          const float2* s_tab, const float4* s_ptab, const SweepGeom& g,
                                          const PmlTable& pml,
                                          const float* __restrict__ pcoef, float* __restrict__ pflux, Lane4 w,
-                                         uint32_t m) {
+                                         uint32_t m, const Scales& sc) {
+    typedef Fs<T> L;
     const size_t S = g.S;
-    float* const E[3] = {volt, volt + S, volt + 2 * S};
-    const float* const H0 = curr;
-    const float* const H1 = curr + S;
-    const float* const H2 = curr + 2 * S;
+    T* const E[3] = {volt, volt + S, volt + 2 * S};
+    const T* const H0 = curr;
+    const T* const H1 = curr + S;
+    const T* const H2 = curr + 2 * S;
     const uint32_t plane = g.cols;
     const uint32_t sy = w.y > 0 ? g.nz : 0u;  // backward neighbours clamp to self at index 0
     const uint32_t sz = w.z0 > 0 ? 1u : 0u;
@@ -456,16 +568,16 @@ __device__ __forceinline__ void e4_march(float* __restrict__ volt, const float* 
         }
         if (!carry) {
             const uint32_t ib = x > 0 ? i - plane : i;
-            hy_m = ld4(H1 + ib);
-            hz_m = ld4(H2 + ib);
+            hy_m = L::ld(H1 + ib, sc.other[1]);
+            hz_m = L::ld(H2 + ib, sc.other[2]);
         }
-        const v4 hx = ld4(H0 + i), hy = ld4(H1 + i), hz = ld4(H2 + i);
-        const v4 hz_ym = ld4(H2 + i - sy), hx_ym = ld4(H0 + i - sy);
-        const float hy_z = H1[i - sz], hx_z = H0[i - sz];  // z-1 of the first cell
+        const v4 hx = L::ld(H0 + i, sc.other[0]), hy = L::ld(H1 + i, sc.other[1]), hz = L::ld(H2 + i, sc.other[2]);
+        const v4 hz_ym = L::ld(H2 + i - sy, sc.other[2]), hx_ym = L::ld(H0 + i - sy, sc.other[0]);
+        const float hy_z = L::ld1(H1 + i - sz, sc.other[1]), hx_z = L::ld1(H0 + i - sz, sc.other[0]);  // z-1
         v4 V[3], a[3], b[3];
         uint32_t q[3];
 #pragma unroll
-        for (int n = 0; n < 3; ++n) V[n] = ld4(E[n] + i);
+        for (int n = 0; n < 3; ++n) V[n] = L::ld(E[n] + i, sc.own[n]);
         coef_fetch<COMP>(cf, S, i, q, a, b);
         v4 F[3], P0[3], P1[3], P2[3];
         uint32_t pq[3];
@@ -479,13 +591,13 @@ __device__ __forceinline__ void e4_march(float* __restrict__ volt, const float* 
         carry = true;
         if (!upml) {
 #pragma unroll
-            for (int n = 0; n < 3; ++n) st4(E[n] + i, fma4(a[n], V[n], b[n] * cur[n]));
+            for (int n = 0; n < 3; ++n) L::st(E[n] + i, fma4(a[n], V[n], b[n] * cur[n]), sc.own_inv[n]);
         } else {
             upml_decode<PCOMP>(s_ptab, pq, P0, P1, P2);
 #pragma unroll
             for (int n = 0; n < 3; ++n) {
                 const v4 Fn = fma4(a[n], F[n], b[n] * cur[n]);
-                st4(E[n] + i, fma4(P2[n], Fn, fma4(P0[n], V[n], -(P1[n] * F[n]))));
+                L::st(E[n] + i, fma4(P2[n], Fn, fma4(P0[n], V[n], -(P1[n] * F[n]))), sc.own_inv[n]);
                 st4(u.flux + n * u.cs, Fn);
             }
         }
@@ -494,17 +606,18 @@ __device__ __forceinline__ void e4_march(float* __restrict__ volt, const float* 
 
 // H update of 4 cells over x in [x0, x1) (see e4_march).  Cells outside the
 // core bounds (x = Nx-1, y = Ny-1, z = Nz-1) keep I unless in UPML.
-template <int MODE, bool COMP, bool PCOMP>
-__device__ __forceinline__ void h4_march(float* __restrict__ curr, const float* __restrict__ volt, const Coef& cf,
+template <int MODE, bool COMP, bool PCOMP, typename T>
+__device__ __forceinline__ void h4_march(T* __restrict__ curr, const T* __restrict__ volt, const Coef& cf,
                                          const float2* s_tab, const float4* s_ptab, const SweepGeom& g,
                                          const PmlTable& pml,
                                          const float* __restrict__ pcoef, float* __restrict__ pflux, Lane4 w,
-                                         uint32_t m) {
+                                         uint32_t m, const Scales& sc) {
+    typedef Fs<T> L;
     const size_t S = g.S;
-    float* const Ic[3] = {curr, curr + S, curr + 2 * S};
-    const float* const E0 = volt;
-    const float* const E1 = volt + S;
-    const float* const E2 = volt + 2 * S;
+    T* const Ic[3] = {curr, curr + S, curr + 2 * S};
+    const T* const E0 = volt;
+    const T* const E1 = volt + S;
+    const T* const E2 = volt + 2 * S;
     const uint32_t plane = g.cols;
     const bool row_core = w.y + 1 < g.ny;
     const bool last = w.z0 + 4 >= g.nz;  // the lane holds the row's last z cell (no core update there)
@@ -525,20 +638,21 @@ __device__ __forceinline__ void h4_march(float* __restrict__ curr, const float* 
             continue;
         }
         if (!carry) {
-            ex = ld4(E0 + i);
-            ey = ld4(E1 + i);
-            ez = ld4(E2 + i);
+            ex = L::ld(E0 + i, sc.other[0]);
+            ey = L::ld(E1 + i, sc.other[1]);
+            ez = L::ld(E2 + i, sc.other[2]);
         }
         // per-cell core flags: cells 0..2 follow `core`, cell 3 also needs z+1 < Nz
         const uint32_t cb = core ? (last ? 7u : 15u) : 0u;
         const uint32_t sx = core ? plane : 0u;
-        const v4 ex_p = ld4(E0 + i + sx), ey_p = ld4(E1 + i + sx), ez_p = ld4(E2 + i + sx);
-        const v4 ez_yp = ld4(E2 + i + sy), ex_yp = ld4(E0 + i + sy);
-        const float ey_z = E1[i + sz], ex_z = E0[i + sz];  // z+1 of the fourth cell
+        const v4 ex_p = L::ld(E0 + i + sx, sc.other[0]), ey_p = L::ld(E1 + i + sx, sc.other[1]),
+                 ez_p = L::ld(E2 + i + sx, sc.other[2]);
+        const v4 ez_yp = L::ld(E2 + i + sy, sc.other[2]), ex_yp = L::ld(E0 + i + sy, sc.other[0]);
+        const float ey_z = L::ld1(E1 + i + sz, sc.other[1]), ex_z = L::ld1(E0 + i + sz, sc.other[0]);  // z+1
         v4 I[3], a[3], b[3];
         uint32_t q[3];
 #pragma unroll
-        for (int n = 0; n < 3; ++n) I[n] = ld4(Ic[n] + i);
+        for (int n = 0; n < 3; ++n) I[n] = L::ld(Ic[n] + i, sc.own[n]);
         coef_fetch<COMP>(cf, S, i, q, a, b);
         v4 F[3], P0[3], P1[3], P2[3];  // P: ii, iifo, iifn
         uint32_t pq[3];
@@ -553,13 +667,14 @@ __device__ __forceinline__ void h4_march(float* __restrict__ curr, const float* 
         carry = core;
         if (!upml) {
 #pragma unroll
-            for (int n = 0; n < 3; ++n) st4(Ic[n] + i, sel4(cb, fma4(a[n], I[n], b[n] * cur[n]), I[n]));
+            for (int n = 0; n < 3; ++n)
+                L::st(Ic[n] + i, sel4(cb, fma4(a[n], I[n], b[n] * cur[n]), I[n]), sc.own_inv[n]);
         } else {
             upml_decode<PCOMP>(s_ptab, pq, P0, P1, P2);
 #pragma unroll
             for (int n = 0; n < 3; ++n) {
                 const v4 Fn = sel4(cb, fma4(a[n], F[n], b[n] * cur[n]), F[n]);
-                st4(Ic[n] + i, fma4(P2[n], Fn, fma4(P0[n], I[n], -(P1[n] * F[n]))));
+                L::st(Ic[n] + i, fma4(P2[n], Fn, fma4(P0[n], I[n], -(P1[n] * F[n]))), sc.own_inv[n]);
                 st4(u.flux + n * u.cs, Fn);
             }
         }
@@ -606,39 +721,43 @@ __device__ __forceinline__ bool pml_lane(const PmlTable& pml, Lane4& w) {
     return w.x0 < w.x1;
 }
 
-#define EFS_SWEEP4_PARAMS                                                                                       \
-    float *__restrict__ dst, const float *__restrict__ src, Coef cf, SweepGeom g, PmlTable pml,                \
-        const float *__restrict__ pcoef, float *__restrict__ pflux
+#define EFS_SWEEP4_PARAMS(T)                                                                                    \
+    T *__restrict__ dst, const T *__restrict__ src, Coef cf, SweepGeom g, PmlTable pml,                        \
+        const float *__restrict__ pcoef, float *__restrict__ pflux, const float *__restrict__ fscale
 
 // Interior (4 waves/SIMD) and UPML (3 waves/SIMD) entry points, each with
 // FP32 arrays or table-compressed (suffix c) coefficients; the interior
-// kernels with suffix m skip the UPML regions.
-#define EFS_SWEEP4_INTERIOR(NAME, MARCH, HAS_PML, COMP)                                                           \
+// kernels with suffix m skip the UPML regions.  FP32 field storage.
+#define EFS_SWEEP4_INTERIOR(NAME, MARCH, HF, HAS_PML, COMP)                                                       \
     __global__ __launch_bounds__(kSweepThreads)                                                                    \
-        __attribute__((amdgpu_waves_per_eu(EFS_SWEEP_WAVES, EFS_SWEEP_WAVES))) void NAME(EFS_SWEEP4_PARAMS) {     \
+        __attribute__((amdgpu_waves_per_eu(EFS_SWEEP_WAVES, EFS_SWEEP_WAVES))) void NAME(EFS_SWEEP4_PARAMS(float)) { \
         __shared__ float2 s_tab[COMP ? 3 * kTab : 1];                                                              \
         coef_stage<COMP>(cf, s_tab);                                                                               \
         Lane4 w;                                                                                                   \
         uint32_t m, flags;                                                                                         \
         if (!interior_lane<HAS_PML>(g, pml, w, m, flags)) return;                                                  \
-        MARCH<kMarchInterior, COMP, false>(dst, src, cf, s_tab, nullptr, g, pml, pcoef, pflux, w, m);              \
+        const Scales sc = load_scales<float>(fscale, HF);                                                          \
+        MARCH<kMarchInterior, COMP, false, float>(dst, src, cf, s_tab, nullptr, g, pml, pcoef, pflux, w, m, sc);   \
     }
 
-#define EFS_SWEEP4_UPML(NAME, MARCH, COMP)                                                                         \
+#define EFS_SWEEP4_UPML(NAME, MARCH, HF, COMP)                                                                     \
     __global__ __launch_bounds__(kSweepThreads) __attribute__((                                                    \
-        amdgpu_waves_per_eu(EFS_SWEEP_WAVES_PML, EFS_SWEEP_WAVES_PML))) void NAME(EFS_SWEEP4_PARAMS) {            \
+        amdgpu_waves_per_eu(EFS_SWEEP_WAVES_PML, EFS_SWEEP_WAVES_PML))) void NAME(EFS_SWEEP4_PARAMS(float)) {     \
         __shared__ float2 s_tab[COMP ? 3 * kTab : 1];                                                              \
         coef_stage<COMP>(cf, s_tab);                                                                               \
         Lane4 w;                                                                                                   \
         if (!pml_lane(pml, w)) return;                                                                             \
-        MARCH<kMarchRegion, COMP, false>(dst, src, cf, s_tab, nullptr, g, pml, pcoef, pflux, w, 1u << w.region);   \
+        const Scales sc = load_scales<float>(fscale, HF);                                                          \
+        MARCH<kMarchRegion, COMP, false, float>(dst, src, cf, s_tab, nullptr, g, pml, pcoef, pflux, w,             \
+                                                1u << w.region, sc);                                               \
     }
 
 // Fused: the grid launch does the UPML cells too (3 waves/SIMD, the UPML
-// register budget); PCOMP: UPML coefficients as byte indices + LDS table.
-#define EFS_SWEEP4_FUSED(NAME, MARCH, COMP, PCOMP)                                                                 \
+// register budget); PCOMP: UPML coefficients as byte indices + LDS table;
+// T: field storage (float or f16).
+#define EFS_SWEEP4_FUSED(NAME, MARCH, HF, COMP, PCOMP, T)                                                          \
     __global__ __launch_bounds__(kSweepThreads) __attribute__((                                                    \
-        amdgpu_waves_per_eu(EFS_SWEEP_WAVES_PML, EFS_SWEEP_WAVES_PML))) void NAME(EFS_SWEEP4_PARAMS) {            \
+        amdgpu_waves_per_eu(EFS_SWEEP_WAVES_PML, EFS_SWEEP_WAVES_PML))) void NAME(EFS_SWEEP4_PARAMS(T)) {         \
         __shared__ float2 s_tab[COMP ? 3 * kTab : 1];                                                              \
         __shared__ float4 s_ptab[PCOMP ? 3 * kTab : 1];                                                            \
         coef_stage<COMP>(cf, s_tab);                                                                               \
@@ -647,32 +766,60 @@ __device__ __forceinline__ bool pml_lane(const PmlTable& pml, Lane4& w) {
         const bool live = interior_lane<true>(g, pml, w, m, flags);                                                \
         upml_stage<PCOMP>(cf, s_ptab, flags);                                                                      \
         if (!live) return;                                                                                         \
-        MARCH<kMarchFused, COMP, PCOMP>(dst, src, cf, s_tab, s_ptab, g, pml, pcoef, pflux, w, m);                  \
+        const Scales sc = load_scales<T>(fscale, HF);                                                              \
+        MARCH<kMarchFused, COMP, PCOMP, T>(dst, src, cf, s_tab, s_ptab, g, pml, pcoef, pflux, w, m, sc);           \
     }
 
-EFS_SWEEP4_INTERIOR(sweep_e4, e4_march, false, false)
-EFS_SWEEP4_INTERIOR(sweep_e4c, e4_march, false, true)
-EFS_SWEEP4_INTERIOR(sweep_e4m, e4_march, true, false)
-EFS_SWEEP4_INTERIOR(sweep_e4mc, e4_march, true, true)
-EFS_SWEEP4_UPML(sweep_e4u, e4_march, false)
-EFS_SWEEP4_UPML(sweep_e4uc, e4_march, true)
-EFS_SWEEP4_FUSED(sweep_e4f, e4_march, false, false)
-EFS_SWEEP4_FUSED(sweep_e4fc, e4_march, true, false)
-EFS_SWEEP4_FUSED(sweep_e4fp, e4_march, false, true)
-EFS_SWEEP4_FUSED(sweep_e4fcp, e4_march, true, true)
-EFS_SWEEP4_INTERIOR(sweep_h4, h4_march, false, false)
-EFS_SWEEP4_INTERIOR(sweep_h4c, h4_march, false, true)
-EFS_SWEEP4_INTERIOR(sweep_h4m, h4_march, true, false)
-EFS_SWEEP4_INTERIOR(sweep_h4mc, h4_march, true, true)
-EFS_SWEEP4_UPML(sweep_h4u, h4_march, false)
-EFS_SWEEP4_UPML(sweep_h4uc, h4_march, true)
-EFS_SWEEP4_FUSED(sweep_h4f, h4_march, false, false)
-EFS_SWEEP4_FUSED(sweep_h4fc, h4_march, true, false)
-EFS_SWEEP4_FUSED(sweep_h4fp, h4_march, false, true)
-EFS_SWEEP4_FUSED(sweep_h4fcp, h4_march, true, true)
+// Grid launch without UPML, FP16 field storage (FP32 storage uses the
+// interior kernels above, which are the same code for HAS_PML = false).
+#define EFS_SWEEP4_PLAIN16(NAME, MARCH, HF, COMP)                                                                  \
+    __global__ __launch_bounds__(kSweepThreads)                                                                    \
+        __attribute__((amdgpu_waves_per_eu(EFS_SWEEP_WAVES, EFS_SWEEP_WAVES))) void NAME(EFS_SWEEP4_PARAMS(f16)) { \
+        __shared__ float2 s_tab[COMP ? 3 * kTab : 1];                                                              \
+        coef_stage<COMP>(cf, s_tab);                                                                               \
+        Lane4 w;                                                                                                   \
+        uint32_t m, flags;                                                                                         \
+        if (!interior_lane<false>(g, pml, w, m, flags)) return;                                                    \
+        const Scales sc = load_scales<f16>(fscale, HF);                                                            \
+        MARCH<kMarchInterior, COMP, false, f16>(dst, src, cf, s_tab, nullptr, g, pml, pcoef, pflux, w, m, sc);     \
+    }
+
+EFS_SWEEP4_INTERIOR(sweep_e4, e4_march, false, false, false)
+EFS_SWEEP4_INTERIOR(sweep_e4c, e4_march, false, false, true)
+EFS_SWEEP4_INTERIOR(sweep_e4m, e4_march, false, true, false)
+EFS_SWEEP4_INTERIOR(sweep_e4mc, e4_march, false, true, true)
+EFS_SWEEP4_UPML(sweep_e4u, e4_march, false, false)
+EFS_SWEEP4_UPML(sweep_e4uc, e4_march, false, true)
+EFS_SWEEP4_FUSED(sweep_e4f, e4_march, false, false, false, float)
+EFS_SWEEP4_FUSED(sweep_e4fc, e4_march, false, true, false, float)
+EFS_SWEEP4_FUSED(sweep_e4fp, e4_march, false, false, true, float)
+EFS_SWEEP4_FUSED(sweep_e4fcp, e4_march, false, true, true, float)
+EFS_SWEEP4_PLAIN16(sweep_e4_h, e4_march, false, false)
+EFS_SWEEP4_PLAIN16(sweep_e4c_h, e4_march, false, true)
+EFS_SWEEP4_FUSED(sweep_e4f_h, e4_march, false, false, false, f16)
+EFS_SWEEP4_FUSED(sweep_e4fc_h, e4_march, false, true, false, f16)
+EFS_SWEEP4_FUSED(sweep_e4fp_h, e4_march, false, false, true, f16)
+EFS_SWEEP4_FUSED(sweep_e4fcp_h, e4_march, false, true, true, f16)
+EFS_SWEEP4_INTERIOR(sweep_h4, h4_march, true, false, false)
+EFS_SWEEP4_INTERIOR(sweep_h4c, h4_march, true, false, true)
+EFS_SWEEP4_INTERIOR(sweep_h4m, h4_march, true, true, false)
+EFS_SWEEP4_INTERIOR(sweep_h4mc, h4_march, true, true, true)
+EFS_SWEEP4_UPML(sweep_h4u, h4_march, true, false)
+EFS_SWEEP4_UPML(sweep_h4uc, h4_march, true, true)
+EFS_SWEEP4_FUSED(sweep_h4f, h4_march, true, false, false, float)
+EFS_SWEEP4_FUSED(sweep_h4fc, h4_march, true, true, false, float)
+EFS_SWEEP4_FUSED(sweep_h4fp, h4_march, true, false, true, float)
+EFS_SWEEP4_FUSED(sweep_h4fcp, h4_march, true, true, true, float)
+EFS_SWEEP4_PLAIN16(sweep_h4_h, h4_march, true, false)
+EFS_SWEEP4_PLAIN16(sweep_h4c_h, h4_march, true, true)
+EFS_SWEEP4_FUSED(sweep_h4f_h, h4_march, true, false, false, f16)
+EFS_SWEEP4_FUSED(sweep_h4fc_h, h4_march, true, true, false, f16)
+EFS_SWEEP4_FUSED(sweep_h4fp_h, h4_march, true, false, true, f16)
+EFS_SWEEP4_FUSED(sweep_h4fcp_h, h4_march, true, true, true, f16)
 #undef EFS_SWEEP4_INTERIOR
 #undef EFS_SWEEP4_UPML
 #undef EFS_SWEEP4_FUSED
+#undef EFS_SWEEP4_PLAIN16
 #undef EFS_SWEEP4_PARAMS
 
 }  // namespace efs

@@ -67,6 +67,8 @@ SIGNATURES = {
     "efs_profile_read": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_uint64)]),
     "efs_path_flags": (C.c_int, [C.c_void_p, C.POINTER(C.c_uint32)]),
     "efs_cfg_pbc": (C.c_int, [C.c_void_p, C.c_uint32, _u32p, _f64p]),
+    "efs_set_storage": (C.c_int, [C.c_void_p, C.c_int]),
+    "efs_field_scales": (C.c_int, [C.c_void_p, _f32p]),
     "efs_cfg_steady_state": (C.c_int, [C.c_void_p, _u32p, C.c_uint32, C.c_uint32]),
     "efs_ss_set": (C.c_int, [C.c_void_p, C.c_int, C.c_uint32]),
     "efs_ss_read": (C.c_int, [C.c_void_p, _f32p, _f32p, C.POINTER(C.c_uint32), C.POINTER(C.c_int)]),
@@ -153,9 +155,14 @@ class HIPEngine:
             raise RuntimeError("HIP device not initialized. Call initGPU() first.")
 
     # -- init / configure -----------------------------------------------
-    def init(self, gridSize, coefficients):
-        """init(gridSize, {vv, vi, ii, iv}) (src/webgpu-engine.mjs:1143-1185)."""
+    def init(self, gridSize, coefficients, storage="fp32"):
+        """init(gridSize, {vv, vi, ii, iv}) (src/webgpu-engine.mjs:1143-1185).
+        storage="fp16" keeps the fields as FP16 with per-component
+        power-of-two scales (updates stay FP32); reads return FP32."""
         self._need_device()
+        if storage not in ("fp32", "fp16"):
+            raise ValueError("storage must be 'fp32' or 'fp16'")
+        self._check(self.lib.efs_set_storage(self.ctx, 1 if storage == "fp16" else 0))
         nx, ny, nz = (int(v) for v in gridSize)
         c = coefficients
         self._check(self.lib.efs_init(self.ctx, nx, ny, nz, f32(c["vv"]), f32(c["vi"]), f32(c["ii"]),
@@ -459,11 +466,17 @@ class HIPEngine:
         self._check(self.lib.efs_profile_read(self.ctx, int(which), C.byref(ms), C.byref(n)))
         return ms.value, n.value
 
+    def fieldScales(self):
+        """Per-component storage scales: volt 0..2, curr 0..2."""
+        out = np.zeros(6, np.float32)
+        self._check(self.lib.efs_field_scales(self.ctx, out))
+        return out
+
     def pathFlags(self):
         """Sweep forms in use: set of 'vec4', 'coef_idx', 'upml4', 'fused', 'upml_idx'."""
         f = C.c_uint32(0)
         self._check(self.lib.efs_path_flags(self.ctx, C.byref(f)))
-        names = ("vec4", "coef_idx", "upml4", "fused", "upml_idx")
+        names = ("vec4", "coef_idx", "upml4", "fused", "upml_idx", "f16")
         return {n for b, n in enumerate(names) if f.value & (1 << b)}
 
 

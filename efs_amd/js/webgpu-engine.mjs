@@ -54,8 +54,13 @@ export class HIPEngine {
     get numTS() { return this._h ? addon.numTS(this._h) : 0; }
     set numTS(v) { this._need(); addon.setNumTS(this._h, v >>> 0); }
 
-    async init(gridSize, coefficients) {
+    // options.storage: 'fp32' (default) or 'fp16' (fields stored as FP16
+    // with per-component power-of-two scales; reads return FP32).
+    async init(gridSize, coefficients, options) {
         this._need();
+        const storage = (options && options.storage) || 'fp32';
+        if (storage !== 'fp32' && storage !== 'fp16') throw new Error("storage must be 'fp32' or 'fp16'");
+        addon.setStorage(this._h, storage === 'fp16' ? 1 : 0);
         const [Nx, Ny, Nz] = gridSize;
         addon.init(this._h, Nx, Ny, Nz, f32(coefficients.vv), f32(coefficients.vi), f32(coefficients.ii), f32(coefficients.iv));
         this.numLines = [Nx, Ny, Nz];
@@ -156,6 +161,13 @@ export class HIPEngine {
     set ssRecording(on) { this._need(); addon.ssSet(this._h, on ? 1 : 0, addon.ssRead(this._h, null, null)[0]); }
     get ssCurrentSample() { return this._h && this._ss ? addon.ssRead(this._h, null, null)[0] : 0; }
     set ssCurrentSample(v) { this._need(); addon.ssSet(this._h, addon.ssRead(this._h, null, null)[1], v >>> 0); }
+
+    fieldScales() {
+        this._need();
+        const out = new Float32Array(6);
+        addon.fieldScales(this._h, out);
+        return out;
+    }
 
     // Period energies and counter (host copy).
     readSteadyState() {

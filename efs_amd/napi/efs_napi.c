@@ -178,6 +178,29 @@ static napi_value js_last_error(napi_env env, napi_callback_info info) {
     return s;
 }
 
+/* setStorage(h, fmt): 0 = FP32, 1 = FP16 (takes effect at init) */
+static napi_value js_set_storage(napi_env env, napi_callback_info info) {
+    BEGIN();
+    CTX(0);
+    uint32_t fmt = u32_arg(&c, 1);
+    BAIL();
+    return status(&c, ctx, efs_set_storage(ctx, (int)fmt));
+}
+
+/* fieldScales(h, out Float32Array(6)) */
+static napi_value js_field_scales(napi_env env, napi_callback_info info) {
+    BEGIN();
+    CTX(0);
+    size_t n;
+    float *out = F32(1, n);
+    BAIL();
+    if (n < 6) {
+        throwf(&c, "efs: fieldScales needs a Float32Array of 6");
+        return NULL;
+    }
+    return status(&c, ctx, efs_field_scales(ctx, out));
+}
+
 static napi_value js_init(napi_env env, napi_callback_info info) {
     BEGIN();
     CTX(0);
@@ -526,6 +549,8 @@ static napi_value Init(napi_env env, napi_value exports) {
         FN("destroy", js_destroy),
         FN("lastError", js_last_error),
         FN("init", js_init),
+        FN("setStorage", js_set_storage),
+        FN("fieldScales", js_field_scales),
         FN("cfgExcitation", js_cfg_excitation),
         FN("cfgPML", js_cfg_pml),
         FN("cfgMur", js_cfg_mur),

@@ -57,9 +57,13 @@ def stress_grid(n=256, seed=STRESS_SEED, pml=8, nfreq=16, n_boxes=40, n_plates=1
             "seed": seed, "n_pml_cells": n_pml, "pml": pml}
 
 
-def configure_engine(eng, w, nf2ff=True):
-    """Configure a WebGPUEngine-style engine with a workload dictionary."""
-    eng.init(w["gridSize"], w["coefficients"])
+def configure_engine(eng, w, nf2ff=True, storage=None):
+    """Configure a WebGPUEngine-style engine with a workload dictionary
+    (storage: field storage of the HIP engine, "fp32" or "fp16")."""
+    if storage is None:
+        eng.init(w["gridSize"], w["coefficients"])
+    else:
+        eng.init(w["gridSize"], w["coefficients"], storage=storage)
     eng.configurePML(w["pmlRegions"])
     eng.configureExcitation(w["excitation"])
     if nf2ff:

@@ -192,9 +192,22 @@ enum {
     EFS_PATH_COEF_IDX = 2,  /* core coefficients as byte indices + tables */
     EFS_PATH_UPML_4 = 4,    /* UPML handled by the 4-wide sweeps */
     EFS_PATH_FUSED = 8,     /* one launch per sweep for core and UPML cells */
-    EFS_PATH_UPML_IDX = 16  /* UPML coefficients as byte indices + tables */
+    EFS_PATH_UPML_IDX = 16, /* UPML coefficients as byte indices + tables */
+    EFS_PATH_F16 = 32       /* FP16 field storage */
 };
 int efs_path_flags(const efs_ctx *ctx, uint32_t *flags);
+
+/* Field storage (no counterpart in the reference, whose fields are FP32):
+ * EFS_STORAGE_F32 (default), or EFS_STORAGE_F16 -- volt and curr held as
+ * FP16 with a power-of-two scale per component, updated in FP32 registers;
+ * the scales follow the field range on the device (checked every 32 steps).
+ * Takes effect at the next efs_init.  FP16 needs Nz % 4 == 0 and UPML
+ * z-bounds that are multiples of 4.  Every read returns FP32 values. */
+enum { EFS_STORAGE_F32 = 0, EFS_STORAGE_F16 = 1 };
+int efs_set_storage(efs_ctx *ctx, int storage);
+/* Current scales: volt components 0..2, curr 0..2 (value = stored * scale;
+ * all 1 with FP32 storage). */
+int efs_field_scales(efs_ctx *ctx, float *scales);
 
 /* Kernel timing with HIP events on the engine's stream. */
 enum {
