@@ -1,6 +1,7 @@
 """Build steps for the native parts of efs_amd.
 
-``build_engine()`` compiles the HIP engine (efs_amd/csrc/efs_engine.hip) for
+``build_engine()`` compiles the HIP engine (efs_amd/csrc/efs_engine.hip, the
+single-grid engine, and efs_amd/csrc/efs_batch.hip, the batched form) for
 gfx950 into efs_amd/libefs_engine.so; ``build_napi()`` compiles the Node
 N-API addon (efs_amd/napi/efs_napi.c) into efs_amd/js/efs_engine.node.
 Both are plain compiler invocations so they run on a machine without a GPU.
@@ -11,7 +12,7 @@ import subprocess
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
-ENGINE_SRC = os.path.join(HERE, "csrc", "efs_engine.hip")
+ENGINE_SRCS = [os.path.join(HERE, "csrc", f) for f in ("efs_engine.hip", "efs_batch.hip")]
 ENGINE_LIB = os.path.join(HERE, "libefs_engine.so")
 NAPI_SRC = os.path.join(HERE, "napi", "efs_napi.c")
 NAPI_OUT = os.path.join(HERE, "js", "efs_engine.node")
@@ -34,7 +35,7 @@ def hipcc():
 
 
 def build_engine(verbose=False):
-    cmd = [hipcc(), *HIPCC_FLAGS, "-o", ENGINE_LIB + ".tmp", ENGINE_SRC]
+    cmd = [hipcc(), *HIPCC_FLAGS, "-o", ENGINE_LIB + ".tmp", *ENGINE_SRCS]
     res = subprocess.run(cmd, capture_output=True, text=True)
     if res.returncode != 0:
         raise RuntimeError("engine build failed:\n" + res.stdout + res.stderr)

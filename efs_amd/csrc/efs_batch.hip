@@ -1,0 +1,1086 @@
+// efs_batch.hip -- batched form: many independent grids (instances) advanced
+// together, one launch per half-step for all of them.
+//
+// Implements include/efs_batch.h.  Per instance the arithmetic is the
+// single-grid engine's (the same e4_march / h4_march device code, the same
+// point-kernel formulas) in the same phase order (src/webgpu-fdtd.mjs:
+// 1094-1172): Mur pre, E sweep, Mur post, excitation, Mur apply, H sweep,
+// [FP16 range control], gather, numTS++.
+//
+// Layout.  Every instance owns six device arrays (volt, curr, vv, vi, ii,
+// iv), NIJK with Nz padded to a multiple of 4 so that every instance runs
+// the 4-cells-per-lane sweeps.  Padding cells have zero coefficients (they
+// stay zero), and the last real z plane of the H coefficients is set to
+// ii = 1, iv = 0, which leaves those cells unchanged exactly as the
+// reference's last-plane rule does (src/webgpu-engine.mjs:88); the real
+// cells never read a padding cell otherwise (E reads z-1, H reads z+1 only
+// below the last real plane).
+//
+// Launch shape.  A packed table of blocks (instance, column tile, x range)
+// covers all instances; each block has about the same cost, so one launch
+// balances uneven grids.  The table is ordered by instance, then x, and
+// workgroup ids are remapped per XCD, so the blocks an XCD runs are
+// neighbours in one instance (shared halo planes in its L2).  The step
+// counter lives on the device (excitation and gather read it, the gather
+// kernel advances it), so a block of steps is captured once into a HIP
+// graph and replayed.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/efs_batch.h"
+#include "sweep.hpp"
+
+namespace efs {
+namespace batch {
+
+struct Inst {
+    void* volt;         // T[3*S]
+    void* curr;
+    const float* vv;    // float[3*S] each
+    const float* vi;
+    const float* ii;
+    const float* iv;
+    uint64_t S;         // component stride
+    uint32_t nx, ny, nz;  // nz padded to a multiple of 4
+    uint32_t cols;      // ny * nz
+};
+
+struct Block {
+    uint32_t inst, tile, x0, x1;
+};
+
+constexpr int kPt = 256;
+constexpr uint32_t kGraphSteps = 32;  // steps per captured graph; also the FP16 range interval
+constexpr int kRangeLo = 6, kRangeHi = 12, kRangeTarget = 10, kRangeSibling = 14;
+
+// ------------------------------------------------------------------ sweeps
+
+template <bool HF, typename T>
+__device__ __forceinline__ void bsweep_body(const Inst* __restrict__ insts, const Block* __restrict__ tab,
+                                            uint32_t nblocks, const float* __restrict__ fscale) {
+    const Block d = tab[xcd_remap(blockIdx.x, nblocks)];
+    const Inst I = insts[d.inst];
+    const uint32_t c0 = (d.tile * kSweepThreads + threadIdx.x) * 4u;
+    if (c0 >= I.cols || d.x0 >= d.x1) return;
+    SweepGeom g{};
+    g.nx = I.nx;
+    g.ny = I.ny;
+    g.nz = I.nz;
+    g.cols = I.cols;
+    g.S = I.S;
+    Lane4 w;
+    w.y = c0 / I.nz;
+    w.z0 = c0 - w.y * I.nz;
+    w.x0 = d.x0;
+    w.x1 = d.x1;
+    w.region = -1;
+    const PmlTable pml{};  // no UPML in the batched form: column mask 0, never read
+    const Scales sc = load_scales<T>(fscale + 6 * d.inst, HF);
+    if (HF) {
+        const Coef cf{I.ii, I.iv, nullptr, nullptr, nullptr, nullptr};
+        h4_march<kMarchInterior, false, false, T>(static_cast<T*>(I.curr), static_cast<const T*>(I.volt), cf,
+                                                  nullptr, nullptr, g, pml, nullptr, nullptr, w, 0u, sc);
+    } else {
+        const Coef cf{I.vv, I.vi, nullptr, nullptr, nullptr, nullptr};
+        e4_march<kMarchInterior, false, false, T>(static_cast<T*>(I.volt), static_cast<const T*>(I.curr), cf,
+                                                  nullptr, nullptr, g, pml, nullptr, nullptr, w, 0u, sc);
+    }
+}
+
+// One kernel per (side, storage, occupancy); the autotuner picks W.
+#define EFS_BSWEEP(NAME, HF, T, W)                                                                       \
+    __global__ __launch_bounds__(kSweepThreads) __attribute__((amdgpu_waves_per_eu(W, W))) void NAME( \
+        const Inst* __restrict__ insts, const Block* __restrict__ tab, uint32_t nblocks,                  \
+        const float* __restrict__ fscale) {                                                               \
+        bsweep_body<HF, T>(insts, tab, nblocks, fscale);                                                  \
+    }
+
+EFS_BSWEEP(bsweep_e_w4, false, float, 4)
+EFS_BSWEEP(bsweep_h_w4, true, float, 4)
+EFS_BSWEEP(bsweep_e_w5, false, float, 5)
+EFS_BSWEEP(bsweep_h_w5, true, float, 5)
+EFS_BSWEEP(bsweep_e_w4_h, false, f16, 4)
+EFS_BSWEEP(bsweep_h_w4_h, true, f16, 4)
+EFS_BSWEEP(bsweep_e_w5_h, false, f16, 5)
+EFS_BSWEEP(bsweep_h_w5_h, true, f16, 5)
+#undef EFS_BSWEEP
+
+// ------------------------------------------------------------ point kernels
+
+// A point of an instance: its field array through a FieldView of that
+// instance (3 components S apart, its 3 scales).
+__device__ __forceinline__ FieldView bview(const Inst& I, bool curr, const float* fscale, uint32_t inst,
+                                           uint32_t half) {
+    return FieldView{curr ? I.curr : I.volt, fscale + 6 * inst + (curr ? 3 : 0), I.S, half};
+}
+
+// Mur pre / post / apply (src/webgpu-fdtd.mjs:992-1023), points of all
+// instances in one list.
+__global__ void k_bmur_pre(const Inst* __restrict__ insts, const float* __restrict__ fscale, uint32_t half,
+                           const uint32_t* __restrict__ inst, const float* __restrict__ coeff,
+                           const uint32_t* __restrict__ nidx, const uint32_t* __restrict__ sidx,
+                           float* __restrict__ saved, uint32_t n) {
+    const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    const FieldView v = bview(insts[inst[k]], false, fscale, inst[k], half);
+    saved[k] = v.ld(sidx[k]) - coeff[k] * v.ld(nidx[k]);
+}
+
+__global__ void k_bmur_post(const Inst* __restrict__ insts, const float* __restrict__ fscale, uint32_t half,
+                            const uint32_t* __restrict__ inst, const float* __restrict__ coeff,
+                            const uint32_t* __restrict__ sidx, float* __restrict__ saved, uint32_t n) {
+    const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    const FieldView v = bview(insts[inst[k]], false, fscale, inst[k], half);
+    saved[k] += coeff[k] * v.ld(sidx[k]);
+}
+
+// Apply list: the last entry of every normal index of an instance (the
+// sequential order of the reference's loop), src = its entry number.
+__global__ void k_bmur_apply(const Inst* __restrict__ insts, const float* __restrict__ fscale, uint32_t half,
+                             const uint32_t* __restrict__ inst, const uint32_t* __restrict__ nidx,
+                             const uint32_t* __restrict__ src, const float* __restrict__ saved, uint32_t n) {
+    const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    const FieldView v = bview(insts[inst[k]], false, fscale, inst[k], half);
+    v.st(nidx[k], saved[src[k]]);
+}
+
+// Excitation (src/webgpu-fdtd.mjs:161-183 semantics, as k_excitation) with
+// a signal per instance: point k reads sig[off[k] + e], e < len[k].
+struct ExcPoint {
+    uint32_t inst, gidx, delay, period;
+    uint32_t off, len;
+    float amp;
+    uint32_t pad;
+};
+
+__global__ void k_bexcitation(const Inst* __restrict__ insts, const float* __restrict__ fscale, uint32_t half,
+                              const ExcPoint* __restrict__ pts, const float* __restrict__ sig, uint32_t n,
+                              const uint32_t* __restrict__ ts) {
+    const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    const ExcPoint p = pts[k];
+    int64_t e = (int64_t)*ts - (int64_t)p.delay;
+    if (e < 0) e = 0;
+    if (p.period > 0) e %= (int64_t)p.period;
+    if (e >= (int64_t)p.len) e = 0;
+    bview(insts[p.inst], false, fscale, p.inst, half).add(p.gidx, p.amp * sig[p.off + e]);
+}
+
+// Buffered gather of every instance's probes (one workgroup), then the step
+// counter advances: every thread reads the counter before the barrier, one
+// thread writes it after the second.
+__global__ __launch_bounds__(kPt) void k_bgather_tick(const Inst* __restrict__ insts, const float* __restrict__ fscale,
+                                                     uint32_t half, const uint32_t* __restrict__ inst,
+                                                     const uint32_t* __restrict__ idx, uint32_t n,
+                                                     float* __restrict__ ring, uint32_t interval,
+                                                     uint32_t max_samples, uint32_t* __restrict__ ts) {
+    const uint32_t t = *ts;
+    __syncthreads();
+    if (n && interval && (t + 1) % interval == 0) {
+        const uint32_t s = (t + 1) / interval - 1;
+        if (s < max_samples) {
+            float* out = ring + (size_t)s * 2 * n;
+            for (uint32_t k = threadIdx.x; k < n; k += blockDim.x) {
+                const Inst& I = insts[inst[k]];
+                out[2 * k] = bview(I, false, fscale, inst[k], half).ld(idx[k]);
+                out[2 * k + 1] = bview(I, true, fscale, inst[k], half).ld(idx[k]);
+            }
+        }
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) ts[0] = t + 1;
+}
+
+// FP16 range control per instance (the single-grid k_absmax16 /
+// k_range_plan / k_range_apply, one segment per instance and component).
+struct Seg {
+    f16* p;     // first stored value of the component
+    uint32_t n; // values (multiple of 4)
+    uint32_t slot;  // inst * 6 + k
+};
+
+__global__ __launch_bounds__(256) void k_babsmax(const Seg* __restrict__ segs, unsigned int* __restrict__ amax) {
+    const Seg s = segs[blockIdx.y];
+    float m = 0.f;
+    for (uint32_t i = 4 * (blockIdx.x * blockDim.x + threadIdx.x); i < s.n; i += 4 * gridDim.x * blockDim.x) {
+        const v4 v = __builtin_convertvector(*reinterpret_cast<const h4*>(s.p + i), v4);
+        m = fmaxf(m, fmaxf(fmaxf(fabsf(v.x), fabsf(v.y)), fmaxf(fabsf(v.z), fabsf(v.w))));
+        if (v.x != v.x || v.y != v.y || v.z != v.z || v.w != v.w) m = __builtin_nanf("");
+    }
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+        const float o = __shfl_down(m, off, 64);
+        m = (o != o || m != m) ? __builtin_nanf("") : fmaxf(m, o);
+    }
+    if ((threadIdx.x & 63u) == 0) atomicMax(amax + s.slot, __float_as_uint(m));
+}
+
+__global__ void k_bplan(unsigned int* __restrict__ amax, float* __restrict__ fscale,
+                        const float* __restrict__ smin, float* __restrict__ factor,
+                        const uint32_t* __restrict__ src_end, uint32_t ninst, const uint32_t* __restrict__ ts) {
+    const uint32_t q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= ninst) return;
+    const uint32_t active = (uint64_t)*ts + 1 < (uint64_t)src_end[q];
+    unsigned int* am = amax + 6 * q;
+    float* sc = fscale + 6 * q;
+    const float* sm = smin + 6 * q;
+    float* fa = factor + 6 * q;
+    float m[6], arr[2] = {0.f, 0.f};
+    for (int k = 0; k < 6; ++k) {
+        m[k] = __uint_as_float(am[k]);
+        am[k] = 0u;
+        if (m[k] < __builtin_huge_valf()) arr[k / 3] = fmaxf(arr[k / 3], m[k] * sc[k]);
+    }
+    for (int k = 0; k < 6; ++k) {
+        float f = 1.0f;
+        if (m[k] > 0.0f && m[k] < __builtin_huge_valf()) {
+            const int e = ilogbf(m[k]);
+            if (e < kRangeLo || e >= kRangeHi) {
+                int shift = kRangeTarget - e;
+                if (shift > 0) {
+                    const int floor_e = active ? ilogbf(sm[k]) : ilogbf(arr[k / 3]) - kRangeSibling;
+                    shift = min(shift, max(0, ilogbf(sc[k]) - floor_e));
+                }
+                f = ldexpf(1.0f, shift);
+                sc[k] = ldexpf(sc[k], -shift);
+            }
+        }
+        fa[k] = f;
+    }
+}
+
+__global__ __launch_bounds__(256) void k_bapply(const Seg* __restrict__ segs, const float* __restrict__ factor) {
+    const Seg s = segs[blockIdx.y];
+    const float f = factor[s.slot];
+    if (f == 1.0f) return;
+    for (uint32_t i = 4 * (blockIdx.x * blockDim.x + threadIdx.x); i < s.n; i += 4 * gridDim.x * blockDim.x) {
+        h4* q = reinterpret_cast<h4*>(s.p + i);
+        *q = __builtin_convertvector(clamp16(__builtin_convertvector(*q, v4) * f), h4);
+    }
+}
+
+}  // namespace batch
+}  // namespace efs
+
+using namespace efs;
+using namespace efs::batch;
+
+namespace {
+
+std::string g_batch_create_error;
+
+struct HostInst {
+    uint32_t nx, ny, nz, nzp;
+    uint64_t N, S;
+    void *volt = nullptr, *curr = nullptr;
+    float *vv = nullptr, *vi = nullptr, *ii = nullptr, *iv = nullptr;
+    // configured point lists (instance-local device indices)
+    std::vector<float> sig;
+    std::vector<ExcPoint> exc;  // off relative to this instance's signal
+    uint32_t exc_period = 0;
+    std::vector<float> mur_c;
+    std::vector<uint32_t> mur_n, mur_s;
+    std::vector<uint32_t> gat;
+    float scale0[6] = {1, 1, 1, 1, 1, 1};  // FP16 initial scales (also the range floor while the source runs)
+    uint32_t src_end = 0;
+};
+
+}  // namespace
+
+struct efs_batch {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    std::string err;
+    bool final = false;
+    bool half = false;
+    std::vector<HostInst> inst;
+    uint64_t cells = 0;
+    uint32_t numTS = 0;
+
+    Inst* d_inst = nullptr;
+    Block* d_tab = nullptr;
+    uint32_t nblocks = 0;
+    uint32_t waves = 4;
+    uint32_t* d_ts = nullptr;
+    float* d_fscale = nullptr;
+    float* d_smin = nullptr;
+    float* d_factor = nullptr;
+    unsigned int* d_amax = nullptr;
+    uint32_t* d_src_end = nullptr;
+    Seg* d_segs = nullptr;
+    uint32_t nsegs = 0;
+    uint32_t seg_chunks = 1;
+
+    // Mur
+    uint32_t mur_n = 0, mur_an = 0;
+    uint32_t *d_mur_inst = nullptr, *d_mur_n = nullptr, *d_mur_s = nullptr;
+    uint32_t *d_mur_ainst = nullptr, *d_mur_an = nullptr, *d_mur_asrc = nullptr;  // apply list
+    float *d_mur_c = nullptr, *d_mur_saved = nullptr;
+    // excitation
+    uint32_t exc_n = 0;
+    ExcPoint* d_exc = nullptr;
+    float* d_sig = nullptr;
+    // gather
+    uint32_t g_n = 0, g_interval = 0, g_max = 0;
+    std::vector<uint32_t> g_off;  // per instance: first gather entry
+    uint32_t *d_g_inst = nullptr, *d_g_idx = nullptr;
+    float* d_ring = nullptr;
+
+    // graph of kGraphSteps steps
+    hipGraphExec_t graph = nullptr;
+    bool graph_ok = false;
+
+    // profiling
+    bool prof = false;
+    std::vector<hipEvent_t> ev_pool;
+    struct Pend {
+        int side;
+        hipEvent_t a, b;
+    };
+    std::vector<Pend> pend;
+    double prof_ms[2] = {0, 0};
+    uint32_t prof_n[2] = {0, 0};
+};
+
+namespace {
+
+int bfail(efs_batch* b, int code, const char* fmt, ...) {
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    if (b) b->err = buf;
+    else g_batch_create_error = buf;
+    return code;
+}
+
+#define BHIP(b, expr)                                                                             \
+    do {                                                                                         \
+        hipError_t e_ = (expr);                                                                  \
+        if (e_ != hipSuccess) return bfail(b, EFS_ERR_HIP, "%s: %s", #expr, hipGetErrorString(e_)); \
+    } while (0)
+#define BTRY(expr)               \
+    do {                         \
+        int rc_ = (expr);        \
+        if (rc_) return rc_;     \
+    } while (0)
+
+template <class T>
+int balloc(efs_batch* b, T*& p, size_t n) {
+    if (p) hipFree(p);
+    p = nullptr;
+    hipError_t e = hipMalloc((void**)&p, (n ? n : 1) * sizeof(T));
+    if (e != hipSuccess) {
+        p = nullptr;
+        return bfail(b, e == hipErrorOutOfMemory ? EFS_ERR_NOMEM : EFS_ERR_HIP, "hipMalloc(%zu bytes): %s",
+                     (n ? n : 1) * sizeof(T), hipGetErrorString(e));
+    }
+    return EFS_OK;
+}
+
+template <class T>
+int bupload(efs_batch* b, T*& p, const T* src, size_t n) {
+    BTRY(balloc(b, p, n));
+    if (n) BHIP(b, hipMemcpy(p, src, n * sizeof(T), hipMemcpyHostToDevice));
+    return EFS_OK;
+}
+
+template <class T>
+void bfree(T*& p) {
+    if (p) hipFree((void*)p);
+    p = nullptr;
+}
+
+uint32_t nblk(size_t n) { return (uint32_t)((n + kPt - 1) / kPt); }
+
+// Instance-local interface index (component*N + x*Ny*Nz + y*Nz + z) ->
+// device index (component*S + x*Ny*Nzp + y*Nzp + z).
+uint32_t btr(const HostInst& h, uint64_t g) {
+    const uint64_t n = g / h.N, r = g - n * h.N;
+    const uint64_t z = r % h.nz, xy = r / h.nz;
+    return (uint32_t)(n * h.S + xy * h.nzp + z);
+}
+
+// Equal-cost block table: every (instance, column tile) is cut along x into
+// chunks of about total/want cells, ordered by instance, then x.
+std::vector<Block> plan_blocks(const efs_batch* b, uint32_t want) {
+    double total = 0.0;
+    for (const HostInst& h : b->inst) total += (double)h.nx * h.ny * h.nzp;
+    const double target = total / std::max(1u, want);
+    std::vector<Block> tab;
+    for (uint32_t q = 0; q < (uint32_t)b->inst.size(); ++q) {
+        const HostInst& h = b->inst[q];
+        const uint64_t cols = (uint64_t)h.ny * h.nzp;
+        const uint32_t tcells = 4 * kSweepThreads;
+        const uint32_t T = (uint32_t)((cols + tcells - 1) / tcells);
+        std::vector<Block> mine;
+        for (uint32_t t = 0; t < T; ++t) {
+            const double cells = (double)(std::min<uint64_t>(cols, (uint64_t)(t + 1) * tcells) - (uint64_t)t * tcells);
+            const double cost = cells * h.nx;
+            const uint32_t n = (uint32_t)std::max(1.0, std::min((double)std::max(1u, h.nx / 2),
+                                                                std::floor(cost / target + 0.5)));
+            for (uint32_t j = 0; j < n; ++j) {
+                const uint32_t x0 = (uint32_t)((uint64_t)h.nx * j / n), x1 = (uint32_t)((uint64_t)h.nx * (j + 1) / n);
+                if (x1 > x0) mine.push_back(Block{q, t, x0, x1});
+            }
+        }
+        std::stable_sort(mine.begin(), mine.end(), [](const Block& a, const Block& c) {
+            return a.x0 + a.x1 != c.x0 + c.x1 ? a.x0 + a.x1 < c.x0 + c.x1 : a.tile < c.tile;
+        });
+        tab.insert(tab.end(), mine.begin(), mine.end());
+    }
+    return tab;
+}
+
+void drop_graph(efs_batch* b);
+
+// (Re)build the block table; a captured graph holds the old table, so it is
+// dropped (build_graph captures a new one).
+int set_blocks(efs_batch* b, uint32_t want) {
+    drop_graph(b);
+    std::vector<Block> tab = plan_blocks(b, want);
+    BHIP(b, hipStreamSynchronize(b->stream));
+    BTRY(bupload(b, b->d_tab, tab.data(), tab.size()));
+    b->nblocks = (uint32_t)tab.size();
+    return EFS_OK;
+}
+
+hipEvent_t take_event(efs_batch* b) {
+    if (!b->ev_pool.empty()) {
+        hipEvent_t e = b->ev_pool.back();
+        b->ev_pool.pop_back();
+        return e;
+    }
+    hipEvent_t e = nullptr;
+    if (hipEventCreate(&e) != hipSuccess) return nullptr;
+    return e;
+}
+
+void launch_sweep(efs_batch* b, bool hf) {
+    hipEvent_t ea = nullptr;
+    if (b->prof) {
+        ea = take_event(b);
+        if (ea) hipEventRecord(ea, b->stream);
+    }
+    auto k = hf ? (b->half ? (b->waves == 5 ? bsweep_h_w5_h : bsweep_h_w4_h) : (b->waves == 5 ? bsweep_h_w5 : bsweep_h_w4))
+                : (b->half ? (b->waves == 5 ? bsweep_e_w5_h : bsweep_e_w4_h) : (b->waves == 5 ? bsweep_e_w5 : bsweep_e_w4));
+    hipLaunchKernelGGL(k, dim3(b->nblocks), dim3(kSweepThreads), 0, b->stream, (const Inst*)b->d_inst,
+                       (const Block*)b->d_tab, b->nblocks, (const float*)b->d_fscale);
+    if (b->prof && ea) {
+        hipEvent_t eb = take_event(b);
+        if (eb) {
+            hipEventRecord(eb, b->stream);
+            b->pend.push_back({hf ? 1 : 0, ea, eb});
+        } else {
+            b->ev_pool.push_back(ea);
+        }
+    }
+}
+
+// One step; `range` adds the FP16 range control before the gather (steps
+// with (t + 1) % kGraphSteps == 0).
+void launch_step(efs_batch* b, bool range) {
+    const uint32_t half = b->half ? 1u : 0u;
+    if (b->mur_n)
+        hipLaunchKernelGGL(k_bmur_pre, dim3(nblk(b->mur_n)), dim3(kPt), 0, b->stream, (const Inst*)b->d_inst,
+                           (const float*)b->d_fscale, half, (const uint32_t*)b->d_mur_inst, (const float*)b->d_mur_c,
+                           (const uint32_t*)b->d_mur_n, (const uint32_t*)b->d_mur_s, b->d_mur_saved, b->mur_n);
+    launch_sweep(b, false);
+    if (b->mur_n)
+        hipLaunchKernelGGL(k_bmur_post, dim3(nblk(b->mur_n)), dim3(kPt), 0, b->stream, (const Inst*)b->d_inst,
+                           (const float*)b->d_fscale, half, (const uint32_t*)b->d_mur_inst, (const float*)b->d_mur_c,
+                           (const uint32_t*)b->d_mur_s, b->d_mur_saved, b->mur_n);
+    if (b->exc_n)
+        hipLaunchKernelGGL(k_bexcitation, dim3(nblk(b->exc_n)), dim3(kPt), 0, b->stream, (const Inst*)b->d_inst,
+                           (const float*)b->d_fscale, half, (const ExcPoint*)b->d_exc, (const float*)b->d_sig,
+                           b->exc_n, (const uint32_t*)b->d_ts);
+    if (b->mur_n)
+        hipLaunchKernelGGL(k_bmur_apply, dim3(nblk(b->mur_an)), dim3(kPt), 0, b->stream, (const Inst*)b->d_inst,
+                           (const float*)b->d_fscale, half, (const uint32_t*)b->d_mur_ainst,
+                           (const uint32_t*)b->d_mur_an, (const uint32_t*)b->d_mur_asrc,
+                           (const float*)b->d_mur_saved, b->mur_an);
+    launch_sweep(b, true);
+    if (b->half && range) {
+        const dim3 grid(b->seg_chunks, b->nsegs);
+        hipLaunchKernelGGL(k_babsmax, grid, dim3(256), 0, b->stream, (const Seg*)b->d_segs, b->d_amax);
+        hipLaunchKernelGGL(k_bplan, dim3(nblk(b->inst.size())), dim3(kPt), 0, b->stream, b->d_amax, b->d_fscale,
+                           (const float*)b->d_smin, b->d_factor, (const uint32_t*)b->d_src_end,
+                           (uint32_t)b->inst.size(), (const uint32_t*)b->d_ts);
+        hipLaunchKernelGGL(k_bapply, grid, dim3(256), 0, b->stream, (const Seg*)b->d_segs,
+                           (const float*)b->d_factor);
+    }
+    hipLaunchKernelGGL(k_bgather_tick, dim3(1), dim3(kPt), 0, b->stream, (const Inst*)b->d_inst,
+                       (const float*)b->d_fscale, half, (const uint32_t*)b->d_g_inst, (const uint32_t*)b->d_g_idx,
+                       b->g_n, b->d_ring, b->g_interval, b->g_max, b->d_ts);
+}
+
+void drop_graph(efs_batch* b) {
+    if (b->graph) hipGraphExecDestroy(b->graph);
+    b->graph = nullptr;
+    b->graph_ok = false;
+}
+
+// Capture kGraphSteps steps (the last one with the FP16 range control): a
+// replay starting at a multiple of kGraphSteps is exactly those steps.
+int build_graph(efs_batch* b) {
+    drop_graph(b);
+    BHIP(b, hipStreamSynchronize(b->stream));
+    hipGraph_t g = nullptr;
+    BHIP(b, hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
+    const bool prof = b->prof;
+    b->prof = false;
+    for (uint32_t s = 0; s < kGraphSteps; ++s) launch_step(b, s + 1 == kGraphSteps);
+    b->prof = prof;
+    hipError_t e = hipStreamEndCapture(b->stream, &g);
+    if (e != hipSuccess) return bfail(b, EFS_ERR_HIP, "graph capture: %s", hipGetErrorString(e));
+    e = hipGraphInstantiate(&b->graph, g, nullptr, nullptr, 0);
+    hipGraphDestroy(g);
+    if (e != hipSuccess) {
+        b->graph = nullptr;
+        return bfail(b, EFS_ERR_HIP, "graph instantiate: %s", hipGetErrorString(e));
+    }
+    b->graph_ok = true;
+    return EFS_OK;
+}
+
+int run_steps(efs_batch* b, uint32_t n) {
+    uint32_t done = 0;
+    while (done < n) {
+        const uint32_t t = b->numTS;
+        if (b->graph_ok && !b->prof && t % kGraphSteps == 0 && n - done >= kGraphSteps) {
+            BHIP(b, hipGraphLaunch(b->graph, b->stream));
+            b->numTS += kGraphSteps;
+            done += kGraphSteps;
+        } else {
+            launch_step(b, (t + 1) % kGraphSteps == 0);
+            BHIP(b, hipGetLastError());
+            b->numTS++;
+            done++;
+        }
+    }
+    return EFS_OK;
+}
+
+int need_open(efs_batch* b) {
+    if (!b) return EFS_ERR_STATE;
+    if (b->final) return bfail(b, EFS_ERR_STATE, "batch already finalized");
+    return EFS_OK;
+}
+
+int need_final(efs_batch* b) {
+    if (!b) return EFS_ERR_STATE;
+    if (!b->final) return bfail(b, EFS_ERR_STATE, "batch not finalized: call efs_batch_finalize() first");
+    return EFS_OK;
+}
+
+int need_inst(efs_batch* b, uint32_t id) {
+    if (id >= b->inst.size()) return bfail(b, EFS_ERR_ARG, "instance %u does not exist (%zu added)", id, b->inst.size());
+    return EFS_OK;
+}
+
+// Initial FP16 scales of one instance from its source (the single-grid
+// engine's rule: the largest injected value lands near 2^kRangeTarget).
+void source_scales(HostInst& h) {
+    float smax = 0.f, va = 0.f;
+    for (float s : h.sig) smax = std::max(smax, std::fabs(s));
+    for (const ExcPoint& p : h.exc) va = std::max(va, std::fabs(p.amp));
+    const float mv = smax * va;
+    if (!(mv > 0.f && std::isfinite(mv))) return;
+    const float vs = std::ldexp(1.0f, std::ilogb(mv) - kRangeTarget);
+    for (int k = 0; k < 3; ++k) {
+        h.scale0[k] = vs;
+        h.scale0[3 + k] = std::ldexp(vs, -8);
+    }
+}
+
+int zero_state(efs_batch* b) {
+    BHIP(b, hipStreamSynchronize(b->stream));
+    for (HostInst& h : b->inst) {
+        const size_t fb = (b->half ? 2 : 4) * 3 * h.S;
+        BHIP(b, hipMemset(h.volt, 0, fb));
+        BHIP(b, hipMemset(h.curr, 0, fb));
+    }
+    if (b->mur_n) BHIP(b, hipMemset(b->d_mur_saved, 0, b->mur_n * sizeof(float)));
+    if (b->g_n && b->g_max) BHIP(b, hipMemset(b->d_ring, 0, (size_t)b->g_max * 2 * b->g_n * sizeof(float)));
+    BHIP(b, hipMemset(b->d_ts, 0, sizeof(uint32_t)));
+    std::vector<float> sc(6 * b->inst.size());
+    for (size_t q = 0; q < b->inst.size(); ++q)
+        for (int k = 0; k < 6; ++k) sc[6 * q + k] = b->inst[q].scale0[k];
+    BHIP(b, hipMemcpy(b->d_fscale, sc.data(), sc.size() * sizeof(float), hipMemcpyHostToDevice));
+    BHIP(b, hipMemcpy(b->d_smin, sc.data(), sc.size() * sizeof(float), hipMemcpyHostToDevice));
+    BHIP(b, hipMemset(b->d_amax, 0, sc.size() * sizeof(unsigned int)));
+    b->numTS = 0;
+    return EFS_OK;
+}
+
+int drain(efs_batch* b) {
+    if (b->pend.empty()) return EFS_OK;
+    BHIP(b, hipStreamSynchronize(b->stream));
+    for (auto& p : b->pend) {
+        float ms = 0.f;
+        if (hipEventElapsedTime(&ms, p.a, p.b) == hipSuccess) {
+            b->prof_ms[p.side] += ms;
+            b->prof_n[p.side]++;
+        }
+        b->ev_pool.push_back(p.a);
+        b->ev_pool.push_back(p.b);
+    }
+    b->pend.clear();
+    return EFS_OK;
+}
+
+}  // namespace
+
+// ===================================================================== C ABI
+
+extern "C" {
+
+int efs_batch_create(efs_batch** out, int device) {
+    if (!out) return bfail(nullptr, EFS_ERR_ARG, "out is NULL");
+    *out = nullptr;
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || n == 0) return bfail(nullptr, EFS_ERR_NO_DEVICE, "no HIP device");
+    if (device < 0 || device >= n) return bfail(nullptr, EFS_ERR_ARG, "device %d out of range (%d devices)", device, n);
+    efs_batch* b = new efs_batch();
+    b->device = device;
+    if (hipSetDevice(device) != hipSuccess || hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess) {
+        delete b;
+        return bfail(nullptr, EFS_ERR_HIP, "stream creation failed");
+    }
+    *out = b;
+    return EFS_OK;
+}
+
+void efs_batch_destroy(efs_batch* b) {
+    if (!b) return;
+    hipSetDevice(b->device);
+    if (b->stream) hipStreamSynchronize(b->stream);
+    drop_graph(b);
+    for (HostInst& h : b->inst) {
+        bfree(h.volt);
+        bfree(h.curr);
+        bfree(h.vv);
+        bfree(h.vi);
+        bfree(h.ii);
+        bfree(h.iv);
+    }
+    bfree(b->d_inst);
+    bfree(b->d_tab);
+    bfree(b->d_ts);
+    bfree(b->d_fscale);
+    bfree(b->d_smin);
+    bfree(b->d_factor);
+    bfree(b->d_amax);
+    bfree(b->d_src_end);
+    bfree(b->d_segs);
+    bfree(b->d_mur_inst);
+    bfree(b->d_mur_n);
+    bfree(b->d_mur_s);
+    bfree(b->d_mur_c);
+    bfree(b->d_mur_saved);
+    bfree(b->d_mur_ainst);
+    bfree(b->d_mur_an);
+    bfree(b->d_mur_asrc);
+    bfree(b->d_exc);
+    bfree(b->d_sig);
+    bfree(b->d_g_inst);
+    bfree(b->d_g_idx);
+    bfree(b->d_ring);
+    for (auto& p : b->pend) {
+        hipEventDestroy(p.a);
+        hipEventDestroy(p.b);
+    }
+    for (hipEvent_t e : b->ev_pool) hipEventDestroy(e);
+    if (b->stream) hipStreamDestroy(b->stream);
+    delete b;
+}
+
+const char* efs_batch_last_error(const efs_batch* b) { return b ? b->err.c_str() : g_batch_create_error.c_str(); }
+
+int efs_batch_set_storage(efs_batch* b, int storage) {
+    BTRY(need_open(b));
+    if (storage != EFS_STORAGE_F32 && storage != EFS_STORAGE_F16)
+        return bfail(b, EFS_ERR_ARG, "unknown storage %d", storage);
+    if (!b->inst.empty()) return bfail(b, EFS_ERR_STATE, "set the storage before adding instances");
+    b->half = storage == EFS_STORAGE_F16;
+    return EFS_OK;
+}
+
+int efs_batch_add(efs_batch* b, uint32_t nx, uint32_t ny, uint32_t nz, const float* vv, const float* vi,
+                  const float* ii, const float* iv, uint32_t* id) {
+    BTRY(need_open(b));
+    if (nx == 0 || ny == 0 || nz == 0) return bfail(b, EFS_ERR_ARG, "grid size must be positive");
+    if (!vv || !vi || !ii || !iv) return bfail(b, EFS_ERR_ARG, "coefficient arrays are required");
+    HostInst h;
+    h.nx = nx;
+    h.ny = ny;
+    h.nz = nz;
+    h.nzp = (nz + 3) & ~3u;
+    h.N = (uint64_t)nx * ny * nz;
+    const uint64_t Np = (uint64_t)nx * ny * h.nzp;
+    h.S = ((Np + 255) & ~(uint64_t)255) + 1088;
+    if (3 * h.S >= (1ull << 32)) return bfail(b, EFS_ERR_ARG, "instance too large for 32-bit indices");
+    if ((uint64_t)ny * h.nzp >= (1ull << 32) / 4) return bfail(b, EFS_ERR_ARG, "plane too large");
+    BHIP(b, hipSetDevice(b->device));
+    // padded copies: pad cells zero; H last real z plane ii = 1, iv = 0
+    std::vector<float> buf(3 * h.S);
+    const float* src[4] = {vv, vi, ii, iv};
+    float** dst[4] = {&h.vv, &h.vi, &h.ii, &h.iv};
+    for (int a = 0; a < 4; ++a) {
+        std::fill(buf.begin(), buf.end(), 0.0f);
+        for (uint32_t n = 0; n < 3; ++n)
+            for (uint64_t xy = 0; xy < (uint64_t)nx * ny; ++xy) {
+                const float* s = src[a] + n * h.N + xy * nz;
+                float* d = buf.data() + n * h.S + xy * h.nzp;
+                std::memcpy(d, s, nz * sizeof(float));
+                if (a >= 2) d[nz - 1] = a == 2 ? 1.0f : 0.0f;
+            }
+        float* p = nullptr;
+        const int rc = bupload(b, p, buf.data(), buf.size());
+        if (rc) {
+            for (int k = 0; k < a; ++k) bfree(*dst[k]);
+            return rc;
+        }
+        *dst[a] = p;
+    }
+    const size_t fb = (b->half ? 2 : 4) * 3 * h.S;
+    hipError_t e1 = hipMalloc(&h.volt, fb), e2 = hipMalloc(&h.curr, fb);
+    if (e1 != hipSuccess || e2 != hipSuccess) {
+        bfree(h.volt);
+        bfree(h.curr);
+        for (int k = 0; k < 4; ++k) bfree(*dst[k]);
+        return bfail(b, EFS_ERR_NOMEM, "field allocation failed");
+    }
+    b->inst.push_back(h);
+    b->cells += h.N;
+    if (id) *id = (uint32_t)b->inst.size() - 1;
+    return EFS_OK;
+}
+
+int efs_batch_cfg_excitation(efs_batch* b, uint32_t id, const float* signal, uint32_t len, const uint32_t* delay,
+                             const float* amp, const uint32_t* dir, const uint32_t* pos, uint32_t n,
+                             uint32_t period) {
+    BTRY(need_open(b));
+    BTRY(need_inst(b, id));
+    HostInst& h = b->inst[id];
+    if (n == 0) {
+        h.exc.clear();
+        h.sig.clear();
+        return EFS_OK;
+    }
+    if (!signal || len == 0) return bfail(b, EFS_ERR_ARG, "excitation signal must not be empty");
+    if (!delay || !amp || !dir || !pos) return bfail(b, EFS_ERR_ARG, "excitation arrays are required");
+    std::vector<ExcPoint> pts(n);
+    for (uint32_t k = 0; k < n; ++k) {
+        if (dir[k] > 2) return bfail(b, EFS_ERR_ARG, "excitation %u: dir %u out of range", k, dir[k]);
+        if (pos[k] >= h.N) return bfail(b, EFS_ERR_ARG, "excitation %u: pos %u out of range", k, pos[k]);
+        pts[k] = ExcPoint{id, btr(h, (uint64_t)dir[k] * h.N + pos[k]), delay[k], period, 0, len, amp[k], 0};
+    }
+    h.sig.assign(signal, signal + len);
+    h.exc = pts;
+    h.exc_period = period;
+    // first step at which no sample is injected any more (the FP16 range
+    // floor follows the source until then)
+    uint64_t end = 0;
+    if (period == 0)
+        for (uint32_t k = 0; k < n; ++k) end = std::max<uint64_t>(end, (uint64_t)delay[k] + len);
+    else
+        end = 0xffffffffu;
+    h.src_end = (uint32_t)std::min<uint64_t>(end, 0xffffffffu);
+    return EFS_OK;
+}
+
+int efs_batch_cfg_mur(efs_batch* b, uint32_t id, uint32_t n, const float* coeff, const uint32_t* normal_idx,
+                      const uint32_t* shifted_idx) {
+    BTRY(need_open(b));
+    BTRY(need_inst(b, id));
+    HostInst& h = b->inst[id];
+    if (n && (!coeff || !normal_idx || !shifted_idx)) return bfail(b, EFS_ERR_ARG, "Mur arrays are required");
+    std::vector<uint32_t> ni(n), si(n);
+    for (uint32_t k = 0; k < n; ++k) {
+        if (normal_idx[k] >= 3 * h.N || shifted_idx[k] >= 3 * h.N)
+            return bfail(b, EFS_ERR_ARG, "Mur point %u: index out of range", k);
+        if (normal_idx[k] / h.N != shifted_idx[k] / h.N)
+            return bfail(b, EFS_ERR_ARG, "Mur point %u: normal and shifted indices of different components", k);
+        ni[k] = btr(h, normal_idx[k]);
+        si[k] = btr(h, shifted_idx[k]);
+    }
+    h.mur_c.assign(coeff, coeff + n);
+    h.mur_n = ni;
+    h.mur_s = si;
+    return EFS_OK;
+}
+
+int efs_batch_cfg_gather(efs_batch* b, uint32_t id, const uint32_t* idx, uint32_t n) {
+    BTRY(need_open(b));
+    BTRY(need_inst(b, id));
+    HostInst& h = b->inst[id];
+    if (n && !idx) return bfail(b, EFS_ERR_ARG, "gather indices are required");
+    std::vector<uint32_t> g(n);
+    for (uint32_t k = 0; k < n; ++k) {
+        if (idx[k] >= 3 * h.N) return bfail(b, EFS_ERR_ARG, "gather index %u out of range", idx[k]);
+        g[k] = btr(h, idx[k]);
+    }
+    h.gat = g;
+    return EFS_OK;
+}
+
+int efs_batch_finalize(efs_batch* b, uint32_t sample_interval, uint32_t max_samples) {
+    BTRY(need_open(b));
+    if (b->inst.empty()) return bfail(b, EFS_ERR_STATE, "no instance added");
+    BHIP(b, hipSetDevice(b->device));
+    const uint32_t ni = (uint32_t)b->inst.size();
+    std::vector<Inst> it(ni);
+    std::vector<uint32_t> src_end(ni);
+    std::vector<Seg> segs;
+    std::vector<uint32_t> mi, mn, ms, gi, gx, ai, an, asrc;
+    std::vector<float> mc, sig;
+    std::vector<ExcPoint> exc;
+    b->g_off.assign(ni + 1, 0);
+    for (uint32_t q = 0; q < ni; ++q) {
+        HostInst& h = b->inst[q];
+        it[q] = Inst{h.volt, h.curr, h.vv, h.vi, h.ii, h.iv, h.S, h.nx, h.ny, h.nzp, h.ny * h.nzp};
+        if (b->half) source_scales(h);
+        src_end[q] = h.src_end;
+        for (uint32_t k = 0; k < 6; ++k)
+            segs.push_back(Seg{static_cast<f16*>(k < 3 ? h.volt : h.curr) + (k % 3) * h.S,
+                               (uint32_t)((uint64_t)h.nx * h.ny * h.nzp), 6 * q + k});
+        const uint32_t m0 = (uint32_t)mc.size();
+        for (size_t k = 0; k < h.mur_c.size(); ++k) {
+            mi.push_back(q);
+            mn.push_back(h.mur_n[k]);
+            ms.push_back(h.mur_s[k]);
+            mc.push_back(h.mur_c[k]);
+        }
+        {
+            // last entry per normal index, in entry order
+            std::vector<uint32_t> keys(h.mur_n);
+            std::sort(keys.begin(), keys.end());
+            keys.erase(std::unique(keys.begin(), keys.end()), keys.end());
+            std::vector<uint8_t> seen(keys.size(), 0);
+            std::vector<uint32_t> win;
+            for (size_t k = h.mur_n.size(); k-- > 0;) {
+                const size_t j = std::lower_bound(keys.begin(), keys.end(), h.mur_n[k]) - keys.begin();
+                if (!seen[j]) {
+                    seen[j] = 1;
+                    win.push_back((uint32_t)k);
+                }
+            }
+            std::reverse(win.begin(), win.end());
+            for (uint32_t k : win) {
+                ai.push_back(q);
+                an.push_back(h.mur_n[k]);
+                asrc.push_back(m0 + k);
+            }
+        }
+        const uint32_t off = (uint32_t)sig.size();
+        sig.insert(sig.end(), h.sig.begin(), h.sig.end());
+        for (ExcPoint p : h.exc) {
+            p.off = off;
+            exc.push_back(p);
+        }
+        b->g_off[q] = (uint32_t)gx.size();
+        for (uint32_t v : h.gat) {
+            gi.push_back(q);
+            gx.push_back(v);
+        }
+    }
+    b->g_off[ni] = (uint32_t)gx.size();
+    if (sig.size() >= (1ull << 32)) return bfail(b, EFS_ERR_ARG, "excitation signals too long");
+    BTRY(bupload(b, b->d_inst, it.data(), it.size()));
+    BTRY(bupload(b, b->d_src_end, src_end.data(), src_end.size()));
+    BTRY(balloc(b, b->d_ts, 1));
+    BTRY(balloc(b, b->d_fscale, 6 * ni));
+    BTRY(balloc(b, b->d_smin, 6 * ni));
+    BTRY(balloc(b, b->d_factor, 6 * ni));
+    BTRY(balloc(b, b->d_amax, 6 * ni));
+    if (b->half) {
+        BTRY(bupload(b, b->d_segs, segs.data(), segs.size()));
+        b->nsegs = (uint32_t)segs.size();
+        uint64_t mx = 0;
+        for (const Seg& s : segs) mx = std::max<uint64_t>(mx, s.n);
+        b->seg_chunks = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(64, (mx / 4 + 255) / 256));
+    }
+    b->mur_n = (uint32_t)mc.size();
+    if (b->mur_n) {
+        BTRY(bupload(b, b->d_mur_inst, mi.data(), mi.size()));
+        BTRY(bupload(b, b->d_mur_n, mn.data(), mn.size()));
+        BTRY(bupload(b, b->d_mur_s, ms.data(), ms.size()));
+        BTRY(bupload(b, b->d_mur_c, mc.data(), mc.size()));
+        BTRY(balloc(b, b->d_mur_saved, mc.size()));
+        BTRY(bupload(b, b->d_mur_ainst, ai.data(), ai.size()));
+        BTRY(bupload(b, b->d_mur_an, an.data(), an.size()));
+        BTRY(bupload(b, b->d_mur_asrc, asrc.data(), asrc.size()));
+    }
+    b->mur_an = (uint32_t)an.size();
+    b->exc_n = (uint32_t)exc.size();
+    if (b->exc_n) {
+        BTRY(bupload(b, b->d_exc, exc.data(), exc.size()));
+        BTRY(bupload(b, b->d_sig, sig.data(), sig.size()));
+    }
+    b->g_n = (uint32_t)gx.size();
+    b->g_interval = sample_interval;
+    b->g_max = b->g_n && sample_interval ? max_samples : 0;
+    if (b->g_n) {
+        BTRY(bupload(b, b->d_g_inst, gi.data(), gi.size()));
+        BTRY(bupload(b, b->d_g_idx, gx.data(), gx.size()));
+        BTRY(balloc(b, b->d_ring, (size_t)std::max(1u, b->g_max) * 2 * b->g_n));
+    }
+    BTRY(set_blocks(b, 2048));
+    b->final = true;
+    BTRY(zero_state(b));
+    return build_graph(b);
+}
+
+int efs_batch_reset(efs_batch* b) {
+    BTRY(need_final(b));
+    BHIP(b, hipSetDevice(b->device));
+    return zero_state(b);
+}
+
+int efs_batch_tune(efs_batch* b, uint32_t steps, uint32_t* blocks_out, uint32_t* waves_out) {
+    BTRY(need_final(b));
+    BHIP(b, hipSetDevice(b->device));
+    if (steps == 0) steps = 8;
+    static const uint32_t kWant[] = {512, 1024, 2048, 4096, 8192};
+    static const uint32_t kWaves[] = {4, 5};
+    double best = 1e300;
+    uint32_t best_w = 2048, best_v = 4;
+    const bool prof = b->prof;
+    for (uint32_t v : kWaves)
+        for (uint32_t want : kWant) {
+            b->waves = v;
+            BTRY(set_blocks(b, want));
+            BTRY(zero_state(b));
+            b->prof = true;
+            b->prof_ms[0] = b->prof_ms[1] = 0.0;
+            b->prof_n[0] = b->prof_n[1] = 0;
+            BTRY(run_steps(b, 2));  // warm
+            BTRY(drain(b));
+            b->prof_ms[0] = b->prof_ms[1] = 0.0;
+            b->prof_n[0] = b->prof_n[1] = 0;
+            BTRY(run_steps(b, steps));
+            BTRY(drain(b));
+            const double t = b->prof_ms[0] + b->prof_ms[1];
+            if (t < best) {
+                best = t;
+                best_w = want;
+                best_v = v;
+            }
+        }
+    b->prof = prof;
+    b->prof_ms[0] = b->prof_ms[1] = 0.0;
+    b->prof_n[0] = b->prof_n[1] = 0;
+    b->waves = best_v;
+    BTRY(set_blocks(b, best_w));
+    BTRY(zero_state(b));
+    BTRY(build_graph(b));
+    if (blocks_out) *blocks_out = b->nblocks;
+    if (waves_out) *waves_out = b->waves;
+    return EFS_OK;
+}
+
+int efs_batch_iterate(efs_batch* b, uint32_t n) {
+    BTRY(need_final(b));
+    BHIP(b, hipSetDevice(b->device));
+    return run_steps(b, n);
+}
+
+int efs_batch_sync(efs_batch* b) {
+    if (!b) return EFS_ERR_STATE;
+    BHIP(b, hipStreamSynchronize(b->stream));
+    return EFS_OK;
+}
+
+int efs_batch_num_ts(efs_batch* b, uint32_t* out) {
+    if (!b || !out) return EFS_ERR_ARG;
+    *out = b->numTS;
+    return EFS_OK;
+}
+
+int efs_batch_read_gather(efs_batch* b, uint32_t id, float* out, uint32_t* num_samples) {
+    BTRY(need_final(b));
+    BTRY(need_inst(b, id));
+    const uint32_t ns = b->g_interval ? std::min(b->g_max, b->numTS / b->g_interval) : 0;
+    if (num_samples) *num_samples = ns;
+    if (!out || ns == 0) return EFS_OK;
+    const uint32_t k0 = b->g_off[id], k1 = b->g_off[id + 1];
+    const uint32_t w = k1 - k0;
+    if (w == 0) return EFS_OK;
+    BHIP(b, hipSetDevice(b->device));
+    BHIP(b, hipStreamSynchronize(b->stream));
+    // one strided copy: sample rows of 2*g_n floats, this instance's columns
+    BHIP(b, hipMemcpy2D(out, 2 * w * sizeof(float), b->d_ring + 2 * k0, 2 * (size_t)b->g_n * sizeof(float),
+                        2 * w * sizeof(float), ns, hipMemcpyDeviceToHost));
+    return EFS_OK;
+}
+
+int efs_batch_get_fields(efs_batch* b, uint32_t id, float* volt, float* curr) {
+    BTRY(need_final(b));
+    BTRY(need_inst(b, id));
+    if (!volt || !curr) return bfail(b, EFS_ERR_ARG, "output arrays are required");
+    BHIP(b, hipSetDevice(b->device));
+    BHIP(b, hipStreamSynchronize(b->stream));
+    const HostInst& h = b->inst[id];
+    float sc[6] = {1, 1, 1, 1, 1, 1};
+    if (b->half) BHIP(b, hipMemcpy(sc, b->d_fscale + 6 * id, sizeof sc, hipMemcpyDeviceToHost));
+    for (int a = 0; a < 2; ++a) {
+        float* dst = a ? curr : volt;
+        const void* src = a ? h.curr : h.volt;
+        if (b->half) {
+            std::vector<_Float16> tmp(3 * h.S);
+            BHIP(b, hipMemcpy(tmp.data(), src, tmp.size() * 2, hipMemcpyDeviceToHost));
+            for (uint32_t n = 0; n < 3; ++n)
+                for (uint64_t xy = 0; xy < (uint64_t)h.nx * h.ny; ++xy)
+                    for (uint32_t z = 0; z < h.nz; ++z)
+                        dst[n * h.N + xy * h.nz + z] = (float)tmp[n * h.S + xy * h.nzp + z] * sc[3 * a + n];
+        } else {
+            std::vector<float> tmp(3 * h.S);
+            BHIP(b, hipMemcpy(tmp.data(), src, tmp.size() * 4, hipMemcpyDeviceToHost));
+            for (uint32_t n = 0; n < 3; ++n)
+                for (uint64_t xy = 0; xy < (uint64_t)h.nx * h.ny; ++xy)
+                    std::memcpy(dst + n * h.N + xy * h.nz, tmp.data() + n * h.S + xy * h.nzp, h.nz * sizeof(float));
+        }
+    }
+    return EFS_OK;
+}
+
+int efs_batch_info(efs_batch* b, uint32_t* blocks, uint32_t* waves, uint32_t* graph_steps, uint64_t* cells) {
+    if (!b) return EFS_ERR_STATE;
+    if (blocks) *blocks = b->nblocks;
+    if (waves) *waves = b->waves;
+    if (graph_steps) *graph_steps = b->graph_ok ? kGraphSteps : 0;
+    if (cells) *cells = b->cells;
+    return EFS_OK;
+}
+
+int efs_batch_profile(efs_batch* b, int enable) {
+    if (!b) return EFS_ERR_STATE;
+    BTRY(drain(b));
+    b->prof = enable != 0;
+    b->prof_ms[0] = b->prof_ms[1] = 0.0;
+    b->prof_n[0] = b->prof_n[1] = 0;
+    return EFS_OK;
+}
+
+int efs_batch_profile_read(efs_batch* b, double* ms_e, uint32_t* n_e, double* ms_h, uint32_t* n_h) {
+    if (!b) return EFS_ERR_STATE;
+    BTRY(drain(b));
+    if (ms_e) *ms_e = b->prof_ms[0];
+    if (n_e) *n_e = b->prof_n[0];
+    if (ms_h) *ms_h = b->prof_ms[1];
+    if (n_h) *n_h = b->prof_n[1];
+    return EFS_OK;
+}
+
+}  // extern "C"

@@ -32,6 +32,7 @@
 #include "reduce.hpp"
 #include "sparse.hpp"
 #include "sweep.hpp"
+#include "sweep_kernels.hpp"
 
 using namespace efs;
 
@@ -109,6 +110,8 @@ struct efs_ctx {
     std::vector<uint32_t> h_mur_normal, h_mur_shifted;
     float *mur_coeff = nullptr, *mur_saved = nullptr;
     uint32_t *mur_normal = nullptr, *mur_shifted = nullptr, *mur_normal_src = nullptr, *mur_shifted_src = nullptr;
+    uint32_t mur_apply_n = 0;          // entries written by Mur apply (last entry per normal index)
+    uint32_t* mur_apply_src = nullptr; // their entry numbers; mur_normal holds their device indices
 
     // ADE (all orders/directions flattened)
     std::vector<uint32_t> h_ade_gidx;
@@ -484,7 +487,7 @@ void free_all(efs_ctx* c) {
     dfree(c->pml_idx); dfree(c->pml_tab_e); dfree(c->pml_tab_h);
     dfree(c->fscale); dfree(c->f_amax); dfree(c->f_factor); dfree(c->f_smin);
     c->half = false;
-    dfree(c->mur_coeff); dfree(c->mur_saved); dfree(c->mur_normal); dfree(c->mur_shifted);
+    dfree(c->mur_coeff); dfree(c->mur_saved); dfree(c->mur_normal); dfree(c->mur_shifted); dfree(c->mur_apply_src);
     dfree(c->mur_normal_src); dfree(c->mur_shifted_src);
     dfree(c->ade_gidx); dfree(c->ade_src_v); dfree(c->ade_src_i); dfree(c->ade_lor);
     for (auto& p : c->ade_c) dfree(p);
@@ -716,8 +719,8 @@ int launch_step(efs_ctx* c) {
                            c->exc_n, t);
     }
     if (c->mur_n) {
-        hipLaunchKernelGGL(k_mur_apply, dim3(nblk(c->mur_n)), dim3(kPointThreads), 0, c->stream, fv(c, 0),
-                           c->mur_normal, c->mur_saved, c->mur_n);
+        hipLaunchKernelGGL(k_mur_apply, dim3(nblk(c->mur_apply_n)), dim3(kPointThreads), 0, c->stream, fv(c, 0),
+                           c->mur_normal, c->mur_apply_src, c->mur_saved, c->mur_apply_n);
     }
     if (c->rlc_n) {
         hipLaunchKernelGGL(k_rlc_apply, dim3(nblk(c->rlc_n)), dim3(kPointThreads), 0, c->stream, fv(c, 0),
@@ -1104,7 +1107,26 @@ int efs_cfg_mur(efs_ctx* c, uint32_t n, const float* coeff, const uint32_t* norm
     c->h_mur_shifted.assign(shifted_idx, shifted_idx + n);
     TRY(dupload(c, c->mur_coeff, coeff, n));
     const std::vector<uint32_t> tn = tr_all(c, normal_idx, n), ts = tr_all(c, shifted_idx, n);
-    TRY(dupload(c, c->mur_normal, tn.data(), n));
+    // apply list: the last entry of every normal index, in entry order
+    std::vector<uint32_t> an, asrc;
+    {
+        std::vector<uint32_t> sorted(tn);
+        std::sort(sorted.begin(), sorted.end());
+        sorted.erase(std::unique(sorted.begin(), sorted.end()), sorted.end());
+        std::vector<uint8_t> seen(sorted.size(), 0);
+        for (uint32_t k = n; k-- > 0;) {
+            const size_t j = std::lower_bound(sorted.begin(), sorted.end(), tn[k]) - sorted.begin();
+            if (!seen[j]) {
+                seen[j] = 1;
+                asrc.push_back(k);
+            }
+        }
+        std::reverse(asrc.begin(), asrc.end());
+        for (uint32_t k : asrc) an.push_back(tn[k]);
+    }
+    TRY(dupload(c, c->mur_normal, an.data(), an.size()));
+    TRY(dupload(c, c->mur_apply_src, asrc.data(), asrc.size()));
+    c->mur_apply_n = (uint32_t)asrc.size();
     TRY(dupload(c, c->mur_shifted, ts.data(), n));
     TRY(dzero(c, c->mur_saved, n));
     c->mur_n = n;

@@ -53,11 +53,16 @@ __global__ void k_mur_post(FieldView volt, const float* __restrict__ coeff, cons
     saved[k] += coeff[k] * volt.ld(shifted[k]);
 }
 
-__global__ void k_mur_apply(FieldView volt, const uint32_t* __restrict__ normal, const float* __restrict__ saved,
-                            uint32_t n) {
+// Mur apply: V[normal] = saved.  An index listed more than once (an edge
+// shared by two boundary faces) keeps the value of its last entry, as the
+// sequential loop of the reference does (src/webgpu-fdtd.mjs:1014-1023):
+// the host passes only those last entries (src = their entry number), so
+// no two threads write one cell.
+__global__ void k_mur_apply(FieldView volt, const uint32_t* __restrict__ normal, const uint32_t* __restrict__ src,
+                            const float* __restrict__ saved, uint32_t n) {
     const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= n) return;
-    volt.st(normal[k], saved[k]);
+    volt.st(normal[k], saved[src[k]]);
 }
 
 // Lorentz / Drude ADE state update for all (order, direction) entries at

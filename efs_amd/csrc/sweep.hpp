@@ -784,42 +784,4 @@ __device__ __forceinline__ bool pml_lane(const PmlTable& pml, Lane4& w) {
         MARCH<kMarchInterior, COMP, false, f16>(dst, src, cf, s_tab, nullptr, g, pml, pcoef, pflux, w, m, sc);     \
     }
 
-EFS_SWEEP4_INTERIOR(sweep_e4, e4_march, false, false, false)
-EFS_SWEEP4_INTERIOR(sweep_e4c, e4_march, false, false, true)
-EFS_SWEEP4_INTERIOR(sweep_e4m, e4_march, false, true, false)
-EFS_SWEEP4_INTERIOR(sweep_e4mc, e4_march, false, true, true)
-EFS_SWEEP4_UPML(sweep_e4u, e4_march, false, false)
-EFS_SWEEP4_UPML(sweep_e4uc, e4_march, false, true)
-EFS_SWEEP4_FUSED(sweep_e4f, e4_march, false, false, false, float)
-EFS_SWEEP4_FUSED(sweep_e4fc, e4_march, false, true, false, float)
-EFS_SWEEP4_FUSED(sweep_e4fp, e4_march, false, false, true, float)
-EFS_SWEEP4_FUSED(sweep_e4fcp, e4_march, false, true, true, float)
-EFS_SWEEP4_PLAIN16(sweep_e4_h, e4_march, false, false)
-EFS_SWEEP4_PLAIN16(sweep_e4c_h, e4_march, false, true)
-EFS_SWEEP4_FUSED(sweep_e4f_h, e4_march, false, false, false, f16)
-EFS_SWEEP4_FUSED(sweep_e4fc_h, e4_march, false, true, false, f16)
-EFS_SWEEP4_FUSED(sweep_e4fp_h, e4_march, false, false, true, f16)
-EFS_SWEEP4_FUSED(sweep_e4fcp_h, e4_march, false, true, true, f16)
-EFS_SWEEP4_INTERIOR(sweep_h4, h4_march, true, false, false)
-EFS_SWEEP4_INTERIOR(sweep_h4c, h4_march, true, false, true)
-EFS_SWEEP4_INTERIOR(sweep_h4m, h4_march, true, true, false)
-EFS_SWEEP4_INTERIOR(sweep_h4mc, h4_march, true, true, true)
-EFS_SWEEP4_UPML(sweep_h4u, h4_march, true, false)
-EFS_SWEEP4_UPML(sweep_h4uc, h4_march, true, true)
-EFS_SWEEP4_FUSED(sweep_h4f, h4_march, true, false, false, float)
-EFS_SWEEP4_FUSED(sweep_h4fc, h4_march, true, true, false, float)
-EFS_SWEEP4_FUSED(sweep_h4fp, h4_march, true, false, true, float)
-EFS_SWEEP4_FUSED(sweep_h4fcp, h4_march, true, true, true, float)
-EFS_SWEEP4_PLAIN16(sweep_h4_h, h4_march, true, false)
-EFS_SWEEP4_PLAIN16(sweep_h4c_h, h4_march, true, true)
-EFS_SWEEP4_FUSED(sweep_h4f_h, h4_march, true, false, false, f16)
-EFS_SWEEP4_FUSED(sweep_h4fc_h, h4_march, true, true, false, f16)
-EFS_SWEEP4_FUSED(sweep_h4fp_h, h4_march, true, false, true, f16)
-EFS_SWEEP4_FUSED(sweep_h4fcp_h, h4_march, true, true, true, f16)
-#undef EFS_SWEEP4_INTERIOR
-#undef EFS_SWEEP4_UPML
-#undef EFS_SWEEP4_FUSED
-#undef EFS_SWEEP4_PLAIN16
-#undef EFS_SWEEP4_PARAMS
-
 }  // namespace efs

@@ -72,6 +72,30 @@ SIGNATURES = {
     "efs_cfg_steady_state": (C.c_int, [C.c_void_p, _u32p, C.c_uint32, C.c_uint32]),
     "efs_ss_set": (C.c_int, [C.c_void_p, C.c_int, C.c_uint32]),
     "efs_ss_read": (C.c_int, [C.c_void_p, _f32p, _f32p, C.POINTER(C.c_uint32), C.POINTER(C.c_int)]),
+    # batched form (include/efs_batch.h)
+    "efs_batch_create": (C.c_int, [C.POINTER(C.c_void_p), C.c_int]),
+    "efs_batch_destroy": (None, [C.c_void_p]),
+    "efs_batch_last_error": (C.c_char_p, [C.c_void_p]),
+    "efs_batch_set_storage": (C.c_int, [C.c_void_p, C.c_int]),
+    "efs_batch_add": (C.c_int, [C.c_void_p, C.c_uint32, C.c_uint32, C.c_uint32, _f32p, _f32p, _f32p, _f32p,
+                                C.POINTER(C.c_uint32)]),
+    "efs_batch_cfg_excitation": (C.c_int, [C.c_void_p, C.c_uint32, _f32p, C.c_uint32, _u32p, _f32p, _u32p, _u32p,
+                                           C.c_uint32, C.c_uint32]),
+    "efs_batch_cfg_mur": (C.c_int, [C.c_void_p, C.c_uint32, C.c_uint32, _f32p, _u32p, _u32p]),
+    "efs_batch_cfg_gather": (C.c_int, [C.c_void_p, C.c_uint32, _u32p, C.c_uint32]),
+    "efs_batch_finalize": (C.c_int, [C.c_void_p, C.c_uint32, C.c_uint32]),
+    "efs_batch_tune": (C.c_int, [C.c_void_p, C.c_uint32, C.POINTER(C.c_uint32), C.POINTER(C.c_uint32)]),
+    "efs_batch_reset": (C.c_int, [C.c_void_p]),
+    "efs_batch_iterate": (C.c_int, [C.c_void_p, C.c_uint32]),
+    "efs_batch_sync": (C.c_int, [C.c_void_p]),
+    "efs_batch_num_ts": (C.c_int, [C.c_void_p, C.POINTER(C.c_uint32)]),
+    "efs_batch_read_gather": (C.c_int, [C.c_void_p, C.c_uint32, C.c_void_p, C.POINTER(C.c_uint32)]),
+    "efs_batch_get_fields": (C.c_int, [C.c_void_p, C.c_uint32, _f32p, _f32p]),
+    "efs_batch_info": (C.c_int, [C.c_void_p, C.POINTER(C.c_uint32), C.POINTER(C.c_uint32), C.POINTER(C.c_uint32),
+                                 C.POINTER(C.c_uint64)]),
+    "efs_batch_profile": (C.c_int, [C.c_void_p, C.c_int]),
+    "efs_batch_profile_read": (C.c_int, [C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_uint32),
+                                         C.POINTER(C.c_double), C.POINTER(C.c_uint32)]),
 }
 
 

@@ -112,6 +112,63 @@ class Materials:
             self.port_g[n, p[0], p[1], p[2]] += length / R
 
 
+def lumped_port(mesh, mat, n, start, stop, R, excite=1.0):
+    """Lumped port along axis n between nodes start and stop (node index
+    triples equal off axis n, or spanning a transverse rectangle), with the
+    conventions of the reference's LumpedPort (src/ports.mjs:220-330):
+    the resistor R is folded into the port edges (R > 0), the soft source
+    drives E = -dir*excite (V/m) on them, i.e. a voltage amplitude of
+    -dir*excite*dl per edge, the voltage probe runs along the port at the
+    transverse centre with weight -1, and the current probe is the loop on
+    the dual lines around the port at its middle (normal n, weight dir; a
+    middle that falls between two dual lines takes the lower one).
+
+    Returns the excitation point lists (pos, dir, amp) and the two probe
+    records (the gather format of efs_amd.ports)."""
+    start, stop = [int(v) for v in start], [int(v) for v in stop]
+    d = 1 if stop[n] >= start[n] else -1
+    lo, hi = min(start[n], stop[n]), max(start[n], stop[n])
+    if hi == lo:
+        raise ValueError("lumped port needs a non-zero extent along its axis")
+    a, b = (n + 1) % 3, (n + 2) % 3
+    nx, ny, nz = mesh.shape
+    rng = [range(min(start[m], stop[m]), max(start[m], stop[m]) + 1) for m in range(3)]
+    rng[n] = range(lo, hi)
+    pos, amp = [], []
+    n_par = len(rng[a]) * len(rng[b])
+    for i in rng[0]:
+        for j in rng[1]:
+            for k in rng[2]:
+                p = (i, j, k)
+                if R > 0:
+                    # strings of hi-lo edges in series (each edge's share of
+                    # R proportional to its length), n_par strings in parallel
+                    L = mesh.lines[n][hi] - mesh.lines[n][lo]
+                    mat.port_g[n, i, j, k] += L / (R * n_par * mesh.primal[n][p[n]])
+                pos.append(i * ny * nz + j * nz + k)
+                amp.append(-d * excite * mesh.primal[n][p[n]])
+    mid = 0.5 * (mesh.lines[n][start[n]] + mesh.lines[n][stop[n]])
+    duals = 0.5 * (mesh.lines[n][:-1] + mesh.lines[n][1:])
+    kd = int(np.argmin(np.abs(duals - mid)))  # first minimum: ties to the lower line
+    centre = [(start[m] + stop[m]) // 2 for m in range(3)]
+    us, ue = list(centre), list(centre)
+    us[n], ue[n] = start[n], stop[n]
+    s, t = [0, 0, 0], [0, 0, 0]
+    for m in (a, b):
+        s[m], t[m] = min(start[m], stop[m]) - 1, max(start[m], stop[m])
+    s[n] = t[n] = kd
+
+    def inside(v, m):
+        return bool(0 <= v <= mesh.n[m] - 2)
+    probes = [
+        {"name": "ut", "type": 0, "start": us, "stop": ue, "weight": -1.0},
+        {"name": "it", "type": 1, "start": s, "stop": t, "normDir": n, "weight": float(d),
+         "startInside": [inside(s[m], m) for m in range(3)], "stopInside": [inside(t[m], m) for m in range(3)]},
+    ]
+    return {"pos": np.array(pos, np.uint32), "dir": np.full(len(pos), n, np.uint32),
+            "amp": np.array(amp, np.float32)}, probes
+
+
 def _pad_cells(a):
     return np.pad(a, 1)
 

@@ -4,11 +4,13 @@
 with 8-cell UPML on all six faces, 40 seeded dielectric boxes (eps_r in
 U(1,10), kappa in U(0,0.05) S/m), 10 PEC plates, a 50-ohm lumped port fed
 by a Gaussian pulse (f0 = 5 GHz, fc = 4 GHz), field energy every 100 steps
-and a six-face NF2FF box at 16 frequencies from 1 to 9 GHz.  n = 256 is the
+and a six-face NF2FF box at 16 frequencies from 1 to 9 GHz.  ``probes``
+holds the port's voltage and current probes (efs_amd.ports format).  n = 256 is the
 full-size workload; smaller n keep the same structure for tests.
 """
 import numpy as np
 
+from . import meshing, ports
 from . import operator as op
 
 STRESS_SEED = 20261203 + 2
@@ -38,15 +40,13 @@ def stress_grid(n=256, seed=STRESS_SEED, pml=8, nfreq=16, n_boxes=40, n_plates=1
         mat.pec_box(lo, hi)
         plates += 1
     op.pec_boundary(mat)
-    # lumped port: 2 z-edges in series at the centre, 50 ohm
-    mat.lumped_resistor(2, (c, c, c - 1), 2, 50.0)
+    # lumped port: 2 z-edges in series at the centre, 50 ohm, 0.5 V per edge
+    port, probes = op.lumped_port(mesh, mat, 2, (c, c, c - 1), (c, c, c + 1), 50.0, excite=500.0)
     coeffs = op.build_coefficients(mesh, mat, dT)
     regions = op.build_upml(mesh, coeffs, [pml] * 6, dT)
     signal = op.gaussian_pulse(5e9, 4e9, dT)
-    lin = lambda x, y, z: x * n * n + y * n + z  # noqa: E731
-    excitation = {"signal": signal, "delay": np.zeros(2, np.uint32), "amp": np.full(2, 0.5, np.float32),
-                  "dir": np.full(2, 2, np.uint32), "pos": np.array([lin(c, c, c - 1), lin(c, c, c)], np.uint32),
-                  "period": 0}
+    excitation = {"signal": signal, "delay": np.zeros(len(port["pos"]), np.uint32), "amp": port["amp"],
+                  "dir": port["dir"], "pos": port["pos"], "period": 0}
     pts, meta, primal, dual = op.nf2ff_box(mesh, [pml + 1] * 3, [n - pml - 2] * 3)
     freqs = np.linspace(1e9, 9e9, nfreq)
     nf2ff = {"surfaceIndices": pts, "numPoints": len(pts) // 3, "omega": list(2 * np.pi * freqs), "dT": dT,
@@ -54,7 +54,7 @@ def stress_grid(n=256, seed=STRESS_SEED, pml=8, nfreq=16, n_boxes=40, n_plates=1
     n_pml = sum(int(np.prod(r["numLines"])) for r in regions)
     return {"gridSize": [n, n, n], "coefficients": op.to_f32(coeffs), "pmlRegions": regions,
             "excitation": excitation, "nf2ff": nf2ff, "pointMeta": meta, "freqs": freqs, "dT": dT,
-            "seed": seed, "n_pml_cells": n_pml, "pml": pml}
+            "seed": seed, "n_pml_cells": n_pml, "pml": pml, "probes": probes}
 
 
 def configure_engine(eng, w, nf2ff=True, storage=None):
@@ -68,3 +68,109 @@ def configure_engine(eng, w, nf2ff=True, storage=None):
     eng.configureExcitation(w["excitation"])
     if nf2ff:
         eng.configureNF2FFAccumulation(w["nf2ff"])
+
+
+PATCH_SEED = 20261203 + 1
+
+
+def _at(lines, v):
+    """Index of the mesh line at coordinate v (lines placed there exactly)."""
+    k = int(np.argmin(np.abs(lines - v)))
+    if abs(lines[k] - v) > 1e-9 * max(1.0, abs(v)):
+        raise ValueError("no mesh line at %g" % v)
+    return k
+
+
+def patch_antenna(patch_w=32.0, patch_l=40.0, f0=3e9, fc=3e9, eps_r=3.38, sub_h=1.524, sub_w=60.0, sub_l=60.0,
+                  feed_x=-6.0, feed_r=50.0, sim_box=(200.0, 200.0, 150.0), nf2ff_freq=2.4e9, k_block=64,
+                  n_ts=30000):
+    """Microstrip patch antenna of the reference's script example
+    (app/examples.mjs:17-103, the openEMS Simple_Patch_Antenna tutorial):
+    a patch_w x patch_l mm PEC patch on a sub_h mm substrate (eps_r,
+    loss tangent 1e-3 at 2.45 GHz) over a sub_w x sub_l mm ground plane,
+    fed by a 50-ohm lumped port at x = feed_x, Gaussian excitation (f0, fc),
+    first-order Mur on all six faces and an NF2FF box 3 lines inside the
+    boundary at nf2ff_freq.  Lengths in mm.  The mesh is the example's:
+    box, patch, ground and substrate lines smoothed to lambda/20 at f0+fc
+    with ratio 1.4, then the feed lines added exactly (86 x 87 x 71 for the
+    defaults).  dT is the CFL limit of the smallest cell."""
+    unit = 1e-3
+    res = op.C0 / (f0 + fc) / unit / 20
+    hw, hl, hpw, hpl = sub_w / 2, sub_l / 2, patch_w / 2, patch_l / 2
+    xs = meshing.grid_lines([-sim_box[0] / 2, sim_box[0] / 2, -hpw, hpw, -hw, hw], res, 1.4, extra=[feed_x, 0.0])
+    ys = meshing.grid_lines([-sim_box[1] / 2, sim_box[1] / 2, -hpl, hpl, -hl, hl], res, 1.4, extra=[0.0])
+    zs = meshing.grid_lines([-sim_box[2] / 3, sim_box[2] * 2 / 3] + [sub_h * i / 4 for i in range(5)], res, 1.4)
+    mesh = op.Mesh(xs * unit, ys * unit, zs * unit)
+    dT = mesh.timestep()
+    mat = op.Materials(mesh)
+    kappa = 1e-3 * 2 * np.pi * 2.45e9 * op.EPS0 * eps_r
+    ix = lambda v: _at(xs, v)  # noqa: E731
+    iy = lambda v: _at(ys, v)  # noqa: E731
+    iz = lambda v: _at(zs, v)  # noqa: E731
+    mat.box((ix(-hw), iy(-hl), iz(0.0)), (ix(hw), iy(hl), iz(sub_h)), eps=eps_r, kappa=kappa)
+    mat.pec_box((ix(-hw), iy(-hl), iz(0.0)), (ix(hw), iy(hl), iz(0.0)))
+    mat.pec_box((ix(-hpw), iy(-hpl), iz(sub_h)), (ix(hpw), iy(hpl), iz(sub_h)))
+    port, probes = op.lumped_port(mesh, mat, 2, (ix(feed_x), iy(0.0), iz(0.0)), (ix(feed_x), iy(0.0), iz(sub_h)),
+                                  feed_r, excite=1.0)
+    coeffs = op.build_coefficients(mesh, mat, dT)
+    signal = op.gaussian_pulse(f0, fc, dT)
+    excitation = {"signal": signal, "delay": np.zeros(len(port["pos"]), np.uint32), "amp": port["amp"],
+                  "dir": port["dir"], "pos": port["pos"], "period": 0}
+    mur = op.mur_config(mesh, range(6), dT)
+    n = mesh.shape
+    ins = 3
+    pts, meta, primal, dual = op.nf2ff_box(mesh, [ins] * 3, [k - 1 - ins for k in n])
+    nf2ff = {"surfaceIndices": pts, "numPoints": len(pts) // 3, "omega": [2 * np.pi * nf2ff_freq], "dT": dT,
+             "primalEdgeLens": primal, "dualEdgeLens": dual, "kBlock": k_block}
+    return {"gridSize": list(n), "coefficients": op.to_f32(coeffs), "mur": mur, "excitation": excitation,
+            "nf2ff": nf2ff, "pointMeta": meta, "freqs": np.array([nf2ff_freq]), "dT": dT, "probes": probes,
+            "mesh": mesh, "nTS": n_ts, "f0": f0, "fc": fc, "patch": [patch_w, patch_l]}
+
+
+def configure_patch(eng, w, nf2ff=True, gather_interval=1, max_samples=None, storage=None):
+    """Configure an engine with a patch workload: coefficients (engines
+    with an init method; the CPU oracle takes them at construction), Mur,
+    excitation, the port probes' buffered gather and optionally the NF2FF
+    accumulation.  Returns the gather slices."""
+    if hasattr(eng, "init"):
+        if storage is None:
+            eng.init(w["gridSize"], w["coefficients"])
+        else:
+            eng.init(w["gridSize"], w["coefficients"], storage=storage)
+    eng.configureMur(w["mur"])
+    eng.configureExcitation(w["excitation"])
+    idx, slices = ports.build_gather_indices(w["probes"], w["gridSize"])
+    n = max_samples if max_samples is not None else w["nTS"] // gather_interval
+    eng.configureProbeGatherBuffered(idx, n, gather_interval)
+    if nf2ff:
+        eng.configureNF2FFAccumulation(w["nf2ff"])
+    return slices
+
+
+BATCH_SEED = 20261203 + 4
+
+
+def patch_sweep_params(n=64, seed=BATCH_SEED, w_lo=20.0, w_hi=50.0):
+    """Patch widths of a design sweep (uniform in [w_lo, w_hi] mm from the
+    seed); the other lengths scale with the width as in the 32 mm design
+    (patch length 1.25 W, substrate 1.875 W square, feed at -0.1875 W) and
+    the simulation box grows with it, which gives graded meshes from about
+    60 x 60 x 50 (W = 20) to 160 x 150 x 90 cells (W = 50)."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for W in rng.uniform(w_lo, w_hi, n):
+        W = float(W)
+        out.append({"patch_w": W, "patch_l": 1.25 * W, "sub_w": 1.875 * W, "sub_l": 1.875 * W,
+                    "feed_x": -0.1875 * W, "sim_box": (8.33 * W - 31.7, 7.43 * W - 10.7, 3.35 * W + 30.5)})
+    return out
+
+
+def add_patch(batch, w):
+    """Add a patch workload to a BatchEngine as one instance: coefficients,
+    Mur, excitation and the port probes.  Returns (instance id, slices)."""
+    i = batch.add(w["gridSize"], w["coefficients"])
+    batch.configureMur(i, w["mur"])
+    batch.configureExcitation(i, w["excitation"])
+    idx, slices = ports.build_gather_indices(w["probes"], w["gridSize"])
+    batch.configureProbeGather(i, idx)
+    return i, slices

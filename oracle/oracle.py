@@ -256,6 +256,13 @@ class Oracle:
         self._f("cfg_nf2ff")(self.h, pts, self._nf_P, om, self._nf_F, float(dT), ptrs, lens, int(maxTS),
                              int(windowType))
 
+    def configureNF2FFAccumulation(self, cfg):
+        """The engine's configuration dict form (surfaceIndices, omega or
+        frequency, dT, primal/dualEdgeLens, maxTS, windowType)."""
+        om = cfg["omega"] if "omega" in cfg else 2 * np.pi * np.atleast_1d(cfg["frequency"])
+        self.configureNF2FF(cfg["surfaceIndices"], om, cfg["dT"], cfg["primalEdgeLens"], cfg["dualEdgeLens"],
+                            cfg.get("maxTS", 0), cfg.get("windowType", 0))
+
     def nf2ffAccumulate(self, ts):
         self._f("nf2ff_accumulate")(self.h, int(ts))
 

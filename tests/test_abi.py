@@ -1,5 +1,6 @@
 """CPU-side checks of the drop-in boundary: the C-ABI library loads and
-exports every entry point include/efs_engine.h declares (no GPU calls)."""
+exports every entry point include/*.h declares (no GPU calls)."""
+import glob
 import os
 import re
 import subprocess
@@ -7,19 +8,21 @@ import subprocess
 import pytest
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
-HEADER = os.path.join(ROOT, "include", "efs_engine.h")
+HEADERS = sorted(glob.glob(os.path.join(ROOT, "include", "*.h")))
 
 
 def declared():
-    src = open(HEADER).read()
-    src = re.sub(r"/\*.*?\*/", "", src, flags=re.S)
-    return sorted(set(re.findall(r"\b(efs_[a-z0-9_]+)\s*\(", src)))
+    names = set()
+    for h in HEADERS:
+        src = re.sub(r"/\*.*?\*/", "", open(h).read(), flags=re.S)
+        names |= set(re.findall(r"\b(efs_[a-z0-9_]+)\s*\(", src))
+    return sorted(names)
 
 
 def test_header_declares_entry_points():
     names = declared()
     for must in ("efs_create", "efs_init", "efs_iterate", "efs_get_fields", "efs_cfg_pml", "efs_cfg_nf2ff",
-                 "efs_farfield", "efs_energy"):
+                 "efs_farfield", "efs_energy", "efs_batch_add", "efs_batch_iterate", "efs_batch_tune"):
         assert must in names
 
 

@@ -1,0 +1,161 @@
+"""The batched form (include/efs_batch.h) on the GPU.
+
+- Each instance of a batch gives the same bits as the single-grid engine
+  run alone on the same arrays (fields and probe traces): both run the same
+  per-cell fused multiply-adds, the batch only changes the launch shape and
+  pads Nz to a multiple of 4 (instances here have Nz % 4 in {0, 2, 3}).
+- Graph replays (blocks of 32 steps) give the same bits as direct launches.
+- The autotuner leaves a reset batch whose results do not depend on the
+  chosen block table.
+- FP16 storage against the FP64 oracle per instance: relative L2 of the
+  port traces and fields within the stated tolerances.
+- Misuse fails with a message and launches nothing.
+"""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+STEPS = 150
+
+
+def say(*a):
+    print(*a, flush=True)
+
+
+def small_patch(W):
+    from efs_amd import workloads
+    return workloads.patch_antenna(patch_w=W, patch_l=1.25 * W, sub_w=1.875 * W, sub_l=1.875 * W,
+                                   feed_x=-0.1875 * W, sim_box=(3 * W + 20, 3 * W + 16, 2 * W + 10))
+
+
+@pytest.fixture(scope="module")
+def patches():
+    return [small_patch(W) for W in (14.0, 17.0, 20.0, 23.0)]
+
+
+def make_batch(ws, storage="fp32", steps=STEPS):
+    from efs_amd import batch, workloads
+    b = batch.BatchEngine(storage=storage)
+    sl = [workloads.add_patch(b, w)[1] for w in ws]
+    b.finalize(1, steps)
+    return b, sl
+
+
+def traces(r, w, slices):
+    from efs_amd import ports
+    return ports.probe_traces(r["data"], r["numSamples"], r["stride"], w["probes"], slices, 1, w["dT"])
+
+
+def test_batch_matches_single_engine(patches):
+    from efs_amd import engine, workloads
+    engine.load_library()
+    say("instances", [w["gridSize"] for w in patches], "steps", STEPS)
+    b, sl = make_batch(patches)
+    info = b.info()
+    say("info", info)
+    assert info["graph_steps"] == 32 and info["instances"] == len(patches)
+    b.iterate(STEPS)
+    assert b.numTS == STEPS
+    for i, w in enumerate(patches):
+        g = engine.HIPEngine()
+        assert g.initGPU()
+        workloads.configure_patch(g, w, nf2ff=False, max_samples=STEPS)
+        g.iterate(STEPS)
+        fs, fb = g.getFields(), b.getFields(i)
+        for k in ("volt", "curr"):
+            np.testing.assert_array_equal(fb[k], fs[k], err_msg=f"instance {i} {k}")
+        rs, rb = g.readProbeGatherBuffered(), b.readProbeGatherBuffered(i)
+        assert rb["numSamples"] == rs["numSamples"] == STEPS
+        np.testing.assert_array_equal(rb["data"], rs["data"])
+        g.destroy()
+    b.destroy()
+
+
+def test_graph_replay_matches_direct_launches(patches):
+    ws = patches[:2]
+    a, _ = make_batch(ws, steps=100)
+    d, _ = make_batch(ws, steps=100)
+    d.profile(True)  # profiling launches every step directly
+    a.iterate(100)
+    d.iterate(100)
+    p = d.profileRead()
+    assert p["e_n"] == 100 and p["h_n"] == 100
+    for i in range(len(ws)):
+        fa, fd = a.getFields(i), d.getFields(i)
+        for k in ("volt", "curr"):
+            np.testing.assert_array_equal(fa[k], fd[k])
+        np.testing.assert_array_equal(a.readProbeGatherBuffered(i)["data"], d.readProbeGatherBuffered(i)["data"])
+    a.destroy()
+    d.destroy()
+
+
+def test_tune_then_same_results(patches):
+    ws = patches[1:3]
+    a, _ = make_batch(ws, steps=64)
+    t, _ = make_batch(ws, steps=64)
+    shape = t.tune(steps=4)
+    say("tuned", shape, t.info())
+    assert shape["waves"] in (4, 5) and shape["blocks"] > 0
+    assert t.numTS == 0
+    z = t.getFields(0)
+    assert not np.any(z["volt"]) and not np.any(z["curr"])
+    a.iterate(64)
+    t.iterate(64)
+    for i in range(len(ws)):
+        fa, ft = a.getFields(i), t.getFields(i)
+        for k in ("volt", "curr"):
+            np.testing.assert_array_equal(fa[k], ft[k])
+    a.destroy()
+    t.destroy()
+
+
+def rel_l2(a, b):
+    a, b = np.asarray(a, np.float64), np.asarray(b, np.float64)
+    return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300))
+
+
+def test_batch_fp16_against_fp64(patches, oracle_mod):
+    """FP16 storage (FP32 updates) per instance against the FP64 oracle:
+    port traces within 5e-3 relative L2, fields within 1e-2 (volt) and
+    3e-2 (curr) after 400 steps."""
+    from efs_amd import workloads
+    steps = 400
+    b, sl = make_batch(patches, storage="fp16", steps=steps)
+    b.iterate(steps)
+    for i, w in enumerate(patches):
+        o = oracle_mod.Oracle(w["gridSize"], w["coefficients"], fp64=True)
+        so = workloads.configure_patch(o, w, nf2ff=False, max_samples=steps)
+        o.iterate(steps)
+        r = o.readProbeGatherBuffered()
+        to = traces({"data": r.ravel(), "numSamples": r.shape[0], "stride": r.shape[1]}, w, so)
+        tb = traces(b.readProbeGatherBuffered(i), w, sl[i])
+        fo, fb = o.getFields(), b.getFields(i)
+        ev, ec = rel_l2(fb["volt"], fo["volt"]), rel_l2(fb["curr"], fo["curr"])
+        eu, ei = rel_l2(tb[0][1], to[0][1]), rel_l2(tb[1][1], to[1][1])
+        say("instance", i, w["gridSize"], "volt", ev, "curr", ec, "ut", eu, "it", ei)
+        assert np.all(np.isfinite(fb["volt"])) and np.all(np.isfinite(fb["curr"]))
+        assert eu <= 5e-3 and ei <= 5e-3
+        assert ev <= 1e-2 and ec <= 3e-2
+    b.destroy()
+
+
+def test_batch_misuse(patches):
+    from efs_amd import batch
+    w = patches[0]
+    b = batch.BatchEngine()
+    with pytest.raises(RuntimeError, match="not finalized"):
+        b.iterate(1)
+    i = b.add(w["gridSize"], w["coefficients"])
+    with pytest.raises(RuntimeError, match="before adding"):
+        b._check(b.lib.efs_batch_set_storage(b.h, 1))
+    with pytest.raises(RuntimeError, match="out of range"):
+        b.configureMur(i, {"coeff": [0.5], "normal_idx": [10 ** 9], "shifted_idx": [0]})
+    with pytest.raises(RuntimeError, match="does not exist"):
+        b.configureProbeGather(7, [0])
+    b.finalize(1, 4)
+    with pytest.raises(RuntimeError, match="already finalized"):
+        b.add(w["gridSize"], w["coefficients"])
+    b.iterate(3)
+    assert b.numTS == 3
+    b.destroy()
