@@ -96,10 +96,10 @@ def precision_sample(storage, n=48, steps=150):
     return out
 
 
-def cpu_baseline(w, seconds_target=15.0):
+def cpu_baseline(w, seconds_target=15.0, what=None):
     """Time the CPU oracle (all host threads, f32 variant) on a bounded
-    sample of the same workload: the same grid, coefficients, UPML and
-    excitation; NF2FF and energy are not part of the sample."""
+    sample of the same workload: the same grid, coefficients, UPML (or Mur)
+    and excitation; NF2FF and energy are not part of the sample."""
     from oracle import oracle as orc
     threads = os.cpu_count() or 1
     try:
@@ -109,7 +109,10 @@ def cpu_baseline(w, seconds_target=15.0):
     threads = min(threads, 16)
     orc.Oracle.set_threads(threads)
     o = orc.Oracle(w["gridSize"], w["coefficients"])
-    o.configurePML(w["pmlRegions"])
+    if "pmlRegions" in w:
+        o.configurePML(w["pmlRegions"])
+    if "mur" in w:
+        o.configureMur(w["mur"])
     o.configureExcitation(w["excitation"])
     o.iterate(1)  # first touch
     n = int(np.prod(w["gridSize"]))
@@ -123,9 +126,177 @@ def cpu_baseline(w, seconds_target=15.0):
         chunk = min(chunk * 2, 64)
     del o
     return {"value": n * steps / elapsed / 1e6, "unit": "MCells/s", "cores": threads, "kind": "port",
-            "sample": f"{steps} steps of the same {w['gridSize'][0]}^3 UPML+excitation workload "
-                      f"({elapsed:.1f} s) with the C oracle (float storage, double arithmetic, OpenMP); "
+            "sample": f"{steps} steps of " + (what or f"the same {w['gridSize'][0]}^3 UPML+excitation workload") +
+                      f" ({elapsed:.1f} s) with the C oracle (float storage, double arithmetic, OpenMP); "
                       f"NF2FF and energy excluded"}
+
+
+BATCH_BYTES = {"fp32": 60.0, "fp16": 42.0}  # per cell and sweep, no UPML (see sweep_bytes)
+
+
+def build_patch_batch(n_inst, storage, max_samples, progress=False):
+    """The design-sweep workload: n_inst patch antennas (widths from the
+    seed) as instances of one BatchEngine; returns (batch, workloads,
+    slices)."""
+    from efs_amd import batch, workloads
+    b = batch.BatchEngine(storage=storage)
+    ws, sl = [], []
+    for k, p in enumerate(workloads.patch_sweep_params(n_inst)):
+        w = workloads.patch_antenna(**p, n_ts=20000)
+        sl.append(workloads.add_patch(b, w)[1])
+        w = {key: w[key] for key in ("gridSize", "dT", "probes", "patch", "excitation", "mur", "coefficients")}
+        ws.append(w)
+        if progress and k % 16 == 15:
+            print(f"# built {k + 1}/{n_inst} instances", file=sys.stderr, flush=True)
+    b.finalize(1, max_samples)
+    return b, ws, sl
+
+
+def time_batch(b, warmup, K):
+    b.iterate(warmup)
+    b.sync()
+    t0 = time.perf_counter()
+    b.iterate(K)
+    b.sync()
+    return time.perf_counter() - t0
+
+
+def s11_error(sample, storage, steps, freqs):
+    """S11 of `sample` patch instances from the batched engine (storage)
+    and from the FP64 oracle, over `steps` steps: max |dS11| in dB over
+    freqs, and the relative L2 of the port traces."""
+    from efs_amd import batch, ports, workloads
+    from oracle import oracle as orc
+    threads = min(len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else 1, 16)
+    orc.Oracle.set_threads(threads)
+    b = batch.BatchEngine(storage=storage)
+    sl = [workloads.add_patch(b, w)[1] for w in sample]
+    b.finalize(1, steps)
+    b.iterate(steps)
+    out = []
+    for i, w in enumerate(sample):
+        r = b.readProbeGatherBuffered(i)
+        tb = ports.probe_traces(r["data"], r["numSamples"], r["stride"], w["probes"], sl[i], 1, w["dT"])
+        o = orc.Oracle(w["gridSize"], w["coefficients"], fp64=True)
+        so = workloads.configure_patch(o, w, nf2ff=False, max_samples=steps)
+        o.iterate(steps)
+        ro = o.readProbeGatherBuffered()
+        to = ports.probe_traces(ro.ravel(), ro.shape[0], ro.shape[1], w["probes"], so, 1, w["dT"])
+        sb = 20 * np.log10(np.abs(ports.port_s11(tb[0], tb[1], freqs)))
+        s64 = 20 * np.log10(np.abs(ports.port_s11(to[0], to[1], freqs)))
+        k = int(np.argmin(s64))
+        out.append({"grid": w["gridSize"], "patch_w_mm": round(w["patch"][0], 2),
+                    "s11_max_abs_err_dB": float(np.max(np.abs(sb - s64))),
+                    "s11_min_dB_fp64": float(s64[k]), "f_res_GHz": float(freqs[k] / 1e9),
+                    "ut_rel_l2": float(np.linalg.norm(tb[0][1] - to[0][1]) / np.linalg.norm(to[0][1])),
+                    "it_rel_l2": float(np.linalg.norm(tb[1][1] - to[1][1]) / np.linalg.norm(to[1][1]))})
+        del o
+    b.destroy()
+    return out
+
+
+def plain_sequential(ws, steps):
+    """The plain single-grid engine over the same instances, one after the
+    other (its aggregate MCells/s for the same design sweep)."""
+    from efs_amd import workloads
+    from efs_amd.engine import HIPEngine
+    cells, secs = 0, 0.0
+    for w in ws:
+        g = HIPEngine()
+        g.initGPU()
+        workloads.configure_patch(g, w, nf2ff=False, max_samples=steps + 20)
+        g.iterate(20)
+        g.sync()
+        t0 = time.perf_counter()
+        g.iterate(steps)
+        g.sync()
+        secs += time.perf_counter() - t0
+        cells += int(np.prod(w["gridSize"])) * steps
+        g.destroy()
+    return cells / secs / 1e6
+
+
+def run_batch(args):
+    n_inst, K, W = args.instances, args.steps, args.warmup
+    storage = args.storage
+    t_build = time.perf_counter()
+    b, ws, _ = build_patch_batch(n_inst, storage, K + W + 64, progress=True)
+    t_build = time.perf_counter() - t_build
+    shape = b.tune(steps=8)
+    info = b.info()
+    cells = info["cells"]
+    elapsed = time_batch(b, W, K)
+    value = cells * K / elapsed / 1e6
+    # sweep durations (direct launches with events)
+    b.profile(True)
+    b.iterate(args.profile_steps)
+    p = b.profileRead()
+    b.profile(False)
+    b.destroy()
+    avg_e, avg_h = p["e_ms"] / max(p["e_n"], 1), p["h_ms"] / max(p["h_n"], 1)
+    cells_p = sum(int(np.prod(w["gridSize"][:2])) * ((w["gridSize"][2] + 3) // 4 * 4) for w in ws)
+    by = BATCH_BYTES[storage] * cells_p
+    dom_name, dom_ms = ("sweep_h", avg_h) if avg_h >= avg_e else ("sweep_e", avg_e)
+    dom_gbs = by / (dom_ms * 1e-3) / 1e9
+    extra = {}
+    print("# plain sequential baseline", file=sys.stderr, flush=True)
+    plain = plain_sequential(ws, args.plain_steps)
+    extra["plain_sequential_value"] = round(plain, 1)
+    extra["speedup_vs_plain"] = round(value / plain, 3)
+    if storage == "fp16" and not args.no_compare:
+        print("# FP32 batch for comparison", file=sys.stderr, flush=True)
+        b32, _, _ = build_patch_batch(n_inst, "fp32", K + W + 64)
+        b32.tune(steps=8)
+        v32 = b32.info()["cells"] * K / time_batch(b32, W, K) / 1e6
+        b32.destroy()
+        extra["batched_fp32_value"] = round(v32, 1)
+        extra["speedup_vs_batched_fp32"] = round(value / v32, 3)
+    if not args.no_compare:
+        print("# S11 against the FP64 oracle", file=sys.stderr, flush=True)
+        order = np.argsort([np.prod(w["gridSize"]) for w in ws])[:args.s11_instances]
+        sample = [ws[i] for i in order]
+        freqs = np.linspace(1e9, 6e9, 501)
+        extra["precision"] = {"against": "FP64 oracle (C restatement, double storage), same instances",
+                              "steps": args.s11_steps, "band_GHz": [1.0, 6.0],
+                              "instances": s11_error(sample, storage, args.s11_steps, freqs)}
+        extra["precision"]["s11_max_abs_err_dB"] = max(x["s11_max_abs_err_dB"]
+                                                       for x in extra["precision"]["instances"])
+    base = None
+    if not args.no_cpu_baseline:
+        w0 = ws[int(np.argmin([np.prod(w["gridSize"]) for w in ws]))]
+        base = cpu_baseline(w0, args.cpu_seconds,
+                            what=f"the smallest instance ({'x'.join(map(str, w0['gridSize']))}, Mur + port)")
+    sizes = [int(np.prod(w["gridSize"])) for w in ws]
+    result = {
+        "metric": METRIC, "value": round(value, 1), "unit": "MCells/s", "n_gpus": 1, "steps": K, "warmup": W,
+        "ms_per_step": round(elapsed / K * 1e3, 4), "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32" if storage == "fp32" else "f16 storage, f32 compute", "data": "synthetic",
+        "config": {"workload": f"batched design sweep: {n_inst} microstrip patch antennas (width 20-50 mm from a "
+                               f"seed, graded meshes), Mur on 6 faces, 50-ohm lumped port, Gaussian f0=fc=3 GHz, "
+                               f"probe gather every step; aggregate over instances",
+                   "instances": n_inst, "cells_total": int(cells), "cells_min": min(sizes), "cells_max": max(sizes),
+                   "seed": int(__import__("efs_amd.workloads", fromlist=["x"]).BATCH_SEED),
+                   "tuned_shape": shape, "graph_steps": info["graph_steps"], "parallelism": "single GPU"},
+        "roofline": {"bound": "hbm", "kernel": "batched " + dom_name, "achieved": round(dom_gbs, 1),
+                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(dom_gbs / HBM_PEAK_GBS, 4),
+                     "traffic": None, "traffic_source": None, "algorithmic_bytes_per_launch": by,
+                     "avg_launch_ms": round(dom_ms, 5),
+                     "kernels": {"sweep_e": {"avg_ms": round(avg_e, 5), "GB/s": round(by / (avg_e * 1e-3) / 1e9, 1)},
+                                 "sweep_h": {"avg_ms": round(avg_h, 5), "GB/s": round(by / (avg_h * 1e-3) / 1e9, 1)}}},
+        "cpu_baseline": base, **extra,
+        "checks": {"host_build_s": round(t_build, 1)},
+    }
+    print(json.dumps(result))
+
+
+def calibrate_copy():
+    import torch
+    a = torch.ones(256 * 1024 * 1024, dtype=torch.float32, device="cuda")
+    b = torch.empty_like(a)
+    torch.mul(a, 1.0, out=b)
+    torch.cuda.synchronize()
+    del a, b
 
 
 def main():
@@ -143,6 +314,14 @@ def main():
                          "FP32 updates)")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="run one elementwise copy of 1 GiB -> 1 GiB first (reference kernel for HBM counters)")
+    ap.add_argument("--workload", choices=["stress", "batch"], default="stress",
+                    help="stress: the 256^3 UPML stress grid; batch: the batched patch-antenna design sweep")
+    ap.add_argument("--instances", type=int, default=64, help="batch: number of patch instances")
+    ap.add_argument("--plain-steps", type=int, default=200, help="batch: steps per instance of the plain baseline")
+    ap.add_argument("--s11-instances", type=int, default=3, help="batch: instances in the FP64 S11 comparison")
+    ap.add_argument("--s11-steps", type=int, default=20000, help="batch: steps of the FP64 S11 comparison")
+    ap.add_argument("--no-compare", action="store_true",
+                    help="skip the comparison runs (FP32 speed and FP64 error) of --storage fp16 / batch")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -150,13 +329,13 @@ def main():
                                    "of this build (use --gpus 1)", "n_gpus": args.gpus}))
         sys.exit(2)
 
+    if args.workload == "batch":
+        if args.calibrate_copy:
+            calibrate_copy()
+        return run_batch(args)
+
     if args.calibrate_copy:
-        import torch
-        a = torch.ones(256 * 1024 * 1024, dtype=torch.float32, device="cuda")
-        b = torch.empty_like(a)
-        torch.mul(a, 1.0, out=b)
-        torch.cuda.synchronize()
-        del a, b
+        calibrate_copy()
 
     from efs_amd import workloads
     from efs_amd.engine import PROF_STEP, PROF_SWEEP_E, PROF_SWEEP_H, HIPEngine
@@ -209,7 +388,7 @@ def main():
     finite = bool(np.all(np.isfinite(acc["accumE"])) and np.all(np.isfinite(energies)))
 
     extra = {}
-    if args.storage == "fp16":
+    if args.storage == "fp16" and not args.no_compare:
         # the same steps on the plain FP32 engine, and the error against FP64
         eng.destroy()
         e32 = HIPEngine()

@@ -43,13 +43,18 @@ namespace batch {
 struct Inst {
     void* volt;         // T[3*S]
     void* curr;
-    const float* vv;    // float[3*S] each
+    const float* vv;    // float[3*S] each (instances without 16-bit indices)
     const float* vi;
     const float* ii;
     const float* iv;
+    const uint16_t* ie; // 16-bit coefficient indices, E and H sides, [3*S] each
+    const uint16_t* ih;
+    const float2* te;   // their tables, [3][tabn] (vv, vi) and (ii, iv)
+    const float2* th;
     uint64_t S;         // component stride
     uint32_t nx, ny, nz;  // nz padded to a multiple of 4
     uint32_t cols;      // ny * nz
+    uint32_t tabn_e, tabn_h;  // table rows per component; 0 = FP32 arrays
 };
 
 struct Block {
@@ -83,14 +88,27 @@ __device__ __forceinline__ void bsweep_body(const Inst* __restrict__ insts, cons
     w.region = -1;
     const PmlTable pml{};  // no UPML in the batched form: column mask 0, never read
     const Scales sc = load_scales<T>(fscale + 6 * d.inst, HF);
+    // the coefficient form is per instance (uniform over the workgroup)
     if (HF) {
-        const Coef cf{I.ii, I.iv, nullptr, nullptr, nullptr, nullptr};
-        h4_march<kMarchInterior, false, false, T>(static_cast<T*>(I.curr), static_cast<const T*>(I.volt), cf,
-                                                  nullptr, nullptr, g, pml, nullptr, nullptr, w, 0u, sc);
+        if (I.tabn_h) {
+            const Coef cf{nullptr, nullptr, nullptr, I.th, nullptr, nullptr, I.ih, I.tabn_h};
+            h4_march<kMarchInterior, kCoefIdx16, false, T>(static_cast<T*>(I.curr), static_cast<const T*>(I.volt),
+                                                           cf, nullptr, nullptr, g, pml, nullptr, nullptr, w, 0u, sc);
+        } else {
+            const Coef cf{I.ii, I.iv, nullptr, nullptr, nullptr, nullptr, nullptr, 0u};
+            h4_march<kMarchInterior, kCoefF32, false, T>(static_cast<T*>(I.curr), static_cast<const T*>(I.volt), cf,
+                                                         nullptr, nullptr, g, pml, nullptr, nullptr, w, 0u, sc);
+        }
     } else {
-        const Coef cf{I.vv, I.vi, nullptr, nullptr, nullptr, nullptr};
-        e4_march<kMarchInterior, false, false, T>(static_cast<T*>(I.volt), static_cast<const T*>(I.curr), cf,
-                                                  nullptr, nullptr, g, pml, nullptr, nullptr, w, 0u, sc);
+        if (I.tabn_e) {
+            const Coef cf{nullptr, nullptr, nullptr, I.te, nullptr, nullptr, I.ie, I.tabn_e};
+            e4_march<kMarchInterior, kCoefIdx16, false, T>(static_cast<T*>(I.volt), static_cast<const T*>(I.curr),
+                                                           cf, nullptr, nullptr, g, pml, nullptr, nullptr, w, 0u, sc);
+        } else {
+            const Coef cf{I.vv, I.vi, nullptr, nullptr, nullptr, nullptr, nullptr, 0u};
+            e4_march<kMarchInterior, kCoefF32, false, T>(static_cast<T*>(I.volt), static_cast<const T*>(I.curr), cf,
+                                                         nullptr, nullptr, g, pml, nullptr, nullptr, w, 0u, sc);
+        }
     }
 }
 
@@ -122,35 +140,44 @@ __device__ __forceinline__ FieldView bview(const Inst& I, bool curr, const float
 }
 
 // Mur pre / post / apply (src/webgpu-fdtd.mjs:992-1023), points of all
-// instances in one list.
+// instances in one list, ordered by (instance, normal index) so that
+// neighbouring threads touch neighbouring cells.
+struct MurPt {
+    uint32_t nidx, sidx;  // normal and shifted cell (device index in the instance)
+    float coeff;
+    uint32_t inst;
+};
+// Apply list: the last entry of every normal index of an instance (the
+// sequential order of the reference's loop); src = that entry's position.
+struct MurApply {
+    uint32_t nidx, src, inst, pad;
+};
+
 __global__ void k_bmur_pre(const Inst* __restrict__ insts, const float* __restrict__ fscale, uint32_t half,
-                           const uint32_t* __restrict__ inst, const float* __restrict__ coeff,
-                           const uint32_t* __restrict__ nidx, const uint32_t* __restrict__ sidx,
-                           float* __restrict__ saved, uint32_t n) {
+                           const MurPt* __restrict__ pts, float* __restrict__ saved, uint32_t n) {
     const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= n) return;
-    const FieldView v = bview(insts[inst[k]], false, fscale, inst[k], half);
-    saved[k] = v.ld(sidx[k]) - coeff[k] * v.ld(nidx[k]);
+    const MurPt p = pts[k];
+    const FieldView v = bview(insts[p.inst], false, fscale, p.inst, half);
+    saved[k] = v.ld(p.sidx) - p.coeff * v.ld(p.nidx);
 }
 
 __global__ void k_bmur_post(const Inst* __restrict__ insts, const float* __restrict__ fscale, uint32_t half,
-                            const uint32_t* __restrict__ inst, const float* __restrict__ coeff,
-                            const uint32_t* __restrict__ sidx, float* __restrict__ saved, uint32_t n) {
+                            const MurPt* __restrict__ pts, float* __restrict__ saved, uint32_t n) {
     const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= n) return;
-    const FieldView v = bview(insts[inst[k]], false, fscale, inst[k], half);
-    saved[k] += coeff[k] * v.ld(sidx[k]);
+    const MurPt p = pts[k];
+    const FieldView v = bview(insts[p.inst], false, fscale, p.inst, half);
+    saved[k] += p.coeff * v.ld(p.sidx);
 }
 
-// Apply list: the last entry of every normal index of an instance (the
-// sequential order of the reference's loop), src = its entry number.
 __global__ void k_bmur_apply(const Inst* __restrict__ insts, const float* __restrict__ fscale, uint32_t half,
-                             const uint32_t* __restrict__ inst, const uint32_t* __restrict__ nidx,
-                             const uint32_t* __restrict__ src, const float* __restrict__ saved, uint32_t n) {
+                             const MurApply* __restrict__ ap, const float* __restrict__ saved, uint32_t n) {
     const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= n) return;
-    const FieldView v = bview(insts[inst[k]], false, fscale, inst[k], half);
-    v.st(nidx[k], saved[src[k]]);
+    const MurApply a = ap[k];
+    const FieldView v = bview(insts[a.inst], false, fscale, a.inst, half);
+    v.st(a.nidx, saved[a.src]);
 }
 
 // Excitation (src/webgpu-fdtd.mjs:161-183 semantics, as k_excitation) with
@@ -283,6 +310,9 @@ struct HostInst {
     uint64_t N, S;
     void *volt = nullptr, *curr = nullptr;
     float *vv = nullptr, *vi = nullptr, *ii = nullptr, *iv = nullptr;
+    uint16_t *ie = nullptr, *ih = nullptr;
+    float2 *te = nullptr, *th = nullptr;
+    uint32_t tabn_e = 0, tabn_h = 0;
     // configured point lists (instance-local device indices)
     std::vector<float> sig;
     std::vector<ExcPoint> exc;  // off relative to this instance's signal
@@ -322,9 +352,9 @@ struct efs_batch {
 
     // Mur
     uint32_t mur_n = 0, mur_an = 0;
-    uint32_t *d_mur_inst = nullptr, *d_mur_n = nullptr, *d_mur_s = nullptr;
-    uint32_t *d_mur_ainst = nullptr, *d_mur_an = nullptr, *d_mur_asrc = nullptr;  // apply list
-    float *d_mur_c = nullptr, *d_mur_saved = nullptr;
+    MurPt* d_mur = nullptr;
+    MurApply* d_mur_apply = nullptr;
+    float* d_mur_saved = nullptr;
     // excitation
     uint32_t exc_n = 0;
     ExcPoint* d_exc = nullptr;
@@ -493,21 +523,18 @@ void launch_step(efs_batch* b, bool range) {
     const uint32_t half = b->half ? 1u : 0u;
     if (b->mur_n)
         hipLaunchKernelGGL(k_bmur_pre, dim3(nblk(b->mur_n)), dim3(kPt), 0, b->stream, (const Inst*)b->d_inst,
-                           (const float*)b->d_fscale, half, (const uint32_t*)b->d_mur_inst, (const float*)b->d_mur_c,
-                           (const uint32_t*)b->d_mur_n, (const uint32_t*)b->d_mur_s, b->d_mur_saved, b->mur_n);
+                           (const float*)b->d_fscale, half, (const MurPt*)b->d_mur, b->d_mur_saved, b->mur_n);
     launch_sweep(b, false);
     if (b->mur_n)
         hipLaunchKernelGGL(k_bmur_post, dim3(nblk(b->mur_n)), dim3(kPt), 0, b->stream, (const Inst*)b->d_inst,
-                           (const float*)b->d_fscale, half, (const uint32_t*)b->d_mur_inst, (const float*)b->d_mur_c,
-                           (const uint32_t*)b->d_mur_s, b->d_mur_saved, b->mur_n);
+                           (const float*)b->d_fscale, half, (const MurPt*)b->d_mur, b->d_mur_saved, b->mur_n);
     if (b->exc_n)
         hipLaunchKernelGGL(k_bexcitation, dim3(nblk(b->exc_n)), dim3(kPt), 0, b->stream, (const Inst*)b->d_inst,
                            (const float*)b->d_fscale, half, (const ExcPoint*)b->d_exc, (const float*)b->d_sig,
                            b->exc_n, (const uint32_t*)b->d_ts);
     if (b->mur_n)
         hipLaunchKernelGGL(k_bmur_apply, dim3(nblk(b->mur_an)), dim3(kPt), 0, b->stream, (const Inst*)b->d_inst,
-                           (const float*)b->d_fscale, half, (const uint32_t*)b->d_mur_ainst,
-                           (const uint32_t*)b->d_mur_an, (const uint32_t*)b->d_mur_asrc,
+                           (const float*)b->d_fscale, half, (const MurApply*)b->d_mur_apply,
                            (const float*)b->d_mur_saved, b->mur_an);
     launch_sweep(b, true);
     if (b->half && range) {
@@ -569,6 +596,50 @@ int run_steps(efs_batch* b, uint32_t n) {
         }
     }
     return EFS_OK;
+}
+
+// Distinct (a, b) pairs of each of the 3 components (S values apart):
+// 16-bit index per value and a [3][tabn] table (tabn = the largest count,
+// rounded up to 64).  False when a component has more than 65536 pairs.
+bool pair_index16(const float* a, const float* b, uint64_t S, std::vector<uint16_t>& idx, std::vector<float2>& tab,
+                  uint32_t& tabn) {
+    constexpr uint32_t H = 1u << 17;  // open-addressing slots (load <= 1/2)
+    std::vector<std::vector<float2>> per(3);
+    idx.assign(3 * S, 0);
+    std::vector<uint64_t> key(H);
+    std::vector<int32_t> val(H);
+    for (uint32_t n = 0; n < 3; ++n) {
+        std::fill(val.begin(), val.end(), -1);
+        uint64_t last_k = ~0ull;
+        uint16_t last_v = 0;
+        for (uint64_t i = n * S; i < (n + 1) * S; ++i) {
+            uint32_t ua, ub;
+            std::memcpy(&ua, a + i, 4);
+            std::memcpy(&ub, b + i, 4);
+            const uint64_t k = ((uint64_t)ua << 32) | ub;
+            if (k == last_k) {
+                idx[i] = last_v;
+                continue;
+            }
+            uint32_t hsh = (uint32_t)((k * 0x9E3779B97F4A7C15ull) >> 47) & (H - 1);
+            while (val[hsh] >= 0 && key[hsh] != k) hsh = (hsh + 1) & (H - 1);
+            if (val[hsh] < 0) {
+                if (per[n].size() >= 65536) return false;
+                key[hsh] = k;
+                val[hsh] = (int32_t)per[n].size();
+                per[n].push_back(float2{a[i], b[i]});
+            }
+            last_k = k;
+            last_v = (uint16_t)val[hsh];
+            idx[i] = last_v;
+        }
+    }
+    size_t m = 1;
+    for (auto& v : per) m = std::max(m, v.size());
+    tabn = (uint32_t)((m + 63) & ~(size_t)63);
+    tab.assign(3 * (size_t)tabn, float2{0.f, 0.f});
+    for (uint32_t n = 0; n < 3; ++n) std::copy(per[n].begin(), per[n].end(), tab.begin() + n * tabn);
+    return true;
 }
 
 int need_open(efs_batch* b) {
@@ -673,6 +744,10 @@ void efs_batch_destroy(efs_batch* b) {
         bfree(h.vi);
         bfree(h.ii);
         bfree(h.iv);
+        bfree(h.ie);
+        bfree(h.ih);
+        bfree(h.te);
+        bfree(h.th);
     }
     bfree(b->d_inst);
     bfree(b->d_tab);
@@ -683,14 +758,9 @@ void efs_batch_destroy(efs_batch* b) {
     bfree(b->d_amax);
     bfree(b->d_src_end);
     bfree(b->d_segs);
-    bfree(b->d_mur_inst);
-    bfree(b->d_mur_n);
-    bfree(b->d_mur_s);
-    bfree(b->d_mur_c);
+    bfree(b->d_mur);
+    bfree(b->d_mur_apply);
     bfree(b->d_mur_saved);
-    bfree(b->d_mur_ainst);
-    bfree(b->d_mur_an);
-    bfree(b->d_mur_asrc);
     bfree(b->d_exc);
     bfree(b->d_sig);
     bfree(b->d_g_inst);
@@ -733,32 +803,58 @@ int efs_batch_add(efs_batch* b, uint32_t nx, uint32_t ny, uint32_t nz, const flo
     if ((uint64_t)ny * h.nzp >= (1ull << 32) / 4) return bfail(b, EFS_ERR_ARG, "plane too large");
     BHIP(b, hipSetDevice(b->device));
     // padded copies: pad cells zero; H last real z plane ii = 1, iv = 0
-    std::vector<float> buf(3 * h.S);
-    const float* src[4] = {vv, vi, ii, iv};
-    float** dst[4] = {&h.vv, &h.vi, &h.ii, &h.iv};
-    for (int a = 0; a < 4; ++a) {
-        std::fill(buf.begin(), buf.end(), 0.0f);
+    std::vector<float> pa(3 * h.S), pb(3 * h.S);
+    for (int side = 0; side < 2; ++side) {
+        const float* sa = side ? ii : vv;
+        const float* sb = side ? iv : vi;
+        std::fill(pa.begin(), pa.end(), 0.0f);
+        std::fill(pb.begin(), pb.end(), 0.0f);
         for (uint32_t n = 0; n < 3; ++n)
             for (uint64_t xy = 0; xy < (uint64_t)nx * ny; ++xy) {
-                const float* s = src[a] + n * h.N + xy * nz;
-                float* d = buf.data() + n * h.S + xy * h.nzp;
-                std::memcpy(d, s, nz * sizeof(float));
-                if (a >= 2) d[nz - 1] = a == 2 ? 1.0f : 0.0f;
+                std::memcpy(pa.data() + n * h.S + xy * h.nzp, sa + n * h.N + xy * nz, nz * sizeof(float));
+                std::memcpy(pb.data() + n * h.S + xy * h.nzp, sb + n * h.N + xy * nz, nz * sizeof(float));
+                if (side) {
+                    pa[n * h.S + xy * h.nzp + nz - 1] = 1.0f;
+                    pb[n * h.S + xy * h.nzp + nz - 1] = 0.0f;
+                }
             }
-        float* p = nullptr;
-        const int rc = bupload(b, p, buf.data(), buf.size());
+        // 16-bit index form when every component has at most 65536 distinct
+        // (a, b) pairs, else the FP32 arrays
+        std::vector<uint16_t> idx;
+        std::vector<float2> tab;
+        uint32_t tabn = 0;
+        int rc = EFS_OK;
+        if (pair_index16(pa.data(), pb.data(), h.S, idx, tab, tabn)) {
+            uint16_t*& di = side ? h.ih : h.ie;
+            float2*& dt = side ? h.th : h.te;
+            rc = bupload(b, di, idx.data(), idx.size());
+            if (!rc) rc = bupload(b, dt, tab.data(), tab.size());
+            (side ? h.tabn_h : h.tabn_e) = tabn;
+        } else {
+            float*& da = side ? h.ii : h.vv;
+            float*& db = side ? h.iv : h.vi;
+            rc = bupload(b, da, pa.data(), pa.size());
+            if (!rc) rc = bupload(b, db, pb.data(), pb.size());
+        }
         if (rc) {
-            for (int k = 0; k < a; ++k) bfree(*dst[k]);
+            for (float** q : {&h.vv, &h.vi, &h.ii, &h.iv}) bfree(*q);
+            bfree(h.ie);
+            bfree(h.ih);
+            bfree(h.te);
+            bfree(h.th);
             return rc;
         }
-        *dst[a] = p;
     }
     const size_t fb = (b->half ? 2 : 4) * 3 * h.S;
     hipError_t e1 = hipMalloc(&h.volt, fb), e2 = hipMalloc(&h.curr, fb);
     if (e1 != hipSuccess || e2 != hipSuccess) {
         bfree(h.volt);
         bfree(h.curr);
-        for (int k = 0; k < 4; ++k) bfree(*dst[k]);
+        for (float** q : {&h.vv, &h.vi, &h.ii, &h.iv}) bfree(*q);
+        bfree(h.ie);
+        bfree(h.ih);
+        bfree(h.te);
+        bfree(h.th);
         return bfail(b, EFS_ERR_NOMEM, "field allocation failed");
     }
     b->inst.push_back(h);
@@ -843,44 +939,32 @@ int efs_batch_finalize(efs_batch* b, uint32_t sample_interval, uint32_t max_samp
     std::vector<Inst> it(ni);
     std::vector<uint32_t> src_end(ni);
     std::vector<Seg> segs;
-    std::vector<uint32_t> mi, mn, ms, gi, gx, ai, an, asrc;
-    std::vector<float> mc, sig;
+    std::vector<uint32_t> gi, gx;
+    std::vector<MurPt> mur;
+    std::vector<MurApply> mapply;
+    std::vector<float> sig;
     std::vector<ExcPoint> exc;
     b->g_off.assign(ni + 1, 0);
     for (uint32_t q = 0; q < ni; ++q) {
         HostInst& h = b->inst[q];
-        it[q] = Inst{h.volt, h.curr, h.vv, h.vi, h.ii, h.iv, h.S, h.nx, h.ny, h.nzp, h.ny * h.nzp};
+        it[q] = Inst{h.volt, h.curr, h.vv, h.vi, h.ii, h.iv, h.ie, h.ih, h.te, h.th,
+                     h.S, h.nx, h.ny, h.nzp, h.ny * h.nzp, h.tabn_e, h.tabn_h};
         if (b->half) source_scales(h);
         src_end[q] = h.src_end;
         for (uint32_t k = 0; k < 6; ++k)
             segs.push_back(Seg{static_cast<f16*>(k < 3 ? h.volt : h.curr) + (k % 3) * h.S,
                                (uint32_t)((uint64_t)h.nx * h.ny * h.nzp), 6 * q + k});
-        const uint32_t m0 = (uint32_t)mc.size();
-        for (size_t k = 0; k < h.mur_c.size(); ++k) {
-            mi.push_back(q);
-            mn.push_back(h.mur_n[k]);
-            ms.push_back(h.mur_s[k]);
-            mc.push_back(h.mur_c[k]);
-        }
         {
-            // last entry per normal index, in entry order
-            std::vector<uint32_t> keys(h.mur_n);
-            std::sort(keys.begin(), keys.end());
-            keys.erase(std::unique(keys.begin(), keys.end()), keys.end());
-            std::vector<uint8_t> seen(keys.size(), 0);
-            std::vector<uint32_t> win;
-            for (size_t k = h.mur_n.size(); k-- > 0;) {
-                const size_t j = std::lower_bound(keys.begin(), keys.end(), h.mur_n[k]) - keys.begin();
-                if (!seen[j]) {
-                    seen[j] = 1;
-                    win.push_back((uint32_t)k);
-                }
-            }
-            std::reverse(win.begin(), win.end());
-            for (uint32_t k : win) {
-                ai.push_back(q);
-                an.push_back(h.mur_n[k]);
-                asrc.push_back(m0 + k);
+            // entries ordered by normal index (stable: equal indices keep
+            // their order); the apply list holds each index's last entry
+            const uint32_t m0 = (uint32_t)mur.size(), nm = (uint32_t)h.mur_c.size();
+            std::vector<uint32_t> ord(nm);
+            for (uint32_t k = 0; k < nm; ++k) ord[k] = k;
+            std::stable_sort(ord.begin(), ord.end(), [&](uint32_t a, uint32_t c) { return h.mur_n[a] < h.mur_n[c]; });
+            for (uint32_t j = 0; j < nm; ++j) {
+                const uint32_t k = ord[j];
+                mur.push_back(MurPt{h.mur_n[k], h.mur_s[k], h.mur_c[k], q});
+                if (j + 1 == nm || h.mur_n[ord[j + 1]] != h.mur_n[k]) mapply.push_back(MurApply{h.mur_n[k], m0 + j, q, 0});
             }
         }
         const uint32_t off = (uint32_t)sig.size();
@@ -911,18 +995,13 @@ int efs_batch_finalize(efs_batch* b, uint32_t sample_interval, uint32_t max_samp
         for (const Seg& s : segs) mx = std::max<uint64_t>(mx, s.n);
         b->seg_chunks = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(64, (mx / 4 + 255) / 256));
     }
-    b->mur_n = (uint32_t)mc.size();
+    b->mur_n = (uint32_t)mur.size();
+    b->mur_an = (uint32_t)mapply.size();
     if (b->mur_n) {
-        BTRY(bupload(b, b->d_mur_inst, mi.data(), mi.size()));
-        BTRY(bupload(b, b->d_mur_n, mn.data(), mn.size()));
-        BTRY(bupload(b, b->d_mur_s, ms.data(), ms.size()));
-        BTRY(bupload(b, b->d_mur_c, mc.data(), mc.size()));
-        BTRY(balloc(b, b->d_mur_saved, mc.size()));
-        BTRY(bupload(b, b->d_mur_ainst, ai.data(), ai.size()));
-        BTRY(bupload(b, b->d_mur_an, an.data(), an.size()));
-        BTRY(bupload(b, b->d_mur_asrc, asrc.data(), asrc.size()));
+        BTRY(bupload(b, b->d_mur, mur.data(), mur.size()));
+        BTRY(bupload(b, b->d_mur_apply, mapply.data(), mapply.size()));
+        BTRY(balloc(b, b->d_mur_saved, mur.size()));
     }
-    b->mur_an = (uint32_t)an.size();
     b->exc_n = (uint32_t)exc.size();
     if (b->exc_n) {
         BTRY(bupload(b, b->d_exc, exc.data(), exc.size()));

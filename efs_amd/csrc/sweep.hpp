@@ -270,11 +270,24 @@ struct Coef {
     // iifn for the H sweep -- when each component has at most 256 triples.
     const uint8_t* pidx;
     const float4* ptab;
+    // 16-bit form (kCoefIdx16): one index per cell and component into a
+    // [3][tabn] table of (a, b) pairs read through the caches (graded
+    // meshes have a few hundred distinct pairs per component, more than
+    // the byte form holds).
+    const uint16_t* idx16;
+    uint32_t tabn;
 };
 
-template <bool COMP>
+// Coefficient forms of the sweeps (template argument COMP).
+enum : int {
+    kCoefF32 = 0,    // two FP32 arrays
+    kCoefIdx8 = 1,   // byte index, 256-entry table staged in LDS
+    kCoefIdx16 = 2,  // 16-bit index, table in global memory (cached)
+};
+
+template <int COMP>
 __device__ __forceinline__ void coef_stage(const Coef& cf, float2* s_tab) {
-    if (COMP) {
+    if (COMP == kCoefIdx8) {
         for (int t = threadIdx.x; t < 3 * kTab; t += kSweepThreads) s_tab[t] = cf.tab[t];
         __syncthreads();
     }
@@ -293,12 +306,20 @@ __device__ __forceinline__ void upml_stage(const Coef& cf, float4* s_ptab, uint3
 // Two steps so a caller can issue every global load of an iteration before
 // the first wait: coef_fetch loads the byte indices (or the FP32 arrays),
 // coef_decode turns indices into values through the LDS table.
-template <bool COMP>
-__device__ __forceinline__ void coef_fetch(const Coef& cf, size_t S, uint32_t i, uint32_t q[3], v4 a[3], v4 b[3]) {
-    if (COMP) {
+template <int COMP>
+__device__ __forceinline__ void coef_fetch(const Coef& cf, size_t S, uint32_t i, uint32_t q[6], v4 a[3], v4 b[3]) {
+    if (COMP == kCoefIdx8) {
 #pragma unroll
         for (int n = 0; n < 3; ++n)
             q[n] = __builtin_nontemporal_load(reinterpret_cast<const uint32_t*>(cf.idx + n * S + i));
+    } else if (COMP == kCoefIdx16) {
+#pragma unroll
+        for (int n = 0; n < 3; ++n) {
+            typedef uint32_t u2 __attribute__((ext_vector_type(2)));
+            const u2 w = __builtin_nontemporal_load(reinterpret_cast<const u2*>(cf.idx16 + n * S + i));
+            q[2 * n] = w.x;
+            q[2 * n + 1] = w.y;
+        }
     } else {
 #pragma unroll
         for (int n = 0; n < 3; ++n) {
@@ -308,9 +329,19 @@ __device__ __forceinline__ void coef_fetch(const Coef& cf, size_t S, uint32_t i,
     }
 }
 
-template <bool COMP>
-__device__ __forceinline__ void coef_decode(const float2* s_tab, const uint32_t q[3], v4 a[3], v4 b[3]) {
-    if (COMP) {
+template <int COMP>
+__device__ __forceinline__ void coef_decode(const float2* s_tab, const uint32_t q[6], v4 a[3], v4 b[3],
+                                            const Coef& cf) {
+    if (COMP == kCoefIdx16) {
+#pragma unroll
+        for (int n = 0; n < 3; ++n) {
+            const float2* t = cf.tab + (size_t)n * cf.tabn;
+            const float2 p0 = t[q[2 * n] & 0xffffu], p1 = t[q[2 * n] >> 16], p2 = t[q[2 * n + 1] & 0xffffu],
+                         p3 = t[q[2 * n + 1] >> 16];
+            a[n] = (v4){p0.x, p1.x, p2.x, p3.x};
+            b[n] = (v4){p0.y, p1.y, p2.y, p3.y};
+        }
+    } else if (COMP == kCoefIdx8) {
 #pragma unroll
         for (int n = 0; n < 3; ++n) {
             const float2* t = s_tab + n * kTab;
@@ -540,7 +571,7 @@ __device__ __forceinline__ void upml_decode(const float4* s_ptab, const uint32_t
 
 // E update of 4 cells over x in [x0, x1); MODE says how UPML cells are
 // treated (column mask m gives the regions the lane's column crosses).
-template <int MODE, bool COMP, bool PCOMP, typename T>
+template <int MODE, int COMP, bool PCOMP, typename T>
 __device__ __forceinline__ void e4_march(T* __restrict__ volt, const T* __restrict__ curr, const Coef& cf,
                                          const float2* s_tab, const float4* s_ptab, const SweepGeom& g,
                                          const PmlTable& pml,
@@ -575,14 +606,14 @@ __device__ __forceinline__ void e4_march(T* __restrict__ volt, const T* __restri
         const v4 hz_ym = L::ld(H2 + i - sy, sc.other[2]), hx_ym = L::ld(H0 + i - sy, sc.other[0]);
         const float hy_z = L::ld1(H1 + i - sz, sc.other[1]), hx_z = L::ld1(H0 + i - sz, sc.other[0]);  // z-1
         v4 V[3], a[3], b[3];
-        uint32_t q[3];
+        uint32_t q[6];
 #pragma unroll
         for (int n = 0; n < 3; ++n) V[n] = L::ld(E[n] + i, sc.own[n]);
         coef_fetch<COMP>(cf, S, i, q, a, b);
         v4 F[3], P0[3], P1[3], P2[3];
         uint32_t pq[3];
         if (upml) upml_fetch<PCOMP>(u, F, P0, P1, P2, pq);
-        coef_decode<COMP>(s_tab, q, a, b);
+        coef_decode<COMP>(s_tab, q, a, b, cf);
         const v4 hy_zm = (v4){hy_z, hy.x, hy.y, hy.z};
         const v4 hx_zm = (v4){hx_z, hx.x, hx.y, hx.z};
         const v4 cur[3] = {hz - hz_ym - hy + hy_zm, hx - hx_zm - hz + hz_m, hy - hy_m - hx + hx_ym};
@@ -606,7 +637,7 @@ __device__ __forceinline__ void e4_march(T* __restrict__ volt, const T* __restri
 
 // H update of 4 cells over x in [x0, x1) (see e4_march).  Cells outside the
 // core bounds (x = Nx-1, y = Ny-1, z = Nz-1) keep I unless in UPML.
-template <int MODE, bool COMP, bool PCOMP, typename T>
+template <int MODE, int COMP, bool PCOMP, typename T>
 __device__ __forceinline__ void h4_march(T* __restrict__ curr, const T* __restrict__ volt, const Coef& cf,
                                          const float2* s_tab, const float4* s_ptab, const SweepGeom& g,
                                          const PmlTable& pml,
@@ -650,14 +681,14 @@ __device__ __forceinline__ void h4_march(T* __restrict__ curr, const T* __restri
         const v4 ez_yp = L::ld(E2 + i + sy, sc.other[2]), ex_yp = L::ld(E0 + i + sy, sc.other[0]);
         const float ey_z = L::ld1(E1 + i + sz, sc.other[1]), ex_z = L::ld1(E0 + i + sz, sc.other[0]);  // z+1
         v4 I[3], a[3], b[3];
-        uint32_t q[3];
+        uint32_t q[6];
 #pragma unroll
         for (int n = 0; n < 3; ++n) I[n] = L::ld(Ic[n] + i, sc.own[n]);
         coef_fetch<COMP>(cf, S, i, q, a, b);
         v4 F[3], P0[3], P1[3], P2[3];  // P: ii, iifo, iifn
         uint32_t pq[3];
         if (upml) upml_fetch<PCOMP>(u, F, P0, P1, P2, pq);
-        coef_decode<COMP>(s_tab, q, a, b);
+        coef_decode<COMP>(s_tab, q, a, b, cf);
         const v4 ey_zp = (v4){ey.y, ey.z, ey.w, ey_z};
         const v4 ex_zp = (v4){ex.y, ex.z, ex.w, ex_z};
         const v4 cur[3] = {ez - ez_yp - ey + ey_zp, ex - ex_zp - ez + ez_p, ey - ey_p - ex + ex_yp};

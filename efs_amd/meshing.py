@@ -172,5 +172,7 @@ def smooth_mesh_lines(lines, max_res, ratio=1.5):
 
 def grid_lines(fixed, max_res, ratio, extra=()):
     """The script API's pattern: smooth the fixed lines, then add lines that
-    must stay exact (feeds, ports) unsmoothed; the grid is the sorted set."""
-    return np.array(sorted(set(smooth_mesh_lines(fixed, max_res, ratio)) | set(float(v) for v in extra)))
+    must stay exact (feeds, ports) unsmoothed.  The grid is the sorted set
+    with near-duplicates merged as unique_lines does (a mirrored centre line
+    can land a rounding error away from an added exact line)."""
+    return np.array(unique_lines(list(smooth_mesh_lines(fixed, max_res, ratio)) + [float(v) for v in extra]))
