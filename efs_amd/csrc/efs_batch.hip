@@ -139,18 +139,15 @@ __device__ __forceinline__ FieldView bview(const Inst& I, bool curr, const float
     return FieldView{curr ? I.curr : I.volt, fscale + 6 * inst + (curr ? 3 : 0), I.S, half};
 }
 
-// Mur pre / post / apply (src/webgpu-fdtd.mjs:992-1023), points of all
-// instances in one list, ordered by (instance, normal index) so that
-// neighbouring threads touch neighbouring cells.
+// Mur ABC (src/webgpu-fdtd.mjs:992-1023), the single-grid engine's scheme
+// (k_mur_pre / k_mur_post_some / k_mur_finish): one entry per normal cell
+// (its last), ordered by (instance, cell); entries whose shifted cell is
+// written between post and apply post early (flag kMurEarly).
+constexpr uint32_t kMurEarly = 0x80000000u;
 struct MurPt {
     uint32_t nidx, sidx;  // normal and shifted cell (device index in the instance)
     float coeff;
-    uint32_t inst;
-};
-// Apply list: the last entry of every normal index of an instance (the
-// sequential order of the reference's loop); src = that entry's position.
-struct MurApply {
-    uint32_t nidx, src, inst, pad;
+    uint32_t inst;        // instance | kMurEarly
 };
 
 __global__ void k_bmur_pre(const Inst* __restrict__ insts, const float* __restrict__ fscale, uint32_t half,
@@ -158,26 +155,32 @@ __global__ void k_bmur_pre(const Inst* __restrict__ insts, const float* __restri
     const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= n) return;
     const MurPt p = pts[k];
-    const FieldView v = bview(insts[p.inst], false, fscale, p.inst, half);
-    saved[k] = v.ld(p.sidx) - p.coeff * v.ld(p.nidx);
+    const uint32_t q = p.inst & ~kMurEarly;
+    const FieldView v = bview(insts[q], false, fscale, q, half);
+    saved[k] = __builtin_fmaf(-p.coeff, v.ld(p.nidx), v.ld(p.sidx));
 }
 
-__global__ void k_bmur_post(const Inst* __restrict__ insts, const float* __restrict__ fscale, uint32_t half,
-                            const MurPt* __restrict__ pts, float* __restrict__ saved, uint32_t n) {
+__global__ void k_bmur_post_some(const Inst* __restrict__ insts, const float* __restrict__ fscale, uint32_t half,
+                                 const MurPt* __restrict__ pts, const uint32_t* __restrict__ list,
+                                 float* __restrict__ saved, uint32_t n) {
+    const uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= n) return;
+    const uint32_t k = list[j];
+    const MurPt p = pts[k];
+    const uint32_t q = p.inst & ~kMurEarly;
+    const FieldView v = bview(insts[q], false, fscale, q, half);
+    saved[k] = __builtin_fmaf(p.coeff, v.ld(p.sidx), saved[k]);
+}
+
+__global__ void k_bmur_finish(const Inst* __restrict__ insts, const float* __restrict__ fscale, uint32_t half,
+                              const MurPt* __restrict__ pts, const float* __restrict__ saved, uint32_t n) {
     const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= n) return;
     const MurPt p = pts[k];
-    const FieldView v = bview(insts[p.inst], false, fscale, p.inst, half);
-    saved[k] += p.coeff * v.ld(p.sidx);
-}
-
-__global__ void k_bmur_apply(const Inst* __restrict__ insts, const float* __restrict__ fscale, uint32_t half,
-                             const MurApply* __restrict__ ap, const float* __restrict__ saved, uint32_t n) {
-    const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
-    if (k >= n) return;
-    const MurApply a = ap[k];
-    const FieldView v = bview(insts[a.inst], false, fscale, a.inst, half);
-    v.st(a.nidx, saved[a.src]);
+    const uint32_t q = p.inst & ~kMurEarly;
+    const FieldView v = bview(insts[q], false, fscale, q, half);
+    const float s = (p.inst & kMurEarly) ? saved[k] : __builtin_fmaf(p.coeff, v.ld(p.sidx), saved[k]);
+    v.st(p.nidx, s);
 }
 
 // Excitation (src/webgpu-fdtd.mjs:161-183 semantics, as k_excitation) with
@@ -351,9 +354,9 @@ struct efs_batch {
     uint32_t seg_chunks = 1;
 
     // Mur
-    uint32_t mur_n = 0, mur_an = 0;
+    uint32_t mur_n = 0, mur_ne = 0;
     MurPt* d_mur = nullptr;
-    MurApply* d_mur_apply = nullptr;
+    uint32_t* d_mur_early = nullptr;
     float* d_mur_saved = nullptr;
     // excitation
     uint32_t exc_n = 0;
@@ -525,17 +528,18 @@ void launch_step(efs_batch* b, bool range) {
         hipLaunchKernelGGL(k_bmur_pre, dim3(nblk(b->mur_n)), dim3(kPt), 0, b->stream, (const Inst*)b->d_inst,
                            (const float*)b->d_fscale, half, (const MurPt*)b->d_mur, b->d_mur_saved, b->mur_n);
     launch_sweep(b, false);
-    if (b->mur_n)
-        hipLaunchKernelGGL(k_bmur_post, dim3(nblk(b->mur_n)), dim3(kPt), 0, b->stream, (const Inst*)b->d_inst,
-                           (const float*)b->d_fscale, half, (const MurPt*)b->d_mur, b->d_mur_saved, b->mur_n);
+    if (b->mur_ne)
+        hipLaunchKernelGGL(k_bmur_post_some, dim3(nblk(b->mur_ne)), dim3(kPt), 0, b->stream, (const Inst*)b->d_inst,
+                           (const float*)b->d_fscale, half, (const MurPt*)b->d_mur, (const uint32_t*)b->d_mur_early,
+                           b->d_mur_saved, b->mur_ne);
     if (b->exc_n)
         hipLaunchKernelGGL(k_bexcitation, dim3(nblk(b->exc_n)), dim3(kPt), 0, b->stream, (const Inst*)b->d_inst,
                            (const float*)b->d_fscale, half, (const ExcPoint*)b->d_exc, (const float*)b->d_sig,
                            b->exc_n, (const uint32_t*)b->d_ts);
     if (b->mur_n)
-        hipLaunchKernelGGL(k_bmur_apply, dim3(nblk(b->mur_an)), dim3(kPt), 0, b->stream, (const Inst*)b->d_inst,
-                           (const float*)b->d_fscale, half, (const MurApply*)b->d_mur_apply,
-                           (const float*)b->d_mur_saved, b->mur_an);
+        hipLaunchKernelGGL(k_bmur_finish, dim3(nblk(b->mur_n)), dim3(kPt), 0, b->stream, (const Inst*)b->d_inst,
+                           (const float*)b->d_fscale, half, (const MurPt*)b->d_mur, (const float*)b->d_mur_saved,
+                           b->mur_n);
     launch_sweep(b, true);
     if (b->half && range) {
         const dim3 grid(b->seg_chunks, b->nsegs);
@@ -759,7 +763,7 @@ void efs_batch_destroy(efs_batch* b) {
     bfree(b->d_src_end);
     bfree(b->d_segs);
     bfree(b->d_mur);
-    bfree(b->d_mur_apply);
+    bfree(b->d_mur_early);
     bfree(b->d_mur_saved);
     bfree(b->d_exc);
     bfree(b->d_sig);
@@ -941,7 +945,7 @@ int efs_batch_finalize(efs_batch* b, uint32_t sample_interval, uint32_t max_samp
     std::vector<Seg> segs;
     std::vector<uint32_t> gi, gx;
     std::vector<MurPt> mur;
-    std::vector<MurApply> mapply;
+    std::vector<uint32_t> mur_early;
     std::vector<float> sig;
     std::vector<ExcPoint> exc;
     b->g_off.assign(ni + 1, 0);
@@ -955,16 +959,23 @@ int efs_batch_finalize(efs_batch* b, uint32_t sample_interval, uint32_t max_samp
             segs.push_back(Seg{static_cast<f16*>(k < 3 ? h.volt : h.curr) + (k % 3) * h.S,
                                (uint32_t)((uint64_t)h.nx * h.ny * h.nzp), 6 * q + k});
         {
-            // entries ordered by normal index (stable: equal indices keep
-            // their order); the apply list holds each index's last entry
-            const uint32_t m0 = (uint32_t)mur.size(), nm = (uint32_t)h.mur_c.size();
+            // one entry per normal cell (its last), ordered by the cell; an
+            // entry posts early when its shifted cell is another entry's
+            // normal cell or an excitation cell
+            const uint32_t nm = (uint32_t)h.mur_c.size();
             std::vector<uint32_t> ord(nm);
             for (uint32_t k = 0; k < nm; ++k) ord[k] = k;
             std::stable_sort(ord.begin(), ord.end(), [&](uint32_t a, uint32_t c) { return h.mur_n[a] < h.mur_n[c]; });
+            std::vector<uint32_t> written;
+            for (uint32_t j = 0; j < nm; ++j) written.push_back(h.mur_n[ord[j]]);
+            for (const ExcPoint& e : h.exc) written.push_back(e.gidx);
+            std::sort(written.begin(), written.end());
             for (uint32_t j = 0; j < nm; ++j) {
                 const uint32_t k = ord[j];
-                mur.push_back(MurPt{h.mur_n[k], h.mur_s[k], h.mur_c[k], q});
-                if (j + 1 == nm || h.mur_n[ord[j + 1]] != h.mur_n[k]) mapply.push_back(MurApply{h.mur_n[k], m0 + j, q, 0});
+                if (j + 1 < nm && h.mur_n[ord[j + 1]] == h.mur_n[k]) continue;
+                const bool early = std::binary_search(written.begin(), written.end(), h.mur_s[k]);
+                if (early) mur_early.push_back((uint32_t)mur.size());
+                mur.push_back(MurPt{h.mur_n[k], h.mur_s[k], h.mur_c[k], q | (early ? kMurEarly : 0u)});
             }
         }
         const uint32_t off = (uint32_t)sig.size();
@@ -996,10 +1007,10 @@ int efs_batch_finalize(efs_batch* b, uint32_t sample_interval, uint32_t max_samp
         b->seg_chunks = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(64, (mx / 4 + 255) / 256));
     }
     b->mur_n = (uint32_t)mur.size();
-    b->mur_an = (uint32_t)mapply.size();
+    b->mur_ne = (uint32_t)mur_early.size();
     if (b->mur_n) {
         BTRY(bupload(b, b->d_mur, mur.data(), mur.size()));
-        BTRY(bupload(b, b->d_mur_apply, mapply.data(), mapply.size()));
+        BTRY(bupload(b, b->d_mur_early, mur_early.data(), mur_early.size()));
         BTRY(balloc(b, b->d_mur_saved, mur.size()));
     }
     b->exc_n = (uint32_t)exc.size();

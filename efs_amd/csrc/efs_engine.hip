@@ -107,11 +107,14 @@ struct efs_ctx {
 
     // Mur
     uint32_t mur_n = 0;
-    std::vector<uint32_t> h_mur_normal, h_mur_shifted;
+    // entries kept: the last one of each normal cell, ordered by that cell
+    std::vector<uint32_t> h_mur_normal, h_mur_shifted;  // interface indices
     float *mur_coeff = nullptr, *mur_saved = nullptr;
     uint32_t *mur_normal = nullptr, *mur_shifted = nullptr, *mur_normal_src = nullptr, *mur_shifted_src = nullptr;
-    uint32_t mur_apply_n = 0;          // entries written by Mur apply (last entry per normal index)
-    uint32_t* mur_apply_src = nullptr; // their entry numbers; mur_normal holds their device indices
+    uint8_t* mur_early = nullptr;       // post done before the excitation (k_mur_post_some)
+    uint32_t* mur_early_list = nullptr;
+    uint32_t mur_ne = 0;
+    std::vector<uint32_t> h_exc_gidx;   // excitation cells (interface indices)
 
     // ADE (all orders/directions flattened)
     std::vector<uint32_t> h_ade_gidx;
@@ -379,6 +382,21 @@ int rebuild_aliases(efs_ctx* c) {
         }
         TRY(dupload(c, c->mur_normal_src, ns.data(), ns.size()));
         TRY(dupload(c, c->mur_shifted_src, ss.data(), ss.size()));
+        // entries whose shifted cell is written between post and apply
+        std::vector<uint32_t> written(c->h_mur_normal);
+        written.insert(written.end(), c->h_exc_gidx.begin(), c->h_exc_gidx.end());
+        if (c->ade_apply) written.insert(written.end(), c->h_ade_gidx.begin(), c->h_ade_gidx.end());
+        std::sort(written.begin(), written.end());
+        std::vector<uint8_t> early(c->mur_n, 0);
+        std::vector<uint32_t> list;
+        for (uint32_t k = 0; k < c->mur_n; ++k)
+            if (std::binary_search(written.begin(), written.end(), c->h_mur_shifted[k])) {
+                early[k] = 1;
+                list.push_back(k);
+            }
+        TRY(dupload(c, c->mur_early, early.data(), early.size()));
+        TRY(dupload(c, c->mur_early_list, list.data(), list.size()));
+        c->mur_ne = (uint32_t)list.size();
     }
     if (c->ade_n) {
         std::vector<uint32_t> sv(c->ade_n);
@@ -487,7 +505,8 @@ void free_all(efs_ctx* c) {
     dfree(c->pml_idx); dfree(c->pml_tab_e); dfree(c->pml_tab_h);
     dfree(c->fscale); dfree(c->f_amax); dfree(c->f_factor); dfree(c->f_smin);
     c->half = false;
-    dfree(c->mur_coeff); dfree(c->mur_saved); dfree(c->mur_normal); dfree(c->mur_shifted); dfree(c->mur_apply_src);
+    dfree(c->mur_coeff); dfree(c->mur_saved); dfree(c->mur_normal); dfree(c->mur_shifted);
+    dfree(c->mur_early); dfree(c->mur_early_list);
     dfree(c->mur_normal_src); dfree(c->mur_shifted_src);
     dfree(c->ade_gidx); dfree(c->ade_src_v); dfree(c->ade_src_i); dfree(c->ade_lor);
     for (auto& p : c->ade_c) dfree(p);
@@ -505,8 +524,10 @@ void reset_extensions(efs_ctx* c) {
     c->exc_n = 0;
     c->pml = PmlTable{};
     c->mur_n = 0;
+    c->mur_ne = 0;
     c->h_mur_normal.clear();
     c->h_mur_shifted.clear();
+    c->h_exc_gidx.clear();
     c->h_ade_gidx.clear();
     c->h_ade_lor.clear();
     for (auto& v : c->h_ade_c) v.clear();
@@ -706,8 +727,9 @@ int launch_step(efs_ctx* c) {
         hipLaunchKernelGGL(k_pbc, dim3(nblk((size_t)a.b0 * a.b1)), dim3(kPointThreads), 0, c->stream, fv(c, 0),
                            c->pbc_imag, a, c->ny, c->nz, (uint64_t)c->S);
     if (c->mur_n) {
-        hipLaunchKernelGGL(k_mur_post, dim3(nblk(c->mur_n)), dim3(kPointThreads), 0, c->stream, fv(c, 0),
-                           c->mur_coeff, c->mur_shifted, c->mur_saved, c->mur_n);
+        if (c->mur_ne)
+            hipLaunchKernelGGL(k_mur_post_some, dim3(nblk(c->mur_ne)), dim3(kPointThreads), 0, c->stream, fv(c, 0),
+                               c->mur_coeff, c->mur_shifted, c->mur_early_list, c->mur_saved, c->mur_ne);
     }
     if (c->ade_n && c->ade_apply) {
         hipLaunchKernelGGL(k_ade_apply, dim3(nblk(c->ade_n)), dim3(kPointThreads), 0, c->stream, fv(c, 0),
@@ -719,8 +741,8 @@ int launch_step(efs_ctx* c) {
                            c->exc_n, t);
     }
     if (c->mur_n) {
-        hipLaunchKernelGGL(k_mur_apply, dim3(nblk(c->mur_apply_n)), dim3(kPointThreads), 0, c->stream, fv(c, 0),
-                           c->mur_normal, c->mur_apply_src, c->mur_saved, c->mur_apply_n);
+        hipLaunchKernelGGL(k_mur_finish, dim3(nblk(c->mur_n)), dim3(kPointThreads), 0, c->stream, fv(c, 0),
+                           c->mur_coeff, c->mur_normal, c->mur_shifted, c->mur_early, c->mur_saved, c->mur_n);
     }
     if (c->rlc_n) {
         hipLaunchKernelGGL(k_rlc_apply, dim3(nblk(c->rlc_n)), dim3(kPointThreads), 0, c->stream, fv(c, 0),
@@ -951,6 +973,8 @@ int efs_cfg_excitation(efs_ctx* c, const float* signal, uint32_t len, const uint
     TRY(need_init(c));
     if (n == 0) {
         c->exc_n = 0;
+        c->h_exc_gidx.clear();
+        c->alias_dirty = true;
         return EFS_OK;
     }
     if (!signal || len == 0) return fail(c, EFS_ERR_ARG, "excitation signal must not be empty");
@@ -966,6 +990,9 @@ int efs_cfg_excitation(efs_ctx* c, const float* signal, uint32_t len, const uint
     TRY(dupload(c, c->exc_delay, delay, n));
     TRY(dupload(c, c->exc_amp, amp, n));
     TRY(dupload(c, c->exc_gidx, g.data(), n));
+    c->h_exc_gidx.resize(n);
+    for (uint32_t k = 0; k < n; ++k) c->h_exc_gidx[k] = dir[k] * (uint32_t)c->N + pos[k];
+    c->alias_dirty = true;  // Mur entries reading an excitation cell post early
     c->exc_n = n;
     c->exc_len = len;
     c->exc_period = period;
@@ -1103,33 +1130,29 @@ int efs_cfg_mur(efs_ctx* c, uint32_t n, const float* coeff, const uint32_t* norm
     for (uint32_t k = 0; k < n; ++k)
         if (normal_idx[k] >= 3 * c->N || shifted_idx[k] >= 3 * c->N)
             return fail(c, EFS_ERR_ARG, "Mur point %u: index out of range", k);
-    c->h_mur_normal.assign(normal_idx, normal_idx + n);
-    c->h_mur_shifted.assign(shifted_idx, shifted_idx + n);
-    TRY(dupload(c, c->mur_coeff, coeff, n));
-    const std::vector<uint32_t> tn = tr_all(c, normal_idx, n), ts = tr_all(c, shifted_idx, n);
-    // apply list: the last entry of every normal index, in entry order
-    std::vector<uint32_t> an, asrc;
-    {
-        std::vector<uint32_t> sorted(tn);
-        std::sort(sorted.begin(), sorted.end());
-        sorted.erase(std::unique(sorted.begin(), sorted.end()), sorted.end());
-        std::vector<uint8_t> seen(sorted.size(), 0);
-        for (uint32_t k = n; k-- > 0;) {
-            const size_t j = std::lower_bound(sorted.begin(), sorted.end(), tn[k]) - sorted.begin();
-            if (!seen[j]) {
-                seen[j] = 1;
-                asrc.push_back(k);
-            }
-        }
-        std::reverse(asrc.begin(), asrc.end());
-        for (uint32_t k : asrc) an.push_back(tn[k]);
+    // keep each normal cell's last entry (the reference's loop lets it win),
+    // ordered by the cell so that neighbouring threads touch neighbouring cells
+    std::vector<uint32_t> ord(n);
+    for (uint32_t k = 0; k < n; ++k) ord[k] = k;
+    std::stable_sort(ord.begin(), ord.end(), [&](uint32_t a, uint32_t b) { return normal_idx[a] < normal_idx[b]; });
+    std::vector<uint32_t> hn, hs;
+    std::vector<float> hc;
+    for (uint32_t j = 0; j < n; ++j) {
+        const uint32_t k = ord[j];
+        if (j + 1 < n && normal_idx[ord[j + 1]] == normal_idx[k]) continue;
+        hn.push_back(normal_idx[k]);
+        hs.push_back(shifted_idx[k]);
+        hc.push_back(coeff[k]);
     }
-    TRY(dupload(c, c->mur_normal, an.data(), an.size()));
-    TRY(dupload(c, c->mur_apply_src, asrc.data(), asrc.size()));
-    c->mur_apply_n = (uint32_t)asrc.size();
-    TRY(dupload(c, c->mur_shifted, ts.data(), n));
-    TRY(dzero(c, c->mur_saved, n));
-    c->mur_n = n;
+    const uint32_t m = (uint32_t)hn.size();
+    c->h_mur_normal = hn;
+    c->h_mur_shifted = hs;
+    TRY(dupload(c, c->mur_coeff, hc.data(), m));
+    const std::vector<uint32_t> tn = tr_all(c, hn.data(), m), ts = tr_all(c, hs.data(), m);
+    TRY(dupload(c, c->mur_normal, tn.data(), m));
+    TRY(dupload(c, c->mur_shifted, ts.data(), m));
+    TRY(dzero(c, c->mur_saved, m));
+    c->mur_n = m;
     c->alias_dirty = true;
     return EFS_OK;
 }
@@ -1167,6 +1190,7 @@ int efs_cfg_ade_add(efs_ctx* c, uint32_t has_lorentz, uint32_t dir, uint32_t nce
 int efs_set_ade_apply(efs_ctx* c, int apply) {
     TRY(need_init(c));
     c->ade_apply = apply != 0;
+    c->alias_dirty = true;
     return EFS_OK;
 }
 

@@ -37,32 +37,39 @@ __global__ void k_excitation(FieldView volt, const float* __restrict__ signal, u
     volt.add(gidx[k], amp[k] * signal[e]);
 }
 
-// Mur pre / post / apply (src/webgpu-fdtd.mjs:992-1023).
+// Mur ABC (src/webgpu-fdtd.mjs:992-1023): per entry
+//   pre    s  = V[sh] - c*V[nor]     (before the E sweep)
+//   post   s += c*V[sh]              (after it)
+//   apply  V[nor] = s                (after the excitation; the last entry of
+//                                     a cell wins, as in the reference's loop)
+// The host keeps only each cell's last entry.  `finish` does post + apply in
+// one pass; entries whose shifted cell is written between post and apply
+// (another entry's normal cell, or an excitation / ADE cell) get their post
+// earlier, at its place in the step, from `k_mur_post_some`, and `finish`
+// then only applies them (flag bit).  Explicit FMAs fix the rounding.
 __global__ void k_mur_pre(FieldView volt, const float* __restrict__ vflux,
                           const float* __restrict__ coeff, const uint32_t* __restrict__ normal_src,
                           const uint32_t* __restrict__ shifted_src, float* __restrict__ saved, uint32_t n) {
     const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= n) return;
-    saved[k] = read_alias(volt, vflux, shifted_src[k]) - coeff[k] * read_alias(volt, vflux, normal_src[k]);
+    saved[k] = __builtin_fmaf(-coeff[k], read_alias(volt, vflux, normal_src[k]), read_alias(volt, vflux, shifted_src[k]));
 }
 
-__global__ void k_mur_post(FieldView volt, const float* __restrict__ coeff, const uint32_t* __restrict__ shifted,
-                           float* __restrict__ saved, uint32_t n) {
-    const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
-    if (k >= n) return;
-    saved[k] += coeff[k] * volt.ld(shifted[k]);
+__global__ void k_mur_post_some(FieldView volt, const float* __restrict__ coeff, const uint32_t* __restrict__ shifted,
+                                const uint32_t* __restrict__ list, float* __restrict__ saved, uint32_t n) {
+    const uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= n) return;
+    const uint32_t k = list[j];
+    saved[k] = __builtin_fmaf(coeff[k], volt.ld(shifted[k]), saved[k]);
 }
 
-// Mur apply: V[normal] = saved.  An index listed more than once (an edge
-// shared by two boundary faces) keeps the value of its last entry, as the
-// sequential loop of the reference does (src/webgpu-fdtd.mjs:1014-1023):
-// the host passes only those last entries (src = their entry number), so
-// no two threads write one cell.
-__global__ void k_mur_apply(FieldView volt, const uint32_t* __restrict__ normal, const uint32_t* __restrict__ src,
-                            const float* __restrict__ saved, uint32_t n) {
+__global__ void k_mur_finish(FieldView volt, const float* __restrict__ coeff, const uint32_t* __restrict__ normal,
+                             const uint32_t* __restrict__ shifted, const uint8_t* __restrict__ early,
+                             const float* __restrict__ saved, uint32_t n) {
     const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= n) return;
-    volt.st(normal[k], saved[src[k]]);
+    const float s = early[k] ? saved[k] : __builtin_fmaf(coeff[k], volt.ld(shifted[k]), saved[k]);
+    volt.st(normal[k], s);
 }
 
 // Lorentz / Drude ADE state update for all (order, direction) entries at
