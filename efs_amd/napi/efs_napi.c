@@ -1,5 +1,6 @@
 /*
- * efs_napi.c -- Node-API binding of libefs_engine.so (include/efs_engine.h).
+ * efs_napi.c -- Node-API binding of libefs_engine.so (include/efs_engine.h,
+ * include/efs_batch.h).
  *
  * A thin, synchronous layer: every function takes the context handle (an
  * external value) plus numbers and TypedArrays, calls the matching efs_*
@@ -13,6 +14,7 @@
 #include <stdio.h>
 #include <string.h>
 
+#include "efs_batch.h"
 #include "efs_engine.h"
 
 #define MAXARGS 16
@@ -539,6 +541,220 @@ static napi_value js_energy(napi_env env, napi_callback_info info) {
     return num(env, e);
 }
 
+
+/* ------------------------------------------------------------ batched form
+ * include/efs_batch.h: batchCreate(device, storage) -> handle | null,
+ * batchAdd(h, nx, ny, nz, vv, vi, ii, iv) -> id, per-instance configure
+ * calls, batchFinalize(h, interval, maxSamples), batchTune(h, steps) ->
+ * [blocks, waves], batchIterate, reads. */
+static efs_batch *batch_arg(call_t *c, size_t i) {
+    void *p = NULL;
+    if (napi_get_value_external(c->env, c->argv[i], &p) != napi_ok || !p) {
+        throwf(c, "efs: invalid batch handle");
+        return NULL;
+    }
+    return (efs_batch *)p;
+}
+
+static napi_value bstatus(call_t *c, efs_batch *b, int rc) {
+    if (c->failed) return NULL;
+    if (rc != EFS_OK) {
+        throwf(c, efs_batch_last_error(b));
+        return NULL;
+    }
+    return undef(c->env);
+}
+
+#define BATCH(i)                                 \
+    efs_batch *b = batch_arg(&c, (i));           \
+    if (!b) return NULL;
+
+static napi_value js_batch_create(napi_env env, napi_callback_info info) {
+    BEGIN();
+    uint32_t dev = c.argc > 0 ? u32_arg(&c, 0) : 0;
+    uint32_t storage = c.argc > 1 ? u32_arg(&c, 1) : 0;
+    BAIL();
+    efs_batch *b = NULL;
+    int rc = efs_batch_create(&b, (int)dev);
+    if (rc == EFS_ERR_NO_DEVICE) {
+        napi_value n;
+        napi_get_null(env, &n);
+        return n;
+    }
+    if (rc != EFS_OK) {
+        throwf(&c, efs_batch_last_error(NULL));
+        return NULL;
+    }
+    rc = efs_batch_set_storage(b, (int)storage);
+    if (rc != EFS_OK) {
+        throwf(&c, efs_batch_last_error(b));
+        efs_batch_destroy(b);
+        return NULL;
+    }
+    napi_value h;
+    napi_create_external(env, b, NULL, NULL, &h);
+    return h;
+}
+
+static napi_value js_batch_destroy(napi_env env, napi_callback_info info) {
+    BEGIN();
+    BATCH(0);
+    efs_batch_destroy(b);
+    return undef(env);
+}
+
+static napi_value js_batch_add(napi_env env, napi_callback_info info) {
+    BEGIN();
+    BATCH(0);
+    size_t a, bb, d, e;
+    uint32_t nx = u32_arg(&c, 1), ny = u32_arg(&c, 2), nz = u32_arg(&c, 3);
+    float *vv = F32(4, a), *vi = F32(5, bb), *ii = F32(6, d), *iv = F32(7, e);
+    BAIL();
+    size_t need = 3 * (size_t)nx * ny * nz;
+    if (a != need || bb != need || d != need || e != need) {
+        throwf(&c, "efs: coefficient arrays must hold 3*Nx*Ny*Nz floats");
+        return NULL;
+    }
+    uint32_t id = 0;
+    if (!bstatus(&c, b, efs_batch_add(b, nx, ny, nz, vv, vi, ii, iv, &id))) return NULL;
+    return num(env, id);
+}
+
+/* batchCfgExcitation(h, id, signal, delay, amp, dir, pos, period) */
+static napi_value js_batch_cfg_excitation(napi_env env, napi_callback_info info) {
+    BEGIN();
+    BATCH(0);
+    size_t ls, l1, l2, l3, l4;
+    uint32_t id = u32_arg(&c, 1);
+    float *sig = F32(2, ls);
+    uint32_t *delay = U32(3, l1);
+    float *amp = F32(4, l2);
+    uint32_t *dir = U32(5, l3), *pos = U32(6, l4);
+    uint32_t period = u32_arg(&c, 7);
+    BAIL();
+    if (l1 != l2 || l2 != l3 || l3 != l4) {
+        throwf(&c, "efs: excitation arrays must have equal length");
+        return NULL;
+    }
+    return bstatus(&c, b, efs_batch_cfg_excitation(b, id, sig, (uint32_t)ls, delay, amp, dir, pos, (uint32_t)l2,
+                                                    period));
+}
+
+/* batchCfgMur(h, id, coeff, normalIdx, shiftedIdx) */
+static napi_value js_batch_cfg_mur(napi_env env, napi_callback_info info) {
+    BEGIN();
+    BATCH(0);
+    size_t n1, n2, n3;
+    uint32_t id = u32_arg(&c, 1);
+    float *coeff = F32(2, n1);
+    uint32_t *ni = U32(3, n2), *si = U32(4, n3);
+    BAIL();
+    if (n1 != n2 || n2 != n3) {
+        throwf(&c, "efs: Mur arrays must have equal length");
+        return NULL;
+    }
+    return bstatus(&c, b, efs_batch_cfg_mur(b, id, (uint32_t)n1, coeff, ni, si));
+}
+
+static napi_value js_batch_cfg_gather(napi_env env, napi_callback_info info) {
+    BEGIN();
+    BATCH(0);
+    size_t n;
+    uint32_t id = u32_arg(&c, 1);
+    uint32_t *idx = U32(2, n);
+    BAIL();
+    return bstatus(&c, b, efs_batch_cfg_gather(b, id, idx, (uint32_t)n));
+}
+
+static napi_value js_batch_finalize(napi_env env, napi_callback_info info) {
+    BEGIN();
+    BATCH(0);
+    uint32_t iv = u32_arg(&c, 1), mx = u32_arg(&c, 2);
+    BAIL();
+    return bstatus(&c, b, efs_batch_finalize(b, iv, mx));
+}
+
+static napi_value js_batch_tune(napi_env env, napi_callback_info info) {
+    BEGIN();
+    BATCH(0);
+    uint32_t steps = u32_arg(&c, 1);
+    BAIL();
+    uint32_t blocks = 0, waves = 0;
+    if (!bstatus(&c, b, efs_batch_tune(b, steps, &blocks, &waves))) return NULL;
+    napi_value arr;
+    napi_create_array_with_length(env, 2, &arr);
+    napi_set_element(env, arr, 0, num(env, blocks));
+    napi_set_element(env, arr, 1, num(env, waves));
+    return arr;
+}
+
+static napi_value js_batch_reset(napi_env env, napi_callback_info info) {
+    BEGIN();
+    BATCH(0);
+    return bstatus(&c, b, efs_batch_reset(b));
+}
+
+static napi_value js_batch_iterate(napi_env env, napi_callback_info info) {
+    BEGIN();
+    BATCH(0);
+    uint32_t n = u32_arg(&c, 1);
+    BAIL();
+    return bstatus(&c, b, efs_batch_iterate(b, n));
+}
+
+static napi_value js_batch_sync(napi_env env, napi_callback_info info) {
+    BEGIN();
+    BATCH(0);
+    return bstatus(&c, b, efs_batch_sync(b));
+}
+
+static napi_value js_batch_num_ts(napi_env env, napi_callback_info info) {
+    BEGIN();
+    BATCH(0);
+    uint32_t v = 0;
+    if (!bstatus(&c, b, efs_batch_num_ts(b, &v))) return NULL;
+    return num(env, v);
+}
+
+/* batchReadGather(h, id, out|null) -> number of samples */
+static napi_value js_batch_read_gather(napi_env env, napi_callback_info info) {
+    BEGIN();
+    BATCH(0);
+    size_t n;
+    uint32_t id = u32_arg(&c, 1);
+    float *out = F32OPT(2, n);
+    BAIL();
+    uint32_t ns = 0;
+    if (!bstatus(&c, b, efs_batch_read_gather(b, id, out, &ns))) return NULL;
+    return num(env, ns);
+}
+
+static napi_value js_batch_get_fields(napi_env env, napi_callback_info info) {
+    BEGIN();
+    BATCH(0);
+    size_t a, d;
+    uint32_t id = u32_arg(&c, 1);
+    float *v = F32(2, a), *cu = F32(3, d);
+    BAIL();
+    return bstatus(&c, b, efs_batch_get_fields(b, id, v, cu));
+}
+
+/* batchInfo(h) -> [blocks, waves, graphSteps, cells] */
+static napi_value js_batch_info(napi_env env, napi_callback_info info) {
+    BEGIN();
+    BATCH(0);
+    uint32_t blocks = 0, waves = 0, gs = 0;
+    uint64_t cells = 0;
+    if (!bstatus(&c, b, efs_batch_info(b, &blocks, &waves, &gs, &cells))) return NULL;
+    napi_value arr;
+    napi_create_array_with_length(env, 4, &arr);
+    napi_set_element(env, arr, 0, num(env, blocks));
+    napi_set_element(env, arr, 1, num(env, waves));
+    napi_set_element(env, arr, 2, num(env, gs));
+    napi_set_element(env, arr, 3, num(env, (double)cells));
+    return arr;
+}
+
 #define FN(name, fn) {name, 0, fn, 0, 0, 0, napi_default, 0}
 
 static napi_value Init(napi_env env, napi_value exports) {
@@ -577,6 +793,21 @@ static napi_value Init(napi_env env, napi_value exports) {
         FN("readNF2FF", js_read_nf2ff),
         FN("farfield", js_farfield),
         FN("energy", js_energy),
+        FN("batchCreate", js_batch_create),
+        FN("batchDestroy", js_batch_destroy),
+        FN("batchAdd", js_batch_add),
+        FN("batchCfgExcitation", js_batch_cfg_excitation),
+        FN("batchCfgMur", js_batch_cfg_mur),
+        FN("batchCfgGather", js_batch_cfg_gather),
+        FN("batchFinalize", js_batch_finalize),
+        FN("batchTune", js_batch_tune),
+        FN("batchReset", js_batch_reset),
+        FN("batchIterate", js_batch_iterate),
+        FN("batchSync", js_batch_sync),
+        FN("batchNumTS", js_batch_num_ts),
+        FN("batchReadGather", js_batch_read_gather),
+        FN("batchGetFields", js_batch_get_fields),
+        FN("batchInfo", js_batch_info),
     };
     napi_define_properties(env, exports, sizeof d / sizeof d[0], d);
     return exports;

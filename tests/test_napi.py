@@ -27,6 +27,7 @@ def test_js_binding_surface():
     assert info["version"].startswith("efs-mi355x")
     if info["devices"] == 0:
         assert info["initGPU"] is False  # the reference resolves false without a GPU
+        assert info["batchInitGPU"] is False
 
 
 def _jsonable(case):
@@ -61,3 +62,42 @@ def test_js_binding_reference_vectors(tmp_path, name):
     assert got["numTS"] == out["numTS"]
     for k in ("volt", "curr"):
         assert rel_l2(got[k], out[k]) <= 1e-5, k
+
+
+@needs_node
+@pytest.mark.gpu
+def test_js_batch_matches_python_batch(tmp_path):
+    """Two patch instances through the JS BatchEngine give the bits of the
+    Python binding of the same library."""
+    from efs_amd import batch, ports, workloads
+    ws = []
+    for W in (14.0, 17.0):
+        ws.append(workloads.patch_antenna(patch_w=W, patch_l=1.25 * W, sub_w=1.875 * W, sub_l=1.875 * W,
+                                          feed_x=-0.1875 * W, sim_box=(3 * W + 20, 3 * W + 16, 2 * W + 10)))
+    steps = 80
+    case = {"storage": "fp32", "steps": steps, "instances": []}
+    for w in ws:
+        idx, _ = ports.build_gather_indices(w["probes"], w["gridSize"])
+        case["instances"].append({"dims": w["gridSize"], "coeffs": w["coefficients"], "mur": w["mur"],
+                                  "excitation": w["excitation"], "probes": idx})
+    src, dst = tmp_path / "batch.json", tmp_path / "out.json"
+    src.write_text(json.dumps(_jsonable(case)))
+    print("node batch", [w["gridSize"] for w in ws], flush=True)
+    res = subprocess.run([NODE, os.path.join(ROOT, "tests", "js", "run_batch.mjs"), str(src), str(dst)],
+                         capture_output=True, text=True, timeout=120, cwd=ROOT)
+    assert res.returncode == 0, res.stderr
+    got = json.loads(dst.read_text())
+    assert got["numTS"] == steps and got["info"]["graphSteps"] == 32
+    b = batch.BatchEngine()
+    for w in ws:
+        workloads.add_patch(b, w)
+    b.finalize(1, steps)
+    b.iterate(steps)
+    for i in range(len(ws)):
+        f = b.getFields(i)
+        for k in ("volt", "curr"):
+            np.testing.assert_array_equal(np.asarray(got["instances"][i][k], np.float32), f[k])
+        g = b.readProbeGatherBuffered(i)
+        assert got["instances"][i]["numSamples"] == g["numSamples"] == steps
+        np.testing.assert_array_equal(np.asarray(got["instances"][i]["gather"], np.float32), g["data"])
+    b.destroy()
