@@ -93,21 +93,25 @@ __device__ __forceinline__ void bsweep_body(const Inst* __restrict__ insts, cons
         if (I.tabn_h) {
             const Coef cf{nullptr, nullptr, nullptr, I.th, nullptr, nullptr, I.ih, I.tabn_h};
             h4_march<kMarchInterior, kCoefIdx16, false, T>(static_cast<T*>(I.curr), static_cast<const T*>(I.volt),
-                                                           cf, nullptr, nullptr, g, pml, nullptr, nullptr, w, 0u, sc);
+                                                           cf, nullptr, nullptr, g, pml, nullptr, nullptr, w, 0u, sc,
+                                                           NfCapture{});
         } else {
             const Coef cf{I.ii, I.iv, nullptr, nullptr, nullptr, nullptr, nullptr, 0u};
             h4_march<kMarchInterior, kCoefF32, false, T>(static_cast<T*>(I.curr), static_cast<const T*>(I.volt), cf,
-                                                         nullptr, nullptr, g, pml, nullptr, nullptr, w, 0u, sc);
+                                                         nullptr, nullptr, g, pml, nullptr, nullptr, w, 0u, sc,
+                                                           NfCapture{});
         }
     } else {
         if (I.tabn_e) {
             const Coef cf{nullptr, nullptr, nullptr, I.te, nullptr, nullptr, I.ie, I.tabn_e};
             e4_march<kMarchInterior, kCoefIdx16, false, T>(static_cast<T*>(I.volt), static_cast<const T*>(I.curr),
-                                                           cf, nullptr, nullptr, g, pml, nullptr, nullptr, w, 0u, sc);
+                                                           cf, nullptr, nullptr, g, pml, nullptr, nullptr, w, 0u, sc,
+                                                           NfCapture{});
         } else {
             const Coef cf{I.vv, I.vi, nullptr, nullptr, nullptr, nullptr, nullptr, 0u};
             e4_march<kMarchInterior, kCoefF32, false, T>(static_cast<T*>(I.volt), static_cast<const T*>(I.curr), cf,
-                                                         nullptr, nullptr, g, pml, nullptr, nullptr, w, 0u, sc);
+                                                         nullptr, nullptr, g, pml, nullptr, nullptr, w, 0u, sc,
+                                                           NfCapture{});
         }
     }
 }
@@ -246,12 +250,22 @@ __global__ __launch_bounds__(256) void k_babsmax(const Seg* __restrict__ segs, u
         m = fmaxf(m, fmaxf(fmaxf(fabsf(v.x), fabsf(v.y)), fmaxf(fabsf(v.z), fabsf(v.w))));
         if (v.x != v.x || v.y != v.y || v.z != v.z || v.w != v.w) m = __builtin_nanf("");
     }
+    // block maximum (waves, then LDS), one atomic per workgroup
+    __shared__ float s_m[4];
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1) {
         const float o = __shfl_down(m, off, 64);
         m = (o != o || m != m) ? __builtin_nanf("") : fmaxf(m, o);
     }
-    if ((threadIdx.x & 63u) == 0) atomicMax(amax + s.slot, __float_as_uint(m));
+    if ((threadIdx.x & 63u) == 0) s_m[threadIdx.x >> 6] = m;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int w = 1; w < 4; ++w) {
+            const float o = s_m[w];
+            m = (o != o || m != m) ? __builtin_nanf("") : fmaxf(m, o);
+        }
+        atomicMax(amax + s.slot, __float_as_uint(m));
+    }
 }
 
 __global__ void k_bplan(unsigned int* __restrict__ amax, float* __restrict__ fscale,
@@ -1004,7 +1018,7 @@ int efs_batch_finalize(efs_batch* b, uint32_t sample_interval, uint32_t max_samp
         b->nsegs = (uint32_t)segs.size();
         uint64_t mx = 0;
         for (const Seg& s : segs) mx = std::max<uint64_t>(mx, s.n);
-        b->seg_chunks = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(64, (mx / 4 + 255) / 256));
+        b->seg_chunks = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(16, (mx / 4 + 255) / 256));
     }
     b->mur_n = (uint32_t)mur.size();
     b->mur_ne = (uint32_t)mur_early.size();

@@ -171,6 +171,10 @@ struct efs_ctx {
     // neighbouring threads of the sampling kernel read neighbouring cells;
     // nf_perm[q] is the caller's index of internal point q.
     std::vector<uint32_t> nf_perm;
+    // z planes of the surface captured by the H sweep (NfCapture); in use
+    // when nf_capture() finds the step's configuration compatible
+    NfCapture nf_cap{};
+    float* nf_slab = nullptr;
     float *nf_el = nullptr, *nf_buf = nullptr, *nf_acc = nullptr;
     double* nf_omega = nullptr;
     float* nf_tw = nullptr;  // [2][K][NC] twiddles
@@ -517,6 +521,7 @@ void free_all(efs_ctx* c) {
     dfree(c->g_idx); dfree(c->g_out); dfree(c->gb_idx); dfree(c->gb_ring);
     dfree(c->pbc_imag); dfree(c->ss_gidx); dfree(c->ss_e);
     dfree(c->nf_pts); dfree(c->nf_el); dfree(c->nf_buf); dfree(c->nf_acc); dfree(c->nf_omega); dfree(c->nf_tw);
+    dfree(c->nf_slab);
     dfree(c->en_partial);
 }
 
@@ -540,6 +545,7 @@ void reset_extensions(efs_ctx* c) {
     c->gb_n = 0;
     c->gb_count = 0;
     c->nf_P = 0;
+    c->nf_cap = NfCapture{};
     c->nf_slot = 0;
     c->pbc_e.clear();
     c->pbc_h.clear();
@@ -653,10 +659,63 @@ FieldView fv(const efs_ctx* c, int curr) {
                      c->half ? 1u : 0u};
 }
 
+// The NF2FF capture for this step, or an inactive one: it needs the 4-wide
+// grid sweeps, no UPML cell in the captured rows, and no hook that changes
+// H after the H sweep (TFSF-I, PBC-I, ADE write-back).
+NfCapture nf_capture(const efs_ctx* c) {
+    const NfCapture& k = c->nf_cap;
+    if (!k.on || !c->nf_P || !c->vec4 || (c->pml.n && !c->pml_split)) return NfCapture{};
+    if (c->tfsf_ni || !c->pbc_h.empty() || (c->ade_n && c->ade_apply)) return NfCapture{};
+    for (uint32_t r = 0; r < c->pml.n; ++r) {
+        const PmlRegion& R = c->pml.r[r];
+        if (R.x0 > k.x1 || R.x1 <= k.x0 || R.y0 > k.y1 || R.y1 <= k.y0) continue;
+        for (uint32_t f = 0; f < k.nf; ++f)
+            if (R.z0 <= k.z[f] + 1 && R.z1 > k.z[f]) return NfCapture{};
+    }
+    return k;
+}
+
+// Choose the captured planes: the (at most two) z planes holding the most
+// surface points, each with its zf and zf+1 in one 4-cell lane and strictly
+// inside the grid; the rectangle covers their points plus one row in +x, +y.
+int plan_nf_capture(efs_ctx* c, const std::vector<uint32_t>& pts, uint32_t P) {
+    c->nf_cap = NfCapture{};
+    if (env_u64("EFS_NF2FF_CAPTURE", 1) == 0 || c->nz < 4) return EFS_OK;
+    std::vector<uint32_t> cnt(c->nz, 0);
+    for (uint32_t p = 0; p < P; ++p) cnt[pts[3 * (size_t)p + 2]]++;
+    std::vector<uint32_t> zs;
+    for (uint32_t z = 0; z < c->nz; ++z)
+        if (cnt[z] >= 256 && z % 4 != 3 && z + 2 < c->nz) zs.push_back(z);
+    std::sort(zs.begin(), zs.end(), [&](uint32_t a, uint32_t b) { return cnt[a] != cnt[b] ? cnt[a] > cnt[b] : a < b; });
+    if (zs.size() > 2) zs.resize(2);
+    if (zs.empty()) return EFS_OK;
+    uint32_t x0 = ~0u, x1 = 0, y0 = ~0u, y1 = 0;
+    for (uint32_t p = 0; p < P; ++p) {
+        const uint32_t z = pts[3 * (size_t)p + 2];
+        if (std::find(zs.begin(), zs.end(), z) == zs.end()) continue;
+        x0 = std::min(x0, pts[3 * (size_t)p]);
+        x1 = std::max(x1, pts[3 * (size_t)p] + 1);
+        y0 = std::min(y0, pts[3 * (size_t)p + 1]);
+        y1 = std::max(y1, pts[3 * (size_t)p + 1] + 1);
+    }
+    if (x1 + 1 >= c->nx || y1 + 1 >= c->ny) return EFS_OK;  // rows must have the core update
+    NfCapture k{};
+    k.on = 1;
+    k.nf = (uint32_t)zs.size();
+    for (uint32_t f = 0; f < k.nf; ++f) k.z[f] = zs[f];
+    k.x0 = x0; k.x1 = x1; k.y0 = y0; k.y1 = y1;
+    k.nx_c = x1 - x0 + 1;
+    k.ny_c = y1 - y0 + 1;
+    TRY(dzero(c, c->nf_slab, (size_t)k.nf * k.nx_c * k.ny_c * kCapFloats));
+    k.slab = c->nf_slab;
+    c->nf_cap = k;
+    return EFS_OK;
+}
+
 // FP16 storage: largest |stored| per component, then rescale where needed.
 void launch_range_control(efs_ctx* c, uint32_t t) {
     const uint32_t N = (uint32_t)c->N;
-    const dim3 grid(std::max(1u, std::min(1024u, (N / 4 + 255) / 256)), 6);
+    const dim3 grid(std::max(1u, std::min(256u, (N / 4 + 255) / 256)), 6);
     hipLaunchKernelGGL(k_absmax16, grid, dim3(256), 0, c->stream, (const f16*)c->volt, (const f16*)c->curr,
                        (uint64_t)c->S, N, c->f_amax);
     const uint32_t active = c->source_periodic || (uint64_t)t + 1 < c->source_end;
@@ -702,15 +761,16 @@ int launch_step(efs_ctx* c) {
                                                   : (c->comp ? sweep_e4fc_h : sweep_e4f_h))
                                    : (c->comp ? sweep_e4c_h : sweep_e4_h);
                 hipLaunchKernelGGL(k16, dim3(c->geom.blocks), dim3(kSweepThreads), 0, c->stream, (f16*)c->volt,
-                                   (const f16*)c->curr, cf, c->geom, c->pml, c->pml_coef, c->pml_vflux, c->fscale);
+                                   (const f16*)c->curr, cf, c->geom, c->pml, c->pml_coef, c->pml_vflux, c->fscale,
+                                   NfCapture{});
             } else {
                 hipLaunchKernelGGL(kern, dim3(c->geom.blocks), dim3(kSweepThreads), 0, c->stream, c->volt, c->curr,
-                                   cf, c->geom, c->pml, c->pml_coef, c->pml_vflux, c->fscale);
+                                   cf, c->geom, c->pml, c->pml_coef, c->pml_vflux, c->fscale, NfCapture{});
             }
             if (has_pml && !c->pml_fused)
                 hipLaunchKernelGGL(c->comp ? sweep_e4uc : sweep_e4u, dim3(c->pml.blocks), dim3(kSweepThreads), 0,
                                    c->stream, c->volt, c->curr, cf, c->geom, c->pml, c->pml_coef, c->pml_vflux,
-                                   c->fscale);
+                                   c->fscale, NfCapture{});
         } else {
             auto kern = has_pml ? sweep_e1<true> : sweep_e1<false>;
             hipLaunchKernelGGL(kern, dim3(c->geom1.blocks), dim3(kSweepThreads), 0, c->stream, c->volt, c->curr,
@@ -766,15 +826,16 @@ int launch_step(efs_ctx* c) {
                                                   : (c->comp ? sweep_h4fc_h : sweep_h4f_h))
                                    : (c->comp ? sweep_h4c_h : sweep_h4_h);
                 hipLaunchKernelGGL(k16, dim3(c->geom.blocks), dim3(kSweepThreads), 0, c->stream, (f16*)c->curr,
-                                   (const f16*)c->volt, cf, c->geom, c->pml, c->pml_coef, c->pml_cflux, c->fscale);
+                                   (const f16*)c->volt, cf, c->geom, c->pml, c->pml_coef, c->pml_cflux, c->fscale,
+                                   nf_capture(c));
             } else {
                 hipLaunchKernelGGL(kern, dim3(c->geom.blocks), dim3(kSweepThreads), 0, c->stream, c->curr, c->volt,
-                                   cf, c->geom, c->pml, c->pml_coef, c->pml_cflux, c->fscale);
+                                   cf, c->geom, c->pml, c->pml_coef, c->pml_cflux, c->fscale, nf_capture(c));
             }
             if (has_pml && !c->pml_fused)
                 hipLaunchKernelGGL(c->comp ? sweep_h4uc : sweep_h4u, dim3(c->pml.blocks), dim3(kSweepThreads), 0,
                                    c->stream, c->curr, c->volt, cf, c->geom, c->pml, c->pml_coef, c->pml_cflux,
-                                   c->fscale);
+                                   c->fscale, NfCapture{});
         } else {
             auto kern = has_pml ? sweep_h1<true> : sweep_h1<false>;
             hipLaunchKernelGGL(kern, dim3(c->geom1.blocks), dim3(kSweepThreads), 0, c->stream, c->curr, c->volt,
@@ -799,7 +860,7 @@ int launch_step(efs_ctx* c) {
         if (c->nf_slot == 0) c->nf_ts0 = t + 1;
         hipLaunchKernelGGL(k_nf2ff_sample, dim3(nblk(c->nf_P)), dim3(kPointThreads), 0, c->stream, fv(c, 0),
                            fv(c, 1), c->nf_pts, c->nf_el, c->nf_geom,
-                           c->nf_buf + (size_t)c->nf_slot * c->nf_rows);
+                           c->nf_buf + (size_t)c->nf_slot * c->nf_rows, nf_capture(c));
         c->nf_slot++;
         if (c->nf_slot == c->nf_K) TRY(nf2ff_fold(c));
     }
@@ -1472,6 +1533,7 @@ int efs_cfg_nf2ff(efs_ctx* c, const uint32_t* points, uint32_t P, const double* 
     HIP_TRY(c, hipStreamSynchronize(c->stream));
     if (P == 0) {
         c->nf_P = 0;
+        c->nf_cap = NfCapture{};
         return EFS_OK;
     }
     if (!points || !omega || F == 0 || !edge_lens || !el_len)
@@ -1509,6 +1571,7 @@ int efs_cfg_nf2ff(efs_ctx* c, const uint32_t* points, uint32_t P, const double* 
         for (int a = 0; a < 3; ++a) sorted[3 * (size_t)q + a] = points[3 * (size_t)perm[q] + a];
     TRY(dupload(c, c->nf_pts, sorted.data(), sorted.size()));
     c->nf_perm = std::move(perm);
+    TRY(plan_nf_capture(c, sorted, P));
     TRY(dupload(c, c->nf_el, edge_lens, el_total));
     TRY(dupload(c, c->nf_omega, omega, F));
     TRY(dzero(c, c->nf_buf, (size_t)K * rows));  // padding rows stay zero
@@ -1628,7 +1691,8 @@ int efs_path_flags(const efs_ctx* c, uint32_t* flags) {
     const bool upml4 = c->vec4 && c->pml.n && c->pml_split;
     *flags = (c->vec4 ? EFS_PATH_VEC4 : 0u) | (c->comp ? EFS_PATH_COEF_IDX : 0u) | (upml4 ? EFS_PATH_UPML_4 : 0u) |
              (upml4 && c->pml_fused ? EFS_PATH_FUSED : 0u) |
-             (upml4 && c->pml_fused && c->pml_comp ? EFS_PATH_UPML_IDX : 0u) | (c->half ? EFS_PATH_F16 : 0u);
+             (upml4 && c->pml_fused && c->pml_comp ? EFS_PATH_UPML_IDX : 0u) | (c->half ? EFS_PATH_F16 : 0u) |
+             (nf_capture(c).on ? EFS_PATH_NF2FF_CAPTURE : 0u);
     return EFS_OK;
 }
 

@@ -33,53 +33,81 @@ struct Nf2ffGeom {
 
 // Cell interpolation of E (mean of 4 edges / primal length; 0 on the last
 // grid line) and H (lerp of 2 edges / dual length along its own axis),
-// as in the NF2FF accumulate WGSL.
-__global__ void k_nf2ff_sample(FieldView volt, FieldView curr,
-                               const uint32_t* __restrict__ pts, const float* __restrict__ el, Nf2ffGeom g,
-                               float* __restrict__ row_out /* buf + slot*6P */) {
-    // points are in grid order; consecutive workgroups share cells, so they
-    // stay on one XCD (own L2)
-    const uint32_t p = xcd_remap(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
-    if (p >= g.P) return;
-    const uint32_t ix = pts[3 * p], iy = pts[3 * p + 1], iz = pts[3 * p + 2];
-    const size_t Ny = g.ny, Nz = g.nz;
-    const size_t b = ix * Ny * Nz + iy * Nz + iz;
-    const size_t sx = Ny * Nz, sy = Nz;
+// as in the NF2FF accumulate WGSL.  V(c, dx, dy, dz) returns volt component
+// c at (ix+dx, iy+dy, iz+dz); I(n, d) returns curr component n at the point
+// or (d = 1) one cell further along axis n.  One body for both sources of
+// the values keeps their arithmetic identical.
+template <class VA, class IA>
+__device__ __forceinline__ void nf_point(uint32_t ix, uint32_t iy, uint32_t iz, const float* __restrict__ el,
+                                         const Nf2ffGeom& g, VA V, IA I, float out[6]) {
+    // no contraction: the same bits whichever accessor supplies the values
+#pragma clang fp contract(off)
     float E0 = 0.f, E1 = 0.f, E2 = 0.f;
     if (!(ix >= g.nx - 1 || iy >= g.ny - 1 || iz >= g.nz - 1)) {
-        E0 = (volt.ldc(0, b) + volt.ldc(0, b + sy) + volt.ldc(0, b + sy + 1) + volt.ldc(0, b + 1)) /
-             (4.0f * el[g.el_off[0] + ix]);
-        E1 = (volt.ldc(1, b) + volt.ldc(1, b + 1) + volt.ldc(1, b + sx + 1) + volt.ldc(1, b + sx)) /
-             (4.0f * el[g.el_off[1] + iy]);
-        E2 = (volt.ldc(2, b) + volt.ldc(2, b + sx) + volt.ldc(2, b + sx + sy) + volt.ldc(2, b + sy)) /
-             (4.0f * el[g.el_off[2] + iz]);
+        E0 = (V(0, 0, 0, 0) + V(0, 0, 1, 0) + V(0, 0, 1, 1) + V(0, 0, 0, 1)) / (4.0f * el[g.el_off[0] + ix]);
+        E1 = (V(1, 0, 0, 0) + V(1, 0, 0, 1) + V(1, 1, 0, 1) + V(1, 1, 0, 0)) / (4.0f * el[g.el_off[1] + iy]);
+        E2 = (V(2, 0, 0, 0) + V(2, 1, 0, 0) + V(2, 1, 1, 0) + V(2, 0, 1, 0)) / (4.0f * el[g.el_off[2] + iz]);
     }
     const uint32_t pos[3] = {ix, iy, iz};
     const uint32_t lim[3] = {g.nx, g.ny, g.nz};
-    const size_t step[3] = {sx, sy, 1};
-    float H[3];
+    out[0] = E0;
+    out[1] = E1;
+    out[2] = E2;
 #pragma unroll
     for (int n = 0; n < 3; ++n) {
         float h = 0.f;
         const float d0 = el[g.el_off[3 + n] + pos[n]];
         if (d0 > 0.f) {
-            h = curr.ldc(n, b) / d0;
+            h = I(n, 0) / d0;
             if (pos[n] < lim[n] - 1) {
                 const float d1 = el[g.el_off[3 + n] + pos[n] + 1];
                 const float dr = d0 / (d0 + d1);
-                const float up = curr.ldc(n, b + step[n]) / d1;
+                const float up = I(n, 1) / d1;
                 h = h * (1.0f - dr) + up * dr;
             }
         }
-        H[n] = h;
+        out[3 + n] = h;
+    }
+}
+
+// One thread per surface point.  Points on a captured z plane (see
+// NfCapture) read the slabs the H sweep wrote; the others read the fields.
+__global__ void k_nf2ff_sample(FieldView volt, FieldView curr,
+                               const uint32_t* __restrict__ pts, const float* __restrict__ el, Nf2ffGeom g,
+                               float* __restrict__ row_out /* buf + slot*rows */, NfCapture cap) {
+    // points are in grid order; consecutive workgroups share cells, so they
+    // stay on one XCD (own L2)
+    const uint32_t p = xcd_remap(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
+    if (p >= g.P) return;
+    const uint32_t ix = pts[3 * p], iy = pts[3 * p + 1], iz = pts[3 * p + 2];
+    float o[6];
+    int f = -1;
+    if (cap.on && ix - cap.x0 < cap.x1 - cap.x0 && iy - cap.y0 < cap.y1 - cap.y0)
+        f = iz == cap.z[0] ? 0 : (cap.nf > 1 && iz == cap.z[1] ? 1 : -1);
+    if (f >= 0) {
+        const float* q = cap.slab + ((size_t)(f * cap.nx_c + (ix - cap.x0)) * cap.ny_c + (iy - cap.y0)) * kCapFloats;
+        const size_t rx = (size_t)cap.ny_c * kCapFloats;  // next x row
+        auto V = [&](int c, int dx, int dy, int dz) { return q[dx * rx + dy * kCapFloats + 2 * c + dz]; };
+        auto I = [&](int n, int d) {
+            return n == 0 ? q[d * rx + 6] : (n == 1 ? q[d * kCapFloats + 7] : q[8 + d]);
+        };
+        nf_point(ix, iy, iz, el, g, V, I, o);
+    } else {
+        const size_t Ny = g.ny, Nz = g.nz;
+        const size_t b = ix * Ny * Nz + iy * Nz + iz;
+        const size_t sx = Ny * Nz, sy = Nz;
+        const size_t step[3] = {sx, sy, 1};
+        auto V = [&](int c, int dx, int dy, int dz) { return volt.ldc(c, b + dx * sx + dy * sy + dz); };
+        auto I = [&](int n, int d) { return curr.ldc(n, b + (d ? step[n] : 0)); };
+        nf_point(ix, iy, iz, el, g, V, I, o);
     }
     const size_t P = g.P, RE = g.rowsE;
-    row_out[p] = E0;
-    row_out[P + p] = E1;
-    row_out[2 * P + p] = E2;
-    row_out[RE + p] = H[0];
-    row_out[RE + P + p] = H[1];
-    row_out[RE + 2 * P + p] = H[2];
+    row_out[p] = o[0];
+    row_out[P + p] = o[1];
+    row_out[2 * P + p] = o[2];
+    row_out[RE + p] = o[3];
+    row_out[RE + P + p] = o[4];
+    row_out[RE + 2 * P + p] = o[5];
 }
 
 // FP64 twiddles for k in [0,kused): tw[(h*K + k)*NC + 2f + {0: re, 1: im}],

@@ -99,6 +99,26 @@ namespace efs {
 // Everything stays on the device: no host synchronisation in iterate().
 constexpr int kRangeLo = 6, kRangeHi = 12, kRangeTarget = 10, kRangeSibling = 14;
 
+// Block-level maximum first (waves, then LDS), one atomic per workgroup:
+// per-wave atomics on the 6 words serialise at the L2.
+__device__ __forceinline__ float absmax_block(float m) {
+    __shared__ float s_m[4];
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+        const float o = __shfl_down(m, off, 64);
+        m = (o != o || m != m) ? __builtin_nanf("") : fmaxf(m, o);
+    }
+    if ((threadIdx.x & 63u) == 0) s_m[threadIdx.x >> 6] = m;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int w = 1; w < (int)(blockDim.x >> 6); ++w) {
+            const float o = s_m[w];
+            m = (o != o || m != m) ? __builtin_nanf("") : fmaxf(m, o);
+        }
+    }
+    return m;  // valid in thread 0
+}
+
 __global__ __launch_bounds__(256) void k_absmax16(const f16* __restrict__ volt, const f16* __restrict__ curr,
                                                   uint64_t S, uint32_t N, unsigned int* __restrict__ amax) {
     const uint32_t k = blockIdx.y;  // component 0..5
@@ -109,13 +129,9 @@ __global__ __launch_bounds__(256) void k_absmax16(const f16* __restrict__ volt, 
         m = fmaxf(m, fmaxf(fmaxf(fabsf(v.x), fabsf(v.y)), fmaxf(fabsf(v.z), fabsf(v.w))));
         if (v.x != v.x || v.y != v.y || v.z != v.z || v.w != v.w) m = __builtin_nanf("");
     }
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1) {
-        const float o = __shfl_down(m, off, 64);
-        m = (o != o || m != m) ? __builtin_nanf("") : fmaxf(m, o);
-    }
+    m = absmax_block(m);
     // non-negative floats order like their bit patterns; NaN sorts above inf
-    if ((threadIdx.x & 63u) == 0) atomicMax(amax + k, __float_as_uint(m));
+    if (threadIdx.x == 0) atomicMax(amax + k, __float_as_uint(m));
 }
 
 __global__ void k_range_plan(unsigned int* __restrict__ amax, float* __restrict__ fscale,

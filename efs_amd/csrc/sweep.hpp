@@ -102,6 +102,25 @@ struct SweepGeom {
     const SweepBlock* tab;
 };
 
+// NF2FF z-face capture by the H sweep.  Points of the NF2FF surface on a
+// z plane read one cache line per (x, y) row and array in NIJK, for 2 of
+// its values.  The H sweep already holds those rows: for each plane zf (zf
+// and zf+1 inside one 4-cell lane) and each row (x, y) of the rectangle
+// [x0, x1] x [y0, y1], the lane that owns them stores 12 floats into
+// slab[(f * nx_c + x - x0) * ny_c + y - y0][12]:
+//   E_x, E_y, E_z at zf and zf+1 (6), then H_x(zf), H_y(zf), H_z(zf),
+//   H_z(zf+1) as stored by the sweep, 2 unused.
+// The NF2FF sample kernel then reads those slabs instead of the rows.
+struct NfCapture {
+    float* slab;
+    uint32_t on;          // 0: off
+    uint32_t nf;          // planes: 1 or 2
+    uint32_t z[2];
+    uint32_t x0, x1, y0, y1;  // inclusive rectangle of rows
+    uint32_t nx_c, ny_c;  // x1 - x0 + 1, y1 - y0 + 1
+};
+constexpr int kCapFloats = 12;
+
 // XCD-contiguous remap of the hardware workgroup id: dispatch hands out
 // workgroup ids round-robin over the 8 XCDs; this gives XCD k the logical
 // range [k*q + min(k,r), ...) so adjacent tiles land on one XCD.
@@ -150,6 +169,8 @@ struct Fs<float> {
     __device__ static __forceinline__ v4 ld(const float* p, float) { return ld4(p); }
     __device__ static __forceinline__ float ld1(const float* p, float) { return *p; }
     __device__ static __forceinline__ void st(float* p, v4 v, float) { st4(p, v); }
+    // the value a later load returns for v stored with (s, 1/s)
+    __device__ static __forceinline__ v4 stored(v4 v, float, float) { return v; }
 };
 // FP16 stores saturate at the largest finite half: a value that outgrew its
 // scale between two range checks is clamped (and the next check rescales)
@@ -170,6 +191,9 @@ struct Fs<f16> {
     __device__ static __forceinline__ void st(f16* p, v4 v, float inv) {
         *reinterpret_cast<h4*>(p) = __builtin_convertvector(clamp16(v * inv), h4);
     }
+    __device__ static __forceinline__ v4 stored(v4 v, float s, float inv) {
+        return __builtin_convertvector(__builtin_convertvector(clamp16(v * inv), h4), v4) * s;
+    }
 };
 
 // A field array (volt or curr, 3 components S apart) as the point kernels
@@ -182,11 +206,15 @@ struct FieldView {
     uint32_t half;
 
     // component c, index j inside the component
+    // (products not contracted into the caller's sums: a load returns the
+    // value a store-and-reload would)
     __device__ __forceinline__ float ldc(uint32_t c, size_t j) const {
+#pragma clang fp contract(off)
         if (!half) return static_cast<const float*>(p)[c * S + j];
         return (float)static_cast<const f16*>(p)[c * S + j] * scale[c];
     }
     __device__ __forceinline__ float ld(size_t i) const {
+#pragma clang fp contract(off)
         if (!half) return static_cast<const float*>(p)[i];
         return (float)static_cast<const f16*>(p)[i] * scale[i / S];
     }
@@ -576,7 +604,7 @@ __device__ __forceinline__ void e4_march(T* __restrict__ volt, const T* __restri
                                          const float2* s_tab, const float4* s_ptab, const SweepGeom& g,
                                          const PmlTable& pml,
                                          const float* __restrict__ pcoef, float* __restrict__ pflux, Lane4 w,
-                                         uint32_t m, const Scales& sc) {
+                                         uint32_t m, const Scales& sc, const NfCapture&) {
     typedef Fs<T> L;
     const size_t S = g.S;
     T* const E[3] = {volt, volt + S, volt + 2 * S};
@@ -642,10 +670,16 @@ __device__ __forceinline__ void h4_march(T* __restrict__ curr, const T* __restri
                                          const float2* s_tab, const float4* s_ptab, const SweepGeom& g,
                                          const PmlTable& pml,
                                          const float* __restrict__ pcoef, float* __restrict__ pflux, Lane4 w,
-                                         uint32_t m, const Scales& sc) {
+                                         uint32_t m, const Scales& sc, const NfCapture& cap) {
     typedef Fs<T> L;
     const size_t S = g.S;
     T* const Ic[3] = {curr, curr + S, curr + 2 * S};
+    // NF2FF capture: planes of the capture inside this lane (bit f), when
+    // the lane's row lies in the rectangle
+    uint32_t capz = 0;
+    if (cap.on && w.y - cap.y0 <= cap.y1 - cap.y0)
+        for (uint32_t f = 0; f < cap.nf; ++f)
+            if (cap.z[f] >= w.z0 && cap.z[f] + 1 < w.z0 + 4) capz |= 1u << f;
     const T* const E0 = volt;
     const T* const E1 = volt + S;
     const T* const E2 = volt + 2 * S;
@@ -692,21 +726,44 @@ __device__ __forceinline__ void h4_march(T* __restrict__ curr, const T* __restri
         const v4 ey_zp = (v4){ey.y, ey.z, ey.w, ey_z};
         const v4 ex_zp = (v4){ex.y, ex.z, ex.w, ex_z};
         const v4 cur[3] = {ez - ez_yp - ey + ey_zp, ex - ex_zp - ez + ez_p, ey - ey_p - ex + ex_yp};
+        const v4 e_x[3] = {ex, ey, ez};  // E at this plane, for the capture
         ex = ex_p;
         ey = ey_p;
         ez = ez_p;
         carry = core;
+        v4 In[3];
         if (!upml) {
 #pragma unroll
-            for (int n = 0; n < 3; ++n)
-                L::st(Ic[n] + i, sel4(cb, fma4(a[n], I[n], b[n] * cur[n]), I[n]), sc.own_inv[n]);
+            for (int n = 0; n < 3; ++n) {
+                In[n] = sel4(cb, fma4(a[n], I[n], b[n] * cur[n]), I[n]);
+                L::st(Ic[n] + i, In[n], sc.own_inv[n]);
+            }
         } else {
             upml_decode<PCOMP>(s_ptab, pq, P0, P1, P2);
 #pragma unroll
             for (int n = 0; n < 3; ++n) {
                 const v4 Fn = sel4(cb, fma4(a[n], F[n], b[n] * cur[n]), F[n]);
-                L::st(Ic[n] + i, fma4(P2[n], Fn, fma4(P0[n], I[n], -(P1[n] * F[n]))), sc.own_inv[n]);
+                In[n] = fma4(P2[n], Fn, fma4(P0[n], I[n], -(P1[n] * F[n])));
+                L::st(Ic[n] + i, In[n], sc.own_inv[n]);
                 st4(u.flux + n * u.cs, Fn);
+            }
+        }
+        if (capz && x - cap.x0 <= cap.x1 - cap.x0) {
+#pragma unroll
+            for (uint32_t f = 0; f < 2; ++f) {
+                if (!(capz & (1u << f))) continue;
+                const uint32_t o = cap.z[f] - w.z0;  // 0..2: zf and zf+1 in this lane
+                float* q = cap.slab +
+                           ((size_t)(f * cap.nx_c + (x - cap.x0)) * cap.ny_c + (w.y - cap.y0)) * kCapFloats;
+                const v4 hx = L::stored(In[0], sc.own[0], sc.own_inv[0]);
+                const v4 hy = L::stored(In[1], sc.own[1], sc.own_inv[1]);
+                const v4 hz = L::stored(In[2], sc.own[2], sc.own_inv[2]);
+                const v4 q0 = {e_x[0][o], e_x[0][o + 1], e_x[1][o], e_x[1][o + 1]};
+                const v4 q1 = {e_x[2][o], e_x[2][o + 1], hx[o], hy[o]};
+                const v4 q2 = {hz[o], hz[o + 1], 0.f, 0.f};
+                st4(q, q0);
+                st4(q + 4, q1);
+                st4(q + 8, q2);
             }
         }
     }
@@ -754,7 +811,7 @@ __device__ __forceinline__ bool pml_lane(const PmlTable& pml, Lane4& w) {
 
 #define EFS_SWEEP4_PARAMS(T)                                                                                    \
     T *__restrict__ dst, const T *__restrict__ src, Coef cf, SweepGeom g, PmlTable pml,                        \
-        const float *__restrict__ pcoef, float *__restrict__ pflux, const float *__restrict__ fscale
+        const float *__restrict__ pcoef, float *__restrict__ pflux, const float *__restrict__ fscale, NfCapture cap
 
 // Interior (4 waves/SIMD) and UPML (3 waves/SIMD) entry points, each with
 // FP32 arrays or table-compressed (suffix c) coefficients; the interior
@@ -768,7 +825,8 @@ __device__ __forceinline__ bool pml_lane(const PmlTable& pml, Lane4& w) {
         uint32_t m, flags;                                                                                         \
         if (!interior_lane<HAS_PML>(g, pml, w, m, flags)) return;                                                  \
         const Scales sc = load_scales<float>(fscale, HF);                                                          \
-        MARCH<kMarchInterior, COMP, false, float>(dst, src, cf, s_tab, nullptr, g, pml, pcoef, pflux, w, m, sc);   \
+        MARCH<kMarchInterior, COMP, false, float>(dst, src, cf, s_tab, nullptr, g, pml, pcoef, pflux, w, m, sc,    \
+                                                  cap);                                                         \
     }
 
 #define EFS_SWEEP4_UPML(NAME, MARCH, HF, COMP)                                                                     \
@@ -780,7 +838,7 @@ __device__ __forceinline__ bool pml_lane(const PmlTable& pml, Lane4& w) {
         if (!pml_lane(pml, w)) return;                                                                             \
         const Scales sc = load_scales<float>(fscale, HF);                                                          \
         MARCH<kMarchRegion, COMP, false, float>(dst, src, cf, s_tab, nullptr, g, pml, pcoef, pflux, w,             \
-                                                1u << w.region, sc);                                               \
+                                                1u << w.region, sc, NfCapture{});                                  \
     }
 
 // Fused: the grid launch does the UPML cells too (3 waves/SIMD, the UPML
@@ -798,7 +856,7 @@ __device__ __forceinline__ bool pml_lane(const PmlTable& pml, Lane4& w) {
         upml_stage<PCOMP>(cf, s_ptab, flags);                                                                      \
         if (!live) return;                                                                                         \
         const Scales sc = load_scales<T>(fscale, HF);                                                              \
-        MARCH<kMarchFused, COMP, PCOMP, T>(dst, src, cf, s_tab, s_ptab, g, pml, pcoef, pflux, w, m, sc);           \
+        MARCH<kMarchFused, COMP, PCOMP, T>(dst, src, cf, s_tab, s_ptab, g, pml, pcoef, pflux, w, m, sc, cap);      \
     }
 
 // Grid launch without UPML, FP16 field storage (FP32 storage uses the
@@ -812,7 +870,7 @@ __device__ __forceinline__ bool pml_lane(const PmlTable& pml, Lane4& w) {
         uint32_t m, flags;                                                                                         \
         if (!interior_lane<false>(g, pml, w, m, flags)) return;                                                    \
         const Scales sc = load_scales<f16>(fscale, HF);                                                            \
-        MARCH<kMarchInterior, COMP, false, f16>(dst, src, cf, s_tab, nullptr, g, pml, pcoef, pflux, w, m, sc);     \
+        MARCH<kMarchInterior, COMP, false, f16>(dst, src, cf, s_tab, nullptr, g, pml, pcoef, pflux, w, m, sc, cap); \
     }
 
 }  // namespace efs

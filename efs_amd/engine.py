@@ -497,10 +497,11 @@ class HIPEngine:
         return out
 
     def pathFlags(self):
-        """Sweep forms in use: set of 'vec4', 'coef_idx', 'upml4', 'fused', 'upml_idx'."""
+        """Forms in use: set of 'vec4', 'coef_idx', 'upml4', 'fused', 'upml_idx',
+        'f16', 'nf2ff_capture'."""
         f = C.c_uint32(0)
         self._check(self.lib.efs_path_flags(self.ctx, C.byref(f)))
-        names = ("vec4", "coef_idx", "upml4", "fused", "upml_idx", "f16")
+        names = ("vec4", "coef_idx", "upml4", "fused", "upml_idx", "f16", "nf2ff_capture")
         return {n for b, n in enumerate(names) if f.value & (1 << b)}
 
 

@@ -343,6 +343,46 @@ def test_nf2ff_accumulation(HIPEngine, oracle_mod, kblock, steps, window, shuffl
             assert err < 2e-5, (F, nm, err)
 
 
+@pytest.mark.parametrize("storage", ["fp32", "fp16"])
+def test_nf2ff_zface_capture(HIPEngine, oracle_mod, monkeypatch, storage):
+    """Surface points on z planes read slabs the H sweep wrote (the planes
+    have z and z+1 in one 4-cell lane, the rows no UPML): the accumulators
+    equal those of the plain gather bit for bit, and the oracle's within the
+    FP32 tolerance."""
+    dims = (30, 28, 40)
+    case = random_case(31, dims, pml_width=4)  # UPML z-bounds multiples of 4 (FP16)
+    pts, primal, dual = nf2ff_setup(dims, inset=5)
+    dT = 1.9e-12
+    omegas = 2 * np.pi * np.array([1.0e9, 3.3e9, 7.0e9])
+    steps = 70
+    cfg = {"surfaceIndices": pts, "omega": list(omegas), "dT": dT, "primalEdgeLens": primal,
+           "dualEdgeLens": dual, "kBlock": 16}
+    out = {}
+    for cap in ("1", "0"):
+        monkeypatch.setenv("EFS_NF2FF_CAPTURE", cap)
+        say("case nf2ff capture", cap, storage)
+        g = HIPEngine()
+        assert g.initGPU()
+        g.init(case["dims"], case["coeffs"], storage=storage)
+        configure(g, case)
+        g.configureNF2FFAccumulation(cfg)
+        assert ("nf2ff_capture" in g.pathFlags()) == (cap == "1")
+        g.iterate(steps)
+        out[cap] = g.readNF2FFAccumulation()
+        g.destroy()
+    for k in ("accumE", "accumH"):
+        np.testing.assert_array_equal(np.asarray(out["1"][k]), np.asarray(out["0"][k]))
+    if storage == "fp32":
+        o = make_oracle(oracle_mod, case, fp64=False)
+        o.configureNF2FF(pts, omegas, dT, primal, dual)
+        o.iterate(steps)
+        refE, refH = o.readNF2FF()
+        for f in range(len(omegas)):
+            got = np.concatenate([np.asarray(out["1"]["accumE"][f]), np.asarray(out["1"]["accumH"][f])])
+            want = np.concatenate([refE[f], refH[f]])
+            assert rel_l2(got, want) <= 2e-5
+
+
 def test_farfield_matches_fp64_oracle(HIPEngine, oracle_mod):
     dims = (14, 13, 15)
     case = random_case(22, dims, pml_width=2)
