@@ -290,6 +290,106 @@ def run_batch(args):
     print(json.dumps(result))
 
 
+def patch_outputs(eng, w, slices, steps, want_ff=True):
+    """S11 over 1-3 GHz (201 points), resonance and Dmax at the NF2FF
+    frequency from an engine (HIP or oracle) after `steps` steps."""
+    from efs_amd import farfield, ports
+    r = eng.readProbeGatherBuffered()
+    if isinstance(r, dict):
+        data, ns, stride = r["data"], r["numSamples"], r["stride"]
+    else:
+        data, ns, stride = r.ravel(), r.shape[0], r.shape[1]
+    tr = ports.probe_traces(data, ns, stride, w["probes"], slices, 1, w["dT"])
+    freqs = np.linspace(1e9, 3e9, 201)
+    s11 = 20 * np.log10(np.abs(ports.port_s11(tr[0], tr[1], freqs)))
+    k = int(np.argmin(s11))
+    out = {"s11_dB": s11, "f_res_GHz": float(freqs[k] / 1e9), "s11_min_dB": float(s11[k])}
+    if want_ff:
+        P = w["nf2ff"]["numPoints"]
+        if hasattr(eng, "readNF2FFAccumulation"):
+            acc = eng.readNF2FFAccumulation()
+            accE, accH = np.asarray(acc["accumE"][0]), np.asarray(acc["accumH"][0])
+        else:
+            e, h = eng.readNF2FF()
+            accE, accH = e[0], h[0]
+        theta, phi = farfield.sphere_grid()
+        if hasattr(eng, "computeNF2FFfarField"):
+            ff = eng.computeNF2FFfarField({"pointMeta": w["pointMeta"], "theta": theta, "phi": phi,
+                                           "center": [0, 0, 0], "frequency": float(w["freqs"][0]), "radius": 1.0})
+        else:
+            from oracle import oracle as orc
+            ff = orc.farfield(accE, accH, w["pointMeta"], theta, phi, np.zeros(3), float(w["freqs"][0]), 1.0)
+        p = farfield.rad_power(accE, accH, w["pointMeta"], P)
+        d = farfield.directivity(ff, len(theta), len(phi), p)
+        out.update({"Dmax_dBi": float(d["DmaxdBi"]), "rad_power_W": p})
+    return out
+
+
+def run_patch(args):
+    """The reference's patch-antenna example (86 x 87 x 71 graded mesh, Mur,
+    lumped port, NF2FF at 2.4 GHz, 30,000 steps) on the plain engine."""
+    from efs_amd import workloads
+    from efs_amd.engine import PROF_SWEEP_E, PROF_SWEEP_H, HIPEngine
+    w = workloads.patch_antenna()
+    n = int(np.prod(w["gridSize"]))
+    K = args.steps if args.steps != 2000 else w["nTS"]
+    g = HIPEngine()
+    if not g.initGPU():
+        raise RuntimeError("no HIP device")
+    slices = workloads.configure_patch(g, w, nf2ff=True, max_samples=K, storage=args.storage)
+    g.sync()
+    t0 = time.perf_counter()
+    g.iterate(K)
+    g.sync()
+    elapsed = time.perf_counter() - t0
+    value = n * K / elapsed / 1e6
+    res = patch_outputs(g, w, slices, K)
+    g.profileEnable(True)
+    g.iterate(args.profile_steps)
+    g.sync()
+    ms_e, ce = g.profileRead(PROF_SWEEP_E)
+    ms_h, ch = g.profileRead(PROF_SWEEP_H)
+    flags = g.pathFlags()
+    g.destroy()
+    avg_e, avg_h = ms_e / ce, ms_h / ch
+    b = sweep_bytes(n, 0, args.storage)
+    dom_name, dom_ms = ("sweep_h", avg_h) if avg_h >= avg_e else ("sweep_e", avg_e)
+    extra = {"results": {"f_res_GHz": res["f_res_GHz"], "s11_min_dB": res["s11_min_dB"],
+                         "Dmax_dBi_2p4GHz": res["Dmax_dBi"]}}
+    if not args.no_compare:
+        print("# FP64 oracle run of the same case", file=sys.stderr, flush=True)
+        from oracle import oracle as orc
+        threads = min(len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else 1, 16)
+        orc.Oracle.set_threads(threads)
+        o = orc.Oracle(w["gridSize"], w["coefficients"], fp64=True)
+        so = workloads.configure_patch(o, w, nf2ff=True, max_samples=K)
+        o.iterate(K)
+        ref = patch_outputs(o, w, so, K)
+        extra["precision"] = {"against": "FP64 oracle (C restatement, double storage), same case and steps",
+                              "s11_max_abs_err_dB_1_3GHz": float(np.max(np.abs(res["s11_dB"] - ref["s11_dB"]))),
+                              "Dmax_err_dB": float(abs(res["Dmax_dBi"] - ref["Dmax_dBi"])),
+                              "fp64": {"f_res_GHz": ref["f_res_GHz"], "s11_min_dB": ref["s11_min_dB"],
+                                       "Dmax_dBi": ref["Dmax_dBi"]}}
+    base = None if args.no_cpu_baseline else cpu_baseline(
+        w, args.cpu_seconds, what=f"the same patch case ({'x'.join(map(str, w['gridSize']))}, Mur + port)")
+    result = {
+        "metric": METRIC, "value": round(value, 1), "unit": "MCells/s", "n_gpus": 1, "steps": K, "warmup": 0,
+        "ms_per_step": round(elapsed / K * 1e3, 4), "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32" if args.storage == "fp32" else "f16 storage, f32 compute", "data": "synthetic",
+        "config": {"workload": "microstrip patch antenna of the reference example: 32x40 mm patch, 1.524 mm "
+                               "eps_r 3.38 substrate, 60x60 mm ground, 50-ohm lumped port, Gaussian f0=fc=3 GHz, "
+                               "Mur on 6 faces, NF2FF box at 2.4 GHz, whole run timed from zero fields",
+                   "grid": w["gridSize"], "dT": w["dT"], "path": sorted(flags), "parallelism": "single GPU"},
+        "roofline": {"bound": "hbm", "kernel": dom_name, "achieved": round(b / (dom_ms * 1e-3) / 1e9, 1),
+                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(b / (dom_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+                     "traffic": None, "traffic_source": None, "algorithmic_bytes_per_launch": b,
+                     "avg_launch_ms": round(dom_ms, 5),
+                     "kernels": {"sweep_e": {"avg_ms": round(avg_e, 5)}, "sweep_h": {"avg_ms": round(avg_h, 5)}}},
+        "cpu_baseline": base, **extra,
+    }
+    print(json.dumps(result))
+
+
 def calibrate_copy():
     import torch
     a = torch.ones(256 * 1024 * 1024, dtype=torch.float32, device="cuda")
@@ -314,8 +414,9 @@ def main():
                          "FP32 updates)")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="run one elementwise copy of 1 GiB -> 1 GiB first (reference kernel for HBM counters)")
-    ap.add_argument("--workload", choices=["stress", "batch"], default="stress",
-                    help="stress: the 256^3 UPML stress grid; batch: the batched patch-antenna design sweep")
+    ap.add_argument("--workload", choices=["stress", "batch", "patch"], default="stress",
+                    help="stress: the 256^3 UPML stress grid; batch: the batched patch-antenna design sweep; "
+                         "patch: the reference's patch-antenna example on the plain engine")
     ap.add_argument("--instances", type=int, default=64, help="batch: number of patch instances")
     ap.add_argument("--plain-steps", type=int, default=200, help="batch: steps per instance of the plain baseline")
     ap.add_argument("--s11-instances", type=int, default=3, help="batch: instances in the FP64 S11 comparison")
@@ -329,10 +430,10 @@ def main():
                                    "of this build (use --gpus 1)", "n_gpus": args.gpus}))
         sys.exit(2)
 
-    if args.workload == "batch":
+    if args.workload in ("batch", "patch"):
         if args.calibrate_copy:
             calibrate_copy()
-        return run_batch(args)
+        return run_batch(args) if args.workload == "batch" else run_patch(args)
 
     if args.calibrate_copy:
         calibrate_copy()

@@ -61,6 +61,11 @@ struct efs_ctx {
 
     uint32_t nx = 0, ny = 0, nz = 0;
     size_t N = 0;      // cells per component (interface layout)
+    // Device rows hold nzp >= nz cells (Nz rounded up to a multiple of 4, so
+    // every grid runs the 4-cells-per-lane sweeps); padding cells have zero
+    // coefficients and stay zero.  Np = Nx*Ny*nzp.
+    uint32_t nzp = 0;
+    size_t Np = 0;
     size_t S = 0;      // component stride of the device arrays (>= N, padded)
     bool vec4 = false; // 4 z cells per lane (Nz % 4 == 0)
     float* pool = nullptr;  // the six field/coefficient arrays, staggered
@@ -264,8 +269,10 @@ int need_init(efs_ctx* c) {
 // Interface field index (component*N + linear position) -> device index
 // (component*S + linear position).
 inline uint32_t tr(const efs_ctx* c, uint64_t gidx) {
-    const uint64_t n = gidx / c->N;
-    return (uint32_t)(n * c->S + (gidx - n * c->N));
+    const uint64_t n = gidx / c->N, r = gidx - n * c->N;
+    if (c->nzp == c->nz) return (uint32_t)(n * c->S + r);
+    const uint64_t xy = r / c->nz;
+    return (uint32_t)(n * c->S + xy * c->nzp + (r - xy * c->nz));
 }
 
 // Distinct (a, b) pairs of one component: byte index per cell and a
@@ -571,7 +578,7 @@ bool box_overlap(const PmlRegion& a, const PmlRegion& b) {
 // The launch gets ~2048 workgroups (8 per CU on 256 CUs) but chunks of 8
 // planes on average; heavy tiles go down to 2-plane chunks.
 int plan_sweeps(efs_ctx* c) {
-    const uint32_t nx = c->nx, ny = c->ny, nz = c->nz;
+    const uint32_t nx = c->nx, ny = c->ny, nz = c->nzp;  // device rows
     const uint64_t cols = (uint64_t)ny * nz;
     const bool upml_in_grid = !(c->vec4 && c->pml_split && !c->pml_fused);
     const double core_b = c->comp ? 39.0 : 60.0;
@@ -609,7 +616,9 @@ int plan_sweeps(efs_ctx* c) {
                         up[(size_t)t * nx + x] = 1;
                     }
         for (double q : cost) total += q;
-        const double nblocks = (double)std::max<uint64_t>(1, std::min<uint64_t>(want, (uint64_t)T * std::max(1u, nx / 8)));
+        // small grids have few column tiles: cut them into more, shorter
+        // chunks (down to 2 planes) so that the launch still fills the GPU
+        const double nblocks = (double)std::max<uint64_t>(1, std::min<uint64_t>(want, (uint64_t)T * std::max(1u, nx / 2)));
         const double target = total / nblocks;
         std::vector<SweepBlock> tab;
         std::vector<double> pre(nx + 1);
@@ -714,7 +723,7 @@ int plan_nf_capture(efs_ctx* c, const std::vector<uint32_t>& pts, uint32_t P) {
 
 // FP16 storage: largest |stored| per component, then rescale where needed.
 void launch_range_control(efs_ctx* c, uint32_t t) {
-    const uint32_t N = (uint32_t)c->N;
+    const uint32_t N = (uint32_t)c->Np;  // padding cells are zero
     const dim3 grid(std::max(1u, std::min(256u, (N / 4 + 255) / 256)), 6);
     hipLaunchKernelGGL(k_absmax16, grid, dim3(256), 0, c->stream, (const f16*)c->volt, (const f16*)c->curr,
                        (uint64_t)c->S, N, c->f_amax);
@@ -785,7 +794,7 @@ int launch_step(efs_ctx* c) {
     }
     for (const PbcAxis& a : c->pbc_e)  // after TFSF-V, before Mur post (src/webgpu-engine.mjs:2461)
         hipLaunchKernelGGL(k_pbc, dim3(nblk((size_t)a.b0 * a.b1)), dim3(kPointThreads), 0, c->stream, fv(c, 0),
-                           c->pbc_imag, a, c->ny, c->nz, (uint64_t)c->S);
+                           c->pbc_imag, a, c->ny, c->nzp, (uint64_t)c->S);
     if (c->mur_n) {
         if (c->mur_ne)
             hipLaunchKernelGGL(k_mur_post_some, dim3(nblk(c->mur_ne)), dim3(kPointThreads), 0, c->stream, fv(c, 0),
@@ -850,7 +859,7 @@ int launch_step(efs_ctx* c) {
     }
     for (const PbcAxis& a : c->pbc_h)  // after TFSF-I (src/webgpu-engine.mjs:2477)
         hipLaunchKernelGGL(k_pbc, dim3(nblk((size_t)a.b0 * a.b1)), dim3(kPointThreads), 0, c->stream, fv(c, 1),
-                           c->pbc_imag, a, c->ny, c->nz, (uint64_t)c->S);
+                           c->pbc_imag, a, c->ny, c->nzp, (uint64_t)c->S);
     if (c->ade_n && c->ade_apply) {
         hipLaunchKernelGGL(k_ade_apply, dim3(nblk(c->ade_n)), dim3(kPointThreads), 0, c->stream, fv(c, 1),
                            c->ade_gidx, c->ade_ia, c->ade_n);
@@ -943,12 +952,15 @@ int efs_init(efs_ctx* c, uint32_t nx, uint32_t ny, uint32_t nz, const float* vv,
     reset_extensions(c);
     c->inited = false;
     c->nx = nx; c->ny = ny; c->nz = nz; c->N = N;
+    c->nzp = env_u64("EFS_VEC", 4) == 4 ? (nz + 3) / 4 * 4 : nz;
+    c->Np = (uint64_t)nx * ny * c->nzp;
+    const uint64_t Np = c->Np;
     // Device layout: component stride S = N rounded up to 256 floats plus a
     // pad, the six arrays in one pool with a stagger between them, so the
     // concurrent streams of a sweep are not spaced at powers of two.
     const uint64_t pad = env_u64("EFS_PAD", 1088);
     const uint64_t stagger = env_u64("EFS_STAGGER", 4160);
-    c->S = ((N + 255) / 256) * 256 + (pad / 4) * 4;
+    c->S = ((Np + 255) / 256) * 256 + (pad / 4) * 4;
     if (3 * c->S >= (1ull << 31)) return fail(c, EFS_ERR_ARG, "grid too large for 32-bit device indices");
     const uint64_t arr = 3 * c->S + (stagger / 64) * 64;
     TRY(dalloc(c, c->pool, 6 * arr));
@@ -959,15 +971,32 @@ int efs_init(efs_ctx* c, uint32_t nx, uint32_t ny, uint32_t nz, const float* vv,
     c->vi = c->pool + 3 * arr;
     c->ii = c->pool + 4 * arr;
     c->iv = c->pool + 5 * arr;
+    // coefficients in the device row layout: padded copies when nzp > nz,
+    // with the last real z plane of the H side set to ii = 1, iv = 0 (those
+    // cells keep I, the reference's last-plane rule, src/webgpu-engine.mjs:88)
+    std::vector<float> padded[4];
     const float* src[4] = {vv, vi, ii, iv};
+    if (c->nzp != nz) {
+        for (int a = 0; a < 4; ++a) {
+            padded[a].assign(3 * Np, 0.0f);
+            for (uint32_t n = 0; n < 3; ++n)
+                for (uint64_t xy = 0; xy < (uint64_t)nx * ny; ++xy) {
+                    float* d = padded[a].data() + n * Np + xy * c->nzp;
+                    std::memcpy(d, src[a] + n * N + xy * nz, nz * sizeof(float));
+                    if (a == 2) d[nz - 1] = 1.0f;
+                    if (a == 3) d[nz - 1] = 0.0f;
+                }
+            src[a] = padded[a].data();
+        }
+    }
     float* dst[4] = {c->vv, c->vi, c->ii, c->iv};
     for (int a = 0; a < 4; ++a)
         for (int n = 0; n < 3; ++n)
-            HIP_TRY(c, hipMemcpy(dst[a] + n * c->S, src[a] + n * N, N * sizeof(float), hipMemcpyHostToDevice));
-    c->vec4 = (nz % 4 == 0) && env_u64("EFS_VEC", 4) == 4;
+            HIP_TRY(c, hipMemcpy(dst[a] + n * c->S, src[a] + n * Np, Np * sizeof(float), hipMemcpyHostToDevice));
+    c->vec4 = c->nzp % 4 == 0 && env_u64("EFS_VEC", 4) == 4;
     c->half = c->storage_req == EFS_STORAGE_F16;
     c->range_interval = (uint32_t)std::max<uint64_t>(1, env_u64("EFS_RANGE_INTERVAL", 32));
-    if (c->half && !c->vec4) return fail(c, EFS_ERR_ARG, "FP16 field storage needs Nz %% 4 == 0");
+    if (c->half && !c->vec4) return fail(c, EFS_ERR_ARG, "FP16 field storage needs the 4-wide sweeps (EFS_VEC=4)");
     {
         const float one[6] = {1.f, 1.f, 1.f, 1.f, 1.f, 1.f};
         TRY(dupload(c, c->fscale, one, 6));
@@ -982,8 +1011,8 @@ int efs_init(efs_ctx* c, uint32_t nx, uint32_t ny, uint32_t nz, const float* vv,
         std::vector<float2> te(3 * 256), th(3 * 256);
         bool ok = true;
         for (int n = 0; n < 3 && ok; ++n) {
-            ok = pair_table(vv + n * N, vi + n * N, N, ie.data() + n * c->S, te.data() + n * 256) &&
-                 pair_table(ii + n * N, iv + n * N, N, ih.data() + n * c->S, th.data() + n * 256);
+            ok = pair_table(src[0] + n * Np, src[1] + n * Np, Np, ie.data() + n * c->S, te.data() + n * 256) &&
+                 pair_table(src[2] + n * Np, src[3] + n * Np, Np, ih.data() + n * c->S, th.data() + n * 256);
         }
         if (ok) {
             TRY(dupload(c, c->cidx_e, ie.data(), ie.size()));
@@ -1414,27 +1443,32 @@ int efs_set_num_ts(efs_ctx* c, uint32_t ts) {
 int efs_get_fields(efs_ctx* c, float* volt, float* curr) {
     TRY(need_init(c));
     HIP_TRY(c, hipStreamSynchronize(c->stream));
+    const size_t rows = (size_t)c->nx * c->ny;
     if (c->half) {  // widen and scale on the host
         float sc[6];
         HIP_TRY(c, hipMemcpy(sc, c->fscale, sizeof sc, hipMemcpyDeviceToHost));
-        std::vector<f16> tmp(c->N);
+        std::vector<f16> tmp(c->Np);
         for (int a = 0; a < 2; ++a) {
             float* out = a ? curr : volt;
             if (!out) continue;
             const f16* src = (const f16*)(a ? c->curr : c->volt);
             for (int n = 0; n < 3; ++n) {
-                HIP_TRY(c, hipMemcpy(tmp.data(), src + n * c->S, c->N * sizeof(f16), hipMemcpyDeviceToHost));
+                HIP_TRY(c, hipMemcpy(tmp.data(), src + n * c->S, c->Np * sizeof(f16), hipMemcpyDeviceToHost));
                 const float s = sc[3 * a + n];
-                for (size_t j = 0; j < c->N; ++j) out[n * c->N + j] = (float)tmp[j] * s;
+                for (size_t r = 0; r < rows; ++r)
+                    for (uint32_t z = 0; z < c->nz; ++z)
+                        out[n * c->N + r * c->nz + z] = (float)tmp[r * c->nzp + z] * s;
             }
         }
         return EFS_OK;
     }
-    for (int n = 0; n < 3; ++n) {
-        if (volt)
-            HIP_TRY(c, hipMemcpy(volt + n * c->N, c->volt + n * c->S, c->N * sizeof(float), hipMemcpyDeviceToHost));
-        if (curr)
-            HIP_TRY(c, hipMemcpy(curr + n * c->N, c->curr + n * c->S, c->N * sizeof(float), hipMemcpyDeviceToHost));
+    for (int a = 0; a < 2; ++a) {
+        float* out = a ? curr : volt;
+        if (!out) continue;
+        const float* src = a ? c->curr : c->volt;
+        for (int n = 0; n < 3; ++n)
+            HIP_TRY(c, hipMemcpy2D(out + n * c->N, c->nz * sizeof(float), src + n * c->S, c->nzp * sizeof(float),
+                                   c->nz * sizeof(float), rows, hipMemcpyDeviceToHost));
     }
     return EFS_OK;
 }
@@ -1442,10 +1476,11 @@ int efs_get_fields(efs_ctx* c, float* volt, float* curr) {
 int efs_upload_fields(efs_ctx* c, const float* volt, const float* curr) {
     TRY(need_init(c));
     HIP_TRY(c, hipStreamSynchronize(c->stream));
+    const size_t rows = (size_t)c->nx * c->ny;
     if (c->half) {  // per component: scale 2^e with the largest value near 2^10, then narrow
         float sc[6];
         HIP_TRY(c, hipMemcpy(sc, c->fscale, sizeof sc, hipMemcpyDeviceToHost));
-        std::vector<f16> tmp(c->N);
+        std::vector<f16> tmp(c->Np, (f16)0.0f);
         for (int a = 0; a < 2; ++a) {
             const float* in = a ? curr : volt;
             if (!in) continue;
@@ -1456,8 +1491,9 @@ int efs_upload_fields(efs_ctx* c, const float* volt, const float* curr) {
                 for (size_t j = 0; j < c->N; ++j) m = std::max(m, std::fabs(src[j]));
                 if (m > 0.f && std::isfinite(m)) sc[3 * a + n] = std::ldexp(1.0f, std::ilogb(m) - kRangeTarget);
                 const float inv = 1.0f / sc[3 * a + n];
-                for (size_t j = 0; j < c->N; ++j) tmp[j] = (f16)(src[j] * inv);
-                HIP_TRY(c, hipMemcpy(dst + n * c->S, tmp.data(), c->N * sizeof(f16), hipMemcpyHostToDevice));
+                for (size_t r = 0; r < rows; ++r)
+                    for (uint32_t z = 0; z < c->nz; ++z) tmp[r * c->nzp + z] = (f16)(src[r * c->nz + z] * inv);
+                HIP_TRY(c, hipMemcpy(dst + n * c->S, tmp.data(), c->Np * sizeof(f16), hipMemcpyHostToDevice));
             }
         }
         HIP_TRY(c, hipMemcpy(c->fscale, sc, sizeof sc, hipMemcpyHostToDevice));
@@ -1466,11 +1502,13 @@ int efs_upload_fields(efs_ctx* c, const float* volt, const float* curr) {
         return EFS_OK;
     }
     c->fields_zero = false;
-    for (int n = 0; n < 3; ++n) {
-        if (volt)
-            HIP_TRY(c, hipMemcpy(c->volt + n * c->S, volt + n * c->N, c->N * sizeof(float), hipMemcpyHostToDevice));
-        if (curr)
-            HIP_TRY(c, hipMemcpy(c->curr + n * c->S, curr + n * c->N, c->N * sizeof(float), hipMemcpyHostToDevice));
+    for (int a = 0; a < 2; ++a) {
+        const float* in = a ? curr : volt;
+        if (!in) continue;
+        float* dst = a ? c->curr : c->volt;
+        for (int n = 0; n < 3; ++n)
+            HIP_TRY(c, hipMemcpy2D(dst + n * c->S, c->nzp * sizeof(float), in + n * c->N, c->nz * sizeof(float),
+                                   c->nz * sizeof(float), rows, hipMemcpyHostToDevice));
     }
     return EFS_OK;
 }
@@ -1551,6 +1589,7 @@ int efs_cfg_nf2ff(efs_ctx* c, const uint32_t* points, uint32_t P, const double* 
     uint64_t el_total = 0;
     Nf2ffGeom g{};
     g.nx = c->nx; g.ny = c->ny; g.nz = c->nz; g.P = P;
+    g.nzs = c->nzp;
     g.S = c->S;
     const uint64_t rowsE = (3 * (uint64_t)P + kFoldRows - 1) / kFoldRows * kFoldRows;
     if (2 * rowsE >= (1ull << 31)) return fail(c, EFS_ERR_ARG, "too many NF2FF points");
@@ -1673,11 +1712,11 @@ int efs_energy(efs_ctx* c, double* out) {
     const uint32_t blocks = std::max(1u, std::min((uint32_t)kEnergyMaxBlocks, (rows + 3) / 4));
     if (c->half)
         hipLaunchKernelGGL(k_energy_partial<f16>, dim3(blocks), dim3(kEnergyThreads), 0, c->stream,
-                           (const f16*)c->volt, (const f16*)c->curr, c->nx, c->ny, c->nz, (uint64_t)c->S, c->fscale,
-                           c->en_partial);
+                           (const f16*)c->volt, (const f16*)c->curr, c->nx, c->ny, c->nz, c->nzp, (uint64_t)c->S,
+                           c->fscale, c->en_partial);
     else
         hipLaunchKernelGGL(k_energy_partial<float>, dim3(blocks), dim3(kEnergyThreads), 0, c->stream, c->volt,
-                           c->curr, c->nx, c->ny, c->nz, (uint64_t)c->S, c->fscale, c->en_partial);
+                           c->curr, c->nx, c->ny, c->nz, c->nzp, (uint64_t)c->S, c->fscale, c->en_partial);
     hipLaunchKernelGGL(k_energy_final, dim3(1), dim3(kEnergyThreads), 0, c->stream, c->en_partial, blocks,
                        c->en_partial + kEnergyMaxBlocks);
     CHECK_LAUNCH(c);

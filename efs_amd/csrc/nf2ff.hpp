@@ -27,7 +27,7 @@ struct Nf2ffGeom {
     uint32_t nx, ny, nz, P;
     uint32_t el_off[6];  // offsets of primal x,y,z / dual x,y,z edge lengths
     uint32_t rowsE;      // first H row
-    uint32_t pad;
+    uint32_t nzs;        // device row length (Nz padded to a multiple of 4)
     uint64_t S;          // component stride of the field arrays
 };
 
@@ -93,7 +93,7 @@ __global__ void k_nf2ff_sample(FieldView volt, FieldView curr,
         };
         nf_point(ix, iy, iz, el, g, V, I, o);
     } else {
-        const size_t Ny = g.ny, Nz = g.nz;
+        const size_t Ny = g.ny, Nz = g.nzs;
         const size_t b = ix * Ny * Nz + iy * Nz + iz;
         const size_t sx = Ny * Nz, sy = Nz;
         const size_t step[3] = {sx, sy, 1};

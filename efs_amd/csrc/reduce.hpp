@@ -29,7 +29,8 @@ template <typename T>
 __global__ __launch_bounds__(kEnergyThreads) void k_energy_partial(const T* __restrict__ volt,
                                                                    const T* __restrict__ curr,
                                                                    uint32_t nx, uint32_t ny, uint32_t nz,
-                                                                   uint64_t S, const float* __restrict__ fscale,
+                                                                   uint32_t nzs, uint64_t S,
+                                                                   const float* __restrict__ fscale,
                                                                    double* __restrict__ partial) {
     __shared__ double s_w[kEnergyThreads / 64];
     const uint32_t lane = threadIdx.x & 63u, wv = threadIdx.x >> 6;
@@ -39,7 +40,7 @@ __global__ __launch_bounds__(kEnergyThreads) void k_energy_partial(const T* __re
     double acc = 0.0;
     for (uint32_t row = blockIdx.x * (kEnergyThreads / 64) + wv; row < rows; row += nwaves) {
         const uint32_t x = row / (ny - 1), y = row - x * (ny - 1);
-        const size_t base = ((size_t)x * ny + y) * nz;
+        const size_t base = ((size_t)x * ny + y) * nzs;  // nzs: device row length
         float s = 0.f;
         for (uint32_t z = lane; z + 1 < nz; z += 64) {
 #pragma unroll

@@ -370,26 +370,42 @@ def mur_config(mesh, faces, dT):
     return {k: (np.concatenate(v).astype(np.float32 if k.startswith("coeff") else np.uint32)) for k, v in out.items()}
 
 
+def _face_edge_lengths(lines):
+    """Midpoint-rule lengths of the lines of one face: half the distance to
+    each neighbour inside the face (app/sim-worker.js:626-639)."""
+    n = len(lines)
+    out = np.zeros(n)
+    if n > 1:
+        out[0] = 0.5 * abs(lines[1] - lines[0])
+        out[1:-1] = 0.5 * np.abs(lines[2:] - lines[:-2])
+        out[-1] = 0.5 * abs(lines[-1] - lines[-2])
+    return out
+
+
 def nf2ff_box(mesh, lo, hi):
-    """Six-face NF2FF surface at node index bounds lo..hi (inclusive): point
-    triplets, point metadata (pos xyz, normalDir, normSign, area, 0, 0) and
-    the primal/dual edge-length tables."""
+    """Six-face NF2FF surface at node index bounds lo..hi (inclusive), as the
+    reference's worker builds it (app/sim-worker.js:429-495, 563-624): faces
+    in the order x-, x+, y-, y+, z-, z+, points of a face in x, y, z loop
+    order; point metadata (pos xyz, normalDir, normSign, area, 0, 0) with the
+    area from the face's own midpoint-rule edge lengths; and the engine's
+    primal/dual edge-length tables (app/sim-worker.js:772-785)."""
     pts, meta = [], []
     for ax in range(3):
-        a1, a2 = (ax + 1) % 3, (ax + 2) % 3
         for side, sign in ((lo[ax], -1.0), (hi[ax], 1.0)):
-            I, J = np.meshgrid(np.arange(lo[a1], hi[a1] + 1), np.arange(lo[a2], hi[a2] + 1), indexing="ij")
-            p = np.zeros((I.size, 3), dtype=np.int64)
-            p[:, ax] = side
-            p[:, a1] = I.ravel()
-            p[:, a2] = J.ravel()
+            rng = [np.arange(lo[d], hi[d] + 1) for d in range(3)]
+            rng[ax] = np.array([side])
+            X, Y, Z = np.meshgrid(*rng, indexing="ij")
+            p = np.stack([X.ravel(), Y.ravel(), Z.ravel()], axis=1).astype(np.int64)
             pts.append(p)
-            m = np.zeros((I.size, 8))
+            m = np.zeros((len(p), 8))
             for d in range(3):
                 m[:, d] = mesh.lines[d][p[:, d]]
+            a1, a2 = (ax + 1) % 3, (ax + 2) % 3
+            e1 = _face_edge_lengths(mesh.lines[a1][rng[a1]])
+            e2 = _face_edge_lengths(mesh.lines[a2][rng[a2]])
             m[:, 3] = ax
             m[:, 4] = sign
-            m[:, 5] = mesh.dual[a1][p[:, a1]] * mesh.dual[a2][p[:, a2]]
+            m[:, 5] = e1[p[:, a1] - rng[a1][0]] * e2[p[:, a2] - rng[a2][0]]
             meta.append(m)
     pts = np.concatenate(pts).astype(np.uint32)
     meta = np.concatenate(meta).astype(np.float32)

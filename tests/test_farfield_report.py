@@ -1,0 +1,91 @@
+"""Far-field reporting (efs_amd/farfield.py) and the NF2FF surface builder.
+
+Pins:
+- rad_power against the textbook Poynting flux 0.5 Re(E x H*) . n A,
+  computed with numpy cross products, for every normal direction and sign;
+- directivity of a Hertzian dipole: the oracle's far field of a single
+  z-directed current element gives Dmax = 1.5 (1.76 dBi) with the power
+  from integrating U over the sphere (intensity_power);
+- the cuts read the dBi map at phi = 0 / 90 degrees (and phi + 180 for
+  negative theta), exactly on grid angles;
+- the surface builder follows the reference's point order and face area
+  rule: the areas of each face sum to the face's area, and the reference's
+  2-degree sphere grid is 91 x 181 (the patch case's far-field grid).
+"""
+import numpy as np
+
+from efs_amd import farfield, operator as op
+
+
+def test_rad_power_is_the_poynting_flux():
+    rng = np.random.default_rng(4)
+    P = 60
+    meta = np.zeros((P, 8))
+    meta[:, 3] = rng.integers(0, 3, P)
+    meta[:, 4] = rng.choice([-1.0, 1.0], P)
+    meta[:, 5] = rng.uniform(1e-6, 3e-6, P)
+    E = rng.normal(size=(P, 3)) + 1j * rng.normal(size=(P, 3))
+    H = rng.normal(size=(P, 3)) + 1j * rng.normal(size=(P, 3))
+    accE = np.stack([E.real, E.imag], axis=2).ravel()
+    accH = np.stack([H.real, H.imag], axis=2).ravel()
+    got = farfield.rad_power(accE, accH, meta.ravel(), P)
+    nhat = np.zeros((P, 3))
+    nhat[np.arange(P), meta[:, 3].astype(int)] = meta[:, 4]
+    want = np.sum(0.5 * np.real(np.sum(np.cross(E, np.conj(H)) * nhat, axis=1)) * meta[:, 5])
+    assert abs(got - want) <= 1e-12 * np.sum(np.abs(want))
+
+
+def test_hertzian_dipole_directivity(oracle_mod):
+    meta = np.array([0, 0, 0, 1, 1.0, 1e-6, 0, 0], float)  # normal +y, J = n x H along z
+    accE = np.zeros(6)
+    accH = np.zeros(6)
+    accH[0] = -1.0
+    theta, phi = farfield.sphere_grid(1, 2)
+    ff = oracle_mod.farfield(accE, accH, meta, theta, phi, np.zeros(3), 1e9, 1.0)
+    p = farfield.intensity_power(ff, theta, phi)
+    d = farfield.directivity(ff, len(theta), len(phi), p)
+    assert abs(d["Dmax"] - 1.5) < 2e-3
+    assert abs(d["DmaxdBi"] - 10 * np.log10(1.5)) < 6e-3
+
+
+def test_cuts_read_the_map():
+    theta, phi = farfield.sphere_grid()
+    nT, nP = len(theta), len(phi)
+    rng = np.random.default_rng(8)
+    e = rng.uniform(0.1, 1.0, nT * nP)
+    d = 5.0
+    tdeg, xz, yz = farfield.cuts(e, d, nT, nP)
+    emax = e.max()
+    dmap = 20 * np.log10(e / emax) + d
+    for k, t in enumerate(tdeg):
+        i = abs(t) // 2
+        if t >= 0:
+            assert xz[k] == dmap[i * nP + 0] and yz[k] == dmap[i * nP + 45]
+        else:
+            assert xz[k] == dmap[i * nP + 90] and yz[k] == dmap[i * nP + 135]
+
+
+def test_surface_builder_order_and_areas():
+    mesh = op.Mesh(np.cumsum(np.r_[0, np.linspace(1, 2, 11)]) * 1e-3, np.arange(9) * 1.5e-3,
+                   np.cumsum(np.r_[0, np.linspace(2, 1, 10)]) * 1e-3)
+    lo, hi = [2, 1, 3], [9, 6, 8]
+    pts, meta, _, _ = op.nf2ff_box(mesh, lo, hi)
+    pts = pts.reshape(-1, 3)
+    meta = meta.reshape(-1, 8)
+    k = 0
+    for ax in range(3):
+        a1, a2 = (ax + 1) % 3, (ax + 2) % 3
+        for side in (lo[ax], hi[ax]):
+            n = (hi[a1] - lo[a1] + 1) * (hi[a2] - lo[a2] + 1)
+            face = pts[k:k + n]
+            assert np.all(face[:, ax] == side)
+            # x, y, z loop order: the linear index increases along the face
+            lin = (face[:, 0] * 100 + face[:, 1]) * 100 + face[:, 2]
+            assert np.all(np.diff(lin) > 0)
+            span1 = mesh.lines[a1][hi[a1]] - mesh.lines[a1][lo[a1]]
+            span2 = mesh.lines[a2][hi[a2]] - mesh.lines[a2][lo[a2]]
+            assert abs(meta[k:k + n, 5].astype(np.float64).sum() - span1 * span2) <= 1e-6 * span1 * span2
+            k += n
+    assert k == len(pts)
+    theta, phi = farfield.sphere_grid()
+    assert (len(theta), len(phi)) == (91, 181)

@@ -147,6 +147,31 @@ def test_compressed_coefficients(HIPEngine, oracle_mod, monkeypatch, name, seed,
     assert_fields_close(fg, o.getFields(), what=name)
 
 
+@pytest.mark.parametrize("name,seed,dims,kw", [
+    ("odd_nz_extensions", 47, (23, 19, 29), dict(pml_width=0, mur=True, tfsf=True, ade=True, rlc=True)),
+    ("odd_nz_upml", 48, (21, 18, 30), dict(pml_width=3, mur=True)),
+], ids=["odd_nz_extensions", "odd_nz_upml"])
+def test_padded_rows_match_unpadded(HIPEngine, monkeypatch, name, seed, dims, kw):
+    """Nz % 4 != 0: device rows padded to a multiple of 4 (4-wide sweeps, or
+    1-wide on the padded rows when the UPML is not aligned) give the bits
+    of the unpadded 1-wide sweeps (EFS_VEC=1): same per-cell arithmetic."""
+    case = random_case(seed, dims, **kw)
+    say("case padded", name, dims)
+    a = make_gpu(HIPEngine, case)
+    monkeypatch.setenv("EFS_VEC", "1")
+    b = make_gpu(HIPEngine, case)
+    monkeypatch.delenv("EFS_VEC")
+    if kw["pml_width"] == 0:
+        assert "vec4" in a.pathFlags()
+    assert "vec4" not in b.pathFlags()
+    a.iterate(30)
+    b.iterate(30)
+    fa, fb = a.getFields(), b.getFields()
+    np.testing.assert_array_equal(fa["volt"], fb["volt"])
+    np.testing.assert_array_equal(fa["curr"], fb["curr"])
+    assert a.energy() == b.energy()
+
+
 def test_split_matches_unsplit(HIPEngine, monkeypatch):
     """Interior + UPML launches vs the one-cell-per-lane fused sweeps on the
     same aligned case: same arithmetic per cell, so identical bits."""

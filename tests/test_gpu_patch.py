@@ -69,7 +69,7 @@ def test_patch_matches_oracle(patch, oracle_mod):
     s_o = ports.port_s11(to[0], to[1], freqs)
     assert np.max(np.abs(s_g - s_o)) <= 1e-4 * np.max(np.abs(s_o))
     flags = g.pathFlags()
-    assert "vec4" not in flags  # Nz = 71: the 1-wide sweeps
+    assert "vec4" in flags  # Nz = 71 runs the 4-wide sweeps on rows padded to 72
     g.destroy()
 
 
@@ -77,15 +77,32 @@ def test_patch_reference_checks(patch):
     """The reference's own checks of this antenna (examples/
     patch_antenna_test.mjs:56-90) on the full 30,000-step HIP run: S11
     below -10 dB somewhere in 1-3 GHz, the resonance (deepest S11) between
-    1.8 and 2.5 GHz, and Re(Zin) > 0 there."""
-    from efs_amd import engine, ports, workloads
+    1.8 and 2.5 GHz, and Re(Zin) > 0 there.  Then the directivity at
+    2.4 GHz: the surface Poynting power and the far-field intensity agree,
+    and Dmax lies within 1 dB of the native 6.6 dBi the reference reports."""
+    from efs_amd import engine, farfield, ports, workloads
     engine.load_library()
     steps = patch["nTS"]
     say("patch full run", patch["gridSize"], steps)
     g = engine.HIPEngine()
     assert g.initGPU()
-    slices = workloads.configure_patch(g, patch, nf2ff=False, max_samples=steps)
+    slices = workloads.configure_patch(g, patch, nf2ff=True, max_samples=steps)
     g.iterate(steps)
+    # directivity at the NF2FF frequency (2.4 GHz) on the reference's grid
+    theta, phi = farfield.sphere_grid()
+    ff = g.computeNF2FFfarField({"pointMeta": patch["pointMeta"], "theta": theta, "phi": phi,
+                                 "center": [0, 0, 0], "frequency": float(patch["freqs"][0]), "radius": 1.0})
+    acc = g.readNF2FFAccumulation()
+    P = patch["nf2ff"]["numPoints"]
+    p_surf = farfield.rad_power(acc["accumE"][0], acc["accumH"][0], patch["pointMeta"], P)
+    p_sphere = farfield.intensity_power(ff, theta, phi)
+    d = farfield.directivity(ff, len(theta), len(phi), p_surf)
+    say("Dmax", d["DmaxdBi"], "dBi; P_surface", p_surf, "P_sphere", p_sphere)
+    # the surface flux and the far-field intensity carry the same power
+    assert abs(p_surf - p_sphere) <= 0.05 * p_surf
+    # the reference's native openEMS result for this antenna is 6.6 dBi
+    # (STATUS.md:100); the mesh timestep and operator details differ here
+    assert 5.6 <= d["DmaxdBi"] <= 7.6
     r = g.readProbeGatherBuffered()
     tr = ports.probe_traces(r["data"], r["numSamples"], r["stride"], patch["probes"], slices, 1, patch["dT"])
     f = np.linspace(1e9, 3e9, 401)
