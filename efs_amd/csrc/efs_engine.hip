@@ -127,6 +127,10 @@ struct efs_ctx {
     std::vector<float> h_ade_c[6];  // v_int v_ext v_lor i_int i_ext i_lor
     bool ade_apply = false;
     bool ade_dirty = false;
+    bool ade_i_live = false;  // some current-side ADE coefficient is non-zero
+    std::vector<uint32_t> h_ti_idx;  // TFSF current points (interface indices)
+    bool cap_clash = false;          // an H-side hook writes inside the NF2FF capture rows
+    bool cap_dirty = true;
     uint32_t ade_n = 0;
     uint32_t *ade_gidx = nullptr, *ade_src_v = nullptr, *ade_src_i = nullptr;
     uint8_t* ade_lor = nullptr;
@@ -426,6 +430,11 @@ int rebuild_ade(efs_ctx* c) {
         TRY(dupload(c, c->ade_gidx, g.data(), c->ade_n));
         TRY(dupload(c, c->ade_lor, c->h_ade_lor.data(), c->ade_n));
         for (int k = 0; k < 6; ++k) TRY(dupload(c, c->ade_c[k], c->h_ade_c[k].data(), c->ade_n));
+        // all-zero current-side coefficients keep the state at 0 (A = 0 A + 0 V)
+        // and the write-back subtracts 0: both passes are skipped
+        c->ade_i_live = false;
+        for (int k = 3; k < 6; ++k)
+            for (float v : c->h_ade_c[k]) c->ade_i_live |= v != 0.0f;
         TRY(dzero(c, c->ade_va, c->ade_n));
         TRY(dzero(c, c->ade_vl, c->ade_n));
         TRY(dzero(c, c->ade_ia, c->ade_n));
@@ -547,6 +556,8 @@ void reset_extensions(efs_ctx* c) {
     c->ade_dirty = false;
     c->alias_dirty = false;
     c->tfsf_nv = c->tfsf_ni = 0;
+    c->h_ti_idx.clear();
+    c->cap_dirty = true;
     c->rlc_n = 0;
     c->g_n = 0;
     c->gb_n = 0;
@@ -575,8 +586,10 @@ bool box_overlap(const PmlRegion& a, const PmlRegion& b) {
 // (60 B, or 27 B byte-indexed); in the interior launch of the split form it
 // costs only the loop.  Blocks that contain UPML cells are flagged (they
 // stage the UPML coefficient tables).
-// The launch gets ~2048 workgroups (8 per CU on 256 CUs) but chunks of 8
-// planes on average; heavy tiles go down to 2-plane chunks.
+// The launch gets ~2048 workgroups (8 per CU on 256 CUs) but no chunk
+// shorter than 2 planes (a chunk re-reads one halo plane): a 256^3 grid
+// gets chunks of 8 planes on average, a grid of a few hundred thousand
+// cells 2-plane chunks (there the march is latency-bound, not HBM-bound).
 int plan_sweeps(efs_ctx* c) {
     const uint32_t nx = c->nx, ny = c->ny, nz = c->nzp;  // device rows
     const uint64_t cols = (uint64_t)ny * nz;
@@ -668,13 +681,37 @@ FieldView fv(const efs_ctx* c, int curr) {
                      c->half ? 1u : 0u};
 }
 
+// Whether a hook that changes H after the H sweep writes inside the
+// captured rows (z planes zf-1 .. zf+2 of the capture rectangle): TFSF
+// current points, or ADE write-back cells with live current coefficients
+// (taken as live while the ADE lists await their rebuild).
+bool hooks_clash(const efs_ctx* c) {
+    const NfCapture& k = c->nf_cap;
+    if (!k.on) return false;
+    const uint32_t N = (uint32_t)c->N, nz = c->nz, ny = c->ny;
+    auto hit = [&](uint32_t gi) {
+        const uint32_t j = gi % N, z = j % nz, y = (j / nz) % ny, x = j / (nz * ny);
+        if (x < k.x0 || x > k.x1 || y < k.y0 || y > k.y1) return false;
+        for (uint32_t f = 0; f < k.nf; ++f)
+            if (z + 1 >= k.z[f] && z <= k.z[f] + 2) return true;
+        return false;
+    };
+    for (uint32_t g : c->h_ti_idx)
+        if (hit(g)) return true;
+    if (c->ade_apply && (c->ade_i_live || c->ade_dirty))
+        for (uint32_t g : c->h_ade_gidx)
+            if (hit(g)) return true;
+    return false;
+}
+
 // The NF2FF capture for this step, or an inactive one: it needs the 4-wide
-// grid sweeps, no UPML cell in the captured rows, and no hook that changes
-// H after the H sweep (TFSF-I, PBC-I, ADE write-back).
+// grid sweeps, no UPML cell in the captured rows, no periodic H faces, and
+// no other hook that changes H after the H sweep inside the captured rows
+// (TFSF-I, ADE write-back).
 NfCapture nf_capture(const efs_ctx* c) {
     const NfCapture& k = c->nf_cap;
     if (!k.on || !c->nf_P || !c->vec4 || (c->pml.n && !c->pml_split)) return NfCapture{};
-    if (c->tfsf_ni || !c->pbc_h.empty() || (c->ade_n && c->ade_apply)) return NfCapture{};
+    if (!c->pbc_h.empty() || (c->cap_dirty ? hooks_clash(c) : c->cap_clash)) return NfCapture{};
     for (uint32_t r = 0; r < c->pml.n; ++r) {
         const PmlRegion& R = c->pml.r[r];
         if (R.x0 > k.x1 || R.x1 <= k.x0 || R.y0 > k.y1 || R.y1 <= k.y0) continue;
@@ -718,6 +755,7 @@ int plan_nf_capture(efs_ctx* c, const std::vector<uint32_t>& pts, uint32_t P) {
     TRY(dzero(c, c->nf_slab, (size_t)k.nf * k.nx_c * k.ny_c * kCapFloats));
     k.slab = c->nf_slab;
     c->nf_cap = k;
+    c->cap_dirty = true;
     return EFS_OK;
 }
 
@@ -818,7 +856,7 @@ int launch_step(efs_ctx* c) {
                            c->rlc_gidx, c->rlc_type, c->rlc_c, c->rlc_vdn, c->rlc_jn, c->rlc_vil, c->rlc_n);
     }
     // pre-current
-    if (c->ade_n) {
+    if (c->ade_n && c->ade_i_live) {
         hipLaunchKernelGGL(k_ade_update, dim3(nblk(c->ade_n)), dim3(kPointThreads), 0, c->stream, fv(c, 1),
                            c->pml_cflux, c->ade_src_i, c->ade_lor, c->ade_c[3], c->ade_c[4], c->ade_c[5],
                            c->ade_ia, c->ade_il, c->ade_n);
@@ -860,7 +898,7 @@ int launch_step(efs_ctx* c) {
     for (const PbcAxis& a : c->pbc_h)  // after TFSF-I (src/webgpu-engine.mjs:2477)
         hipLaunchKernelGGL(k_pbc, dim3(nblk((size_t)a.b0 * a.b1)), dim3(kPointThreads), 0, c->stream, fv(c, 1),
                            c->pbc_imag, a, c->ny, c->nzp, (uint64_t)c->S);
-    if (c->ade_n && c->ade_apply) {
+    if (c->ade_n && c->ade_apply && c->ade_i_live) {
         hipLaunchKernelGGL(k_ade_apply, dim3(nblk(c->ade_n)), dim3(kPointThreads), 0, c->stream, fv(c, 1),
                            c->ade_gidx, c->ade_ia, c->ade_n);
     }
@@ -1255,6 +1293,8 @@ int efs_cfg_ade_clear(efs_ctx* c) {
     for (auto& v : c->h_ade_c) v.clear();
     c->ade_n = 0;
     c->ade_dirty = false;
+    c->alias_dirty = true;
+    c->cap_dirty = true;
     return EFS_OK;
 }
 
@@ -1274,6 +1314,7 @@ int efs_cfg_ade_add(efs_ctx* c, uint32_t has_lorentz, uint32_t dir, uint32_t nce
         for (int a = 0; a < 6; ++a) c->h_ade_c[a].push_back(src[a] ? src[a][k] : 0.f);
     }
     c->ade_dirty = true;
+    c->cap_dirty = true;
     return EFS_OK;
 }
 
@@ -1281,6 +1322,7 @@ int efs_set_ade_apply(efs_ctx* c, int apply) {
     TRY(need_init(c));
     c->ade_apply = apply != 0;
     c->alias_dirty = true;
+    c->cap_dirty = true;
     return EFS_OK;
 }
 
@@ -1317,6 +1359,8 @@ int efs_cfg_tfsf(efs_ctx* c, const float* signal, uint32_t len, uint32_t period,
     if (period && (nv || ni)) c->source_periodic = true;
     c->tfsf_nv = nv;
     c->tfsf_ni = ni;
+    c->h_ti_idx.assign(i_idx, i_idx + ni);
+    c->cap_dirty = true;
     if (nv || ni) TRY(source_scales(c, signal, len, v_amp, nv, i_amp, ni));
     return EFS_OK;
 }
@@ -1421,6 +1465,10 @@ int efs_iterate(efs_ctx* c, uint32_t n) {
     HIP_TRY(c, hipSetDevice(c->device));
     if (c->ade_dirty) TRY(rebuild_ade(c));
     if (c->alias_dirty) TRY(rebuild_aliases(c));
+    if (c->cap_dirty) {
+        c->cap_clash = hooks_clash(c);
+        c->cap_dirty = false;
+    }
     if (n) c->fields_zero = false;
     for (uint32_t s = 0; s < n; ++s) TRY(launch_step(c));
     return EFS_OK;

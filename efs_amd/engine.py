@@ -315,16 +315,22 @@ class HIPEngine:
         """configureTFSF (src/webgpu-engine.mjs:1527-1645)."""
         self._need_device()
         sig = f32(config["signal"])
-        vp = config.get("voltagePoints") or []
-        cp = config.get("currentPoints") or []
+        # a list of point records, or a dict of columns of the same keys
+        vp = config.get("voltagePoints")
+        cp = config.get("currentPoints")
 
         def cols(pts):
-            if not pts:
-                return u32([0]), u32([0]), f32([0]), f32([0])
-            return (u32([p["field_idx"] for p in pts]), u32([p["delay_int"] for p in pts]),
-                    f32([p["delay_frac"] for p in pts]), f32([p["amp"] for p in pts]))
-        self._check(self.lib.efs_cfg_tfsf(self.ctx, sig, len(sig), int(config.get("period", 0) or 0), len(vp),
-                                          *cols(vp), len(cp), *cols(cp)))
+            if pts is None or len(pts) == 0:
+                return 0, (u32([0]), u32([0]), f32([0]), f32([0]))
+            if isinstance(pts, dict):
+                return len(pts["field_idx"]), (u32(pts["field_idx"]), u32(pts["delay_int"]),
+                                               f32(pts["delay_frac"]), f32(pts["amp"]))
+            return len(pts), (u32([p["field_idx"] for p in pts]), u32([p["delay_int"] for p in pts]),
+                              f32([p["delay_frac"] for p in pts]), f32([p["amp"] for p in pts]))
+        nv, v = cols(vp)
+        ni, i = cols(cp)
+        self._check(self.lib.efs_cfg_tfsf(self.ctx, sig, len(sig), int(config.get("period", 0) or 0), nv, *v,
+                                          ni, *i))
 
     RLC_KEYS = ("i2v", "ilv", "vvd", "vv2", "vj1", "vj2", "ib0", "b1", "b2")
 

@@ -14,6 +14,10 @@ Restates the reference worker's reporting step:
                  grid (app/sim-worker.js:1020-1040)
 
 `sphere_grid` makes the reference's 2-degree theta/phi grid.
+
+For scatterers under a plane wave (no counterpart in the reference's
+worker): `incident_spectrum` and `rcs` turn the far field of the
+scattered-field NF2FF surface into a bistatic radar cross section.
 """
 import numpy as np
 
@@ -81,3 +85,22 @@ def intensity_power(ff, theta, phi):
     U = np.asarray(ff, np.float64).reshape(len(theta), len(phi), 5)[:, :, 0]
     g = U * np.sin(theta)[:, None]
     return float(np.trapezoid(np.trapezoid(g, phi, axis=1), theta))
+
+
+def incident_spectrum(signal, omega, dT, e0=1.0):
+    """|E_inc| at each angular frequency with the NF2FF accumulation's DFT
+    (sum over steps of field x exp(-j w t) dT): the plane wave at any point
+    is the source signal delayed, which changes only the phase."""
+    s = np.asarray(signal, np.float64)
+    k = np.arange(len(s))
+    om = np.atleast_1d(np.asarray(omega, np.float64))
+    return np.abs(np.exp(-1j * np.outer(om, k) * dT) @ s) * dT * abs(e0)
+
+
+def rcs(ff, e_inc):
+    """Bistatic radar cross section (m^2) per angle from a far-field result
+    (5 values per angle, U first, radius 1) and the incident |E| at the same
+    frequency: sigma = 4 pi U / (|E_inc|^2 / (2 Z0))."""
+    U = np.asarray(ff, np.float64).reshape(-1, 5)[:, 0]
+    z0 = 376.73031346177066
+    return 4 * np.pi * U / (e_inc * e_inc / (2 * z0))

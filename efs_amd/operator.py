@@ -56,11 +56,14 @@ class Mesh:
     def shape(self):
         return tuple(self.n)
 
-    def timestep(self, factor=1.0):
-        """CFL limit of the smallest cell, times ``factor``."""
+    def timestep(self, factor=1.0, wp_max=0.0):
+        """CFL limit of the smallest cell, times ``factor``.  With Drude or
+        Lorentz poles (largest plasma angular frequency wp_max) the
+        leapfrog update of field and polarisation current is stable when
+        (dT / dT_cfl)^2 + (wp_max dT / 2)^2 <= 1, which this limit meets."""
         m = [np.min(d[:-1]) if len(d) > 1 else np.inf for d in self.primal]
         s = sum(1.0 / v ** 2 for v in m if np.isfinite(v))
-        return factor / (C0 * np.sqrt(s))
+        return factor / np.sqrt(C0 * C0 * s + wp_max * wp_max / 4)
 
 
 def uniform_mesh(n, delta):
@@ -415,6 +418,198 @@ def nf2ff_box(mesh, lo, hi):
             primal[d][-1] = primal[d][-2]
     dual = [mesh.dual[d].astype(np.float32) for d in range(3)]
     return pts.ravel(), meta.ravel(), primal, dual
+
+
+def _edge_share(mesh, cellv, n):
+    """Sum over the (up to 4) cells around each E edge along n of a cell
+    quantity times the cell's share of the edge's dual face -- the weights
+    build_coefficients uses for eps (cells outside the grid count 0)."""
+    N = mesh.shape
+    P = _pad_cells(np.asarray(cellv, np.float64))
+    nP, nPP = (n + 1) % 3, (n + 2) % 3
+    out = np.zeros(N)
+    for sP, wP in ((-1, mesh.half_lo[nP]), (0, mesh.half_hi[nP])):
+        for sPP, wPP in ((-1, mesh.half_lo[nPP]), (0, mesh.half_hi[nPP])):
+            sl = [None, None, None]
+            sl[n] = slice(1, N[n] + 1)
+            sl[nP] = slice(1 + sP, 1 + sP + N[nP])
+            sl[nPP] = slice(1 + sPP, 1 + sPP + N[nPP])
+            out += P[tuple(sl)] * (_bcast(wP, nP) * _bcast(wPP, nPP))
+    return out
+
+
+def ade_orders(mesh, coeffs, dT, poles):
+    """Drude / Lorentz poles as ADE orders in the configureLorentzADE format
+    (src/webgpu-engine.mjs:1358-1475), for the openEMS write-back form
+    (field -= ADE state after each update).
+
+    poles: list of {"mask": bool over the cells, "wp": plasma angular
+    frequency, "w0": pole angular frequency (0 for Drude), "gamma": damping
+    (1/s)}; eps_inf is the cells' eps in the main coefficients.
+
+    Model (per E edge, I_p the polarisation current through the dual face,
+    s = sum over the touching cells of eps0 wp^2 x dual-face share):
+        dI_p/dt = -gamma I_p - w0^2 Q + s E,   dQ/dt = I_p
+    with the central-difference ADE of Taflove & Hagness (ch. 9):
+        I_p^{n+1/2} = k I_p^{n-1/2} + dT s E^n / (1 + gamma dT/2) - (Lorentz term)
+        k = (1 - gamma dT/2) / (1 + gamma dT/2).
+    In the engine's units the state is A = vi I_p (volts, what the write-back
+    subtracts from the edge voltage V = E dl) and, for Lorentz, L = w0^2 Q dl / s:
+        L   += v_Lor A,        v_Lor = dT w0^2 dl / (s vi)
+        A    = v_int A + v_ext (V - L)   (Drude: L = 0)
+        v_int = k,   v_ext = vi dT s / (dl (1 + gamma dT / 2)).
+    Each order lists the edges any direction couples to (shared positions,
+    zero coefficients where a direction has none); the H side is zero."""
+    N = mesh.shape
+    orders = []
+    if poles and dT > mesh.timestep(wp_max=max(float(p["wp"]) for p in poles)) * (1 + 1e-12):
+        raise ValueError("dT exceeds the stability limit with these poles (Mesh.timestep(wp_max=...))")
+    for pole in poles:
+        wp, w0, gam = float(pole["wp"]), float(pole.get("w0", 0.0) or 0.0), float(pole.get("gamma", 0.0) or 0.0)
+        cellv = np.where(np.asarray(pole["mask"], bool), EPS0 * wp * wp, 0.0)
+        k = (1 - gam * dT / 2) / (1 + gam * dT / 2)
+        per = []
+        live = np.zeros(N, bool)
+        for n in range(3):
+            s = _edge_share(mesh, cellv, n)
+            vi = coeffs["vi"][n].reshape(N)
+            dl = _bcast(mesh.primal[n], n)
+            ok = (s > 0) & (vi != 0) & (dl > 0)
+            per.append((s, vi, np.broadcast_to(dl, N), ok))
+            live |= ok
+        pos = np.flatnonzero(live.ravel())
+        dirs = []
+        for n, (s, vi, dl, ok) in enumerate(per):
+            s_, vi_, dl_, ok_ = (a.ravel()[pos] for a in (s, vi, dl, ok))
+            with np.errstate(divide="ignore", invalid="ignore"):
+                v_ext = np.where(ok_, vi_ * dT * s_ / (dl_ * (1 + gam * dT / 2)), 0.0)
+                v_lor = np.where(ok_, dT * w0 * w0 * dl_ / (s_ * vi_), 0.0)
+            d = {"dir": n, "pos_idx": pos.astype(np.uint32),
+                 "v_int_ADE": np.where(ok_, k, 0.0).astype(np.float32),
+                 "v_ext_ADE": v_ext.astype(np.float32)}
+            if w0 > 0:
+                d["v_Lor_ADE"] = v_lor.astype(np.float32)
+            dirs.append(d)
+        orders.append({"numCells": int(len(pos)), "hasLorentz": w0 > 0, "directions": dirs})
+    return {"orders": orders}
+
+
+def _tf_mask(mesh, lo, hi, comp, kind):
+    """Total-field membership of component comp of E ("e") or H ("h") on the
+    grid: its position (an edge midpoint along comp for E, across comp for
+    H, in doubled index units) lies in the closed box [lo, hi] of nodes."""
+    axes = []
+    for a in range(3):
+        i = np.arange(mesh.n[a])
+        half = (a == comp) if kind == "e" else (a != comp)
+        c2 = 2 * i + (1 if half else 0)
+        axes.append((c2 >= 2 * lo[a]) & (c2 <= 2 * hi[a]))
+    return axes[0][:, None, None] & axes[1][None, :, None] & axes[2][None, None, :]
+
+
+def tfsf_plane_wave(mesh, coeffs, lo, hi, dT, signal, prop_axis=0, prop_sign=1, e_axis=2, e0=1.0):
+    """Total-field / scattered-field plane wave in the configureTFSF format
+    (src/webgpu-engine.mjs:1527-1645; injection F[idx] += amp * s(numTS -
+    delay_int + delay_frac), src/webgpu-fdtd.mjs:683-739).
+
+    The wave travels along prop_sign * axis prop_axis with E along e_axis:
+    E_inc = e0 s(t - tau) e_hat, H_inc = (k_hat x e_hat) e0 / Z0 s(t - tau),
+    tau = distance past the entry face / c.  The total-field region is the
+    closed node box [lo, hi]; every update whose stencil mixes a total-field
+    and a scattered-field value gets one point: the neighbour's incident
+    value (E dl on a primal edge, H dl' on a dual edge) times the update's
+    curl coefficient and sign, added when the updated value is total and
+    the neighbour scattered, subtracted in the opposite case.  Timing: the
+    voltage update of step n reads currents at (n + 1/2) dT, the current
+    update voltages at (n + 1) dT; delays are shifted by a whole number of
+    steps so that none is negative.  The box must lie in vacuum, outside
+    UPML."""
+    N = mesh.shape
+    ehat = np.zeros(3)
+    ehat[e_axis] = 1.0
+    khat = np.zeros(3)
+    khat[prop_axis] = float(prop_sign)
+    hhat = np.cross(khat, ehat)
+    a = prop_axis
+    ref = mesh.lines[a][lo[a]] if prop_sign > 0 else mesh.lines[a][hi[a]]
+    node = mesh.lines[a]
+    mid = node + mesh.primal[a] / 2
+    tfE = [_tf_mask(mesh, lo, hi, n, "e") for n in range(3)]
+    tfH = [_tf_mask(mesh, lo, hi, n, "h") for n in range(3)]
+    lists = {"v": [], "i": []}
+    Ntot = int(np.prod(N))
+
+    def shifted(arr, axis, off):
+        """arr at p + off e_axis (False / 0 outside the grid)."""
+        out = np.zeros_like(arr)
+        src = [slice(None)] * 3
+        dst = [slice(None)] * 3
+        if off < 0:
+            src[axis], dst[axis] = slice(0, N[axis] + off), slice(-off, None)
+        else:
+            src[axis], dst[axis] = slice(off, None), slice(0, N[axis] - off)
+        out[tuple(dst)] = arr[tuple(src)]
+        return out
+
+    for kind in ("v", "i"):
+        own = tfE if kind == "v" else tfH
+        other = tfH if kind == "v" else tfE
+        ohat = hhat / Z0 if kind == "v" else ehat
+        step = -1 if kind == "v" else 1
+        coef = coeffs["vi"] if kind == "v" else coeffs["iv"]
+        for n in range(3):
+            nP, nPP = (n + 1) % 3, (n + 2) % 3
+            # (neighbour component, offset axis or None, sign) of the update of n
+            terms = ((nPP, None, 1.0), (nPP, nP, -1.0), (nP, None, -1.0), (nP, nPP, 1.0))
+            cn = coef[n].reshape(N)
+            for m, ax, sgn in terms:
+                if ohat[m] == 0.0:
+                    continue
+                nb = other[m] if ax is None else shifted(other[m], ax, step)
+                valid = np.ones(N, bool)
+                if ax is not None:
+                    # at the grid edge the stencil reads the cell itself
+                    edge = [slice(None)] * 3
+                    edge[ax] = 0 if step < 0 else N[ax] - 1
+                    valid[tuple(edge)] = False
+                mix = (own[n] != nb) & valid & (cn != 0)
+                idx = np.argwhere(mix)
+                if len(idx) == 0:
+                    continue
+                side = np.where(own[n][mix], 1.0, -1.0)
+                q = idx.copy()
+                if ax is not None:
+                    q[:, ax] += step
+                # incident value of neighbour component m at q
+                if kind == "v":
+                    length = mesh.dual[m][q[:, m]]
+                    halfa = m != a  # H_m sits at a midpoint along axes other than m
+                else:
+                    length = mesh.primal[m][q[:, m]]
+                    halfa = m == a
+                pos = mid[q[:, a]] if halfa else node[q[:, a]]
+                tau = prop_sign * (pos - ref) / C0
+                inc = e0 * ohat[m] * length
+                amp = cn[mix] * sgn * side * inc
+                lin = n * Ntot + (idx[:, 0] * N[1] + idx[:, 1]) * N[2] + idx[:, 2]
+                D = tau / dT - (0.5 if kind == "v" else 1.0)
+                lists[kind].append((lin, D, amp))
+    Dall = np.concatenate([np.concatenate([t[1] for t in lists[k]]) for k in lists if lists[k]])
+    shift = max(0.0, float(np.ceil(-Dall.min()))) if len(Dall) else 0.0
+    out = {"signal": np.asarray(signal, np.float32), "period": 0}
+    for kind, key in (("v", "voltagePoints"), ("i", "currentPoints")):
+        if not lists[kind]:
+            out[key] = {"field_idx": np.zeros(0, np.uint32), "delay_int": np.zeros(0, np.uint32),
+                        "delay_frac": np.zeros(0, np.float32), "amp": np.zeros(0, np.float32)}
+            continue
+        lin = np.concatenate([t[0] for t in lists[kind]])
+        D = np.concatenate([t[1] for t in lists[kind]]) + shift
+        amp = np.concatenate([t[2] for t in lists[kind]])
+        di = np.ceil(D - 1e-9)
+        out[key] = {"field_idx": lin.astype(np.uint32), "delay_int": di.astype(np.uint32),
+                    "delay_frac": np.clip(di - D, 0.0, 1.0).astype(np.float32), "amp": amp.astype(np.float32)}
+    out["delay_shift"] = shift
+    return out
 
 
 def to_f32(coeffs):

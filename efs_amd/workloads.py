@@ -174,3 +174,92 @@ def add_patch(batch, w):
     idx, slices = ports.build_gather_indices(w["probes"], w["gridSize"])
     batch.configureProbeGather(i, idx)
     return i, slices
+
+
+DISPERSIVE_SEED = 20261203 + 3
+
+
+DRUDE = {"eps_inf": 1.0, "wp": 2 * np.pi * 12e9, "w0": 0.0, "gamma": 1e9}
+LORENTZ = {"eps_inf": 2.0, "wp": 2 * np.pi * 6e9 * np.sqrt(3.0), "w0": 2 * np.pi * 6e9, "gamma": 1e9}
+
+
+def dispersive_scatter(n=192, seed=DISPERSIVE_SEED, pml=8, sphere="drude", slab=True, f0=6e9, fc=5e9,
+                       rcs_freqs=(3e9, 5e9, 7e9, 9e9), n_angles=64, n_ts=15000, k_block=64, radius=None):
+    """Dispersive scatterers under a plane wave: an n^3 grid of 1 mm cells
+    with UPML on all six faces, a Drude sphere (plasma frequency 2 pi 12 GHz,
+    damping 1e9 1/s; sphere="lorentz" makes it of the slab's material, None
+    leaves it out) whose centre and radius come from the seed, next to a
+    single-pole Lorentz slab (eps_inf 2, delta eps 3, pole 6 GHz, damping
+    1e9 1/s; the slab is 1/10 of the grid thick along x), both in ADE form
+    with the openEMS write-back; a total-field / scattered-field plane wave
+    along +x with E along z (Gaussian f0, fc); and an NF2FF box in the
+    scattered-field shell for the bistatic RCS in the xy plane (n_angles
+    angles from phi = 0, the forward direction) at rcs_freqs.
+    Layout along each axis: UPML [0, pml), NF2FF box at pml + 4, total-field
+    box from pml + 8 to n - 1 - (pml + 8).  dT is the CFL limit reduced for
+    the poles (Mesh.timestep(wp_max=...))."""
+    mesh = op.uniform_mesh((n, n, n), 1e-3)
+    mats = ([DRUDE if sphere == "drude" else LORENTZ] if sphere else []) + ([LORENTZ] if slab else [])
+    dT = mesh.timestep(wp_max=max([m["wp"] for m in mats] + [0.0]))
+    mat = op.Materials(mesh)
+    rng = np.random.default_rng(seed)
+    nf_lo, tf_lo = pml + 4, pml + 8
+    tf_hi = n - 1 - tf_lo
+    span = tf_hi - tf_lo
+    cells = np.indices(mat.eps.shape, dtype=np.float64) + 0.5  # cell centres in cell units
+    poles = []
+    info = {}
+    if sphere:
+        r = radius if radius is not None else float(rng.uniform(0.10, 0.14) * span)
+        cx = tf_lo + span * float(rng.uniform(0.28, 0.34))
+        cy, cz = (tf_lo + span * float(rng.uniform(0.45, 0.55)) for _ in range(2))
+        m = (cells[0] - cx) ** 2 + (cells[1] - cy) ** 2 + (cells[2] - cz) ** 2 <= r * r
+        p = DRUDE if sphere == "drude" else LORENTZ
+        mat.eps[m] = p["eps_inf"]
+        poles.append({"mask": m, **{k: p[k] for k in ("wp", "w0", "gamma")}})
+        info["sphere"] = {"centre": [cx, cy, cz], "radius": r, "cells": int(m.sum()), "material": dict(p)}
+    if slab:
+        t = max(2, n // 10)
+        x0 = tf_lo + int(0.62 * span)
+        m = np.zeros(mat.eps.shape, bool)
+        m[x0:x0 + t, tf_lo + span // 5:tf_hi - span // 5, tf_lo + span // 5:tf_hi - span // 5] = True
+        mat.eps[m] = LORENTZ["eps_inf"]
+        poles.append({"mask": m, **{k: LORENTZ[k] for k in ("wp", "w0", "gamma")}})
+        info["slab"] = {"x": [x0, x0 + t], "cells": int(m.sum())}
+    op.pec_boundary(mat)
+    coeffs = op.build_coefficients(mesh, mat, dT)
+    ade = op.ade_orders(mesh, coeffs, dT, poles) if poles else None
+    regions = op.build_upml(mesh, coeffs, [pml] * 6, dT)
+    signal = op.gaussian_pulse(f0, fc, dT)
+    tfsf = op.tfsf_plane_wave(mesh, coeffs, [tf_lo] * 3, [tf_hi] * 3, dT, signal, prop_axis=0, prop_sign=1,
+                              e_axis=2)
+    pts, meta, primal, dual = op.nf2ff_box(mesh, [nf_lo] * 3, [n - 1 - nf_lo] * 3)
+    freqs = np.asarray(rcs_freqs, np.float64)
+    nf2ff = {"surfaceIndices": pts, "numPoints": len(pts) // 3, "omega": list(2 * np.pi * freqs), "dT": dT,
+             "primalEdgeLens": primal, "dualEdgeLens": dual, "kBlock": k_block}
+    phi = 2 * np.pi * np.arange(n_angles) / n_angles
+    n_pml = sum(int(np.prod(r["numLines"])) for r in regions)
+    centre = [float(mesh.lines[d][n // 2]) for d in range(3)]
+    return {"gridSize": [n, n, n], "coefficients": op.to_f32(coeffs), "pmlRegions": regions, "ade": ade,
+            "tfsf": tfsf, "nf2ff": nf2ff, "pointMeta": meta, "freqs": freqs, "dT": dT, "signal": signal,
+            "theta": np.array([np.pi / 2]), "phi": phi, "centre": centre, "n_pml_cells": n_pml, "nTS": n_ts,
+            "mesh": mesh, "seed": seed, "info": info}
+
+
+def configure_dispersive(eng, w, nf2ff=True, storage=None):
+    """Configure an engine with a dispersive_scatter workload (engines with
+    an init method; the CPU oracle takes the coefficients at construction)."""
+    if hasattr(eng, "init"):
+        if storage is None:
+            eng.init(w["gridSize"], w["coefficients"])
+        else:
+            eng.init(w["gridSize"], w["coefficients"], storage=storage)
+    eng.configurePML(w["pmlRegions"])
+    if w["ade"] is not None:
+        if hasattr(eng, "configureLorentzADE"):
+            eng.configureLorentzADE(w["ade"], apply=True)
+        else:
+            eng.configureLorentz(w["ade"], apply=True)
+    eng.configureTFSF(w["tfsf"])
+    if nf2ff:
+        eng.configureNF2FFAccumulation(w["nf2ff"])

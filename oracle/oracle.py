@@ -206,14 +206,21 @@ class Oracle:
 
     def configureTFSF(self, cfg):
         sig = _f64(cfg["signal"])
-        vp = cfg.get("voltagePoints") or []
-        cp = cfg.get("currentPoints") or []
+        # a list of point records, or a dict of columns of the same keys
+        vp = cfg.get("voltagePoints")
+        cp = cfg.get("currentPoints")
 
         def cols(pts):
-            return (_u32([p["field_idx"] for p in pts]), _u32([p["delay_int"] for p in pts]),
-                    _f64([p["delay_frac"] for p in pts]), _f64([p["amp"] for p in pts]))
-        self._f("cfg_tfsf")(self.h, sig, len(sig), int(cfg.get("period", 0) or 0), len(vp), *cols(vp),
-                            len(cp), *cols(cp))
+            if pts is None or len(pts) == 0:
+                return 0, (_u32([0]), _u32([0]), _f64([0]), _f64([0]))
+            if isinstance(pts, dict):
+                return len(pts["field_idx"]), (_u32(pts["field_idx"]), _u32(pts["delay_int"]),
+                                               _f64(pts["delay_frac"]), _f64(pts["amp"]))
+            return len(pts), (_u32([p["field_idx"] for p in pts]), _u32([p["delay_int"] for p in pts]),
+                              _f64([p["delay_frac"] for p in pts]), _f64([p["amp"] for p in pts]))
+        nv, v = cols(vp)
+        ni, i = cols(cp)
+        self._f("cfg_tfsf")(self.h, sig, len(sig), int(cfg.get("period", 0) or 0), nv, *v, ni, *i)
 
     RLC_KEYS = ("i2v", "ilv", "vvd", "vv2", "vj1", "vj2", "ib0", "b1", "b2")
 
