@@ -1718,8 +1718,17 @@ int efs_farfield(efs_ctx* c, uint32_t f, const float* meta, const float* theta, 
     if (!(radius > 0)) return fail(c, EFS_ERR_ARG, "radius must be > 0");
     TRY(nf2ff_fold(c));
     float *d_meta = nullptr, *d_th = nullptr, *d_ph = nullptr, *d_out = nullptr;
+    double* d_part = nullptr;
     int rc = EFS_OK;
     const size_t na = (size_t)nT * nP;
+    if (na > 0xffffffffull / 256) return fail(c, EFS_ERR_ARG, "too many far-field angles");
+    // split the surface into chunks of whole LDS tiles so that the launch
+    // has ~2048 workgroups even for a few angles
+    const uint32_t ablk = nblk(na, 256);
+    const uint32_t tiles = (c->nf_P + kFarTile - 1) / kFarTile;
+    const uint32_t want = std::max(1u, std::min(tiles, (2048u + ablk - 1) / ablk));
+    const uint32_t chunk = ((tiles + want - 1) / want) * kFarTile;
+    const uint32_t nchunks = (c->nf_P + chunk - 1) / chunk;
     std::vector<float> meta_q(8 * (size_t)c->nf_P);  // caller order -> internal point order
     for (size_t q = 0; q < c->nf_P; ++q)
         std::memcpy(&meta_q[8 * q], meta + 8 * (size_t)c->nf_perm[q], 8 * sizeof(float));
@@ -1728,6 +1737,7 @@ int efs_farfield(efs_ctx* c, uint32_t f, const float* meta, const float* theta, 
         if ((rc = dupload(c, d_th, theta, nT))) break;
         if ((rc = dupload(c, d_ph, phi, nP))) break;
         if ((rc = dalloc(c, d_out, 5 * na))) break;
+        if ((rc = dalloc(c, d_part, (size_t)nchunks * na * 8))) break;
         const double C0 = 299792458.0, Z0 = 376.73031346177066;
         const double k = 2 * M_PI * frequency / C0;
         const double fmag = k / (4 * M_PI * radius), fph = -k * radius;
@@ -1737,15 +1747,16 @@ int efs_farfield(efs_ctx* c, uint32_t f, const float* meta, const float* theta, 
         q.cx = (float)center[0]; q.cy = (float)center[1]; q.cz = (float)center[2];
         q.fac_re = (float)(fmag * -std::sin(fph));
         q.fac_im = (float)(fmag * std::cos(fph));
-        hipLaunchKernelGGL(k_farfield, dim3(nblk(na)), dim3(256), 0, c->stream, c->nf_acc + 2 * (size_t)f,
-                           c->nf_NC, c->nf_geom.rowsE, d_meta, d_th, d_ph, q, d_out);
+        hipLaunchKernelGGL(k_farfield, dim3(ablk, nchunks), dim3(256), 0, c->stream, c->nf_acc + 2 * (size_t)f,
+                           c->nf_NC, c->nf_geom.rowsE, d_meta, d_th, d_ph, q, chunk, d_part);
+        hipLaunchKernelGGL(k_farfield_final, dim3(ablk), dim3(256), 0, c->stream, d_part, nchunks, q, d_out);
         hipError_t e = hipGetLastError();
         if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
         if (e == hipSuccess) e = hipMemcpy(out, d_out, 5 * na * sizeof(float), hipMemcpyDeviceToHost);
         if (e != hipSuccess) rc = fail(c, EFS_ERR_HIP, "far field: %s", hipGetErrorString(e));
     } while (0);
     hipStreamSynchronize(c->stream);
-    dfree(d_meta); dfree(d_th); dfree(d_ph); dfree(d_out);
+    dfree(d_meta); dfree(d_th); dfree(d_ph); dfree(d_out); dfree(d_part);
     return rc;
 }
 

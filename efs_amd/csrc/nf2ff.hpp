@@ -210,23 +210,32 @@ struct FarParams {
 constexpr int kFarTile = 256;
 
 // acc_f = acc + 2f: the frequency's (re, im) column pair, NC floats per row.
+// Grid: x over angles (one thread each), y over chunks of `chunk` surface
+// points (a multiple of kFarTile); each workgroup writes the 8 FP64 radiation
+// integrals (N_theta, N_phi, L_theta, L_phi; re, im) of its angles over its
+// points to partial[(chunk index * nA + angle) * 8], and k_farfield_final
+// adds the chunks in order, so the result does not depend on scheduling.
+// Splitting the points keeps the launch wide when there are few angles
+// (an RCS cut of 64 angles is one workgroup of angles).
 __global__ __launch_bounds__(256) void k_farfield(const float* __restrict__ acc_f, uint32_t NC, uint32_t rowsE,
                                                   const float* __restrict__ meta, const float* __restrict__ theta,
-                                                  const float* __restrict__ phi, FarParams q,
-                                                  float* __restrict__ out) {
+                                                  const float* __restrict__ phi, FarParams q, uint32_t chunk,
+                                                  double* __restrict__ partial) {
     __shared__ float s_pt[kFarTile][18];  // pos xyz, nd, sign, area, E re/im x3, H re/im x3
+    const uint32_t nA = q.nT * q.nP;
     const uint32_t a = blockIdx.x * blockDim.x + threadIdx.x;
-    const bool active = a < q.nT * q.nP;
+    const bool active = a < nA;
     float sT = 0.f, cT = 1.f, sP = 0.f, cP = 1.f;
     if (active) {
         sincosf(theta[a / q.nP], &sT, &cT);
         sincosf(phi[a % q.nP], &sP, &cP);
     }
+    const uint32_t p0 = blockIdx.y * chunk, p1 = min(q.P, p0 + chunk);
     double Ntr = 0, Nti = 0, Npr = 0, Npi = 0, Ltr = 0, Lti = 0, Lpr = 0, Lpi = 0;
-    for (uint32_t base = 0; base < q.P; base += kFarTile) {
+    for (uint32_t base = p0; base < p1; base += kFarTile) {
         __syncthreads();
         const uint32_t p = base + threadIdx.x;
-        if (threadIdx.x < kFarTile && p < q.P) {
+        if (threadIdx.x < kFarTile && p < p1) {
             float* s = s_pt[threadIdx.x];
             s[0] = meta[8 * (size_t)p] - q.cx;
             s[1] = meta[8 * (size_t)p + 1] - q.cy;
@@ -246,7 +255,7 @@ __global__ __launch_bounds__(256) void k_farfield(const float* __restrict__ acc_
         }
         __syncthreads();
         if (!active) continue;
-        const uint32_t cnt = min((uint32_t)kFarTile, q.P - base);
+        const uint32_t cnt = min((uint32_t)kFarTile, p1 - base);
         for (uint32_t j = 0; j < cnt; ++j) {
             const float* s = s_pt[j];
             const int nd = (int)s[3];
@@ -284,6 +293,25 @@ __global__ __launch_bounds__(256) void k_farfield(const float* __restrict__ acc_
         }
     }
     if (!active) return;
+    double* o = partial + ((size_t)blockIdx.y * nA + a) * 8;
+    o[0] = Ntr; o[1] = Nti; o[2] = Npr; o[3] = Npi;
+    o[4] = Ltr; o[5] = Lti; o[6] = Lpr; o[7] = Lpi;
+}
+
+// Sum of the chunk partials per angle, then E_theta, E_phi and the radiation
+// intensity (5 floats per angle).
+__global__ __launch_bounds__(256) void k_farfield_final(const double* __restrict__ partial, uint32_t nchunks,
+                                                        FarParams q, float* __restrict__ out) {
+    const uint32_t nA = q.nT * q.nP;
+    const uint32_t a = blockIdx.x * blockDim.x + threadIdx.x;
+    if (a >= nA) return;
+    double v[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    for (uint32_t c = 0; c < nchunks; ++c) {
+        const double* p = partial + ((size_t)c * nA + a) * 8;
+#pragma unroll
+        for (int i = 0; i < 8; ++i) v[i] += p[i];
+    }
+    const double Ntr = v[0], Nti = v[1], Npr = v[2], Npi = v[3], Ltr = v[4], Lti = v[5], Lpr = v[6], Lpi = v[7];
     const double fr = q.fac_re, fi = q.fac_im, Z0 = q.Z0;
     const double a1r = Lpr + Z0 * Ntr, a1i = Lpi + Z0 * Nti;
     const double Etr = -(fr * a1r - fi * a1i), Eti = -(fr * a1i + fi * a1r);
