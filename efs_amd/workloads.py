@@ -124,7 +124,7 @@ def patch_antenna(patch_w=32.0, patch_l=40.0, f0=3e9, fc=3e9, eps_r=3.38, sub_h=
              "primalEdgeLens": primal, "dualEdgeLens": dual, "kBlock": k_block}
     return {"gridSize": list(n), "coefficients": op.to_f32(coeffs), "mur": mur, "excitation": excitation,
             "nf2ff": nf2ff, "pointMeta": meta, "freqs": np.array([nf2ff_freq]), "dT": dT, "probes": probes,
-            "mesh": mesh, "nTS": n_ts, "f0": f0, "fc": fc, "patch": [patch_w, patch_l]}
+            "mesh": mesh, "lines_mm": [xs, ys, zs], "nTS": n_ts, "f0": f0, "fc": fc, "patch": [patch_w, patch_l]}
 
 
 def configure_patch(eng, w, nf2ff=True, gather_interval=1, max_samples=None, storage=None):
