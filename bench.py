@@ -486,6 +486,133 @@ def run_dispersive(args):
     print(json.dumps(result))
 
 
+def large_outputs(g, w, slices, K, f_idx):
+    """Per-port S11 (dB) over 2-18 GHz and Dmax at freqs[f_idx] of an engine
+    after K steps."""
+    from efs_amd import farfield, ports
+    r = g.readProbeGatherBuffered()
+    tr = ports.probe_traces(r["data"], r["numSamples"], r["stride"], w["probes"], slices, 1, w["dT"])
+    f = np.linspace(2e9, 18e9, 81)
+    s11 = np.stack([20 * np.log10(np.abs(ports.port_s11(tr[2 * p], tr[2 * p + 1], f)))
+                    for p in range(len(w["probes"]) // 2)])
+    theta, phi = farfield.sphere_grid(5, 5)
+    ff = g.computeNF2FFfarField({"pointMeta": w["pointMeta"], "theta": theta, "phi": phi, "center": [0.2, 0.2, 0.15],
+                                 "frequency": float(w["freqs"][f_idx]), "radius": 1.0, "freqIndex": f_idx})
+    acc = g.readNF2FFAccumulation()
+    P = w["nf2ff"]["numPoints"]
+    p_rad = farfield.rad_power(acc["accumE"][f_idx], acc["accumH"][f_idx], w["pointMeta"], P)
+    d = farfield.directivity(ff, len(theta), len(phi), p_rad)
+    return s11, float(d["DmaxdBi"])
+
+
+def run_large(args):
+    """The large mixed-precision workload (800 x 800 x 600 = 384 M cells,
+    UPML-8, 200 seeded boxes, a 4 x 4 lumped-port array, NF2FF at 32
+    frequencies with K = 64) on one GPU; FP16 storage by default here."""
+    from efs_amd import ports, workloads
+    from efs_amd.engine import PROF_SWEEP_E, PROF_SWEEP_H, HIPEngine
+
+    def say(*a):
+        print("#", *a, file=sys.stderr, flush=True)
+    threads = min(16, len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else 1)
+    t_b = time.perf_counter()
+    w = workloads.large_array(threads=threads, progress=lambda m: say(m, f"{time.perf_counter() - t_b:.1f} s"))
+    n = int(np.prod(w["gridSize"]))
+    K = args.steps if args.steps != 2000 else w["nTS"]
+    f_idx = int(np.argmin(np.abs(w["freqs"] - w["f0"])))
+
+    def run(storage, profile):
+        g = HIPEngine()
+        if not g.initGPU():
+            raise RuntimeError("no HIP device")
+        slices = workloads.configure_large(g, w, storage=storage, max_samples=K)
+        g.sync()
+        say(storage, "configured", f"{time.perf_counter() - t_b:.1f} s")
+        t0 = time.perf_counter()
+        done = 0
+        while done < K:  # a progress line per chunk; the stream stays busy in between
+            step = min(2000, K - done)
+            g.iterate(step)
+            g.sync()
+            done += step
+            say(storage, "steps", done)
+        elapsed = time.perf_counter() - t0
+        s11, dmax = large_outputs(g, w, slices, K, f_idx)
+        kern = None
+        if profile:
+            g.profileEnable(True)
+            g.iterate(args.profile_steps)
+            g.sync()
+            ms_e, ce = g.profileRead(PROF_SWEEP_E)
+            ms_h, ch = g.profileRead(PROF_SWEEP_H)
+            kern = (ms_e / ce, ms_h / ch)
+        flags = g.pathFlags()
+        g.destroy()
+        return {"value": n * K / elapsed / 1e6, "elapsed": elapsed, "s11": s11, "dmax": dmax, "kern": kern,
+                "flags": flags}
+
+    main = run(args.storage, True)
+    extra = {"results": {"Dmax_dBi_at_f0": round(main["dmax"], 3),
+                         "s11_min_dB_per_port": [round(float(v), 2) for v in main["s11"].min(axis=1)]}}
+    if args.storage == "fp16" and not args.no_compare:
+        ref = run("fp32", False)
+        extra["fp32_value"] = round(ref["value"], 1)
+        extra["speedup_vs_fp32"] = round(main["value"] / ref["value"], 3)
+        extra["precision_vs_fp32_same_grid"] = {
+            "s11_max_abs_err_dB_2_18GHz": float(np.max(np.abs(main["s11"] - ref["s11"]))),
+            "Dmax_err_dB": abs(main["dmax"] - ref["dmax"])}
+    if not args.no_compare:
+        say("FP64 oracle sample")
+        from oracle import oracle as orc
+        orc.Oracle.set_threads(threads)
+        ws = workloads.large_array(dims=(96, 96, 72), port_pitch=8, n_boxes=20, n_ts=1500, nfreq=4, threads=threads)
+        Ks = ws["nTS"]
+        fs = np.linspace(2e9, 18e9, 81)
+        outs = []
+        for eng in ("gpu", "fp64"):
+            if eng == "gpu":
+                e = HIPEngine()
+                e.initGPU()
+                sl = workloads.configure_large(e, ws, nf2ff=False, storage=args.storage, max_samples=Ks)
+                e.iterate(Ks)
+                r = e.readProbeGatherBuffered()
+                data, ns, stride = r["data"], r["numSamples"], r["stride"]
+                e.destroy()
+            else:
+                e = orc.Oracle(ws["gridSize"], ws["coefficients"], fp64=True)
+                sl = workloads.configure_large(e, ws, nf2ff=False, max_samples=Ks)
+                e.iterate(Ks)
+                r = e.readProbeGatherBuffered()
+                data, ns, stride = r.ravel(), r.shape[0], r.shape[1]
+            tr = ports.probe_traces(data, ns, stride, ws["probes"], sl, 1, ws["dT"])
+            outs.append(np.stack([20 * np.log10(np.abs(ports.port_s11(tr[2 * p], tr[2 * p + 1], fs)))
+                                  for p in range(len(ws["probes"]) // 2)]))
+        extra["precision_vs_fp64"] = {"against": "FP64 oracle, 96 x 96 x 72 sample of the same builder (4 x 4 ports, "
+                                                 "20 boxes), 1,500 steps",
+                                      "s11_max_abs_err_dB_2_18GHz": float(np.max(np.abs(outs[0] - outs[1])))}
+    kern = main["kern"]
+    b = sweep_bytes(n, w["n_pml_cells"], args.storage)
+    dom_name, dom_ms = ("sweep_h", kern[1]) if kern[1] >= kern[0] else ("sweep_e", kern[0])
+    result = {
+        "metric": METRIC, "value": round(main["value"], 1), "unit": "MCells/s", "n_gpus": 1, "steps": K, "warmup": 0,
+        "ms_per_step": round(main["elapsed"] / K * 1e3, 4), "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f32" if args.storage == "fp32" else "f16 storage, f32 compute",
+        "data": "synthetic",
+        "config": {"workload": "large mixed-precision grid: 800 x 800 x 600 x 0.5 mm (384 M cells), UPML-8, 200 seeded "
+                               "boxes (150 dielectric, 50 PEC plates), 4 x 4 array of 50-ohm lumped ports (Gaussian f0 "
+                               "10 GHz, fc 8 GHz), NF2FF six-face box at 32 frequencies (K = 64); whole run timed from "
+                               "zero fields", "grid": w["gridSize"], "dT": w["dT"], "path": sorted(main["flags"]),
+                   "nf2ff_points": w["nf2ff"]["numPoints"], "parallelism": "single GPU"},
+        "roofline": {"bound": "hbm", "kernel": dom_name, "achieved": round(b / (dom_ms * 1e-3) / 1e9, 1),
+                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(b / (dom_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+                     "traffic": None, "traffic_source": None, "algorithmic_bytes_per_launch": b,
+                     "avg_launch_ms": round(dom_ms, 5),
+                     "kernels": {"sweep_e": {"avg_ms": round(kern[0], 5)}, "sweep_h": {"avg_ms": round(kern[1], 5)}}},
+        "cpu_baseline": None, **extra,
+    }
+    print(json.dumps(result))
+
+
 def calibrate_copy():
     import torch
     a = torch.ones(256 * 1024 * 1024, dtype=torch.float32, device="cuda")
@@ -510,10 +637,11 @@ def main():
                          "FP32 updates)")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="run one elementwise copy of 1 GiB -> 1 GiB first (reference kernel for HBM counters)")
-    ap.add_argument("--workload", choices=["stress", "batch", "patch", "dispersive"], default="stress",
+    ap.add_argument("--workload", choices=["stress", "batch", "patch", "dispersive", "large"], default="stress",
                     help="stress: the 256^3 UPML stress grid; batch: the batched patch-antenna design sweep; "
                          "patch: the reference's patch-antenna example on the plain engine; dispersive: "
-                         "Drude/Lorentz scatterers under a TFSF plane wave (192^3) with the RCS cut")
+                         "Drude/Lorentz scatterers under a TFSF plane wave (192^3) with the RCS cut; large: the "
+                         "384 M-cell mixed-precision grid with a 4 x 4 port array")
     ap.add_argument("--instances", type=int, default=64, help="batch: number of patch instances")
     ap.add_argument("--plain-steps", type=int, default=200, help="batch: steps per instance of the plain baseline")
     ap.add_argument("--s11-instances", type=int, default=3, help="batch: instances in the FP64 S11 comparison")
@@ -527,10 +655,11 @@ def main():
                                    "of this build (use --gpus 1)", "n_gpus": args.gpus}))
         sys.exit(2)
 
-    if args.workload in ("batch", "patch", "dispersive"):
+    if args.workload in ("batch", "patch", "dispersive", "large"):
         if args.calibrate_copy:
             calibrate_copy()
-        return {"batch": run_batch, "patch": run_patch, "dispersive": run_dispersive}[args.workload](args)
+        return {"batch": run_batch, "patch": run_patch, "dispersive": run_dispersive,
+                "large": run_large}[args.workload](args)
 
     if args.calibrate_copy:
         calibrate_copy()

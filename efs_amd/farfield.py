@@ -15,6 +15,9 @@ Restates the reference worker's reporting step:
 
 `sphere_grid` makes the reference's 2-degree theta/phi grid.
 
+`circular` gives the circular-polarisation components of
+src/nf2ff.mjs:554-573.
+
 For scatterers under a plane wave (no counterpart in the reference's
 worker): `incident_spectrum` and `rcs` turn the far field of the
 scattered-field NF2FF surface into a bistatic radar cross section.
@@ -104,3 +107,14 @@ def rcs(ff, e_inc):
     U = np.asarray(ff, np.float64).reshape(-1, 5)[:, 0]
     z0 = 376.73031346177066
     return 4 * np.pi * U / (e_inc * e_inc / (2 * z0))
+
+
+def circular(ff):
+    """Circular-polarisation magnitudes per angle from a far-field result
+    (5 values per angle), with the reference's labels (src/nf2ff.mjs:
+    554-573): E_cprh = |E_theta - j E_phi| / sqrt(2), E_cplh =
+    |E_theta + j E_phi| / sqrt(2)."""
+    f = np.asarray(ff, np.float64).reshape(-1, 5)
+    et = f[:, 1] + 1j * f[:, 2]
+    ep = f[:, 3] + 1j * f[:, 4]
+    return np.abs(et - 1j * ep) / np.sqrt(2.0), np.abs(et + 1j * ep) / np.sqrt(2.0)

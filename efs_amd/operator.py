@@ -231,6 +231,35 @@ def build_coefficients(mesh, mat, dT):
     return {"vv": vv, "vi": vi, "ii": ii, "iv": iv}
 
 
+def build_coefficients_threaded(mesh, mat, dT, threads=8, chunk=None):
+    """build_coefficients over x-slabs on a thread pool (numpy releases the
+    interpreter lock in its array loops).  Each slab is built as its own
+    sub-grid with one node of halo on each side, so every node's arithmetic
+    sees the same operands as in the whole-grid build: the result is the
+    same numbers."""
+    from concurrent.futures import ThreadPoolExecutor
+    nx = mesh.n[0]
+    if threads <= 1 or nx < 8:
+        return build_coefficients(mesh, mat, dT)
+    out = {k: np.zeros((3,) + mesh.shape) for k in ("vv", "vi", "ii", "iv")}
+    chunk = chunk or max(2, -(-nx // (4 * threads)))
+
+    def work(x0):
+        x1 = min(nx, x0 + chunk)
+        a, b = max(0, x0 - 1), min(nx, x1 + 1)  # nodes a .. b-1 (halo included)
+        sub = Mesh(mesh.lines[0][a:b], mesh.lines[1], mesh.lines[2])
+        sm = Materials.__new__(Materials)
+        sm.mesh = sub
+        sm.eps, sm.kappa, sm.mu = mat.eps[a:b - 1], mat.kappa[a:b - 1], mat.mu[a:b - 1]
+        sm.pec, sm.port_g = mat.pec[:, a:b], mat.port_g[:, a:b]
+        co = build_coefficients(sub, sm, dT)
+        for k in out:
+            out[k][:, x0:x1] = co[k][:, x0 - a:x1 - a]
+    with ThreadPoolExecutor(threads) as ex:
+        list(ex.map(work, range(0, nx, chunk)))
+    return out
+
+
 def pml_sigma_profile(width_cells, dl, R0=1e-6, g=2.5):
     s0 = -np.log(R0) * np.log(g) / (2 * dl * Z0 * (g ** width_cells - 1))
     return lambda d: np.where(d > 0, s0 * g ** (d / dl), 0.0)

@@ -263,3 +263,100 @@ def configure_dispersive(eng, w, nf2ff=True, storage=None):
     eng.configureTFSF(w["tfsf"])
     if nf2ff:
         eng.configureNF2FFAccumulation(w["nf2ff"])
+
+
+LARGE_SEED = 20261203 + 5
+
+
+def large_array(dims=(800, 800, 600), cell=0.5e-3, seed=LARGE_SEED, pml=8, n_boxes=200, pec_fraction=0.25,
+                ports=(4, 4), port_pitch=40, f0=10e9, fc=8e9, nfreq=32, k_block=64, n_ts=8000, threads=16,
+                progress=None):
+    """Large mixed-precision workload: a dims grid of `cell` cubes (800 x 800
+    x 600 x 0.5 mm = 384 M cells by default) with UPML on all six faces,
+    n_boxes seeded boxes (a pec_fraction of them PEC plates, the others
+    dielectric with eps_r in U(1, 10) and kappa in U(0, 0.05) S/m), and a
+    ports[0] x ports[1] array of z-directed 50-ohm lumped ports (port_pitch
+    cells apart, centred, all driven by one Gaussian pulse f0, fc), the
+    ports' voltage and current probes, and a six-face NF2FF box at nfreq
+    frequencies over the pulse's band (f0 - fc .. f0 + fc).  The boxes keep
+    clear of the port array.  Coefficients come from the threaded slab
+    builder (the same numbers as the whole-grid one)."""
+    say = progress or (lambda *a: None)
+    mesh = op.uniform_mesh(dims, cell)
+    dT = mesh.timestep()
+    mat = op.Materials(mesh)
+    rng = np.random.default_rng(seed)
+    n = np.array(dims)
+    lo_lim, hi_lim = np.full(3, pml + 2), n - pml - 2
+    span = hi_lim - lo_lim
+    c = n // 2
+    pa = np.array([port_pitch * (ports[0] - 1) / 2, port_pitch * (ports[1] - 1) / 2, 4])
+    keep_lo, keep_hi = c - pa - 4, c + pa + 4
+    placed, pec = 0, 0
+    n_pec = int(round(pec_fraction * n_boxes))
+    while placed < n_boxes:
+        size = np.maximum(rng.integers(span // 40, span // 8 + 1), 1)
+        lo = np.array([rng.integers(lo_lim[m], max(hi_lim[m] - size[m], lo_lim[m] + 1)) for m in range(3)])
+        hi = lo + size
+        if all(lo[m] <= keep_hi[m] and hi[m] >= keep_lo[m] for m in range(3)):
+            continue  # keep the port array clear
+        if pec < n_pec:
+            ax = int(rng.integers(0, 3))
+            lo[ax] = hi[ax] = int(rng.integers(lo[ax], hi[ax] + 1))
+            mat.pec_box(lo, hi)
+            pec += 1
+        else:
+            mat.box(lo, hi, eps=rng.uniform(1.0, 10.0), kappa=rng.uniform(0.0, 0.05))
+        placed += 1
+    op.pec_boundary(mat)
+    exc = {"pos": [], "dir": [], "amp": []}
+    probes = []
+    for i in range(ports[0]):
+        for j in range(ports[1]):
+            px = int(c[0] + (i - (ports[0] - 1) / 2) * port_pitch)
+            py = int(c[1] + (j - (ports[1] - 1) / 2) * port_pitch)
+            port, pr = op.lumped_port(mesh, mat, 2, (px, py, c[2] - 1), (px, py, c[2] + 1), 50.0, excite=1.0)
+            for k in exc:
+                exc[k].append(port[k])
+            for p in pr:
+                p["name"] = "%s_%d_%d" % (p["name"], i, j)
+            probes += pr
+    say("geometry done")
+    coeffs = op.build_coefficients_threaded(mesh, mat, dT, threads=threads)
+    del mat
+    say("coefficients done")
+    regions = op.build_upml(mesh, coeffs, [pml] * 6, dT)
+    say("upml done")
+    coeffs = op.to_f32(coeffs)
+    signal = op.gaussian_pulse(f0, fc, dT)
+    pos = np.concatenate(exc["pos"])
+    excitation = {"signal": signal, "delay": np.zeros(len(pos), np.uint32),
+                  "amp": np.concatenate(exc["amp"]).astype(np.float32), "dir": np.concatenate(exc["dir"]),
+                  "pos": pos, "period": 0}
+    pts, meta, primal, dual = op.nf2ff_box(mesh, [pml + 2] * 3, [k - pml - 3 for k in dims])
+    freqs = np.linspace(f0 - fc, f0 + fc, nfreq)
+    nf2ff = {"surfaceIndices": pts, "numPoints": len(pts) // 3, "omega": list(2 * np.pi * freqs), "dT": dT,
+             "primalEdgeLens": primal, "dualEdgeLens": dual, "kBlock": k_block}
+    n_pml = sum(int(np.prod(r["numLines"])) for r in regions)
+    say("workload done")
+    return {"gridSize": list(dims), "coefficients": coeffs, "pmlRegions": regions, "excitation": excitation,
+            "probes": probes, "nf2ff": nf2ff, "pointMeta": meta, "freqs": freqs, "dT": dT, "nTS": n_ts,
+            "n_pml_cells": n_pml, "ports": ports, "f0": f0, "fc": fc, "seed": seed}
+
+
+def configure_large(eng, w, nf2ff=True, storage=None, max_samples=None):
+    """Configure an engine with a large_array workload (coefficients through
+    init for engines that have it), the port probes' buffered gather and
+    optionally the NF2FF accumulation; returns the gather slices."""
+    if hasattr(eng, "init"):
+        if storage is None:
+            eng.init(w["gridSize"], w["coefficients"])
+        else:
+            eng.init(w["gridSize"], w["coefficients"], storage=storage)
+    eng.configurePML(w["pmlRegions"])
+    eng.configureExcitation(w["excitation"])
+    idx, slices = ports.build_gather_indices(w["probes"], w["gridSize"])
+    eng.configureProbeGatherBuffered(idx, max_samples if max_samples is not None else w["nTS"], 1)
+    if nf2ff:
+        eng.configureNF2FFAccumulation(w["nf2ff"])
+    return slices

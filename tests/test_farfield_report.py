@@ -89,3 +89,29 @@ def test_surface_builder_order_and_areas():
     assert k == len(pts)
     theta, phi = farfield.sphere_grid()
     assert (len(theta), len(phi)) == (91, 181)
+
+
+def test_circular_components(oracle_mod):
+    """Two crossed current elements (x and y, equal amplitude, +-90 degrees
+    apart) radiate a circularly polarised wave along z: one of the two
+    components vanishes there and the other carries all of |E|; the phase
+    sign picks which (the labels are the reference's).  Everywhere
+    |E_cprh|^2 + |E_cplh|^2 = |E_theta|^2 + |E_phi|^2."""
+    theta, phi = farfield.sphere_grid(10, 10)
+    out = {}
+    for q in (1.0, -1.0):
+        # normal +z: J = n x H, so J_x = -H_y and J_y = H_x; J_y = j q J_x
+        meta = np.array([0, 0, 0, 2, 1.0, 1e-6, 0, 0], float)
+        accE = np.zeros(6)
+        accH = np.zeros(6)
+        accH[2] = -1.0          # H_y re -> J_x = 1
+        accH[1] = q             # H_x im -> J_y = j q
+        ff = oracle_mod.farfield(accE, accH, meta, theta, phi, np.zeros(3), 1e9, 1.0)
+        rh, lh = farfield.circular(ff)
+        f = np.asarray(ff).reshape(-1, 5)
+        np.testing.assert_allclose(rh ** 2 + lh ** 2, np.sum(f[:, 1:] ** 2, axis=1), rtol=1e-12, atol=1e-40)
+        out[q] = (rh[0], lh[0])  # theta = 0
+    (rh_p, lh_p), (rh_m, lh_m) = out[1.0], out[-1.0]
+    assert min(rh_p, lh_p) < 1e-9 * max(rh_p, lh_p)
+    assert min(rh_m, lh_m) < 1e-9 * max(rh_m, lh_m)
+    assert (rh_p > lh_p) != (rh_m > lh_m)
