@@ -42,13 +42,13 @@ def sweep_kernel_symbol(flags, side):
     """Name of the kernel that runs the E or H sweep for the path flags."""
     if "vec4" not in flags:
         return f"efs::sweep_{side}1<true>"
-    suffix = ""
+    cc = "c" if "coef_idx" in flags else "x" if "coef_idx16" in flags else ""
     if "fused" in flags:
-        suffix = "f" + ("c" if "coef_idx" in flags else "") + ("p" if "upml_idx" in flags else "")
+        suffix = "f" + cc + ("p" if "upml_idx" in flags else "")
     elif "upml4" in flags:
-        suffix = "mc" if "coef_idx" in flags else "m"
+        suffix = "m" + cc
     else:
-        suffix = "c" if "coef_idx" in flags else ""
+        suffix = cc
     return f"efs::sweep_{side}4{suffix}" + ("_h" if "f16" in flags else "")
 
 

@@ -19,7 +19,7 @@ NAPI_OUT = os.path.join(HERE, "js", "efs_engine.node")
 NODE_INCLUDE = "/usr/include/node"
 
 HIPCC_FLAGS = [
-    "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-shared",
+    "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-shared", "-pthread",
     "-Wno-unused-result", "-Wno-unused-value",
     # packed FP32 instructions stay off (hipcc's host pass warns that it does
     # not know the feature; that is expected)

@@ -25,6 +25,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "../../include/efs_engine.h"
@@ -74,6 +75,11 @@ struct efs_ctx {
     bool comp = false;
     uint8_t *cidx_e = nullptr, *cidx_h = nullptr;
     float2 *ctab_e = nullptr, *ctab_h = nullptr;
+    // 16-bit form, when a component has more than 256 distinct pairs
+    bool comp16 = false;
+    uint16_t *cidx16_e = nullptr, *cidx16_h = nullptr;
+    float2 *ctab16_e = nullptr, *ctab16_h = nullptr;
+    uint32_t tabn_e = 0, tabn_h = 0;
     float *volt = nullptr, *curr = nullptr, *vv = nullptr, *vi = nullptr, *ii = nullptr, *iv = nullptr;
     uint32_t numTS = 0;
     SweepGeom geom{};   // 4-wide sweeps
@@ -314,6 +320,39 @@ bool pair_table(const float* a, const float* b, size_t n, uint8_t* idx, float2* 
     return true;
 }
 
+// The same for at most 65,536 pairs: 16-bit indices, the distinct pairs in
+// first-seen order in tab.
+bool pair_table16(const float* a, const float* b, size_t n, uint16_t* idx, std::vector<float2>& tab) {
+    constexpr uint32_t H = 1u << 17;  // open-addressing slots (load <= 1/2)
+    std::vector<uint64_t> key(H);
+    std::vector<int32_t> val(H, -1);
+    tab.clear();
+    uint64_t last_k = ~0ull;
+    uint16_t last_v = 0;
+    for (size_t i = 0; i < n; ++i) {
+        uint32_t ua, ub;
+        std::memcpy(&ua, a + i, 4);
+        std::memcpy(&ub, b + i, 4);
+        const uint64_t k = ((uint64_t)ua << 32) | ub;
+        if (k == last_k) {
+            idx[i] = last_v;
+            continue;
+        }
+        uint32_t h = (uint32_t)((k * 0x9E3779B97F4A7C15ull) >> 47) & (H - 1);
+        while (val[h] >= 0 && key[h] != k) h = (h + 1) & (H - 1);
+        if (val[h] < 0) {
+            if (tab.size() == 65536) return false;
+            key[h] = k;
+            val[h] = (int32_t)tab.size();
+            tab.push_back(make_float2(a[i], b[i]));
+        }
+        last_k = k;
+        last_v = (uint16_t)val[h];
+        idx[i] = last_v;
+    }
+    return true;
+}
+
 // Distinct (a, b, c) triples of one component, accumulated over several
 // arrays (the UPML regions): byte index per cell into a 256-entry table of
 // (a, b, c, 0).  add() returns false once there are more than 256 triples.
@@ -519,7 +558,9 @@ void free_all(efs_ctx* c) {
     dfree(c->sweep_tab4); dfree(c->sweep_tab1);
     c->volt = c->curr = c->vv = c->vi = c->ii = c->iv = nullptr;
     dfree(c->cidx_e); dfree(c->cidx_h); dfree(c->ctab_e); dfree(c->ctab_h);
+    dfree(c->cidx16_e); dfree(c->cidx16_h); dfree(c->ctab16_e); dfree(c->ctab16_h);
     c->comp = false;
+    c->comp16 = false;
     dfree(c->exc_signal); dfree(c->exc_amp); dfree(c->exc_delay); dfree(c->exc_gidx);
     dfree(c->pml_coef); dfree(c->pml_vflux); dfree(c->pml_cflux);
     dfree(c->pml_idx); dfree(c->pml_tab_e); dfree(c->pml_tab_h);
@@ -594,7 +635,7 @@ int plan_sweeps(efs_ctx* c) {
     const uint32_t nx = c->nx, ny = c->ny, nz = c->nzp;  // device rows
     const uint64_t cols = (uint64_t)ny * nz;
     const bool upml_in_grid = !(c->vec4 && c->pml_split && !c->pml_fused);
-    const double core_b = c->comp ? 39.0 : 60.0;
+    const double core_b = c->comp ? 39.0 : c->comp16 ? 42.0 : 60.0;
     const double upml_b = upml_in_grid ? core_b + (c->pml_comp ? 27.0 : 60.0) : 2.0;
     const uint64_t want = env_u64("EFS_SWEEP_BLOCKS", 2048);
     for (int v = 0; v < 2; ++v) {
@@ -799,14 +840,18 @@ int launch_step(efs_ctx* c) {
     {
         ProfScope s(c, EFS_PROF_SWEEP_E);
         if (c->vec4 && (!has_pml || c->pml_split)) {
-            const Coef cf{c->vv, c->vi, c->cidx_e, c->ctab_e, c->pml_idx, c->pml_tab_e};
-            auto kern = has_pml ? (c->comp ? sweep_e4mc : sweep_e4m) : (c->comp ? sweep_e4c : sweep_e4);
+            const Coef cf{c->vv, c->vi, c->cidx_e, c->comp16 ? c->ctab16_e : c->ctab_e, c->pml_idx, c->pml_tab_e,
+                          c->cidx16_e, c->tabn_e};
+            const bool x16 = c->comp16;
+            auto kern = has_pml ? (x16 ? sweep_e4mx : c->comp ? sweep_e4mc : sweep_e4m)
+                                : (x16 ? sweep_e4x : c->comp ? sweep_e4c : sweep_e4);
             if (has_pml && c->pml_fused)
-                kern = c->pml_comp ? (c->comp ? sweep_e4fcp : sweep_e4fp) : (c->comp ? sweep_e4fc : sweep_e4f);
+                kern = c->pml_comp ? (x16 ? sweep_e4fxp : c->comp ? sweep_e4fcp : sweep_e4fp)
+                                   : (x16 ? sweep_e4fx : c->comp ? sweep_e4fc : sweep_e4f);
             if (c->half) {
-                auto k16 = has_pml ? (c->pml_comp ? (c->comp ? sweep_e4fcp_h : sweep_e4fp_h)
-                                                  : (c->comp ? sweep_e4fc_h : sweep_e4f_h))
-                                   : (c->comp ? sweep_e4c_h : sweep_e4_h);
+                auto k16 = has_pml ? (c->pml_comp ? (x16 ? sweep_e4fxp_h : c->comp ? sweep_e4fcp_h : sweep_e4fp_h)
+                                                  : (x16 ? sweep_e4fx_h : c->comp ? sweep_e4fc_h : sweep_e4f_h))
+                                   : (x16 ? sweep_e4x_h : c->comp ? sweep_e4c_h : sweep_e4_h);
                 hipLaunchKernelGGL(k16, dim3(c->geom.blocks), dim3(kSweepThreads), 0, c->stream, (f16*)c->volt,
                                    (const f16*)c->curr, cf, c->geom, c->pml, c->pml_coef, c->pml_vflux, c->fscale,
                                    NfCapture{});
@@ -815,7 +860,8 @@ int launch_step(efs_ctx* c) {
                                    cf, c->geom, c->pml, c->pml_coef, c->pml_vflux, c->fscale, NfCapture{});
             }
             if (has_pml && !c->pml_fused)
-                hipLaunchKernelGGL(c->comp ? sweep_e4uc : sweep_e4u, dim3(c->pml.blocks), dim3(kSweepThreads), 0,
+                hipLaunchKernelGGL(x16 ? sweep_e4ux : c->comp ? sweep_e4uc : sweep_e4u, dim3(c->pml.blocks),
+                                   dim3(kSweepThreads), 0,
                                    c->stream, c->volt, c->curr, cf, c->geom, c->pml, c->pml_coef, c->pml_vflux,
                                    c->fscale, NfCapture{});
         } else {
@@ -864,14 +910,18 @@ int launch_step(efs_ctx* c) {
     {
         ProfScope s(c, EFS_PROF_SWEEP_H);
         if (c->vec4 && (!has_pml || c->pml_split)) {
-            const Coef cf{c->ii, c->iv, c->cidx_h, c->ctab_h, c->pml_idx, c->pml_tab_h};
-            auto kern = has_pml ? (c->comp ? sweep_h4mc : sweep_h4m) : (c->comp ? sweep_h4c : sweep_h4);
+            const Coef cf{c->ii, c->iv, c->cidx_h, c->comp16 ? c->ctab16_h : c->ctab_h, c->pml_idx, c->pml_tab_h,
+                          c->cidx16_h, c->tabn_h};
+            const bool x16 = c->comp16;
+            auto kern = has_pml ? (x16 ? sweep_h4mx : c->comp ? sweep_h4mc : sweep_h4m)
+                                : (x16 ? sweep_h4x : c->comp ? sweep_h4c : sweep_h4);
             if (has_pml && c->pml_fused)
-                kern = c->pml_comp ? (c->comp ? sweep_h4fcp : sweep_h4fp) : (c->comp ? sweep_h4fc : sweep_h4f);
+                kern = c->pml_comp ? (x16 ? sweep_h4fxp : c->comp ? sweep_h4fcp : sweep_h4fp)
+                                   : (x16 ? sweep_h4fx : c->comp ? sweep_h4fc : sweep_h4f);
             if (c->half) {
-                auto k16 = has_pml ? (c->pml_comp ? (c->comp ? sweep_h4fcp_h : sweep_h4fp_h)
-                                                  : (c->comp ? sweep_h4fc_h : sweep_h4f_h))
-                                   : (c->comp ? sweep_h4c_h : sweep_h4_h);
+                auto k16 = has_pml ? (c->pml_comp ? (x16 ? sweep_h4fxp_h : c->comp ? sweep_h4fcp_h : sweep_h4fp_h)
+                                                  : (x16 ? sweep_h4fx_h : c->comp ? sweep_h4fc_h : sweep_h4f_h))
+                                   : (x16 ? sweep_h4x_h : c->comp ? sweep_h4c_h : sweep_h4_h);
                 hipLaunchKernelGGL(k16, dim3(c->geom.blocks), dim3(kSweepThreads), 0, c->stream, (f16*)c->curr,
                                    (const f16*)c->volt, cf, c->geom, c->pml, c->pml_coef, c->pml_cflux, c->fscale,
                                    nf_capture(c));
@@ -880,7 +930,8 @@ int launch_step(efs_ctx* c) {
                                    cf, c->geom, c->pml, c->pml_coef, c->pml_cflux, c->fscale, nf_capture(c));
             }
             if (has_pml && !c->pml_fused)
-                hipLaunchKernelGGL(c->comp ? sweep_h4uc : sweep_h4u, dim3(c->pml.blocks), dim3(kSweepThreads), 0,
+                hipLaunchKernelGGL(x16 ? sweep_h4ux : c->comp ? sweep_h4uc : sweep_h4u, dim3(c->pml.blocks),
+                                   dim3(kSweepThreads), 0,
                                    c->stream, c->curr, c->volt, cf, c->geom, c->pml, c->pml_coef, c->pml_cflux,
                                    c->fscale, NfCapture{});
         } else {
@@ -1058,6 +1109,36 @@ int efs_init(efs_ctx* c, uint32_t nx, uint32_t ny, uint32_t nz, const float* vv,
             TRY(dupload(c, c->ctab_e, te.data(), te.size()));
             TRY(dupload(c, c->ctab_h, th.data(), th.size()));
             c->comp = true;
+        }
+    }
+    // more than 256 pairs somewhere: 16-bit indices (six tables built on six
+    // threads), up to 65,536 pairs per component
+    c->comp16 = false;
+    if (c->vec4 && !c->comp && env_u64("EFS_COMPRESS", 1) != 0) {
+        std::vector<uint16_t> ie(3 * c->S, 0), ih(3 * c->S, 0);
+        std::vector<float2> tabs[6];
+        bool ok[6];
+        std::vector<std::thread> th;
+        for (int k = 0; k < 6; ++k)
+            th.emplace_back([&, k] {
+                const int n = k % 3, side = k / 3;
+                ok[k] = pair_table16(src[2 * side] + n * Np, src[2 * side + 1] + n * Np, Np,
+                                     (side ? ih : ie).data() + n * c->S, tabs[k]);
+            });
+        for (auto& t : th) t.join();
+        if (ok[0] && ok[1] && ok[2] && ok[3] && ok[4] && ok[5]) {
+            for (int side = 0; side < 2; ++side) {
+                size_t m = 1;
+                for (int n = 0; n < 3; ++n) m = std::max(m, tabs[3 * side + n].size());
+                const uint32_t tabn = (uint32_t)((m + 63) & ~(size_t)63);
+                std::vector<float2> t(3 * (size_t)tabn, make_float2(0.f, 0.f));
+                for (int n = 0; n < 3; ++n) std::copy(tabs[3 * side + n].begin(), tabs[3 * side + n].end(),
+                                                      t.begin() + (size_t)n * tabn);
+                TRY(dupload(c, side ? c->cidx16_h : c->cidx16_e, (side ? ih : ie).data(), ie.size()));
+                TRY(dupload(c, side ? c->ctab16_h : c->ctab16_e, t.data(), t.size()));
+                (side ? c->tabn_h : c->tabn_e) = tabn;
+            }
+            c->comp16 = true;
         }
     }
     TRY(dalloc(c, c->en_partial, kEnergyMaxBlocks + 1));
@@ -1790,7 +1871,7 @@ int efs_path_flags(const efs_ctx* c, uint32_t* flags) {
     *flags = (c->vec4 ? EFS_PATH_VEC4 : 0u) | (c->comp ? EFS_PATH_COEF_IDX : 0u) | (upml4 ? EFS_PATH_UPML_4 : 0u) |
              (upml4 && c->pml_fused ? EFS_PATH_FUSED : 0u) |
              (upml4 && c->pml_fused && c->pml_comp ? EFS_PATH_UPML_IDX : 0u) | (c->half ? EFS_PATH_F16 : 0u) |
-             (nf_capture(c).on ? EFS_PATH_NF2FF_CAPTURE : 0u);
+             (nf_capture(c).on ? EFS_PATH_NF2FF_CAPTURE : 0u) | (c->comp16 ? EFS_PATH_COEF_IDX16 : 0u);
     return EFS_OK;
 }
 

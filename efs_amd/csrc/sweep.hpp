@@ -819,7 +819,7 @@ __device__ __forceinline__ bool pml_lane(const PmlTable& pml, Lane4& w) {
 #define EFS_SWEEP4_INTERIOR(NAME, MARCH, HF, HAS_PML, COMP)                                                       \
     __global__ __launch_bounds__(kSweepThreads)                                                                    \
         __attribute__((amdgpu_waves_per_eu(EFS_SWEEP_WAVES, EFS_SWEEP_WAVES))) void NAME(EFS_SWEEP4_PARAMS(float)) { \
-        __shared__ float2 s_tab[COMP ? 3 * kTab : 1];                                                              \
+        __shared__ float2 s_tab[COMP == kCoefIdx8 ? 3 * kTab : 1];                                                              \
         coef_stage<COMP>(cf, s_tab);                                                                               \
         Lane4 w;                                                                                                   \
         uint32_t m, flags;                                                                                         \
@@ -832,7 +832,7 @@ __device__ __forceinline__ bool pml_lane(const PmlTable& pml, Lane4& w) {
 #define EFS_SWEEP4_UPML(NAME, MARCH, HF, COMP)                                                                     \
     __global__ __launch_bounds__(kSweepThreads) __attribute__((                                                    \
         amdgpu_waves_per_eu(EFS_SWEEP_WAVES_PML, EFS_SWEEP_WAVES_PML))) void NAME(EFS_SWEEP4_PARAMS(float)) {     \
-        __shared__ float2 s_tab[COMP ? 3 * kTab : 1];                                                              \
+        __shared__ float2 s_tab[COMP == kCoefIdx8 ? 3 * kTab : 1];                                                              \
         coef_stage<COMP>(cf, s_tab);                                                                               \
         Lane4 w;                                                                                                   \
         if (!pml_lane(pml, w)) return;                                                                             \
@@ -847,7 +847,7 @@ __device__ __forceinline__ bool pml_lane(const PmlTable& pml, Lane4& w) {
 #define EFS_SWEEP4_FUSED(NAME, MARCH, HF, COMP, PCOMP, T)                                                          \
     __global__ __launch_bounds__(kSweepThreads) __attribute__((                                                    \
         amdgpu_waves_per_eu(EFS_SWEEP_WAVES_PML, EFS_SWEEP_WAVES_PML))) void NAME(EFS_SWEEP4_PARAMS(T)) {         \
-        __shared__ float2 s_tab[COMP ? 3 * kTab : 1];                                                              \
+        __shared__ float2 s_tab[COMP == kCoefIdx8 ? 3 * kTab : 1];                                                              \
         __shared__ float4 s_ptab[PCOMP ? 3 * kTab : 1];                                                            \
         coef_stage<COMP>(cf, s_tab);                                                                               \
         Lane4 w;                                                                                                   \
@@ -864,7 +864,7 @@ __device__ __forceinline__ bool pml_lane(const PmlTable& pml, Lane4& w) {
 #define EFS_SWEEP4_PLAIN16(NAME, MARCH, HF, COMP)                                                                  \
     __global__ __launch_bounds__(kSweepThreads)                                                                    \
         __attribute__((amdgpu_waves_per_eu(EFS_SWEEP_WAVES, EFS_SWEEP_WAVES))) void NAME(EFS_SWEEP4_PARAMS(f16)) { \
-        __shared__ float2 s_tab[COMP ? 3 * kTab : 1];                                                              \
+        __shared__ float2 s_tab[COMP == kCoefIdx8 ? 3 * kTab : 1];                                                              \
         coef_stage<COMP>(cf, s_tab);                                                                               \
         Lane4 w;                                                                                                   \
         uint32_t m, flags;                                                                                         \

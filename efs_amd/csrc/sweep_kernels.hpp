@@ -22,6 +22,14 @@ EFS_SWEEP4_FUSED(sweep_e4f_h, e4_march, false, false, false, f16)
 EFS_SWEEP4_FUSED(sweep_e4fc_h, e4_march, false, true, false, f16)
 EFS_SWEEP4_FUSED(sweep_e4fp_h, e4_march, false, false, true, f16)
 EFS_SWEEP4_FUSED(sweep_e4fcp_h, e4_march, false, true, true, f16)
+EFS_SWEEP4_INTERIOR(sweep_e4x, e4_march, false, false, kCoefIdx16)
+EFS_SWEEP4_INTERIOR(sweep_e4mx, e4_march, false, true, kCoefIdx16)
+EFS_SWEEP4_UPML(sweep_e4ux, e4_march, false, kCoefIdx16)
+EFS_SWEEP4_FUSED(sweep_e4fx, e4_march, false, kCoefIdx16, false, float)
+EFS_SWEEP4_FUSED(sweep_e4fxp, e4_march, false, kCoefIdx16, true, float)
+EFS_SWEEP4_PLAIN16(sweep_e4x_h, e4_march, false, kCoefIdx16)
+EFS_SWEEP4_FUSED(sweep_e4fx_h, e4_march, false, kCoefIdx16, false, f16)
+EFS_SWEEP4_FUSED(sweep_e4fxp_h, e4_march, false, kCoefIdx16, true, f16)
 EFS_SWEEP4_INTERIOR(sweep_h4, h4_march, true, false, false)
 EFS_SWEEP4_INTERIOR(sweep_h4c, h4_march, true, false, true)
 EFS_SWEEP4_INTERIOR(sweep_h4m, h4_march, true, true, false)
@@ -38,6 +46,14 @@ EFS_SWEEP4_FUSED(sweep_h4f_h, h4_march, true, false, false, f16)
 EFS_SWEEP4_FUSED(sweep_h4fc_h, h4_march, true, true, false, f16)
 EFS_SWEEP4_FUSED(sweep_h4fp_h, h4_march, true, false, true, f16)
 EFS_SWEEP4_FUSED(sweep_h4fcp_h, h4_march, true, true, true, f16)
+EFS_SWEEP4_INTERIOR(sweep_h4x, h4_march, true, false, kCoefIdx16)
+EFS_SWEEP4_INTERIOR(sweep_h4mx, h4_march, true, true, kCoefIdx16)
+EFS_SWEEP4_UPML(sweep_h4ux, h4_march, true, kCoefIdx16)
+EFS_SWEEP4_FUSED(sweep_h4fx, h4_march, true, kCoefIdx16, false, float)
+EFS_SWEEP4_FUSED(sweep_h4fxp, h4_march, true, kCoefIdx16, true, float)
+EFS_SWEEP4_PLAIN16(sweep_h4x_h, h4_march, true, kCoefIdx16)
+EFS_SWEEP4_FUSED(sweep_h4fx_h, h4_march, true, kCoefIdx16, false, f16)
+EFS_SWEEP4_FUSED(sweep_h4fxp_h, h4_march, true, kCoefIdx16, true, f16)
 
 #undef EFS_SWEEP4_INTERIOR
 #undef EFS_SWEEP4_UPML

@@ -194,7 +194,8 @@ enum {
     EFS_PATH_FUSED = 8,     /* one launch per sweep for core and UPML cells */
     EFS_PATH_UPML_IDX = 16, /* UPML coefficients as byte indices + tables */
     EFS_PATH_F16 = 32,      /* FP16 field storage */
-    EFS_PATH_NF2FF_CAPTURE = 64 /* NF2FF z-face points read slabs the H sweep wrote */
+    EFS_PATH_NF2FF_CAPTURE = 64, /* NF2FF z-face points read slabs the H sweep wrote */
+    EFS_PATH_COEF_IDX16 = 128    /* core coefficients as 16-bit indices + cached tables */
 };
 int efs_path_flags(const efs_ctx *ctx, uint32_t *flags);
 

@@ -147,6 +147,45 @@ def test_compressed_coefficients(HIPEngine, oracle_mod, monkeypatch, name, seed,
     assert_fields_close(fg, o.getFields(), what=name)
 
 
+Q16CASES = [
+    ("q16_core", 51, (20, 18, 32), dict(quantize=300)),
+    ("q16_pml_fused", 52, (24, 20, 32), dict(pml_width=4, quantize=400, mur=True, ade=True, tfsf=True)),
+    ("q16_odd_nz", 53, (18, 16, 27), dict(quantize=250)),
+]
+
+
+@pytest.mark.parametrize("storage", ["fp32", "fp16"])
+@pytest.mark.parametrize("name,seed,dims,kw", Q16CASES, ids=[c[0] for c in Q16CASES])
+def test_coefficient_idx16(HIPEngine, oracle_mod, monkeypatch, name, seed, dims, kw, storage):
+    """More than 256 distinct (a, b) pairs per component but at most 65,536:
+    the 16-bit index form, with the same bits as the FP32 arrays (same
+    values, same arithmetic), and (FP32) the oracle's fields."""
+    case = random_case(seed, dims, **kw)
+    say("case idx16", name, seed, dims, storage)
+    g = HIPEngine()
+    assert g.initGPU()
+    g.init(case["dims"], case["coeffs"], storage=storage)
+    configure(g, case)
+    assert "coef_idx16" in g.pathFlags() and "coef_idx" not in g.pathFlags()
+    monkeypatch.setenv("EFS_COMPRESS", "0")
+    u = HIPEngine()
+    assert u.initGPU()
+    u.init(case["dims"], case["coeffs"], storage=storage)
+    configure(u, case)
+    assert "coef_idx16" not in u.pathFlags()
+    for e in (g, u):
+        e.iterate(30)
+    fg, fu = g.getFields(), u.getFields()
+    np.testing.assert_array_equal(fg["volt"], fu["volt"])
+    np.testing.assert_array_equal(fg["curr"], fu["curr"])
+    if storage == "fp32":
+        o = make_oracle(oracle_mod, case)
+        o.iterate(30)
+        assert_fields_close(fg, o.getFields(), what=name)
+    g.destroy()
+    u.destroy()
+
+
 @pytest.mark.parametrize("name,seed,dims,kw", [
     ("odd_nz_extensions", 47, (23, 19, 29), dict(pml_width=0, mur=True, tfsf=True, ade=True, rlc=True)),
     ("odd_nz_upml", 48, (21, 18, 30), dict(pml_width=3, mur=True)),
