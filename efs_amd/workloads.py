@@ -269,15 +269,17 @@ LARGE_SEED = 20261203 + 5
 
 
 def large_array(dims=(800, 800, 600), cell=0.5e-3, seed=LARGE_SEED, pml=8, n_boxes=200, pec_fraction=0.25,
-                ports=(4, 4), port_pitch=40, f0=10e9, fc=8e9, nfreq=32, k_block=64, n_ts=8000, threads=16,
+                ports=(4, 4), port_pitch=40, arm=14, f0=10e9, fc=8e9, nfreq=32, k_block=64, n_ts=8000, threads=16,
                 progress=None):
     """Large mixed-precision workload: a dims grid of `cell` cubes (800 x 800
     x 600 x 0.5 mm = 384 M cells by default) with UPML on all six faces,
     n_boxes seeded boxes (a pec_fraction of them PEC plates, the others
     dielectric with eps_r in U(1, 10) and kappa in U(0, 0.05) S/m), and a
-    ports[0] x ports[1] array of z-directed 50-ohm lumped ports (port_pitch
-    cells apart, centred, all driven by one Gaussian pulse f0, fc), the
-    ports' voltage and current probes, and a six-face NF2FF box at nfreq
+    ports[0] x ports[1] array of centre-fed z dipoles (thin PEC arms of
+    `arm` cells each side of a one-edge 50-ohm lumped port: 14.5 mm long, a
+    half wave near 10 GHz at 0.5 mm cells; port_pitch cells apart, centred,
+    all driven by one Gaussian pulse f0, fc), the ports' voltage and current
+    probes, and a six-face NF2FF box at nfreq
     frequencies over the pulse's band (f0 - fc .. f0 + fc).  The boxes keep
     clear of the port array.  Coefficients come from the threaded slab
     builder (the same numbers as the whole-grid one)."""
@@ -290,7 +292,7 @@ def large_array(dims=(800, 800, 600), cell=0.5e-3, seed=LARGE_SEED, pml=8, n_box
     lo_lim, hi_lim = np.full(3, pml + 2), n - pml - 2
     span = hi_lim - lo_lim
     c = n // 2
-    pa = np.array([port_pitch * (ports[0] - 1) / 2, port_pitch * (ports[1] - 1) / 2, 4])
+    pa = np.array([port_pitch * (ports[0] - 1) / 2, port_pitch * (ports[1] - 1) / 2, arm + 2])
     keep_lo, keep_hi = c - pa - 4, c + pa + 4
     placed, pec = 0, 0
     n_pec = int(round(pec_fraction * n_boxes))
@@ -315,7 +317,9 @@ def large_array(dims=(800, 800, 600), cell=0.5e-3, seed=LARGE_SEED, pml=8, n_box
         for j in range(ports[1]):
             px = int(c[0] + (i - (ports[0] - 1) / 2) * port_pitch)
             py = int(c[1] + (j - (ports[1] - 1) / 2) * port_pitch)
-            port, pr = op.lumped_port(mesh, mat, 2, (px, py, c[2] - 1), (px, py, c[2] + 1), 50.0, excite=1.0)
+            mat.pec_box((px, py, c[2] + 1), (px, py, c[2] + 1 + arm))
+            mat.pec_box((px, py, c[2] - arm), (px, py, c[2]))
+            port, pr = op.lumped_port(mesh, mat, 2, (px, py, c[2]), (px, py, c[2] + 1), 50.0, excite=1.0)
             for k in exc:
                 exc[k].append(port[k])
             for p in pr:

@@ -26,12 +26,14 @@ def test_threaded_builder_same_numbers():
 
 def test_large_array_structure():
     w = workloads.large_array(dims=(96, 96, 72), port_pitch=8, n_boxes=20, nfreq=4, n_ts=100, threads=4)
-    assert len(w["probes"]) == 32 and len(w["excitation"]["pos"]) == 32  # 16 ports x 2 edges
+    assert len(w["probes"]) == 32 and len(w["excitation"]["pos"]) == 16  # 16 one-edge ports
     co = w["coefficients"]
     N = 96 * 96 * 72
     vi = co["vi"].reshape(3, 96, 96, 72)
-    # every port edge is a live, lumped-loaded edge (vv < 1 from the 50-ohm conductance)
+    # every port edge is a live, lumped-loaded edge (vv < 1 from the 50-ohm
+    # conductance) between two PEC dipole arms
     for p in w["excitation"]["pos"]:
         x, y, z = p // (96 * 72), (p // 72) % 96, p % 72
         assert vi[2, x, y, z] > 0 and co["vv"][2 * N + p] < 1.0
+        assert vi[2, x, y, z + 1] == 0 and vi[2, x, y, z - 1] == 0
     assert len(w["nf2ff"]["omega"]) == 4 and w["n_pml_cells"] > 0
