@@ -111,6 +111,15 @@ struct SweepGeom {
 //   E_x, E_y, E_z at zf and zf+1 (6), then H_x(zf), H_y(zf), H_z(zf),
 //   H_z(zf+1) as stored by the sweep, 2 unused.
 // The NF2FF sample kernel then reads those slabs instead of the rows.
+// Unroll factor of the x march (build-time tuning knob; 1 = no unrolling;
+// 2 measured no faster in FP32 and spills the FP16 fused H sweep).
+#ifndef EFS_MARCH_UNROLL
+#define EFS_MARCH_UNROLL 1
+#endif
+#define EFS_PRAGMA_STR2(x) #x
+#define EFS_PRAGMA_STR(x) EFS_PRAGMA_STR2(x)
+#define EFS_MARCH_UNROLL_PRAGMA _Pragma(EFS_PRAGMA_STR(unroll EFS_MARCH_UNROLL))
+
 struct NfCapture {
     float* slab;
     uint32_t on;          // 0: off
@@ -618,6 +627,7 @@ __device__ __forceinline__ void e4_march(T* __restrict__ volt, const T* __restri
     bool carry = false;  // hy_m/hz_m hold H at x-1
     v4 hy_m = 0.f, hz_m = 0.f;
     PmlCursor u{-1, m ? 0u : 0xffffffffu, nullptr, nullptr, nullptr, 0u, 0u};
+    EFS_MARCH_UNROLL_PRAGMA
     for (uint32_t x = w.x0; x < w.x1; ++x, i += plane) {
         cursor_at<false>(u, pml, m, x, w.y, w.z0, pcoef, pflux, cf.pidx);
         const bool upml = MODE == kMarchRegion || (MODE == kMarchFused && u.k >= 0);
@@ -694,6 +704,7 @@ __device__ __forceinline__ void h4_march(T* __restrict__ curr, const T* __restri
     bool carry = false;  // ex/ey/ez hold E at x
     v4 ex = 0.f, ey = 0.f, ez = 0.f;
     PmlCursor u{-1, m ? 0u : 0xffffffffu, nullptr, nullptr, nullptr, 0u, 0u};
+    EFS_MARCH_UNROLL_PRAGMA
     for (uint32_t x = w.x0; x < w.x1; ++x, i += plane) {
         const bool core = row_core && (x + 1 < g.nx);
         cursor_at<true>(u, pml, m, x, w.y, w.z0, pcoef, pflux, cf.pidx);

@@ -433,3 +433,86 @@ def load(path, k_block=64):
     """build() of an openEMS XML file."""
     with open(path) as f:
         return build(f.read(), k_block=k_block)
+
+
+def run(path, steps=None, storage="fp32", out=None, freqs=None):
+    """Run an openEMS XML file on the HIP engine: build it, step nTS (or
+    `steps`) steps, and return the probe traces, port S11 when the file
+    has a ut/it probe pair (names ending in ut<k> / it<k>), and Dmax at
+    each NF2FF frequency.  With `out`, the result is also written there as
+    JSON."""
+    import json
+    from . import farfield, ports
+    from .engine import HIPEngine
+    w = load(path)
+    K = int(steps or w["nTS"])
+    g = HIPEngine()
+    if not g.initGPU():
+        raise RuntimeError("no HIP device")
+    g.init(w["gridSize"], w["coefficients"], storage=storage)
+    if "pmlRegions" in w:
+        g.configurePML(w["pmlRegions"])
+    if "mur" in w:
+        g.configureMur(w["mur"])
+    g.configureExcitation(w["excitation"])
+    idx, slices = ports.build_gather_indices(w["probes"], w["gridSize"])
+    if len(idx):
+        g.configureProbeGatherBuffered(idx, K, 1)
+    if "nf2ff" in w:
+        g.configureNF2FFAccumulation(w["nf2ff"])
+    g.iterate(K)
+    g.sync()
+    res = {"gridSize": w["gridSize"], "dT": w["dT"], "steps": K, "probes": {}}
+    traces = {}
+    if len(idx):
+        r = g.readProbeGatherBuffered()
+        tr = ports.probe_traces(r["data"], r["numSamples"], r["stride"], w["probes"], slices, 1, w["dT"])
+        for p, (t, v) in zip(w["probes"], tr):
+            traces[p["name"]] = (t, v)
+            res["probes"][p["name"]] = {"time_s": t.tolist(), "value": v.tolist()}
+    f = np.asarray(freqs if freqs is not None else np.linspace(max(w["f0"] - w["fc"], 0.0) or w["fc"] / 100,
+                                                                 w["f0"] + w["fc"], 201), np.float64)
+    res["s11"] = {}
+    for name, (t, v) in traces.items():
+        if "ut" in name:
+            other = name.replace("ut", "it", 1)
+            if other in traces:
+                s = ports.s11_at((t, v), traces[other], f)
+                res["s11"][name.replace("ut", "", 1) or "port"] = {"freq_Hz": f.tolist(),
+                                                                    "s11_dB": s["s11_dB"].tolist()}
+    if "nf2ff" in w:
+        theta, phi = farfield.sphere_grid()
+        acc = g.readNF2FFAccumulation()
+        res["Dmax_dBi"] = []
+        for k, fr in enumerate(w["freqs"]):
+            ff = g.computeNF2FFfarField({"pointMeta": w["pointMeta"], "theta": theta, "phi": phi,
+                                         "center": [0, 0, 0], "frequency": float(fr), "radius": 1.0,
+                                         "freqIndex": k})
+            p_rad = farfield.rad_power(np.asarray(acc["accumE"][k]), np.asarray(acc["accumH"][k]),
+                                       w["pointMeta"], w["nf2ff"]["numPoints"])
+            res["Dmax_dBi"].append(farfield.directivity(ff, len(theta), len(phi), p_rad)["DmaxdBi"])
+    g.destroy()
+    if out:
+        with open(out, "w") as fh:
+            json.dump(res, fh)
+    return res
+
+
+def main(argv=None):
+    import argparse
+    ap = argparse.ArgumentParser(description="Run an openEMS XML simulation on the MI355X engine.")
+    ap.add_argument("xml")
+    ap.add_argument("--steps", type=int, default=None, help="time steps (default: NumberOfTimesteps)")
+    ap.add_argument("--storage", choices=["fp32", "fp16"], default="fp32")
+    ap.add_argument("--out", default=None, help="write probe traces, S11 and Dmax as JSON")
+    a = ap.parse_args(argv)
+    res = run(a.xml, a.steps, a.storage, a.out)
+    for name, s in res["s11"].items():
+        k = int(np.argmin(s["s11_dB"]))
+        print("port %s: min S11 %.2f dB at %.4g GHz" % (name, s["s11_dB"][k], s["freq_Hz"][k] / 1e9))
+    for k, d in enumerate(res.get("Dmax_dBi", [])):
+        print("Dmax %.2f dBi (NF2FF frequency %d)" % (d, k))
+
+
+if __name__ == "__main__":
+    main()

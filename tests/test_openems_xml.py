@@ -146,3 +146,24 @@ def test_unsupported_input_raises():
         ox.parse(xml.replace("<Cylinder ", "<Polygon ").replace("</Cylinder>", "</Polygon>"))
     with pytest.raises(ValueError):
         ox.parse("<notopenEMS/>")
+
+
+@pytest.mark.gpu
+def test_run_coax_on_gpu(tmp_path):
+    """The command-line path (openems_xml.run) on the HIP engine: the coax
+    file's probe traces match native openEMS as on the oracle, and the
+    result file holds the port's S11."""
+    from efs_amd import engine
+    engine.load_library()
+    fx = _fixture("coax_openems.json")
+    path = tmp_path / "coax.xml"
+    path.write_text(docs.coax_xml())
+    res = ox.run(str(path), out=str(tmp_path / "coax.json"))
+    dT = res["dT"]
+    for name in ("ut1", "it1"):
+        t = np.asarray(res["probes"][name]["time_s"])
+        v = np.asarray(res["probes"][name]["value"])
+        tf, vf = np.asarray(fx["probe_" + name]["time_s"]), np.asarray(fx["probe_" + name]["voltage"])
+        vi = np.interp(tf, t - dT, v)
+        assert np.linalg.norm(vi - vf) / np.linalg.norm(vf) < 0.02, name
+    assert "1" in res["s11"] and (tmp_path / "coax.json").exists()
