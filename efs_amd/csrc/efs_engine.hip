@@ -1789,14 +1789,17 @@ int efs_read_nf2ff(efs_ctx* c, float* accE, float* accH) {
     return EFS_OK;
 }
 
-int efs_farfield(efs_ctx* c, uint32_t f, const float* meta, const float* theta, uint32_t nT, const float* phi,
-                 uint32_t nP, const double* center, double frequency, double radius, float* out) {
+static int farfield_impl(efs_ctx* c, uint32_t f, const float* meta, const float* theta, uint32_t nT, const float* phi,
+                         uint32_t nP, const double* center, double frequency, double radius, int mdir, double mpos,
+                         int mtype, int refrule, float* out) {
     TRY(need_init(c));
     if (!c->nf_P) return fail(c, EFS_ERR_STATE, "NF2FF accumulation not configured");
     if (f >= c->nf_F) return fail(c, EFS_ERR_ARG, "frequency index out of range");
     if (!meta || !theta || !phi || !center || !out || !nT || !nP)
         return fail(c, EFS_ERR_ARG, "far-field arrays are required");
     if (!(radius > 0)) return fail(c, EFS_ERR_ARG, "radius must be > 0");
+    if (mdir > 2 || (mdir >= 0 && mtype != 0 && mtype != 1))
+        return fail(c, EFS_ERR_ARG, "mirror direction must be -1..2 and type 0 (PEC) or 1 (PMC)");
     TRY(nf2ff_fold(c));
     float *d_meta = nullptr, *d_th = nullptr, *d_ph = nullptr, *d_out = nullptr;
     double* d_part = nullptr;
@@ -1806,10 +1809,11 @@ int efs_farfield(efs_ctx* c, uint32_t f, const float* meta, const float* theta, 
     // split the surface into chunks of whole LDS tiles so that the launch
     // has ~2048 workgroups even for a few angles
     const uint32_t ablk = nblk(na, 256);
-    const uint32_t tiles = (c->nf_P + kFarTile - 1) / kFarTile;
+    const uint32_t Pv = mdir >= 0 ? 2 * c->nf_P : c->nf_P;  // the image adds a second pass over the points
+    const uint32_t tiles = (Pv + kFarTile - 1) / kFarTile;
     const uint32_t want = std::max(1u, std::min(tiles, (2048u + ablk - 1) / ablk));
     const uint32_t chunk = ((tiles + want - 1) / want) * kFarTile;
-    const uint32_t nchunks = (c->nf_P + chunk - 1) / chunk;
+    const uint32_t nchunks = (Pv + chunk - 1) / chunk;
     std::vector<float> meta_q(8 * (size_t)c->nf_P);  // caller order -> internal point order
     for (size_t q = 0; q < c->nf_P; ++q)
         std::memcpy(&meta_q[8 * q], meta + 8 * (size_t)c->nf_perm[q], 8 * sizeof(float));
@@ -1828,6 +1832,18 @@ int efs_farfield(efs_ctx* c, uint32_t f, const float* meta, const float* theta, 
         q.cx = (float)center[0]; q.cy = (float)center[1]; q.cz = (float)center[2];
         q.fac_re = (float)(fmag * -std::sin(fph));
         q.fac_im = (float)(fmag * std::cos(fph));
+        q.mdir = mdir;
+        q.mpos = (float)mpos;
+        q.msign = refrule ? 1.0f : -1.0f;
+        for (int k3 = 0; k3 < 3; ++k3) q.ef[k3] = q.hf[k3] = 1.0f;
+        if (mdir >= 0) {
+            const int a1 = (mdir + 1) % 3, a2 = (mdir + 2) % 3;
+            if (mtype == 0) {  // PEC: tangential E and normal H reversed
+                q.hf[mdir] = -1.0f; q.ef[a1] = -1.0f; q.ef[a2] = -1.0f;
+            } else {           // PMC: normal E and tangential H reversed
+                q.ef[mdir] = -1.0f; q.hf[a1] = -1.0f; q.hf[a2] = -1.0f;
+            }
+        }
         hipLaunchKernelGGL(k_farfield, dim3(ablk, nchunks), dim3(256), 0, c->stream, c->nf_acc + 2 * (size_t)f,
                            c->nf_NC, c->nf_geom.rowsE, d_meta, d_th, d_ph, q, chunk, d_part);
         hipLaunchKernelGGL(k_farfield_final, dim3(ablk), dim3(256), 0, c->stream, d_part, nchunks, q, d_out);
@@ -1839,6 +1855,18 @@ int efs_farfield(efs_ctx* c, uint32_t f, const float* meta, const float* theta, 
     hipStreamSynchronize(c->stream);
     dfree(d_meta); dfree(d_th); dfree(d_ph); dfree(d_out); dfree(d_part);
     return rc;
+}
+
+int efs_farfield(efs_ctx* c, uint32_t f, const float* meta, const float* theta, uint32_t nT, const float* phi,
+                 uint32_t nP, const double* center, double frequency, double radius, float* out) {
+    return farfield_impl(c, f, meta, theta, nT, phi, nP, center, frequency, radius, -1, 0.0, 0, 0, out);
+}
+
+int efs_farfield_mirror(efs_ctx* c, uint32_t f, const float* meta, const float* theta, uint32_t nT,
+                        const float* phi, uint32_t nP, const double* center, double frequency, double radius,
+                        int mirror_dir, double mirror_pos, int mirror_type, int reference_rule, float* out) {
+    return farfield_impl(c, f, meta, theta, nT, phi, nP, center, frequency, radius, mirror_dir, mirror_pos,
+                         mirror_type, reference_rule, out);
 }
 
 int efs_energy(efs_ctx* c, double* out) {

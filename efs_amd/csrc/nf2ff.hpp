@@ -205,6 +205,12 @@ __global__ __launch_bounds__(256) void k_nf2ff_fold(const float* __restrict__ bu
 struct FarParams {
     uint32_t P, nT, nP, pad;
     float k, Z0, cx, cy, cz, fac_re, fac_im, pad2;
+    // mirror (mdir >= 0): points P .. 2P-1 are the surface reflected in the
+    // plane x[mdir] = mpos, fields times ef / hf, normal sign times msign
+    // for faces perpendicular to the plane
+    int32_t mdir;
+    float mpos, msign;
+    float ef[3], hf[3];
 };
 
 constexpr int kFarTile = 256;
@@ -230,27 +236,36 @@ __global__ __launch_bounds__(256) void k_farfield(const float* __restrict__ acc_
         sincosf(theta[a / q.nP], &sT, &cT);
         sincosf(phi[a % q.nP], &sP, &cP);
     }
-    const uint32_t p0 = blockIdx.y * chunk, p1 = min(q.P, p0 + chunk);
+    const uint32_t Pv = q.mdir >= 0 ? 2 * q.P : q.P;  // points of both passes
+    const uint32_t p0 = blockIdx.y * chunk, p1 = min(Pv, p0 + chunk);
     double Ntr = 0, Nti = 0, Npr = 0, Npi = 0, Ltr = 0, Lti = 0, Lpr = 0, Lpi = 0;
     for (uint32_t base = p0; base < p1; base += kFarTile) {
         __syncthreads();
-        const uint32_t p = base + threadIdx.x;
-        if (threadIdx.x < kFarTile && p < p1) {
+        const uint32_t pv = base + threadIdx.x;
+        if (threadIdx.x < kFarTile && pv < p1) {
+            const bool img = pv >= q.P;
+            const uint32_t p = img ? pv - q.P : pv;
             float* s = s_pt[threadIdx.x];
-            s[0] = meta[8 * (size_t)p] - q.cx;
-            s[1] = meta[8 * (size_t)p + 1] - q.cy;
-            s[2] = meta[8 * (size_t)p + 2] - q.cz;
+            float pos[3] = {meta[8 * (size_t)p], meta[8 * (size_t)p + 1], meta[8 * (size_t)p + 2]};
             s[3] = meta[8 * (size_t)p + 3];
             s[4] = meta[8 * (size_t)p + 4];
             s[5] = meta[8 * (size_t)p + 5];
+            if (img) {
+                pos[q.mdir] = 2.0f * q.mpos - pos[q.mdir];
+                if ((int)s[3] == q.mdir) s[4] *= q.msign;
+            }
+            s[0] = pos[0] - q.cx;
+            s[1] = pos[1] - q.cy;
+            s[2] = pos[2] - q.cz;
 #pragma unroll
             for (int c = 0; c < 3; ++c) {
                 const float* e = acc_f + ((size_t)c * q.P + p) * NC;
                 const float* hh = acc_f + ((size_t)rowsE + (size_t)c * q.P + p) * NC;
-                s[6 + 2 * c] = e[0];
-                s[7 + 2 * c] = e[1];
-                s[12 + 2 * c] = hh[0];
-                s[13 + 2 * c] = hh[1];
+                const float fe = img ? q.ef[c] : 1.0f, fh = img ? q.hf[c] : 1.0f;
+                s[6 + 2 * c] = e[0] * fe;
+                s[7 + 2 * c] = e[1] * fe;
+                s[12 + 2 * c] = hh[0] * fh;
+                s[13 + 2 * c] = hh[1] * fh;
             }
         }
         __syncthreads();

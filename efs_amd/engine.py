@@ -62,6 +62,8 @@ SIGNATURES = {
     "efs_read_nf2ff": (C.c_int, [C.c_void_p, _f32p, _f32p]),
     "efs_farfield": (C.c_int, [C.c_void_p, C.c_uint32, _f32p, _f32p, C.c_uint32, _f32p, C.c_uint32, _f64p,
                                C.c_double, C.c_double, _f32p]),
+    "efs_farfield_mirror": (C.c_int, [C.c_void_p, C.c_uint32, _f32p, _f32p, C.c_uint32, _f32p, C.c_uint32, _f64p,
+                                      C.c_double, C.c_double, C.c_int, C.c_double, C.c_int, C.c_int, _f32p]),
     "efs_energy": (C.c_int, [C.c_void_p, C.POINTER(C.c_double)]),
     "efs_profile_enable": (C.c_int, [C.c_void_p, C.c_int]),
     "efs_profile_read": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_uint64)]),
@@ -455,12 +457,24 @@ class HIPEngine:
 
     def computeNF2FFfarField(self, config):
         """computeNF2FFfarField (src/webgpu-engine.mjs:2871-2967): 5 floats
-        per angle (P_rad, Et_re, Et_im, Ep_re, Ep_im)."""
+        per angle (P_rad, Et_re, Et_im, Ep_re, Ep_im).  Optional
+        config["mirror"] = {"type": "PEC"|"PMC", "direction": 0..2,
+        "position": p, "referenceRule": bool} adds the surface's image in
+        that plane (src/nf2ff.mjs:377-398; efs_farfield_mirror)."""
         th, ph = f32(config["theta"]), f32(config["phi"])
         out = np.empty(5 * len(th) * len(ph), dtype=np.float32)
-        self._check(self.lib.efs_farfield(self.ctx, int(config.get("freqIndex", 0)), f32(config["pointMeta"]), th,
-                                          len(th), ph, len(ph), f64(config["center"]), float(config["frequency"]),
-                                          float(config["radius"]), out))
+        m = config.get("mirror")
+        if m is None:
+            self._check(self.lib.efs_farfield(self.ctx, int(config.get("freqIndex", 0)), f32(config["pointMeta"]),
+                                              th, len(th), ph, len(ph), f64(config["center"]),
+                                              float(config["frequency"]), float(config["radius"]), out))
+            return out
+        mtype = {"PEC": 0, "PMC": 1}[str(m.get("type", "PEC")).upper()]
+        self._check(self.lib.efs_farfield_mirror(self.ctx, int(config.get("freqIndex", 0)), f32(config["pointMeta"]),
+                                                 th, len(th), ph, len(ph), f64(config["center"]),
+                                                 float(config["frequency"]), float(config["radius"]),
+                                                 int(m["direction"]), float(m.get("position", 0.0)), mtype,
+                                                 1 if m.get("referenceRule") else 0, out))
         return out
 
     # -- energy ----------------------------------------------------------------

@@ -179,6 +179,19 @@ int efs_farfield(efs_ctx *ctx, uint32_t freq_idx, const float *point_meta, const
                  uint32_t n_theta, const float *phi, uint32_t n_phi, const double *center,
                  double frequency, double radius, float *out);
 
+/* The mirror option of the reference's CPU far field (src/nf2ff.mjs:377-398):
+ * the surface is added again reflected in the plane x[mirror_dir] =
+ * mirror_pos with the image fields of a PEC (mirror_type 0: tangential E and
+ * normal H reversed) or PMC (1: normal E and tangential H reversed) plane.
+ * The reflected copy of a face perpendicular to the plane faces the other
+ * way, so its normal sign is reversed (image theory); reference_rule = 1
+ * keeps the original normals, as the reference does.  mirror_dir = -1 is
+ * efs_farfield. */
+int efs_farfield_mirror(efs_ctx *ctx, uint32_t freq_idx, const float *point_meta, const float *theta,
+                        uint32_t n_theta, const float *phi, uint32_t n_phi, const double *center, double frequency,
+                        double radius, int mirror_dir, double mirror_pos, int mirror_type, int reference_rule,
+                        float *out);
+
 /* computeEnergy() (src/webgpu-engine.mjs:2974-3158): sum of volt^2+curr^2
  * over the (nx-1)(ny-1)(nz-1) interior, all components; FP32 squares,
  * FP64 reduction.  Synchronous (returns the current value, not the

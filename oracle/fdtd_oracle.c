@@ -734,10 +734,16 @@ void PFX(read_nf2ff)(const orc *e, double *accE, double *accH) {
 /* ------------------------------------------------------------------ */
 
 /* accE/accH: 6P doubles (one frequency); meta: 8 per point (pos xyz,
- * normalDir, normSign, area, pad, pad); out: 5 per angle. */
-void PFX(farfield)(const double *accE, const double *accH, const double *meta, uint32_t P,
-                   const double *theta, uint32_t nT, const double *phi, uint32_t nP,
-                   const double center[3], double freq, double radius, double *out) {
+ * normalDir, normSign, area, pad, pad); out: 5 per angle.
+ * Mirror (mdir >= 0; src/nf2ff.mjs:377-398): the surface is added a second
+ * time reflected in the plane x[mdir] = mpos with the image fields of a PEC
+ * (mtype 0: tangential E and normal H reversed) or PMC (mtype 1: normal E
+ * and tangential H reversed) plane.  The reflected copy of a face
+ * perpendicular to the plane faces the other way, so its normal sign is
+ * reversed; refrule = 1 keeps it, as the reference does. */
+static void farfield_core(const double *accE, const double *accH, const double *meta, uint32_t P,
+                          const double *theta, uint32_t nT, const double *phi, uint32_t nP, const double center[3],
+                          double freq, double radius, int mdir, double mpos, int mtype, int refrule, double *out) {
     const double C0 = 299792458.0;
     const double Z0 = 376.73031346177066;
     const double k = 2 * M_PI * freq / C0;
@@ -745,16 +751,35 @@ void PFX(farfield)(const double *accE, const double *accH, const double *meta, u
     const double fph = -k * radius;
     const double fre = fmag * (-sin(fph));
     const double fim = fmag * cos(fph);
+    double ef[3] = {1, 1, 1}, hf[3] = {1, 1, 1};
+    if (mdir >= 0) {
+        const int a = (mdir + 1) % 3, b = (mdir + 2) % 3;
+        if (mtype == 0) { hf[mdir] = -1; ef[a] = -1; ef[b] = -1; }
+        else { ef[mdir] = -1; hf[a] = -1; hf[b] = -1; }
+    }
+    const int passes = mdir >= 0 ? 2 : 1;
 #pragma omp parallel for schedule(dynamic, 4)
     for (long a = 0; a < (long)nT * nP; ++a) {
         const double th = theta[a / nP], ph = phi[a % nP];
         const double sT = sin(th), cT = cos(th), sP = sin(ph), cP = cos(ph);
         double Ntr = 0, Nti = 0, Npr = 0, Npi = 0, Ltr = 0, Lti = 0, Lpr = 0, Lpi = 0;
+        for (int pass = 0; pass < passes; ++pass)
         for (uint32_t p = 0; p < P; ++p) {
             const double *m = meta + 8 * (size_t)p;
             const int nd = (int)m[3];
-            const double s = m[4], area = m[5];
-            const double *E = accE + 6 * (size_t)p, *H = accH + 6 * (size_t)p;
+            double s = m[4];
+            const double area = m[5];
+            double pos[3] = {m[0], m[1], m[2]};
+            double E[6], H[6];
+            for (int c = 0; c < 3; ++c)
+                for (int r = 0; r < 2; ++r) {
+                    E[2 * c + r] = accE[6 * (size_t)p + 2 * c + r] * (pass ? ef[c] : 1.0);
+                    H[2 * c + r] = accH[6 * (size_t)p + 2 * c + r] * (pass ? hf[c] : 1.0);
+                }
+            if (pass) {
+                pos[mdir] = 2 * mpos - pos[mdir];
+                if (nd == mdir && !refrule) s = -s;
+            }
             double J[3][2] = {{0, 0}, {0, 0}, {0, 0}}, M[3][2] = {{0, 0}, {0, 0}, {0, 0}};
             for (int r = 0; r < 2; ++r) {
                 if (nd == 0) {
@@ -768,7 +793,7 @@ void PFX(farfield)(const double *accE, const double *accH, const double *meta, u
                     M[0][r] = s * E[2 + r]; M[1][r] = -s * E[0 + r];
                 }
             }
-            const double rc = (m[0] - center[0]) * sT * cP + (m[1] - center[1]) * sT * sP + (m[2] - center[2]) * cT;
+            const double rc = (pos[0] - center[0]) * sT * cP + (pos[1] - center[1]) * sT * sP + (pos[2] - center[2]) * cT;
             const double phs = k * rc;
             const double aer = area * cos(phs), aei = area * sin(phs);
             double Jt[2], Jp[2], Mt[2], Mp[2];
@@ -791,6 +816,19 @@ void PFX(farfield)(const double *accE, const double *accH, const double *meta, u
         o[0] = (Etr * Etr + Eti * Eti + Epr * Epr + Epi * Epi) / (2.0 * Z0);
         o[1] = Etr; o[2] = Eti; o[3] = Epr; o[4] = Epi;
     }
+}
+
+void PFX(farfield)(const double *accE, const double *accH, const double *meta, uint32_t P,
+                   const double *theta, uint32_t nT, const double *phi, uint32_t nP,
+                   const double center[3], double freq, double radius, double *out) {
+    farfield_core(accE, accH, meta, P, theta, nT, phi, nP, center, freq, radius, -1, 0.0, 0, 0, out);
+}
+
+void PFX(farfield_mirror)(const double *accE, const double *accH, const double *meta, uint32_t P,
+                          const double *theta, uint32_t nT, const double *phi, uint32_t nP,
+                          const double center[3], double freq, double radius, int mdir, double mpos, int mtype,
+                          int refrule, double *out) {
+    farfield_core(accE, accH, meta, P, theta, nT, phi, nP, center, freq, radius, mdir, mpos, mtype, refrule, out);
 }
 
 /* ------------------------------------------------------------------ */

@@ -75,6 +75,8 @@ def _load():
         f("read_nf2ff", None, vp, _f64p, _f64p)
         f("farfield", None, _f64p, _f64p, _f64p, C.c_uint32, _f64p, C.c_uint32, _f64p, C.c_uint32,
           _f64p, C.c_double, C.c_double, _f64p)
+        f("farfield_mirror", None, _f64p, _f64p, _f64p, C.c_uint32, _f64p, C.c_uint32, _f64p, C.c_uint32,
+          _f64p, C.c_double, C.c_double, C.c_int, C.c_double, C.c_int, C.c_int, _f64p)
         f("energy", C.c_double, vp)
         f("cfg_gather_buffered", None, vp, _u32p, C.c_uint32, C.c_uint32, C.c_uint32)
         f("read_gather_buffered", C.c_uint32, vp, _f64p)
@@ -356,13 +358,21 @@ class Oracle:
         self._f("iterate")(self.h, int(n))
 
 
-def farfield(accE, accH, pointMeta, theta, phi, center, frequency, radius, fp64=True):
-    """Far-field per (theta, phi): 5 values [P_rad, Et_re, Et_im, Ep_re, Ep_im]."""
+def farfield(accE, accH, pointMeta, theta, phi, center, frequency, radius, fp64=True, mirror=None):
+    """Far-field per (theta, phi): 5 values [P_rad, Et_re, Et_im, Ep_re, Ep_im].
+    mirror: {"type": "PEC"|"PMC", "direction": 0..2, "position": p,
+    "referenceRule": bool} adds the image of the surface in that plane."""
     lib = _load()
-    fn = lib.orc64_farfield if fp64 else lib.orc32_farfield
     P = len(pointMeta) // 8
     th, ph = _f64(theta), _f64(phi)
     out = np.zeros(5 * len(th) * len(ph))
-    fn(_f64(accE), _f64(accH), _f64(pointMeta), P, th, len(th), ph, len(ph), _f64(center),
-       float(frequency), float(radius), out)
+    if mirror is None:
+        fn = lib.orc64_farfield if fp64 else lib.orc32_farfield
+        fn(_f64(accE), _f64(accH), _f64(pointMeta), P, th, len(th), ph, len(ph), _f64(center),
+           float(frequency), float(radius), out)
+        return out
+    fn = lib.orc64_farfield_mirror if fp64 else lib.orc32_farfield_mirror
+    fn(_f64(accE), _f64(accH), _f64(pointMeta), P, th, len(th), ph, len(ph), _f64(center), float(frequency),
+       float(radius), int(mirror["direction"]), float(mirror.get("position", 0.0)),
+       0 if str(mirror.get("type", "PEC")).upper() == "PEC" else 1, 1 if mirror.get("referenceRule") else 0, out)
     return out

@@ -13,6 +13,7 @@ Pins:
   2-degree sphere grid is 91 x 181 (the patch case's far-field grid).
 """
 import numpy as np
+import pytest
 
 from efs_amd import farfield, operator as op
 
@@ -115,3 +116,71 @@ def test_circular_components(oracle_mod):
     assert min(rh_p, lh_p) < 1e-9 * max(rh_p, lh_p)
     assert min(rh_m, lh_m) < 1e-9 * max(rh_m, lh_m)
     assert (rh_p > lh_p) != (rh_m > lh_m)
+
+
+def _dipole_fields(P, p_hat, src, k):
+    """Exact fields (near and far zone, exp(+j w t)) of a unit current element
+    I dl = 1 along p_hat at src."""
+    z0 = 376.73031346177066
+    r = P - src
+    R = np.linalg.norm(r, axis=1)[:, None]
+    rh = r / R
+    pr = (rh @ p_hat)[:, None]
+    e = np.exp(-1j * k * R)
+    E = (z0 / (4 * np.pi)) * e * (-(1j * k / R + 1 / R ** 2 + 1 / (1j * k * R ** 3)) * (p_hat[None, :] - pr * rh)
+                                  + 2 * (1 / R ** 2 + 1 / (1j * k * R ** 3)) * pr * rh)
+    H = (1 / (4 * np.pi)) * e * (1j * k / R + 1 / R ** 2) * np.cross(p_hat[None, :], rh)
+    return E, H
+
+
+def _box(lo, hi, n):
+    pts, meta = [], []
+    for ax in range(3):
+        a1, a2 = (ax + 1) % 3, (ax + 2) % 3
+        for side, sgn in ((lo[ax], -1.0), (hi[ax], 1.0)):
+            u, v = np.linspace(lo[a1], hi[a1], n), np.linspace(lo[a2], hi[a2], n)
+            du, dv = np.full(n, u[1] - u[0]), np.full(n, v[1] - v[0])
+            du[[0, -1]] /= 2
+            dv[[0, -1]] /= 2
+            U, V = np.meshgrid(u, v, indexing="ij")
+            p = np.zeros((n * n, 3))
+            p[:, ax], p[:, a1], p[:, a2] = side, U.ravel(), V.ravel()
+            m = np.zeros((n * n, 8))
+            m[:, :3], m[:, 3], m[:, 4], m[:, 5] = p, ax, sgn, np.outer(du, dv).ravel()
+            pts.append(p)
+            meta.append(m)
+    return np.concatenate(pts), np.concatenate(meta)
+
+
+def _acc(F):
+    return np.stack([F.real, F.imag], axis=2).ravel()
+
+
+@pytest.mark.parametrize("kind", ["PEC", "PMC"])
+@pytest.mark.parametrize("pdir", [(0.0, 0.0, 1.0), (1.0, 0.0, 0.0)])
+def test_mirror_is_image_theory(oracle_mod, kind, pdir):
+    """A current element at height h above the plane z = 0, its fields on a
+    closed box above the plane, far field with a PEC / PMC mirror: equal to
+    the far field of the element and its image (PEC: horizontal currents
+    reversed; PMC: vertical reversed) from a box enclosing both, to 1e-3 of
+    the peak.  The reference's rule (mirrored faces keep their normals,
+    src/nf2ff.mjs:377-398) misses this by up to half the peak."""
+    f = 1e9
+    k = 2 * np.pi * f / 299792458.0
+    h, a = 0.06, 0.03
+    p_hat = np.array(pdir)
+    theta, phi = np.linspace(0.01, np.pi - 0.01, 25), np.array([0.0, np.pi / 4, np.pi / 2])
+    pts, meta = _box([-a, -a, h - a], [a, a, h + a], 31)
+    E, H = _dipole_fields(pts, p_hat, np.array([0, 0, h]), k)
+    img = np.array([-p_hat[0], -p_hat[1], p_hat[2]]) if kind == "PEC" else np.array([p_hat[0], p_hat[1], -p_hat[2]])
+    pts2, meta2 = _box([-a, -a, -h - a], [a, a, h + a], 61)
+    E1, H1 = _dipole_fields(pts2, p_hat, np.array([0, 0, h]), k)
+    E2, H2 = _dipole_fields(pts2, img, np.array([0, 0, -h]), k)
+    ref = oracle_mod.farfield(_acc(E1 + E2), _acc(H1 + H2), meta2.ravel(), theta, phi, np.zeros(3), f, 1.0)
+    mir = {"type": kind, "direction": 2, "position": 0.0}
+    got = oracle_mod.farfield(_acc(E), _acc(H), meta.ravel(), theta, phi, np.zeros(3), f, 1.0, mirror=mir)
+    U, Ur = got.reshape(-1, 5)[:, 0], ref.reshape(-1, 5)[:, 0]
+    assert np.max(np.abs(U - Ur)) < 1e-3 * Ur.max()
+    old = oracle_mod.farfield(_acc(E), _acc(H), meta.ravel(), theta, phi, np.zeros(3), f, 1.0,
+                              mirror=dict(mir, referenceRule=True)).reshape(-1, 5)[:, 0]
+    assert np.max(np.abs(old - Ur)) > 0.02 * Ur.max()
