@@ -273,10 +273,21 @@ export class HIPEngine {
         return { accumE: split(accumE), accumH: split(accumH) };
     }
 
+    // Optional config.mirror = { type: 'PEC' | 'PMC', direction, position,
+    // referenceRule }: the surface's image in that plane (the mirror option of
+    // src/nf2ff.mjs:377-398, with image-theory normals unless referenceRule).
     async computeNF2FFfarField(config) {
         const out = new Float32Array(5 * config.theta.length * config.phi.length);
-        addon.farfield(this._h, (config.freqIndex || 0) >>> 0, f32(config.pointMeta), f32(config.theta), f32(config.phi),
-            f64(config.center), config.frequency, config.radius, out);
+        const m = config.mirror;
+        if (!m) {
+            addon.farfield(this._h, (config.freqIndex || 0) >>> 0, f32(config.pointMeta), f32(config.theta),
+                f32(config.phi), f64(config.center), config.frequency, config.radius, out);
+        } else {
+            addon.farfield(this._h, (config.freqIndex || 0) >>> 0, f32(config.pointMeta), f32(config.theta),
+                f32(config.phi), f64(config.center), config.frequency, config.radius, out,
+                m.direction, m.position || 0, String(m.type || 'PEC').toUpperCase() === 'PMC' ? 1 : 0,
+                m.referenceRule ? 1 : 0);
+        }
         return out;
     }
 

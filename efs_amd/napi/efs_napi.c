@@ -515,7 +515,8 @@ static napi_value js_read_nf2ff(napi_env env, napi_callback_info info) {
     return status(&c, ctx, efs_read_nf2ff(ctx, e, h));
 }
 
-/* farfield(h, fIdx, meta, theta, phi, center(F64[3]), freq, radius, out) */
+/* farfield(h, fIdx, meta, theta, phi, center(F64[3]), freq, radius, out
+ *          [, mirrorDir, mirrorPos, mirrorType, referenceRule]) */
 static napi_value js_farfield(napi_env env, napi_callback_info info) {
     BEGIN();
     CTX(0);
@@ -525,12 +526,20 @@ static napi_value js_farfield(napi_env env, napi_callback_info info) {
     double *ce = F64(5, lc);
     double freq = f64_arg(&c, 6), rad = f64_arg(&c, 7);
     float *out = F32(8, lo);
+    const int mirror = c.argc > 9;
+    const int mdir = mirror ? (int)f64_arg(&c, 9) : -1;
+    const double mpos = mirror ? f64_arg(&c, 10) : 0.0;
+    const int mtype = mirror ? (int)f64_arg(&c, 11) : 0;
+    const int refrule = mirror ? (int)f64_arg(&c, 12) : 0;
     BAIL();
     if (lc != 3 || lo != 5 * lt * lp) {
         throwf(&c, "efs: far-field center must hold 3 values and out 5 per angle");
         return NULL;
     }
-    return status(&c, ctx, efs_farfield(ctx, f, meta, th, (uint32_t)lt, ph, (uint32_t)lp, ce, freq, rad, out));
+    if (!mirror)
+        return status(&c, ctx, efs_farfield(ctx, f, meta, th, (uint32_t)lt, ph, (uint32_t)lp, ce, freq, rad, out));
+    return status(&c, ctx, efs_farfield_mirror(ctx, f, meta, th, (uint32_t)lt, ph, (uint32_t)lp, ce, freq, rad,
+                                               mdir, mpos, mtype, refrule, out));
 }
 
 static napi_value js_energy(napi_env env, napi_callback_info info) {
