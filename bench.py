@@ -35,7 +35,8 @@ def sweep_bytes(n_cells, n_pml, storage="fp32"):
     return per_cell * n_cells + 60.0 * n_pml
 
 
-TRAFFIC_FILE = os.path.join(ROOT, "profiles", "fused_sweeps_256cube_stress_traffic.json")
+TRAFFIC_FILES = {"fp32": os.path.join(ROOT, "profiles", "fused_sweeps_256cube_stress_traffic.json"),
+                 "fp16": os.path.join(ROOT, "profiles", "fused_sweeps_fp16_256cube_stress_traffic.json")}
 
 
 def sweep_kernel_symbol(flags, side):
@@ -52,12 +53,13 @@ def sweep_kernel_symbol(flags, side):
     return f"efs::sweep_{side}4{suffix}" + ("_h" if "f16" in flags else "")
 
 
-def measured_traffic(kernel, size):
+def measured_traffic(kernel, size, storage="fp32"):
     """HBM bytes per launch of `kernel` from the committed rocprofv3
     FETCH_SIZE / WRITE_SIZE measurement of this workload (None if absent)."""
-    if size != 256 or not os.path.exists(TRAFFIC_FILE):
+    path = TRAFFIC_FILES.get(storage)
+    if size != 256 or not path or not os.path.exists(path):
         return None
-    with open(TRAFFIC_FILE) as f:
+    with open(path) as f:
         t = json.load(f)
     return t.get(kernel, {}).get("hbm_bytes_per_launch")
 
@@ -709,7 +711,7 @@ def main():
     gbs_h = b / (avg_h * 1e-3) / 1e9
     dom_name, dom_ms = ("sweep_h", avg_h) if avg_h >= avg_e else ("sweep_e", avg_e)
     dom_sym = sweep_kernel_symbol(eng.pathFlags(), dom_name[-1])
-    traffic = measured_traffic(dom_sym, args.size)
+    traffic = measured_traffic(dom_sym, args.size, args.storage)
     dom_gbs = b / (dom_ms * 1e-3) / 1e9
     acc = eng.readNF2FFAccumulation()
     finite = bool(np.all(np.isfinite(acc["accumE"])) and np.all(np.isfinite(energies)))
@@ -766,7 +768,7 @@ def main():
             "unit": "GB/s",
             "frac": round(dom_gbs / HBM_PEAK_GBS, 4),
             "traffic": round(traffic) if traffic else None,
-            "traffic_source": (os.path.relpath(TRAFFIC_FILE, ROOT) + f" [{dom_sym}]") if traffic else None,
+            "traffic_source": (os.path.relpath(TRAFFIC_FILES[args.storage], ROOT) + f" [{dom_sym}]") if traffic else None,
             "algorithmic_bytes_per_launch": b,
             "avg_launch_ms": round(dom_ms, 5),
             "kernels": {"sweep_e": {"avg_ms": round(avg_e, 5), "GB/s": round(gbs_e, 1)},
