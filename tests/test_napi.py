@@ -101,3 +101,49 @@ def test_js_batch_matches_python_batch(tmp_path):
         assert got["instances"][i]["numSamples"] == g["numSamples"] == steps
         np.testing.assert_array_equal(np.asarray(got["instances"][i]["gather"], np.float32), g["data"])
     b.destroy()
+
+
+@needs_node
+@pytest.mark.gpu
+def test_js_farfield_mirror(tmp_path):
+    """The far field through the JS class, with and without a mirror plane,
+    gives the bits of the Python binding of the same library."""
+    from efs_amd import engine, operator as op
+    n = 24
+    mesh = op.uniform_mesh((n, n, n), 1e-3)
+    dT = mesh.timestep()
+    mat = op.Materials(mesh)
+    op.pec_boundary(mat)
+    port, _ = op.lumped_port(mesh, mat, 2, (12, 12, 11), (12, 12, 13), 50.0, excite=100.0)
+    co = op.to_f32(op.build_coefficients(mesh, mat, dT))
+    exc = {"signal": op.gaussian_pulse(20e9, 15e9, dT), "delay": [0] * len(port["pos"]), "amp": port["amp"],
+           "dir": port["dir"], "pos": port["pos"], "period": 0}
+    pts, meta, primal, dual = op.nf2ff_box(mesh, [4] * 3, [19] * 3)
+    freq = 15e9
+    nf = {"surfaceIndices": pts, "numPoints": len(pts) // 3, "omega": 2 * np.pi * freq, "dT": dT,
+          "primalEdgeLens": primal, "dualEdgeLens": dual}
+    theta = np.linspace(0, np.pi, 13)
+    phi = np.linspace(0, 2 * np.pi, 12, endpoint=False)
+    mirror = {"type": "PEC", "direction": 2, "position": 2e-3}
+    case = {"dims": [n] * 3, "coeffs": co, "excitation": exc, "nf2ff": nf, "steps": 120, "pointMeta": meta,
+            "theta": theta, "phi": phi, "center": [12e-3] * 3, "frequency": freq, "mirror": mirror}
+    src, dst = tmp_path / "ff.json", tmp_path / "out.json"
+    src.write_text(json.dumps(_jsonable(case)))
+    print("node farfield", flush=True)
+    res = subprocess.run([NODE, os.path.join(ROOT, "tests", "js", "run_farfield.mjs"), str(src), str(dst)],
+                         capture_output=True, text=True, timeout=120, cwd=ROOT)
+    assert res.returncode == 0, res.stderr
+    got = json.loads(dst.read_text())
+    g = engine.HIPEngine()
+    assert g.initGPU()
+    g.init([n] * 3, co)
+    g.configureExcitation(exc)
+    g.configureNF2FFAccumulation(nf)
+    g.iterate(120)
+    cfg = {"pointMeta": meta, "theta": theta, "phi": phi, "center": [12e-3] * 3, "frequency": freq, "radius": 1.0}
+    plain = g.computeNF2FFfarField(cfg)
+    mirrored = g.computeNF2FFfarField(dict(cfg, mirror=mirror))
+    g.destroy()
+    np.testing.assert_array_equal(np.asarray(got["plain"], np.float32), plain)
+    np.testing.assert_array_equal(np.asarray(got["mirrored"], np.float32), mirrored)
+    assert np.max(np.abs(mirrored - plain)) > 1e-3 * np.max(np.abs(plain))
