@@ -117,6 +117,13 @@ class BatchEngine:
         return {"blocks": b.value, "waves": w.value, "graph_steps": g.value, "cells": c.value,
                 "instances": len(self._grids)}
 
+    def murSplit(self):
+        """Mur entries applied by the point kernels ("listed") and by the E
+        sweep itself ("in_sweep", z-face entries)."""
+        a, s = C.c_uint32(), C.c_uint32()
+        self._check(self.lib.efs_batch_mur_split(self.h, C.byref(a), C.byref(s)))
+        return {"listed": a.value, "in_sweep": s.value}
+
     def readProbeGatherBuffered(self, i):
         """{data, numSamples, stride} of instance i (readProbeGatherBuffered
         form, src/webgpu-engine.mjs:2689-2715)."""

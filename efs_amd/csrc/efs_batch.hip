@@ -29,7 +29,9 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdarg>
+#include <cstdint>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <string>
 #include <vector>
@@ -55,6 +57,7 @@ struct Inst {
     uint32_t nx, ny, nz;  // nz padded to a multiple of 4
     uint32_t cols;      // ny * nz
     uint32_t tabn_e, tabn_h;  // table rows per component; 0 = FP32 arrays
+    ZMur zm;                  // z-face Mur done in the E sweep
 };
 
 struct Block {
@@ -102,16 +105,17 @@ __device__ __forceinline__ void bsweep_body(const Inst* __restrict__ insts, cons
                                                            NfCapture{});
         }
     } else {
+        const ZMur& zm = I.zm;
         if (I.tabn_e) {
             const Coef cf{nullptr, nullptr, nullptr, I.te, nullptr, nullptr, I.ie, I.tabn_e};
             e4_march<kMarchInterior, kCoefIdx16, false, T>(static_cast<T*>(I.volt), static_cast<const T*>(I.curr),
                                                            cf, nullptr, nullptr, g, pml, nullptr, nullptr, w, 0u, sc,
-                                                           NfCapture{});
+                                                           NfCapture{}, zm);
         } else {
             const Coef cf{I.vv, I.vi, nullptr, nullptr, nullptr, nullptr, nullptr, 0u};
             e4_march<kMarchInterior, kCoefF32, false, T>(static_cast<T*>(I.volt), static_cast<const T*>(I.curr), cf,
                                                          nullptr, nullptr, g, pml, nullptr, nullptr, w, 0u, sc,
-                                                           NfCapture{});
+                                                         NfCapture{}, zm);
         }
     }
 }
@@ -330,6 +334,7 @@ struct HostInst {
     uint16_t *ie = nullptr, *ih = nullptr;
     float2 *te = nullptr, *th = nullptr;
     uint32_t tabn_e = 0, tabn_h = 0;
+    ZMur zm;  // z-face Mur of the E sweep (set at finalize)
     // configured point lists (instance-local device indices)
     std::vector<float> sig;
     std::vector<ExcPoint> exc;  // off relative to this instance's signal
@@ -369,6 +374,7 @@ struct efs_batch {
 
     // Mur
     uint32_t mur_n = 0, mur_ne = 0;
+    uint32_t mur_swept = 0;  // z-face entries the E sweep applies
     MurPt* d_mur = nullptr;
     uint32_t* d_mur_early = nullptr;
     float* d_mur_saved = nullptr;
@@ -949,6 +955,78 @@ int efs_batch_cfg_gather(efs_batch* b, uint32_t id, const uint32_t* idx, uint32_
     return EFS_OK;
 }
 
+struct MurKeep {
+    uint32_t k;
+    bool early;
+    bool fused = false;
+};
+
+// Which kept Mur entries of an instance the E sweep applies itself (ZMur):
+// tangential x / y entries of the z = 0 face (shifted cell z = 1) and of the
+// z = Nz-1 face (shifted cell z = Nz-2, in the same 4-cell lane: (Nz-1) % 4
+// != 0; Nz >= 6), not posting early, whose normal cell is neither an excitation cell
+// nor another kept entry's shifted cell.  Nothing writes those cells between
+// the sweep and the apply kernel, so the sweep's fused update gives the
+// list's numbers.  A (face, component) slot goes to the sweep only when its
+// candidates share one coefficient and fill an x-y rectangle exactly (the
+// usual whole-face lists do); otherwise its entries stay in the list.
+// EFS_BATCH_ZMUR=0 keeps every entry in the list.
+void zmur_split(HostInst& h, std::vector<MurKeep>& keep, bool on) {
+    h.zm = ZMur{};
+    if (!on || keep.empty() || h.nz < 6) return;
+    std::vector<uint32_t> shifted, exc;
+    for (const MurKeep& e : keep) shifted.push_back(h.mur_s[e.k]);
+    for (const ExcPoint& p : h.exc) exc.push_back(p.gidx);
+    std::sort(shifted.begin(), shifted.end());
+    std::sort(exc.begin(), exc.end());
+    std::vector<uint32_t> cand[4];  // positions in keep
+    std::vector<uint64_t> row(keep.size());
+    for (uint32_t j = 0; j < (uint32_t)keep.size(); ++j) {
+        const MurKeep& e = keep[j];
+        if (e.early) continue;
+        const uint32_t ni = h.mur_n[e.k], si = h.mur_s[e.k];
+        const uint64_t n = ni / h.S, r = ni - n * h.S, z = r % h.nzp, xy = r / h.nzp;
+        const uint64_t rs = si - n * h.S;  // same component (checked at configure)
+        if (n > 1 || rs / h.nzp != xy) continue;
+        const uint64_t zs = rs % h.nzp;
+        int f = -1;
+        if (z == 0 && zs == 1)
+            f = 0;
+        else if (z == h.nz - 1 && zs == h.nz - 2 && (h.nz - 1) % 4 != 0)
+            f = 1;
+        if (f < 0) continue;
+        if (std::binary_search(shifted.begin(), shifted.end(), ni) || std::binary_search(exc.begin(), exc.end(), ni))
+            continue;
+        row[j] = xy;
+        cand[2 * f + n].push_back(j);
+    }
+    for (int s = 0; s < 4; ++s) {
+        if (cand[s].empty()) continue;
+        const float c = h.mur_c[keep[cand[s][0]].k];
+        int64_t x0 = INT64_MAX, x1 = -1, y0 = INT64_MAX, y1 = -1;
+        bool same = true;
+        for (uint32_t j : cand[s]) {
+            const float cj = h.mur_c[keep[j].k];
+            same = same && std::memcmp(&cj, &c, sizeof c) == 0;
+            const int64_t x = (int64_t)(row[j] / h.ny), y = (int64_t)(row[j] % h.ny);
+            x0 = std::min(x0, x);
+            x1 = std::max(x1, x);
+            y0 = std::min(y0, y);
+            y1 = std::max(y1, y);
+        }
+        // normal cells are distinct, so a count equal to the area fills it
+        if (!same || (uint64_t)(x1 - x0 + 1) * (uint64_t)(y1 - y0 + 1) != cand[s].size()) continue;
+        h.zm.on = 1;
+        h.zm.c[s] = c;
+        h.zm.x0[s] = (int32_t)x0;
+        h.zm.x1[s] = (int32_t)x1;
+        h.zm.y0[s] = (int32_t)y0;
+        h.zm.y1[s] = (int32_t)y1;
+        for (uint32_t j : cand[s]) keep[j].fused = true;
+    }
+    h.zm.nzr = h.nz;
+}
+
 int efs_batch_finalize(efs_batch* b, uint32_t sample_interval, uint32_t max_samples) {
     BTRY(need_open(b));
     if (b->inst.empty()) return bfail(b, EFS_ERR_STATE, "no instance added");
@@ -962,11 +1040,13 @@ int efs_batch_finalize(efs_batch* b, uint32_t sample_interval, uint32_t max_samp
     std::vector<uint32_t> mur_early;
     std::vector<float> sig;
     std::vector<ExcPoint> exc;
+    const char* zenv = std::getenv("EFS_BATCH_ZMUR");
+    const bool zmur = !(zenv && zenv[0] == '0');
+    uint32_t swept = 0;
     b->g_off.assign(ni + 1, 0);
     for (uint32_t q = 0; q < ni; ++q) {
         HostInst& h = b->inst[q];
-        it[q] = Inst{h.volt, h.curr, h.vv, h.vi, h.ii, h.iv, h.ie, h.ih, h.te, h.th,
-                     h.S, h.nx, h.ny, h.nzp, h.ny * h.nzp, h.tabn_e, h.tabn_h};
+
         if (b->half) source_scales(h);
         src_end[q] = h.src_end;
         for (uint32_t k = 0; k < 6; ++k)
@@ -981,6 +1061,7 @@ int efs_batch_finalize(efs_batch* b, uint32_t sample_interval, uint32_t max_samp
             for (uint32_t k = 0; k < nm; ++k) ord[k] = k;
             std::stable_sort(ord.begin(), ord.end(), [&](uint32_t a, uint32_t c) { return h.mur_n[a] < h.mur_n[c]; });
             std::vector<uint32_t> written;
+            std::vector<MurKeep> keep;
             for (uint32_t j = 0; j < nm; ++j) written.push_back(h.mur_n[ord[j]]);
             for (const ExcPoint& e : h.exc) written.push_back(e.gidx);
             std::sort(written.begin(), written.end());
@@ -988,10 +1069,19 @@ int efs_batch_finalize(efs_batch* b, uint32_t sample_interval, uint32_t max_samp
                 const uint32_t k = ord[j];
                 if (j + 1 < nm && h.mur_n[ord[j + 1]] == h.mur_n[k]) continue;
                 const bool early = std::binary_search(written.begin(), written.end(), h.mur_s[k]);
-                if (early) mur_early.push_back((uint32_t)mur.size());
-                mur.push_back(MurPt{h.mur_n[k], h.mur_s[k], h.mur_c[k], q | (early ? kMurEarly : 0u)});
+                keep.push_back({k, early});
+            }
+            zmur_split(h, keep, zmur);
+            for (const auto& e : keep) swept += e.fused ? 1u : 0u;
+            for (const auto& e : keep) {
+                if (e.fused) continue;
+                if (e.early) mur_early.push_back((uint32_t)mur.size());
+                mur.push_back(MurPt{h.mur_n[e.k], h.mur_s[e.k], h.mur_c[e.k], q | (e.early ? kMurEarly : 0u)});
             }
         }
+        it[q] = Inst{h.volt, h.curr, h.vv, h.vi, h.ii, h.iv, h.ie, h.ih, h.te, h.th,
+                     h.S, h.nx, h.ny, h.nzp, h.ny * h.nzp, h.tabn_e, h.tabn_h,
+                     h.zm};
         const uint32_t off = (uint32_t)sig.size();
         sig.insert(sig.end(), h.sig.begin(), h.sig.end());
         for (ExcPoint p : h.exc) {
@@ -1022,6 +1112,7 @@ int efs_batch_finalize(efs_batch* b, uint32_t sample_interval, uint32_t max_samp
     }
     b->mur_n = (uint32_t)mur.size();
     b->mur_ne = (uint32_t)mur_early.size();
+    b->mur_swept = swept;
     if (b->mur_n) {
         BTRY(bupload(b, b->d_mur, mur.data(), mur.size()));
         BTRY(bupload(b, b->d_mur_early, mur_early.data(), mur_early.size()));
@@ -1165,6 +1256,13 @@ int efs_batch_info(efs_batch* b, uint32_t* blocks, uint32_t* waves, uint32_t* gr
     if (waves) *waves = b->waves;
     if (graph_steps) *graph_steps = b->graph_ok ? kGraphSteps : 0;
     if (cells) *cells = b->cells;
+    return EFS_OK;
+}
+
+int efs_batch_mur_split(efs_batch* b, uint32_t* listed, uint32_t* in_sweep) {
+    BTRY(need_final(b));
+    if (listed) *listed = b->mur_n;
+    if (in_sweep) *in_sweep = b->mur_swept;
     return EFS_OK;
 }
 

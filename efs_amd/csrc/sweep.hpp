@@ -120,6 +120,25 @@ struct SweepGeom {
 #define EFS_PRAGMA_STR(x) EFS_PRAGMA_STR2(x)
 #define EFS_MARCH_UNROLL_PRAGMA _Pragma(EFS_PRAGMA_STR(unroll EFS_MARCH_UNROLL))
 
+// Mur on the z = 0 and z = Nz-1 faces done inside the E march (batched
+// form).  Slot s = 2f + n: face f (0: z = 0, shifted cell z = 1; 1: z =
+// Nz-1, shifted cell z = Nz-2, both in one lane; Nz >= 6 keeps the two faces
+// in different lanes) and tangential component n (0: x, 1: y), over the rows
+// x0[s] <= x <= x1[s], y0[s] <= y <= y1[s] (empty: x0 > x1), coefficient
+// c[s].  The march turns the slots of its row into per-lane registers before
+// the loop, so the scalar copies are dead inside it.  The lane forms the pre value from the
+// cells it loaded before the update and the post value from the shifted cell
+// as stored, with the Mur kernels' fused multiply-adds: the same numbers as
+// pre / post / apply in their own passes, for entries whose cells nothing
+// else touches between sweep and apply.
+struct ZMur {
+    uint32_t on = 0;
+    uint32_t nzr = 0;  // the real Nz (rows are padded)
+    float c[4] = {0.f, 0.f, 0.f, 0.f};
+    int32_t x0[4] = {1, 1, 1, 1}, x1[4] = {0, 0, 0, 0};
+    int32_t y0[4] = {1, 1, 1, 1}, y1[4] = {0, 0, 0, 0};
+};
+
 struct NfCapture {
     float* slab;
     uint32_t on;          // 0: off
@@ -180,6 +199,7 @@ struct Fs<float> {
     __device__ static __forceinline__ void st(float* p, v4 v, float) { st4(p, v); }
     // the value a later load returns for v stored with (s, 1/s)
     __device__ static __forceinline__ v4 stored(v4 v, float, float) { return v; }
+    __device__ static __forceinline__ float stored1(float v, float, float) { return v; }
 };
 // FP16 stores saturate at the largest finite half: a value that outgrew its
 // scale between two range checks is clamped (and the next check rescales)
@@ -202,6 +222,9 @@ struct Fs<f16> {
     }
     __device__ static __forceinline__ v4 stored(v4 v, float s, float inv) {
         return __builtin_convertvector(__builtin_convertvector(clamp16(v * inv), h4), v4) * s;
+    }
+    __device__ static __forceinline__ float stored1(float v, float s, float inv) {
+        return (float)(f16)clamp16(v * inv) * s;
     }
 };
 
@@ -613,7 +636,7 @@ __device__ __forceinline__ void e4_march(T* __restrict__ volt, const T* __restri
                                          const float2* s_tab, const float4* s_ptab, const SweepGeom& g,
                                          const PmlTable& pml,
                                          const float* __restrict__ pcoef, float* __restrict__ pflux, Lane4 w,
-                                         uint32_t m, const Scales& sc, const NfCapture&) {
+                                         uint32_t m, const Scales& sc, const NfCapture&, const ZMur& zm = ZMur{}) {
     typedef Fs<T> L;
     const size_t S = g.S;
     T* const E[3] = {volt, volt + S, volt + 2 * S};
@@ -626,6 +649,33 @@ __device__ __forceinline__ void e4_march(T* __restrict__ volt, const T* __restri
     uint32_t i = w.x0 * plane + w.y * g.nz + w.z0;
     bool carry = false;  // hy_m/hz_m hold H at x-1
     v4 hy_m = 0.f, hz_m = 0.f;
+    // z-face Mur of this lane's row: normal / shifted cell at lane offsets
+    // zo / zs; component n applies when x - zxl[n] <= zxs[n] (unsigned; an
+    // idle component has zxl = 2^31, zxs = 0)
+    bool zany = false;
+    uint32_t zo = 0, zs = 1, zxl[2] = {0x80000000u, 0x80000000u}, zxs[2] = {0u, 0u};
+    float zc[2] = {0.f, 0.f};
+    if (zm.on) {
+        // slots picked with selects between the two faces' (uniform) values,
+        // not by a lane-dependent index into the struct
+        const bool f0 = w.z0 == 0;
+        const bool f1 = !f0 && zm.nzr >= 6 && zm.nzr - 1 >= w.z0 && zm.nzr - 1 < w.z0 + 4 && (zm.nzr - 1) % 4 != 0;
+        if (f1) {
+            zo = zm.nzr - 1 - w.z0;
+            zs = zo - 1;
+        }
+#pragma unroll
+        for (int n = 0; n < 2; ++n) {
+            const int32_t x0 = f1 ? zm.x0[2 + n] : zm.x0[n], x1 = f1 ? zm.x1[2 + n] : zm.x1[n];
+            const int32_t y0 = f1 ? zm.y0[2 + n] : zm.y0[n], y1 = f1 ? zm.y1[2 + n] : zm.y1[n];
+            if ((f0 || f1) && (int32_t)w.y >= y0 && (int32_t)w.y <= y1 && x0 <= x1) {
+                zxl[n] = (uint32_t)x0;
+                zxs[n] = (uint32_t)(x1 - x0);
+                zc[n] = f1 ? zm.c[2 + n] : zm.c[n];
+                zany = true;
+            }
+        }
+    }
     PmlCursor u{-1, m ? 0u : 0xffffffffu, nullptr, nullptr, nullptr, 0u, 0u};
     EFS_MARCH_UNROLL_PRAGMA
     for (uint32_t x = w.x0; x < w.x1; ++x, i += plane) {
@@ -659,8 +709,20 @@ __device__ __forceinline__ void e4_march(T* __restrict__ volt, const T* __restri
         hz_m = hz;
         carry = true;
         if (!upml) {
+            v4 vn[3];
 #pragma unroll
-            for (int n = 0; n < 3; ++n) L::st(E[n] + i, fma4(a[n], V[n], b[n] * cur[n]), sc.own_inv[n]);
+            for (int n = 0; n < 3; ++n) vn[n] = fma4(a[n], V[n], b[n] * cur[n]);
+            if (zany) {
+#pragma unroll
+                for (int n = 0; n < 2; ++n) {
+                    if (x - zxl[n] > zxs[n]) continue;
+                    const float pre = __builtin_fmaf(-zc[n], V[n][zo], V[n][zs]);
+                    const float st = L::stored1(vn[n][zs], sc.own[n], sc.own_inv[n]);
+                    vn[n][zo] = __builtin_fmaf(zc[n], st, pre);
+                }
+            }
+#pragma unroll
+            for (int n = 0; n < 3; ++n) L::st(E[n] + i, vn[n], sc.own_inv[n]);
         } else {
             upml_decode<PCOMP>(s_ptab, pq, P0, P1, P2);
 #pragma unroll

@@ -97,6 +97,12 @@ int efs_batch_get_fields(efs_batch *b, uint32_t id, float *volt, float *curr);
  * kernels, steps per graph replay, total cells (sum over instances). */
 int efs_batch_info(efs_batch *b, uint32_t *blocks, uint32_t *waves, uint32_t *graph_steps, uint64_t *cells);
 
+/* Where the Mur entries went at finalize (no counterpart in the reference; a
+ * diagnostic): entries applied by the point kernels, and z-face entries the
+ * E sweep applies itself (0 with EFS_BATCH_ZMUR=0).  Both give the same
+ * numbers. */
+int efs_batch_mur_split(efs_batch *b, uint32_t *listed, uint32_t *in_sweep);
+
 /* Per-sweep device timing (HIP events on the batch stream): enable, then
  * read the summed milliseconds and counts of the E and H sweep launches
  * made while enabled (graph replays are not split: with profiling on,

@@ -5,6 +5,8 @@
   per-cell fused multiply-adds, the batch only changes the launch shape and
   pads Nz to a multiple of 4 (instances here have Nz % 4 in {0, 2, 3}).
 - Graph replays (blocks of 32 steps) give the same bits as direct launches.
+- Mur on the z faces applied inside the E sweep gives the same bits as the
+  Mur point kernels (FP32 and FP16).
 - The autotuner leaves a reset batch whose results do not depend on the
   chosen block table.
 - FP16 storage against the FP64 oracle per instance: relative L2 of the
@@ -108,6 +110,31 @@ def test_tune_then_same_results(patches):
             np.testing.assert_array_equal(fa[k], ft[k])
     a.destroy()
     t.destroy()
+
+
+@pytest.mark.parametrize("storage", ["fp32", "fp16"])
+def test_zface_mur_in_sweep_same_bits(patches, storage, monkeypatch):
+    """The z-face Mur entries the E sweep applies itself give the same bits
+    as the same entries run through the Mur point kernels
+    (EFS_BATCH_ZMUR=0), in FP32 and FP16 storage."""
+    ws = patches[:3]
+    say("zmur", storage, [w["gridSize"] for w in ws])
+    a, _ = make_batch(ws, storage=storage, steps=120)
+    monkeypatch.setenv("EFS_BATCH_ZMUR", "0")
+    d, _ = make_batch(ws, storage=storage, steps=120)
+    sa, sd = a.murSplit(), d.murSplit()
+    say("mur split", sa, sd)
+    assert sa["in_sweep"] > 0 and sd["in_sweep"] == 0
+    assert sa["listed"] + sa["in_sweep"] == sd["listed"]
+    a.iterate(120)
+    d.iterate(120)
+    for i in range(len(ws)):
+        fa, fd = a.getFields(i), d.getFields(i)
+        for k in ("volt", "curr"):
+            np.testing.assert_array_equal(fa[k], fd[k], err_msg=f"instance {i} {k}")
+        np.testing.assert_array_equal(a.readProbeGatherBuffered(i)["data"], d.readProbeGatherBuffered(i)["data"])
+    a.destroy()
+    d.destroy()
 
 
 def rel_l2(a, b):
