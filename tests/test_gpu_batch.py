@@ -137,6 +137,118 @@ def test_zface_mur_in_sweep_same_bits(patches, storage, monkeypatch):
     d.destroy()
 
 
+def six_face_mur(dims, order, coef):
+    """Mur lists on the six faces of a grid in the given face order: each face
+    (axis a, side) lists its two tangential components over the whole face,
+    shifted one cell inward, with coefficient coef[(a, side)]."""
+    nx, ny, nz = dims
+    N = nx * ny * nz
+    x, y, z = np.meshgrid(np.arange(nx), np.arange(ny), np.arange(nz), indexing="ij")
+    nidx, sidx, co = [], [], []
+    for a, side in order:
+        c = [x, y, z]
+        on = (c[a] == (0 if side == 0 else dims[a] - 1)).ravel()
+        sh = [v.copy() for v in c]
+        sh[a] = sh[a] + (1 if side == 0 else -1)
+        lin_n = (c[0] * ny * nz + c[1] * nz + c[2]).ravel()[on]
+        lin_s = (sh[0] * ny * nz + sh[1] * nz + sh[2]).ravel()[on]
+        for n in range(3):
+            if n == a:
+                continue
+            nidx.append(n * N + lin_n)
+            sidx.append(n * N + lin_s)
+            co.append(np.full(lin_n.size, coef[(a, side)], np.float32))
+    return {"normal_idx": np.concatenate(nidx).astype(np.uint32), "shifted_idx": np.concatenate(sidx).astype(np.uint32),
+            "coeff": np.concatenate(co)}
+
+
+def synthetic_instances():
+    """Instances whose z-face Mur entries take every branch of the split:
+    both faces in the sweep (Nz = 14), the Nz-1 face left in the lists
+    ((Nz-1) % 4 == 0, Nz = 13), a face with an excitation cell on it and a
+    face with one odd coefficient (both stay listed), Nz = 5 (nothing in the
+    sweep) and the z faces listed first (the x and y faces then win the
+    shared edge cells)."""
+    import cases
+    faces = [(a, s) for a in range(3) for s in (0, 1)]
+    out = []
+    for k, (dims, order) in enumerate([((10, 9, 14), faces), ((9, 8, 13), faces), ((8, 10, 11), faces),
+                                       ((7, 7, 5), faces), ((6, 12, 10), faces[::-1])]):
+        rng = np.random.default_rng(700 + k)
+        N = int(np.prod(dims))
+        coef = {f: float(np.float32(rng.uniform(0.2, 0.5))) for f in faces}
+        mur = six_face_mur(dims, order, coef)
+        pos = rng.integers(0, N, 3).astype(np.uint32)
+        exc = {"signal": cases.f32(rng, 17, -1, 1), "delay": np.zeros(3, np.uint32), "amp": cases.f32(rng, 3, 0.5, 1),
+               "dir": rng.integers(0, 3, 3).astype(np.uint32), "pos": pos, "period": 0}
+        if k == 2:
+            nx, ny, nz = dims
+            # an Ex excitation cell on the z = 0 face, and one odd coefficient on
+            # the z = Nz-1 face's Ey entries
+            exc["pos"] = np.append(pos, np.uint32(cases.lin(dims, 3, 4, 0)))
+            exc["dir"] = np.append(exc["dir"], np.uint32(0))
+            exc["delay"] = np.append(exc["delay"], np.uint32(0))
+            exc["amp"] = np.append(exc["amp"], np.float32(0.7))
+            odd = np.flatnonzero(mur["normal_idx"] == N + cases.lin(dims, 2, 5, nz - 1))
+            mur["coeff"][odd] = np.float32(0.123)
+        out.append({"dims": dims, "coeffs": cases.random_coeffs(rng, N), "mur": mur, "exc": exc,
+                    "probes": rng.integers(0, 3 * N, 5).astype(np.uint32)})
+    return out
+
+
+@pytest.mark.parametrize("storage", ["fp32", "fp16"])
+def test_zface_mur_split_branches(storage, monkeypatch):
+    """Synthetic instances through every branch of the z-face split: the
+    batch with the split gives the bits of the batch without it, and in FP32
+    the bits of the single-grid engine."""
+    from efs_amd import batch, engine
+    engine.load_library()
+    insts = synthetic_instances()
+    steps = 80
+
+    def make():
+        b = batch.BatchEngine(storage=storage)
+        for c in insts:
+            i = b.add(c["dims"], c["coeffs"])
+            b.configureMur(i, c["mur"])
+            b.configureExcitation(i, c["exc"])
+            b.configureProbeGather(i, c["probes"])
+        b.finalize(1, steps)
+        return b
+    say("split branches", storage, [c["dims"] for c in insts])
+    a = make()
+    monkeypatch.setenv("EFS_BATCH_ZMUR", "0")
+    d = make()
+    monkeypatch.delenv("EFS_BATCH_ZMUR")
+    sa, sd = a.murSplit(), d.murSplit()
+    say("mur split", sa, sd)
+    assert sa["in_sweep"] > 0 and sd["in_sweep"] == 0 and sa["listed"] + sa["in_sweep"] == sd["listed"]
+    a.iterate(steps)
+    d.iterate(steps)
+    for i, c in enumerate(insts):
+        fa, fd = a.getFields(i), d.getFields(i)
+        for k in ("volt", "curr"):
+            assert np.all(np.isfinite(fa[k]))
+            np.testing.assert_array_equal(fa[k], fd[k], err_msg=f"instance {i} {k}")
+        ga = a.readProbeGatherBuffered(i)["data"]
+        np.testing.assert_array_equal(ga, d.readProbeGatherBuffered(i)["data"])
+        if storage == "fp32":
+            g = engine.HIPEngine()
+            assert g.initGPU()
+            g.init(c["dims"], c["coeffs"])
+            g.configureMur(c["mur"])
+            g.configureExcitation(c["exc"])
+            g.configureProbeGatherBuffered(c["probes"], steps, 1)
+            g.iterate(steps)
+            fs = g.getFields()
+            for k in ("volt", "curr"):
+                np.testing.assert_array_equal(fa[k], fs[k], err_msg=f"instance {i} {k} vs single engine")
+            np.testing.assert_array_equal(ga, g.readProbeGatherBuffered()["data"])
+            g.destroy()
+    a.destroy()
+    d.destroy()
+
+
 def rel_l2(a, b):
     a, b = np.asarray(a, np.float64), np.asarray(b, np.float64)
     return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300))
