@@ -638,6 +638,7 @@ int plan_sweeps(efs_ctx* c) {
     const double core_b = c->comp ? 39.0 : c->comp16 ? 42.0 : 60.0;
     const double upml_b = upml_in_grid ? core_b + (c->pml_comp ? 27.0 : 60.0) : 2.0;
     const uint64_t want = env_u64("EFS_SWEEP_BLOCKS", 2048);
+    const uint32_t min_planes = (uint32_t)std::max<uint64_t>(1, env_u64("EFS_SWEEP_MIN_PLANES", 1));
     for (int v = 0; v < 2; ++v) {
         const uint64_t tcells = (uint64_t)kSweepThreads * (v == 0 ? 4u : 1u);
         const uint32_t T = (uint32_t)((cols + tcells - 1) / tcells);
@@ -671,8 +672,10 @@ int plan_sweeps(efs_ctx* c) {
                     }
         for (double q : cost) total += q;
         // small grids have few column tiles: cut them into more, shorter
-        // chunks (down to 2 planes) so that the launch still fills the GPU
-        const double nblocks = (double)std::max<uint64_t>(1, std::min<uint64_t>(want, (uint64_t)T * std::max(1u, nx / 2)));
+        // chunks (down to min_planes planes, default 1) so that the launch
+        // still fills the GPU
+        const double nblocks =
+            (double)std::max<uint64_t>(1, std::min<uint64_t>(want, (uint64_t)T * std::max(1u, nx / min_planes)));
         const double target = total / nblocks;
         std::vector<SweepBlock> tab;
         std::vector<double> pre(nx + 1);
