@@ -778,6 +778,9 @@ int plan_nf_capture(efs_ctx* c, const std::vector<uint32_t>& pts, uint32_t P) {
         if (cnt[z] >= 256 && z % 4 != 3 && z + 2 < c->nz) zs.push_back(z);
     std::sort(zs.begin(), zs.end(), [&](uint32_t a, uint32_t b) { return cnt[a] != cnt[b] ? cnt[a] > cnt[b] : a < b; });
     if (zs.size() > 2) zs.resize(2);
+    // the H sweep captures at most one plane per 4-cell lane: two planes in
+    // one lane keep the busier one (the other one's points are gathered)
+    if (zs.size() == 2 && zs[0] / 4 == zs[1] / 4) zs.resize(1);
     if (zs.empty()) return EFS_OK;
     uint32_t x0 = ~0u, x1 = 0, y0 = ~0u, y1 = 0;
     for (uint32_t p = 0; p < P; ++p) {

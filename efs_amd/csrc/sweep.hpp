@@ -752,6 +752,10 @@ __device__ __forceinline__ void h4_march(T* __restrict__ curr, const T* __restri
     if (cap.on && w.y - cap.y0 <= cap.y1 - cap.y0)
         for (uint32_t f = 0; f < cap.nf; ++f)
             if (cap.z[f] >= w.z0 && cap.z[f] + 1 < w.z0 + 4) capz |= 1u << f;
+    // the lane's capture plane and its offset in the lane (zf and zf+1 at
+    // capo, capo + 1 <= 3)
+    const uint32_t capf = (capz & 1u) ? 0u : 1u;
+    const uint32_t capo = capz ? cap.z[capf] - w.z0 : 0u;
     const T* const E0 = volt;
     const T* const E1 = volt + S;
     const T* const E2 = volt + 2 * S;
@@ -822,18 +826,24 @@ __device__ __forceinline__ void h4_march(T* __restrict__ curr, const T* __restri
             }
         }
         if (capz && x - cap.x0 <= cap.x1 - cap.x0) {
-#pragma unroll
-            for (uint32_t f = 0; f < 2; ++f) {
-                if (!(capz & (1u << f))) continue;
-                const uint32_t o = cap.z[f] - w.z0;  // 0..2: zf and zf+1 in this lane
+            // a lane holds at most one capture plane (the host keeps two
+            // planes in different lanes): the planes of all lanes share one
+            // block, so a wave runs it once
+            {
+                const uint32_t f = capf, o = capo;
+                const bool o1 = o == 1u, o2 = o == 2u;
+                // elements o and o + 1 of a lane vector
+                auto lo = [&](const v4& v) { return o2 ? v.z : (o1 ? v.y : v.x); };
+                auto hi = [&](const v4& v) { return o2 ? v.w : (o1 ? v.z : v.y); };
                 float* q = cap.slab +
                            ((size_t)(f * cap.nx_c + (x - cap.x0)) * cap.ny_c + (w.y - cap.y0)) * kCapFloats;
+                // H as stored (the same conversions as the stores above)
                 const v4 hx = L::stored(In[0], sc.own[0], sc.own_inv[0]);
                 const v4 hy = L::stored(In[1], sc.own[1], sc.own_inv[1]);
                 const v4 hz = L::stored(In[2], sc.own[2], sc.own_inv[2]);
-                const v4 q0 = {e_x[0][o], e_x[0][o + 1], e_x[1][o], e_x[1][o + 1]};
-                const v4 q1 = {e_x[2][o], e_x[2][o + 1], hx[o], hy[o]};
-                const v4 q2 = {hz[o], hz[o + 1], 0.f, 0.f};
+                const v4 q0 = {lo(e_x[0]), hi(e_x[0]), lo(e_x[1]), hi(e_x[1])};
+                const v4 q1 = {lo(e_x[2]), hi(e_x[2]), lo(hx), lo(hy)};
+                const v4 q2 = {lo(hz), hi(hz), 0.f, 0.f};
                 st4(q, q0);
                 st4(q + 4, q1);
                 st4(q + 8, q2);
