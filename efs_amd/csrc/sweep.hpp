@@ -200,6 +200,13 @@ struct Fs<float> {
     // the value a later load returns for v stored with (s, 1/s)
     __device__ static __forceinline__ v4 stored(v4 v, float, float) { return v; }
     __device__ static __forceinline__ float stored1(float v, float, float) { return v; }
+    // raw (stored-unit) forms for the 4-wide marches: value = raw * sc(s)
+    __device__ static __forceinline__ float sc(float) { return 1.0f; }
+    __device__ static __forceinline__ v4 ldr(const float* p) { return ld4(p); }
+    __device__ static __forceinline__ float ldr1(const float* p) { return *p; }
+    __device__ static __forceinline__ void str(float* p, v4 v) { st4(p, v); }
+    __device__ static __forceinline__ v4 storedr(v4 v) { return v; }
+    __device__ static __forceinline__ float storedr1(float v) { return v; }
 };
 // FP16 stores saturate at the largest finite half: a value that outgrew its
 // scale between two range checks is clamped (and the next check rescales)
@@ -226,6 +233,22 @@ struct Fs<f16> {
     __device__ static __forceinline__ float stored1(float v, float s, float inv) {
         return (float)(f16)clamp16(v * inv) * s;
     }
+    // raw (stored-unit) forms: the marches keep fields as stored values and
+    // apply the power-of-two scales inside the curl, where they are exact,
+    // so each load and store saves its multiply and the results are the
+    // bits of the scaled form
+    __device__ static __forceinline__ float sc(float s) { return s; }
+    __device__ static __forceinline__ v4 ldr(const f16* p) {
+        return __builtin_convertvector(*reinterpret_cast<const h4*>(p), v4);
+    }
+    __device__ static __forceinline__ float ldr1(const f16* p) { return (float)*p; }
+    __device__ static __forceinline__ void str(f16* p, v4 v) {
+        *reinterpret_cast<h4*>(p) = __builtin_convertvector(clamp16(v), h4);
+    }
+    __device__ static __forceinline__ v4 storedr(v4 v) {
+        return __builtin_convertvector(__builtin_convertvector(clamp16(v), h4), v4);
+    }
+    __device__ static __forceinline__ float storedr1(float v) { return (float)(f16)clamp16(v); }
 };
 
 // A field array (volt or curr, 3 components S apart) as the point kernels
@@ -685,18 +708,19 @@ __device__ __forceinline__ void e4_march(T* __restrict__ volt, const T* __restri
             carry = false;
             continue;
         }
+        // fields in stored units (raw); value = raw * L::sc(scale)
         if (!carry) {
             const uint32_t ib = x > 0 ? i - plane : i;
-            hy_m = L::ld(H1 + ib, sc.other[1]);
-            hz_m = L::ld(H2 + ib, sc.other[2]);
+            hy_m = L::ldr(H1 + ib);
+            hz_m = L::ldr(H2 + ib);
         }
-        const v4 hx = L::ld(H0 + i, sc.other[0]), hy = L::ld(H1 + i, sc.other[1]), hz = L::ld(H2 + i, sc.other[2]);
-        const v4 hz_ym = L::ld(H2 + i - sy, sc.other[2]), hx_ym = L::ld(H0 + i - sy, sc.other[0]);
-        const float hy_z = L::ld1(H1 + i - sz, sc.other[1]), hx_z = L::ld1(H0 + i - sz, sc.other[0]);  // z-1
+        const v4 hx = L::ldr(H0 + i), hy = L::ldr(H1 + i), hz = L::ldr(H2 + i);
+        const v4 hz_ym = L::ldr(H2 + i - sy), hx_ym = L::ldr(H0 + i - sy);
+        const float hy_z = L::ldr1(H1 + i - sz), hx_z = L::ldr1(H0 + i - sz);  // z-1
         v4 V[3], a[3], b[3];
         uint32_t q[6];
 #pragma unroll
-        for (int n = 0; n < 3; ++n) V[n] = L::ld(E[n] + i, sc.own[n]);
+        for (int n = 0; n < 3; ++n) V[n] = L::ldr(E[n] + i);
         coef_fetch<COMP>(cf, S, i, q, a, b);
         v4 F[3], P0[3], P1[3], P2[3];
         uint32_t pq[3];
@@ -704,31 +728,39 @@ __device__ __forceinline__ void e4_march(T* __restrict__ volt, const T* __restri
         coef_decode<COMP>(s_tab, q, a, b, cf);
         const v4 hy_zm = (v4){hy_z, hy.x, hy.y, hy.z};
         const v4 hx_zm = (v4){hx_z, hx.x, hx.y, hx.z};
-        const v4 cur[3] = {hz - hz_ym - hy + hy_zm, hx - hx_zm - hz + hz_m, hy - hy_m - hx + hx_ym};
+        // curl in real units: ((hz - hz_ym) - hy) + hy_zm and its two
+        // rotations, each raw term scaled exactly by its component's scale
+        const float s0 = L::sc(sc.other[0]), s1 = L::sc(sc.other[1]), s2 = L::sc(sc.other[2]);
+        const v4 cur[3] = {fma4((v4)s1, hy_zm, fma4((v4)(-s1), hy, s2 * (hz - hz_ym))),
+                           fma4((v4)s2, hz_m, fma4((v4)(-s2), hz, s0 * (hx - hx_zm))),
+                           fma4((v4)s0, hx_ym, fma4((v4)(-s0), hx, s1 * (hy - hy_m)))};
         hy_m = hy;
         hz_m = hz;
         carry = true;
         if (!upml) {
+            // raw update: a * V_raw + (b * curl) / s_own (exact scaling)
             v4 vn[3];
 #pragma unroll
-            for (int n = 0; n < 3; ++n) vn[n] = fma4(a[n], V[n], b[n] * cur[n]);
+            for (int n = 0; n < 3; ++n) vn[n] = fma4(a[n], V[n], (b[n] * cur[n]) * L::sc(sc.own_inv[n]));
             if (zany) {
+                // Mur on scaled values commutes with the exact scaling
 #pragma unroll
                 for (int n = 0; n < 2; ++n) {
                     if (x - zxl[n] > zxs[n]) continue;
                     const float pre = __builtin_fmaf(-zc[n], V[n][zo], V[n][zs]);
-                    const float st = L::stored1(vn[n][zs], sc.own[n], sc.own_inv[n]);
+                    const float st = L::storedr1(vn[n][zs]);
                     vn[n][zo] = __builtin_fmaf(zc[n], st, pre);
                 }
             }
 #pragma unroll
-            for (int n = 0; n < 3; ++n) L::st(E[n] + i, vn[n], sc.own_inv[n]);
+            for (int n = 0; n < 3; ++n) L::str(E[n] + i, vn[n]);
         } else {
             upml_decode<PCOMP>(s_ptab, pq, P0, P1, P2);
 #pragma unroll
             for (int n = 0; n < 3; ++n) {
                 const v4 Fn = fma4(a[n], F[n], b[n] * cur[n]);
-                L::st(E[n] + i, fma4(P2[n], Fn, fma4(P0[n], V[n], -(P1[n] * F[n]))), sc.own_inv[n]);
+                const v4 Vs = V[n] * L::sc(sc.own[n]);
+                L::str(E[n] + i, fma4(P2[n], Fn, fma4(P0[n], Vs, -(P1[n] * F[n]))) * L::sc(sc.own_inv[n]));
                 st4(u.flux + n * u.cs, Fn);
             }
         }
@@ -779,22 +811,22 @@ __device__ __forceinline__ void h4_march(T* __restrict__ curr, const T* __restri
             carry = false;
             continue;
         }
+        // fields in stored units (raw); value = raw * L::sc(scale)
         if (!carry) {
-            ex = L::ld(E0 + i, sc.other[0]);
-            ey = L::ld(E1 + i, sc.other[1]);
-            ez = L::ld(E2 + i, sc.other[2]);
+            ex = L::ldr(E0 + i);
+            ey = L::ldr(E1 + i);
+            ez = L::ldr(E2 + i);
         }
         // per-cell core flags: cells 0..2 follow `core`, cell 3 also needs z+1 < Nz
         const uint32_t cb = core ? (last ? 7u : 15u) : 0u;
         const uint32_t sx = core ? plane : 0u;
-        const v4 ex_p = L::ld(E0 + i + sx, sc.other[0]), ey_p = L::ld(E1 + i + sx, sc.other[1]),
-                 ez_p = L::ld(E2 + i + sx, sc.other[2]);
-        const v4 ez_yp = L::ld(E2 + i + sy, sc.other[2]), ex_yp = L::ld(E0 + i + sy, sc.other[0]);
-        const float ey_z = L::ld1(E1 + i + sz, sc.other[1]), ex_z = L::ld1(E0 + i + sz, sc.other[0]);  // z+1
+        const v4 ex_p = L::ldr(E0 + i + sx), ey_p = L::ldr(E1 + i + sx), ez_p = L::ldr(E2 + i + sx);
+        const v4 ez_yp = L::ldr(E2 + i + sy), ex_yp = L::ldr(E0 + i + sy);
+        const float ey_z = L::ldr1(E1 + i + sz), ex_z = L::ldr1(E0 + i + sz);  // z+1
         v4 I[3], a[3], b[3];
         uint32_t q[6];
 #pragma unroll
-        for (int n = 0; n < 3; ++n) I[n] = L::ld(Ic[n] + i, sc.own[n]);
+        for (int n = 0; n < 3; ++n) I[n] = L::ldr(Ic[n] + i);
         coef_fetch<COMP>(cf, S, i, q, a, b);
         v4 F[3], P0[3], P1[3], P2[3];  // P: ii, iifo, iifn
         uint32_t pq[3];
@@ -802,26 +834,33 @@ __device__ __forceinline__ void h4_march(T* __restrict__ curr, const T* __restri
         coef_decode<COMP>(s_tab, q, a, b, cf);
         const v4 ey_zp = (v4){ey.y, ey.z, ey.w, ey_z};
         const v4 ex_zp = (v4){ex.y, ex.z, ex.w, ex_z};
-        const v4 cur[3] = {ez - ez_yp - ey + ey_zp, ex - ex_zp - ez + ez_p, ey - ey_p - ex + ex_yp};
-        const v4 e_x[3] = {ex, ey, ez};  // E at this plane, for the capture
+        // curl in real units: ((ez - ez_yp) - ey) + ey_zp and its two
+        // rotations, each raw term scaled exactly by its component's scale
+        const float s0 = L::sc(sc.other[0]), s1 = L::sc(sc.other[1]), s2 = L::sc(sc.other[2]);
+        const v4 cur[3] = {fma4((v4)s1, ey_zp, fma4((v4)(-s1), ey, s2 * (ez - ez_yp))),
+                           fma4((v4)s2, ez_p, fma4((v4)(-s2), ez, s0 * (ex - ex_zp))),
+                           fma4((v4)s0, ex_yp, fma4((v4)(-s0), ex, s1 * (ey - ey_p)))};
+        const v4 e_x[3] = {ex, ey, ez};  // E at this plane (raw), for the capture
         ex = ex_p;
         ey = ey_p;
         ez = ez_p;
         carry = core;
-        v4 In[3];
+        v4 In[3];  // new I, raw
         if (!upml) {
+            // raw update: a * I_raw + (b * curl) / s_own (exact scaling)
 #pragma unroll
             for (int n = 0; n < 3; ++n) {
-                In[n] = sel4(cb, fma4(a[n], I[n], b[n] * cur[n]), I[n]);
-                L::st(Ic[n] + i, In[n], sc.own_inv[n]);
+                In[n] = sel4(cb, fma4(a[n], I[n], (b[n] * cur[n]) * L::sc(sc.own_inv[n])), I[n]);
+                L::str(Ic[n] + i, In[n]);
             }
         } else {
             upml_decode<PCOMP>(s_ptab, pq, P0, P1, P2);
 #pragma unroll
             for (int n = 0; n < 3; ++n) {
                 const v4 Fn = sel4(cb, fma4(a[n], F[n], b[n] * cur[n]), F[n]);
-                In[n] = fma4(P2[n], Fn, fma4(P0[n], I[n], -(P1[n] * F[n])));
-                L::st(Ic[n] + i, In[n], sc.own_inv[n]);
+                const v4 Is = I[n] * L::sc(sc.own[n]);
+                In[n] = fma4(P2[n], Fn, fma4(P0[n], Is, -(P1[n] * F[n]))) * L::sc(sc.own_inv[n]);
+                L::str(Ic[n] + i, In[n]);
                 st4(u.flux + n * u.cs, Fn);
             }
         }
@@ -837,12 +876,13 @@ __device__ __forceinline__ void h4_march(T* __restrict__ curr, const T* __restri
                 auto hi = [&](const v4& v) { return o2 ? v.w : (o1 ? v.z : v.y); };
                 float* q = cap.slab +
                            ((size_t)(f * cap.nx_c + (x - cap.x0)) * cap.ny_c + (w.y - cap.y0)) * kCapFloats;
-                // H as stored (the same conversions as the stores above)
-                const v4 hx = L::stored(In[0], sc.own[0], sc.own_inv[0]);
-                const v4 hy = L::stored(In[1], sc.own[1], sc.own_inv[1]);
-                const v4 hz = L::stored(In[2], sc.own[2], sc.own_inv[2]);
-                const v4 q0 = {lo(e_x[0]), hi(e_x[0]), lo(e_x[1]), hi(e_x[1])};
-                const v4 q1 = {lo(e_x[2]), hi(e_x[2]), lo(hx), lo(hy)};
+                // real units: E (raw) and H as stored, times their scales
+                const v4 hx = L::storedr(In[0]) * L::sc(sc.own[0]);
+                const v4 hy = L::storedr(In[1]) * L::sc(sc.own[1]);
+                const v4 hz = L::storedr(In[2]) * L::sc(sc.own[2]);
+                const float t0 = L::sc(sc.other[0]), t1 = L::sc(sc.other[1]), t2 = L::sc(sc.other[2]);
+                const v4 q0 = {lo(e_x[0]) * t0, hi(e_x[0]) * t0, lo(e_x[1]) * t1, hi(e_x[1]) * t1};
+                const v4 q1 = {lo(e_x[2]) * t2, hi(e_x[2]) * t2, lo(hx), lo(hy)};
                 const v4 q2 = {lo(hz), hi(hz), 0.f, 0.f};
                 st4(q, q0);
                 st4(q + 4, q1);
