@@ -120,6 +120,29 @@ def test_scalar_lane_path(HIPEngine, oracle_mod, monkeypatch, name, seed, dims, 
     g.destroy()
 
 
+@pytest.mark.parametrize("storage", ["fp32", "fp16"])
+def test_chunk_lengths_same_bits(HIPEngine, monkeypatch, storage):
+    """The x chunk a workgroup marches (1 plane each, the default on small
+    grids, or long chunks from a small block target) changes only which x
+    neighbours come from registers: fields stay bit-identical."""
+    case = random_case(61, (26, 20, 32), pml_width=4, mur=True, ade=True, tfsf=True)
+    out = []
+    for blocks, planes in (("2048", "1"), ("16", "1"), ("2048", "4")):
+        monkeypatch.setenv("EFS_SWEEP_BLOCKS", blocks)
+        monkeypatch.setenv("EFS_SWEEP_MIN_PLANES", planes)
+        say("case chunks", storage, blocks, planes)
+        g = HIPEngine()
+        assert g.initGPU()
+        g.init(case["dims"], case["coeffs"], storage=storage)
+        configure(g, case)
+        g.iterate(40)
+        out.append(g.getFields())
+        g.destroy()
+    for f in out[1:]:
+        np.testing.assert_array_equal(f["volt"], out[0]["volt"])
+        np.testing.assert_array_equal(f["curr"], out[0]["curr"])
+
+
 QCASES = [
     ("q_core", 41, (16, 14, 32), dict(quantize=23)),
     ("q_pml_split", 42, (20, 18, 24), dict(pml_width=3, quantize=50, mur=True, ade=True, tfsf=True, rlc=True)),
