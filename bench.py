@@ -25,14 +25,32 @@ METRIC = "MCells/s & %HBM roofline: plain FP32 vs batched/mixed at fixed S11 err
 HBM_PEAK_GBS = 8000.0
 
 
-def sweep_bytes(n_cells, n_pml, storage="fp32"):
+def coef_bytes(storage, indexed):
+    """Bytes per coefficient in the algorithmic count: 4 (FP32), and for
+    FP16 fields with indexed coefficients (byte or 16-bit indices into
+    tables, at most 2 B per coefficient) the FP16-coefficient figure, 2."""
+    return 2.0 if storage == "fp16" and indexed else 4.0
+
+
+def sweep_bytes(n_cells, n_pml, storage="fp32", flags=()):
     """Algorithmic HBM bytes of one E (or H) sweep launch.
     Per cell: read V, vv, vi, H (3 components each) and write V = 15 values:
-    60 B with FP32 fields, 42 B with FP16 fields (9 field values x 2 B + 6
-    FP32 coefficients).  Per UPML cell in addition: flux read + write and
-    three UPML coefficients for 3 components = 60 B."""
-    per_cell = 42.0 if storage == "fp16" else 60.0
+    60 B with FP32 fields; with FP16 fields 9 field values x 2 B + 6
+    coefficients = 42 B (FP32 coefficient arrays) or 30 B (indexed
+    coefficients, `flags` holding coef_idx or coef_idx16).  Per UPML cell in
+    addition: flux read + write and three UPML coefficients for 3 components
+    = 60 B."""
+    indexed = bool({"coef_idx", "coef_idx16"} & set(flags))
+    per_cell = 18.0 + 6 * coef_bytes(storage, indexed) if storage == "fp16" else 60.0
     return per_cell * n_cells + 60.0 * n_pml
+
+
+def byte_model(storage, flags=()):
+    indexed = bool({"coef_idx", "coef_idx16"} & set(flags))
+    if storage != "fp16":
+        return "60 B/cell (FP32 fields and coefficients) + 60 B/UPML cell"
+    return ("30 B/cell (FP16 fields, indexed coefficients counted at 2 B) + 60 B/UPML cell" if indexed
+            else "42 B/cell (FP16 fields, FP32 coefficients) + 60 B/UPML cell")
 
 
 TRAFFIC_FILES = {"fp32": os.path.join(ROOT, "profiles", "fused_sweeps_256cube_stress_traffic.json"),
@@ -138,7 +156,6 @@ def cpu_baseline(w, seconds_target=15.0, what=None):
                       f"NF2FF and energy excluded"}
 
 
-BATCH_BYTES = {"fp32": 60.0, "fp16": 42.0}  # per cell and sweep, no UPML (see sweep_bytes)
 
 
 def build_patch_batch(n_inst, storage, max_samples, progress=False):
@@ -232,6 +249,7 @@ def run_batch(args):
     shape = b.tune(steps=8)
     info = b.info()
     cells = info["cells"]
+    indexed = [b.coefIndexed(i) for i in range(len(ws))]
     elapsed = time_batch(b, W, K)
     value = cells * K / elapsed / 1e6
     # sweep durations (direct launches with events)
@@ -241,8 +259,11 @@ def run_batch(args):
     b.profile(False)
     b.destroy()
     avg_e, avg_h = p["e_ms"] / max(p["e_n"], 1), p["h_ms"] / max(p["h_n"], 1)
-    cells_p = sum(int(np.prod(w["gridSize"][:2])) * ((w["gridSize"][2] + 3) // 4 * 4) for w in ws)
-    by = BATCH_BYTES[storage] * cells_p
+    # per cell and sweep: FP32 60 B; FP16 fields with 16-bit coefficient
+    # indices count coefficients at 2 B (30 B), FP32 coefficient arrays 42 B
+    per_inst = [int(np.prod(w["gridSize"][:2])) * ((w["gridSize"][2] + 3) // 4 * 4) for w in ws]
+    by = sum(c * (18.0 + 6 * coef_bytes(storage, ix) if storage == "fp16" else 60.0)
+             for c, ix in zip(per_inst, indexed))
     dom_name, dom_ms = ("sweep_h", avg_h) if avg_h >= avg_e else ("sweep_e", avg_e)
     dom_gbs = by / (dom_ms * 1e-3) / 1e9
     extra = {}
@@ -288,6 +309,9 @@ def run_batch(args):
         "roofline": {"bound": "hbm", "kernel": "batched " + dom_name, "achieved": round(dom_gbs, 1),
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(dom_gbs / HBM_PEAK_GBS, 4),
                      "traffic": None, "traffic_source": None, "algorithmic_bytes_per_launch": by,
+                     "algorithmic_model": ("60 B/cell (FP32 fields and coefficients)" if storage == "fp32" else
+                                           f"FP16 fields: {sum(indexed)} of {len(indexed)} instances with 16-bit "
+                                           "coefficient indices at 30 B/cell, the others 42 B/cell"),
                      "avg_launch_ms": round(dom_ms, 5),
                      "kernels": {"sweep_e": {"avg_ms": round(avg_e, 5), "GB/s": round(by / (avg_e * 1e-3) / 1e9, 1)},
                                  "sweep_h": {"avg_ms": round(avg_h, 5), "GB/s": round(by / (avg_h * 1e-3) / 1e9, 1)}}},
@@ -359,7 +383,7 @@ def run_patch(args):
     flags = g.pathFlags()
     g.destroy()
     avg_e, avg_h = ms_e / ce, ms_h / ch
-    b = sweep_bytes(n, 0, args.storage)
+    b = sweep_bytes(n, 0, args.storage, flags)
     dom_name, dom_ms = ("sweep_h", avg_h) if avg_h >= avg_e else ("sweep_e", avg_e)
     extra = {"results": {"f_res_GHz": res["f_res_GHz"], "s11_min_dB": res["s11_min_dB"],
                          "Dmax_dBi_2p4GHz": res["Dmax_dBi"]}}
@@ -390,6 +414,7 @@ def run_patch(args):
         "roofline": {"bound": "hbm", "kernel": dom_name, "achieved": round(b / (dom_ms * 1e-3) / 1e9, 1),
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(b / (dom_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
                      "traffic": None, "traffic_source": None, "algorithmic_bytes_per_launch": b,
+                     "algorithmic_model": byte_model(args.storage, flags),
                      "avg_launch_ms": round(dom_ms, 5),
                      "kernels": {"sweep_e": {"avg_ms": round(avg_e, 5)}, "sweep_h": {"avg_ms": round(avg_h, 5)}}},
         "cpu_baseline": base, **extra,
@@ -438,7 +463,7 @@ def run_dispersive(args):
     flags = g.pathFlags()
     g.destroy()
     avg_e, avg_h = ms_e / ce, ms_h / ch
-    b = sweep_bytes(n, w["n_pml_cells"], args.storage)
+    b = sweep_bytes(n, w["n_pml_cells"], args.storage, flags)
     dom_name, dom_ms = ("sweep_h", avg_h) if avg_h >= avg_e else ("sweep_e", avg_e)
     extra = {"results": {"rcs_xy_cut": cut, "angles": len(w["phi"])}}
     if not args.no_compare:
@@ -481,6 +506,7 @@ def run_dispersive(args):
         "roofline": {"bound": "hbm", "kernel": dom_name, "achieved": round(b / (dom_ms * 1e-3) / 1e9, 1),
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(b / (dom_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
                      "traffic": None, "traffic_source": None, "algorithmic_bytes_per_launch": b,
+                     "algorithmic_model": byte_model(args.storage, flags),
                      "avg_launch_ms": round(dom_ms, 5),
                      "kernels": {"sweep_e": {"avg_ms": round(avg_e, 5)}, "sweep_h": {"avg_ms": round(avg_h, 5)}}},
         "cpu_baseline": base, **extra,
@@ -593,7 +619,7 @@ def run_large(args):
                                                  "20 boxes), 1,500 steps",
                                       "s11_max_abs_err_dB_2_18GHz": float(np.max(np.abs(outs[0] - outs[1])))}
     kern = main["kern"]
-    b = sweep_bytes(n, w["n_pml_cells"], args.storage)
+    b = sweep_bytes(n, w["n_pml_cells"], args.storage, main["flags"])
     dom_name, dom_ms = ("sweep_h", kern[1]) if kern[1] >= kern[0] else ("sweep_e", kern[0])
     result = {
         "metric": METRIC, "value": round(main["value"], 1), "unit": "MCells/s", "n_gpus": 1, "steps": K, "warmup": 0,
@@ -608,6 +634,7 @@ def run_large(args):
         "roofline": {"bound": "hbm", "kernel": dom_name, "achieved": round(b / (dom_ms * 1e-3) / 1e9, 1),
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(b / (dom_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
                      "traffic": None, "traffic_source": None, "algorithmic_bytes_per_launch": b,
+                     "algorithmic_model": byte_model(args.storage, main["flags"]),
                      "avg_launch_ms": round(dom_ms, 5),
                      "kernels": {"sweep_e": {"avg_ms": round(kern[0], 5)}, "sweep_h": {"avg_ms": round(kern[1], 5)}}},
         "cpu_baseline": None, **extra,
@@ -737,11 +764,12 @@ def main():
     ms_s, cs = eng.profileRead(PROF_STEP)
     eng.profileEnable(False)
     avg_e, avg_h, avg_s = ms_e / ce, ms_h / ch, ms_s / cs
-    b = sweep_bytes(n, w["n_pml_cells"], args.storage)
+    flags_main = eng.pathFlags()
+    b = sweep_bytes(n, w["n_pml_cells"], args.storage, flags_main)
     gbs_e = b / (avg_e * 1e-3) / 1e9
     gbs_h = b / (avg_h * 1e-3) / 1e9
     dom_name, dom_ms = ("sweep_h", avg_h) if avg_h >= avg_e else ("sweep_e", avg_e)
-    dom_sym = sweep_kernel_symbol(eng.pathFlags(), dom_name[-1])
+    dom_sym = sweep_kernel_symbol(flags_main, dom_name[-1])
     traffic = measured_traffic(dom_sym, args.size, args.storage)
     dom_gbs = b / (dom_ms * 1e-3) / 1e9
     acc = eng.readNF2FFAccumulation()
@@ -807,6 +835,7 @@ def main():
             "traffic": round(traffic) if traffic else None,
             "traffic_source": (os.path.relpath(TRAFFIC_FILES[args.storage], ROOT) + f" [{dom_sym}]") if traffic else None,
             "algorithmic_bytes_per_launch": b,
+            "algorithmic_model": byte_model(args.storage, flags_main),
             "avg_launch_ms": round(dom_ms, 5),
             "kernels": {"sweep_e": {"avg_ms": round(avg_e, 5), "GB/s": round(gbs_e, 1)},
                         "sweep_h": {"avg_ms": round(avg_h, 5), "GB/s": round(gbs_h, 1)},

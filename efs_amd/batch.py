@@ -117,6 +117,13 @@ class BatchEngine:
         return {"blocks": b.value, "waves": w.value, "graph_steps": g.value, "cells": c.value,
                 "instances": len(self._grids)}
 
+    def coefIndexed(self, i):
+        """True when instance i keeps its coefficients as 16-bit indices into
+        tables (both sides), False for FP32 coefficient arrays."""
+        v = C.c_uint32()
+        self._check(self.lib.efs_batch_coef_form(self.h, i, C.byref(v)))
+        return bool(v.value)
+
     def murSplit(self):
         """Mur entries applied by the point kernels ("listed") and by the E
         sweep itself ("in_sweep", z-face entries)."""

@@ -1259,6 +1259,13 @@ int efs_batch_info(efs_batch* b, uint32_t* blocks, uint32_t* waves, uint32_t* gr
     return EFS_OK;
 }
 
+int efs_batch_coef_form(efs_batch* b, uint32_t id, uint32_t* idx16) {
+    if (!b) return EFS_ERR_STATE;
+    BTRY(need_inst(b, id));
+    if (idx16) *idx16 = (b->inst[id].tabn_e && b->inst[id].tabn_h) ? 1u : 0u;
+    return EFS_OK;
+}
+
 int efs_batch_mur_split(efs_batch* b, uint32_t* listed, uint32_t* in_sweep) {
     BTRY(need_final(b));
     if (listed) *listed = b->mur_n;

@@ -103,6 +103,11 @@ int efs_batch_info(efs_batch *b, uint32_t *blocks, uint32_t *waves, uint32_t *gr
  * numbers. */
 int efs_batch_mur_split(efs_batch *b, uint32_t *listed, uint32_t *in_sweep);
 
+/* Coefficient form of instance id (no counterpart in the reference; a
+ * diagnostic): *idx16 = 1 when both sides use 16-bit indices into tables,
+ * 0 when they are FP32 arrays. */
+int efs_batch_coef_form(efs_batch *b, uint32_t id, uint32_t *idx16);
+
 /* Per-sweep device timing (HIP events on the batch stream): enable, then
  * read the summed milliseconds and counts of the E and H sweep launches
  * made while enabled (graph replays are not split: with profiling on,
