@@ -116,17 +116,44 @@ def precision_sample(storage, n=48, steps=150):
     return out
 
 
-def cpu_baseline(w, seconds_target=15.0, what=None):
-    """Time the CPU oracle (all host threads, f32 variant) on a bounded
-    sample of the same workload: the same grid, coefficients, UPML (or Mur)
-    and excitation; NF2FF and energy are not part of the sample."""
-    from oracle import oracle as orc
-    threads = os.cpu_count() or 1
+def host_threads():
+    """CPU threads this process may use: the CPUs it is allowed to run on,
+    capped by OMP_NUM_THREADS when that is set (a GPU host's per-GPU CPU
+    share)."""
     try:
-        threads = len(os.sched_getaffinity(0))
+        n = len(os.sched_getaffinity(0))
     except AttributeError:
+        n = os.cpu_count() or 1
+    cap = os.environ.get("OMP_NUM_THREADS", "")
+    return max(1, min(n, int(cap))) if cap.isdigit() and int(cap) > 0 else n
+
+
+def host_info():
+    """CPU model and counts of the host the CPU baseline runs on."""
+    model = None
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
         pass
-    threads = min(threads, 16)
+    try:
+        allowed = len(os.sched_getaffinity(0))
+    except AttributeError:
+        allowed = None
+    return {"cpu_model": model, "cpus_online": os.cpu_count(), "cpus_allowed": allowed,
+            "omp_num_threads": os.environ.get("OMP_NUM_THREADS")}
+
+
+def cpu_baseline(w, seconds_target=15.0, what=None):
+    """Time the CPU oracle (every thread this process may use, f32 variant)
+    on a bounded sample of the same workload: the same grid, coefficients,
+    UPML (or Mur) and excitation; NF2FF and energy are not part of the
+    sample."""
+    from oracle import oracle as orc
+    threads = host_threads()
     orc.Oracle.set_threads(threads)
     o = orc.Oracle(w["gridSize"], w["coefficients"])
     if "pmlRegions" in w:
@@ -153,7 +180,9 @@ def cpu_baseline(w, seconds_target=15.0, what=None):
     return {"value": n * steps / elapsed / 1e6, "unit": "MCells/s", "cores": threads, "kind": "port",
             "sample": f"{steps} steps of " + (what or f"the same {w['gridSize'][0]}^3 UPML+excitation workload") +
                       f" ({elapsed:.1f} s) with the C oracle (float storage, double arithmetic, OpenMP); "
-                      f"NF2FF and energy excluded"}
+                      f"NF2FF and energy excluded; the oracle is the parity checker, a plain restatement of the "
+                      f"reference's CPU engine, not a tuned CPU engine",
+            "host": host_info()}
 
 
 
@@ -191,8 +220,7 @@ def s11_error(sample, storage, steps, freqs):
     freqs, and the relative L2 of the port traces."""
     from efs_amd import batch, ports, workloads
     from oracle import oracle as orc
-    threads = min(len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else 1, 16)
-    orc.Oracle.set_threads(threads)
+    orc.Oracle.set_threads(host_threads())
     b = batch.BatchEngine(storage=storage)
     sl = [workloads.add_patch(b, w)[1] for w in sample]
     b.finalize(1, steps)
@@ -390,8 +418,7 @@ def run_patch(args):
     if not args.no_compare:
         print("# FP64 oracle run of the same case", file=sys.stderr, flush=True)
         from oracle import oracle as orc
-        threads = min(len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else 1, 16)
-        orc.Oracle.set_threads(threads)
+        orc.Oracle.set_threads(host_threads())
         o = orc.Oracle(w["gridSize"], w["coefficients"], fp64=True)
         so = workloads.configure_patch(o, w, nf2ff=True, max_samples=K)
         o.iterate(K)
@@ -542,7 +569,7 @@ def run_large(args):
 
     def say(*a):
         print("#", *a, file=sys.stderr, flush=True)
-    threads = min(16, len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else 1)
+    threads = host_threads()
     t_b = time.perf_counter()
     w = workloads.large_array(threads=threads, progress=lambda m: say(m, f"{time.perf_counter() - t_b:.1f} s"))
     n = int(np.prod(w["gridSize"]))
@@ -642,6 +669,21 @@ def run_large(args):
     print(json.dumps(result))
 
 
+def time_stress_window(eng, K, E):
+    """The timed steps of the stress workload: K steps with the field
+    energy read every E steps, and once at the end when K is not a multiple
+    of E (so a window shorter than E still holds one energy call).  Returns
+    the energies read."""
+    energies, done = [], 0
+    while done < K:
+        k = min(E, K - done)
+        eng.iterate(k)
+        done += k
+        if done % E == 0 or done == K:
+            energies.append(eng.energy())
+    return energies
+
+
 def calibrate_copy():
     import torch
     a = torch.ones(256 * 1024 * 1024, dtype=torch.float32, device="cuda")
@@ -737,20 +779,21 @@ def main():
         raise RuntimeError("no HIP device")
     workloads.configure_engine(eng, w, storage=args.storage)
 
-    eng.iterate(args.warmup)
+    K, E = args.steps, args.energy_interval
+    kb = int(w["nf2ff"]["kBlock"])
+    # The NF2FF surface is sampled every step and folded into the
+    # frequency bins every kb samples.  The untimed lead-in (warmup plus
+    # `align` steps) ends the timed window on a fold, so the window holds
+    # at least one fold at any step count, and every window of kb steps
+    # or more holds floor-or-ceil(K / kb) of them.
+    align = (-(args.warmup + K)) % kb
+    lead = args.warmup + align
+    eng.iterate(lead)
     eng.energy()
     eng.sync()
-
-    K, E = args.steps, args.energy_interval
-    energies = []
+    folds = (lead + K) // kb - lead // kb
     t0 = time.perf_counter()
-    done = 0
-    while done < K:
-        k = min(E, K - done)
-        eng.iterate(k)
-        done += k
-        if done % E == 0:
-            energies.append(eng.energy())
+    energies = time_stress_window(eng, K, E)
     eng.sync()
     elapsed = time.perf_counter() - t0
     mcells = n * K / elapsed / 1e6
@@ -782,16 +825,10 @@ def main():
         e32 = HIPEngine()
         e32.initGPU()
         workloads.configure_engine(e32, w)
-        e32.iterate(args.warmup)
+        e32.iterate(lead)
         e32.sync()
         t1 = time.perf_counter()
-        done = 0
-        while done < K:
-            k = min(E, K - done)
-            e32.iterate(k)
-            done += k
-            if done % E == 0:
-                e32.energy()
+        time_stress_window(e32, K, E)
         e32.sync()
         v32 = n * K / (time.perf_counter() - t1) / 1e6
         e32.destroy()
@@ -819,9 +856,10 @@ def main():
         "dtype": "f32" if args.storage == "fp32" else "f16 storage, f32 compute",
         "data": "synthetic",
         "config": {
-            "workload": f"plain FP32 stress grid {args.size}^3, 1 mm cells, UPML-8 on 6 faces, 40 dielectric boxes, "
-                        f"10 PEC plates, 50-ohm lumped port, Gaussian f0=5 GHz fc=4 GHz, energy every {E} steps, "
-                        f"NF2FF 6-face box {w['nf2ff']['numPoints']} points x 16 frequencies (1-9 GHz)",
+            "workload": (f"{'plain FP32' if args.storage == 'fp32' else 'FP16-storage (FP32 update)'} stress grid "
+                         f"{args.size}^3, 1 mm cells, UPML-8 on 6 faces, 40 dielectric boxes, "
+                         f"10 PEC plates, 50-ohm lumped port, Gaussian f0=5 GHz fc=4 GHz, energy every {E} steps, "
+                         f"NF2FF 6-face box {w['nf2ff']['numPoints']} points x 16 frequencies (1-9 GHz)"),
             "grid": w["gridSize"], "pml_cells": w["n_pml_cells"], "dT": w["dT"], "seed": w["seed"],
             "nf2ff_k_block": w["nf2ff"]["kBlock"], "parallelism": "single GPU",
         },
@@ -846,7 +884,9 @@ def main():
         "cpu_baseline": base,
         **extra,
         "checks": {"finite": finite, "energy_last": energies[-1] if energies else None,
-                   "host_build_s": round(t_build, 1)},
+                   "host_build_s": round(t_build, 1),
+                   "timed_window": {"steps": K, "nf2ff_folds": folds, "energy_calls": len(energies),
+                                    "untimed_lead_in_steps": lead, "nf2ff_k_block": kb}},
     }
     print(json.dumps(result))
 

@@ -1220,6 +1220,19 @@ int efs_batch_read_gather(efs_batch* b, uint32_t id, float* out, uint32_t* num_s
     return EFS_OK;
 }
 
+int efs_batch_sizes(efs_batch* b, uint32_t id, uint32_t* dims, uint32_t* gather_n) {
+    if (!b) return EFS_ERR_STATE;
+    BTRY(need_inst(b, id));
+    const HostInst& h = b->inst[id];
+    if (dims) {
+        dims[0] = h.nx;
+        dims[1] = h.ny;
+        dims[2] = h.nz;
+    }
+    if (gather_n) *gather_n = (uint32_t)h.gat.size();
+    return EFS_OK;
+}
+
 int efs_batch_get_fields(efs_batch* b, uint32_t id, float* volt, float* curr) {
     BTRY(need_final(b));
     BTRY(need_inst(b, id));

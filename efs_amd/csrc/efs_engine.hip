@@ -192,6 +192,10 @@ struct efs_ctx {
     NfCapture nf_cap{};
     float* nf_slab = nullptr;
     float *nf_el = nullptr, *nf_buf = nullptr, *nf_acc = nullptr;
+    // compensated accumulation: rounding errors of acc (value = acc + cmp);
+    // nullptr in the plain form
+    float* nf_cmp = nullptr;
+    bool nf_comp_req = true;
     double* nf_omega = nullptr;
     float* nf_tw = nullptr;  // [2][K][NC] twiddles
 
@@ -548,7 +552,7 @@ int nf2ff_fold(efs_ctx* c) {
     const size_t lds = (size_t)((kused + 1) & ~1u) * kFoldCols * sizeof(float);
     const dim3 grid(c->nf_rows / kFoldRows, c->nf_NC / kFoldCols);
     hipLaunchKernelGGL(k_nf2ff_fold, grid, dim3(256), lds, c->stream, c->nf_buf, c->nf_rows,
-                       c->nf_geom.rowsE, c->nf_K, kused, c->nf_tw, c->nf_NC, c->nf_acc);
+                       c->nf_geom.rowsE, c->nf_K, kused, c->nf_tw, c->nf_NC, c->nf_acc, c->nf_cmp);
     CHECK_LAUNCH(c);
     c->nf_slot = 0;
     return EFS_OK;
@@ -578,7 +582,7 @@ void free_all(efs_ctx* c) {
     dfree(c->rlc_gidx); dfree(c->rlc_type); dfree(c->rlc_c); dfree(c->rlc_vdn); dfree(c->rlc_jn); dfree(c->rlc_vil);
     dfree(c->g_idx); dfree(c->g_out); dfree(c->gb_idx); dfree(c->gb_ring);
     dfree(c->pbc_imag); dfree(c->ss_gidx); dfree(c->ss_e);
-    dfree(c->nf_pts); dfree(c->nf_el); dfree(c->nf_buf); dfree(c->nf_acc); dfree(c->nf_omega); dfree(c->nf_tw);
+    dfree(c->nf_pts); dfree(c->nf_el); dfree(c->nf_buf); dfree(c->nf_acc); dfree(c->nf_cmp); dfree(c->nf_omega); dfree(c->nf_tw);
     dfree(c->nf_slab);
     dfree(c->en_partial);
 }
@@ -1107,9 +1111,12 @@ int efs_init(efs_ctx* c, uint32_t nx, uint32_t ny, uint32_t nz, const float* vv,
         TRY(dupload(c, c->f_factor, one, 6));
         TRY(dupload(c, c->f_smin, one, 6));
     }
-    // coefficient compression (4-wide sweeps): both fields, all components
+    // coefficient compression (4-wide sweeps): both fields, all components.
+    // EFS_COMPRESS: 1 (default) byte form where it fits, else 16-bit; 16 the
+    // 16-bit form even where the byte form fits; 0 FP32 arrays only.
+    const uint64_t cmode = env_u64("EFS_COMPRESS", 1);
     c->comp = false;
-    if (c->vec4 && env_u64("EFS_COMPRESS", 1) != 0) {
+    if (c->vec4 && cmode != 0 && cmode != 16) {
         std::vector<uint8_t> ie(3 * c->S, 0), ih(3 * c->S, 0);
         std::vector<float2> te(3 * 256), th(3 * 256);
         bool ok = true;
@@ -1128,7 +1135,7 @@ int efs_init(efs_ctx* c, uint32_t nx, uint32_t ny, uint32_t nz, const float* vv,
     // more than 256 pairs somewhere: 16-bit indices (six tables built on six
     // threads), up to 65,536 pairs per component
     c->comp16 = false;
-    if (c->vec4 && !c->comp && env_u64("EFS_COMPRESS", 1) != 0) {
+    if (c->vec4 && !c->comp && cmode != 0) {
         std::vector<uint16_t> ie(3 * c->S, 0), ih(3 * c->S, 0);
         std::vector<float2> tabs[6];
         bool ok[6];
@@ -1758,6 +1765,8 @@ int efs_cfg_nf2ff(efs_ctx* c, const uint32_t* points, uint32_t P, const double* 
     TRY(dupload(c, c->nf_omega, omega, F));
     TRY(dzero(c, c->nf_buf, (size_t)K * rows));  // padding rows stay zero
     TRY(dzero(c, c->nf_acc, (size_t)rows * NC));
+    dfree(c->nf_cmp);
+    if (c->nf_comp_req) TRY(dzero(c, c->nf_cmp, (size_t)rows * NC));
     TRY(dalloc(c, c->nf_tw, (size_t)2 * K * NC));
     c->nf_geom = g;
     c->nf_P = P; c->nf_F = F; c->nf_K = K;
@@ -1781,6 +1790,11 @@ int efs_read_nf2ff(efs_ctx* c, float* accE, float* accH) {
     const size_t P = c->nf_P, NC = c->nf_NC, RE = c->nf_geom.rowsE;
     std::vector<float> acc((size_t)c->nf_rows * NC);
     HIP_TRY(c, hipMemcpy(acc.data(), c->nf_acc, acc.size() * sizeof(float), hipMemcpyDeviceToHost));
+    if (c->nf_cmp) {  // compensated form: value = acc + cmp
+        std::vector<float> cmp(acc.size());
+        HIP_TRY(c, hipMemcpy(cmp.data(), c->nf_cmp, cmp.size() * sizeof(float), hipMemcpyDeviceToHost));
+        for (size_t i = 0; i < acc.size(); ++i) acc[i] += cmp[i];
+    }
     // device [row][2f + re/im] (grid-ordered points) -> caller [f][6p + 2k + re/im]
     for (size_t q = 0; q < P; ++q) {
         const size_t p = c->nf_perm[q];
@@ -1805,7 +1819,7 @@ int efs_read_nf2ff(efs_ctx* c, float* accE, float* accH) {
 
 static int farfield_impl(efs_ctx* c, uint32_t f, const float* meta, const float* theta, uint32_t nT, const float* phi,
                          uint32_t nP, const double* center, double frequency, double radius, int mdir, double mpos,
-                         int mtype, int refrule, float* out) {
+                         int mtype, int image_normals, float* out) {
     TRY(need_init(c));
     if (!c->nf_P) return fail(c, EFS_ERR_STATE, "NF2FF accumulation not configured");
     if (f >= c->nf_F) return fail(c, EFS_ERR_ARG, "frequency index out of range");
@@ -1848,7 +1862,7 @@ static int farfield_impl(efs_ctx* c, uint32_t f, const float* meta, const float*
         q.fac_im = (float)(fmag * std::cos(fph));
         q.mdir = mdir;
         q.mpos = (float)mpos;
-        q.msign = refrule ? 1.0f : -1.0f;
+        q.msign = image_normals ? -1.0f : 1.0f;
         for (int k3 = 0; k3 < 3; ++k3) q.ef[k3] = q.hf[k3] = 1.0f;
         if (mdir >= 0) {
             const int a1 = (mdir + 1) % 3, a2 = (mdir + 2) % 3;
@@ -1859,6 +1873,7 @@ static int farfield_impl(efs_ctx* c, uint32_t f, const float* meta, const float*
             }
         }
         hipLaunchKernelGGL(k_farfield, dim3(ablk, nchunks), dim3(256), 0, c->stream, c->nf_acc + 2 * (size_t)f,
+                           c->nf_cmp ? c->nf_cmp + 2 * (size_t)f : nullptr,
                            c->nf_NC, c->nf_geom.rowsE, d_meta, d_th, d_ph, q, chunk, d_part);
         hipLaunchKernelGGL(k_farfield_final, dim3(ablk), dim3(256), 0, c->stream, d_part, nchunks, q, d_out);
         hipError_t e = hipGetLastError();
@@ -1878,9 +1893,16 @@ int efs_farfield(efs_ctx* c, uint32_t f, const float* meta, const float* theta, 
 
 int efs_farfield_mirror(efs_ctx* c, uint32_t f, const float* meta, const float* theta, uint32_t nT,
                         const float* phi, uint32_t nP, const double* center, double frequency, double radius,
-                        int mirror_dir, double mirror_pos, int mirror_type, int reference_rule, float* out) {
+                        int mirror_dir, double mirror_pos, int mirror_type, int image_normals, float* out) {
     return farfield_impl(c, f, meta, theta, nT, phi, nP, center, frequency, radius, mirror_dir, mirror_pos,
-                         mirror_type, reference_rule, out);
+                         mirror_type, image_normals, out);
+}
+
+int efs_set_nf2ff_compensated(efs_ctx* c, int on) {
+    if (!c) return EFS_ERR_STATE;
+    if (on != 0 && on != 1) return fail(c, EFS_ERR_ARG, "compensated must be 0 or 1");
+    c->nf_comp_req = on != 0;
+    return EFS_OK;
 }
 
 int efs_energy(efs_ctx* c, double* out) {
@@ -1904,6 +1926,19 @@ int efs_energy(efs_ctx* c, double* out) {
     CHECK_LAUNCH(c);
     HIP_TRY(c, hipStreamSynchronize(c->stream));
     HIP_TRY(c, hipMemcpy(out, c->en_partial + kEnergyMaxBlocks, sizeof(double), hipMemcpyDeviceToHost));
+    return EFS_OK;
+}
+
+int efs_get_sizes(const efs_ctx* c, efs_sizes* out) {
+    if (!c || !out) return EFS_ERR_ARG;
+    *out = efs_sizes{};
+    out->nx = c->nx; out->ny = c->ny; out->nz = c->nz;
+    out->gather_n = c->g_n;
+    out->gather_buffered_n = c->gb_n;
+    out->gather_buffered_max = c->gb_max;
+    out->nf2ff_points = c->nf_P;
+    out->nf2ff_freqs = c->nf_F;
+    out->steady_state_n = c->ss_n;
     return EFS_OK;
 }
 

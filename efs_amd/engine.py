@@ -28,6 +28,12 @@ _u32p = np.ctypeslib.ndpointer(dtype=np.uint32, flags="C_CONTIGUOUS")
 _f64p = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
 _lib = None
 
+class EfsSizes(C.Structure):
+    """efs_sizes of include/efs_engine.h."""
+    _fields_ = [(n, C.c_uint32) for n in ("nx", "ny", "nz", "gather_n", "gather_buffered_n", "gather_buffered_max",
+                                          "nf2ff_points", "nf2ff_freqs", "steady_state_n")]
+
+
 # Every entry point of include/efs_engine.h with its ctypes signature.
 SIGNATURES = {
     "efs_version": (C.c_char_p, []),
@@ -68,6 +74,8 @@ SIGNATURES = {
     "efs_profile_enable": (C.c_int, [C.c_void_p, C.c_int]),
     "efs_profile_read": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_uint64)]),
     "efs_path_flags": (C.c_int, [C.c_void_p, C.POINTER(C.c_uint32)]),
+    "efs_get_sizes": (C.c_int, [C.c_void_p, C.POINTER(EfsSizes)]),
+    "efs_set_nf2ff_compensated": (C.c_int, [C.c_void_p, C.c_int]),
     "efs_cfg_pbc": (C.c_int, [C.c_void_p, C.c_uint32, _u32p, _f64p]),
     "efs_set_storage": (C.c_int, [C.c_void_p, C.c_int]),
     "efs_field_scales": (C.c_int, [C.c_void_p, _f32p]),
@@ -93,6 +101,7 @@ SIGNATURES = {
     "efs_batch_num_ts": (C.c_int, [C.c_void_p, C.POINTER(C.c_uint32)]),
     "efs_batch_read_gather": (C.c_int, [C.c_void_p, C.c_uint32, C.c_void_p, C.POINTER(C.c_uint32)]),
     "efs_batch_get_fields": (C.c_int, [C.c_void_p, C.c_uint32, _f32p, _f32p]),
+    "efs_batch_sizes": (C.c_int, [C.c_void_p, C.c_uint32, _u32p, C.POINTER(C.c_uint32)]),
     "efs_batch_info": (C.c_int, [C.c_void_p, C.POINTER(C.c_uint32), C.POINTER(C.c_uint32), C.POINTER(C.c_uint32),
                                  C.POINTER(C.c_uint64)]),
     "efs_batch_mur_split": (C.c_int, [C.c_void_p, C.POINTER(C.c_uint32), C.POINTER(C.c_uint32)]),
@@ -432,13 +441,19 @@ class HIPEngine:
     def configureNF2FFAccumulation(self, config):
         """configureNF2FFAccumulation (src/webgpu-engine.mjs:2729-2827).
         ``omega`` may be a scalar (reference form) or a list of F values;
-        ``kBlock`` sets the buffered steps per DFT fold (default 64)."""
+        ``kBlock`` sets the buffered steps per DFT fold (default 64);
+        ``compensated`` (default True) keeps the accumulators' rounding
+        errors (TwoSum per fold), False adds with a plain FP32 +=."""
         pts = u32(config["surfaceIndices"])
-        P = int(config.get("numPoints", len(pts) // 3))
+        P = len(pts) // 3
+        if "numPoints" in config and int(config["numPoints"]) != P:
+            raise ValueError(f"configureNF2FFAccumulation: numPoints {config['numPoints']} does not match "
+                             f"surfaceIndices ({P} points)")
         if P == 0:
             return
         om = f64(np.atleast_1d(config["omega"]))
         els = [f32(a) for a in list(config["primalEdgeLens"]) + list(config["dualEdgeLens"])]
+        self._check(self.lib.efs_set_nf2ff_compensated(self.ctx, 0 if config.get("compensated", True) is False else 1))
         self._check(self.lib.efs_cfg_nf2ff(self.ctx, pts, P, om, len(om), float(config["dT"]),
                                            f32(np.concatenate(els)), u32([len(a) for a in els]),
                                            int(config.get("maxTS", 0) or 0), int(config.get("windowType", 0) or 0),
@@ -461,8 +476,11 @@ class HIPEngine:
         """computeNF2FFfarField (src/webgpu-engine.mjs:2871-2967): 5 floats
         per angle (P_rad, Et_re, Et_im, Ep_re, Ep_im).  Optional
         config["mirror"] = {"type": "PEC"|"PMC", "direction": 0..2,
-        "position": p, "referenceRule": bool} adds the surface's image in
-        that plane (src/nf2ff.mjs:377-398; efs_farfield_mirror)."""
+        "position": p, "imageNormals": bool} adds the surface's image in
+        that plane (src/nf2ff.mjs:377-398; efs_farfield_mirror).  By default
+        the reflected faces keep their normals, as the reference does;
+        imageNormals=True reverses the normals of the reflected faces
+        perpendicular to the plane (image theory)."""
         th, ph = f32(config["theta"]), f32(config["phi"])
         out = np.empty(5 * len(th) * len(ph), dtype=np.float32)
         m = config.get("mirror")
@@ -476,7 +494,7 @@ class HIPEngine:
                                                  th, len(th), ph, len(ph), f64(config["center"]),
                                                  float(config["frequency"]), float(config["radius"]),
                                                  int(m["direction"]), float(m.get("position", 0.0)), mtype,
-                                                 1 if m.get("referenceRule") else 0, out))
+                                                 1 if m.get("imageNormals") else 0, out))
         return out
 
     # -- energy ----------------------------------------------------------------

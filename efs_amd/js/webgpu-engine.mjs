@@ -253,12 +253,18 @@ export class HIPEngine {
     // configureNF2FFAccumulation (src/webgpu-engine.mjs:2729-2827); omega may
     // be a number (reference form) or an array of angular frequencies.
     configureNF2FFAccumulation(config) {
-        const numPoints = config.numPoints;
+        // the point count is the surface list's; a numPoints that disagrees
+        // with it is refused rather than trusted for the read sizes
+        const numPoints = Math.floor(config.surfaceIndices.length / 3);
+        if (config.numPoints !== undefined && config.numPoints !== numPoints)
+            throw new Error(`configureNF2FFAccumulation: numPoints ${config.numPoints} does not match ` +
+                `surfaceIndices (${numPoints} points)`);
         if (numPoints === 0) return;
         const els = [...config.primalEdgeLens, ...config.dualEdgeLens].map(f32);
         const omegas = typeof config.omega === 'number' ? [config.omega] : Array.from(config.omega);
         addon.cfgNF2FF(this._h, u32(config.surfaceIndices), f64(omegas), config.dT, concatF32(els),
-            u32(els.map((a) => a.length)), (config.maxTS || 0) >>> 0, (config.windowType || 0) >>> 0, (config.kBlock || 0) >>> 0);
+            u32(els.map((a) => a.length)), (config.maxTS || 0) >>> 0, (config.windowType || 0) >>> 0, (config.kBlock || 0) >>> 0,
+            config.compensated === false ? 0 : 1);
         this._nf2ff = { numPoints, F: omegas.length, scalar: typeof config.omega === 'number' };
     }
 
@@ -274,8 +280,10 @@ export class HIPEngine {
     }
 
     // Optional config.mirror = { type: 'PEC' | 'PMC', direction, position,
-    // referenceRule }: the surface's image in that plane (the mirror option of
-    // src/nf2ff.mjs:377-398, with image-theory normals unless referenceRule).
+    // imageNormals }: the surface's image in that plane (the mirror option of
+    // src/nf2ff.mjs:377-398: the reflected faces keep their normals, as there;
+    // imageNormals: true reverses the normals of reflected faces perpendicular
+    // to the plane, as image theory has it).
     async computeNF2FFfarField(config) {
         const out = new Float32Array(5 * config.theta.length * config.phi.length);
         const m = config.mirror;
@@ -286,7 +294,7 @@ export class HIPEngine {
             addon.farfield(this._h, (config.freqIndex || 0) >>> 0, f32(config.pointMeta), f32(config.theta),
                 f32(config.phi), f64(config.center), config.frequency, config.radius, out,
                 m.direction, m.position || 0, String(m.type || 'PEC').toUpperCase() === 'PMC' ? 1 : 0,
-                m.referenceRule ? 1 : 0);
+                m.imageNormals ? 1 : 0);
         }
         return out;
     }

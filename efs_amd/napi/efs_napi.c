@@ -123,6 +123,29 @@ static napi_value status(call_t *c, efs_ctx *ctx, int rc) {
     return undef(c->env);
 }
 
+/* The context's sizes, to check the caller's arrays against what the
+ * engine will read or write (a mismatch throws instead of overrunning the
+ * typed array). */
+static int sizes_of(call_t *c, efs_ctx *ctx, efs_sizes *s) {
+    if (c->failed) return -1;
+    if (efs_get_sizes(ctx, s) != EFS_OK) {
+        throwf(c, "efs: invalid engine handle");
+        return -1;
+    }
+    return 0;
+}
+
+static int check_len(call_t *c, const void *p, size_t len, size_t need, const char *msg) {
+    if (c->failed) return -1;
+    if (p && len != need) {
+        char buf[256];
+        snprintf(buf, sizeof buf, "efs: %s (got %zu, expected %zu)", msg, len, need);
+        throwf(c, buf);
+        return -1;
+    }
+    return 0;
+}
+
 #define BEGIN()                                  \
     call_t c;                                    \
     if (begin(env, info, &c)) return NULL;
@@ -384,6 +407,10 @@ static napi_value js_ss_read(napi_env env, napi_callback_info info) {
     size_t a, b;
     float *e1 = F32OPT(1, a), *e2 = F32OPT(2, b);
     BAIL();
+    efs_sizes s;
+    if (sizes_of(&c, ctx, &s) || check_len(&c, e1, a, s.steady_state_n, "ssRead needs one float per probe") ||
+        check_len(&c, e2, b, s.steady_state_n, "ssRead needs one float per probe"))
+        return NULL;
     uint32_t cur = 0;
     int rec = 0;
     if (!status(&c, ctx, efs_ss_read(ctx, e1, e2, &cur, &rec))) return NULL;
@@ -429,6 +456,12 @@ static napi_value js_get_fields(napi_env env, napi_callback_info info) {
     size_t a, b;
     float *v = F32OPT(1, a), *cu = F32OPT(2, b);
     BAIL();
+    efs_sizes s;
+    if (sizes_of(&c, ctx, &s)) return NULL;
+    const size_t n3 = 3 * (size_t)s.nx * s.ny * s.nz;
+    if (check_len(&c, v, a, n3, "getFields: volt must hold 3*Nx*Ny*Nz floats") ||
+        check_len(&c, cu, b, n3, "getFields: curr must hold 3*Nx*Ny*Nz floats"))
+        return NULL;
     return status(&c, ctx, efs_get_fields(ctx, v, cu));
 }
 
@@ -438,6 +471,12 @@ static napi_value js_upload_fields(napi_env env, napi_callback_info info) {
     size_t a, b;
     float *v = F32OPT(1, a), *cu = F32OPT(2, b);
     BAIL();
+    efs_sizes s;
+    if (sizes_of(&c, ctx, &s)) return NULL;
+    const size_t n3 = 3 * (size_t)s.nx * s.ny * s.nz;
+    if (check_len(&c, v, a, n3, "uploadFields: volt must hold 3*Nx*Ny*Nz floats") ||
+        check_len(&c, cu, b, n3, "uploadFields: curr must hold 3*Nx*Ny*Nz floats"))
+        return NULL;
     return status(&c, ctx, efs_upload_fields(ctx, v, cu));
 }
 
@@ -456,6 +495,9 @@ static napi_value js_read_gather(napi_env env, napi_callback_info info) {
     size_t n;
     float *out = F32(1, n);
     BAIL();
+    efs_sizes s;
+    if (sizes_of(&c, ctx, &s) || check_len(&c, out, n, 2 * (size_t)s.gather_n, "readGather: out must hold 2 floats per index"))
+        return NULL;
     return status(&c, ctx, efs_read_gather(ctx, out));
 }
 
@@ -477,11 +519,23 @@ static napi_value js_read_gather_buffered(napi_env env, napi_callback_info info)
     float *out = F32OPT(1, n);
     BAIL();
     uint32_t ns = 0;
+    efs_sizes s;
+    if (sizes_of(&c, ctx, &s)) return NULL;
+    if (out) {
+        if (!status(&c, ctx, efs_read_gather_buffered(ctx, NULL, &ns))) return NULL;
+        if (n < (size_t)ns * 2 * s.gather_buffered_n) {
+            char buf[200];
+            snprintf(buf, sizeof buf, "efs: readGatherBuffered: out holds %zu floats, %zu needed", n,
+                     (size_t)ns * 2 * s.gather_buffered_n);
+            throwf(&c, buf);
+            return NULL;
+        }
+    }
     if (!status(&c, ctx, efs_read_gather_buffered(ctx, out, &ns))) return NULL;
     return num(env, ns);
 }
 
-/* cfgNF2FF(h, points, omega(F64), dT, edgeLens, elLen, maxTS, windowType, kBlock) */
+/* cfgNF2FF(h, points, omega(F64), dT, edgeLens, elLen, maxTS, windowType, kBlock[, compensated = 1]) */
 static napi_value js_cfg_nf2ff(napi_env env, napi_callback_info info) {
     BEGIN();
     CTX(0);
@@ -492,7 +546,9 @@ static napi_value js_cfg_nf2ff(napi_env env, napi_callback_info info) {
     float *el = F32(4, ne);
     uint32_t *ell = U32(5, nl);
     uint32_t maxTS = u32_arg(&c, 6), win = u32_arg(&c, 7), kb = u32_arg(&c, 8);
+    const uint32_t comp = c.argc > 9 ? u32_arg(&c, 9) : 1;
     BAIL();
+    if (!status(&c, ctx, efs_set_nf2ff_compensated(ctx, comp ? 1 : 0))) return NULL;
     if (np_ % 3 || nl != 6) {
         throwf(&c, "efs: NF2FF points must be triplets and elLen must hold 6 lengths");
         return NULL;
@@ -512,11 +568,17 @@ static napi_value js_read_nf2ff(napi_env env, napi_callback_info info) {
     size_t a, b;
     float *e = F32(1, a), *h = F32(2, b);
     BAIL();
+    efs_sizes s;
+    if (sizes_of(&c, ctx, &s)) return NULL;
+    const size_t need = (size_t)s.nf2ff_freqs * 6 * s.nf2ff_points;
+    if (check_len(&c, e, a, need, "readNF2FF: accumE must hold F*6*P floats") ||
+        check_len(&c, h, b, need, "readNF2FF: accumH must hold F*6*P floats"))
+        return NULL;
     return status(&c, ctx, efs_read_nf2ff(ctx, e, h));
 }
 
 /* farfield(h, fIdx, meta, theta, phi, center(F64[3]), freq, radius, out
- *          [, mirrorDir, mirrorPos, mirrorType, referenceRule]) */
+ *          [, mirrorDir, mirrorPos, mirrorType, imageNormals]) */
 static napi_value js_farfield(napi_env env, napi_callback_info info) {
     BEGIN();
     CTX(0);
@@ -530,16 +592,20 @@ static napi_value js_farfield(napi_env env, napi_callback_info info) {
     const int mdir = mirror ? (int)f64_arg(&c, 9) : -1;
     const double mpos = mirror ? f64_arg(&c, 10) : 0.0;
     const int mtype = mirror ? (int)f64_arg(&c, 11) : 0;
-    const int refrule = mirror ? (int)f64_arg(&c, 12) : 0;
+    const int image_normals = mirror ? (int)f64_arg(&c, 12) : 0;
     BAIL();
     if (lc != 3 || lo != 5 * lt * lp) {
         throwf(&c, "efs: far-field center must hold 3 values and out 5 per angle");
         return NULL;
     }
+    efs_sizes s;
+    if (sizes_of(&c, ctx, &s) || check_len(&c, meta, lm, 8 * (size_t)s.nf2ff_points,
+                                           "farfield: pointMeta must hold 8 floats per NF2FF point"))
+        return NULL;
     if (!mirror)
         return status(&c, ctx, efs_farfield(ctx, f, meta, th, (uint32_t)lt, ph, (uint32_t)lp, ce, freq, rad, out));
     return status(&c, ctx, efs_farfield_mirror(ctx, f, meta, th, (uint32_t)lt, ph, (uint32_t)lp, ce, freq, rad,
-                                               mdir, mpos, mtype, refrule, out));
+                                               mdir, mpos, mtype, image_normals, out));
 }
 
 static napi_value js_energy(napi_env env, napi_callback_info info) {
@@ -733,7 +799,17 @@ static napi_value js_batch_read_gather(napi_env env, napi_callback_info info) {
     uint32_t id = u32_arg(&c, 1);
     float *out = F32OPT(2, n);
     BAIL();
-    uint32_t ns = 0;
+    uint32_t ns = 0, gn = 0;
+    if (!bstatus(&c, b, efs_batch_sizes(b, id, NULL, &gn))) return NULL;
+    if (out) {
+        if (!bstatus(&c, b, efs_batch_read_gather(b, id, NULL, &ns))) return NULL;
+        if (n < (size_t)ns * 2 * gn) {
+            char buf[200];
+            snprintf(buf, sizeof buf, "efs: batchReadGather: out holds %zu floats, %zu needed", n, (size_t)ns * 2 * gn);
+            throwf(&c, buf);
+            return NULL;
+        }
+    }
     if (!bstatus(&c, b, efs_batch_read_gather(b, id, out, &ns))) return NULL;
     return num(env, ns);
 }
@@ -745,6 +821,12 @@ static napi_value js_batch_get_fields(napi_env env, napi_callback_info info) {
     uint32_t id = u32_arg(&c, 1);
     float *v = F32(2, a), *cu = F32(3, d);
     BAIL();
+    uint32_t dims[3] = {0, 0, 0};
+    if (!bstatus(&c, b, efs_batch_sizes(b, id, dims, NULL))) return NULL;
+    const size_t n3 = 3 * (size_t)dims[0] * dims[1] * dims[2];
+    if (check_len(&c, v, a, n3, "batchGetFields: volt must hold 3*Nx*Ny*Nz floats") ||
+        check_len(&c, cu, d, n3, "batchGetFields: curr must hold 3*Nx*Ny*Nz floats"))
+        return NULL;
     return bstatus(&c, b, efs_batch_get_fields(b, id, v, cu));
 }
 

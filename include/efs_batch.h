@@ -92,6 +92,9 @@ int efs_batch_read_gather(efs_batch *b, uint32_t id, float *out, uint32_t *num_s
 
 /* getFields of instance id (3*nx*ny*nz floats each, NIJK, real units). */
 int efs_batch_get_fields(efs_batch *b, uint32_t id, float *volt, float *curr);
+/* Grid size (dims[3], unpadded) and number of gathered indices of instance
+ * id, for bindings that check the caller's buffers. */
+int efs_batch_sizes(efs_batch *b, uint32_t id, uint32_t *dims, uint32_t *gather_n);
 
 /* Launch shape in use: blocks per sweep launch, waves per SIMD of the sweep
  * kernels, steps per graph replay, total cells (sum over instances). */

@@ -183,20 +183,45 @@ int efs_farfield(efs_ctx *ctx, uint32_t freq_idx, const float *point_meta, const
  * the surface is added again reflected in the plane x[mirror_dir] =
  * mirror_pos with the image fields of a PEC (mirror_type 0: tangential E and
  * normal H reversed) or PMC (1: normal E and tangential H reversed) plane.
- * The reflected copy of a face perpendicular to the plane faces the other
- * way, so its normal sign is reversed (image theory); reference_rule = 1
- * keeps the original normals, as the reference does.  mirror_dir = -1 is
- * efs_farfield. */
+ * image_normals = 0 (the reference's rule) keeps the original face normals
+ * in the reflected copy, as src/nf2ff.mjs:377-398 does; image_normals = 1
+ * reverses the normal sign of the reflected copy of each face perpendicular
+ * to the plane, which faces the other way (image theory).  mirror_dir = -1
+ * is efs_farfield. */
 int efs_farfield_mirror(efs_ctx *ctx, uint32_t freq_idx, const float *point_meta, const float *theta,
                         uint32_t n_theta, const float *phi, uint32_t n_phi, const double *center, double frequency,
-                        double radius, int mirror_dir, double mirror_pos, int mirror_type, int reference_rule,
+                        double radius, int mirror_dir, double mirror_pos, int mirror_type, int image_normals,
                         float *out);
+
+/* NF2FF accumulation form (no counterpart in the reference, which adds each
+ * step's term into FP32 accumulators with a plain +=,
+ * src/webgpu-engine.mjs:785-798): 1 (default) = compensated, every DFT
+ * fold adds its block into the accumulators with an error-free TwoSum and
+ * keeps the rounding errors in a second array, so the read value (acc +
+ * error) does not drift with the number of folds; 0 = plain FP32 +=.
+ * Takes effect at the next efs_cfg_nf2ff. */
+int efs_set_nf2ff_compensated(efs_ctx *ctx, int on);
 
 /* computeEnergy() (src/webgpu-engine.mjs:2974-3158): sum of volt^2+curr^2
  * over the (nx-1)(ny-1)(nz-1) interior, all components; FP32 squares,
  * FP64 reduction.  Synchronous (returns the current value, not the
  * previous call's); the lagging form is built on top by the host wrappers. */
 int efs_energy(efs_ctx *ctx, double *out);
+
+/* Sizes of what the context holds, for bindings that check the caller's
+ * buffers before a read or upload (no counterpart in the reference, whose
+ * readbacks are sized by WebGPU): grid, plain and buffered gather lists,
+ * NF2FF points and frequencies, steady-state probes.  Zero where nothing is
+ * configured. */
+typedef struct {
+    uint32_t nx, ny, nz;
+    uint32_t gather_n;                   /* efs_cfg_gather indices */
+    uint32_t gather_buffered_n;          /* efs_cfg_gather_buffered indices */
+    uint32_t gather_buffered_max;        /* its max_samples */
+    uint32_t nf2ff_points, nf2ff_freqs;  /* P, F */
+    uint32_t steady_state_n;             /* steady-state probes */
+} efs_sizes;
+int efs_get_sizes(const efs_ctx *ctx, efs_sizes *out);
 
 /* Which sweep forms the context uses (diagnostic; no counterpart in the
  * reference): bit set = form in use. */

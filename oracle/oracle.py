@@ -361,7 +361,9 @@ class Oracle:
 def farfield(accE, accH, pointMeta, theta, phi, center, frequency, radius, fp64=True, mirror=None):
     """Far-field per (theta, phi): 5 values [P_rad, Et_re, Et_im, Ep_re, Ep_im].
     mirror: {"type": "PEC"|"PMC", "direction": 0..2, "position": p,
-    "referenceRule": bool} adds the image of the surface in that plane."""
+    "imageNormals": bool} adds the image of the surface in that plane; the
+    reflected faces keep their normals (the reference's rule,
+    src/nf2ff.mjs:377-398) unless imageNormals is set."""
     lib = _load()
     P = len(pointMeta) // 8
     th, ph = _f64(theta), _f64(phi)
@@ -374,5 +376,5 @@ def farfield(accE, accH, pointMeta, theta, phi, center, frequency, radius, fp64=
     fn = lib.orc64_farfield_mirror if fp64 else lib.orc32_farfield_mirror
     fn(_f64(accE), _f64(accH), _f64(pointMeta), P, th, len(th), ph, len(ph), _f64(center), float(frequency),
        float(radius), int(mirror["direction"]), float(mirror.get("position", 0.0)),
-       0 if str(mirror.get("type", "PEC")).upper() == "PEC" else 1, 1 if mirror.get("referenceRule") else 0, out)
+       0 if str(mirror.get("type", "PEC")).upper() == "PEC" else 1, 0 if mirror.get("imageNormals") else 1, out)
     return out

@@ -163,8 +163,9 @@ def test_mirror_is_image_theory(oracle_mod, kind, pdir):
     closed box above the plane, far field with a PEC / PMC mirror: equal to
     the far field of the element and its image (PEC: horizontal currents
     reversed; PMC: vertical reversed) from a box enclosing both, to 1e-3 of
-    the peak.  The reference's rule (mirrored faces keep their normals,
-    src/nf2ff.mjs:377-398) misses this by up to half the peak."""
+    the peak, with imageNormals set.  The default, the reference's rule
+    (mirrored faces keep their normals, src/nf2ff.mjs:377-398), misses this
+    by up to half the peak."""
     f = 1e9
     k = 2 * np.pi * f / 299792458.0
     h, a = 0.06, 0.03
@@ -177,10 +178,10 @@ def test_mirror_is_image_theory(oracle_mod, kind, pdir):
     E1, H1 = _dipole_fields(pts2, p_hat, np.array([0, 0, h]), k)
     E2, H2 = _dipole_fields(pts2, img, np.array([0, 0, -h]), k)
     ref = oracle_mod.farfield(_acc(E1 + E2), _acc(H1 + H2), meta2.ravel(), theta, phi, np.zeros(3), f, 1.0)
-    mir = {"type": kind, "direction": 2, "position": 0.0}
+    mir = {"type": kind, "direction": 2, "position": 0.0, "imageNormals": True}
     got = oracle_mod.farfield(_acc(E), _acc(H), meta.ravel(), theta, phi, np.zeros(3), f, 1.0, mirror=mir)
     U, Ur = got.reshape(-1, 5)[:, 0], ref.reshape(-1, 5)[:, 0]
     assert np.max(np.abs(U - Ur)) < 1e-3 * Ur.max()
     old = oracle_mod.farfield(_acc(E), _acc(H), meta.ravel(), theta, phi, np.zeros(3), f, 1.0,
-                              mirror=dict(mir, referenceRule=True)).reshape(-1, 5)[:, 0]
+                              mirror=dict(mir, imageNormals=False)).reshape(-1, 5)[:, 0]
     assert np.max(np.abs(old - Ur)) > 0.02 * Ur.max()

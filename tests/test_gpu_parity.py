@@ -500,11 +500,12 @@ def test_farfield_matches_fp64_oracle(HIPEngine, oracle_mod):
     ref = ref.reshape(-1, 5)
     assert rel_l2(out[:, 0], ref[:, 0]) < 1e-5
     assert rel_l2(out[:, 1:], ref[:, 1:]) < 1e-5
-    # the mirror option (image in a PEC / PMC plane; image-theory normals or
-    # the reference's rule) against the FP64 oracle's restatement
+    # the mirror option (image in a PEC / PMC plane; the reference's rule by
+    # default, image-theory normals on request) against the FP64 oracle's
+    # restatement
     for mirror in ({"type": "PEC", "direction": 2, "position": 1e-3},
-                   {"type": "PMC", "direction": 0, "position": -2e-3},
-                   {"type": "PEC", "direction": 1, "position": 0.0, "referenceRule": True}):
+                   {"type": "PMC", "direction": 0, "position": -2e-3, "imageNormals": True},
+                   {"type": "PEC", "direction": 1, "position": 0.0, "imageNormals": True}):
         out = g.computeNF2FFfarField({"pointMeta": meta.ravel(), "theta": theta, "phi": phi, "center": center,
                                       "frequency": freq, "radius": 1.0, "mirror": mirror}).reshape(-1, 5)
         ref = oracle_mod.farfield(acc["accumE"][0].astype(np.float64), acc["accumH"][0].astype(np.float64),

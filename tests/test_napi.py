@@ -147,3 +147,21 @@ def test_js_farfield_mirror(tmp_path):
     np.testing.assert_array_equal(np.asarray(got["plain"], np.float32), plain)
     np.testing.assert_array_equal(np.asarray(got["mirrored"], np.float32), mirrored)
     assert np.max(np.abs(mirrored - plain)) > 1e-3 * np.max(np.abs(plain))
+
+
+@needs_node
+@pytest.mark.gpu
+def test_js_addon_checks_array_lengths(tmp_path):
+    """Every addon read or upload that touches a caller's typed array
+    throws when the array's length is not the size the engine holds
+    (grid, gather lists, NF2FF points and frequencies), instead of
+    overrunning it; the right lengths still work."""
+    dst = tmp_path / "lengths.json"
+    print("node lengths", flush=True)
+    res = subprocess.run([NODE, os.path.join(ROOT, "tests", "js", "check_lengths.mjs"), str(dst)],
+                         capture_output=True, text=True, timeout=120, cwd=ROOT)
+    assert res.returncode == 0, res.stderr
+    got = json.loads(dst.read_text())
+    assert got.pop("samples") == 4
+    for name, msg in got.items():
+        assert msg is not None and msg.startswith("efs:"), (name, msg)
