@@ -105,6 +105,11 @@ struct efs_ctx {
     // device and adjusted every range_interval steps by the range kernels
     int storage_req = EFS_STORAGE_F32;
     bool half = false;
+    // FP64 storage (the precision reference run of the same engine): fields,
+    // UPML flux, Mur state and the NF2FF buffer and accumulators in double,
+    // updated in double by the 1-wide sweeps; coefficients stay the FP32
+    // values given.  Arrays of R = double take twice the floats (rw()).
+    bool dbl = false;
     float* fscale = nullptr;        // [6]
     unsigned int* f_amax = nullptr; // [6] running max |stored| (float bits)
     float* f_factor = nullptr;      // [6] last rescale factors
@@ -545,7 +550,19 @@ int nf2ff_fold(efs_ctx* c) {
     if (!c->nf_P || c->nf_slot == 0) return EFS_OK;
     const uint32_t kused = c->nf_slot;
     const uint32_t nt = 2 * kused * c->nf_NC;
-    hipLaunchKernelGGL(k_nf2ff_twiddle, dim3(nblk(nt)), dim3(kPointThreads), 0, c->stream, c->nf_tw, c->nf_K,
+    if (c->dbl) {  // FP64 storage: FP64 twiddles, buffer and accumulators
+        hipLaunchKernelGGL(k_nf2ff_twiddle<double>, dim3(nblk(nt)), dim3(kPointThreads), 0, c->stream,
+                           (double*)c->nf_tw, c->nf_K, kused, c->nf_F, c->nf_NC, c->nf_ts0, c->nf_omega, c->nf_dT,
+                           c->nf_maxTS, c->nf_window, c->nf_wnorm);
+        const dim3 grid(c->nf_rows / kFold64Rows, c->nf_NC / kFoldCols);
+        hipLaunchKernelGGL(k_nf2ff_fold64, grid, dim3(kFold64Rows), (size_t)kused * kFoldCols * sizeof(double),
+                           c->stream, (const double*)c->nf_buf, c->nf_rows, c->nf_geom.rowsE, c->nf_K, kused,
+                           (const double*)c->nf_tw, c->nf_NC, (double*)c->nf_acc);
+        CHECK_LAUNCH(c);
+        c->nf_slot = 0;
+        return EFS_OK;
+    }
+    hipLaunchKernelGGL(k_nf2ff_twiddle<float>, dim3(nblk(nt)), dim3(kPointThreads), 0, c->stream, c->nf_tw, c->nf_K,
                        kused, c->nf_F, c->nf_NC, c->nf_ts0, c->nf_omega, c->nf_dT, c->nf_maxTS, c->nf_window,
                        c->nf_wnorm);
     CHECK_LAUNCH(c);
@@ -730,6 +747,42 @@ FieldView fv(const efs_ctx* c, int curr) {
                      c->half ? 1u : 0u};
 }
 
+FieldView64 fv64(const efs_ctx* c, int curr) {
+    return FieldView64{reinterpret_cast<double*>(curr ? c->curr : c->volt), (uint64_t)c->S};
+}
+
+// Floats per element of the arrays held in the arithmetic type (2 for FP64).
+inline size_t rw(const efs_ctx* c) { return c->dbl ? 2 : 1; }
+
+template <class FV>
+typename FV::R* rp(float* p) { return reinterpret_cast<typename FV::R*>(p); }
+
+// The point-kernel launches that the FP64 form shares, for view type FV.
+template <class FV>
+void launch_mur_pre(efs_ctx* c, FV v) {
+    hipLaunchKernelGGL(k_mur_pre<FV>, dim3(nblk(c->mur_n, c->pt)), dim3(c->pt), 0, c->stream, v,
+                       rp<FV>(c->pml_vflux), c->mur_coeff, c->mur_normal_src, c->mur_shifted_src,
+                       rp<FV>(c->mur_saved), c->mur_n);
+}
+
+template <class FV>
+void launch_mur_post_some(efs_ctx* c, FV v) {
+    hipLaunchKernelGGL(k_mur_post_some<FV>, dim3(nblk(c->mur_ne, c->pt)), dim3(c->pt), 0, c->stream, v,
+                       c->mur_coeff, c->mur_shifted, c->mur_early_list, rp<FV>(c->mur_saved), c->mur_ne);
+}
+
+template <class FV>
+void launch_excitation(efs_ctx* c, FV v, uint32_t t) {
+    hipLaunchKernelGGL(k_excitation<FV>, dim3(nblk(c->exc_n, c->pt)), dim3(c->pt), 0, c->stream, v, c->exc_signal,
+                       c->exc_len, c->exc_period, c->exc_delay, c->exc_amp, c->exc_gidx, c->exc_n, t);
+}
+
+template <class FV>
+void launch_mur_finish(efs_ctx* c, FV v) {
+    hipLaunchKernelGGL(k_mur_finish<FV>, dim3(nblk(c->mur_n, c->pt)), dim3(c->pt), 0, c->stream, v, c->mur_coeff,
+                       c->mur_normal, c->mur_shifted, c->mur_early, rp<FV>(c->mur_saved), c->mur_n);
+}
+
 // Whether a hook that changes H after the H sweep writes inside the
 // captured rows (z planes zf-1 .. zf+2 of the capture rectangle): TFSF
 // current points, or ADE write-back cells with live current coefficients
@@ -840,9 +893,8 @@ int launch_step(efs_ctx* c) {
                            c->ade_va, c->ade_vl, c->ade_n);
     }
     if (c->mur_n) {
-        hipLaunchKernelGGL(k_mur_pre, dim3(nblk(c->mur_n, c->pt)), dim3(c->pt), 0, c->stream, fv(c, 0),
-                           c->pml_vflux, c->mur_coeff, c->mur_normal_src, c->mur_shifted_src, c->mur_saved,
-                           c->mur_n);
+        if (c->dbl) launch_mur_pre(c, fv64(c, 0));
+        else launch_mur_pre(c, fv(c, 0));
     }
     if (c->rlc_n) {
         hipLaunchKernelGGL(k_rlc_pre, dim3(nblk(c->rlc_n, c->pt)), dim3(c->pt), 0, c->stream, c->rlc_type,
@@ -875,6 +927,11 @@ int launch_step(efs_ctx* c) {
                                    dim3(kSweepThreads), 0,
                                    c->stream, c->volt, c->curr, cf, c->geom, c->pml, c->pml_coef, c->pml_vflux,
                                    c->fscale, NfCapture{});
+        } else if (c->dbl) {
+            auto kern = has_pml ? sweep_e1<true, double> : sweep_e1<false, double>;
+            hipLaunchKernelGGL(kern, dim3(c->geom1.blocks), dim3(kSweepThreads), 0, c->stream, (double*)c->volt,
+                               (const double*)c->curr, c->vv, c->vi, c->geom1, c->pml, c->pml_coef,
+                               (double*)c->pml_vflux);
         } else {
             auto kern = has_pml ? sweep_e1<true> : sweep_e1<false>;
             hipLaunchKernelGGL(kern, dim3(c->geom1.blocks), dim3(kSweepThreads), 0, c->stream, c->volt, c->curr,
@@ -891,22 +948,22 @@ int launch_step(efs_ctx* c) {
         hipLaunchKernelGGL(k_pbc, dim3(nblk((size_t)a.b0 * a.b1, c->pt)), dim3(c->pt), 0, c->stream, fv(c, 0),
                            c->pbc_imag, a, c->ny, c->nzp, (uint64_t)c->S);
     if (c->mur_n) {
-        if (c->mur_ne)
-            hipLaunchKernelGGL(k_mur_post_some, dim3(nblk(c->mur_ne, c->pt)), dim3(c->pt), 0, c->stream, fv(c, 0),
-                               c->mur_coeff, c->mur_shifted, c->mur_early_list, c->mur_saved, c->mur_ne);
+        if (c->mur_ne) {
+            if (c->dbl) launch_mur_post_some(c, fv64(c, 0));
+            else launch_mur_post_some(c, fv(c, 0));
+        }
     }
     if (c->ade_n && c->ade_apply) {
         hipLaunchKernelGGL(k_ade_apply, dim3(nblk(c->ade_n, c->pt)), dim3(c->pt), 0, c->stream, fv(c, 0),
                            c->ade_gidx, c->ade_va, c->ade_n);
     }
     if (c->exc_n) {
-        hipLaunchKernelGGL(k_excitation, dim3(nblk(c->exc_n, c->pt)), dim3(c->pt), 0, c->stream, fv(c, 0),
-                           c->exc_signal, c->exc_len, c->exc_period, c->exc_delay, c->exc_amp, c->exc_gidx,
-                           c->exc_n, t);
+        if (c->dbl) launch_excitation(c, fv64(c, 0), t);
+        else launch_excitation(c, fv(c, 0), t);
     }
     if (c->mur_n) {
-        hipLaunchKernelGGL(k_mur_finish, dim3(nblk(c->mur_n, c->pt)), dim3(c->pt), 0, c->stream, fv(c, 0),
-                           c->mur_coeff, c->mur_normal, c->mur_shifted, c->mur_early, c->mur_saved, c->mur_n);
+        if (c->dbl) launch_mur_finish(c, fv64(c, 0));
+        else launch_mur_finish(c, fv(c, 0));
     }
     if (c->rlc_n) {
         hipLaunchKernelGGL(k_rlc_apply, dim3(nblk(c->rlc_n, c->pt)), dim3(c->pt), 0, c->stream, fv(c, 0),
@@ -945,6 +1002,11 @@ int launch_step(efs_ctx* c) {
                                    dim3(kSweepThreads), 0,
                                    c->stream, c->curr, c->volt, cf, c->geom, c->pml, c->pml_coef, c->pml_cflux,
                                    c->fscale, NfCapture{});
+        } else if (c->dbl) {
+            auto kern = has_pml ? sweep_h1<true, double> : sweep_h1<false, double>;
+            hipLaunchKernelGGL(kern, dim3(c->geom1.blocks), dim3(kSweepThreads), 0, c->stream, (double*)c->curr,
+                               (const double*)c->volt, c->ii, c->iv, c->geom1, c->pml, c->pml_coef,
+                               (double*)c->pml_cflux);
         } else {
             auto kern = has_pml ? sweep_h1<true> : sweep_h1<false>;
             hipLaunchKernelGGL(kern, dim3(c->geom1.blocks), dim3(kSweepThreads), 0, c->stream, c->curr, c->volt,
@@ -967,16 +1029,26 @@ int launch_step(efs_ctx* c) {
     // engine hooks
     if (c->nf_P) {
         if (c->nf_slot == 0) c->nf_ts0 = t + 1;
-        hipLaunchKernelGGL(k_nf2ff_sample, dim3(nblk(c->nf_P, c->pt)), dim3(c->pt), 0, c->stream, fv(c, 0),
-                           fv(c, 1), c->nf_pts, c->nf_el, c->nf_geom,
-                           c->nf_buf + (size_t)c->nf_slot * c->nf_rows, nf_capture(c));
+        if (c->dbl)
+            hipLaunchKernelGGL(k_nf2ff_sample<FieldView64>, dim3(nblk(c->nf_P, c->pt)), dim3(c->pt), 0, c->stream,
+                               fv64(c, 0), fv64(c, 1), c->nf_pts, c->nf_el, c->nf_geom,
+                               (double*)c->nf_buf + (size_t)c->nf_slot * c->nf_rows, NfCapture{});
+        else
+            hipLaunchKernelGGL(k_nf2ff_sample<FieldView>, dim3(nblk(c->nf_P, c->pt)), dim3(c->pt), 0, c->stream,
+                               fv(c, 0), fv(c, 1), c->nf_pts, c->nf_el, c->nf_geom,
+                               c->nf_buf + (size_t)c->nf_slot * c->nf_rows, nf_capture(c));
         c->nf_slot++;
         if (c->nf_slot == c->nf_K) TRY(nf2ff_fold(c));
     }
     if (c->half && (t + 1) % c->range_interval == 0) launch_range_control(c, t);
     if (c->gb_n && (t + 1) % c->gb_interval == 0 && c->gb_count < c->gb_max) {
-        hipLaunchKernelGGL(k_gather, dim3(nblk(c->gb_n, c->pt)), dim3(c->pt), 0, c->stream, fv(c, 0), fv(c, 1),
-                           c->gb_idx, c->gb_ring + (size_t)c->gb_count * 2 * c->gb_n, c->gb_n);
+        float* ring = c->gb_ring + (size_t)c->gb_count * 2 * c->gb_n;
+        if (c->dbl)
+            hipLaunchKernelGGL(k_gather<FieldView64>, dim3(nblk(c->gb_n, c->pt)), dim3(c->pt), 0, c->stream,
+                               fv64(c, 0), fv64(c, 1), c->gb_idx, ring, c->gb_n);
+        else
+            hipLaunchKernelGGL(k_gather<FieldView>, dim3(nblk(c->gb_n, c->pt)), dim3(c->pt), 0, c->stream, fv(c, 0),
+                               fv(c, 1), c->gb_idx, ring, c->gb_n);
         c->gb_count++;
     }
     CHECK_LAUNCH(c);
@@ -1023,6 +1095,7 @@ int efs_create(efs_ctx** out, int device) {
     if (e == hipSuccess) e = hipStreamCreate(&c->stream);
     if (e != hipSuccess) {
         g_create_error = std::string("HIP setup failed: ") + hipGetErrorString(e);
+        if (c->stream) hipStreamDestroy(c->stream);
         delete c;
         return EFS_ERR_HIP;
     }
@@ -1070,14 +1143,16 @@ int efs_init(efs_ctx* c, uint32_t nx, uint32_t ny, uint32_t nz, const float* vv,
     c->S = ((Np + 255) / 256) * 256 + (pad / 4) * 4;
     if (3 * c->S >= (1ull << 31)) return fail(c, EFS_ERR_ARG, "grid too large for 32-bit device indices");
     const uint64_t arr = 3 * c->S + (stagger / 64) * 64;
-    TRY(dalloc(c, c->pool, 6 * arr));
-    HIP_TRY(c, hipMemset(c->pool, 0, 6 * arr * sizeof(float)));
+    c->dbl = c->storage_req == EFS_STORAGE_F64;
+    const uint64_t farr = c->dbl ? 2 * arr : arr;  // the two field arrays (double: twice the floats)
+    TRY(dalloc(c, c->pool, 2 * farr + 4 * arr));
+    HIP_TRY(c, hipMemset(c->pool, 0, (2 * farr + 4 * arr) * sizeof(float)));
     c->volt = c->pool;
-    c->curr = c->pool + arr;
-    c->vv = c->pool + 2 * arr;
-    c->vi = c->pool + 3 * arr;
-    c->ii = c->pool + 4 * arr;
-    c->iv = c->pool + 5 * arr;
+    c->curr = c->pool + farr;
+    c->vv = c->pool + 2 * farr;
+    c->vi = c->vv + arr;
+    c->ii = c->vv + 2 * arr;
+    c->iv = c->vv + 3 * arr;
     // coefficients in the device row layout: padded copies when nzp > nz,
     // with the last real z plane of the H side set to ii = 1, iv = 0 (those
     // cells keep I, the reference's last-plane rule, src/webgpu-engine.mjs:88)
@@ -1100,7 +1175,8 @@ int efs_init(efs_ctx* c, uint32_t nx, uint32_t ny, uint32_t nz, const float* vv,
     for (int a = 0; a < 4; ++a)
         for (int n = 0; n < 3; ++n)
             HIP_TRY(c, hipMemcpy(dst[a] + n * c->S, src[a] + n * Np, Np * sizeof(float), hipMemcpyHostToDevice));
-    c->vec4 = c->nzp % 4 == 0 && env_u64("EFS_VEC", 4) == 4;
+    // FP64 storage runs the 1-wide sweeps (on the same padded rows)
+    c->vec4 = !c->dbl && c->nzp % 4 == 0 && env_u64("EFS_VEC", 4) == 4;
     c->half = c->storage_req == EFS_STORAGE_F16;
     c->range_interval = (uint32_t)std::max<uint64_t>(1, env_u64("EFS_RANGE_INTERVAL", 32));
     if (c->half && !c->vec4) return fail(c, EFS_ERR_ARG, "FP16 field storage needs the 4-wide sweeps (EFS_VEC=4)");
@@ -1294,8 +1370,8 @@ int efs_cfg_pml(efs_ctx* c, uint32_t nreg, const uint32_t* start, const uint32_t
             off_in += T;
         }
         TRY(dupload(c, c->pml_coef, coef.data(), coef.size()));
-        TRY(dzero(c, c->pml_vflux, total));
-        TRY(dzero(c, c->pml_cflux, total));
+        TRY(dzero(c, c->pml_vflux, total * rw(c)));
+        TRY(dzero(c, c->pml_cflux, total * rw(c)));
     } else {
         dfree(c->pml_coef); dfree(c->pml_vflux); dfree(c->pml_cflux);
     }
@@ -1381,7 +1457,7 @@ int efs_cfg_mur(efs_ctx* c, uint32_t n, const float* coeff, const uint32_t* norm
     const std::vector<uint32_t> tn = tr_all(c, hn.data(), m), ts = tr_all(c, hs.data(), m);
     TRY(dupload(c, c->mur_normal, tn.data(), m));
     TRY(dupload(c, c->mur_shifted, ts.data(), m));
-    TRY(dzero(c, c->mur_saved, m));
+    TRY(dzero(c, c->mur_saved, m * rw(c)));
     c->mur_n = m;
     c->alias_dirty = true;
     return EFS_OK;
@@ -1400,10 +1476,18 @@ int efs_cfg_ade_clear(efs_ctx* c) {
     return EFS_OK;
 }
 
+// Extensions the FP64 storage form does not run (it covers the core sweeps,
+// UPML, Mur, excitation, gathers, NF2FF and energy).
+static int need_not_f64(efs_ctx* c, uint32_t n, const char* what) {
+    if (c->dbl && n) return fail(c, EFS_ERR_ARG, "%s is not supported with FP64 field storage", what);
+    return EFS_OK;
+}
+
 int efs_cfg_ade_add(efs_ctx* c, uint32_t has_lorentz, uint32_t dir, uint32_t ncells, const uint32_t* pos_idx,
                     const float* v_int, const float* v_ext, const float* v_lor, const float* i_int,
                     const float* i_ext, const float* i_lor) {
     TRY(need_init(c));
+    TRY(need_not_f64(c, ncells, "ADE"));
     if (dir > 2) return fail(c, EFS_ERR_ARG, "ADE direction %u out of range", dir);
     if (ncells && !pos_idx) return fail(c, EFS_ERR_ARG, "ADE pos_idx is required");
     for (uint32_t k = 0; k < ncells; ++k)
@@ -1433,6 +1517,7 @@ int efs_cfg_tfsf(efs_ctx* c, const float* signal, uint32_t len, uint32_t period,
                  uint32_t ni, const uint32_t* i_idx, const uint32_t* i_dint, const float* i_dfrac,
                  const float* i_amp) {
     TRY(need_init(c));
+    TRY(need_not_f64(c, nv + ni, "TFSF"));
     if ((nv || ni) && (!signal || len == 0)) return fail(c, EFS_ERR_ARG, "TFSF signal must not be empty");
     for (uint32_t k = 0; k < nv; ++k)
         if (v_idx[k] >= 3 * c->N) return fail(c, EFS_ERR_ARG, "TFSF voltage point %u out of range", k);
@@ -1470,6 +1555,7 @@ int efs_cfg_tfsf(efs_ctx* c, const float* signal, uint32_t len, uint32_t period,
 int efs_cfg_rlc(efs_ctx* c, uint32_t n, const uint32_t* field_idx, const uint32_t* direction,
                 const uint32_t* type_flag, const float* coeffs) {
     TRY(need_init(c));
+    TRY(need_not_f64(c, n, "RLC"));
     HIP_TRY(c, hipStreamSynchronize(c->stream));
     if (n == 0) {
         c->rlc_n = 0;
@@ -1493,6 +1579,7 @@ int efs_cfg_rlc(efs_ctx* c, uint32_t n, const uint32_t* field_idx, const uint32_
 
 int efs_cfg_pbc(efs_ctx* c, uint32_t n, const uint32_t* axis, const double* phase) {
     TRY(need_init(c));
+    TRY(need_not_f64(c, n, "PBC"));
     HIP_TRY(c, hipStreamSynchronize(c->stream));
     if (n > 3) return fail(c, EFS_ERR_ARG, "at most 3 periodic axes");
     if (n && (!axis || !phase)) return fail(c, EFS_ERR_ARG, "PBC axes and phases are required");
@@ -1530,6 +1617,7 @@ int efs_cfg_pbc(efs_ctx* c, uint32_t n, const uint32_t* axis, const double* phas
 
 int efs_cfg_steady_state(efs_ctx* c, const uint32_t* probe_idx, uint32_t n, uint32_t period_samples) {
     TRY(need_init(c));
+    TRY(need_not_f64(c, n, "steady-state detection"));
     HIP_TRY(c, hipStreamSynchronize(c->stream));
     if (n && !probe_idx) return fail(c, EFS_ERR_ARG, "steady-state probe indices are required");
     for (uint32_t k = 0; k < n; ++k)
@@ -1594,6 +1682,20 @@ int efs_get_fields(efs_ctx* c, float* volt, float* curr) {
     TRY(need_init(c));
     HIP_TRY(c, hipStreamSynchronize(c->stream));
     const size_t rows = (size_t)c->nx * c->ny;
+    if (c->dbl) {  // narrow on the host
+        std::vector<double> tmp(c->Np);
+        for (int a = 0; a < 2; ++a) {
+            float* out = a ? curr : volt;
+            if (!out) continue;
+            const double* src = (const double*)(a ? c->curr : c->volt);
+            for (int n = 0; n < 3; ++n) {
+                HIP_TRY(c, hipMemcpy(tmp.data(), src + n * c->S, c->Np * sizeof(double), hipMemcpyDeviceToHost));
+                for (size_t r = 0; r < rows; ++r)
+                    for (uint32_t z = 0; z < c->nz; ++z) out[n * c->N + r * c->nz + z] = (float)tmp[r * c->nzp + z];
+            }
+        }
+        return EFS_OK;
+    }
     if (c->half) {  // widen and scale on the host
         float sc[6];
         HIP_TRY(c, hipMemcpy(sc, c->fscale, sizeof sc, hipMemcpyDeviceToHost));
@@ -1623,10 +1725,47 @@ int efs_get_fields(efs_ctx* c, float* volt, float* curr) {
     return EFS_OK;
 }
 
+int efs_get_fields_f64(efs_ctx* c, double* volt, double* curr) {
+    TRY(need_init(c));
+    if (!c->dbl) {  // FP32 / FP16 storage: the FP32 values, widened
+        std::vector<float> v(volt ? 3 * c->N : 0), i(curr ? 3 * c->N : 0);
+        TRY(efs_get_fields(c, volt ? v.data() : nullptr, curr ? i.data() : nullptr));
+        std::copy(v.begin(), v.end(), volt);
+        std::copy(i.begin(), i.end(), curr);
+        return EFS_OK;
+    }
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    const size_t rows = (size_t)c->nx * c->ny;
+    for (int a = 0; a < 2; ++a) {
+        double* out = a ? curr : volt;
+        if (!out) continue;
+        const double* src = (const double*)(a ? c->curr : c->volt);
+        for (int n = 0; n < 3; ++n)
+            HIP_TRY(c, hipMemcpy2D(out + n * c->N, c->nz * sizeof(double), src + n * c->S, c->nzp * sizeof(double),
+                                   c->nz * sizeof(double), rows, hipMemcpyDeviceToHost));
+    }
+    return EFS_OK;
+}
+
 int efs_upload_fields(efs_ctx* c, const float* volt, const float* curr) {
     TRY(need_init(c));
     HIP_TRY(c, hipStreamSynchronize(c->stream));
     const size_t rows = (size_t)c->nx * c->ny;
+    if (c->dbl) {  // widen on the host (padding cells stay zero)
+        std::vector<double> tmp(c->Np, 0.0);
+        for (int a = 0; a < 2; ++a) {
+            const float* in = a ? curr : volt;
+            if (!in) continue;
+            double* dst = (double*)(a ? c->curr : c->volt);
+            for (int n = 0; n < 3; ++n) {
+                for (size_t r = 0; r < rows; ++r)
+                    for (uint32_t z = 0; z < c->nz; ++z) tmp[r * c->nzp + z] = in[n * c->N + r * c->nz + z];
+                HIP_TRY(c, hipMemcpy(dst + n * c->S, tmp.data(), c->Np * sizeof(double), hipMemcpyHostToDevice));
+            }
+        }
+        c->fields_zero = false;
+        return EFS_OK;
+    }
     if (c->half) {  // per component: scale 2^e with the largest value near 2^10, then narrow
         float sc[6];
         HIP_TRY(c, hipMemcpy(sc, c->fscale, sizeof sc, hipMemcpyDeviceToHost));
@@ -1679,8 +1818,12 @@ int efs_cfg_gather(efs_ctx* c, const uint32_t* idx, uint32_t n) {
 int efs_read_gather(efs_ctx* c, float* out) {
     TRY(need_init(c));
     if (!c->g_n) return fail(c, EFS_ERR_STATE, "probe gather not configured");
-    hipLaunchKernelGGL(k_gather, dim3(nblk(c->g_n)), dim3(kPointThreads), 0, c->stream, fv(c, 0), fv(c, 1), c->g_idx,
-                       c->g_out, c->g_n);
+    if (c->dbl)
+        hipLaunchKernelGGL(k_gather<FieldView64>, dim3(nblk(c->g_n)), dim3(kPointThreads), 0, c->stream, fv64(c, 0),
+                           fv64(c, 1), c->g_idx, c->g_out, c->g_n);
+    else
+        hipLaunchKernelGGL(k_gather<FieldView>, dim3(nblk(c->g_n)), dim3(kPointThreads), 0, c->stream, fv(c, 0),
+                           fv(c, 1), c->g_idx, c->g_out, c->g_n);
     CHECK_LAUNCH(c);
     HIP_TRY(c, hipStreamSynchronize(c->stream));
     HIP_TRY(c, hipMemcpy(out, c->g_out, 2 * (size_t)c->g_n * sizeof(float), hipMemcpyDeviceToHost));
@@ -1763,11 +1906,11 @@ int efs_cfg_nf2ff(efs_ctx* c, const uint32_t* points, uint32_t P, const double* 
     TRY(plan_nf_capture(c, sorted, P));
     TRY(dupload(c, c->nf_el, edge_lens, el_total));
     TRY(dupload(c, c->nf_omega, omega, F));
-    TRY(dzero(c, c->nf_buf, (size_t)K * rows));  // padding rows stay zero
-    TRY(dzero(c, c->nf_acc, (size_t)rows * NC));
-    dfree(c->nf_cmp);
-    if (c->nf_comp_req) TRY(dzero(c, c->nf_cmp, (size_t)rows * NC));
-    TRY(dalloc(c, c->nf_tw, (size_t)2 * K * NC));
+    TRY(dzero(c, c->nf_buf, (size_t)K * rows * rw(c)));  // padding rows stay zero
+    TRY(dzero(c, c->nf_acc, (size_t)rows * NC * rw(c)));
+    dfree(c->nf_cmp);  // FP64 accumulators need no compensation
+    if (c->nf_comp_req && !c->dbl) TRY(dzero(c, c->nf_cmp, (size_t)rows * NC));
+    TRY(dalloc(c, c->nf_tw, (size_t)2 * K * NC * rw(c)));
     c->nf_geom = g;
     c->nf_P = P; c->nf_F = F; c->nf_K = K;
     c->nf_rows = rows; c->nf_NC = NC;
@@ -1782,34 +1925,44 @@ int efs_cfg_nf2ff(efs_ctx* c, const uint32_t* points, uint32_t P, const double* 
     return EFS_OK;
 }
 
-int efs_read_nf2ff(efs_ctx* c, float* accE, float* accH) {
+// The accumulators in the caller's layout, as T (float or double): device
+// [row][2f + re/im] (grid-ordered points) -> caller [f][6p + 2k + re/im];
+// the compensated form adds its error array (in double).
+}  // extern "C"
+
+template <typename T>
+static int read_nf2ff_t(efs_ctx* c, T* accE, T* accH) {
     TRY(need_init(c));
     if (!c->nf_P) return fail(c, EFS_ERR_STATE, "NF2FF accumulation not configured");
     TRY(nf2ff_fold(c));
     HIP_TRY(c, hipStreamSynchronize(c->stream));
     const size_t P = c->nf_P, NC = c->nf_NC, RE = c->nf_geom.rowsE;
-    std::vector<float> acc((size_t)c->nf_rows * NC);
-    HIP_TRY(c, hipMemcpy(acc.data(), c->nf_acc, acc.size() * sizeof(float), hipMemcpyDeviceToHost));
-    if (c->nf_cmp) {  // compensated form: value = acc + cmp
-        std::vector<float> cmp(acc.size());
-        HIP_TRY(c, hipMemcpy(cmp.data(), c->nf_cmp, cmp.size() * sizeof(float), hipMemcpyDeviceToHost));
-        for (size_t i = 0; i < acc.size(); ++i) acc[i] += cmp[i];
+    std::vector<double> acc((size_t)c->nf_rows * NC);
+    if (c->dbl) {
+        HIP_TRY(c, hipMemcpy(acc.data(), c->nf_acc, acc.size() * sizeof(double), hipMemcpyDeviceToHost));
+    } else {
+        std::vector<float> a32(acc.size());
+        HIP_TRY(c, hipMemcpy(a32.data(), c->nf_acc, a32.size() * sizeof(float), hipMemcpyDeviceToHost));
+        std::copy(a32.begin(), a32.end(), acc.begin());
+        if (c->nf_cmp) {  // compensated form: value = acc + cmp
+            HIP_TRY(c, hipMemcpy(a32.data(), c->nf_cmp, a32.size() * sizeof(float), hipMemcpyDeviceToHost));
+            for (size_t i = 0; i < acc.size(); ++i) acc[i] += (double)a32[i];
+        }
     }
-    // device [row][2f + re/im] (grid-ordered points) -> caller [f][6p + 2k + re/im]
     for (size_t q = 0; q < P; ++q) {
         const size_t p = c->nf_perm[q];
         for (int k = 0; k < 3; ++k) {
-            const float* e = acc.data() + (k * P + q) * NC;
-            const float* h = acc.data() + (RE + k * P + q) * NC;
+            const double* e = acc.data() + (k * P + q) * NC;
+            const double* h = acc.data() + (RE + k * P + q) * NC;
             for (uint32_t f = 0; f < c->nf_F; ++f) {
                 const size_t o = (size_t)f * 6 * P + 6 * p + 2 * k;
                 if (accE) {
-                    accE[o] = e[2 * f];
-                    accE[o + 1] = e[2 * f + 1];
+                    accE[o] = (T)e[2 * f];
+                    accE[o + 1] = (T)e[2 * f + 1];
                 }
                 if (accH) {
-                    accH[o] = h[2 * f];
-                    accH[o + 1] = h[2 * f + 1];
+                    accH[o] = (T)h[2 * f];
+                    accH[o + 1] = (T)h[2 * f + 1];
                 }
             }
         }
@@ -1817,15 +1970,90 @@ int efs_read_nf2ff(efs_ctx* c, float* accE, float* accH) {
     return EFS_OK;
 }
 
-static int farfield_impl(efs_ctx* c, uint32_t f, const float* meta, const float* theta, uint32_t nT, const float* phi,
-                         uint32_t nP, const double* center, double frequency, double radius, int mdir, double mpos,
-                         int mtype, int image_normals, float* out) {
+extern "C" {
+
+int efs_read_nf2ff(efs_ctx* c, float* accE, float* accH) { return read_nf2ff_t(c, accE, accH); }
+
+int efs_read_nf2ff_f64(efs_ctx* c, double* accE, double* accH) { return read_nf2ff_t(c, accE, accH); }
+
+}  // extern "C"
+
+// Far-field constants for one call (frequency, radius, centre, mirror).
+static FarParams far_params(const efs_ctx* c, uint32_t nT, uint32_t nP, const double* center, double frequency,
+                            double radius, int mdir, double mpos, int mtype, int image_normals) {
+    const double C0 = 299792458.0, Z0 = 376.73031346177066;
+    const double k = 2 * M_PI * frequency / C0;
+    const double fmag = k / (4 * M_PI * radius), fph = -k * radius;
+    FarParams q{};
+    q.P = c->nf_P; q.nT = nT; q.nP = nP;
+    q.k = (float)k; q.Z0 = (float)Z0;
+    q.cx = (float)center[0]; q.cy = (float)center[1]; q.cz = (float)center[2];
+    q.fac_re = (float)(fmag * -std::sin(fph));
+    q.fac_im = (float)(fmag * std::cos(fph));
+    q.mdir = mdir;
+    q.mpos = (float)mpos;
+    q.msign = image_normals ? -1.0f : 1.0f;
+    for (int k3 = 0; k3 < 3; ++k3) q.ef[k3] = q.hf[k3] = 1.0f;
+    if (mdir >= 0) {
+        const int a1 = (mdir + 1) % 3, a2 = (mdir + 2) % 3;
+        if (mtype == 0) {  // PEC: tangential E and normal H reversed
+            q.hf[mdir] = -1.0f; q.ef[a1] = -1.0f; q.ef[a2] = -1.0f;
+        } else {           // PMC: normal E and tangential H reversed
+            q.ef[mdir] = -1.0f; q.hf[a1] = -1.0f; q.hf[a2] = -1.0f;
+        }
+    }
+    return q;
+}
+
+// Point chunks of the far-field launch: whole LDS tiles, about 2048
+// workgroups even for a few angles, at most max_chunks chunks.
+struct FarPlan {
+    uint32_t ablk, chunk, nchunks;
+};
+static FarPlan far_plan(const efs_ctx* c, size_t na, bool mirror, uint32_t max_chunks) {
+    FarPlan p{};
+    p.ablk = nblk(na, 256);
+    const uint32_t Pv = mirror ? 2 * c->nf_P : c->nf_P;  // the image adds a second pass over the points
+    const uint32_t tiles = (Pv + kFarTile - 1) / kFarTile;
+    const uint32_t want = std::max(1u, std::min({tiles, (2048u + p.ablk - 1) / p.ablk, max_chunks}));
+    p.chunk = ((tiles + want - 1) / want) * kFarTile;
+    p.nchunks = (Pv + p.chunk - 1) / p.chunk;
+    return p;
+}
+
+// The far-field kernels on the context's stream: per-chunk FP64 partials
+// into `part` (nchunks * angles * 8 doubles), then their ordered sum and the
+// 5 output floats per angle into `out`.  meta is in internal point order,
+// or in the caller's order when perm is given.
+static void launch_farfield(efs_ctx* c, uint32_t f, const float* meta, const uint32_t* perm, const float* theta,
+                            const float* phi, const FarParams& q, const FarPlan& p, double* part, float* out) {
+    if (c->dbl)
+        hipLaunchKernelGGL(k_farfield<double>, dim3(p.ablk, p.nchunks), dim3(256), 0, c->stream,
+                           (const double*)c->nf_acc + 2 * (size_t)f, nullptr, c->nf_NC, c->nf_geom.rowsE, meta, perm,
+                           theta, phi, q, p.chunk, part);
+    else
+        hipLaunchKernelGGL(k_farfield<float>, dim3(p.ablk, p.nchunks), dim3(256), 0, c->stream,
+                           c->nf_acc + 2 * (size_t)f, c->nf_cmp ? c->nf_cmp + 2 * (size_t)f : nullptr, c->nf_NC,
+                           c->nf_geom.rowsE, meta, perm, theta, phi, q, p.chunk, part);
+    hipLaunchKernelGGL(k_farfield_final, dim3(p.ablk), dim3(256), 0, c->stream, part, p.nchunks, q, out);
+}
+
+static int farfield_checks(efs_ctx* c, uint32_t f, const void* meta, const void* theta, uint32_t nT, const void* phi,
+                           uint32_t nP, const double* center, double radius, const void* out) {
     TRY(need_init(c));
     if (!c->nf_P) return fail(c, EFS_ERR_STATE, "NF2FF accumulation not configured");
     if (f >= c->nf_F) return fail(c, EFS_ERR_ARG, "frequency index out of range");
     if (!meta || !theta || !phi || !center || !out || !nT || !nP)
         return fail(c, EFS_ERR_ARG, "far-field arrays are required");
     if (!(radius > 0)) return fail(c, EFS_ERR_ARG, "radius must be > 0");
+    if ((size_t)nT * nP > 0xffffffffull / 256) return fail(c, EFS_ERR_ARG, "too many far-field angles");
+    return EFS_OK;
+}
+
+static int farfield_impl(efs_ctx* c, uint32_t f, const float* meta, const float* theta, uint32_t nT, const float* phi,
+                         uint32_t nP, const double* center, double frequency, double radius, int mdir, double mpos,
+                         int mtype, int image_normals, float* out) {
+    TRY(farfield_checks(c, f, meta, theta, nT, phi, nP, center, radius, out));
     if (mdir > 2 || (mdir >= 0 && mtype != 0 && mtype != 1))
         return fail(c, EFS_ERR_ARG, "mirror direction must be -1..2 and type 0 (PEC) or 1 (PMC)");
     TRY(nf2ff_fold(c));
@@ -1833,15 +2061,7 @@ static int farfield_impl(efs_ctx* c, uint32_t f, const float* meta, const float*
     double* d_part = nullptr;
     int rc = EFS_OK;
     const size_t na = (size_t)nT * nP;
-    if (na > 0xffffffffull / 256) return fail(c, EFS_ERR_ARG, "too many far-field angles");
-    // split the surface into chunks of whole LDS tiles so that the launch
-    // has ~2048 workgroups even for a few angles
-    const uint32_t ablk = nblk(na, 256);
-    const uint32_t Pv = mdir >= 0 ? 2 * c->nf_P : c->nf_P;  // the image adds a second pass over the points
-    const uint32_t tiles = (Pv + kFarTile - 1) / kFarTile;
-    const uint32_t want = std::max(1u, std::min(tiles, (2048u + ablk - 1) / ablk));
-    const uint32_t chunk = ((tiles + want - 1) / want) * kFarTile;
-    const uint32_t nchunks = (Pv + chunk - 1) / chunk;
+    const FarPlan plan = far_plan(c, na, mdir >= 0, 0xffffffffu);
     std::vector<float> meta_q(8 * (size_t)c->nf_P);  // caller order -> internal point order
     for (size_t q = 0; q < c->nf_P; ++q)
         std::memcpy(&meta_q[8 * q], meta + 8 * (size_t)c->nf_perm[q], 8 * sizeof(float));
@@ -1850,32 +2070,9 @@ static int farfield_impl(efs_ctx* c, uint32_t f, const float* meta, const float*
         if ((rc = dupload(c, d_th, theta, nT))) break;
         if ((rc = dupload(c, d_ph, phi, nP))) break;
         if ((rc = dalloc(c, d_out, 5 * na))) break;
-        if ((rc = dalloc(c, d_part, (size_t)nchunks * na * 8))) break;
-        const double C0 = 299792458.0, Z0 = 376.73031346177066;
-        const double k = 2 * M_PI * frequency / C0;
-        const double fmag = k / (4 * M_PI * radius), fph = -k * radius;
-        FarParams q{};
-        q.P = c->nf_P; q.nT = nT; q.nP = nP;
-        q.k = (float)k; q.Z0 = (float)Z0;
-        q.cx = (float)center[0]; q.cy = (float)center[1]; q.cz = (float)center[2];
-        q.fac_re = (float)(fmag * -std::sin(fph));
-        q.fac_im = (float)(fmag * std::cos(fph));
-        q.mdir = mdir;
-        q.mpos = (float)mpos;
-        q.msign = image_normals ? -1.0f : 1.0f;
-        for (int k3 = 0; k3 < 3; ++k3) q.ef[k3] = q.hf[k3] = 1.0f;
-        if (mdir >= 0) {
-            const int a1 = (mdir + 1) % 3, a2 = (mdir + 2) % 3;
-            if (mtype == 0) {  // PEC: tangential E and normal H reversed
-                q.hf[mdir] = -1.0f; q.ef[a1] = -1.0f; q.ef[a2] = -1.0f;
-            } else {           // PMC: normal E and tangential H reversed
-                q.ef[mdir] = -1.0f; q.hf[a1] = -1.0f; q.hf[a2] = -1.0f;
-            }
-        }
-        hipLaunchKernelGGL(k_farfield, dim3(ablk, nchunks), dim3(256), 0, c->stream, c->nf_acc + 2 * (size_t)f,
-                           c->nf_cmp ? c->nf_cmp + 2 * (size_t)f : nullptr,
-                           c->nf_NC, c->nf_geom.rowsE, d_meta, d_th, d_ph, q, chunk, d_part);
-        hipLaunchKernelGGL(k_farfield_final, dim3(ablk), dim3(256), 0, c->stream, d_part, nchunks, q, d_out);
+        if ((rc = dalloc(c, d_part, (size_t)plan.nchunks * na * 8))) break;
+        const FarParams q = far_params(c, nT, nP, center, frequency, radius, mdir, mpos, mtype, image_normals);
+        launch_farfield(c, f, d_meta, nullptr, d_th, d_ph, q, plan, d_part, d_out);
         hipError_t e = hipGetLastError();
         if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
         if (e == hipSuccess) e = hipMemcpy(out, d_out, 5 * na * sizeof(float), hipMemcpyDeviceToHost);
@@ -1885,6 +2082,8 @@ static int farfield_impl(efs_ctx* c, uint32_t f, const float* meta, const float*
     dfree(d_meta); dfree(d_th); dfree(d_ph); dfree(d_out); dfree(d_part);
     return rc;
 }
+
+extern "C" {
 
 int efs_farfield(efs_ctx* c, uint32_t f, const float* meta, const float* theta, uint32_t nT, const float* phi,
                  uint32_t nP, const double* center, double frequency, double radius, float* out) {
@@ -1914,7 +2113,11 @@ int efs_energy(efs_ctx* c, double* out) {
     }
     const uint32_t rows = (c->nx - 1) * (c->ny - 1);
     const uint32_t blocks = std::max(1u, std::min((uint32_t)kEnergyMaxBlocks, (rows + 3) / 4));
-    if (c->half)
+    if (c->dbl)
+        hipLaunchKernelGGL(k_energy_partial<double>, dim3(blocks), dim3(kEnergyThreads), 0, c->stream,
+                           (const double*)c->volt, (const double*)c->curr, c->nx, c->ny, c->nz, c->nzp,
+                           (uint64_t)c->S, c->fscale, c->en_partial);
+    else if (c->half)
         hipLaunchKernelGGL(k_energy_partial<f16>, dim3(blocks), dim3(kEnergyThreads), 0, c->stream,
                            (const f16*)c->volt, (const f16*)c->curr, c->nx, c->ny, c->nz, c->nzp, (uint64_t)c->S,
                            c->fscale, c->en_partial);
@@ -1948,13 +2151,15 @@ int efs_path_flags(const efs_ctx* c, uint32_t* flags) {
     *flags = (c->vec4 ? EFS_PATH_VEC4 : 0u) | (c->comp ? EFS_PATH_COEF_IDX : 0u) | (upml4 ? EFS_PATH_UPML_4 : 0u) |
              (upml4 && c->pml_fused ? EFS_PATH_FUSED : 0u) |
              (upml4 && c->pml_fused && c->pml_comp ? EFS_PATH_UPML_IDX : 0u) | (c->half ? EFS_PATH_F16 : 0u) |
-             (nf_capture(c).on ? EFS_PATH_NF2FF_CAPTURE : 0u) | (c->comp16 ? EFS_PATH_COEF_IDX16 : 0u);
+             (nf_capture(c).on ? EFS_PATH_NF2FF_CAPTURE : 0u) | (c->comp16 ? EFS_PATH_COEF_IDX16 : 0u) |
+             (c->dbl ? EFS_PATH_F64 : 0u);
     return EFS_OK;
 }
 
 int efs_set_storage(efs_ctx* c, int storage) {
     if (!c) return EFS_ERR_STATE;
-    if (storage != EFS_STORAGE_F32 && storage != EFS_STORAGE_F16) return fail(c, EFS_ERR_ARG, "unknown storage %d", storage);
+    if (storage != EFS_STORAGE_F32 && storage != EFS_STORAGE_F16 && storage != EFS_STORAGE_F64)
+        return fail(c, EFS_ERR_ARG, "unknown storage %d", storage);
     c->storage_req = storage;
     return EFS_OK;
 }

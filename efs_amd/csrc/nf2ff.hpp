@@ -45,16 +45,17 @@ struct Nf2ffGeom {
 // c at (ix+dx, iy+dy, iz+dz); I(n, d) returns curr component n at the point
 // or (d = 1) one cell further along axis n.  One body for both sources of
 // the values keeps their arithmetic identical.
-template <class VA, class IA>
+// R: arithmetic type (float; double for FP64 field storage).
+template <class R, class VA, class IA>
 __device__ __forceinline__ void nf_point(uint32_t ix, uint32_t iy, uint32_t iz, const float* __restrict__ el,
-                                         const Nf2ffGeom& g, VA V, IA I, float out[6]) {
+                                         const Nf2ffGeom& g, VA V, IA I, R out[6]) {
     // no contraction: the same bits whichever accessor supplies the values
 #pragma clang fp contract(off)
-    float E0 = 0.f, E1 = 0.f, E2 = 0.f;
+    R E0 = 0, E1 = 0, E2 = 0;
     if (!(ix >= g.nx - 1 || iy >= g.ny - 1 || iz >= g.nz - 1)) {
-        E0 = (V(0, 0, 0, 0) + V(0, 0, 1, 0) + V(0, 0, 1, 1) + V(0, 0, 0, 1)) / (4.0f * el[g.el_off[0] + ix]);
-        E1 = (V(1, 0, 0, 0) + V(1, 0, 0, 1) + V(1, 1, 0, 1) + V(1, 1, 0, 0)) / (4.0f * el[g.el_off[1] + iy]);
-        E2 = (V(2, 0, 0, 0) + V(2, 1, 0, 0) + V(2, 1, 1, 0) + V(2, 0, 1, 0)) / (4.0f * el[g.el_off[2] + iz]);
+        E0 = (V(0, 0, 0, 0) + V(0, 0, 1, 0) + V(0, 0, 1, 1) + V(0, 0, 0, 1)) / ((R)4 * (R)el[g.el_off[0] + ix]);
+        E1 = (V(1, 0, 0, 0) + V(1, 0, 0, 1) + V(1, 1, 0, 1) + V(1, 1, 0, 0)) / ((R)4 * (R)el[g.el_off[1] + iy]);
+        E2 = (V(2, 0, 0, 0) + V(2, 1, 0, 0) + V(2, 1, 1, 0) + V(2, 0, 1, 0)) / ((R)4 * (R)el[g.el_off[2] + iz]);
     }
     const uint32_t pos[3] = {ix, iy, iz};
     const uint32_t lim[3] = {g.nx, g.ny, g.nz};
@@ -63,15 +64,15 @@ __device__ __forceinline__ void nf_point(uint32_t ix, uint32_t iy, uint32_t iz, 
     out[2] = E2;
 #pragma unroll
     for (int n = 0; n < 3; ++n) {
-        float h = 0.f;
-        const float d0 = el[g.el_off[3 + n] + pos[n]];
-        if (d0 > 0.f) {
+        R h = 0;
+        const R d0 = el[g.el_off[3 + n] + pos[n]];
+        if (d0 > (R)0) {
             h = I(n, 0) / d0;
             if (pos[n] < lim[n] - 1) {
-                const float d1 = el[g.el_off[3 + n] + pos[n] + 1];
-                const float dr = d0 / (d0 + d1);
-                const float up = I(n, 1) / d1;
-                h = h * (1.0f - dr) + up * dr;
+                const R d1 = el[g.el_off[3 + n] + pos[n] + 1];
+                const R dr = d0 / (d0 + d1);
+                const R up = I(n, 1) / d1;
+                h = h * ((R)1 - dr) + up * dr;
             }
         }
         out[3 + n] = h;
@@ -80,34 +81,36 @@ __device__ __forceinline__ void nf_point(uint32_t ix, uint32_t iy, uint32_t iz, 
 
 // One thread per surface point.  Points on a captured z plane (see
 // NfCapture) read the slabs the H sweep wrote; the others read the fields.
-__global__ void k_nf2ff_sample(FieldView volt, FieldView curr,
+template <class FV>
+__global__ void k_nf2ff_sample(FV volt, FV curr,
                                const uint32_t* __restrict__ pts, const float* __restrict__ el, Nf2ffGeom g,
-                               float* __restrict__ row_out /* buf + slot*rows */, NfCapture cap) {
+                               typename FV::R* __restrict__ row_out /* buf + slot*rows */, NfCapture cap) {
+    typedef typename FV::R R;
     // points are in grid order; consecutive workgroups share cells, so they
     // stay on one XCD (own L2)
     const uint32_t p = xcd_remap(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
     if (p >= g.P) return;
     const uint32_t ix = pts[3 * p], iy = pts[3 * p + 1], iz = pts[3 * p + 2];
-    float o[6];
+    R o[6];
     int f = -1;
     if (cap.on && ix - cap.x0 < cap.x1 - cap.x0 && iy - cap.y0 < cap.y1 - cap.y0)
         f = iz == cap.z[0] ? 0 : (cap.nf > 1 && iz == cap.z[1] ? 1 : -1);
     if (f >= 0) {
         const float* q = cap.slab + ((size_t)(f * cap.nx_c + (ix - cap.x0)) * cap.ny_c + (iy - cap.y0)) * kCapFloats;
         const size_t rx = (size_t)cap.ny_c * kCapFloats;  // next x row
-        auto V = [&](int c, int dx, int dy, int dz) { return q[dx * rx + dy * kCapFloats + 2 * c + dz]; };
+        auto V = [&](int c, int dx, int dy, int dz) { return (R)q[dx * rx + dy * kCapFloats + 2 * c + dz]; };
         auto I = [&](int n, int d) {
-            return n == 0 ? q[d * rx + 6] : (n == 1 ? q[d * kCapFloats + 7] : q[8 + d]);
+            return (R)(n == 0 ? q[d * rx + 6] : (n == 1 ? q[d * kCapFloats + 7] : q[8 + d]));
         };
-        nf_point(ix, iy, iz, el, g, V, I, o);
+        nf_point<R>(ix, iy, iz, el, g, V, I, o);
     } else {
         const size_t Ny = g.ny, Nz = g.nzs;
         const size_t b = ix * Ny * Nz + iy * Nz + iz;
         const size_t sx = Ny * Nz, sy = Nz;
         const size_t step[3] = {sx, sy, 1};
-        auto V = [&](int c, int dx, int dy, int dz) { return volt.ldc(c, b + dx * sx + dy * sy + dz); };
-        auto I = [&](int n, int d) { return curr.ldc(n, b + (d ? step[n] : 0)); };
-        nf_point(ix, iy, iz, el, g, V, I, o);
+        auto V = [&](int c, int dx, int dy, int dz) { return (R)volt.ldc(c, b + dx * sx + dy * sy + dz); };
+        auto I = [&](int n, int d) { return (R)curr.ldc(n, b + (d ? step[n] : 0)); };
+        nf_point<R>(ix, iy, iz, el, g, V, I, o);
     }
     const size_t P = g.P, RE = g.rowsE;
     row_out[p] = o[0];
@@ -121,14 +124,16 @@ __global__ void k_nf2ff_sample(FieldView volt, FieldView curr,
 // FP64 twiddles for k in [0,kused): tw[(h*K + k)*NC + 2f + {0: re, 1: im}],
 // h = 0 (E) or 1 (H, half-step later), zero for f >= F.  The Hann weight
 // uses the integer step index for both, as the reference does.
-__global__ void k_nf2ff_twiddle(float* __restrict__ tw, uint32_t K, uint32_t kused, uint32_t F, uint32_t NC,
+// T: float (the matrix-core fold) or double (the FP64 fold).
+template <typename T>
+__global__ void k_nf2ff_twiddle(T* __restrict__ tw, uint32_t K, uint32_t kused, uint32_t F, uint32_t NC,
                                 uint32_t ts0, const double* __restrict__ omega, double dT, uint32_t maxTS,
                                 uint32_t window, double wnorm) {
     const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= 2 * kused * NC) return;
     const uint32_t c = t % NC, k = (t / NC) % kused, h = t / (NC * kused);
     const uint32_t f = c / 2;
-    float v = 0.f;
+    T v = 0;
     if (f < F) {
         const double n = (double)ts0 + (double)k;
         double w = 1.0;
@@ -137,7 +142,7 @@ __global__ void k_nf2ff_twiddle(float* __restrict__ tw, uint32_t K, uint32_t kus
         const double ph = omega[f] * (n + (h ? 0.5 : 0.0)) * dT;
         double sn, cs;
         sincos(ph, &sn, &cs);
-        v = (float)((c & 1u) ? -sn * s : cs * s);
+        v = (T)((c & 1u) ? -sn * s : cs * s);
     }
     tw[((size_t)h * K + k) * NC + c] = v;
 }
@@ -238,6 +243,38 @@ __global__ __launch_bounds__(256) void k_nf2ff_fold(const float* __restrict__ bu
     }
 }
 
+// FP64 fold of the FP64 storage form (its precision reference run): the
+// same product in double on the vector ALUs, one thread per row and 32
+// accumulator columns (16 frequencies) of column tile blockIdx.y, twiddles
+// of the row block's half (E or H; rowsE is a multiple of 512, so a block
+// of 256 rows is one or the other) in LDS.
+constexpr int kFold64Rows = 256;
+__global__ __launch_bounds__(kFold64Rows) void k_nf2ff_fold64(const double* __restrict__ buf, uint32_t rows,
+                                                              uint32_t rowsE, uint32_t K, uint32_t kused,
+                                                              const double* __restrict__ tw, uint32_t NC,
+                                                              double* __restrict__ acc) {
+    extern __shared__ double s_w64[];  // [kused][kFoldCols]
+    const uint32_t base = blockIdx.x * kFold64Rows;
+    const uint32_t h = base >= rowsE ? 1u : 0u;
+    const uint32_t c0 = blockIdx.y * kFoldCols;
+    for (uint32_t t = threadIdx.x; t < kused * kFoldCols; t += blockDim.x)
+        s_w64[t] = tw[((size_t)h * K + t / kFoldCols) * NC + c0 + t % kFoldCols];
+    __syncthreads();
+    const uint32_t row = base + threadIdx.x;
+    if (row >= rows) return;
+    double d[kFoldCols];
+#pragma unroll
+    for (int c = 0; c < kFoldCols; ++c) d[c] = 0.0;
+    for (uint32_t k = 0; k < kused; ++k) {
+        const double b = buf[(size_t)k * rows + row];
+#pragma unroll
+        for (int c = 0; c < kFoldCols; ++c) d[c] = __builtin_fma(b, s_w64[k * kFoldCols + c], d[c]);
+    }
+    double* a = acc + (size_t)row * NC + c0;
+#pragma unroll
+    for (int c = 0; c < kFoldCols; ++c) a[c] += d[c];
+}
+
 // Far field for one frequency bin (NF2FF far-field WGSL,
 // src/webgpu-engine.mjs:802-986): thread per (theta, phi); surface points
 // are staged through LDS; sums are carried in FP64.
@@ -262,9 +299,11 @@ constexpr int kFarTile = 256;
 // adds the chunks in order, so the result does not depend on scheduling.
 // Splitting the points keeps the launch wide when there are few angles
 // (an RCS cut of 64 angles is one workgroup of angles).
-__global__ __launch_bounds__(256) void k_farfield(const float* __restrict__ acc_f, const float* __restrict__ cmp_f,
+template <typename A>
+__global__ __launch_bounds__(256) void k_farfield(const A* __restrict__ acc_f, const A* __restrict__ cmp_f,
                                                   uint32_t NC, uint32_t rowsE,
-                                                  const float* __restrict__ meta, const float* __restrict__ theta,
+                                                  const float* __restrict__ meta, const uint32_t* __restrict__ perm,
+                                                  const float* __restrict__ theta,
                                                   const float* __restrict__ phi, FarParams q, uint32_t chunk,
                                                   double* __restrict__ partial) {
     __shared__ float s_pt[kFarTile][18];  // pos xyz, nd, sign, area, E re/im x3, H re/im x3
@@ -285,11 +324,13 @@ __global__ __launch_bounds__(256) void k_farfield(const float* __restrict__ acc_
         if (threadIdx.x < kFarTile && pv < p1) {
             const bool img = pv >= q.P;
             const uint32_t p = img ? pv - q.P : pv;
+            // meta in internal point order, or (perm) in the caller's order
+            const float* m = meta + 8 * (size_t)(perm ? perm[p] : p);
             float* s = s_pt[threadIdx.x];
-            float pos[3] = {meta[8 * (size_t)p], meta[8 * (size_t)p + 1], meta[8 * (size_t)p + 2]};
-            s[3] = meta[8 * (size_t)p + 3];
-            s[4] = meta[8 * (size_t)p + 4];
-            s[5] = meta[8 * (size_t)p + 5];
+            float pos[3] = {m[0], m[1], m[2]};
+            s[3] = m[3];
+            s[4] = m[4];
+            s[5] = m[5];
             if (img) {
                 pos[q.mdir] = 2.0f * q.mpos - pos[q.mdir];
                 if ((int)s[3] == q.mdir) s[4] *= q.msign;
@@ -300,16 +341,16 @@ __global__ __launch_bounds__(256) void k_farfield(const float* __restrict__ acc_
 #pragma unroll
             for (int c = 0; c < 3; ++c) {
                 const size_t oe = ((size_t)c * q.P + p) * NC, oh = ((size_t)rowsE + (size_t)c * q.P + p) * NC;
-                float e0 = acc_f[oe], e1 = acc_f[oe + 1], h0 = acc_f[oh], h1 = acc_f[oh + 1];
+                A e0 = acc_f[oe], e1 = acc_f[oe + 1], h0 = acc_f[oh], h1 = acc_f[oh + 1];
                 if (cmp_f) {  // compensated accumulators: value = acc + cmp
                     e0 += cmp_f[oe]; e1 += cmp_f[oe + 1];
                     h0 += cmp_f[oh]; h1 += cmp_f[oh + 1];
                 }
                 const float fe = img ? q.ef[c] : 1.0f, fh = img ? q.hf[c] : 1.0f;
-                s[6 + 2 * c] = e0 * fe;
-                s[7 + 2 * c] = e1 * fe;
-                s[12 + 2 * c] = h0 * fh;
-                s[13 + 2 * c] = h1 * fh;
+                s[6 + 2 * c] = (float)e0 * fe;
+                s[7 + 2 * c] = (float)e1 * fe;
+                s[12 + 2 * c] = (float)h0 * fh;
+                s[13 + 2 * c] = (float)h1 * fh;
             }
         }
         __syncthreads();

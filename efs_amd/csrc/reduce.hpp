@@ -24,7 +24,19 @@ __device__ __forceinline__ double wave_sum(double v) {
 }
 
 // T: field storage; with FP16 each component's squares are scaled by its
-// (power-of-two) scale squared, so the sum is of the same values.
+// (power-of-two) scale squared, so the sum is of the same values.  Squares
+// and row sums are FP32, or FP64 for FP64 storage.
+template <typename T>
+struct EnergyArith {
+    typedef float type;
+    static constexpr bool kHalf = Fs<T>::kHalf;
+};
+template <>
+struct EnergyArith<double> {
+    typedef double type;
+    static constexpr bool kHalf = false;
+};
+
 template <typename T>
 __global__ __launch_bounds__(kEnergyThreads) void k_energy_partial(const T* __restrict__ volt,
                                                                    const T* __restrict__ curr,
@@ -41,12 +53,13 @@ __global__ __launch_bounds__(kEnergyThreads) void k_energy_partial(const T* __re
     for (uint32_t row = blockIdx.x * (kEnergyThreads / 64) + wv; row < rows; row += nwaves) {
         const uint32_t x = row / (ny - 1), y = row - x * (ny - 1);
         const size_t base = ((size_t)x * ny + y) * nzs;  // nzs: device row length
-        float s = 0.f;
+        typedef typename EnergyArith<T>::type E;
+        E s = 0;
         for (uint32_t z = lane; z + 1 < nz; z += 64) {
 #pragma unroll
             for (int n = 0; n < 3; ++n) {
-                float v = (float)volt[n * N + base + z], c = (float)curr[n * N + base + z];
-                if (Fs<T>::kHalf) {
+                E v = (E)volt[n * N + base + z], c = (E)curr[n * N + base + z];
+                if (EnergyArith<T>::kHalf) {
                     v *= fscale[n];
                     c *= fscale[3 + n];
                 }

@@ -18,23 +18,32 @@ namespace efs {
 constexpr uint32_t kAliasFlux = 0x80000000u;
 constexpr int kPointThreads = 256;
 
-__device__ __forceinline__ float read_alias(const FieldView& field, const float* __restrict__ flux, uint32_t a) {
+// The kernels that the FP64 storage form also runs (excitation, Mur,
+// gather, NF2FF sampling) are templates over the field view FV: FieldView
+// (FP32 / FP16 storage, FP32 arithmetic) or FieldView64 (FP64 storage and
+// arithmetic); R = FV::R is the arithmetic type, and state arrays the
+// kernels keep (Mur's saved values, UPML flux) are R.
+template <class FV>
+__device__ __forceinline__ typename FV::R read_alias(const FV& field, const typename FV::R* __restrict__ flux,
+                                                     uint32_t a) {
     return (a & kAliasFlux) ? flux[a & ~kAliasFlux] : field.ld(a);
 }
 
 // apply_excitation (src/webgpu-fdtd.mjs:161-183 semantics: clamp at 0,
 // optional period, past-the-end reads signal[0]).
-__global__ void k_excitation(FieldView volt, const float* __restrict__ signal, uint32_t len,
+template <class FV>
+__global__ void k_excitation(FV volt, const float* __restrict__ signal, uint32_t len,
                              uint32_t period, const uint32_t* __restrict__ delay,
                              const float* __restrict__ amp, const uint32_t* __restrict__ gidx, uint32_t n,
                              uint32_t numTS) {
+    typedef typename FV::R R;
     const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= n) return;
     int64_t e = (int64_t)numTS - (int64_t)delay[k];
     if (e < 0) e = 0;
     if (period > 0) e %= (int64_t)period;
     if (e >= (int64_t)len) e = 0;
-    volt.add(gidx[k], amp[k] * signal[e]);
+    volt.add(gidx[k], (R)amp[k] * (R)signal[e]);
 }
 
 // Mur ABC (src/webgpu-fdtd.mjs:992-1023): per entry
@@ -47,28 +56,34 @@ __global__ void k_excitation(FieldView volt, const float* __restrict__ signal, u
 // (another entry's normal cell, or an excitation / ADE cell) get their post
 // earlier, at its place in the step, from `k_mur_post_some`, and `finish`
 // then only applies them (flag bit).  Explicit FMAs fix the rounding.
-__global__ void k_mur_pre(FieldView volt, const float* __restrict__ vflux,
+template <class FV>
+__global__ void k_mur_pre(FV volt, const typename FV::R* __restrict__ vflux,
                           const float* __restrict__ coeff, const uint32_t* __restrict__ normal_src,
-                          const uint32_t* __restrict__ shifted_src, float* __restrict__ saved, uint32_t n) {
+                          const uint32_t* __restrict__ shifted_src, typename FV::R* __restrict__ saved, uint32_t n) {
+    typedef typename FV::R R;
     const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= n) return;
-    saved[k] = __builtin_fmaf(-coeff[k], read_alias(volt, vflux, normal_src[k]), read_alias(volt, vflux, shifted_src[k]));
+    saved[k] = fma1(-(R)coeff[k], read_alias(volt, vflux, normal_src[k]), read_alias(volt, vflux, shifted_src[k]));
 }
 
-__global__ void k_mur_post_some(FieldView volt, const float* __restrict__ coeff, const uint32_t* __restrict__ shifted,
-                                const uint32_t* __restrict__ list, float* __restrict__ saved, uint32_t n) {
+template <class FV>
+__global__ void k_mur_post_some(FV volt, const float* __restrict__ coeff, const uint32_t* __restrict__ shifted,
+                                const uint32_t* __restrict__ list, typename FV::R* __restrict__ saved, uint32_t n) {
+    typedef typename FV::R R;
     const uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
     if (j >= n) return;
     const uint32_t k = list[j];
-    saved[k] = __builtin_fmaf(coeff[k], volt.ld(shifted[k]), saved[k]);
+    saved[k] = fma1((R)coeff[k], volt.ld(shifted[k]), saved[k]);
 }
 
-__global__ void k_mur_finish(FieldView volt, const float* __restrict__ coeff, const uint32_t* __restrict__ normal,
+template <class FV>
+__global__ void k_mur_finish(FV volt, const float* __restrict__ coeff, const uint32_t* __restrict__ normal,
                              const uint32_t* __restrict__ shifted, const uint8_t* __restrict__ early,
-                             const float* __restrict__ saved, uint32_t n) {
+                             const typename FV::R* __restrict__ saved, uint32_t n) {
+    typedef typename FV::R R;
     const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= n) return;
-    const float s = early[k] ? saved[k] : __builtin_fmaf(coeff[k], volt.ld(shifted[k]), saved[k]);
+    const R s = early[k] ? saved[k] : fma1((R)coeff[k], volt.ld(shifted[k]), saved[k]);
     volt.st(normal[k], s);
 }
 
@@ -204,13 +219,13 @@ __global__ void k_ss_accumulate(FieldView volt, const uint32_t* __restrict__ gid
 }
 
 // Probe gather into out[2*k], out[2*k+1] (src/webgpu-engine.mjs:2593-2665).
-__global__ void k_gather(FieldView volt, FieldView curr, const uint32_t* __restrict__ idx, float* __restrict__ out,
-                         uint32_t n) {
+template <class FV>
+__global__ void k_gather(FV volt, FV curr, const uint32_t* __restrict__ idx, float* __restrict__ out, uint32_t n) {
     const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= n) return;
     const uint32_t j = idx[k];
-    out[2 * k] = volt.ld(j);
-    out[2 * k + 1] = curr.ld(j);
+    out[2 * k] = (float)volt.ld(j);
+    out[2 * k + 1] = (float)curr.ld(j);
 }
 
 }  // namespace efs

@@ -255,6 +255,7 @@ struct Fs<f16> {
 // see it: FP32, or FP16 with per-component scales (value = stored *
 // scale[n], n = index / S; powers of two, so the scaling is exact).
 struct FieldView {
+    typedef float R;  // arithmetic type of the point kernels
     void* p;
     const float* scale;  // 3 scales of this array (FP16 storage)
     uint64_t S;
@@ -306,6 +307,18 @@ struct FieldView {
     }
 };
 
+// FP64 field storage (the precision reference run of the same engine):
+// values and the point kernels' arithmetic in double.
+struct FieldView64 {
+    typedef double R;
+    double* p;
+    uint64_t S;
+    __device__ __forceinline__ double ldc(uint32_t c, size_t j) const { return p[c * S + j]; }
+    __device__ __forceinline__ double ld(size_t i) const { return p[i]; }
+    __device__ __forceinline__ void st(size_t i, double v) const { p[i] = v; }
+    __device__ __forceinline__ void add(size_t i, double v) const { atomicAdd(p + i, v); }
+};
+
 // Per-component scales of the two field arrays (FP16 storage): fscale[n]
 // for volt component n, fscale[3+n] for curr.  own = the array the sweep
 // writes, other = the array it reads.
@@ -334,6 +347,7 @@ __device__ __forceinline__ Scales load_scales(const float* __restrict__ fscale, 
 //   core  V' = fma(a, V, b*curl)
 //   UPML  F' = fma(a, F, b*curl),  V' = fma(vvfn, F', fma(vv_p, V, -(vvfo*F)))
 __device__ __forceinline__ float fma1(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
+__device__ __forceinline__ double fma1(double a, double b, double c) { return __builtin_fma(a, b, c); }
 __device__ __forceinline__ v4 fma4(v4 a, v4 b, v4 c) { return __builtin_elementwise_fma(a, b, c); }
 
 __device__ __forceinline__ v4 sel4(uint32_t bits, v4 t, v4 f) {
@@ -441,33 +455,35 @@ __device__ __forceinline__ void coef_decode(const float2* s_tab, const uint32_t 
 // One cell of component n with curl formed; k = UPML region or -1; core =
 // whether the core update applies (always for E); HF selects the H-side
 // UPML coefficients (ii, iifo, iifn follow vv, vvfo, vvfn in each region).
-template <bool HF>
-__device__ __forceinline__ void cell1(float* __restrict__ fld, const float* __restrict__ a,
-                                      const float* __restrict__ b, size_t j, bool core, float curl, int k,
+// V is the field type: float, or double for FP64 storage (coefficients stay
+// the FP32 values they were given, widened; the arithmetic is in V).
+template <bool HF, typename V>
+__device__ __forceinline__ void cell1(V* __restrict__ fld, const float* __restrict__ a,
+                                      const float* __restrict__ b, size_t j, bool core, V curl, int k,
                                       const PmlTable& pml, const float* __restrict__ pcoef,
-                                      float* __restrict__ pflux, uint32_t n, uint32_t x, uint32_t y, uint32_t z) {
+                                      V* __restrict__ pflux, uint32_t n, uint32_t x, uint32_t y, uint32_t z) {
     if (k < 0) {
-        if (core) fld[j] = fma1(ldnt(a + j), fld[j], ldnt(b + j) * curl);
+        if (core) fld[j] = fma1((V)ldnt(a + j), fld[j], (V)ldnt(b + j) * curl);
         return;
     }
     const PmlRegion& R = pml.r[k];
     const size_t p = (size_t)n * R.cs + ((size_t)(x - R.x0) * R.ny + (y - R.y0)) * R.nz + (z - R.z0);
     const size_t T = 3 * (size_t)R.cs;
     const float* cf = pcoef + R.coef + (HF ? 3 * T : 0);
-    float* fl = pflux + R.flux;
-    const float F = fl[p];
-    const float Fn = core ? fma1(ldnt(a + j), F, ldnt(b + j) * curl) : F;
-    fld[j] = fma1(ldnt(cf + 2 * T + p), Fn, fma1(ldnt(cf + p), fld[j], -(ldnt(cf + T + p) * F)));
+    V* fl = pflux + R.flux;
+    const V F = fl[p];
+    const V Fn = core ? fma1((V)ldnt(a + j), F, (V)ldnt(b + j) * curl) : F;
+    fld[j] = fma1((V)ldnt(cf + 2 * T + p), Fn, fma1((V)ldnt(cf + p), fld[j], -((V)ldnt(cf + T + p) * F)));
     fl[p] = Fn;
 }
 
-template <bool HAS_PML>
-__global__ __launch_bounds__(kSweepThreads) void sweep_e1(float* __restrict__ volt,
-                                                          const float* __restrict__ curr,
+template <bool HAS_PML, typename V = float>
+__global__ __launch_bounds__(kSweepThreads) void sweep_e1(V* __restrict__ volt,
+                                                          const V* __restrict__ curr,
                                                           const float* __restrict__ vv,
                                                           const float* __restrict__ vi, SweepGeom g,
                                                           PmlTable pml, const float* __restrict__ pcoef,
-                                                          float* __restrict__ pflux) {
+                                                          V* __restrict__ pflux) {
     const SweepBlock d = g.tab[xcd_remap(blockIdx.x, g.blocks)];
     const uint32_t c = d.tile * kSweepThreads + threadIdx.x;
     if (c >= g.cols) return;
@@ -482,28 +498,28 @@ __global__ __launch_bounds__(kSweepThreads) void sweep_e1(float* __restrict__ vo
 
     size_t i = (size_t)x0 * plane + c;
     const size_t ib = x0 > 0 ? i - plane : i;
-    float hy_m = curr[S + ib], hz_m = curr[2 * S + ib];  // Hy, Hz at x-1 (self at x = 0)
+    V hy_m = curr[S + ib], hz_m = curr[2 * S + ib];  // Hy, Hz at x-1 (self at x = 0)
     for (uint32_t x = x0; x < x1; ++x, i += plane) {
-        const float hx = curr[i], hy = curr[S + i], hz = curr[2 * S + i];
-        const float hz_ym = curr[2 * S + i - sy], hx_ym = curr[i - sy];
-        const float hy_zm = curr[S + i - sz], hx_zm = curr[i - sz];
-        const float curl[3] = {hz - hz_ym - hy + hy_zm, hx - hx_zm - hz + hz_m, hy - hy_m - hx + hx_ym};
+        const V hx = curr[i], hy = curr[S + i], hz = curr[2 * S + i];
+        const V hz_ym = curr[2 * S + i - sy], hx_ym = curr[i - sy];
+        const V hy_zm = curr[S + i - sz], hx_zm = curr[i - sz];
+        const V curl[3] = {hz - hz_ym - hy + hy_zm, hx - hx_zm - hz + hz_m, hy - hy_m - hx + hx_ym};
         hy_m = hy;
         hz_m = hz;
         const int k = (HAS_PML && mask) ? pml_find(pml, mask, x) : -1;
 #pragma unroll
         for (int n = 0; n < 3; ++n)
-            cell1<false>(volt, vv, vi, n * S + i, true, curl[n], k, pml, pcoef, pflux, n, x, y, z);
+            cell1<false, V>(volt, vv, vi, n * S + i, true, curl[n], k, pml, pcoef, pflux, n, x, y, z);
     }
 }
 
-template <bool HAS_PML>
-__global__ __launch_bounds__(kSweepThreads) void sweep_h1(float* __restrict__ curr,
-                                                          const float* __restrict__ volt,
+template <bool HAS_PML, typename V = float>
+__global__ __launch_bounds__(kSweepThreads) void sweep_h1(V* __restrict__ curr,
+                                                          const V* __restrict__ volt,
                                                           const float* __restrict__ ii,
                                                           const float* __restrict__ iv, SweepGeom g,
                                                           PmlTable pml, const float* __restrict__ pcoef,
-                                                          float* __restrict__ pflux) {
+                                                          V* __restrict__ pflux) {
     const SweepBlock d = g.tab[xcd_remap(blockIdx.x, g.blocks)];
     const uint32_t c = d.tile * kSweepThreads + threadIdx.x;
     if (c >= g.cols) return;
@@ -517,7 +533,7 @@ __global__ __launch_bounds__(kSweepThreads) void sweep_h1(float* __restrict__ cu
     if (!col_core && !mask) return;
 
     size_t i = (size_t)x0 * plane + c;
-    float ex = 0.f, ey = 0.f, ez = 0.f;  // E at x, carried from the previous plane
+    V ex = 0, ey = 0, ez = 0;  // E at x, carried from the previous plane
     if (col_core) {
         ex = volt[i];
         ey = volt[S + i];
@@ -527,11 +543,11 @@ __global__ __launch_bounds__(kSweepThreads) void sweep_h1(float* __restrict__ cu
         const bool core = col_core && (x + 1 < g.nx);
         const int k = (HAS_PML && mask) ? pml_find(pml, mask, x) : -1;
         if (!core && k < 0) continue;
-        float curl[3] = {0.f, 0.f, 0.f};
+        V curl[3] = {0, 0, 0};
         if (core) {
-            const float ex_p = volt[i + plane], ey_p = volt[S + i + plane], ez_p = volt[2 * S + i + plane];
-            const float ez_yp = volt[2 * S + i + g.nz], ey_zp = volt[S + i + 1];
-            const float ex_zp = volt[i + 1], ex_yp = volt[i + g.nz];
+            const V ex_p = volt[i + plane], ey_p = volt[S + i + plane], ez_p = volt[2 * S + i + plane];
+            const V ez_yp = volt[2 * S + i + g.nz], ey_zp = volt[S + i + 1];
+            const V ex_zp = volt[i + 1], ex_yp = volt[i + g.nz];
             curl[0] = ez - ez_yp - ey + ey_zp;
             curl[1] = ex - ex_zp - ez + ez_p;
             curl[2] = ey - ey_p - ex + ex_yp;
@@ -541,7 +557,7 @@ __global__ __launch_bounds__(kSweepThreads) void sweep_h1(float* __restrict__ cu
         }
 #pragma unroll
         for (int n = 0; n < 3; ++n)
-            cell1<true>(curr, ii, iv, n * S + i, core, curl[n], k, pml, pcoef, pflux, n, x, y, z);
+            cell1<true, V>(curr, ii, iv, n * S + i, core, curl[n], k, pml, pcoef, pflux, n, x, y, z);
     }
 }
 

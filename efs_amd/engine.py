@@ -20,7 +20,7 @@ from . import build as _build
 
 LIB_PATH = _build.ENGINE_LIB
 
-EFS_OK, EFS_ERR_NO_DEVICE = 0, 1
+EFS_OK, EFS_ERR_NO_DEVICE, EFS_ERR_STATE, EFS_ERR_ARG, EFS_ERR_HIP, EFS_ERR_NOMEM = 0, 1, 2, 3, 4, 5
 PROF_SWEEP_E, PROF_SWEEP_H, PROF_STEP = 0, 1, 2
 
 _f32p = np.ctypeslib.ndpointer(dtype=np.float32, flags="C_CONTIGUOUS")
@@ -66,6 +66,8 @@ SIGNATURES = {
     "efs_cfg_nf2ff": (C.c_int, [C.c_void_p, _u32p, C.c_uint32, _f64p, C.c_uint32, C.c_double, _f32p, _u32p,
                                 C.c_uint32, C.c_uint32, C.c_uint32]),
     "efs_read_nf2ff": (C.c_int, [C.c_void_p, _f32p, _f32p]),
+    "efs_read_nf2ff_f64": (C.c_int, [C.c_void_p, _f64p, _f64p]),
+    "efs_get_fields_f64": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p]),
     "efs_farfield": (C.c_int, [C.c_void_p, C.c_uint32, _f32p, _f32p, C.c_uint32, _f32p, C.c_uint32, _f64p,
                                C.c_double, C.c_double, _f32p]),
     "efs_farfield_mirror": (C.c_int, [C.c_void_p, C.c_uint32, _f32p, _f32p, C.c_uint32, _f32p, C.c_uint32, _f64p,
@@ -195,11 +197,14 @@ class HIPEngine:
     def init(self, gridSize, coefficients, storage="fp32"):
         """init(gridSize, {vv, vi, ii, iv}) (src/webgpu-engine.mjs:1143-1185).
         storage="fp16" keeps the fields as FP16 with per-component
-        power-of-two scales (updates stay FP32); reads return FP32."""
+        power-of-two scales (updates stay FP32); storage="fp64" is the FP64
+        run of the same engine (fields, UPML flux, Mur state and NF2FF in
+        double; no ADE, TFSF, RLC, PBC or steady state); reads return FP32."""
         self._need_device()
-        if storage not in ("fp32", "fp16"):
-            raise ValueError("storage must be 'fp32' or 'fp16'")
-        self._check(self.lib.efs_set_storage(self.ctx, 1 if storage == "fp16" else 0))
+        codes = {"fp32": 0, "fp16": 1, "fp64": 2}
+        if storage not in codes:
+            raise ValueError("storage must be 'fp32', 'fp16' or 'fp64'")
+        self._check(self.lib.efs_set_storage(self.ctx, codes[storage]))
         nx, ny, nz = (int(v) for v in gridSize)
         c = coefficients
         self._check(self.lib.efs_init(self.ctx, nx, ny, nz, f32(c["vv"]), f32(c["vi"]), f32(c["ii"]),
@@ -376,12 +381,15 @@ class HIPEngine:
         self._check(self.lib.efs_set_num_ts(self.ctx, int(v)))
 
     # -- fields ------------------------------------------------------------
-    def getFields(self):
-        """getFields() (src/webgpu-engine.mjs:2533-2557)."""
+    def getFields(self, fp64=False):
+        """getFields() (src/webgpu-engine.mjs:2533-2557); fp64=True returns
+        float64 arrays (the FP64 storage form's values; efs_get_fields_f64)."""
         self._need_device()
-        v = np.empty(3 * self.totalCells, dtype=np.float32)
-        c = np.empty(3 * self.totalCells, dtype=np.float32)
-        self._check(self.lib.efs_get_fields(self.ctx, v.ctypes.data, c.ctypes.data))
+        dt = np.float64 if fp64 else np.float32
+        v = np.empty(3 * self.totalCells, dtype=dt)
+        c = np.empty(3 * self.totalCells, dtype=dt)
+        fn = self.lib.efs_get_fields_f64 if fp64 else self.lib.efs_get_fields
+        self._check(fn(self.ctx, v.ctypes.data, c.ctypes.data))
         return {"volt": v, "curr": c}
 
     def uploadFields(self, volt, curr):
@@ -460,14 +468,16 @@ class HIPEngine:
                                            int(config.get("kBlock", 0) or 0)))
         self._nf2ff = (P, len(om), np.isscalar(config["omega"]))
 
-    def readNF2FFAccumulation(self):
-        """{accumE, accumH}: 6P floats each for a scalar omega, else [F, 6P]."""
+    def readNF2FFAccumulation(self, fp64=False):
+        """{accumE, accumH}: 6P floats each for a scalar omega, else [F, 6P];
+        fp64=True returns float64 (efs_read_nf2ff_f64)."""
         if not self._nf2ff:
             return None
         P, F, scalar = self._nf2ff
-        e = np.empty(F * 6 * P, dtype=np.float32)
-        h = np.empty(F * 6 * P, dtype=np.float32)
-        self._check(self.lib.efs_read_nf2ff(self.ctx, e, h))
+        dt = np.float64 if fp64 else np.float32
+        e = np.empty(F * 6 * P, dtype=dt)
+        h = np.empty(F * 6 * P, dtype=dt)
+        self._check((self.lib.efs_read_nf2ff_f64 if fp64 else self.lib.efs_read_nf2ff)(self.ctx, e, h))
         if scalar:
             return {"accumE": e, "accumH": h}
         return {"accumE": e.reshape(F, 6 * P), "accumH": h.reshape(F, 6 * P)}
@@ -538,10 +548,10 @@ class HIPEngine:
 
     def pathFlags(self):
         """Forms in use: set of 'vec4', 'coef_idx', 'upml4', 'fused', 'upml_idx',
-        'f16', 'nf2ff_capture', 'coef_idx16'."""
+        'f16', 'nf2ff_capture', 'coef_idx16', 'f64'."""
         f = C.c_uint32(0)
         self._check(self.lib.efs_path_flags(self.ctx, C.byref(f)))
-        names = ("vec4", "coef_idx", "upml4", "fused", "upml_idx", "f16", "nf2ff_capture", "coef_idx16")
+        names = ("vec4", "coef_idx", "upml4", "fused", "upml_idx", "f16", "nf2ff_capture", "coef_idx16", "f64")
         return {n for b, n in enumerate(names) if f.value & (1 << b)}
 
 

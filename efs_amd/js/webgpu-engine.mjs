@@ -54,13 +54,15 @@ export class HIPEngine {
     get numTS() { return this._h ? addon.numTS(this._h) : 0; }
     set numTS(v) { this._need(); addon.setNumTS(this._h, v >>> 0); }
 
-    // options.storage: 'fp32' (default) or 'fp16' (fields stored as FP16
-    // with per-component power-of-two scales; reads return FP32).
+    // options.storage: 'fp32' (default), 'fp16' (fields stored as FP16
+    // with per-component power-of-two scales) or 'fp64' (the FP64 run of the
+    // same engine); reads return FP32.
     async init(gridSize, coefficients, options) {
         this._need();
         const storage = (options && options.storage) || 'fp32';
-        if (storage !== 'fp32' && storage !== 'fp16') throw new Error("storage must be 'fp32' or 'fp16'");
-        addon.setStorage(this._h, storage === 'fp16' ? 1 : 0);
+        const codes = { fp32: 0, fp16: 1, fp64: 2 };
+        if (!(storage in codes)) throw new Error("storage must be 'fp32', 'fp16' or 'fp64'");
+        addon.setStorage(this._h, codes[storage]);
         const [Nx, Ny, Nz] = gridSize;
         addon.init(this._h, Nx, Ny, Nz, f32(coefficients.vv), f32(coefficients.vi), f32(coefficients.ii), f32(coefficients.iv));
         this.numLines = [Nx, Ny, Nz];

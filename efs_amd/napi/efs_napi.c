@@ -203,7 +203,7 @@ static napi_value js_last_error(napi_env env, napi_callback_info info) {
     return s;
 }
 
-/* setStorage(h, fmt): 0 = FP32, 1 = FP16 (takes effect at init) */
+/* setStorage(h, fmt): 0 = FP32, 1 = FP16, 2 = FP64 (takes effect at init) */
 static napi_value js_set_storage(napi_env env, napi_callback_info info) {
     BEGIN();
     CTX(0);

@@ -141,6 +141,10 @@ int efs_set_num_ts(efs_ctx *ctx, uint32_t ts);
  * 3*nx*ny*nz floats each; either pointer may be NULL. */
 int efs_get_fields(efs_ctx *ctx, float *volt, float *curr);
 int efs_upload_fields(efs_ctx *ctx, const float *volt, const float *curr);
+/* The fields as FP64 (no counterpart in the reference): the stored values of
+ * the FP64 storage form, the FP32 values widened otherwise.  Same layout and
+ * NULL rules as efs_get_fields. */
+int efs_get_fields_f64(efs_ctx *ctx, double *volt, double *curr);
 
 /* configureProbeGather / readProbeGather (src/webgpu-engine.mjs:2577-2613,
  * 2697-2715): out receives 2*n floats [volt0, curr0, volt1, ...]. */
@@ -170,6 +174,10 @@ int efs_cfg_nf2ff(efs_ctx *ctx, const uint32_t *points, uint32_t P, const double
  * receive F*6*P floats, frequency-major, each frequency in the reference
  * layout [6*p + 2*c + {0: re, 1: im}]. */
 int efs_read_nf2ff(efs_ctx *ctx, float *accE, float *accH);
+/* The same accumulators as FP64 (no counterpart in the reference): the FP64
+ * form's values, or the FP32 accumulators plus their compensation terms
+ * summed in double. */
+int efs_read_nf2ff_f64(efs_ctx *ctx, double *accE, double *accH);
 
 /* computeNF2FFfarField (src/webgpu-engine.mjs:2871-2967) for frequency bin
  * freq_idx: point_meta 8 floats per point (pos x,y,z [m], normalDir,
@@ -233,17 +241,23 @@ enum {
     EFS_PATH_UPML_IDX = 16, /* UPML coefficients as byte indices + tables */
     EFS_PATH_F16 = 32,      /* FP16 field storage */
     EFS_PATH_NF2FF_CAPTURE = 64, /* NF2FF z-face points read slabs the H sweep wrote */
-    EFS_PATH_COEF_IDX16 = 128    /* core coefficients as 16-bit indices + cached tables */
+    EFS_PATH_COEF_IDX16 = 128,   /* core coefficients as 16-bit indices + cached tables */
+    EFS_PATH_F64 = 256           /* FP64 field storage and arithmetic (1-wide sweeps) */
 };
 int efs_path_flags(const efs_ctx *ctx, uint32_t *flags);
 
 /* Field storage (no counterpart in the reference, whose fields are FP32):
- * EFS_STORAGE_F32 (default), or EFS_STORAGE_F16 -- volt and curr held as
- * FP16 with a power-of-two scale per component, updated in FP32 registers;
- * the scales follow the field range on the device (checked every 32 steps).
- * Takes effect at the next efs_init.  FP16 needs Nz % 4 == 0 and UPML
- * z-bounds that are multiples of 4.  Every read returns FP32 values. */
-enum { EFS_STORAGE_F32 = 0, EFS_STORAGE_F16 = 1 };
+ * EFS_STORAGE_F32 (default); EFS_STORAGE_F16 -- volt and curr held as FP16
+ * with a power-of-two scale per component, updated in FP32 registers, the
+ * scales following the field range on the device (checked every 32 steps);
+ * FP16 needs UPML z-bounds that are multiples of 4.  EFS_STORAGE_F64 -- the
+ * precision reference run of the same engine: fields, UPML flux, Mur state
+ * and the NF2FF buffer, fold and accumulators in FP64 (the coefficients are
+ * the FP32 values given, widened); it runs the core sweeps, UPML, Mur,
+ * excitation, gathers, NF2FF, far field and energy, and refuses ADE, TFSF,
+ * RLC, PBC and steady-state configuration.  Takes effect at the next
+ * efs_init.  Every read returns FP32 values. */
+enum { EFS_STORAGE_F32 = 0, EFS_STORAGE_F16 = 1, EFS_STORAGE_F64 = 2 };
 int efs_set_storage(efs_ctx *ctx, int storage);
 /* Current scales: volt components 0..2, curr 0..2 (value = stored * scale;
  * all 1 with FP32 storage). */
