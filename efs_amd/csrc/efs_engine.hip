@@ -24,6 +24,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <thread>
 #include <vector>
@@ -37,9 +38,36 @@
 
 using namespace efs;
 
+// Device scratch of the allocation-free far field (efs_farfield_noalloc):
+// the FP64 partial integrals, 8 per (point chunk, angle).  One copy per GPU,
+// shared by every context on it and held until the process ends.  In host
+// code `scratch` only names the symbol: its device address comes from
+// hipGetSymbolAddress, looked up once per device (efs_create, with the
+// device current) and kept in g_scratch.
+constexpr size_t kFarScratchDoubles = (size_t)4 << 17;  // 4 MiB
+static __device__ __align__(256) double scratch[kFarScratchDoubles];
+
 namespace {
 
 std::string g_create_error;
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+double* g_scratch[kMaxDevices] = {};
+
+// Device address of this device's scratch (the device must be current).
+hipError_t scratch_address(int device, double** out) {
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (device < 0 || device >= kMaxDevices) return hipErrorInvalidDevice;
+    if (!g_scratch[device]) {
+        void* p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess) return e;
+        g_scratch[device] = static_cast<double*>(p);
+    }
+    *out = g_scratch[device];
+    return hipSuccess;
+}
 
 struct Timed {
     int which;
@@ -206,6 +234,12 @@ struct efs_ctx {
 
     // energy
     double* en_partial = nullptr;
+
+    // this device's far-field scratch (efs_farfield_noalloc) and the NF2FF
+    // point permutation on the device (internal point q -> caller's index)
+    double* scratch = nullptr;
+    hipError_t scratch_err = hipSuccess;
+    uint32_t* nf_perm_d = nullptr;
 
     // profiling
     bool prof = false;
@@ -599,7 +633,7 @@ void free_all(efs_ctx* c) {
     dfree(c->rlc_gidx); dfree(c->rlc_type); dfree(c->rlc_c); dfree(c->rlc_vdn); dfree(c->rlc_jn); dfree(c->rlc_vil);
     dfree(c->g_idx); dfree(c->g_out); dfree(c->gb_idx); dfree(c->gb_ring);
     dfree(c->pbc_imag); dfree(c->ss_gidx); dfree(c->ss_e);
-    dfree(c->nf_pts); dfree(c->nf_el); dfree(c->nf_buf); dfree(c->nf_acc); dfree(c->nf_cmp); dfree(c->nf_omega); dfree(c->nf_tw);
+    dfree(c->nf_pts); dfree(c->nf_el); dfree(c->nf_buf); dfree(c->nf_acc); dfree(c->nf_cmp); dfree(c->nf_omega); dfree(c->nf_perm_d); dfree(c->nf_tw);
     dfree(c->nf_slab);
     dfree(c->en_partial);
 }
@@ -1093,6 +1127,9 @@ int efs_create(efs_ctx** out, int device) {
     }
     hipError_t e = hipSetDevice(device);
     if (e == hipSuccess) e = hipStreamCreate(&c->stream);
+    // the no-alloc far field's scratch; on failure only that entry point
+    // reports it (c->scratch stays null, scratch_err keeps the code)
+    if (e == hipSuccess) c->scratch_err = scratch_address(device, &c->scratch);
     if (e != hipSuccess) {
         g_create_error = std::string("HIP setup failed: ") + hipGetErrorString(e);
         if (c->stream) hipStreamDestroy(c->stream);
@@ -1902,6 +1939,7 @@ int efs_cfg_nf2ff(efs_ctx* c, const uint32_t* points, uint32_t P, const double* 
     for (uint32_t q = 0; q < P; ++q)
         for (int a = 0; a < 3; ++a) sorted[3 * (size_t)q + a] = points[3 * (size_t)perm[q] + a];
     TRY(dupload(c, c->nf_pts, sorted.data(), sorted.size()));
+    TRY(dupload(c, c->nf_perm_d, perm.data(), perm.size()));
     c->nf_perm = std::move(perm);
     TRY(plan_nf_capture(c, sorted, P));
     TRY(dupload(c, c->nf_el, edge_lens, el_total));
@@ -2095,6 +2133,28 @@ int efs_farfield_mirror(efs_ctx* c, uint32_t f, const float* meta, const float* 
                         int mirror_dir, double mirror_pos, int mirror_type, int image_normals, float* out) {
     return farfield_impl(c, f, meta, theta, nT, phi, nP, center, frequency, radius, mirror_dir, mirror_pos,
                          mirror_type, image_normals, out);
+}
+
+int efs_farfield_noalloc(efs_ctx* c, uint32_t f, const float* meta_dev, const float* theta_dev, uint32_t nT,
+                         const float* phi_dev, uint32_t nP, const double* center, double frequency, double radius,
+                         float* out_dev) {
+    TRY(farfield_checks(c, f, meta_dev, theta_dev, nT, phi_dev, nP, center, radius, out_dev));
+    if (!c->scratch)
+        return fail(c, EFS_ERR_HIP, "far-field scratch unavailable: %s", hipGetErrorString(c->scratch_err));
+    if (!c->nf_perm_d) return fail(c, EFS_ERR_STATE, "NF2FF accumulation not configured");
+    const size_t na = (size_t)nT * nP;
+    const size_t cap_chunks = kFarScratchDoubles / (8 * na);  // chunks of partials that fit the scratch
+    if (cap_chunks < 1)
+        return fail(c, EFS_ERR_ARG, "efs_farfield_noalloc: %zu angles exceed the scratch (at most %zu angles)", na,
+                    kFarScratchDoubles / 8);
+    TRY(nf2ff_fold(c));
+    const FarPlan plan = far_plan(c, na, false, (uint32_t)std::min<size_t>(cap_chunks, 0xffffffffu));
+    const FarParams q = far_params(c, nT, nP, center, frequency, radius, -1, 0.0, 0, 0);
+    // every partial the final kernel reads is written by this call's first
+    // kernel, so nothing needs clearing between calls
+    launch_farfield(c, f, meta_dev, c->nf_perm_d, theta_dev, phi_dev, q, plan, c->scratch, out_dev);
+    CHECK_LAUNCH(c);
+    return EFS_OK;
 }
 
 int efs_set_nf2ff_compensated(efs_ctx* c, int on) {

@@ -78,6 +78,8 @@ SIGNATURES = {
     "efs_path_flags": (C.c_int, [C.c_void_p, C.POINTER(C.c_uint32)]),
     "efs_get_sizes": (C.c_int, [C.c_void_p, C.POINTER(EfsSizes)]),
     "efs_set_nf2ff_compensated": (C.c_int, [C.c_void_p, C.c_int]),
+    "efs_farfield_noalloc": (C.c_int, [C.c_void_p, C.c_uint32, C.c_void_p, C.c_void_p, C.c_uint32, C.c_void_p,
+                                       C.c_uint32, _f64p, C.c_double, C.c_double, C.c_void_p]),
     "efs_cfg_pbc": (C.c_int, [C.c_void_p, C.c_uint32, _u32p, _f64p]),
     "efs_set_storage": (C.c_int, [C.c_void_p, C.c_int]),
     "efs_field_scales": (C.c_int, [C.c_void_p, _f32p]),

@@ -201,6 +201,23 @@ int efs_farfield_mirror(efs_ctx *ctx, uint32_t freq_idx, const float *point_meta
                         double radius, int mirror_dir, double mirror_pos, int mirror_type, int image_normals,
                         float *out);
 
+/* The far field of efs_farfield with nothing allocated, for callers that
+ * cannot allocate device memory (code under HIP graph capture, tight
+ * serving loops, harnesses that forbid hipMalloc).  No counterpart in the
+ * reference.  point_meta_dev (8 floats per NF2FF point, the caller's point
+ * order), theta_dev, phi_dev and out_dev (5 floats per angle, theta-major)
+ * are DEVICE pointers; center is a host array of 3.  The FP64 partial sums
+ * go to a static device scratch of 4 MiB per GPU (allocated with the code
+ * object, held until the process ends), so n_theta * n_phi may be at most
+ * 65,536 (larger requests fail with EFS_ERR_ARG).  The work is enqueued on
+ * the context's stream and the call returns without waiting (efs_sync
+ * waits).  Every context on one GPU shares that GPU's scratch: calls that
+ * overlap on one GPU (other streams, other threads) race on it.  No mirror
+ * option. */
+int efs_farfield_noalloc(efs_ctx *ctx, uint32_t freq_idx, const float *point_meta_dev, const float *theta_dev,
+                         uint32_t n_theta, const float *phi_dev, uint32_t n_phi, const double *center,
+                         double frequency, double radius, float *out_dev);
+
 /* NF2FF accumulation form (no counterpart in the reference, which adds each
  * step's term into FP32 accumulators with a plain +=,
  * src/webgpu-engine.mjs:785-798): 1 (default) = compensated, every DFT
