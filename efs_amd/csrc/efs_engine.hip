@@ -146,6 +146,21 @@ struct efs_ctx {
     bool fields_zero = true;        // no upload and no step since init: scales are free to choose
     uint64_t source_end = 0;        // first step index at which no excitation / TFSF sample is injected
     bool source_periodic = false;
+    // regions as given (z-bounds) and, for regions widened to lane
+    // boundaries, the padding cells (field device index, flux index)
+    std::vector<uint32_t> pml_tz0, pml_tz1;
+    uint32_t *pml_pad_g = nullptr, *pml_pad_f = nullptr;
+    uint32_t pml_pad_n = 0;
+    std::vector<uint64_t> h_pad;  // (device index << 32 | flux index) of the padding cells, sorted
+    // padding cells that a hook writes after a sweep (V side: excitation,
+    // Mur apply, TFSF-V, ADE write-back, RLC, PBC-V; I side: TFSF-I, ADE
+    // write-back, PBC-I): their flux is set to the field again after the
+    // hook phase (k_pml_pad_sync), rebuilt when a configuration changes
+    bool pad_dirty = true;
+    uint32_t *pad_hv_g = nullptr, *pad_hv_f = nullptr, *pad_hi_g = nullptr, *pad_hi_f = nullptr;
+    uint32_t pad_hv_n = 0, pad_hi_n = 0;
+    std::vector<uint32_t> h_tv_idx;    // TFSF voltage points (interface indices)
+    std::vector<uint32_t> h_rlc_dev;   // RLC cells (device indices)
     uint8_t* pml_idx = nullptr;   // per region: E then H component indices (PmlRegion::pidx)
     float4* pml_tab_e = nullptr;  // [3][256] (vv, vvfo, vvfn, 0)
     float4* pml_tab_h = nullptr;  // [3][256] (ii, iifo, iifn, 0)
@@ -453,6 +468,66 @@ uint64_t env_u64(const char* name, uint64_t dflt) {
     return std::strtoull(v, nullptr, 10);
 }
 
+// ---- widened UPML regions: padding cells' flux follows the field ----
+FieldView fv(const efs_ctx* c, int curr);
+
+int pml_pad_sync(efs_ctx* c) {
+    if (!c->pml_pad_n) return EFS_OK;
+    hipLaunchKernelGGL(k_pml_pad_sync, dim3(nblk(c->pml_pad_n)), dim3(kPointThreads), 0, c->stream, fv(c, 0),
+                       c->pml_vflux, c->pml_pad_g, c->pml_pad_f, c->pml_pad_n);
+    hipLaunchKernelGGL(k_pml_pad_sync, dim3(nblk(c->pml_pad_n)), dim3(kPointThreads), 0, c->stream, fv(c, 1),
+                       c->pml_cflux, c->pml_pad_g, c->pml_pad_f, c->pml_pad_n);
+    CHECK_LAUNCH(c);
+    return EFS_OK;
+}
+
+// The padding cells among the cells the hooks write after each sweep.
+int rebuild_pad_hooks(efs_ctx* c) {
+    c->pad_dirty = false;
+    c->pad_hv_n = c->pad_hi_n = 0;
+    if (!c->pml_pad_n) return EFS_OK;
+    const bool pbc = !c->pbc_e.empty() || !c->pbc_h.empty();  // whole face planes: every padding cell
+    std::vector<uint32_t> v, i;
+    for (uint32_t g : c->h_exc_gidx) v.push_back(tr(c, g));
+    for (uint32_t g : c->h_mur_normal) v.push_back(tr(c, g));
+    for (uint32_t g : c->h_tv_idx) v.push_back(tr(c, g));
+    v.insert(v.end(), c->h_rlc_dev.begin(), c->h_rlc_dev.end());
+    if (c->ade_apply)
+        for (uint32_t g : c->h_ade_gidx) {
+            v.push_back(tr(c, g));
+            i.push_back(tr(c, g));
+        }
+    for (uint32_t g : c->h_ti_idx) i.push_back(tr(c, g));
+    auto pick = [&](std::vector<uint32_t>& dev, uint32_t*& dg, uint32_t*& df, uint32_t& n) -> int {
+        std::vector<uint32_t> pg, pf;
+        if (pbc) {
+            for (uint64_t e : c->h_pad) {
+                pg.push_back((uint32_t)(e >> 32));
+                pf.push_back((uint32_t)e);
+            }
+        } else {
+            std::sort(dev.begin(), dev.end());
+            dev.erase(std::unique(dev.begin(), dev.end()), dev.end());
+            for (uint32_t d : dev) {
+                auto it = std::lower_bound(c->h_pad.begin(), c->h_pad.end(), (uint64_t)d << 32);
+                if (it != c->h_pad.end() && (uint32_t)(*it >> 32) == d) {
+                    pg.push_back(d);
+                    pf.push_back((uint32_t)*it);
+                }
+            }
+        }
+        n = (uint32_t)pg.size();
+        if (n) {
+            TRY(dupload(c, dg, pg.data(), n));
+            TRY(dupload(c, df, pf.data(), n));
+        }
+        return EFS_OK;
+    };
+    TRY(pick(v, c->pad_hv_g, c->pad_hv_f, c->pad_hv_n));
+    TRY(pick(i, c->pad_hi_g, c->pad_hi_f, c->pad_hi_n));
+    return EFS_OK;
+}
+
 // ---- alias resolution: field index -> UPML flux index when in a region ----
 uint32_t alias_of(const efs_ctx* c, uint32_t gidx) {
     const uint32_t n = (uint32_t)(gidx / c->N);
@@ -462,7 +537,8 @@ uint32_t alias_of(const efs_ctx* c, uint32_t gidx) {
     const uint32_t y = rem / c->nz, z = rem - y * c->nz;
     for (uint32_t k = 0; k < c->pml.n; ++k) {
         const PmlRegion& R = c->pml.r[k];
-        if (x >= R.x0 && x < R.x1 && y >= R.y0 && y < R.y1 && z >= R.z0 && z < R.z1) {
+        // the region as given (a widened region's padding cells read the field)
+        if (x >= R.x0 && x < R.x1 && y >= R.y0 && y < R.y1 && z >= c->pml_tz0[k] && z < c->pml_tz1[k]) {
             const uint64_t p = R.flux + (uint64_t)n * R.cs + ((uint64_t)(x - R.x0) * R.ny + (y - R.y0)) * R.nz +
                                (z - R.z0);
             return kAliasFlux | (uint32_t)p;
@@ -620,6 +696,10 @@ void free_all(efs_ctx* c) {
     dfree(c->exc_signal); dfree(c->exc_amp); dfree(c->exc_delay); dfree(c->exc_gidx);
     dfree(c->pml_coef); dfree(c->pml_vflux); dfree(c->pml_cflux);
     dfree(c->pml_idx); dfree(c->pml_tab_e); dfree(c->pml_tab_h);
+    dfree(c->pml_pad_g); dfree(c->pml_pad_f);
+    c->pml_pad_n = 0;
+    dfree(c->pad_hv_g); dfree(c->pad_hv_f); dfree(c->pad_hi_g); dfree(c->pad_hi_f);
+    c->pad_hv_n = c->pad_hi_n = 0;
     dfree(c->fscale); dfree(c->f_amax); dfree(c->f_factor); dfree(c->f_smin);
     c->half = false;
     dfree(c->mur_coeff); dfree(c->mur_saved); dfree(c->mur_normal); dfree(c->mur_shifted);
@@ -641,6 +721,14 @@ void free_all(efs_ctx* c) {
 void reset_extensions(efs_ctx* c) {
     c->exc_n = 0;
     c->pml = PmlTable{};
+    c->pml_tz0.clear();
+    c->pml_tz1.clear();
+    c->pml_pad_n = 0;
+    c->h_pad.clear();
+    c->pad_dirty = true;
+    c->pad_hv_n = c->pad_hi_n = 0;
+    c->h_tv_idx.clear();
+    c->h_rlc_dev.clear();
     c->mur_n = 0;
     c->mur_ne = 0;
     c->h_mur_normal.clear();
@@ -841,18 +929,23 @@ bool hooks_clash(const efs_ctx* c) {
 }
 
 // The NF2FF capture for this step, or an inactive one: it needs the 4-wide
-// grid sweeps, no UPML cell in the captured rows, no periodic H faces, and
-// no other hook that changes H after the H sweep inside the captured rows
-// (TFSF-I, ADE write-back).
+// grid sweeps, no UPML cell in the captured rows unless the sweeps are the
+// fused form, no periodic H faces, and no other hook that changes H after
+// the H sweep inside the captured rows (TFSF-I, ADE write-back).
 NfCapture nf_capture(const efs_ctx* c) {
     const NfCapture& k = c->nf_cap;
     if (!k.on || !c->nf_P || !c->vec4 || (c->pml.n && !c->pml_split)) return NfCapture{};
     if (!c->pbc_h.empty() || (c->cap_dirty ? hooks_clash(c) : c->cap_clash)) return NfCapture{};
-    for (uint32_t r = 0; r < c->pml.n; ++r) {
+    // In the fused sweeps a lane computes its UPML cells' final values
+    // before it captures them; only the two-launch form updates UPML cells
+    // in a later launch, after the capture.
+    for (uint32_t r = 0; r < c->pml.n && !c->pml_fused; ++r) {
+        // the region as given: padding cells of a widened region update as
+        // core cells, so the capture holds their values too
         const PmlRegion& R = c->pml.r[r];
         if (R.x0 > k.x1 || R.x1 <= k.x0 || R.y0 > k.y1 || R.y1 <= k.y0) continue;
         for (uint32_t f = 0; f < k.nf; ++f)
-            if (R.z0 <= k.z[f] + 1 && R.z1 > k.z[f]) return NfCapture{};
+            if (c->pml_tz0[r] <= k.z[f] + 1 && c->pml_tz1[r] > k.z[f]) return NfCapture{};
     }
     return k;
 }
@@ -1003,6 +1096,9 @@ int launch_step(efs_ctx* c) {
         hipLaunchKernelGGL(k_rlc_apply, dim3(nblk(c->rlc_n, c->pt)), dim3(c->pt), 0, c->stream, fv(c, 0),
                            c->rlc_gidx, c->rlc_type, c->rlc_c, c->rlc_vdn, c->rlc_jn, c->rlc_vil, c->rlc_n);
     }
+    if (c->pad_hv_n)  // padding cells of widened UPML regions that the hooks above wrote
+        hipLaunchKernelGGL(k_pml_pad_sync, dim3(nblk(c->pad_hv_n, c->pt)), dim3(c->pt), 0, c->stream, fv(c, 0),
+                           c->pml_vflux, c->pad_hv_g, c->pad_hv_f, c->pad_hv_n);
     // pre-current
     if (c->ade_n && c->ade_i_live) {
         hipLaunchKernelGGL(k_ade_update, dim3(nblk(c->ade_n, c->pt)), dim3(c->pt), 0, c->stream, fv(c, 1),
@@ -1060,6 +1156,9 @@ int launch_step(efs_ctx* c) {
         hipLaunchKernelGGL(k_ade_apply, dim3(nblk(c->ade_n, c->pt)), dim3(c->pt), 0, c->stream, fv(c, 1),
                            c->ade_gidx, c->ade_ia, c->ade_n);
     }
+    if (c->pad_hi_n)
+        hipLaunchKernelGGL(k_pml_pad_sync, dim3(nblk(c->pad_hi_n, c->pt)), dim3(c->pt), 0, c->stream, fv(c, 1),
+                           c->pml_cflux, c->pad_hi_g, c->pad_hi_f, c->pad_hi_n);
     // engine hooks
     if (c->nf_P) {
         if (c->nf_slot == 0) c->nf_ts0 = t + 1;
@@ -1314,6 +1413,7 @@ int source_scales(efs_ctx* c, const float* signal, uint32_t len, const float* va
 int efs_cfg_excitation(efs_ctx* c, const float* signal, uint32_t len, const uint32_t* delay, const float* amp,
                        const uint32_t* dir, const uint32_t* pos, uint32_t n, uint32_t period) {
     TRY(need_init(c));
+    c->pad_dirty = true;
     if (n == 0) {
         c->exc_n = 0;
         c->h_exc_gidx.clear();
@@ -1348,11 +1448,11 @@ int efs_cfg_excitation(efs_ctx* c, const float* signal, uint32_t len, const uint
 int efs_cfg_pml(efs_ctx* c, uint32_t nreg, const uint32_t* start, const uint32_t* num, const float* vv,
                 const float* vvfo, const float* vvfn, const float* ii, const float* iifo, const float* iifn) {
     TRY(need_init(c));
+    c->pad_dirty = true;
     if (nreg > (uint32_t)kMaxPml) return fail(c, EFS_ERR_ARG, "at most %d UPML regions are supported", kMaxPml);
     HIP_TRY(c, hipStreamSynchronize(c->stream));
     PmlTable t{};
-    uint64_t total = 0;
-    bool aligned = true;
+    std::vector<uint32_t> tz0(nreg), tz1(nreg);  // the regions' z-bounds as given
     for (uint32_t k = 0; k < nreg; ++k) {
         const uint32_t* s = start + 3 * k;
         const uint32_t* m = num + 3 * k;
@@ -1364,15 +1464,51 @@ int efs_cfg_pml(efs_ctx* c, uint32_t nreg, const uint32_t* start, const uint32_t
         R.x0 = s[0]; R.x1 = s[0] + m[0];
         R.y0 = s[1]; R.y1 = s[1] + m[1];
         R.z0 = s[2]; R.z1 = s[2] + m[2];
-        R.ny = m[1]; R.nz = m[2];
-        R.cs = m[0] * m[1] * m[2];
+        tz0[k] = R.z0;
+        tz1[k] = R.z1;
+        for (uint32_t j = 0; j < k; ++j)
+            if (box_overlap(t.r[j], R)) return fail(c, EFS_ERR_ARG, "UPML regions %u and %u overlap", j, k);
+    }
+    // The 4-wide sweeps need region z-bounds on 4-cell lane boundaries.
+    // openEMS's PML_n slabs are n + 1 lines deep, so they usually are not;
+    // such regions are widened to the enclosing lane boundaries (within the
+    // padded rows) when the widened boxes still do not overlap.  A padding
+    // cell carries the UPML coefficients (0, 0, 1): its update is then
+    // V' = F' = a*F + b*curl, and with its flux kept equal to its field
+    // (k_pml_pad_sync after configuration and uploads) that is the core
+    // update to the bit.  Reads of the flux (alias indices) use the given
+    // bounds, so padding cells read the field.
+    bool widen = false;
+    if (c->vec4 && env_u64("EFS_PML_WIDEN", 1) != 0) {
+        PmlTable w = t;
+        bool need = false;
+        for (uint32_t k = 0; k < nreg; ++k) {
+            PmlRegion& R = w.r[k];
+            const uint32_t z0 = R.z0 / 4 * 4, z1 = std::min((R.z1 + 3) / 4 * 4, c->nzp);
+            need = need || z0 != R.z0 || z1 != R.z1;
+            R.z0 = z0;
+            R.z1 = z1;
+        }
+        bool ok = need;
+        for (uint32_t k = 0; k < nreg && ok; ++k)
+            for (uint32_t j = 0; j < k && ok; ++j) ok = !box_overlap(w.r[j], w.r[k]);
+        if (ok) {
+            t = w;
+            widen = true;
+        }
+    }
+    uint64_t total = 0;
+    bool aligned = true;
+    for (uint32_t k = 0; k < nreg; ++k) {
+        PmlRegion& R = t.r[k];
+        R.ny = R.y1 - R.y0;
+        R.nz = R.z1 - R.z0;
+        R.cs = (R.x1 - R.x0) * R.ny * R.nz;
         total = (total + 3) / 4 * 4;  // 16-byte aligned region bases
         R.flux = total;
         R.coef = 6 * total;
         total += 3 * (uint64_t)R.cs;
         aligned = aligned && (R.z0 % 4 == 0) && (R.nz % 4 == 0);
-        for (uint32_t j = 0; j < k; ++j)
-            if (box_overlap(t.r[j], R)) return fail(c, EFS_ERR_ARG, "UPML regions %u and %u overlap", j, k);
     }
     if (total >= (1ull << 31)) return fail(c, EFS_ERR_ARG, "UPML volume too large");
     // UPML launch geometry: per region, 256-lane tiles over its (y, z/4)
@@ -1394,23 +1530,58 @@ int efs_cfg_pml(efs_ctx* c, uint32_t nreg, const uint32_t* start, const uint32_t
     }
     t.n = nreg;
     t.blocks = blocks;
+    // the six coefficient arrays in the engine's region layout (padding
+    // cells of widened regions: identity coefficients 0, 0, 1)
+    std::vector<float> coef;
+    std::vector<uint32_t> pad_g, pad_f;  // padding cells: field (device index) and flux index
     if (nreg) {
         if (!vv || !vvfo || !vvfn || !ii || !iifo || !iifn)
             return fail(c, EFS_ERR_ARG, "UPML coefficient arrays are required");
-        std::vector<float> coef(6 * total, 0.f);
+        coef.assign(6 * total, 0.f);
         const float* src[6] = {vv, vvfo, vvfn, ii, iifo, iifn};
+        const float ident[3] = {0.f, 0.f, 1.f};
         uint64_t off_in = 0;
         for (uint32_t k = 0; k < nreg; ++k) {
-            const uint64_t T = 3 * (uint64_t)t.r[k].cs;
+            const PmlRegion& R = t.r[k];
+            const uint64_t T = 3 * (uint64_t)R.cs;
+            const uint32_t nzg = tz1[k] - tz0[k];  // the caller's row length
+            const uint64_t csg = (uint64_t)(R.x1 - R.x0) * R.ny * nzg;
             for (int a = 0; a < 6; ++a)
-                std::memcpy(coef.data() + t.r[k].coef + a * T, src[a] + off_in, T * sizeof(float));
-            off_in += T;
+                for (uint32_t n = 0; n < 3; ++n)
+                    for (uint32_t x = 0; x < R.x1 - R.x0; ++x)
+                        for (uint32_t y = 0; y < R.ny; ++y) {
+                            float* d = coef.data() + R.coef + a * T + n * (uint64_t)R.cs + ((uint64_t)x * R.ny + y) * R.nz;
+                            const float* sr = src[a] + off_in + n * csg + ((uint64_t)x * R.ny + y) * nzg;
+                            for (uint32_t z = R.z0; z < R.z1; ++z)
+                                d[z - R.z0] = (z >= tz0[k] && z < tz1[k]) ? sr[z - tz0[k]] : ident[a % 3];
+                        }
+            off_in += 3 * csg;
+            if (widen)
+                for (uint32_t n = 0; n < 3; ++n)
+                    for (uint32_t x = R.x0; x < R.x1; ++x)
+                        for (uint32_t y = R.y0; y < R.y1; ++y)
+                            for (uint32_t z = R.z0; z < std::min(R.z1, c->nz); ++z) {
+                                if (z >= tz0[k] && z < tz1[k]) continue;
+                                pad_g.push_back((uint32_t)(n * c->S + ((uint64_t)x * c->ny + y) * c->nzp + z));
+                                pad_f.push_back((uint32_t)(R.flux + n * (uint64_t)R.cs +
+                                                           ((uint64_t)(x - R.x0) * R.ny + (y - R.y0)) * R.nz + (z - R.z0)));
+                            }
         }
         TRY(dupload(c, c->pml_coef, coef.data(), coef.size()));
         TRY(dzero(c, c->pml_vflux, total * rw(c)));
         TRY(dzero(c, c->pml_cflux, total * rw(c)));
     } else {
         dfree(c->pml_coef); dfree(c->pml_vflux); dfree(c->pml_cflux);
+    }
+    c->pml_tz0 = tz0;
+    c->pml_tz1 = tz1;
+    c->pml_pad_n = (uint32_t)pad_g.size();
+    c->h_pad.resize(pad_g.size());
+    for (size_t q = 0; q < pad_g.size(); ++q) c->h_pad[q] = ((uint64_t)pad_g[q] << 32) | pad_f[q];
+    std::sort(c->h_pad.begin(), c->h_pad.end());
+    if (c->pml_pad_n) {
+        TRY(dupload(c, c->pml_pad_g, pad_g.data(), pad_g.size()));
+        TRY(dupload(c, c->pml_pad_f, pad_f.data(), pad_f.size()));
     }
     // UPML coefficient compression: one table per field and component over
     // all regions, byte indices per region cell (region-local NIJK, like the
@@ -1425,16 +1596,15 @@ int efs_cfg_pml(efs_ctx* c, uint32_t nreg, const uint32_t* start, const uint32_t
         }
         std::vector<uint8_t> idx(ibytes);
         TripleTable tabs[2][3];
-        const float* src[2][3] = {{vv, vvfo, vvfn}, {ii, iifo, iifn}};
         bool ok = true;
-        uint64_t off_in = 0;
         for (uint32_t k = 0; k < nreg && ok; ++k) {
-            const uint64_t cs = t.r[k].cs;
+            const uint64_t cs = t.r[k].cs, T = 3 * cs;
+            const float* base = coef.data() + t.r[k].coef;  // the engine-layout arrays built above
             for (int h = 0; h < 2 && ok; ++h)
                 for (int n = 0; n < 3 && ok; ++n)
-                    ok = tabs[h][n].add(src[h][0] + off_in + n * cs, src[h][1] + off_in + n * cs,
-                                        src[h][2] + off_in + n * cs, cs, idx.data() + t.r[k].pidx + (3 * h + n) * cs);
-            off_in += 3 * cs;
+                    ok = tabs[h][n].add(base + (3 * h) * T + n * cs, base + (3 * h + 1) * T + n * cs,
+                                        base + (3 * h + 2) * T + n * cs, cs,
+                                        idx.data() + t.r[k].pidx + (3 * h + n) * cs);
         }
         if (ok) {
             std::vector<float4> te(3 * 256), th(3 * 256);
@@ -1454,8 +1624,10 @@ int efs_cfg_pml(efs_ctx* c, uint32_t nreg, const uint32_t* start, const uint32_t
     c->pml_fused = env_u64("EFS_PML_FUSED", 1) != 0;
     if (c->half && nreg && !(c->pml_split && c->pml_fused)) {
         c->pml = PmlTable{};
-        return fail(c, EFS_ERR_ARG, "FP16 field storage needs UPML z-bounds that are multiples of 4");
+        return fail(c, EFS_ERR_ARG, "FP16 field storage needs UPML regions the 4-wide sweeps can take "
+                                    "(z-bounds on 4-cell boundaries after widening)");
     }
+    TRY(pml_pad_sync(c));
     TRY(plan_sweeps(c));
     c->alias_dirty = true;
     return EFS_OK;
@@ -1464,6 +1636,7 @@ int efs_cfg_pml(efs_ctx* c, uint32_t nreg, const uint32_t* start, const uint32_t
 int efs_cfg_mur(efs_ctx* c, uint32_t n, const float* coeff, const uint32_t* normal_idx,
                 const uint32_t* shifted_idx) {
     TRY(need_init(c));
+    c->pad_dirty = true;
     HIP_TRY(c, hipStreamSynchronize(c->stream));
     if (n == 0) {
         c->mur_n = 0;
@@ -1502,6 +1675,7 @@ int efs_cfg_mur(efs_ctx* c, uint32_t n, const float* coeff, const uint32_t* norm
 
 int efs_cfg_ade_clear(efs_ctx* c) {
     TRY(need_init(c));
+    c->pad_dirty = true;
     HIP_TRY(c, hipStreamSynchronize(c->stream));
     c->h_ade_gidx.clear();
     c->h_ade_lor.clear();
@@ -1524,6 +1698,7 @@ int efs_cfg_ade_add(efs_ctx* c, uint32_t has_lorentz, uint32_t dir, uint32_t nce
                     const float* v_int, const float* v_ext, const float* v_lor, const float* i_int,
                     const float* i_ext, const float* i_lor) {
     TRY(need_init(c));
+    c->pad_dirty = true;
     TRY(need_not_f64(c, ncells, "ADE"));
     if (dir > 2) return fail(c, EFS_ERR_ARG, "ADE direction %u out of range", dir);
     if (ncells && !pos_idx) return fail(c, EFS_ERR_ARG, "ADE pos_idx is required");
@@ -1543,6 +1718,7 @@ int efs_cfg_ade_add(efs_ctx* c, uint32_t has_lorentz, uint32_t dir, uint32_t nce
 
 int efs_set_ade_apply(efs_ctx* c, int apply) {
     TRY(need_init(c));
+    c->pad_dirty = true;
     c->ade_apply = apply != 0;
     c->alias_dirty = true;
     c->cap_dirty = true;
@@ -1554,6 +1730,7 @@ int efs_cfg_tfsf(efs_ctx* c, const float* signal, uint32_t len, uint32_t period,
                  uint32_t ni, const uint32_t* i_idx, const uint32_t* i_dint, const float* i_dfrac,
                  const float* i_amp) {
     TRY(need_init(c));
+    c->pad_dirty = true;
     TRY(need_not_f64(c, nv + ni, "TFSF"));
     if ((nv || ni) && (!signal || len == 0)) return fail(c, EFS_ERR_ARG, "TFSF signal must not be empty");
     for (uint32_t k = 0; k < nv; ++k)
@@ -1584,6 +1761,7 @@ int efs_cfg_tfsf(efs_ctx* c, const float* signal, uint32_t len, uint32_t period,
     c->tfsf_nv = nv;
     c->tfsf_ni = ni;
     c->h_ti_idx.assign(i_idx, i_idx + ni);
+    c->h_tv_idx.assign(v_idx, v_idx + nv);
     c->cap_dirty = true;
     if (nv || ni) TRY(source_scales(c, signal, len, v_amp, nv, i_amp, ni));
     return EFS_OK;
@@ -1592,10 +1770,12 @@ int efs_cfg_tfsf(efs_ctx* c, const float* signal, uint32_t len, uint32_t period,
 int efs_cfg_rlc(efs_ctx* c, uint32_t n, const uint32_t* field_idx, const uint32_t* direction,
                 const uint32_t* type_flag, const float* coeffs) {
     TRY(need_init(c));
+    c->pad_dirty = true;
     TRY(need_not_f64(c, n, "RLC"));
     HIP_TRY(c, hipStreamSynchronize(c->stream));
     if (n == 0) {
         c->rlc_n = 0;
+        c->h_rlc_dev.clear();
         return EFS_OK;
     }
     std::vector<uint32_t> g(n);
@@ -1605,6 +1785,7 @@ int efs_cfg_rlc(efs_ctx* c, uint32_t n, const uint32_t* field_idx, const uint32_
         g[k] = tr(c, (uint64_t)direction[k] * c->N + field_idx[k]);
     }
     TRY(dupload(c, c->rlc_gidx, g.data(), n));
+    c->h_rlc_dev = g;
     TRY(dupload(c, c->rlc_type, type_flag, n));
     TRY(dupload(c, c->rlc_c, coeffs, 9 * (size_t)n));
     TRY(dzero(c, c->rlc_vdn, 3 * (size_t)n));
@@ -1616,6 +1797,7 @@ int efs_cfg_rlc(efs_ctx* c, uint32_t n, const uint32_t* field_idx, const uint32_
 
 int efs_cfg_pbc(efs_ctx* c, uint32_t n, const uint32_t* axis, const double* phase) {
     TRY(need_init(c));
+    c->pad_dirty = true;
     TRY(need_not_f64(c, n, "PBC"));
     HIP_TRY(c, hipStreamSynchronize(c->stream));
     if (n > 3) return fail(c, EFS_ERR_ARG, "at most 3 periodic axes");
@@ -1692,6 +1874,7 @@ int efs_iterate(efs_ctx* c, uint32_t n) {
     HIP_TRY(c, hipSetDevice(c->device));
     if (c->ade_dirty) TRY(rebuild_ade(c));
     if (c->alias_dirty) TRY(rebuild_aliases(c));
+    if (c->pad_dirty) TRY(rebuild_pad_hooks(c));
     if (c->cap_dirty) {
         c->cap_clash = hooks_clash(c);
         c->cap_dirty = false;
@@ -1801,7 +1984,7 @@ int efs_upload_fields(efs_ctx* c, const float* volt, const float* curr) {
             }
         }
         c->fields_zero = false;
-        return EFS_OK;
+        return pml_pad_sync(c);
     }
     if (c->half) {  // per component: scale 2^e with the largest value near 2^10, then narrow
         float sc[6];
@@ -1825,7 +2008,7 @@ int efs_upload_fields(efs_ctx* c, const float* volt, const float* curr) {
         HIP_TRY(c, hipMemcpy(c->fscale, sc, sizeof sc, hipMemcpyHostToDevice));
         HIP_TRY(c, hipMemcpy(c->f_smin, sc, sizeof sc, hipMemcpyHostToDevice));
         c->fields_zero = false;
-        return EFS_OK;
+        return pml_pad_sync(c);
     }
     c->fields_zero = false;
     for (int a = 0; a < 2; ++a) {
@@ -1836,7 +2019,7 @@ int efs_upload_fields(efs_ctx* c, const float* volt, const float* curr) {
             HIP_TRY(c, hipMemcpy2D(dst + n * c->S, c->nzp * sizeof(float), in + n * c->N, c->nz * sizeof(float),
                                    c->nz * sizeof(float), rows, hipMemcpyHostToDevice));
     }
-    return EFS_OK;
+    return pml_pad_sync(c);
 }
 
 int efs_cfg_gather(efs_ctx* c, const uint32_t* idx, uint32_t n) {

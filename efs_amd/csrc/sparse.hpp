@@ -218,6 +218,16 @@ __global__ void k_ss_accumulate(FieldView volt, const uint32_t* __restrict__ gid
     else e2[k] += v * v;
 }
 
+// Padding cells of UPML regions widened to 4-cell lane boundaries: their
+// flux is the field's value (the padding's identity coefficients then give
+// the core update), set after the regions are configured and after uploads.
+__global__ void k_pml_pad_sync(FieldView field, float* __restrict__ flux, const uint32_t* __restrict__ gidx,
+                               const uint32_t* __restrict__ fidx, uint32_t n) {
+    const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    flux[fidx[k]] = field.ld(gidx[k]);
+}
+
 // Probe gather into out[2*k], out[2*k+1] (src/webgpu-engine.mjs:2593-2665).
 template <class FV>
 __global__ void k_gather(FV volt, FV curr, const uint32_t* __restrict__ idx, float* __restrict__ out, uint32_t n) {

@@ -41,6 +41,10 @@ import xml.etree.ElementTree as ET
 import numpy as np
 
 from . import operator as op
+from . import ports
+
+# Interpolation of E / H point probes (openEMS ProbeBox types 2 and 3).
+FIELD_PROBE_INTERPOLATION = "none"
 
 _FACES = ("xmin", "xmax", "ymin", "ymax", "zmin", "zmax")
 _SUPPORTED_PRIMS = ("Box", "Cylinder", "CylindricalShell", "Sphere", "Curve")
@@ -243,24 +247,43 @@ class _Grid:
 
     def curve_path(self, vertices):
         """Edges (n, i, j, k, sign) of the mesh path through the vertices:
-        each leg snaps its ends to mesh nodes and walks in diagonal rounds,
-        each round driving one edge along every axis that still has to
-        advance, all from the round's starting node, then moving that node
-        one step along each of them (a one-cell diagonal drives the three
-        edges at its first node, as openEMS's trace of the reference's
-        cavity fixture shows)."""
+        each leg snaps its ends to mesh nodes and walks one edge at a time
+        from the start node to the end node, each step along the axis whose
+        next node lies nearest the straight line between the leg's two
+        vertices (ties to the lower axis), the grid walk openEMS uses for
+        curves (Operator::FindPath).  A one-cell diagonal of the
+        reference's cavity fixture thus drives its edges shortest cell side
+        first, which reproduces openEMS's probe trace (tests/golden/
+        cavity_openems.json)."""
         out = []
-        nodes = [[snap(self.lines[m], v[m]) for m in range(3)] for v in vertices]
-        for a, b in zip(nodes[:-1], nodes[1:]):
+        for va, vb in zip(vertices[:-1], vertices[1:]):
+            a = [snap(self.lines[m], va[m]) for m in range(3)]
+            b = [snap(self.lines[m], vb[m]) for m in range(3)]
+            p0 = np.asarray(va, np.float64)
+            d = np.asarray(vb, np.float64) - p0
+            dd = float(d @ d)
+
+            def dist2(node):
+                q = np.array([self.lines[m][node[m]] for m in range(3)]) - p0
+                t = float(q @ d) / dd if dd > 0 else 0.0
+                r = q - t * d
+                return float(r @ r)
             cur = list(a)
-            while cur != list(b):
-                moves = [(m, 1 if b[m] > cur[m] else -1) for m in range(3) if b[m] != cur[m]]
-                for m, step in moves:
-                    e = list(cur)
-                    e[m] = cur[m] if step > 0 else cur[m] - 1
-                    out.append((m, e[0], e[1], e[2], float(step)))
-                for m, step in moves:
-                    cur[m] += step
+            while cur != b:
+                best, bm, bs = None, -1, 0
+                for m in range(3):
+                    if cur[m] == b[m]:
+                        continue
+                    step = 1 if b[m] > cur[m] else -1
+                    nxt = list(cur)
+                    nxt[m] += step
+                    dv = dist2(nxt)
+                    if best is None or dv < best:
+                        best, bm, bs = dv, m, step
+                e = list(cur)
+                e[bm] = cur[bm] if bs > 0 else cur[bm] - 1
+                out.append((bm, e[0], e[1], e[2], float(bs)))
+                cur[bm] += bs
         return out
 
 
@@ -339,7 +362,6 @@ def build(sim, k_block=64):
     mur_faces = [f for f, (kind, _) in enumerate(bcs) if kind == "MUR"]
     if mur_faces:
         out["mur"] = op.mur_config(mesh, mur_faces, dT)
-    out["coefficients"] = op.to_f32(coeffs)
     # excitation signal and points
     ex = sim["excitation"]
     if ex["type"] != 0:
@@ -350,8 +372,9 @@ def build(sim, k_block=64):
     for prop in props:
         if prop["type"] != "Excitation":
             continue
-        if int(float(prop["attrs"].get("Type", "0"))) != 0:
-            raise NotImplementedError("excitation type %s is not supported (soft E only)" % prop["attrs"]["Type"])
+        etype = int(float(prop["attrs"].get("Type", "0")))
+        if etype not in (0, 1):
+            raise NotImplementedError("excitation type %s is not supported (soft or hard E only)" % etype)
         vec = [float(v) for v in prop["attrs"].get("Excite", "0,0,0").split(",")]
         for n in range(3):
             if vec[n] == 0:
@@ -369,10 +392,14 @@ def build(sim, k_block=64):
                 w = np.ones(len(idx)) if prop["weight"] is None else \
                     weight(prop["weight"][n], c[:, 0], c[:, 1], c[:, 2])
                 a = vec[n] * w * mesh.primal[n][idx[:, n]]
+                if etype == 1:  # hard E: the edge's own update is switched off, the source sets it
+                    coeffs["vv"][n, idx[:, 0], idx[:, 1], idx[:, 2]] = 0.0
+                    coeffs["vi"][n, idx[:, 0], idx[:, 1], idx[:, 2]] = 0.0
                 keep = a != 0
                 pos.append(((idx[:, 0] * N[1] + idx[:, 1]) * N[2] + idx[:, 2])[keep])
                 dirs.append(np.full(int(keep.sum()), n))
                 amp.append(a[keep])
+    out["coefficients"] = op.to_f32(coeffs)
     if not pos:
         raise ValueError("the simulation has no excitation")
     pos = np.concatenate(pos)
@@ -411,6 +438,10 @@ def build(sim, k_block=64):
                 probes.append({"name": prop["name"], "type": 1, "start": lo, "stop": hi, "normDir": nd,
                                "weight": wgt, "startInside": [inside(lo[m], m) for m in range(3)],
                                "stopInside": [inside(hi[m], m) for m in range(3)]})
+            elif t in (2, 3):  # E / H field at the node nearest P1
+                node = [snap(g.lines[m], p1[m]) for m in range(3)]
+                probes.append({"name": prop["name"], "type": t, "start": node, "stop": node, "weight": wgt,
+                               "taps": ports.field_taps(mesh, t, node, FIELD_PROBE_INTERPOLATION)})
             else:
                 raise NotImplementedError("probe type %d is not supported" % t)
     out["probes"] = probes

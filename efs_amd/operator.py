@@ -266,11 +266,17 @@ def pml_sigma_profile(width_cells, dl, R0=1e-6, g=2.5):
 
 
 def pml_regions_for_shell(mesh, widths):
-    """Non-overlapping UPML boxes for widths (xmin,xmax,ymin,ymax,zmin,zmax):
-    x slabs span the full y,z range, y slabs the x interior, z slabs the
-    x and y interior."""
+    """Non-overlapping UPML boxes for widths (xmin,xmax,ymin,ymax,zmin,zmax),
+    each PML_w slab w + 1 lines deep as openEMS sets its UPML range
+    (Operator_Ext_UPML::Create_UPML: lower lines 0..w, upper lines
+    N-1-w..N-1, the grading measured from line w resp. N-1-w): the upper
+    slab's first line holds the half-cell-deep E and H components (E along
+    the axis, H across it), which the grading gives a nonzero conductivity;
+    the lower slab's last line has none, but openEMS computes its
+    coefficients with the UPML formulas too.  x slabs span the full y,z
+    range, y slabs the x interior, z slabs the x and y interior."""
     nx, ny, nz = mesh.shape
-    w = widths
+    w = [k + 1 if k else 0 for k in widths]
     regs = []
     if w[0]:
         regs.append(([0, 0, 0], [w[0], ny, nz]))
@@ -357,15 +363,17 @@ def pec_boundary(mat, faces=range(6)):
 
 
 def gaussian_pulse(f0, fc, dT):
-    """openEMS-style Gaussian: cos(2 pi f0 (t - T0)) exp(-((t - T0) 2 pi fc / 3)^2),
-    T0 = 9 / (2 pi fc), over 2 T0.  Sample 0 is zero and sample k >= 1 is the
-    pulse at t = (k-1) dT: the engine reads sample 0 once the pulse has
-    ended (excitation position past the end), so that must inject nothing."""
+    """openEMS's Gaussian excitation: sample k = 0 .. int(2 T0 / dT) is
+    cos(2 pi f0 (t - T0)) exp(-((t - T0) 2 pi fc / 3)^2) at t = k dT,
+    T0 = 9 / (2 pi fc).  The engine (like openEMS's Apply2Voltages) reads
+    sample 0 once the pulse has ended, so openEMS keeps injecting g(0) =
+    cos(2 pi f0 T0) e^-9 after the pulse; the reference's native openEMS
+    fixtures carry that constant (tests/fixtures/coax: the late-time mean of
+    the port voltage and current), and these samples reproduce them."""
     T0 = 9.0 / (2 * np.pi * fc)
-    n = int(2 * T0 / dT)
+    n = int(2 * T0 / dT) + 1
     t = np.arange(n) * dT
-    pulse = np.cos(2 * np.pi * f0 * (t - T0)) * np.exp(-((t - T0) * 2 * np.pi * fc / 3) ** 2)
-    return np.concatenate([[0.0], pulse]).astype(np.float32)
+    return (np.cos(2 * np.pi * f0 * (t - T0)) * np.exp(-((t - T0) * 2 * np.pi * fc / 3) ** 2)).astype(np.float32)
 
 
 def mur_config(mesh, faces, dT):

@@ -22,7 +22,10 @@ quantities:
 A probe is a dict {type: 0 voltage | 1 current, start, stop (node or dual
 line indices), weight, and for current probes normDir, startInside,
 stopInside} -- the fields of the reference's probe records
-(getProbeInfo, src/embind_api.cpp:921-980).
+(getProbeInfo, src/embind_api.cpp:921-980).  Field probes (openEMS ProbeBox
+types 2 and 3: E and H at a mesh node) carry their interpolation as taps
+[[component, edge component, x, y, z, weight], ...] over volt (type 2) or
+curr (type 3); their trace is the field vector per sample (field_taps).
 """
 import numpy as np
 
@@ -40,6 +43,16 @@ def build_gather_indices(probes, grid_size):
     for p in probes:
         off = len(indices)
         s, t = [int(v) for v in p["start"]], [int(v) for v in p["stop"]]
+        if p["type"] in (2, 3):
+            comp, wts = [], []
+            for c, n, x, y, z, wt in p["taps"]:
+                indices.append(lin(int(n), (x, y, z)))
+                comp.append(int(c))
+                wts.append(float(wt))
+            slices.append({"offset": off, "count": len(indices) - off,
+                           "type": "efield" if p["type"] == 2 else "hfield",
+                           "comp": np.array(comp, np.int64), "w": np.array(wts)})
+            continue
         if p["type"] == 0:
             signs = []
             for n in range(3):
@@ -94,27 +107,94 @@ def _current_legs(d, s, t, si, so):
 def compute_probe_integrals(gathered, probes, slices):
     """One value per probe from one gather sample (interleaved volt, curr
     pairs per index): voltage probes sum volt, current probes sum curr, with
-    the edge signs, times the probe weight."""
+    the edge signs, times the probe weight; field probes give their
+    magnitude (field_values has the vector)."""
     g = np.asarray(gathered, dtype=np.float64).reshape(-1, 2)
     out = np.zeros(len(probes))
     for k, (p, sl) in enumerate(zip(probes, slices)):
+        if sl["type"] in ("efield", "hfield"):
+            out[k] = float(np.linalg.norm(field_values(g, p, sl)))
+            continue
         col = 0 if sl["type"] == "voltage" else 1
         vals = g[sl["offset"]:sl["offset"] + sl["count"], col]
         out[k] = float(np.sum(vals * sl["signs"])) * float(p.get("weight", 1.0))
     return out
 
 
+def field_values(g, p, sl):
+    """The field vector of a field probe from one gather sample (g: the
+    (volt, curr) pairs per gathered index)."""
+    col = 0 if sl["type"] == "efield" else 1
+    vals = g[sl["offset"]:sl["offset"] + sl["count"], col] * sl["w"]
+    return np.bincount(sl["comp"], weights=vals, minlength=3)[:3] * float(p.get("weight", 1.0))
+
+
 def probe_traces(data, num_samples, stride, probes, slices, sample_interval, dT):
-    """(time, value) per probe from a buffered gather readback."""
-    d = np.asarray(data, dtype=np.float64)[:num_samples * stride].reshape(num_samples, stride)
-    vals = np.stack([compute_probe_integrals(d[s], probes, slices) for s in range(num_samples)]) \
-        if num_samples else np.zeros((0, len(probes)))
+    """(time, value) per probe from a buffered gather readback; a field
+    probe's value is its [num_samples, 3] field vector, sampled on the
+    primal (E) or dual (H, half a step later) time grid."""
+    g = np.asarray(data, dtype=np.float64)[:num_samples * stride].reshape(num_samples, stride // 2, 2)
     steps = (np.arange(num_samples) + 1) * sample_interval
     out = []
-    for k, p in enumerate(probes):
-        half = 0.5 if p["type"] == 1 else 0.0
-        out.append(((steps + half) * dT, vals[:, k]))
+    for p, sl in zip(probes, slices):
+        half = 0.5 if p["type"] in (1, 3) else 0.0
+        wt = float(p.get("weight", 1.0))
+        if sl["type"] in ("efield", "hfield"):
+            col = 0 if sl["type"] == "efield" else 1
+            vals = g[:, sl["offset"]:sl["offset"] + sl["count"], col] * sl["w"]
+            v = np.stack([vals[:, sl["comp"] == c].sum(axis=1) for c in range(3)], axis=1) * wt
+        else:
+            col = 0 if sl["type"] == "voltage" else 1
+            v = (g[:, sl["offset"]:sl["offset"] + sl["count"], col] * sl["signs"]).sum(axis=1) * wt
+        out.append(((steps + half) * dT, v))
     return out
+
+
+def field_taps(mesh, kind, node, interpolation="node"):
+    """Taps of an E (kind 2) or H (kind 3) point probe at mesh node `node`
+    as openEMS's field probes read them (Engine_Interface_FDTD: the E field
+    of an edge is volt / edge length, the H field curr / dual edge length).
+    interpolation "node" (openEMS NODE_INTERPOLATE): E_n is the mean of the
+    two n-edges meeting at the node, weighted by the other edge's length
+    (the lower-boundary node takes its own edge, the last node 0); H_n the
+    mean of the four H_n around the node in the plane across n (0 on the
+    grid's outer planes).  "none": the field of the edge at the node."""
+    N = mesh.shape
+    p = [int(v) for v in node]
+    taps = []
+    for n in range(3):
+        if kind == 2:
+            d = mesh.primal[n][p[n]]
+            if interpolation == "none":
+                if d > 0:
+                    taps.append([n, n, *p, 1.0 / d])
+                continue
+            if p[n] == N[n] - 1 or d == 0:
+                continue
+            if p[n] == 0:
+                taps.append([n, n, *p, 1.0 / d])
+                continue
+            q = list(p)
+            q[n] -= 1
+            dd = mesh.primal[n][q[n]]
+            r = d / (d + dd)
+            taps.append([n, n, *p, (1.0 - r) / d])
+            taps.append([n, n, *q, r / dd])
+        else:
+            nP, nPP = (n + 1) % 3, (n + 2) % 3
+            if interpolation == "none":
+                d = mesh.dual[n][p[n]]
+                if d > 0:
+                    taps.append([n, n, *p, 1.0 / d])
+                continue
+            if any(p[m] == N[m] - 1 for m in range(3)) or p[nP] == 0 or p[nPP] == 0:
+                continue
+            for a, b in ((0, 0), (1, 0), (1, 1), (0, 1)):
+                q = list(p)
+                q[nP] -= a
+                q[nPP] -= b
+                taps.append([n, n, *q, 0.25 / mesh.dual[n][q[n]]])
+    return taps
 
 
 def dft_time2freq(time, values, freq, signal_type="pulse"):

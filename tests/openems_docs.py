@@ -132,3 +132,45 @@ def coax_xml(res=20.0, length=200.0, ri=100.0, rai=230.0, raa=240.0):
     props += _prop("ProbeBox", [("Name", "it1"), ("Type", "1"), ("Weight", "1"), ("NormDir", "-1")],
                    _prims(_box([-cur, -cur, mid_z], [cur, cur, mid_z])))
     return _doc(2000, 0.0, 1e9, ["0", "0", "0", "0", "0", "PML_8"], 1e-3, [mx, my, mz], props, end="1e-6")
+
+
+def dipole_xml(du=1e-6, f_max=1e9):
+    """The dipole fixture's simulation: an infinitesimal dipole (hard E
+    source on a two-cell z curve, lambda/50 long at f_max) at the centre of
+    a cube of +-20 dipole lengths meshed at half a dipole length, Mur on
+    all six faces, 500 steps; E and H point probes at +-4.5 half-lengths on
+    the x axis, snapped to the mesh as the fixture's generator snaps them
+    (nearest line, the first of two equally near)."""
+    c0 = 299792458.0
+    L = c0 / f_max / du / 50
+    h = L / 2
+    lines = []
+    v = -20 * L
+    while v <= 20 * L + 0.001:
+        lines.append(v)
+        v += h
+
+    def near(val):
+        return min(lines, key=lambda m: abs(m - val))
+    s = 4.5 * h
+    props = _prop("Excitation", [("Name", "infDipole"), ("Type", "1"), ("Excite", "0,0,1")],
+                  _prims('<Curve Priority="1">%s%s</Curve>' % (_pt("Vertex", [0, 0, -h]), _pt("Vertex", [0, 0, h]))))
+    for name, x, t in (("et1", -s, 2), ("et2", s, 2), ("ht1", -s, 3), ("ht2", s, 3)):
+        p = [near(x), near(0.0), near(0.0)]
+        props += _prop("ProbeBox", [("Name", name), ("Type", str(t)), ("Weight", "1"), ("NormDir", "-1")],
+                       _prims(_box(p, p)))
+    return _doc(500, 0.0, f_max, ["2"] * 6, du, [lines, lines, lines], props, end="1e-20")
+
+
+def engine_comparison_xml():
+    """The engine-comparison fixture's simulation: the cavity of cavity_xml
+    on a coarser 16 x 8 x 18 mesh, soft source on the one-cell diagonal
+    from node (10, 5, 12), V_z probe from node (5, 3, 4) to (5, 3, 5), 500
+    steps."""
+    mx, my, mz = np.linspace(0, 5e-2, 16), np.linspace(0, 2e-2, 8), np.linspace(0, 6e-2, 18)
+    props = _prop("Excitation", [("Name", "exc"), ("Type", "0"), ("Excite", "1,1,1")],
+                  _prims('<Curve Priority="0">%s%s</Curve>' % (_pt("Vertex", [mx[10], my[5], mz[12]]),
+                                                               _pt("Vertex", [mx[11], my[6], mz[13]]))))
+    props += _prop("ProbeBox", [("Name", "vp"), ("Type", "0"), ("Weight", "1"), ("NormDir", "-1")],
+                   _prims(_box([mx[5], my[3], mz[4]], [mx[5], my[3], mz[5]])))
+    return _doc(500, 5.5e9, 4.5e9, ["0"] * 6, 1.0, [mx, my, mz], props, end="1e-20")

@@ -139,8 +139,9 @@ def test_fp16_scales_follow_the_fields(HIPEngine):
 
 def test_fp16_needs_aligned_upml(HIPEngine, monkeypatch):
     """Any Nz works (device rows are padded to a multiple of 4); UPML
-    regions need z-bounds that are multiples of 4, and the 1-wide sweeps
-    (forced) have no FP16 form."""
+    regions whose z-bounds miss the 4-cell lanes are widened to them (the
+    fused sweeps run), and are refused only with the widening switched off
+    (EFS_PML_WIDEN=0); the 1-wide sweeps (forced) have no FP16 form."""
     g = HIPEngine()
     assert g.initGPU()
     N = 8 * 8 * 10
@@ -153,5 +154,9 @@ def test_fp16_needs_aligned_upml(HIPEngine, monkeypatch):
     monkeypatch.delenv("EFS_VEC")
     case = random_case(5, (12, 12, 16), pml_width=3)
     g.init(case["dims"], case["coeffs"], storage="fp16")
-    with pytest.raises(RuntimeError, match="multiples of 4"):
+    g.configurePML(case["pml"])
+    assert {"f16", "fused"} <= g.pathFlags()
+    monkeypatch.setenv("EFS_PML_WIDEN", "0")
+    g.init(case["dims"], case["coeffs"], storage="fp16")
+    with pytest.raises(RuntimeError, match="4-cell boundaries"):
         g.configurePML(case["pml"])

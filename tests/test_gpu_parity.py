@@ -582,3 +582,59 @@ def test_invalid_configuration_is_rejected(HIPEngine):
         fresh.iterate(1)  # before init
     g.iterate(2)  # engine still usable after rejected calls
     assert g.numTS == 2
+
+
+@pytest.mark.parametrize("upload_after", [False, True])
+def test_widened_upml_regions(HIPEngine, oracle_mod, monkeypatch, upload_after):
+    """openEMS's PML_n slabs are n + 1 lines deep (efs_amd.operator.
+    pml_regions_for_shell), so their z-bounds miss the 4-cell lanes; the
+    engine widens them to lane boundaries with identity padding cells whose
+    flux follows the field.  The 4-wide fused sweeps then give the bits of
+    the 1-wide sweeps on the regions as given (EFS_VEC=1), whether the
+    fields were uploaded before the UPML was configured or after, and the
+    oracle's fields within the FP32 tolerance; FP16 storage takes them."""
+    from efs_amd import operator as op
+    from cases import random_pml
+    dims = (20, 18, 24)
+    regions = op.pml_regions_for_shell(op.uniform_mesh(dims, 1e-3), [4] * 6)
+    assert any(s[2] % 4 or n[2] % 4 for s, n in regions)
+    case = random_case(71, dims, mur=False)
+    case["pml"] = random_pml(np.random.default_rng(72), regions)
+    steps = 40
+
+    def run(storage="fp32"):
+        g = HIPEngine()
+        assert g.initGPU()
+        g.init(case["dims"], case["coeffs"], storage=storage)
+        if not upload_after:
+            g.uploadFields(case["volt0"], case["curr0"])
+        g.configurePML(case["pml"])
+        if upload_after:
+            g.uploadFields(case["volt0"], case["curr0"])
+        g.configureExcitation(case["excitation"])
+        flags = g.pathFlags()
+        g.iterate(steps)
+        f = g.getFields()
+        g.destroy()
+        return flags, f
+    say("case widened upml, upload after", upload_after)
+    flags4, f4 = run()
+    assert {"vec4", "upml4", "fused"} <= flags4
+    monkeypatch.setenv("EFS_VEC", "1")
+    flags1, f1 = run()
+    assert "vec4" not in flags1
+    monkeypatch.delenv("EFS_VEC")
+    for k in ("volt", "curr"):
+        np.testing.assert_array_equal(f4[k], f1[k])
+    o = oracle_mod.Oracle(case["dims"], case["coeffs"])
+    o.uploadFields(case["volt0"], case["curr0"])
+    o.configurePML(case["pml"])
+    o.configureExcitation(case["excitation"])
+    o.iterate(steps)
+    assert_fields_close(f4, o.getFields(), what="widened upml")
+    # FP16 storage takes the widened regions on the fused path (its accuracy
+    # on them is the stress-grid test of tests/test_gpu_fp16.py, whose UPML
+    # slabs are widened too; this case's random coefficients grow without
+    # bound, which FP16 cannot hold)
+    flags16, _ = run("fp16")
+    assert "f16" in flags16 and "fused" in flags16
