@@ -30,6 +30,7 @@
 #include <vector>
 
 #include "../../include/efs_engine.h"
+#include "hooks.hpp"
 #include "nf2ff.hpp"
 #include "reduce.hpp"
 #include "sparse.hpp"
@@ -174,6 +175,14 @@ struct efs_ctx {
     uint8_t* mur_early = nullptr;       // post done before the excitation (k_mur_post_some)
     uint32_t* mur_early_list = nullptr;
     uint32_t mur_ne = 0;
+    // one-launch point work between the sweeps (k_post_e, k_post_h)
+    bool fuse_hooks = true;
+    bool post_e_ok = false;         // deferred lists built
+    uint8_t *mur_defer = nullptr, *exc_defer = nullptr;
+    uint2* mur_b = nullptr;         // deferred Mur entries: (normal cell, entry)
+    uint32_t* exc_b = nullptr;
+    uint32_t mur_nb = 0, exc_nb = 0;
+    bool mur_pre_ahead = false;     // the previous step's k_post_h ran this step's Mur pre pass
     std::vector<uint32_t> h_exc_gidx;   // excitation cells (interface indices)
 
     // ADE (all orders/directions flattened)
@@ -547,6 +556,46 @@ uint32_t alias_of(const efs_ctx* c, uint32_t gidx) {
     return tr(c, gidx);
 }
 
+// Lists of k_post_e / k_post_e_b.  Deferred cells: the shifted cells of
+// early entries (phase A reads them).  A Mur entry whose normal cell is
+// deferred applies in phase B; an excitation entry on a deferred cell adds
+// in phase B, one on a Mur normal cell is dropped (the apply overwrites
+// it).  Without ADE write-back, "early" is exactly: shifted cell = some
+// normal or excitation cell.
+int plan_post_e(efs_ctx* c, const std::vector<uint8_t>& early) {
+    c->post_e_ok = false;
+    if (!c->fuse_hooks) return EFS_OK;
+    std::vector<uint32_t> sh_early, nor(c->h_mur_normal);
+    for (uint32_t k = 0; k < c->mur_n; ++k)
+        if (early[k]) sh_early.push_back(c->h_mur_shifted[k]);
+    std::sort(sh_early.begin(), sh_early.end());
+    std::sort(nor.begin(), nor.end());
+    auto in = [](const std::vector<uint32_t>& v, uint32_t x) { return std::binary_search(v.begin(), v.end(), x); };
+    std::vector<uint8_t> mdef(c->mur_n, 0), edef(std::max<uint32_t>(c->exc_n, 1), 0);
+    std::vector<uint2> mb;
+    std::vector<uint32_t> eb;
+    for (uint32_t k = 0; k < c->mur_n; ++k)
+        if (in(sh_early, c->h_mur_normal[k])) {
+            mdef[k] = 1;
+            mb.push_back(make_uint2(tr(c, c->h_mur_normal[k]), k));
+        }
+    for (uint32_t k = 0; k < c->exc_n; ++k) {
+        const bool on_normal = in(nor, c->h_exc_gidx[k]);
+        if (on_normal || in(sh_early, c->h_exc_gidx[k])) {
+            edef[k] = 1;
+            if (!on_normal) eb.push_back(k);
+        }
+    }
+    TRY(dupload(c, c->mur_defer, mdef.data(), mdef.size()));
+    TRY(dupload(c, c->exc_defer, edef.data(), edef.size()));
+    if (!mb.empty()) TRY(dupload(c, c->mur_b, mb.data(), mb.size()));
+    if (!eb.empty()) TRY(dupload(c, c->exc_b, eb.data(), eb.size()));
+    c->mur_nb = (uint32_t)mb.size();
+    c->exc_nb = (uint32_t)eb.size();
+    c->post_e_ok = true;
+    return EFS_OK;
+}
+
 int rebuild_aliases(efs_ctx* c) {
     if (c->mur_n) {
         std::vector<uint32_t> ns(c->mur_n), ss(c->mur_n);
@@ -571,6 +620,7 @@ int rebuild_aliases(efs_ctx* c) {
         TRY(dupload(c, c->mur_early, early.data(), early.size()));
         TRY(dupload(c, c->mur_early_list, list.data(), list.size()));
         c->mur_ne = (uint32_t)list.size();
+        TRY(plan_post_e(c, early));
     }
     if (c->ade_n) {
         std::vector<uint32_t> sv(c->ade_n);
@@ -704,6 +754,9 @@ void free_all(efs_ctx* c) {
     c->half = false;
     dfree(c->mur_coeff); dfree(c->mur_saved); dfree(c->mur_normal); dfree(c->mur_shifted);
     dfree(c->mur_early); dfree(c->mur_early_list);
+    dfree(c->mur_defer); dfree(c->exc_defer); dfree(c->mur_b); dfree(c->exc_b);
+    c->post_e_ok = false;
+    c->mur_nb = c->exc_nb = 0;
     dfree(c->mur_normal_src); dfree(c->mur_shifted_src);
     dfree(c->ade_gidx); dfree(c->ade_src_v); dfree(c->ade_src_i); dfree(c->ade_lor);
     for (auto& p : c->ade_c) dfree(p);
@@ -783,6 +836,7 @@ int plan_sweeps(efs_ctx* c) {
     const double upml_b = upml_in_grid ? core_b + (c->pml_comp ? 27.0 : 60.0) : 2.0;
     const uint64_t want = env_u64("EFS_SWEEP_BLOCKS", 2048);
     const uint32_t min_planes = (uint32_t)std::max<uint64_t>(1, env_u64("EFS_SWEEP_MIN_PLANES", 1));
+    const uint32_t min_chunk = (uint32_t)std::max<uint64_t>(1, env_u64("EFS_SWEEP_MIN_CHUNK", 2));
     for (int v = 0; v < 2; ++v) {
         const uint64_t tcells = (uint64_t)kSweepThreads * (v == 0 ? 4u : 1u);
         const uint32_t T = (uint32_t)((cols + tcells - 1) / tcells);
@@ -827,7 +881,7 @@ int plan_sweeps(efs_ctx* c) {
             const double* ct = &cost[(size_t)t * nx];
             pre[0] = 0.0;
             for (uint32_t x = 0; x < nx; ++x) pre[x + 1] = pre[x] + ct[x];
-            const uint32_t n = (uint32_t)std::max(1.0, std::min((double)std::max(1u, nx / 2),
+            const uint32_t n = (uint32_t)std::max(1.0, std::min((double)std::max(1u, nx / min_chunk),
                                                                 std::floor(pre[nx] / target + 0.5)));
             // boundary j at the plane whose prefix cost is nearest j/n of the tile's total
             uint32_t x = 0, b0 = 0;
@@ -897,6 +951,37 @@ template <class FV>
 void launch_excitation(efs_ctx* c, FV v, uint32_t t) {
     hipLaunchKernelGGL(k_excitation<FV>, dim3(nblk(c->exc_n, c->pt)), dim3(c->pt), 0, c->stream, v, c->exc_signal,
                        c->exc_len, c->exc_period, c->exc_delay, c->exc_amp, c->exc_gidx, c->exc_n, t);
+}
+
+// Mur post + apply and the excitation: k_post_e, then k_post_e_b for the
+// deferred entries.
+template <class FV>
+void launch_post_e(efs_ctx* c, FV v, uint32_t t) {
+    PostE<typename FV::R> q{};
+    q.signal = c->exc_signal;
+    q.len = c->exc_len;
+    q.period = c->exc_period;
+    q.delay = c->exc_delay;
+    q.gidx = c->exc_gidx;
+    q.amp = c->exc_amp;
+    q.exc_n = c->exc_n;
+    q.numTS = t;
+    q.exc_defer = c->exc_defer;
+    q.exc_b = c->exc_b;
+    q.exc_nb = c->exc_nb;
+    q.coeff = c->mur_coeff;
+    q.normal = c->mur_normal;
+    q.shifted = c->mur_shifted;
+    q.saved = rp<FV>(c->mur_saved);
+    q.mur_n = c->mur_n;
+    q.mur_defer = c->mur_defer;
+    q.mur_b = c->mur_b;
+    q.mur_nb = c->mur_nb;
+    hipLaunchKernelGGL(k_post_e<FV>, dim3(nblk((size_t)c->mur_n + c->exc_n, kPostThreads)), dim3(kPostThreads), 0,
+                       c->stream, v, q);
+    if (c->exc_nb + c->mur_nb)
+        hipLaunchKernelGGL(k_post_e_b<FV>, dim3(nblk((size_t)c->exc_nb + c->mur_nb, kPostThreads)),
+                           dim3(kPostThreads), 0, c->stream, v, q);
 }
 
 template <class FV>
@@ -1004,7 +1089,43 @@ void launch_range_control(efs_ctx* c, uint32_t t) {
                        N, c->f_factor);
 }
 
-int launch_step(efs_ctx* c) {
+// NF2FF sample of this step (when configured), probe gather into `ring`
+// (null: none due) and, with `pre`, the next step's Mur pre pass, in one
+// launch (k_post_h).
+template <class FV>
+void launch_post_h(efs_ctx* c, FV v, FV i, NfCapture cap, float* ring, bool pre) {
+    PostH<FV> q{};
+    if (c->nf_P) {
+        q.pts = c->nf_pts;
+        q.el = c->nf_el;
+        q.g = c->nf_geom;
+        q.row_out = rp<FV>(c->nf_buf) + (size_t)c->nf_slot * c->nf_rows;
+        q.cap = cap;
+        q.nb_sample = nblk(c->nf_P, c->pt);
+    }
+    if (ring) {
+        q.gidx = c->gb_idx;
+        q.gout = ring;
+        q.gn = c->gb_n;
+        q.nb_gather = nblk(c->gb_n, c->pt);
+    }
+    uint32_t nb_pre = 0;
+    if (pre) {
+        q.vflux = rp<FV>(c->pml_vflux);
+        q.coeff = c->mur_coeff;
+        q.normal_src = c->mur_normal_src;
+        q.shifted_src = c->mur_shifted_src;
+        q.saved = rp<FV>(c->mur_saved);
+        q.mur_n = c->mur_n;
+        nb_pre = nblk(c->mur_n, c->pt);
+    }
+    hipLaunchKernelGGL(k_post_h<FV>, dim3(q.nb_sample + q.nb_gather + nb_pre), dim3(c->pt), 0, c->stream, v, i, q);
+}
+
+// One step.  next_follows: the same efs_iterate call runs another step
+// right after this one (its Mur pre pass may then run in this step's
+// k_post_h).
+int launch_step(efs_ctx* c, bool next_follows) {
     const uint32_t t = c->numTS;
     ProfScope step_scope(c, EFS_PROF_STEP);
     const bool has_pml = c->pml.n > 0;
@@ -1019,10 +1140,11 @@ int launch_step(efs_ctx* c) {
                            c->pml_vflux, c->ade_src_v, c->ade_lor, c->ade_c[0], c->ade_c[1], c->ade_c[2],
                            c->ade_va, c->ade_vl, c->ade_n);
     }
-    if (c->mur_n) {
+    if (c->mur_n && !c->mur_pre_ahead) {
         if (c->dbl) launch_mur_pre(c, fv64(c, 0));
         else launch_mur_pre(c, fv(c, 0));
     }
+    c->mur_pre_ahead = false;
     if (c->rlc_n) {
         hipLaunchKernelGGL(k_rlc_pre, dim3(nblk(c->rlc_n, c->pt)), dim3(c->pt), 0, c->stream, c->rlc_type,
                            c->rlc_c, c->rlc_vdn, c->rlc_vil, c->rlc_n);
@@ -1074,23 +1196,29 @@ int launch_step(efs_ctx* c) {
     for (const PbcAxis& a : c->pbc_e)  // after TFSF-V, before Mur post (src/webgpu-engine.mjs:2461)
         hipLaunchKernelGGL(k_pbc, dim3(nblk((size_t)a.b0 * a.b1, c->pt)), dim3(c->pt), 0, c->stream, fv(c, 0),
                            c->pbc_imag, a, c->ny, c->nzp, (uint64_t)c->S);
-    if (c->mur_n) {
-        if (c->mur_ne) {
-            if (c->dbl) launch_mur_post_some(c, fv64(c, 0));
-            else launch_mur_post_some(c, fv(c, 0));
+    if (c->mur_n && c->post_e_ok && !(c->ade_n && c->ade_apply)) {
+        // Mur post + excitation + Mur apply in one launch
+        if (c->dbl) launch_post_e(c, fv64(c, 0), t);
+        else launch_post_e(c, fv(c, 0), t);
+    } else {
+        if (c->mur_n) {
+            if (c->mur_ne) {
+                if (c->dbl) launch_mur_post_some(c, fv64(c, 0));
+                else launch_mur_post_some(c, fv(c, 0));
+            }
         }
-    }
-    if (c->ade_n && c->ade_apply) {
-        hipLaunchKernelGGL(k_ade_apply, dim3(nblk(c->ade_n, c->pt)), dim3(c->pt), 0, c->stream, fv(c, 0),
-                           c->ade_gidx, c->ade_va, c->ade_n);
-    }
-    if (c->exc_n) {
-        if (c->dbl) launch_excitation(c, fv64(c, 0), t);
-        else launch_excitation(c, fv(c, 0), t);
-    }
-    if (c->mur_n) {
-        if (c->dbl) launch_mur_finish(c, fv64(c, 0));
-        else launch_mur_finish(c, fv(c, 0));
+        if (c->ade_n && c->ade_apply) {
+            hipLaunchKernelGGL(k_ade_apply, dim3(nblk(c->ade_n, c->pt)), dim3(c->pt), 0, c->stream, fv(c, 0),
+                               c->ade_gidx, c->ade_va, c->ade_n);
+        }
+        if (c->exc_n) {
+            if (c->dbl) launch_excitation(c, fv64(c, 0), t);
+            else launch_excitation(c, fv(c, 0), t);
+        }
+        if (c->mur_n) {
+            if (c->dbl) launch_mur_finish(c, fv64(c, 0));
+            else launch_mur_finish(c, fv(c, 0));
+        }
     }
     if (c->rlc_n) {
         hipLaunchKernelGGL(k_rlc_apply, dim3(nblk(c->rlc_n, c->pt)), dim3(c->pt), 0, c->stream, fv(c, 0),
@@ -1160,6 +1288,28 @@ int launch_step(efs_ctx* c) {
         hipLaunchKernelGGL(k_pml_pad_sync, dim3(nblk(c->pad_hi_n, c->pt)), dim3(c->pt), 0, c->stream, fv(c, 1),
                            c->pml_cflux, c->pad_hi_g, c->pad_hi_f, c->pad_hi_n);
     // engine hooks
+    const bool range_due = c->half && (t + 1) % c->range_interval == 0;
+    const bool gather_due = c->gb_n && (t + 1) % c->gb_interval == 0 && c->gb_count < c->gb_max;
+    // NF2FF sampling, the probe gather and the next step's Mur pre pass in
+    // one launch; not when the FP16 range control (which rescales the
+    // fields) comes between them
+    const bool pre_ahead = c->fuse_hooks && next_follows && c->mur_n && !range_due;
+    const int parts = (c->nf_P ? 1 : 0) + (gather_due ? 1 : 0) + (pre_ahead ? 1 : 0);
+    if (c->fuse_hooks && !range_due && parts >= 2) {
+        if (c->nf_P && c->nf_slot == 0) c->nf_ts0 = t + 1;
+        float* ring = gather_due ? c->gb_ring + (size_t)c->gb_count * 2 * c->gb_n : nullptr;
+        if (c->dbl) launch_post_h(c, fv64(c, 0), fv64(c, 1), NfCapture{}, ring, pre_ahead);
+        else launch_post_h(c, fv(c, 0), fv(c, 1), nf_capture(c), ring, pre_ahead);
+        if (gather_due) c->gb_count++;
+        c->mur_pre_ahead = pre_ahead;
+        if (c->nf_P) {
+            c->nf_slot++;
+            if (c->nf_slot == c->nf_K) TRY(nf2ff_fold(c));
+        }
+        CHECK_LAUNCH(c);
+        c->numTS++;
+        return EFS_OK;
+    }
     if (c->nf_P) {
         if (c->nf_slot == 0) c->nf_ts0 = t + 1;
         if (c->dbl)
@@ -1173,8 +1323,8 @@ int launch_step(efs_ctx* c) {
         c->nf_slot++;
         if (c->nf_slot == c->nf_K) TRY(nf2ff_fold(c));
     }
-    if (c->half && (t + 1) % c->range_interval == 0) launch_range_control(c, t);
-    if (c->gb_n && (t + 1) % c->gb_interval == 0 && c->gb_count < c->gb_max) {
+    if (range_due) launch_range_control(c, t);
+    if (gather_due) {
         float* ring = c->gb_ring + (size_t)c->gb_count * 2 * c->gb_n;
         if (c->dbl)
             hipLaunchKernelGGL(k_gather<FieldView64>, dim3(nblk(c->gb_n, c->pt)), dim3(c->pt), 0, c->stream,
@@ -1315,6 +1465,7 @@ int efs_init(efs_ctx* c, uint32_t nx, uint32_t ny, uint32_t nz, const float* vv,
     c->vec4 = !c->dbl && c->nzp % 4 == 0 && env_u64("EFS_VEC", 4) == 4;
     c->half = c->storage_req == EFS_STORAGE_F16;
     c->range_interval = (uint32_t)std::max<uint64_t>(1, env_u64("EFS_RANGE_INTERVAL", 32));
+    c->fuse_hooks = env_u64("EFS_FUSE_HOOKS", 1) != 0;
     if (c->half && !c->vec4) return fail(c, EFS_ERR_ARG, "FP16 field storage needs the 4-wide sweeps (EFS_VEC=4)");
     {
         const float one[6] = {1.f, 1.f, 1.f, 1.f, 1.f, 1.f};
@@ -1880,7 +2031,8 @@ int efs_iterate(efs_ctx* c, uint32_t n) {
         c->cap_dirty = false;
     }
     if (n) c->fields_zero = false;
-    for (uint32_t s = 0; s < n; ++s) TRY(launch_step(c));
+    c->mur_pre_ahead = false;
+    for (uint32_t s = 0; s < n; ++s) TRY(launch_step(c, s + 1 < n));
     return EFS_OK;
 }
 

@@ -1,0 +1,131 @@
+// hooks.hpp -- the per-step point work of one phase in one launch.
+//
+// Each point kernel (Mur, excitation, probe gather, NF2FF sampling) moves a
+// few kilobytes but costs a full launch on the GPU: on a grid of half a
+// million cells the six of them took more time than the two sweeps.  The
+// kernels here run the work that sits between two sweeps in one launch,
+// in an order that keeps every result bit-identical to the separate
+// kernels in the reference's phase order (src/webgpu-engine.mjs:2407-2504).
+//
+// After the E sweep (k_post_e, then k_post_e_b when its lists are not
+// empty): Mur post + apply and the excitation.  The reference order is post
+// (all entries), excitation, apply (all entries).  Phase A (k_post_e) runs,
+// one thread per Mur entry and per excitation entry, every operation that
+// no other operation of the phase reads or writes:
+//  - Mur entry k forms s = saved[k] + c*V[shifted] (the post) from its
+//    shifted cell, which nothing writes in this phase (an entry whose
+//    shifted cell is some normal or excitation cell is "early", and every
+//    cell an early entry reads is deferred), and applies V[normal] = s
+//    unless its normal cell is deferred; a deferred entry keeps s in
+//    saved[k];
+//  - an excitation entry adds its sample unless its cell is deferred or a
+//    Mur normal cell: there the apply overwrites the sum, so the entry has
+//    no effect and is dropped.
+// Phase B (k_post_e_b, its own launch, so phase A is complete everywhere)
+// adds the deferred excitation entries and applies V[normal] = saved[k]
+// for the deferred Mur entries; the two lists touch disjoint cells.  The
+// host builds the lists; a box of Mur faces defers its edges.
+//
+// After the H sweep (k_post_h): NF2FF sampling of the step, the probe
+// gather and the next step's Mur pre pass.  All three only read the
+// fields (the pre pass reads E, which nothing writes between the H sweep
+// and the next E sweep), so they run side by side on disjoint workgroups.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "nf2ff.hpp"
+#include "sparse.hpp"
+
+namespace efs {
+
+constexpr int kPostThreads = 256;
+
+template <class R>
+struct PostE {
+    // excitation (k_excitation's arguments)
+    const float* signal;
+    uint32_t len, period;
+    const uint32_t *delay, *gidx;
+    const float* amp;
+    uint32_t exc_n, numTS;
+    const uint8_t* exc_defer;   // 1: not in phase A (deferred or dropped)
+    const uint32_t* exc_b;      // deferred excitation entries
+    uint32_t exc_nb;
+    // Mur (device indices)
+    const float* coeff;
+    const uint32_t *normal, *shifted;
+    R* saved;
+    uint32_t mur_n;
+    const uint8_t* mur_defer;   // 1: phase B applies saved[k]
+    const uint2* mur_b;         // deferred Mur entries: (normal cell, k)
+    uint32_t mur_nb;
+};
+
+template <class FV>
+__global__ __launch_bounds__(kPostThreads) void k_post_e(FV volt, PostE<typename FV::R> q) {
+    typedef typename FV::R R;
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < q.mur_n) {
+        const R s = fma1((R)q.coeff[i], volt.ld(q.shifted[i]), q.saved[i]);
+        if (q.mur_defer[i]) q.saved[i] = s;
+        else volt.st(q.normal[i], s);
+    } else if (i - q.mur_n < q.exc_n) {
+        const uint32_t k = i - q.mur_n;
+        if (!q.exc_defer[k]) excitation_one(volt, q.signal, q.len, q.period, q.delay, q.amp, q.gidx, k, q.numTS);
+    }
+}
+
+template <class FV>
+__global__ __launch_bounds__(kPostThreads) void k_post_e_b(FV volt, PostE<typename FV::R> q) {
+    const uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j < q.exc_nb) {
+        excitation_one(volt, q.signal, q.len, q.period, q.delay, q.amp, q.gidx, q.exc_b[j], q.numTS);
+    } else if (j - q.exc_nb < q.mur_nb) {
+        const uint2 e = q.mur_b[j - q.exc_nb];
+        volt.st(e.x, q.saved[e.y]);
+    }
+}
+
+template <class FV>
+struct PostH {
+    typedef typename FV::R R;
+    // NF2FF sampling (k_nf2ff_sample's arguments), workgroups [0, nb_sample)
+    const uint32_t* pts;
+    const float* el;
+    Nf2ffGeom g;
+    R* row_out;
+    NfCapture cap;
+    uint32_t nb_sample;
+    // probe gather, the next nb_gather workgroups
+    const uint32_t* gidx;
+    float* gout;
+    uint32_t gn, nb_gather;
+    // the next step's Mur pre pass, the rest
+    const R* vflux;
+    const float* coeff;
+    const uint32_t *normal_src, *shifted_src;
+    R* saved;
+    uint32_t mur_n;
+};
+
+template <class FV>
+__global__ void k_post_h(FV volt, FV curr, PostH<FV> q) {
+    uint32_t b = blockIdx.x;
+    if (b < q.nb_sample) {  // the first workgroups: the same XCD spread as k_nf2ff_sample
+        const uint32_t p = xcd_remap(b, q.nb_sample) * blockDim.x + threadIdx.x;
+        nf2ff_sample_point(volt, curr, q.pts, q.el, q.g, q.row_out, q.cap, p);
+        return;
+    }
+    b -= q.nb_sample;
+    if (b < q.nb_gather) {
+        const uint32_t k = b * blockDim.x + threadIdx.x;
+        if (k < q.gn) gather_one(volt, curr, q.gidx, q.gout, k);
+        return;
+    }
+    b -= q.nb_gather;
+    const uint32_t k = b * blockDim.x + threadIdx.x;
+    if (k < q.mur_n) mur_pre_one(volt, q.vflux, q.coeff, q.normal_src, q.shifted_src, q.saved, k);
+}
+
+}  // namespace efs

@@ -82,13 +82,11 @@ __device__ __forceinline__ void nf_point(uint32_t ix, uint32_t iy, uint32_t iz, 
 // One thread per surface point.  Points on a captured z plane (see
 // NfCapture) read the slabs the H sweep wrote; the others read the fields.
 template <class FV>
-__global__ void k_nf2ff_sample(FV volt, FV curr,
-                               const uint32_t* __restrict__ pts, const float* __restrict__ el, Nf2ffGeom g,
-                               typename FV::R* __restrict__ row_out /* buf + slot*rows */, NfCapture cap) {
+__device__ __forceinline__ void nf2ff_sample_point(const FV& volt, const FV& curr, const uint32_t* __restrict__ pts,
+                                                   const float* __restrict__ el, const Nf2ffGeom& g,
+                                                   typename FV::R* __restrict__ row_out, const NfCapture& cap,
+                                                   uint32_t p) {
     typedef typename FV::R R;
-    // points are in grid order; consecutive workgroups share cells, so they
-    // stay on one XCD (own L2)
-    const uint32_t p = xcd_remap(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
     if (p >= g.P) return;
     const uint32_t ix = pts[3 * p], iy = pts[3 * p + 1], iz = pts[3 * p + 2];
     R o[6];
@@ -119,6 +117,16 @@ __global__ void k_nf2ff_sample(FV volt, FV curr,
     row_out[RE + p] = o[3];
     row_out[RE + P + p] = o[4];
     row_out[RE + 2 * P + p] = o[5];
+}
+
+template <class FV>
+__global__ void k_nf2ff_sample(FV volt, FV curr,
+                               const uint32_t* __restrict__ pts, const float* __restrict__ el, Nf2ffGeom g,
+                               typename FV::R* __restrict__ row_out /* buf + slot*rows */, NfCapture cap) {
+    // points are in grid order; consecutive workgroups share cells, so they
+    // stay on one XCD (own L2)
+    const uint32_t p = xcd_remap(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
+    nf2ff_sample_point(volt, curr, pts, el, g, row_out, cap, p);
 }
 
 // FP64 twiddles for k in [0,kused): tw[(h*K + k)*NC + 2f + {0: re, 1: im}],

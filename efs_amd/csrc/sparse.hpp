@@ -32,18 +32,26 @@ __device__ __forceinline__ typename FV::R read_alias(const FV& field, const type
 // apply_excitation (src/webgpu-fdtd.mjs:161-183 semantics: clamp at 0,
 // optional period, past-the-end reads signal[0]).
 template <class FV>
-__global__ void k_excitation(FV volt, const float* __restrict__ signal, uint32_t len,
-                             uint32_t period, const uint32_t* __restrict__ delay,
-                             const float* __restrict__ amp, const uint32_t* __restrict__ gidx, uint32_t n,
-                             uint32_t numTS) {
+__device__ __forceinline__ void excitation_one(const FV& volt, const float* __restrict__ signal, uint32_t len,
+                                               uint32_t period, const uint32_t* __restrict__ delay,
+                                               const float* __restrict__ amp, const uint32_t* __restrict__ gidx,
+                                               uint32_t k, uint32_t numTS) {
     typedef typename FV::R R;
-    const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
-    if (k >= n) return;
     int64_t e = (int64_t)numTS - (int64_t)delay[k];
     if (e < 0) e = 0;
     if (period > 0) e %= (int64_t)period;
     if (e >= (int64_t)len) e = 0;
     volt.add(gidx[k], (R)amp[k] * (R)signal[e]);
+}
+
+template <class FV>
+__global__ void k_excitation(FV volt, const float* __restrict__ signal, uint32_t len,
+                             uint32_t period, const uint32_t* __restrict__ delay,
+                             const float* __restrict__ amp, const uint32_t* __restrict__ gidx, uint32_t n,
+                             uint32_t numTS) {
+    const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    excitation_one(volt, signal, len, period, delay, amp, gidx, k, numTS);
 }
 
 // Mur ABC (src/webgpu-fdtd.mjs:992-1023): per entry
@@ -57,13 +65,21 @@ __global__ void k_excitation(FV volt, const float* __restrict__ signal, uint32_t
 // earlier, at its place in the step, from `k_mur_post_some`, and `finish`
 // then only applies them (flag bit).  Explicit FMAs fix the rounding.
 template <class FV>
+__device__ __forceinline__ void mur_pre_one(const FV& volt, const typename FV::R* __restrict__ vflux,
+                                            const float* __restrict__ coeff, const uint32_t* __restrict__ normal_src,
+                                            const uint32_t* __restrict__ shifted_src,
+                                            typename FV::R* __restrict__ saved, uint32_t k) {
+    typedef typename FV::R R;
+    saved[k] = fma1(-(R)coeff[k], read_alias(volt, vflux, normal_src[k]), read_alias(volt, vflux, shifted_src[k]));
+}
+
+template <class FV>
 __global__ void k_mur_pre(FV volt, const typename FV::R* __restrict__ vflux,
                           const float* __restrict__ coeff, const uint32_t* __restrict__ normal_src,
                           const uint32_t* __restrict__ shifted_src, typename FV::R* __restrict__ saved, uint32_t n) {
-    typedef typename FV::R R;
     const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= n) return;
-    saved[k] = fma1(-(R)coeff[k], read_alias(volt, vflux, normal_src[k]), read_alias(volt, vflux, shifted_src[k]));
+    mur_pre_one(volt, vflux, coeff, normal_src, shifted_src, saved, k);
 }
 
 template <class FV>
@@ -230,12 +246,18 @@ __global__ void k_pml_pad_sync(FieldView field, float* __restrict__ flux, const 
 
 // Probe gather into out[2*k], out[2*k+1] (src/webgpu-engine.mjs:2593-2665).
 template <class FV>
-__global__ void k_gather(FV volt, FV curr, const uint32_t* __restrict__ idx, float* __restrict__ out, uint32_t n) {
-    const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
-    if (k >= n) return;
+__device__ __forceinline__ void gather_one(const FV& volt, const FV& curr, const uint32_t* __restrict__ idx,
+                                           float* __restrict__ out, uint32_t k) {
     const uint32_t j = idx[k];
     out[2 * k] = (float)volt.ld(j);
     out[2 * k + 1] = (float)curr.ld(j);
+}
+
+template <class FV>
+__global__ void k_gather(FV volt, FV curr, const uint32_t* __restrict__ idx, float* __restrict__ out, uint32_t n) {
+    const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    gather_one(volt, curr, idx, out, k);
 }
 
 }  // namespace efs
