@@ -632,6 +632,8 @@ int rebuild_aliases(efs_ctx* c) {
     return EFS_OK;
 }
 
+inline size_t rw(const efs_ctx* c);
+
 int rebuild_ade(efs_ctx* c) {
     c->ade_n = (uint32_t)c->h_ade_gidx.size();
     if (c->ade_n) {
@@ -644,10 +646,11 @@ int rebuild_ade(efs_ctx* c) {
         c->ade_i_live = false;
         for (int k = 3; k < 6; ++k)
             for (float v : c->h_ade_c[k]) c->ade_i_live |= v != 0.0f;
-        TRY(dzero(c, c->ade_va, c->ade_n));
-        TRY(dzero(c, c->ade_vl, c->ade_n));
-        TRY(dzero(c, c->ade_ia, c->ade_n));
-        TRY(dzero(c, c->ade_il, c->ade_n));
+        // the state in the arithmetic type (double in the FP64 storage form)
+        TRY(dzero(c, c->ade_va, c->ade_n * rw(c)));
+        TRY(dzero(c, c->ade_vl, c->ade_n * rw(c)));
+        TRY(dzero(c, c->ade_ia, c->ade_n * rw(c)));
+        TRY(dzero(c, c->ade_il, c->ade_n * rw(c)));
     }
     c->ade_dirty = false;
     c->alias_dirty = true;
@@ -953,6 +956,30 @@ void launch_excitation(efs_ctx* c, FV v, uint32_t t) {
                        c->exc_len, c->exc_period, c->exc_delay, c->exc_amp, c->exc_gidx, c->exc_n, t);
 }
 
+// ADE state update / write-back and TFSF injection on the voltage (curr = 0)
+// or current (curr = 1) side.
+template <class FV>
+void launch_ade_update(efs_ctx* c, FV f, int curr) {
+    hipLaunchKernelGGL(k_ade_update<FV>, dim3(nblk(c->ade_n, c->pt)), dim3(c->pt), 0, c->stream, f,
+                       rp<FV>(curr ? c->pml_cflux : c->pml_vflux), curr ? c->ade_src_i : c->ade_src_v, c->ade_lor,
+                       c->ade_c[3 * curr], c->ade_c[3 * curr + 1], c->ade_c[3 * curr + 2],
+                       rp<FV>(curr ? c->ade_ia : c->ade_va), rp<FV>(curr ? c->ade_il : c->ade_vl), c->ade_n);
+}
+
+template <class FV>
+void launch_ade_apply(efs_ctx* c, FV f, int curr) {
+    hipLaunchKernelGGL(k_ade_apply<FV>, dim3(nblk(c->ade_n, c->pt)), dim3(c->pt), 0, c->stream, f, c->ade_gidx,
+                       rp<FV>(curr ? c->ade_ia : c->ade_va), c->ade_n);
+}
+
+template <class FV>
+void launch_tfsf(efs_ctx* c, FV f, int curr, uint32_t t) {
+    hipLaunchKernelGGL(k_tfsf<FV>, dim3(nblk(curr ? c->tfsf_ni : c->tfsf_nv, c->pt)), dim3(c->pt), 0, c->stream, f,
+                       c->tfsf_signal, c->tfsf_len, c->tfsf_period, curr ? c->ti_idx : c->tv_idx,
+                       curr ? c->ti_dint : c->tv_dint, curr ? c->ti_dfrac : c->tv_dfrac, curr ? c->ti_amp : c->tv_amp,
+                       curr ? c->tfsf_ni : c->tfsf_nv, t);
+}
+
 // Mur post + apply and the excitation: k_post_e, then k_post_e_b for the
 // deferred entries.
 template <class FV>
@@ -1136,9 +1163,8 @@ int launch_step(efs_ctx* c, bool next_follows) {
         c->ss_current++;
     }
     if (c->ade_n) {
-        hipLaunchKernelGGL(k_ade_update, dim3(nblk(c->ade_n, c->pt)), dim3(c->pt), 0, c->stream, fv(c, 0),
-                           c->pml_vflux, c->ade_src_v, c->ade_lor, c->ade_c[0], c->ade_c[1], c->ade_c[2],
-                           c->ade_va, c->ade_vl, c->ade_n);
+        if (c->dbl) launch_ade_update(c, fv64(c, 0), 0);
+        else launch_ade_update(c, fv(c, 0), 0);
     }
     if (c->mur_n && !c->mur_pre_ahead) {
         if (c->dbl) launch_mur_pre(c, fv64(c, 0));
@@ -1189,9 +1215,8 @@ int launch_step(efs_ctx* c, bool next_follows) {
     }
     // post-voltage / apply-to-voltages
     if (c->tfsf_nv) {
-        hipLaunchKernelGGL(k_tfsf, dim3(nblk(c->tfsf_nv, c->pt)), dim3(c->pt), 0, c->stream, fv(c, 0),
-                           c->tfsf_signal, c->tfsf_len, c->tfsf_period, c->tv_idx, c->tv_dint, c->tv_dfrac,
-                           c->tv_amp, c->tfsf_nv, t);
+        if (c->dbl) launch_tfsf(c, fv64(c, 0), 0, t);
+        else launch_tfsf(c, fv(c, 0), 0, t);
     }
     for (const PbcAxis& a : c->pbc_e)  // after TFSF-V, before Mur post (src/webgpu-engine.mjs:2461)
         hipLaunchKernelGGL(k_pbc, dim3(nblk((size_t)a.b0 * a.b1, c->pt)), dim3(c->pt), 0, c->stream, fv(c, 0),
@@ -1208,8 +1233,8 @@ int launch_step(efs_ctx* c, bool next_follows) {
             }
         }
         if (c->ade_n && c->ade_apply) {
-            hipLaunchKernelGGL(k_ade_apply, dim3(nblk(c->ade_n, c->pt)), dim3(c->pt), 0, c->stream, fv(c, 0),
-                               c->ade_gidx, c->ade_va, c->ade_n);
+            if (c->dbl) launch_ade_apply(c, fv64(c, 0), 0);
+            else launch_ade_apply(c, fv(c, 0), 0);
         }
         if (c->exc_n) {
             if (c->dbl) launch_excitation(c, fv64(c, 0), t);
@@ -1229,9 +1254,8 @@ int launch_step(efs_ctx* c, bool next_follows) {
                            c->pml_vflux, c->pad_hv_g, c->pad_hv_f, c->pad_hv_n);
     // pre-current
     if (c->ade_n && c->ade_i_live) {
-        hipLaunchKernelGGL(k_ade_update, dim3(nblk(c->ade_n, c->pt)), dim3(c->pt), 0, c->stream, fv(c, 1),
-                           c->pml_cflux, c->ade_src_i, c->ade_lor, c->ade_c[3], c->ade_c[4], c->ade_c[5],
-                           c->ade_ia, c->ade_il, c->ade_n);
+        if (c->dbl) launch_ade_update(c, fv64(c, 1), 1);
+        else launch_ade_update(c, fv(c, 1), 1);
     }
     {
         ProfScope s(c, EFS_PROF_SWEEP_H);
@@ -1273,16 +1297,15 @@ int launch_step(efs_ctx* c, bool next_follows) {
     }
     // post-current
     if (c->tfsf_ni) {
-        hipLaunchKernelGGL(k_tfsf, dim3(nblk(c->tfsf_ni, c->pt)), dim3(c->pt), 0, c->stream, fv(c, 1),
-                           c->tfsf_signal, c->tfsf_len, c->tfsf_period, c->ti_idx, c->ti_dint, c->ti_dfrac,
-                           c->ti_amp, c->tfsf_ni, t);
+        if (c->dbl) launch_tfsf(c, fv64(c, 1), 1, t);
+        else launch_tfsf(c, fv(c, 1), 1, t);
     }
     for (const PbcAxis& a : c->pbc_h)  // after TFSF-I (src/webgpu-engine.mjs:2477)
         hipLaunchKernelGGL(k_pbc, dim3(nblk((size_t)a.b0 * a.b1, c->pt)), dim3(c->pt), 0, c->stream, fv(c, 1),
                            c->pbc_imag, a, c->ny, c->nzp, (uint64_t)c->S);
     if (c->ade_n && c->ade_apply && c->ade_i_live) {
-        hipLaunchKernelGGL(k_ade_apply, dim3(nblk(c->ade_n, c->pt)), dim3(c->pt), 0, c->stream, fv(c, 1),
-                           c->ade_gidx, c->ade_ia, c->ade_n);
+        if (c->dbl) launch_ade_apply(c, fv64(c, 1), 1);
+        else launch_ade_apply(c, fv(c, 1), 1);
     }
     if (c->pad_hi_n)
         hipLaunchKernelGGL(k_pml_pad_sync, dim3(nblk(c->pad_hi_n, c->pt)), dim3(c->pt), 0, c->stream, fv(c, 1),
@@ -1850,7 +1873,7 @@ int efs_cfg_ade_add(efs_ctx* c, uint32_t has_lorentz, uint32_t dir, uint32_t nce
                     const float* i_ext, const float* i_lor) {
     TRY(need_init(c));
     c->pad_dirty = true;
-    TRY(need_not_f64(c, ncells, "ADE"));
+
     if (dir > 2) return fail(c, EFS_ERR_ARG, "ADE direction %u out of range", dir);
     if (ncells && !pos_idx) return fail(c, EFS_ERR_ARG, "ADE pos_idx is required");
     for (uint32_t k = 0; k < ncells; ++k)
@@ -1882,7 +1905,7 @@ int efs_cfg_tfsf(efs_ctx* c, const float* signal, uint32_t len, uint32_t period,
                  const float* i_amp) {
     TRY(need_init(c));
     c->pad_dirty = true;
-    TRY(need_not_f64(c, nv + ni, "TFSF"));
+
     if ((nv || ni) && (!signal || len == 0)) return fail(c, EFS_ERR_ARG, "TFSF signal must not be empty");
     for (uint32_t k = 0; k < nv; ++k)
         if (v_idx[k] >= 3 * c->N) return fail(c, EFS_ERR_ARG, "TFSF voltage point %u out of range", k);

@@ -105,25 +105,30 @@ __global__ void k_mur_finish(FV volt, const float* __restrict__ coeff, const uin
 
 // Lorentz / Drude ADE state update for all (order, direction) entries at
 // once (src/webgpu-fdtd.mjs:405-454).  lorentz[k] selects the Lorentz form.
-__global__ void k_ade_update(FieldView field, const float* __restrict__ flux,
+// The state A, Lr and the flux are in the arithmetic type R (double in the
+// FP64 storage form); the coefficients stay the FP32 values given.
+template <class FV>
+__global__ void k_ade_update(FV field, const typename FV::R* __restrict__ flux,
                              const uint32_t* __restrict__ src, const uint8_t* __restrict__ lorentz,
                              const float* __restrict__ c_int, const float* __restrict__ c_ext,
-                             const float* __restrict__ c_lor, float* __restrict__ A, float* __restrict__ Lr,
-                             uint32_t n) {
+                             const float* __restrict__ c_lor, typename FV::R* __restrict__ A,
+                             typename FV::R* __restrict__ Lr, uint32_t n) {
+    typedef typename FV::R R;
     const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= n) return;
-    const float V = read_alias(field, flux, src[k]);
+    const R V = read_alias(field, flux, src[k]);
     if (lorentz[k]) {
-        const float l = Lr[k] + c_lor[k] * A[k];
+        const R l = Lr[k] + (R)c_lor[k] * A[k];
         Lr[k] = l;
-        A[k] = c_int[k] * A[k] + c_ext[k] * (V - l);
+        A[k] = (R)c_int[k] * A[k] + (R)c_ext[k] * (V - l);
     } else {
-        A[k] = c_int[k] * A[k] + c_ext[k] * V;
+        A[k] = (R)c_int[k] * A[k] + (R)c_ext[k] * V;
     }
 }
 
 // openEMS write-back of the ADE state (optional): field -= A.
-__global__ void k_ade_apply(FieldView field, const uint32_t* __restrict__ gidx, const float* __restrict__ A,
+template <class FV>
+__global__ void k_ade_apply(FV field, const uint32_t* __restrict__ gidx, const typename FV::R* __restrict__ A,
                             uint32_t n) {
     const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= n) return;
@@ -133,19 +138,21 @@ __global__ void k_ade_apply(FieldView field, const uint32_t* __restrict__ gidx, 
 // TFSF injection (src/webgpu-fdtd.mjs:683-739): linear interpolation
 // between signal[d] and signal[d+1]; skipped before the delay and at or
 // past the last sample.
-__global__ void k_tfsf(FieldView field, const float* __restrict__ signal, uint32_t len,
+template <class FV>
+__global__ void k_tfsf(FV field, const float* __restrict__ signal, uint32_t len,
                        uint32_t period, const uint32_t* __restrict__ gidx, const uint32_t* __restrict__ dint,
                        const float* __restrict__ dfrac, const float* __restrict__ amp, uint32_t n,
                        uint32_t numTS) {
+    typedef typename FV::R R;
     const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= n) return;
     int64_t d = (int64_t)numTS - (int64_t)dint[k];
     if (d < 0) return;
     if (period > 0) d %= (int64_t)period;
     if (d >= (int64_t)len - 1) return;
-    const float delta = dfrac[k];
-    const float sig = (1.0f - delta) * signal[d] + delta * signal[d + 1];
-    field.add(gidx[k], amp[k] * sig);
+    const R delta = dfrac[k];
+    const R sig = ((R)1 - delta) * (R)signal[d] + delta * (R)signal[d + 1];
+    field.add(gidx[k], (R)amp[k] * sig);
 }
 
 // Lumped RLC, CPU-engine order: history shift + parallel inductor current

@@ -4,8 +4,9 @@ accumulators in double on the 1-wide sweeps -- against the FP64 oracle (the
 C restatement in double).  Both evaluate the same formulas in double from
 the same FP32 coefficients; they differ by operation order and fused
 multiply-adds only, so the tolerance is 1e-12 relative L2 (measured values
-are printed), six orders below the FP32 engine's 1e-5 bar.  The engine
-refuses the extensions this form does not run."""
+are printed), six orders below the FP32 engine's 1e-5 bar.  ADE and TFSF
+run in this form too; the engine refuses the extensions it does not run
+(RLC, periodic boundaries, steady-state detection)."""
 import numpy as np
 import pytest
 
@@ -100,15 +101,41 @@ def test_fp64_engine_matches_fp64_oracle(oracle_mod, name, seed, dims, kw):
     g.destroy()
 
 
-def test_fp64_refuses_unsupported_extensions():
+@pytest.mark.parametrize("apply", [False, True], ids=["ade", "ade_write_back"])
+def test_fp64_ade_tfsf_match_fp64_oracle(oracle_mod, apply):
+    """ADE (Drude and Lorentz orders; with and without the openEMS
+    write-back) and TFSF on UPML + Mur in FP64 storage, 30 steps: at this
+    length the write-back changes the fields by ~3 %, so the 1e-12 bar
+    checks the ADE arithmetic (later the random coefficients grow the
+    fields past 1e18)."""
     from efs_amd import engine
     engine.load_library()
-    case = random_case(64, (12, 12, 12), ade=True, tfsf=True, rlc=True, pbc=[(0, 0.0)])
+    case = random_case(65, (18, 16, 20), pml_width=3, ade=True, tfsf=True, mur=True)
+    say("case fp64 ade/tfsf, write-back", apply)
     g = engine.HIPEngine()
     assert g.initGPU()
     g.init(case["dims"], case["coeffs"], storage="fp64")
-    for fn, arg in ((g.configureLorentzADE, case["ade"]), (g.configureTFSF, case["tfsf"]),
-                    (g.configureRLC, case["rlc"]), (g.configurePBC, case["pbc"])):
+    configure(g, case, ade_apply=apply)
+    o = oracle_mod.Oracle(case["dims"], case["coeffs"], fp64=True)
+    configure(o, case, ade_apply=apply)
+    g.iterate(30)
+    o.iterate(30)
+    gf, of = g.getFields(fp64=True), o.getFields()
+    for k in ("volt", "curr"):
+        err = rel_l2(gf[k], of[k])
+        say(k, err)
+        assert np.all(np.isfinite(gf[k])) and err <= TOL64
+    g.destroy()
+
+
+def test_fp64_refuses_unsupported_extensions():
+    from efs_amd import engine
+    engine.load_library()
+    case = random_case(64, (12, 12, 12), rlc=True, pbc=[(0, 0.0)])
+    g = engine.HIPEngine()
+    assert g.initGPU()
+    g.init(case["dims"], case["coeffs"], storage="fp64")
+    for fn, arg in ((g.configureRLC, case["rlc"]), (g.configurePBC, case["pbc"])):
         with pytest.raises(Exception, match="FP64"):
             fn(arg)
     g.destroy()
