@@ -277,13 +277,18 @@ def run_batch(args):
     t_build = time.perf_counter()
     b, ws, _ = build_patch_batch(n_inst, storage, K + W + 64, progress=True)
     t_build = time.perf_counter() - t_build
-    shape = b.tune(steps=8)
+    shape = b.tune(steps=32)
     info = b.info()
     cells = info["cells"]
     indexed = [b.coefIndexed(i) for i in range(len(ws))]
     elapsed = time_batch(b, W, K)
     value = cells * K / elapsed / 1e6
-    # sweep durations (direct launches with events)
+    # sweep durations (direct launches with events); the persistent form
+    # has no sweep launches of its own: its sweeps are timed in the launch
+    # form at the same block count
+    persistent = shape["mode"] == "persistent"
+    if persistent:
+        b.setShape("launch", 256, 4, shape["blocks"])
     b.profile(True)
     b.iterate(args.profile_steps)
     p = b.profileRead()

@@ -89,11 +89,29 @@ class BatchEngine:
         self._check(self.lib.efs_batch_finalize(self.h, int(sampleInterval), int(maxSamples)))
         self._final = True
 
-    def tune(self, steps=8):
-        """Autotune the block table and occupancy; leaves the batch reset."""
+    def tune(self, steps=32):
+        """Autotune the launch shape (form, column tile width, occupancy,
+        block count) on whole steps; leaves the batch reset.  Returns
+        shape()."""
         b, w = C.c_uint32(), C.c_uint32()
         self._check(self.lib.efs_batch_tune(self.h, int(steps), C.byref(b), C.byref(w)))
-        return {"blocks": b.value, "waves": w.value}
+        return self.shape()
+
+    MODES = {"launch": 0, "persistent": 1}
+
+    def setShape(self, mode="launch", tile_threads=256, waves=4, blocks=2048):
+        """Force a launch shape: mode "launch" (one launch per phase, graph
+        replays) or "persistent" (one cooperative launch per up to 64
+        steps, grid barriers between phases; 256-lane tiles, 3 waves per
+        SIMD).  Fields and the step count are kept."""
+        self._check(self.lib.efs_batch_set_shape(self.h, self.MODES[mode], int(tile_threads), int(waves),
+                                                 int(blocks)))
+
+    def shape(self):
+        m, t, w, b, g = C.c_int(), C.c_uint32(), C.c_uint32(), C.c_uint32(), C.c_uint32()
+        self._check(self.lib.efs_batch_get_shape(self.h, C.byref(m), C.byref(t), C.byref(w), C.byref(b), C.byref(g)))
+        mode = {v: k for k, v in self.MODES.items()}[m.value]
+        return {"mode": mode, "tile_threads": t.value, "waves": w.value, "blocks": b.value, "grid": g.value}
 
     # -- running -------------------------------------------------------------
     def reset(self):

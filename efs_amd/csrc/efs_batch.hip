@@ -37,6 +37,7 @@
 #include <vector>
 
 #include "../../include/efs_batch.h"
+#include "grid_barrier.hpp"
 #include "sweep.hpp"
 
 namespace efs {
@@ -70,12 +71,13 @@ constexpr int kRangeLo = 6, kRangeHi = 12, kRangeTarget = 10, kRangeSibling = 14
 
 // ------------------------------------------------------------------ sweeps
 
-template <bool HF, typename T>
-__device__ __forceinline__ void bsweep_body(const Inst* __restrict__ insts, const Block* __restrict__ tab,
-                                            uint32_t nblocks, const float* __restrict__ fscale) {
-    const Block d = tab[xcd_remap(blockIdx.x, nblocks)];
+// One block of the table: the instance's column tile (TPB lanes of 4 z
+// cells) marched over [x0, x1).  COH: the scales are read with coherent
+// loads (the persistent kernel rewrites them between phases).
+template <bool HF, typename T, int TPB, bool COH = false>
+__device__ __forceinline__ void bsweep_block(const Inst* __restrict__ insts, const Block d, const float* fscale) {
     const Inst I = insts[d.inst];
-    const uint32_t c0 = (d.tile * kSweepThreads + threadIdx.x) * 4u;
+    const uint32_t c0 = (d.tile * TPB + threadIdx.x) * 4u;
     if (c0 >= I.cols || d.x0 >= d.x1) return;
     SweepGeom g{};
     g.nx = I.nx;
@@ -90,7 +92,7 @@ __device__ __forceinline__ void bsweep_body(const Inst* __restrict__ insts, cons
     w.x1 = d.x1;
     w.region = -1;
     const PmlTable pml{};  // no UPML in the batched form: column mask 0, never read
-    const Scales sc = load_scales<T>(fscale + 6 * d.inst, HF);
+    const Scales sc = COH ? load_scales_coherent<T>(fscale + 6 * d.inst, HF) : load_scales<T>(fscale + 6 * d.inst, HF);
     // the coefficient form is per instance (uniform over the workgroup)
     if (HF) {
         if (I.tabn_h) {
@@ -120,23 +122,14 @@ __device__ __forceinline__ void bsweep_body(const Inst* __restrict__ insts, cons
     }
 }
 
-// One kernel per (side, storage, occupancy); the autotuner picks W.
-#define EFS_BSWEEP(NAME, HF, T, W)                                                                       \
-    __global__ __launch_bounds__(kSweepThreads) __attribute__((amdgpu_waves_per_eu(W, W))) void NAME( \
-        const Inst* __restrict__ insts, const Block* __restrict__ tab, uint32_t nblocks,                  \
-        const float* __restrict__ fscale) {                                                               \
-        bsweep_body<HF, T>(insts, tab, nblocks, fscale);                                                  \
-    }
-
-EFS_BSWEEP(bsweep_e_w4, false, float, 4)
-EFS_BSWEEP(bsweep_h_w4, true, float, 4)
-EFS_BSWEEP(bsweep_e_w5, false, float, 5)
-EFS_BSWEEP(bsweep_h_w5, true, float, 5)
-EFS_BSWEEP(bsweep_e_w4_h, false, f16, 4)
-EFS_BSWEEP(bsweep_h_w4_h, true, f16, 4)
-EFS_BSWEEP(bsweep_e_w5_h, false, f16, 5)
-EFS_BSWEEP(bsweep_h_w5_h, true, f16, 5)
-#undef EFS_BSWEEP
+// One kernel per (side, storage, occupancy W, tile width TPB); the
+// autotuner picks W and TPB.
+template <bool HF, typename T, int W, int TPB>
+__global__ __launch_bounds__(TPB) __attribute__((amdgpu_waves_per_eu(W, W))) void bsweep(
+    const Inst* __restrict__ insts, const Block* __restrict__ tab, uint32_t nblocks,
+    const float* __restrict__ fscale) {
+    bsweep_block<HF, T, TPB>(insts, tab[xcd_remap(blockIdx.x, nblocks)], fscale);
+}
 
 // ------------------------------------------------------------ point kernels
 
@@ -158,21 +151,19 @@ struct MurPt {
     uint32_t inst;        // instance | kMurEarly
 };
 
-__global__ void k_bmur_pre(const Inst* __restrict__ insts, const float* __restrict__ fscale, uint32_t half,
-                           const MurPt* __restrict__ pts, float* __restrict__ saved, uint32_t n) {
-    const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
-    if (k >= n) return;
+// Point work, one item per call; the point kernels and the persistent
+// kernel share these bodies.
+__device__ __forceinline__ void bmur_pre_one(const Inst* __restrict__ insts, const float* fscale, uint32_t half,
+                                             const MurPt* __restrict__ pts, float* saved, uint32_t k) {
     const MurPt p = pts[k];
     const uint32_t q = p.inst & ~kMurEarly;
     const FieldView v = bview(insts[q], false, fscale, q, half);
     saved[k] = __builtin_fmaf(-p.coeff, v.ld(p.nidx), v.ld(p.sidx));
 }
 
-__global__ void k_bmur_post_some(const Inst* __restrict__ insts, const float* __restrict__ fscale, uint32_t half,
-                                 const MurPt* __restrict__ pts, const uint32_t* __restrict__ list,
-                                 float* __restrict__ saved, uint32_t n) {
-    const uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
-    if (j >= n) return;
+__device__ __forceinline__ void bmur_post_one(const Inst* __restrict__ insts, const float* fscale, uint32_t half,
+                                              const MurPt* __restrict__ pts, const uint32_t* __restrict__ list,
+                                              float* saved, uint32_t j) {
     const uint32_t k = list[j];
     const MurPt p = pts[k];
     const uint32_t q = p.inst & ~kMurEarly;
@@ -180,10 +171,8 @@ __global__ void k_bmur_post_some(const Inst* __restrict__ insts, const float* __
     saved[k] = __builtin_fmaf(p.coeff, v.ld(p.sidx), saved[k]);
 }
 
-__global__ void k_bmur_finish(const Inst* __restrict__ insts, const float* __restrict__ fscale, uint32_t half,
-                              const MurPt* __restrict__ pts, const float* __restrict__ saved, uint32_t n) {
-    const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
-    if (k >= n) return;
+__device__ __forceinline__ void bmur_finish_one(const Inst* __restrict__ insts, const float* fscale, uint32_t half,
+                                                const MurPt* __restrict__ pts, const float* saved, uint32_t k) {
     const MurPt p = pts[k];
     const uint32_t q = p.inst & ~kMurEarly;
     const FieldView v = bview(insts[q], false, fscale, q, half);
@@ -200,17 +189,59 @@ struct ExcPoint {
     uint32_t pad;
 };
 
-__global__ void k_bexcitation(const Inst* __restrict__ insts, const float* __restrict__ fscale, uint32_t half,
-                              const ExcPoint* __restrict__ pts, const float* __restrict__ sig, uint32_t n,
-                              const uint32_t* __restrict__ ts) {
-    const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
-    if (k >= n) return;
+__device__ __forceinline__ void bexc_one(const Inst* __restrict__ insts, const float* fscale, uint32_t half,
+                                         const ExcPoint* __restrict__ pts, const float* __restrict__ sig, uint32_t k,
+                                         uint32_t t) {
     const ExcPoint p = pts[k];
-    int64_t e = (int64_t)*ts - (int64_t)p.delay;
+    int64_t e = (int64_t)t - (int64_t)p.delay;
     if (e < 0) e = 0;
     if (p.period > 0) e %= (int64_t)p.period;
     if (e >= (int64_t)p.len) e = 0;
     bview(insts[p.inst], false, fscale, p.inst, half).add(p.gidx, p.amp * sig[p.off + e]);
+}
+
+// Gather entry k at step t (the step's fields; sample slot from the
+// interval, as the single-grid buffered gather).
+__device__ __forceinline__ void bgather_one(const Inst* __restrict__ insts, const float* fscale, uint32_t half,
+                                            const uint32_t* __restrict__ inst, const uint32_t* __restrict__ idx,
+                                            uint32_t n, float* ring, uint32_t s, uint32_t k) {
+    float* out = ring + (size_t)s * 2 * n;
+    const Inst& I = insts[inst[k]];
+    out[2 * k] = bview(I, false, fscale, inst[k], half).ld(idx[k]);
+    out[2 * k + 1] = bview(I, true, fscale, inst[k], half).ld(idx[k]);
+}
+
+__device__ __forceinline__ bool gather_slot(uint32_t t, uint32_t n, uint32_t interval, uint32_t max_samples,
+                                            uint32_t& s) {
+    if (!n || !interval || (t + 1) % interval != 0) return false;
+    s = (t + 1) / interval - 1;
+    return s < max_samples;
+}
+
+__global__ void k_bmur_pre(const Inst* __restrict__ insts, const float* __restrict__ fscale, uint32_t half,
+                           const MurPt* __restrict__ pts, float* __restrict__ saved, uint32_t n) {
+    const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < n) bmur_pre_one(insts, fscale, half, pts, saved, k);
+}
+
+__global__ void k_bmur_post_some(const Inst* __restrict__ insts, const float* __restrict__ fscale, uint32_t half,
+                                 const MurPt* __restrict__ pts, const uint32_t* __restrict__ list,
+                                 float* __restrict__ saved, uint32_t n) {
+    const uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j < n) bmur_post_one(insts, fscale, half, pts, list, saved, j);
+}
+
+__global__ void k_bmur_finish(const Inst* __restrict__ insts, const float* __restrict__ fscale, uint32_t half,
+                              const MurPt* __restrict__ pts, const float* __restrict__ saved, uint32_t n) {
+    const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < n) bmur_finish_one(insts, fscale, half, pts, saved, k);
+}
+
+__global__ void k_bexcitation(const Inst* __restrict__ insts, const float* __restrict__ fscale, uint32_t half,
+                              const ExcPoint* __restrict__ pts, const float* __restrict__ sig, uint32_t n,
+                              const uint32_t* __restrict__ ts) {
+    const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < n) bexc_one(insts, fscale, half, pts, sig, k, *ts);
 }
 
 // Buffered gather of every instance's probes (one workgroup), then the step
@@ -223,17 +254,9 @@ __global__ __launch_bounds__(kPt) void k_bgather_tick(const Inst* __restrict__ i
                                                      uint32_t max_samples, uint32_t* __restrict__ ts) {
     const uint32_t t = *ts;
     __syncthreads();
-    if (n && interval && (t + 1) % interval == 0) {
-        const uint32_t s = (t + 1) / interval - 1;
-        if (s < max_samples) {
-            float* out = ring + (size_t)s * 2 * n;
-            for (uint32_t k = threadIdx.x; k < n; k += blockDim.x) {
-                const Inst& I = insts[inst[k]];
-                out[2 * k] = bview(I, false, fscale, inst[k], half).ld(idx[k]);
-                out[2 * k + 1] = bview(I, true, fscale, inst[k], half).ld(idx[k]);
-            }
-        }
-    }
+    uint32_t s;
+    if (gather_slot(t, n, interval, max_samples, s))
+        for (uint32_t k = threadIdx.x; k < n; k += blockDim.x) bgather_one(insts, fscale, half, inst, idx, n, ring, s, k);
     __syncthreads();
     if (threadIdx.x == 0) ts[0] = t + 1;
 }
@@ -246,10 +269,13 @@ struct Seg {
     uint32_t slot;  // inst * 6 + k
 };
 
-__global__ __launch_bounds__(256) void k_babsmax(const Seg* __restrict__ segs, unsigned int* __restrict__ amax) {
-    const Seg s = segs[blockIdx.y];
+// Largest |stored| of chunk c (of nchunks) of segment s, one atomic per
+// workgroup (every thread of the workgroup calls it).
+__device__ __forceinline__ void babsmax_item(const Seg* __restrict__ segs, unsigned int* amax, uint32_t seg,
+                                             uint32_t c, uint32_t nchunks) {
+    const Seg s = segs[seg];
     float m = 0.f;
-    for (uint32_t i = 4 * (blockIdx.x * blockDim.x + threadIdx.x); i < s.n; i += 4 * gridDim.x * blockDim.x) {
+    for (uint32_t i = 4 * (c * blockDim.x + threadIdx.x); i < s.n; i += 4 * nchunks * blockDim.x) {
         const v4 v = __builtin_convertvector(*reinterpret_cast<const h4*>(s.p + i), v4);
         m = fmaxf(m, fmaxf(fmaxf(fabsf(v.x), fabsf(v.y)), fmaxf(fabsf(v.z), fabsf(v.w))));
         if (v.x != v.x || v.y != v.y || v.z != v.z || v.w != v.w) m = __builtin_nanf("");
@@ -264,20 +290,19 @@ __global__ __launch_bounds__(256) void k_babsmax(const Seg* __restrict__ segs, u
     if ((threadIdx.x & 63u) == 0) s_m[threadIdx.x >> 6] = m;
     __syncthreads();
     if (threadIdx.x == 0) {
-        for (int w = 1; w < 4; ++w) {
+        for (uint32_t w = 1; w < blockDim.x / 64u; ++w) {
             const float o = s_m[w];
             m = (o != o || m != m) ? __builtin_nanf("") : fmaxf(m, o);
         }
         atomicMax(amax + s.slot, __float_as_uint(m));
     }
+    __syncthreads();
 }
 
-__global__ void k_bplan(unsigned int* __restrict__ amax, float* __restrict__ fscale,
-                        const float* __restrict__ smin, float* __restrict__ factor,
-                        const uint32_t* __restrict__ src_end, uint32_t ninst, const uint32_t* __restrict__ ts) {
-    const uint32_t q = blockIdx.x * blockDim.x + threadIdx.x;
-    if (q >= ninst) return;
-    const uint32_t active = (uint64_t)*ts + 1 < (uint64_t)src_end[q];
+__device__ __forceinline__ void bplan_one(unsigned int* amax, float* fscale, const float* __restrict__ smin,
+                                          float* factor, const uint32_t* __restrict__ src_end, uint32_t q,
+                                          uint32_t t) {
+    const uint32_t active = (uint64_t)t + 1 < (uint64_t)src_end[q];
     unsigned int* am = amax + 6 * q;
     float* sc = fscale + 6 * q;
     const float* sm = smin + 6 * q;
@@ -306,15 +331,140 @@ __global__ void k_bplan(unsigned int* __restrict__ amax, float* __restrict__ fsc
     }
 }
 
-__global__ __launch_bounds__(256) void k_bapply(const Seg* __restrict__ segs, const float* __restrict__ factor) {
-    const Seg s = segs[blockIdx.y];
+__device__ __forceinline__ void bapply_item(const Seg* __restrict__ segs, const float* factor, uint32_t seg,
+                                            uint32_t c, uint32_t nchunks) {
+    const Seg s = segs[seg];
     const float f = factor[s.slot];
     if (f == 1.0f) return;
-    for (uint32_t i = 4 * (blockIdx.x * blockDim.x + threadIdx.x); i < s.n; i += 4 * gridDim.x * blockDim.x) {
+    for (uint32_t i = 4 * (c * blockDim.x + threadIdx.x); i < s.n; i += 4 * nchunks * blockDim.x) {
         h4* q = reinterpret_cast<h4*>(s.p + i);
         *q = __builtin_convertvector(clamp16(__builtin_convertvector(*q, v4) * f), h4);
     }
 }
+
+__global__ __launch_bounds__(256) void k_babsmax(const Seg* __restrict__ segs, unsigned int* __restrict__ amax) {
+    babsmax_item(segs, amax, blockIdx.y, blockIdx.x, gridDim.x);
+}
+
+__global__ void k_bplan(unsigned int* __restrict__ amax, float* __restrict__ fscale,
+                        const float* __restrict__ smin, float* __restrict__ factor,
+                        const uint32_t* __restrict__ src_end, uint32_t ninst, const uint32_t* __restrict__ ts) {
+    const uint32_t q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q < ninst) bplan_one(amax, fscale, smin, factor, src_end, q, *ts);
+}
+
+__global__ __launch_bounds__(256) void k_bapply(const Seg* __restrict__ segs, const float* __restrict__ factor) {
+    bapply_item(segs, factor, blockIdx.y, blockIdx.x, gridDim.x);
+}
+
+// ------------------------------------------------------- persistent kernel
+
+// Everything one step reads and writes, for the persistent kernel.
+struct PArgs {
+    const Inst* insts;
+    const Block* tab;
+    uint32_t nblocks;
+    uint32_t half;
+    float* fscale;
+    // Mur
+    const MurPt* mur;
+    const uint32_t* mur_early;
+    float* mur_saved;
+    uint32_t mur_n, mur_ne;
+    // excitation
+    const ExcPoint* exc;
+    const float* sig;
+    uint32_t exc_n;
+    // gather
+    uint32_t g_n;
+    const uint32_t* g_inst;
+    const uint32_t* g_idx;
+    float* ring;
+    uint32_t g_interval, g_max;
+    // FP16 range control
+    const Seg* segs;
+    uint32_t nsegs, seg_chunks;
+    unsigned int* amax;
+    const float* smin;
+    float* factor;
+    const uint32_t* src_end;
+    uint32_t ninst;
+    // step counter, barrier, steps of this launch
+    uint32_t* ts;
+    GridBarState* bar;
+    uint32_t flat;
+    uint32_t nsteps;
+};
+
+// nsteps steps of every instance in one launch: the phases of a step
+// (Mur pre, E sweep, early Mur post, excitation, Mur post + apply, H sweep,
+// [FP16 range control], gather) run in the order of the launch form, each
+// over the whole grid of resident workgroups, separated by grid barriers.
+// The next step's Mur pre shares the gather's phase (both only read the
+// fields).  Sweep blocks are dealt to workgroups round-robin through the
+// same XCD-contiguous table order as the launch form.  Results are the
+// launch form's bits.  On a barrier timeout every workgroup leaves; the
+// error word tells the host.
+template <typename T, int W>
+__global__ __launch_bounds__(kSweepThreads) __attribute__((amdgpu_waves_per_eu(W, W))) void bpersist(PArgs a) {
+    GridBar gb;
+    if (!gb.init(a.bar, a.flat != 0)) return;
+    const uint32_t G = gridDim.x, wg = blockIdx.x;
+    const uint32_t gtid = wg * blockDim.x + threadIdx.x, gthreads = G * blockDim.x;
+    const uint32_t t0 = __hip_atomic_load(a.ts, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    const uint32_t half = a.half;
+    for (uint32_t k = gtid; k < a.mur_n; k += gthreads) bmur_pre_one(a.insts, a.fscale, half, a.mur, a.mur_saved, k);
+    if (!gb.sync()) return;
+    for (uint32_t s = 0; s < a.nsteps; ++s) {
+        const uint32_t t = t0 + s;
+        for (uint32_t j = wg; j < a.nblocks; j += G)
+            bsweep_block<false, T, kSweepThreads, true>(a.insts, a.tab[xcd_remap(j, a.nblocks)], a.fscale);
+        if (!gb.sync()) return;
+        if (a.mur_ne) {
+            for (uint32_t j = gtid; j < a.mur_ne; j += gthreads)
+                bmur_post_one(a.insts, a.fscale, half, a.mur, a.mur_early, a.mur_saved, j);
+            if (!gb.sync()) return;
+        }
+        if (a.exc_n) {
+            for (uint32_t k = gtid; k < a.exc_n; k += gthreads) bexc_one(a.insts, a.fscale, half, a.exc, a.sig, k, t);
+            if (!gb.sync()) return;
+        }
+        if (a.mur_n) {
+            for (uint32_t k = gtid; k < a.mur_n; k += gthreads)
+                bmur_finish_one(a.insts, a.fscale, half, a.mur, a.mur_saved, k);
+            if (!gb.sync()) return;
+        }
+        for (uint32_t j = wg; j < a.nblocks; j += G)
+            bsweep_block<true, T, kSweepThreads, true>(a.insts, a.tab[xcd_remap(j, a.nblocks)], a.fscale);
+        if (!gb.sync()) return;
+        if (half && (t + 1) % kGraphSteps == 0) {
+            const uint32_t items = a.nsegs * a.seg_chunks;
+            for (uint32_t it = wg; it < items; it += G) babsmax_item(a.segs, a.amax, it / a.seg_chunks, it % a.seg_chunks, a.seg_chunks);
+            if (!gb.sync()) return;
+            for (uint32_t q = gtid; q < a.ninst; q += gthreads) bplan_one(a.amax, a.fscale, a.smin, a.factor, a.src_end, q, t);
+            if (!gb.sync()) return;
+            for (uint32_t it = wg; it < items; it += G) bapply_item(a.segs, a.factor, it / a.seg_chunks, it % a.seg_chunks, a.seg_chunks);
+            if (!gb.sync()) return;
+        }
+        uint32_t slot;
+        if (gather_slot(t, a.g_n, a.g_interval, a.g_max, slot))
+            for (uint32_t k = gtid; k < a.g_n; k += gthreads)
+                bgather_one(a.insts, a.fscale, half, a.g_inst, a.g_idx, a.g_n, a.ring, slot, k);
+        if (s + 1 < a.nsteps) {
+            for (uint32_t k = gtid; k < a.mur_n; k += gthreads)
+                bmur_pre_one(a.insts, a.fscale, half, a.mur, a.mur_saved, k);
+            if (!gb.sync()) return;
+        }
+    }
+    if (gtid == 0) __hip_atomic_store(a.ts, t0 + a.nsteps, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Occupancy of the persistent kernel: both marches and the point work live
+// in one kernel, which needs up to 168 registers without spilling (at 4
+// waves per SIMD, 128, it spills).
+constexpr uint32_t kPersistWaves = 3;
+template __global__ void bpersist<float, kPersistWaves>(PArgs);
+template __global__ void bpersist<f16, kPersistWaves>(PArgs);
 
 }  // namespace batch
 }  // namespace efs
@@ -361,7 +511,14 @@ struct efs_batch {
     Inst* d_inst = nullptr;
     Block* d_tab = nullptr;
     uint32_t nblocks = 0;
+    uint32_t want = 2048;  // blocks asked of the planner
     uint32_t waves = 4;
+    uint32_t tpb = kSweepThreads;  // lanes per column tile (launch form)
+    int mode = EFS_BATCH_LAUNCH;
+    uint32_t pgrid = 0;            // workgroups of the persistent kernel
+    GridBarState* d_bar = nullptr;
+    bool bar_flat = false;
+    bool persist_ran = false;
     uint32_t* d_ts = nullptr;
     float* d_fscale = nullptr;
     float* d_smin = nullptr;
@@ -464,9 +621,9 @@ uint32_t btr(const HostInst& h, uint64_t g) {
     return (uint32_t)(n * h.S + xy * h.nzp + z);
 }
 
-// Equal-cost block table: every (instance, column tile) is cut along x into
-// chunks of about total/want cells, ordered by instance, then x.
-std::vector<Block> plan_blocks(const efs_batch* b, uint32_t want) {
+// Equal-cost block table: every (instance, column tile of tpb lanes) is cut
+// along x into chunks of about total/want cells, ordered by instance, then x.
+std::vector<Block> plan_blocks(const efs_batch* b, uint32_t want, uint32_t tpb) {
     double total = 0.0;
     for (const HostInst& h : b->inst) total += (double)h.nx * h.ny * h.nzp;
     const double target = total / std::max(1u, want);
@@ -474,7 +631,7 @@ std::vector<Block> plan_blocks(const efs_batch* b, uint32_t want) {
     for (uint32_t q = 0; q < (uint32_t)b->inst.size(); ++q) {
         const HostInst& h = b->inst[q];
         const uint64_t cols = (uint64_t)h.ny * h.nzp;
-        const uint32_t tcells = 4 * kSweepThreads;
+        const uint32_t tcells = 4 * tpb;
         const uint32_t T = (uint32_t)((cols + tcells - 1) / tcells);
         std::vector<Block> mine;
         for (uint32_t t = 0; t < T; ++t) {
@@ -501,7 +658,10 @@ void drop_graph(efs_batch* b);
 // dropped (build_graph captures a new one).
 int set_blocks(efs_batch* b, uint32_t want) {
     drop_graph(b);
-    std::vector<Block> tab = plan_blocks(b, want);
+    b->want = want;
+    // the persistent kernel runs 256-lane tiles
+    const uint32_t tpb = b->mode == EFS_BATCH_PERSISTENT ? (uint32_t)kSweepThreads : b->tpb;
+    std::vector<Block> tab = plan_blocks(b, want, tpb);
     BHIP(b, hipStreamSynchronize(b->stream));
     BTRY(bupload(b, b->d_tab, tab.data(), tab.size()));
     b->nblocks = (uint32_t)tab.size();
@@ -519,16 +679,30 @@ hipEvent_t take_event(efs_batch* b) {
     return e;
 }
 
+typedef void (*SweepFn)(const Inst*, const Block*, uint32_t, const float*);
+
+template <bool HF, typename T>
+SweepFn sweep_kernel_t(uint32_t waves, uint32_t tpb) {
+    if (tpb == 128) return waves == 5 ? bsweep<HF, T, 5, 128> : bsweep<HF, T, 4, 128>;
+    if (tpb == 512) return waves == 5 ? bsweep<HF, T, 5, 512> : bsweep<HF, T, 4, 512>;
+    return waves == 5 ? bsweep<HF, T, 5, 256> : bsweep<HF, T, 4, 256>;
+}
+
+SweepFn sweep_kernel(bool hf, bool half, uint32_t waves, uint32_t tpb) {
+    if (hf) return half ? sweep_kernel_t<true, f16>(waves, tpb) : sweep_kernel_t<true, float>(waves, tpb);
+    return half ? sweep_kernel_t<false, f16>(waves, tpb) : sweep_kernel_t<false, float>(waves, tpb);
+}
+
+bool valid_tpb(uint32_t t) { return t == 128 || t == 256 || t == 512; }
+
 void launch_sweep(efs_batch* b, bool hf) {
     hipEvent_t ea = nullptr;
     if (b->prof) {
         ea = take_event(b);
         if (ea) hipEventRecord(ea, b->stream);
     }
-    auto k = hf ? (b->half ? (b->waves == 5 ? bsweep_h_w5_h : bsweep_h_w4_h) : (b->waves == 5 ? bsweep_h_w5 : bsweep_h_w4))
-                : (b->half ? (b->waves == 5 ? bsweep_e_w5_h : bsweep_e_w4_h) : (b->waves == 5 ? bsweep_e_w5 : bsweep_e_w4));
-    hipLaunchKernelGGL(k, dim3(b->nblocks), dim3(kSweepThreads), 0, b->stream, (const Inst*)b->d_inst,
-                       (const Block*)b->d_tab, b->nblocks, (const float*)b->d_fscale);
+    hipLaunchKernelGGL(sweep_kernel(hf, b->half, b->waves, b->tpb), dim3(b->nblocks), dim3(b->tpb), 0, b->stream,
+                       (const Inst*)b->d_inst, (const Block*)b->d_tab, b->nblocks, (const float*)b->d_fscale);
     if (b->prof && ea) {
         hipEvent_t eb = take_event(b);
         if (eb) {
@@ -585,6 +759,7 @@ void drop_graph(efs_batch* b) {
 // replay starting at a multiple of kGraphSteps is exactly those steps.
 int build_graph(efs_batch* b) {
     drop_graph(b);
+    if (b->mode == EFS_BATCH_PERSISTENT) return EFS_OK;  // one launch already covers many steps
     BHIP(b, hipStreamSynchronize(b->stream));
     hipGraph_t g = nullptr;
     BHIP(b, hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
@@ -604,8 +779,94 @@ int build_graph(efs_batch* b) {
     return EFS_OK;
 }
 
+// Steps of one persistent launch at most (the host regains control between
+// launches; the FP16 range control runs inside at every 32nd step).
+constexpr uint32_t kPersistSteps = 64;
+
+// Workgroups of the persistent kernel: every one must be resident at once
+// (occupancy of the kernel x compute units).
+int persist_grid(efs_batch* b) {
+    const void* f = b->half ? reinterpret_cast<const void*>(bpersist<f16, kPersistWaves>)
+                            : reinterpret_cast<const void*>(bpersist<float, kPersistWaves>);
+    int per_cu = 0, cus = 0, coop = 0;
+    BHIP(b, hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, f, kSweepThreads, 0));
+    BHIP(b, hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, b->device));
+    BHIP(b, hipDeviceGetAttribute(&coop, hipDeviceAttributeCooperativeLaunch, b->device));
+    if (!coop) return bfail(b, EFS_ERR_HIP, "device %d has no cooperative launch", b->device);
+    if (per_cu <= 0) return bfail(b, EFS_ERR_HIP, "persistent kernel does not fit a compute unit");
+    b->pgrid = (uint32_t)(per_cu * cus) & ~7u;  // a multiple of the 8 XCDs
+    if (b->pgrid == 0) return bfail(b, EFS_ERR_HIP, "persistent grid is empty");
+    if (!b->d_bar) BTRY(balloc(b, b->d_bar, 1));
+    return EFS_OK;
+}
+
+int launch_persistent(efs_batch* b, uint32_t nsteps) {
+    PArgs a{};
+    a.insts = b->d_inst;
+    a.tab = b->d_tab;
+    a.nblocks = b->nblocks;
+    a.half = b->half ? 1u : 0u;
+    a.fscale = b->d_fscale;
+    a.mur = b->d_mur;
+    a.mur_early = b->d_mur_early;
+    a.mur_saved = b->d_mur_saved;
+    a.mur_n = b->mur_n;
+    a.mur_ne = b->mur_ne;
+    a.exc = b->d_exc;
+    a.sig = b->d_sig;
+    a.exc_n = b->exc_n;
+    a.g_n = b->g_n;
+    a.g_inst = b->d_g_inst;
+    a.g_idx = b->d_g_idx;
+    a.ring = b->d_ring;
+    a.g_interval = b->g_interval;
+    a.g_max = b->g_max;
+    a.segs = b->d_segs;
+    a.nsegs = b->half ? b->nsegs : 0u;
+    a.seg_chunks = b->seg_chunks;
+    a.amax = b->d_amax;
+    a.smin = b->d_smin;
+    a.factor = b->d_factor;
+    a.src_end = b->d_src_end;
+    a.ninst = (uint32_t)b->inst.size();
+    a.ts = b->d_ts;
+    a.bar = b->d_bar;
+    a.flat = b->bar_flat ? 1u : 0u;
+    a.nsteps = nsteps;
+    hipLaunchKernelGGL(k_gridbar_clear, dim3(1), dim3(256), 0, b->stream, reinterpret_cast<uint32_t*>(b->d_bar));
+    BHIP(b, hipGetLastError());
+    void* args[] = {&a};
+    const void* f = b->half ? reinterpret_cast<const void*>(bpersist<f16, kPersistWaves>)
+                            : reinterpret_cast<const void*>(bpersist<float, kPersistWaves>);
+    BHIP(b, hipLaunchCooperativeKernel(f, dim3(b->pgrid), dim3(kSweepThreads), args, 0, b->stream));
+    b->persist_ran = true;
+    return EFS_OK;
+}
+
+// After persistent launches: a barrier that timed out (grid not resident)
+// is reported here (the stream is synchronised).
+int check_persistent(efs_batch* b) {
+    if (!b->persist_ran) return EFS_OK;
+    BHIP(b, hipStreamSynchronize(b->stream));
+    uint32_t err = 0;
+    BHIP(b, hipMemcpy(&err, reinterpret_cast<const uint32_t*>(b->d_bar) + offsetof(GridBarState, err) / 4,
+                      sizeof err, hipMemcpyDeviceToHost));
+    b->persist_ran = false;
+    if (err) return bfail(b, EFS_ERR_HIP, "persistent kernel: a grid barrier timed out (workgroups not co-resident)");
+    return EFS_OK;
+}
+
 int run_steps(efs_batch* b, uint32_t n) {
     uint32_t done = 0;
+    if (b->mode == EFS_BATCH_PERSISTENT) {
+        while (done < n) {
+            const uint32_t k = std::min(kPersistSteps, n - done);
+            BTRY(launch_persistent(b, k));
+            b->numTS += k;
+            done += k;
+        }
+        return EFS_OK;
+    }
     while (done < n) {
         const uint32_t t = b->numTS;
         if (b->graph_ok && !b->prof && t % kGraphSteps == 0 && n - done >= kGraphSteps) {
@@ -748,6 +1009,12 @@ int efs_batch_create(efs_batch** out, int device) {
     if (device < 0 || device >= n) return bfail(nullptr, EFS_ERR_ARG, "device %d out of range (%d devices)", device, n);
     efs_batch* b = new efs_batch();
     b->device = device;
+    {
+        // grid-barrier form of the persistent kernel: "flat" = a write-back
+        // per workgroup (the reference form, for measurement)
+        const char* e = std::getenv("EFS_BATCH_BARRIER");
+        b->bar_flat = e && std::strcmp(e, "flat") == 0;
+    }
     if (hipSetDevice(device) != hipSuccess || hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess) {
         delete b;
         return bfail(nullptr, EFS_ERR_HIP, "stream creation failed");
@@ -790,6 +1057,7 @@ void efs_batch_destroy(efs_batch* b) {
     bfree(b->d_g_inst);
     bfree(b->d_g_idx);
     bfree(b->d_ring);
+    bfree(b->d_bar);
     for (auto& p : b->pend) {
         hipEventDestroy(p.a);
         hipEventDestroy(p.b);
@@ -1146,42 +1414,108 @@ int efs_batch_reset(efs_batch* b) {
 int efs_batch_tune(efs_batch* b, uint32_t steps, uint32_t* blocks_out, uint32_t* waves_out) {
     BTRY(need_final(b));
     BHIP(b, hipSetDevice(b->device));
-    if (steps == 0) steps = 8;
-    static const uint32_t kWant[] = {512, 1024, 2048, 4096, 8192};
-    static const uint32_t kWaves[] = {4, 5};
+    steps = std::max<uint32_t>(kGraphSteps, (steps + kGraphSteps - 1) / kGraphSteps * kGraphSteps);
+    struct Cand {
+        int mode;
+        uint32_t tpb, waves, want;
+    };
+    std::vector<Cand> cand;
+    for (uint32_t tpb : {128u, 256u, 512u})
+        for (uint32_t v : {4u, 5u})
+            for (uint32_t want : {1024u, 2048u, 4096u, 8192u}) cand.push_back({EFS_BATCH_LAUNCH, tpb, v, want});
+    const int mode0 = b->mode;
+    b->mode = EFS_BATCH_PERSISTENT;
+    const bool can_persist = persist_grid(b) == EFS_OK;
+    b->mode = mode0;
+    b->err.clear();
+    if (can_persist)
+        for (uint32_t m : {1u, 2u, 4u, 8u}) cand.push_back({EFS_BATCH_PERSISTENT, 256u, kPersistWaves, m * b->pgrid});
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    BHIP(b, hipEventCreate(&e0));
+    BHIP(b, hipEventCreate(&e1));
     double best = 1e300;
-    uint32_t best_w = 2048, best_v = 4;
+    Cand pick{EFS_BATCH_LAUNCH, kSweepThreads, 4u, 2048u};
     const bool prof = b->prof;
-    for (uint32_t v : kWaves)
-        for (uint32_t want : kWant) {
-            b->waves = v;
-            BTRY(set_blocks(b, want));
-            BTRY(zero_state(b));
-            b->prof = true;
-            b->prof_ms[0] = b->prof_ms[1] = 0.0;
-            b->prof_n[0] = b->prof_n[1] = 0;
-            BTRY(run_steps(b, 2));  // warm
-            BTRY(drain(b));
-            b->prof_ms[0] = b->prof_ms[1] = 0.0;
-            b->prof_n[0] = b->prof_n[1] = 0;
-            BTRY(run_steps(b, steps));
-            BTRY(drain(b));
-            const double t = b->prof_ms[0] + b->prof_ms[1];
-            if (t < best) {
-                best = t;
-                best_w = want;
-                best_v = v;
-            }
+    b->prof = false;
+    int rc = EFS_OK;
+    for (const Cand& c : cand) {
+        b->mode = c.mode;
+        b->tpb = c.tpb;
+        b->waves = c.waves;
+        if ((rc = set_blocks(b, c.want)) || (rc = zero_state(b)) || (rc = build_graph(b))) break;
+        if ((rc = run_steps(b, kGraphSteps))) break;  // warm
+        hipEventRecord(e0, b->stream);
+        if ((rc = run_steps(b, steps))) break;
+        hipEventRecord(e1, b->stream);
+        if (hipEventSynchronize(e1) != hipSuccess) {
+            rc = bfail(b, EFS_ERR_HIP, "tuner: timing failed");
+            break;
         }
+        if (check_persistent(b) != EFS_OK) {  // a shape that cannot run (not resident): skip it
+            b->err.clear();
+            continue;
+        }
+        float ms = 0.f;
+        hipEventElapsedTime(&ms, e0, e1);
+        if (ms < best) {
+            best = ms;
+            pick = c;
+        }
+    }
+    hipEventDestroy(e0);
+    hipEventDestroy(e1);
     b->prof = prof;
+    if (rc) return rc;
     b->prof_ms[0] = b->prof_ms[1] = 0.0;
     b->prof_n[0] = b->prof_n[1] = 0;
-    b->waves = best_v;
-    BTRY(set_blocks(b, best_w));
+    b->mode = pick.mode;
+    b->tpb = pick.tpb;
+    b->waves = pick.waves;
+    BTRY(set_blocks(b, pick.want));
     BTRY(zero_state(b));
     BTRY(build_graph(b));
     if (blocks_out) *blocks_out = b->nblocks;
     if (waves_out) *waves_out = b->waves;
+    return EFS_OK;
+}
+
+int efs_batch_set_shape(efs_batch* b, int mode, uint32_t tile_threads, uint32_t waves, uint32_t blocks) {
+    BTRY(need_final(b));
+    BHIP(b, hipSetDevice(b->device));
+    if (mode != EFS_BATCH_LAUNCH && mode != EFS_BATCH_PERSISTENT) return bfail(b, EFS_ERR_ARG, "unknown mode %d", mode);
+    if (mode == EFS_BATCH_LAUNCH && !valid_tpb(tile_threads))
+        return bfail(b, EFS_ERR_ARG, "tile_threads must be 128, 256 or 512 (got %u)", tile_threads);
+    if (mode == EFS_BATCH_PERSISTENT && tile_threads != kSweepThreads)
+        return bfail(b, EFS_ERR_ARG, "the persistent kernel runs 256-lane tiles (got %u)", tile_threads);
+    if (mode == EFS_BATCH_LAUNCH && waves != 4 && waves != 5)
+        return bfail(b, EFS_ERR_ARG, "waves must be 4 or 5 (got %u)", waves);
+    if (mode == EFS_BATCH_PERSISTENT && waves != kPersistWaves)
+        return bfail(b, EFS_ERR_ARG, "the persistent kernel runs at %u waves per SIMD (got %u)", kPersistWaves, waves);
+    if (blocks == 0) return bfail(b, EFS_ERR_ARG, "blocks must be positive");
+    BTRY(check_persistent(b));
+    const int mode0 = b->mode;
+    b->mode = mode;
+    if (mode == EFS_BATCH_PERSISTENT) {
+        const int rc = persist_grid(b);
+        if (rc) {
+            b->mode = mode0;
+            return rc;
+        }
+    }
+    b->tpb = tile_threads;
+    b->waves = waves;
+    BTRY(set_blocks(b, blocks));
+    return build_graph(b);
+}
+
+int efs_batch_get_shape(efs_batch* b, int* mode, uint32_t* tile_threads, uint32_t* waves, uint32_t* blocks,
+                        uint32_t* grid) {
+    BTRY(need_final(b));
+    if (mode) *mode = b->mode;
+    if (tile_threads) *tile_threads = b->mode == EFS_BATCH_PERSISTENT ? (uint32_t)kSweepThreads : b->tpb;
+    if (waves) *waves = b->waves;
+    if (blocks) *blocks = b->nblocks;
+    if (grid) *grid = b->mode == EFS_BATCH_PERSISTENT ? b->pgrid : b->nblocks;
     return EFS_OK;
 }
 
@@ -1194,7 +1528,7 @@ int efs_batch_iterate(efs_batch* b, uint32_t n) {
 int efs_batch_sync(efs_batch* b) {
     if (!b) return EFS_ERR_STATE;
     BHIP(b, hipStreamSynchronize(b->stream));
-    return EFS_OK;
+    return check_persistent(b);
 }
 
 int efs_batch_num_ts(efs_batch* b, uint32_t* out) {
@@ -1213,6 +1547,7 @@ int efs_batch_read_gather(efs_batch* b, uint32_t id, float* out, uint32_t* num_s
     const uint32_t w = k1 - k0;
     if (w == 0) return EFS_OK;
     BHIP(b, hipSetDevice(b->device));
+    BTRY(check_persistent(b));
     BHIP(b, hipStreamSynchronize(b->stream));
     // one strided copy: sample rows of 2*g_n floats, this instance's columns
     BHIP(b, hipMemcpy2D(out, 2 * w * sizeof(float), b->d_ring + 2 * k0, 2 * (size_t)b->g_n * sizeof(float),
@@ -1238,6 +1573,7 @@ int efs_batch_get_fields(efs_batch* b, uint32_t id, float* volt, float* curr) {
     BTRY(need_inst(b, id));
     if (!volt || !curr) return bfail(b, EFS_ERR_ARG, "output arrays are required");
     BHIP(b, hipSetDevice(b->device));
+    BTRY(check_persistent(b));
     BHIP(b, hipStreamSynchronize(b->stream));
     const HostInst& h = b->inst[id];
     float sc[6] = {1, 1, 1, 1, 1, 1};

@@ -342,6 +342,30 @@ __device__ __forceinline__ Scales load_scales(const float* __restrict__ fscale, 
     return sc;
 }
 
+// The same scales read with device-coherent loads, for a kernel that
+// rewrites them itself between two grid barriers (a uniform plain load may
+// be served by the scalar cache, which a barrier does not invalidate).
+template <typename T>
+__device__ __forceinline__ Scales load_scales_coherent(const float* fscale, bool own_is_curr) {
+    Scales sc;
+#pragma unroll
+    for (int n = 0; n < 3; ++n) {
+        if (Fs<T>::kHalf) {
+            const float* po = fscale + (own_is_curr ? 3 : 0) + n;
+            const float* pt = fscale + (own_is_curr ? 0 : 3) + n;
+            // uniform values: moved to scalar registers
+            sc.own[n] = __builtin_bit_cast(float, __builtin_amdgcn_readfirstlane(__builtin_bit_cast(
+                                                      int, __hip_atomic_load(po, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))));
+            sc.own_inv[n] = 1.0f / sc.own[n];
+            sc.other[n] = __builtin_bit_cast(float, __builtin_amdgcn_readfirstlane(__builtin_bit_cast(
+                                                        int, __hip_atomic_load(pt, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))));
+        } else {
+            sc.own[n] = sc.own_inv[n] = sc.other[n] = 1.0f;
+        }
+    }
+    return sc;
+}
+
 // Explicit fused multiply-adds fix the rounding of every update, so the
 // 1-wide and 4-wide sweeps (and the interior/UPML split) give identical bits:
 //   core  V' = fma(a, V, b*curl)

@@ -69,12 +69,38 @@ int efs_batch_cfg_gather(efs_batch *b, uint32_t id, const uint32_t *idx, uint32_
  * instances (max_samples 0 = no gather).  No instance can be added after. */
 int efs_batch_finalize(efs_batch *b, uint32_t sample_interval, uint32_t max_samples);
 
-/* Autotuner: time `steps` steps for each candidate block-table shape
- * (x-chunk length per tile, hence blocks per launch) and kernel occupancy,
- * keep the fastest.  Changes fields (call before the real run and reset
- * with efs_batch_reset).  *blocks / *waves receive the chosen shape (may be
+/* Autotuner: time `steps` whole steps (rounded up to a multiple of 32, as
+ * they run: graph replays or persistent launches) for each candidate shape
+ * -- launch form: column tile width 128/256/512 lanes x occupancy 4/5
+ * waves per SIMD x 1024..8192 blocks; persistent form: 1, 2, 4 or 8 blocks
+ * per resident workgroup -- and keep the fastest (efs_batch_get_shape
+ * reports it).  Changes fields (call before the real run and reset with
+ * efs_batch_reset).  *blocks / *waves receive the chosen shape (may be
  * NULL). */
 int efs_batch_tune(efs_batch *b, uint32_t steps, uint32_t *blocks, uint32_t *waves);
+
+/* Launch forms of a step (no counterpart in the reference, whose engine
+ * submits one compute pass per kernel):
+ *   EFS_BATCH_LAUNCH      one launch per half-step and per point pass over
+ *                         every instance, 32-step blocks replayed from a
+ *                         captured HIP graph;
+ *   EFS_BATCH_PERSISTENT  one cooperative launch of resident workgroups
+ *                         runs up to 64 steps, the phases of a step
+ *                         separated by grid barriers.
+ * Both give the same bits. */
+#define EFS_BATCH_LAUNCH 0
+#define EFS_BATCH_PERSISTENT 1
+
+/* Set the launch shape after efs_batch_finalize: the form, the column tile
+ * width (lanes of 4 z cells per workgroup: 128, 256 or 512; the persistent
+ * kernel runs 256), the occupancy the sweep kernels are compiled for (4 or
+ * 5 waves per SIMD; persistent: 3) and the blocks asked of the equal-cost
+ * planner.  Fields and step count are kept. */
+int efs_batch_set_shape(efs_batch *b, int mode, uint32_t tile_threads, uint32_t waves, uint32_t blocks);
+/* The shape in use; *grid = workgroups per launch (persistent: resident
+ * workgroups; launch form: the block count). */
+int efs_batch_get_shape(efs_batch *b, int *mode, uint32_t *tile_threads, uint32_t *waves, uint32_t *blocks,
+                        uint32_t *grid);
 
 /* Zero every field, Mur state and gather ring and set numTS to 0. */
 int efs_batch_reset(efs_batch *b);

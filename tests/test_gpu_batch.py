@@ -9,6 +9,11 @@
   Mur point kernels (FP32 and FP16).
 - The autotuner leaves a reset batch whose results do not depend on the
   chosen block table.
+- Every launch shape gives the same bits: the persistent kernel (grid
+  barriers between the phases, both barrier forms) and the launch form at
+  each column tile width, in FP32 and FP16 (the FP16 range control runs
+  inside the persistent launch), over iterate calls that start and end off
+  the 32- and 64-step boundaries.
 - FP16 storage against the FP64 oracle per instance: relative L2 of the
   port traces and fields within the stated tolerances.
 - Misuse fails with a message and launches nothing.
@@ -98,7 +103,8 @@ def test_tune_then_same_results(patches):
     t, _ = make_batch(ws, steps=64)
     shape = t.tune(steps=4)
     say("tuned", shape, t.info())
-    assert shape["waves"] in (4, 5) and shape["blocks"] > 0
+    assert shape["mode"] in ("launch", "persistent") and shape["blocks"] > 0
+    assert shape["waves"] in ((4, 5) if shape["mode"] == "launch" else (3,))
     assert t.numTS == 0
     z = t.getFields(0)
     assert not np.any(z["volt"]) and not np.any(z["curr"])
@@ -110,6 +116,55 @@ def test_tune_then_same_results(patches):
             np.testing.assert_array_equal(fa[k], ft[k])
     a.destroy()
     t.destroy()
+
+
+SHAPES = [("launch", 128, 4, 1024), ("launch", 512, 5, 4096), ("persistent", 256, 3, 768),
+          ("persistent", 256, 3, 6000), ("persistent-flat", 256, 3, 2048)]
+
+
+@pytest.mark.parametrize("storage", ["fp32", "fp16"])
+def test_launch_shapes_same_bits(patches, storage, monkeypatch):
+    """The reference shape (launch form, 256-lane tiles, 4 waves, 2048
+    blocks) against other tile widths and the persistent kernel, over
+    iterate calls of 37 and 70 steps (off the graph and persistent-launch
+    boundaries; FP16 crosses three range checks)."""
+    ws = patches[:3]
+    calls = (37, 70)
+    ref, _ = make_batch(ws, storage=storage, steps=sum(calls))
+    say("reference shape", ref.shape())
+    for n in calls:
+        ref.iterate(n)
+    ref.sync()
+    want = [ref.getFields(i) for i in range(len(ws))]
+    want_g = [ref.readProbeGatherBuffered(i)["data"] for i in range(len(ws))]
+    ref.destroy()
+    for mode, tpb, waves, blocks in SHAPES:
+        if mode == "persistent-flat":
+            monkeypatch.setenv("EFS_BATCH_BARRIER", "flat")
+        else:
+            monkeypatch.delenv("EFS_BATCH_BARRIER", raising=False)
+        b, _ = make_batch(ws, storage=storage, steps=sum(calls))
+        b.setShape(mode.split("-")[0], tpb, waves, blocks)
+        say("shape", mode, b.shape())
+        for n in calls:
+            b.iterate(n)
+        b.sync()
+        assert b.numTS == sum(calls)
+        for i in range(len(ws)):
+            got = b.getFields(i)
+            for k in ("volt", "curr"):
+                np.testing.assert_array_equal(got[k], want[i][k], err_msg=f"{mode} {tpb} instance {i} {k}")
+            np.testing.assert_array_equal(b.readProbeGatherBuffered(i)["data"], want_g[i])
+        b.destroy()
+
+
+def test_shape_rejects_invalid(patches):
+    b, _ = make_batch(patches[:1], steps=4)
+    for args in (("launch", 384, 4, 100), ("launch", 256, 3, 100), ("persistent", 512, 3, 100),
+                 ("persistent", 256, 4, 100), ("launch", 256, 4, 0)):
+        with pytest.raises(RuntimeError):
+            b.setShape(*args)
+    b.destroy()
 
 
 @pytest.mark.parametrize("storage", ["fp32", "fp16"])
