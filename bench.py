@@ -288,7 +288,7 @@ def run_batch(args):
     # form at the same block count
     persistent = shape["mode"] == "persistent"
     if persistent:
-        b.setShape("launch", 256, 4, shape["blocks"])
+        b.setShape("launch", 256, 4, 4, shape["blocks"])
     b.profile(True)
     b.iterate(args.profile_steps)
     p = b.profileRead()

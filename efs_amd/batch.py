@@ -99,19 +99,22 @@ class BatchEngine:
 
     MODES = {"launch": 0, "persistent": 1}
 
-    def setShape(self, mode="launch", tile_threads=256, waves=4, blocks=2048):
+    def setShape(self, mode="launch", tile_threads=256, z_cells=4, waves=4, blocks=2048):
         """Force a launch shape: mode "launch" (one launch per phase, graph
-        replays) or "persistent" (one cooperative launch per up to 64
-        steps, grid barriers between phases; 256-lane tiles, 3 waves per
-        SIMD).  Fields and the step count are kept."""
-        self._check(self.lib.efs_batch_set_shape(self.h, self.MODES[mode], int(tile_threads), int(waves),
-                                                 int(blocks)))
+        replays; lanes of 4 z cells, or 8 in FP16 storage) or "persistent"
+        (one cooperative launch per up to 64 steps, grid barriers between
+        phases; 256-lane tiles of 4-cell lanes, 3 waves per SIMD).  Fields
+        and the step count are kept."""
+        self._check(self.lib.efs_batch_set_shape(self.h, self.MODES[mode], int(tile_threads), int(z_cells),
+                                                 int(waves), int(blocks)))
 
     def shape(self):
-        m, t, w, b, g = C.c_int(), C.c_uint32(), C.c_uint32(), C.c_uint32(), C.c_uint32()
-        self._check(self.lib.efs_batch_get_shape(self.h, C.byref(m), C.byref(t), C.byref(w), C.byref(b), C.byref(g)))
+        m, t, z, w, b, g = C.c_int(), C.c_uint32(), C.c_uint32(), C.c_uint32(), C.c_uint32(), C.c_uint32()
+        self._check(self.lib.efs_batch_get_shape(self.h, C.byref(m), C.byref(t), C.byref(z), C.byref(w), C.byref(b),
+                                                 C.byref(g)))
         mode = {v: k for k, v in self.MODES.items()}[m.value]
-        return {"mode": mode, "tile_threads": t.value, "waves": w.value, "blocks": b.value, "grid": g.value}
+        return {"mode": mode, "tile_threads": t.value, "z_cells": z.value, "waves": w.value, "blocks": b.value,
+                "grid": g.value}
 
     # -- running -------------------------------------------------------------
     def reset(self):

@@ -39,6 +39,7 @@
 #include "../../include/efs_batch.h"
 #include "grid_barrier.hpp"
 #include "sweep.hpp"
+#include "sweep_z8.hpp"
 
 namespace efs {
 namespace batch {
@@ -71,13 +72,13 @@ constexpr int kRangeLo = 6, kRangeHi = 12, kRangeTarget = 10, kRangeSibling = 14
 
 // ------------------------------------------------------------------ sweeps
 
-// One block of the table: the instance's column tile (TPB lanes of 4 z
-// cells) marched over [x0, x1).  COH: the scales are read with coherent
-// loads (the persistent kernel rewrites them between phases).
-template <bool HF, typename T, int TPB, bool COH = false>
+// One block of the table: the instance's column tile (TPB lanes of Z = 4
+// or 8 z cells) marched over [x0, x1).  COH: the scales are read with
+// coherent loads (the persistent kernel rewrites them between phases).
+template <bool HF, typename T, int TPB, bool COH = false, int Z = 4>
 __device__ __forceinline__ void bsweep_block(const Inst* __restrict__ insts, const Block d, const float* fscale) {
     const Inst I = insts[d.inst];
-    const uint32_t c0 = (d.tile * TPB + threadIdx.x) * 4u;
+    const uint32_t c0 = (d.tile * TPB + threadIdx.x) * (uint32_t)Z;
     if (c0 >= I.cols || d.x0 >= d.x1) return;
     SweepGeom g{};
     g.nx = I.nx;
@@ -93,6 +94,27 @@ __device__ __forceinline__ void bsweep_block(const Inst* __restrict__ insts, con
     w.region = -1;
     const PmlTable pml{};  // no UPML in the batched form: column mask 0, never read
     const Scales sc = COH ? load_scales_coherent<T>(fscale + 6 * d.inst, HF) : load_scales<T>(fscale + 6 * d.inst, HF);
+    if (Z == 8) {
+        // rows padded to a multiple of 8 (checked by the host)
+        if (HF) {
+            if (I.tabn_h) {
+                const Coef cf{nullptr, nullptr, nullptr, I.th, nullptr, nullptr, I.ih, I.tabn_h};
+                h8_march<kCoefIdx16, T>(static_cast<T*>(I.curr), static_cast<const T*>(I.volt), cf, g, w, sc);
+            } else {
+                const Coef cf{I.ii, I.iv, nullptr, nullptr, nullptr, nullptr, nullptr, 0u};
+                h8_march<kCoefF32, T>(static_cast<T*>(I.curr), static_cast<const T*>(I.volt), cf, g, w, sc);
+            }
+        } else {
+            if (I.tabn_e) {
+                const Coef cf{nullptr, nullptr, nullptr, I.te, nullptr, nullptr, I.ie, I.tabn_e};
+                e8_march<kCoefIdx16, T>(static_cast<T*>(I.volt), static_cast<const T*>(I.curr), cf, g, w, sc, I.zm);
+            } else {
+                const Coef cf{I.vv, I.vi, nullptr, nullptr, nullptr, nullptr, nullptr, 0u};
+                e8_march<kCoefF32, T>(static_cast<T*>(I.volt), static_cast<const T*>(I.curr), cf, g, w, sc, I.zm);
+            }
+        }
+        return;
+    }
     // the coefficient form is per instance (uniform over the workgroup)
     if (HF) {
         if (I.tabn_h) {
@@ -122,13 +144,13 @@ __device__ __forceinline__ void bsweep_block(const Inst* __restrict__ insts, con
     }
 }
 
-// One kernel per (side, storage, occupancy W, tile width TPB); the
-// autotuner picks W and TPB.
-template <bool HF, typename T, int W, int TPB>
+// One kernel per (side, storage, occupancy W, tile width TPB, z cells per
+// lane Z); the autotuner picks W, TPB and Z.
+template <bool HF, typename T, int W, int TPB, int Z>
 __global__ __launch_bounds__(TPB) __attribute__((amdgpu_waves_per_eu(W, W))) void bsweep(
     const Inst* __restrict__ insts, const Block* __restrict__ tab, uint32_t nblocks,
     const float* __restrict__ fscale) {
-    bsweep_block<HF, T, TPB>(insts, tab[xcd_remap(blockIdx.x, nblocks)], fscale);
+    bsweep_block<HF, T, TPB, false, Z>(insts, tab[xcd_remap(blockIdx.x, nblocks)], fscale);
 }
 
 // ------------------------------------------------------------ point kernels
@@ -514,6 +536,8 @@ struct efs_batch {
     uint32_t want = 2048;  // blocks asked of the planner
     uint32_t waves = 4;
     uint32_t tpb = kSweepThreads;  // lanes per column tile (launch form)
+    uint32_t zc = 4;               // z cells per lane (launch form: 4, or 8 when every row is padded to 8)
+    uint32_t zpad = 4;             // rows padded to a multiple of this (FP16: 8)
     int mode = EFS_BATCH_LAUNCH;
     uint32_t pgrid = 0;            // workgroups of the persistent kernel
     GridBarState* d_bar = nullptr;
@@ -623,7 +647,7 @@ uint32_t btr(const HostInst& h, uint64_t g) {
 
 // Equal-cost block table: every (instance, column tile of tpb lanes) is cut
 // along x into chunks of about total/want cells, ordered by instance, then x.
-std::vector<Block> plan_blocks(const efs_batch* b, uint32_t want, uint32_t tpb) {
+std::vector<Block> plan_blocks(const efs_batch* b, uint32_t want, uint32_t tpb, uint32_t zc) {
     double total = 0.0;
     for (const HostInst& h : b->inst) total += (double)h.nx * h.ny * h.nzp;
     const double target = total / std::max(1u, want);
@@ -631,7 +655,7 @@ std::vector<Block> plan_blocks(const efs_batch* b, uint32_t want, uint32_t tpb) 
     for (uint32_t q = 0; q < (uint32_t)b->inst.size(); ++q) {
         const HostInst& h = b->inst[q];
         const uint64_t cols = (uint64_t)h.ny * h.nzp;
-        const uint32_t tcells = 4 * tpb;
+        const uint32_t tcells = zc * tpb;
         const uint32_t T = (uint32_t)((cols + tcells - 1) / tcells);
         std::vector<Block> mine;
         for (uint32_t t = 0; t < T; ++t) {
@@ -660,8 +684,8 @@ int set_blocks(efs_batch* b, uint32_t want) {
     drop_graph(b);
     b->want = want;
     // the persistent kernel runs 256-lane tiles
-    const uint32_t tpb = b->mode == EFS_BATCH_PERSISTENT ? (uint32_t)kSweepThreads : b->tpb;
-    std::vector<Block> tab = plan_blocks(b, want, tpb);
+    const bool pers = b->mode == EFS_BATCH_PERSISTENT;
+    std::vector<Block> tab = plan_blocks(b, want, pers ? (uint32_t)kSweepThreads : b->tpb, pers ? 4u : b->zc);
     BHIP(b, hipStreamSynchronize(b->stream));
     BTRY(bupload(b, b->d_tab, tab.data(), tab.size()));
     b->nblocks = (uint32_t)tab.size();
@@ -682,15 +706,20 @@ hipEvent_t take_event(efs_batch* b) {
 typedef void (*SweepFn)(const Inst*, const Block*, uint32_t, const float*);
 
 template <bool HF, typename T>
-SweepFn sweep_kernel_t(uint32_t waves, uint32_t tpb) {
-    if (tpb == 128) return waves == 5 ? bsweep<HF, T, 5, 128> : bsweep<HF, T, 4, 128>;
-    if (tpb == 512) return waves == 5 ? bsweep<HF, T, 5, 512> : bsweep<HF, T, 4, 512>;
-    return waves == 5 ? bsweep<HF, T, 5, 256> : bsweep<HF, T, 4, 256>;
+SweepFn sweep_kernel_t(uint32_t waves, uint32_t tpb, uint32_t z) {
+    if constexpr (Fs<T>::kHalf) {
+        // 8-cell lanes (FP16 rows padded to 8): 3 waves per SIMD, 147-152
+        // registers (at 4 waves they spill and run 30 % slower)
+        if (z == 8) return tpb == 512 ? bsweep<HF, T, 3, 512, 8> : bsweep<HF, T, 3, 256, 8>;
+    }
+    if (tpb == 128) return waves == 5 ? bsweep<HF, T, 5, 128, 4> : bsweep<HF, T, 4, 128, 4>;
+    if (tpb == 512) return waves == 5 ? bsweep<HF, T, 5, 512, 4> : bsweep<HF, T, 4, 512, 4>;
+    return waves == 5 ? bsweep<HF, T, 5, 256, 4> : bsweep<HF, T, 4, 256, 4>;
 }
 
-SweepFn sweep_kernel(bool hf, bool half, uint32_t waves, uint32_t tpb) {
-    if (hf) return half ? sweep_kernel_t<true, f16>(waves, tpb) : sweep_kernel_t<true, float>(waves, tpb);
-    return half ? sweep_kernel_t<false, f16>(waves, tpb) : sweep_kernel_t<false, float>(waves, tpb);
+SweepFn sweep_kernel(bool hf, bool half, uint32_t waves, uint32_t tpb, uint32_t z) {
+    if (hf) return half ? sweep_kernel_t<true, f16>(waves, tpb, z) : sweep_kernel_t<true, float>(waves, tpb, z);
+    return half ? sweep_kernel_t<false, f16>(waves, tpb, z) : sweep_kernel_t<false, float>(waves, tpb, z);
 }
 
 bool valid_tpb(uint32_t t) { return t == 128 || t == 256 || t == 512; }
@@ -701,7 +730,7 @@ void launch_sweep(efs_batch* b, bool hf) {
         ea = take_event(b);
         if (ea) hipEventRecord(ea, b->stream);
     }
-    hipLaunchKernelGGL(sweep_kernel(hf, b->half, b->waves, b->tpb), dim3(b->nblocks), dim3(b->tpb), 0, b->stream,
+    hipLaunchKernelGGL(sweep_kernel(hf, b->half, b->waves, b->tpb, b->zc), dim3(b->nblocks), dim3(b->tpb), 0, b->stream,
                        (const Inst*)b->d_inst, (const Block*)b->d_tab, b->nblocks, (const float*)b->d_fscale);
     if (b->prof && ea) {
         hipEvent_t eb = take_event(b);
@@ -1075,6 +1104,9 @@ int efs_batch_set_storage(efs_batch* b, int storage) {
         return bfail(b, EFS_ERR_ARG, "unknown storage %d", storage);
     if (!b->inst.empty()) return bfail(b, EFS_ERR_STATE, "set the storage before adding instances");
     b->half = storage == EFS_STORAGE_F16;
+    // FP16 rows are padded to 8 cells, so the sweeps may run 8-cell lanes
+    // (16-byte accesses); FP32 lanes of 4 cells already move 16 bytes
+    b->zpad = b->half ? 8u : 4u;
     return EFS_OK;
 }
 
@@ -1087,7 +1119,7 @@ int efs_batch_add(efs_batch* b, uint32_t nx, uint32_t ny, uint32_t nz, const flo
     h.nx = nx;
     h.ny = ny;
     h.nz = nz;
-    h.nzp = (nz + 3) & ~3u;
+    h.nzp = (nz + b->zpad - 1) / b->zpad * b->zpad;
     h.N = (uint64_t)nx * ny * nz;
     const uint64_t Np = (uint64_t)nx * ny * h.nzp;
     h.S = ((Np + 255) & ~(uint64_t)255) + 1088;
@@ -1417,30 +1449,34 @@ int efs_batch_tune(efs_batch* b, uint32_t steps, uint32_t* blocks_out, uint32_t*
     steps = std::max<uint32_t>(kGraphSteps, (steps + kGraphSteps - 1) / kGraphSteps * kGraphSteps);
     struct Cand {
         int mode;
-        uint32_t tpb, waves, want;
+        uint32_t tpb, zc, waves, want;
     };
     std::vector<Cand> cand;
     for (uint32_t tpb : {128u, 256u, 512u})
         for (uint32_t v : {4u, 5u})
-            for (uint32_t want : {1024u, 2048u, 4096u, 8192u}) cand.push_back({EFS_BATCH_LAUNCH, tpb, v, want});
+            for (uint32_t want : {1024u, 2048u, 4096u, 8192u}) cand.push_back({EFS_BATCH_LAUNCH, tpb, 4u, v, want});
+    if (b->zpad == 8)
+        for (uint32_t tpb : {256u, 512u})
+            for (uint32_t want : {1024u, 2048u, 4096u, 8192u}) cand.push_back({EFS_BATCH_LAUNCH, tpb, 8u, 3u, want});
     const int mode0 = b->mode;
     b->mode = EFS_BATCH_PERSISTENT;
     const bool can_persist = persist_grid(b) == EFS_OK;
     b->mode = mode0;
     b->err.clear();
     if (can_persist)
-        for (uint32_t m : {1u, 2u, 4u, 8u}) cand.push_back({EFS_BATCH_PERSISTENT, 256u, kPersistWaves, m * b->pgrid});
+        for (uint32_t m : {1u, 2u, 4u, 8u}) cand.push_back({EFS_BATCH_PERSISTENT, 256u, 4u, kPersistWaves, m * b->pgrid});
     hipEvent_t e0 = nullptr, e1 = nullptr;
     BHIP(b, hipEventCreate(&e0));
     BHIP(b, hipEventCreate(&e1));
     double best = 1e300;
-    Cand pick{EFS_BATCH_LAUNCH, kSweepThreads, 4u, 2048u};
+    Cand pick{EFS_BATCH_LAUNCH, kSweepThreads, 4u, 4u, 2048u};
     const bool prof = b->prof;
     b->prof = false;
     int rc = EFS_OK;
     for (const Cand& c : cand) {
         b->mode = c.mode;
         b->tpb = c.tpb;
+        b->zc = c.zc;
         b->waves = c.waves;
         if ((rc = set_blocks(b, c.want)) || (rc = zero_state(b)) || (rc = build_graph(b))) break;
         if ((rc = run_steps(b, kGraphSteps))) break;  // warm
@@ -1470,6 +1506,7 @@ int efs_batch_tune(efs_batch* b, uint32_t steps, uint32_t* blocks_out, uint32_t*
     b->prof_n[0] = b->prof_n[1] = 0;
     b->mode = pick.mode;
     b->tpb = pick.tpb;
+    b->zc = pick.zc;
     b->waves = pick.waves;
     BTRY(set_blocks(b, pick.want));
     BTRY(zero_state(b));
@@ -1479,18 +1516,29 @@ int efs_batch_tune(efs_batch* b, uint32_t steps, uint32_t* blocks_out, uint32_t*
     return EFS_OK;
 }
 
-int efs_batch_set_shape(efs_batch* b, int mode, uint32_t tile_threads, uint32_t waves, uint32_t blocks) {
+int efs_batch_set_shape(efs_batch* b, int mode, uint32_t tile_threads, uint32_t z_cells, uint32_t waves,
+                        uint32_t blocks) {
     BTRY(need_final(b));
     BHIP(b, hipSetDevice(b->device));
     if (mode != EFS_BATCH_LAUNCH && mode != EFS_BATCH_PERSISTENT) return bfail(b, EFS_ERR_ARG, "unknown mode %d", mode);
-    if (mode == EFS_BATCH_LAUNCH && !valid_tpb(tile_threads))
-        return bfail(b, EFS_ERR_ARG, "tile_threads must be 128, 256 or 512 (got %u)", tile_threads);
-    if (mode == EFS_BATCH_PERSISTENT && tile_threads != kSweepThreads)
-        return bfail(b, EFS_ERR_ARG, "the persistent kernel runs 256-lane tiles (got %u)", tile_threads);
-    if (mode == EFS_BATCH_LAUNCH && waves != 4 && waves != 5)
-        return bfail(b, EFS_ERR_ARG, "waves must be 4 or 5 (got %u)", waves);
-    if (mode == EFS_BATCH_PERSISTENT && waves != kPersistWaves)
-        return bfail(b, EFS_ERR_ARG, "the persistent kernel runs at %u waves per SIMD (got %u)", kPersistWaves, waves);
+    if (mode == EFS_BATCH_PERSISTENT) {
+        if (tile_threads != kSweepThreads || z_cells != 4 || waves != kPersistWaves)
+            return bfail(b, EFS_ERR_ARG,
+                         "the persistent kernel runs 256-lane tiles of 4-cell lanes at %u waves per SIMD (got %u, %u, %u)",
+                         kPersistWaves, tile_threads, z_cells, waves);
+    } else if (z_cells == 4) {
+        if (!valid_tpb(tile_threads))
+            return bfail(b, EFS_ERR_ARG, "tile_threads must be 128, 256 or 512 (got %u)", tile_threads);
+        if (waves != 4 && waves != 5) return bfail(b, EFS_ERR_ARG, "waves must be 4 or 5 (got %u)", waves);
+    } else if (z_cells == 8) {
+        if (b->zpad != 8)
+            return bfail(b, EFS_ERR_ARG, "8-cell lanes need rows padded to 8 cells (FP16 storage)");
+        if (tile_threads != 256 && tile_threads != 512)
+            return bfail(b, EFS_ERR_ARG, "8-cell lanes run 256- or 512-lane tiles (got %u)", tile_threads);
+        if (waves != 3) return bfail(b, EFS_ERR_ARG, "8-cell lanes run 3 waves per SIMD (got %u)", waves);
+    } else {
+        return bfail(b, EFS_ERR_ARG, "z_cells must be 4 or 8 (got %u)", z_cells);
+    }
     if (blocks == 0) return bfail(b, EFS_ERR_ARG, "blocks must be positive");
     BTRY(check_persistent(b));
     const int mode0 = b->mode;
@@ -1503,19 +1551,22 @@ int efs_batch_set_shape(efs_batch* b, int mode, uint32_t tile_threads, uint32_t 
         }
     }
     b->tpb = tile_threads;
+    b->zc = z_cells;
     b->waves = waves;
     BTRY(set_blocks(b, blocks));
     return build_graph(b);
 }
 
-int efs_batch_get_shape(efs_batch* b, int* mode, uint32_t* tile_threads, uint32_t* waves, uint32_t* blocks,
-                        uint32_t* grid) {
+int efs_batch_get_shape(efs_batch* b, int* mode, uint32_t* tile_threads, uint32_t* z_cells, uint32_t* waves,
+                        uint32_t* blocks, uint32_t* grid) {
     BTRY(need_final(b));
+    const bool pers = b->mode == EFS_BATCH_PERSISTENT;
     if (mode) *mode = b->mode;
-    if (tile_threads) *tile_threads = b->mode == EFS_BATCH_PERSISTENT ? (uint32_t)kSweepThreads : b->tpb;
+    if (tile_threads) *tile_threads = pers ? (uint32_t)kSweepThreads : b->tpb;
+    if (z_cells) *z_cells = pers ? 4u : b->zc;
     if (waves) *waves = b->waves;
     if (blocks) *blocks = b->nblocks;
-    if (grid) *grid = b->mode == EFS_BATCH_PERSISTENT ? b->pgrid : b->nblocks;
+    if (grid) *grid = pers ? b->pgrid : b->nblocks;
     return EFS_OK;
 }
 

@@ -72,9 +72,10 @@ int efs_batch_finalize(efs_batch *b, uint32_t sample_interval, uint32_t max_samp
 /* Autotuner: time `steps` whole steps (rounded up to a multiple of 32, as
  * they run: graph replays or persistent launches) for each candidate shape
  * -- launch form: column tile width 128/256/512 lanes x occupancy 4/5
- * waves per SIMD x 1024..8192 blocks; persistent form: 1, 2, 4 or 8 blocks
- * per resident workgroup -- and keep the fastest (efs_batch_get_shape
- * reports it).  Changes fields (call before the real run and reset with
+ * waves per SIMD x 1024..8192 blocks with 4-cell lanes, and (FP16) 256/512
+ * lanes x 1024..8192 blocks with 8-cell lanes at 3 waves; persistent
+ * form: 1, 2, 4 or 8 blocks per resident workgroup -- and keep the fastest
+ * (efs_batch_get_shape reports it).  Changes fields (call before the real run and reset with
  * efs_batch_reset).  *blocks / *waves receive the chosen shape (may be
  * NULL). */
 int efs_batch_tune(efs_batch *b, uint32_t steps, uint32_t *blocks, uint32_t *waves);
@@ -92,15 +93,20 @@ int efs_batch_tune(efs_batch *b, uint32_t steps, uint32_t *blocks, uint32_t *wav
 #define EFS_BATCH_PERSISTENT 1
 
 /* Set the launch shape after efs_batch_finalize: the form, the column tile
- * width (lanes of 4 z cells per workgroup: 128, 256 or 512; the persistent
- * kernel runs 256), the occupancy the sweep kernels are compiled for (4 or
- * 5 waves per SIMD; persistent: 3) and the blocks asked of the equal-cost
- * planner.  Fields and step count are kept. */
-int efs_batch_set_shape(efs_batch *b, int mode, uint32_t tile_threads, uint32_t waves, uint32_t blocks);
+ * width (lanes per workgroup), the z cells per lane, the occupancy the
+ * sweep kernels are compiled for (waves per SIMD) and the blocks asked of
+ * the equal-cost planner.  Valid shapes:
+ *   launch form, 4-cell lanes: 128/256/512 lanes, 4 or 5 waves;
+ *   launch form, 8-cell lanes (FP16 storage, whose rows are padded to 8
+ *   cells): 256/512 lanes, 3 waves;
+ *   persistent: 256 lanes of 4 cells, 3 waves.
+ * Fields and step count are kept. */
+int efs_batch_set_shape(efs_batch *b, int mode, uint32_t tile_threads, uint32_t z_cells, uint32_t waves,
+                        uint32_t blocks);
 /* The shape in use; *grid = workgroups per launch (persistent: resident
  * workgroups; launch form: the block count). */
-int efs_batch_get_shape(efs_batch *b, int *mode, uint32_t *tile_threads, uint32_t *waves, uint32_t *blocks,
-                        uint32_t *grid);
+int efs_batch_get_shape(efs_batch *b, int *mode, uint32_t *tile_threads, uint32_t *z_cells, uint32_t *waves,
+                        uint32_t *blocks, uint32_t *grid);
 
 /* Zero every field, Mur state and gather ring and set numTS to 0. */
 int efs_batch_reset(efs_batch *b);

@@ -118,14 +118,16 @@ def test_tune_then_same_results(patches):
     t.destroy()
 
 
-SHAPES = [("launch", 128, 4, 1024), ("launch", 512, 5, 4096), ("persistent", 256, 3, 768),
-          ("persistent", 256, 3, 6000), ("persistent-flat", 256, 3, 2048)]
+SHAPES = [("launch", 128, 4, 4, 1024), ("launch", 512, 4, 5, 4096), ("persistent", 256, 4, 3, 768),
+          ("persistent", 256, 4, 3, 6000), ("persistent-flat", 256, 4, 3, 2048)]
+SHAPES_FP16 = [("launch", 256, 8, 3, 2048), ("launch", 512, 8, 3, 1024)]
 
 
 @pytest.mark.parametrize("storage", ["fp32", "fp16"])
 def test_launch_shapes_same_bits(patches, storage, monkeypatch):
-    """The reference shape (launch form, 256-lane tiles, 4 waves, 2048
-    blocks) against other tile widths and the persistent kernel, over
+    """The reference shape (launch form, 256-lane tiles of 4-cell lanes, 4
+    waves, 2048 blocks) against other tile widths, 8-cell lanes (FP16) and
+    the persistent kernel, over
     iterate calls of 37 and 70 steps (off the graph and persistent-launch
     boundaries; FP16 crosses three range checks)."""
     ws = patches[:3]
@@ -138,13 +140,13 @@ def test_launch_shapes_same_bits(patches, storage, monkeypatch):
     want = [ref.getFields(i) for i in range(len(ws))]
     want_g = [ref.readProbeGatherBuffered(i)["data"] for i in range(len(ws))]
     ref.destroy()
-    for mode, tpb, waves, blocks in SHAPES:
+    for mode, tpb, zc, waves, blocks in SHAPES + (SHAPES_FP16 if storage == "fp16" else []):
         if mode == "persistent-flat":
             monkeypatch.setenv("EFS_BATCH_BARRIER", "flat")
         else:
             monkeypatch.delenv("EFS_BATCH_BARRIER", raising=False)
         b, _ = make_batch(ws, storage=storage, steps=sum(calls))
-        b.setShape(mode.split("-")[0], tpb, waves, blocks)
+        b.setShape(mode.split("-")[0], tpb, zc, waves, blocks)
         say("shape", mode, b.shape())
         for n in calls:
             b.iterate(n)
@@ -160,8 +162,9 @@ def test_launch_shapes_same_bits(patches, storage, monkeypatch):
 
 def test_shape_rejects_invalid(patches):
     b, _ = make_batch(patches[:1], steps=4)
-    for args in (("launch", 384, 4, 100), ("launch", 256, 3, 100), ("persistent", 512, 3, 100),
-                 ("persistent", 256, 4, 100), ("launch", 256, 4, 0)):
+    for args in (("launch", 384, 4, 4, 100), ("launch", 256, 4, 3, 100), ("persistent", 512, 4, 3, 100),
+                 ("persistent", 256, 4, 4, 100), ("launch", 256, 4, 4, 0), ("launch", 256, 8, 4, 100),
+                 ("launch", 256, 6, 4, 100), ("launch", 256, 8, 3, 100)):
         with pytest.raises(RuntimeError):
             b.setShape(*args)
     b.destroy()
