@@ -297,7 +297,7 @@ def run_batch(args):
     avg_e, avg_h = p["e_ms"] / max(p["e_n"], 1), p["h_ms"] / max(p["h_n"], 1)
     # per cell and sweep: FP32 60 B; FP16 fields with 16-bit coefficient
     # indices count coefficients at 2 B (30 B), FP32 coefficient arrays 42 B
-    per_inst = [int(np.prod(w["gridSize"][:2])) * ((w["gridSize"][2] + 3) // 4 * 4) for w in ws]
+    per_inst = [int(np.prod(w["gridSize"])) for w in ws]  # real cells (row padding is not algorithmic work)
     by = sum(c * (18.0 + 6 * coef_bytes(storage, ix) if storage == "fp16" else 60.0)
              for c, ix in zip(per_inst, indexed))
     dom_name, dom_ms = ("sweep_h", avg_h) if avg_h >= avg_e else ("sweep_e", avg_e)

@@ -537,7 +537,11 @@ struct efs_batch {
     uint32_t waves = 4;
     uint32_t tpb = kSweepThreads;  // lanes per column tile (launch form)
     uint32_t zc = 4;               // z cells per lane (launch form: 4, or 8 when every row is padded to 8)
-    uint32_t zpad = 4;             // rows padded to a multiple of this (FP16: 8)
+    // Rows are padded to a multiple of zpad cells: 8 (default) lets the
+    // sweeps run 8-cell lanes (16-byte FP16 accesses) and was faster for
+    // 4-cell lanes too (64-instance design sweep: FP32 47.3k -> 48.5k,
+    // FP16 68.4k -> 71.8k MCells/s); EFS_BATCH_ZPAD=4 pads to 4.
+    uint32_t zpad = 8;
     int mode = EFS_BATCH_LAUNCH;
     uint32_t pgrid = 0;            // workgroups of the persistent kernel
     GridBarState* d_bar = nullptr;
@@ -707,11 +711,9 @@ typedef void (*SweepFn)(const Inst*, const Block*, uint32_t, const float*);
 
 template <bool HF, typename T>
 SweepFn sweep_kernel_t(uint32_t waves, uint32_t tpb, uint32_t z) {
-    if constexpr (Fs<T>::kHalf) {
-        // 8-cell lanes (FP16 rows padded to 8): 3 waves per SIMD, 147-152
-        // registers (at 4 waves they spill and run 30 % slower)
-        if (z == 8) return tpb == 512 ? bsweep<HF, T, 3, 512, 8> : bsweep<HF, T, 3, 256, 8>;
-    }
+    // 8-cell lanes (rows padded to 8): 3 waves per SIMD, 147-165 registers
+    // (at 4 waves they spill and run 30 % slower)
+    if (z == 8) return tpb == 512 ? bsweep<HF, T, 3, 512, 8> : bsweep<HF, T, 3, 256, 8>;
     if (tpb == 128) return waves == 5 ? bsweep<HF, T, 5, 128, 4> : bsweep<HF, T, 4, 128, 4>;
     if (tpb == 512) return waves == 5 ? bsweep<HF, T, 5, 512, 4> : bsweep<HF, T, 4, 512, 4>;
     return waves == 5 ? bsweep<HF, T, 5, 256, 4> : bsweep<HF, T, 4, 256, 4>;
@@ -1038,6 +1040,7 @@ int efs_batch_create(efs_batch** out, int device) {
     if (device < 0 || device >= n) return bfail(nullptr, EFS_ERR_ARG, "device %d out of range (%d devices)", device, n);
     efs_batch* b = new efs_batch();
     b->device = device;
+    if (const char* e = std::getenv("EFS_BATCH_ZPAD")) b->zpad = std::atoi(e) == 4 ? 4u : 8u;
     {
         // grid-barrier form of the persistent kernel: "flat" = a write-back
         // per workgroup (the reference form, for measurement)
@@ -1104,9 +1107,6 @@ int efs_batch_set_storage(efs_batch* b, int storage) {
         return bfail(b, EFS_ERR_ARG, "unknown storage %d", storage);
     if (!b->inst.empty()) return bfail(b, EFS_ERR_STATE, "set the storage before adding instances");
     b->half = storage == EFS_STORAGE_F16;
-    // FP16 rows are padded to 8 cells, so the sweeps may run 8-cell lanes
-    // (16-byte accesses); FP32 lanes of 4 cells already move 16 bytes
-    b->zpad = b->half ? 8u : 4u;
     return EFS_OK;
 }
 
@@ -1532,7 +1532,7 @@ int efs_batch_set_shape(efs_batch* b, int mode, uint32_t tile_threads, uint32_t 
         if (waves != 4 && waves != 5) return bfail(b, EFS_ERR_ARG, "waves must be 4 or 5 (got %u)", waves);
     } else if (z_cells == 8) {
         if (b->zpad != 8)
-            return bfail(b, EFS_ERR_ARG, "8-cell lanes need rows padded to 8 cells (FP16 storage)");
+            return bfail(b, EFS_ERR_ARG, "8-cell lanes need rows padded to 8 cells (EFS_BATCH_ZPAD=4 was set)");
         if (tile_threads != 256 && tile_threads != 512)
             return bfail(b, EFS_ERR_ARG, "8-cell lanes run 256- or 512-lane tiles (got %u)", tile_threads);
         if (waves != 3) return bfail(b, EFS_ERR_ARG, "8-cell lanes run 3 waves per SIMD (got %u)", waves);

@@ -72,8 +72,8 @@ int efs_batch_finalize(efs_batch *b, uint32_t sample_interval, uint32_t max_samp
 /* Autotuner: time `steps` whole steps (rounded up to a multiple of 32, as
  * they run: graph replays or persistent launches) for each candidate shape
  * -- launch form: column tile width 128/256/512 lanes x occupancy 4/5
- * waves per SIMD x 1024..8192 blocks with 4-cell lanes, and (FP16) 256/512
- * lanes x 1024..8192 blocks with 8-cell lanes at 3 waves; persistent
+ * waves per SIMD x 1024..8192 blocks with 4-cell lanes, and 256/512 lanes x
+ * 1024..8192 blocks with 8-cell lanes at 3 waves; persistent
  * form: 1, 2, 4 or 8 blocks per resident workgroup -- and keep the fastest
  * (efs_batch_get_shape reports it).  Changes fields (call before the real run and reset with
  * efs_batch_reset).  *blocks / *waves receive the chosen shape (may be
@@ -97,8 +97,8 @@ int efs_batch_tune(efs_batch *b, uint32_t steps, uint32_t *blocks, uint32_t *wav
  * sweep kernels are compiled for (waves per SIMD) and the blocks asked of
  * the equal-cost planner.  Valid shapes:
  *   launch form, 4-cell lanes: 128/256/512 lanes, 4 or 5 waves;
- *   launch form, 8-cell lanes (FP16 storage, whose rows are padded to 8
- *   cells): 256/512 lanes, 3 waves;
+ *   launch form, 8-cell lanes (rows are padded to 8 cells unless the
+ *   environment sets EFS_BATCH_ZPAD=4): 256/512 lanes, 3 waves;
  *   persistent: 256 lanes of 4 cells, 3 waves.
  * Fields and step count are kept. */
 int efs_batch_set_shape(efs_batch *b, int mode, uint32_t tile_threads, uint32_t z_cells, uint32_t waves,

@@ -120,7 +120,7 @@ def test_tune_then_same_results(patches):
 
 SHAPES = [("launch", 128, 4, 4, 1024), ("launch", 512, 4, 5, 4096), ("persistent", 256, 4, 3, 768),
           ("persistent", 256, 4, 3, 6000), ("persistent-flat", 256, 4, 3, 2048)]
-SHAPES_FP16 = [("launch", 256, 8, 3, 2048), ("launch", 512, 8, 3, 1024)]
+SHAPES_Z8 = [("launch", 256, 8, 3, 2048), ("launch", 512, 8, 3, 1024)]
 
 
 @pytest.mark.parametrize("storage", ["fp32", "fp16"])
@@ -140,7 +140,7 @@ def test_launch_shapes_same_bits(patches, storage, monkeypatch):
     want = [ref.getFields(i) for i in range(len(ws))]
     want_g = [ref.readProbeGatherBuffered(i)["data"] for i in range(len(ws))]
     ref.destroy()
-    for mode, tpb, zc, waves, blocks in SHAPES + (SHAPES_FP16 if storage == "fp16" else []):
+    for mode, tpb, zc, waves, blocks in SHAPES + SHAPES_Z8:
         if mode == "persistent-flat":
             monkeypatch.setenv("EFS_BATCH_BARRIER", "flat")
         else:
@@ -160,13 +160,20 @@ def test_launch_shapes_same_bits(patches, storage, monkeypatch):
         b.destroy()
 
 
-def test_shape_rejects_invalid(patches):
+def test_shape_rejects_invalid(patches, monkeypatch):
     b, _ = make_batch(patches[:1], steps=4)
     for args in (("launch", 384, 4, 4, 100), ("launch", 256, 4, 3, 100), ("persistent", 512, 4, 3, 100),
                  ("persistent", 256, 4, 4, 100), ("launch", 256, 4, 4, 0), ("launch", 256, 8, 4, 100),
-                 ("launch", 256, 6, 4, 100), ("launch", 256, 8, 3, 100)):
+                 ("launch", 256, 6, 4, 100), ("launch", 128, 8, 3, 100)):
         with pytest.raises(RuntimeError):
             b.setShape(*args)
+    b.destroy()
+    # rows padded to 4 cells: no 8-cell lanes
+    monkeypatch.setenv("EFS_BATCH_ZPAD", "4")
+    b, _ = make_batch(patches[:1], steps=4)
+    with pytest.raises(RuntimeError):
+        b.setShape("launch", 256, 8, 3, 100)
+    b.setShape("launch", 256, 4, 4, 100)
     b.destroy()
 
 
