@@ -2406,15 +2406,19 @@ static FarParams far_params(const efs_ctx* c, uint32_t nT, uint32_t nP, const do
 struct FarPlan {
     uint32_t ablk, chunk, nchunks;
 };
-static FarPlan far_plan(const efs_ctx* c, size_t na, bool mirror, uint32_t max_chunks) {
+static FarPlan far_plan_n(uint32_t P, size_t na, bool mirror, uint32_t max_chunks) {
     FarPlan p{};
     p.ablk = nblk(na, 256);
-    const uint32_t Pv = mirror ? 2 * c->nf_P : c->nf_P;  // the image adds a second pass over the points
+    const uint32_t Pv = mirror ? 2 * P : P;  // the image adds a second pass over the points
     const uint32_t tiles = (Pv + kFarTile - 1) / kFarTile;
     const uint32_t want = std::max(1u, std::min({tiles, (2048u + p.ablk - 1) / p.ablk, max_chunks}));
     p.chunk = ((tiles + want - 1) / want) * kFarTile;
     p.nchunks = (Pv + p.chunk - 1) / p.chunk;
     return p;
+}
+
+static FarPlan far_plan(const efs_ctx* c, size_t na, bool mirror, uint32_t max_chunks) {
+    return far_plan_n(c->nf_P, na, mirror, max_chunks);
 }
 
 // The far-field kernels on the context's stream: per-chunk FP64 partials
@@ -2431,7 +2435,7 @@ static void launch_farfield(efs_ctx* c, uint32_t f, const float* meta, const uin
         hipLaunchKernelGGL(k_farfield<float>, dim3(p.ablk, p.nchunks), dim3(256), 0, c->stream,
                            c->nf_acc + 2 * (size_t)f, c->nf_cmp ? c->nf_cmp + 2 * (size_t)f : nullptr, c->nf_NC,
                            c->nf_geom.rowsE, meta, perm, theta, phi, q, p.chunk, part);
-    hipLaunchKernelGGL(k_farfield_final, dim3(p.ablk), dim3(256), 0, c->stream, part, p.nchunks, q, out);
+    hipLaunchKernelGGL(k_farfield_final<float>, dim3(p.ablk), dim3(256), 0, c->stream, part, p.nchunks, q, out);
 }
 
 static int farfield_checks(efs_ctx* c, uint32_t f, const void* meta, const void* theta, uint32_t nT, const void* phi,
@@ -2513,6 +2517,70 @@ int efs_farfield_noalloc(efs_ctx* c, uint32_t f, const float* meta_dev, const fl
     launch_farfield(c, f, meta_dev, c->nf_perm_d, theta_dev, phi_dev, q, plan, c->scratch, out_dev);
     CHECK_LAUNCH(c);
     return EFS_OK;
+}
+
+int efs_farfield_surface(efs_ctx* c, uint32_t P, const float* meta, const double* acc_e, const double* acc_h,
+                         const float* theta, uint32_t nT, const float* phi, uint32_t nP, const double* center,
+                         double frequency, double radius, double* out) {
+    if (!c) return EFS_ERR_STATE;
+    if (!P || !meta || !acc_e || !acc_h || !theta || !phi || !center || !out || !nT || !nP)
+        return fail(c, EFS_ERR_ARG, "far-field arrays are required");
+    if (!(radius > 0)) return fail(c, EFS_ERR_ARG, "radius must be > 0");
+    if (!(frequency > 0)) return fail(c, EFS_ERR_ARG, "frequency must be > 0");
+    if ((size_t)nT * nP > 0xffffffffull / 256) return fail(c, EFS_ERR_ARG, "too many far-field angles");
+    if ((uint64_t)P * 6 >= (1ull << 31)) return fail(c, EFS_ERR_ARG, "too many surface points");
+    for (uint32_t p = 0; p < P; ++p) {
+        const float nd = meta[8 * (size_t)p + 3];
+        if (!(nd == 0.f || nd == 1.f || nd == 2.f))
+            return fail(c, EFS_ERR_ARG, "point %u: normal direction must be 0, 1 or 2", p);
+    }
+    HIP_TRY(c, hipSetDevice(c->device));
+    // the kernel's row layout: E rows c*P + p, then H rows 3P + c*P + p,
+    // (re, im) per row
+    std::vector<double> acc(12 * (size_t)P);
+    for (uint32_t p = 0; p < P; ++p)
+        for (uint32_t k = 0; k < 3; ++k)
+            for (uint32_t r = 0; r < 2; ++r) {
+                acc[2 * ((size_t)k * P + p) + r] = acc_e[6 * (size_t)p + 2 * k + r];
+                acc[2 * (3 * (size_t)P + (size_t)k * P + p) + r] = acc_h[6 * (size_t)p + 2 * k + r];
+            }
+    const size_t na = (size_t)nT * nP;
+    const FarPlan plan = far_plan_n(P, na, false, 0xffffffffu);
+    const double C0 = 299792458.0, Z0 = 376.73031346177066;
+    const double k = 2 * M_PI * frequency / C0;
+    const double fmag = k / (4 * M_PI * radius), fph = -k * radius;
+    FarParams q{};
+    q.P = P; q.nT = nT; q.nP = nP;
+    q.k = (float)k; q.Z0 = (float)Z0;
+    q.cx = (float)center[0]; q.cy = (float)center[1]; q.cz = (float)center[2];
+    q.fac_re = (float)(fmag * -std::sin(fph));
+    q.fac_im = (float)(fmag * std::cos(fph));
+    q.mdir = -1;
+    q.msign = 1.0f;
+    for (int k3 = 0; k3 < 3; ++k3) q.ef[k3] = q.hf[k3] = 1.0f;
+    double *d_acc = nullptr, *d_part = nullptr, *d_out = nullptr;
+    float *d_meta = nullptr, *d_th = nullptr, *d_ph = nullptr;
+    int rc = EFS_OK;
+    do {
+        if ((rc = dupload(c, d_acc, acc.data(), acc.size()))) break;
+        if ((rc = dupload(c, d_meta, meta, 8 * (size_t)P))) break;
+        if ((rc = dupload(c, d_th, theta, nT))) break;
+        if ((rc = dupload(c, d_ph, phi, nP))) break;
+        if ((rc = dalloc(c, d_out, 5 * na))) break;
+        if ((rc = dalloc(c, d_part, (size_t)plan.nchunks * na * 8))) break;
+        hipLaunchKernelGGL(k_farfield<double>, dim3(plan.ablk, plan.nchunks), dim3(256), 0, c->stream,
+                           (const double*)d_acc, (const double*)nullptr, 2u, 3 * P, (const float*)d_meta,
+                           (const uint32_t*)nullptr, (const float*)d_th, (const float*)d_ph, q, plan.chunk, d_part);
+        hipLaunchKernelGGL(k_farfield_final<double>, dim3(plan.ablk), dim3(256), 0, c->stream,
+                           (const double*)d_part, plan.nchunks, q, d_out);
+        hipError_t e = hipGetLastError();
+        if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+        if (e == hipSuccess) e = hipMemcpy(out, d_out, 5 * na * sizeof(double), hipMemcpyDeviceToHost);
+        if (e != hipSuccess) rc = fail(c, EFS_ERR_HIP, "surface far field: %s", hipGetErrorString(e));
+    } while (0);
+    hipStreamSynchronize(c->stream);
+    dfree(d_acc); dfree(d_meta); dfree(d_th); dfree(d_ph); dfree(d_out); dfree(d_part);
+    return rc;
 }
 
 int efs_set_nf2ff_compensated(efs_ctx* c, int on) {

@@ -408,8 +408,9 @@ __global__ __launch_bounds__(256) void k_farfield(const A* __restrict__ acc_f, c
 
 // Sum of the chunk partials per angle, then E_theta, E_phi and the radiation
 // intensity (5 floats per angle).
+template <typename O>
 __global__ __launch_bounds__(256) void k_farfield_final(const double* __restrict__ partial, uint32_t nchunks,
-                                                        FarParams q, float* __restrict__ out) {
+                                                        FarParams q, O* __restrict__ out) {
     const uint32_t nA = q.nT * q.nP;
     const uint32_t a = blockIdx.x * blockDim.x + threadIdx.x;
     if (a >= nA) return;
@@ -425,12 +426,12 @@ __global__ __launch_bounds__(256) void k_farfield_final(const double* __restrict
     const double Etr = -(fr * a1r - fi * a1i), Eti = -(fr * a1i + fi * a1r);
     const double a2r = Ltr - Z0 * Npr, a2i = Lti - Z0 * Npi;
     const double Epr = fr * a2r - fi * a2i, Epi = fr * a2i + fi * a2r;
-    float* o = out + 5 * (size_t)a;
-    o[0] = (float)((Etr * Etr + Eti * Eti + Epr * Epr + Epi * Epi) / (2.0 * Z0));
-    o[1] = (float)Etr;
-    o[2] = (float)Eti;
-    o[3] = (float)Epr;
-    o[4] = (float)Epi;
+    O* o = out + 5 * (size_t)a;
+    o[0] = (O)((Etr * Etr + Eti * Eti + Epr * Epr + Epi * Epi) / (2.0 * Z0));
+    o[1] = (O)Etr;
+    o[2] = (O)Eti;
+    o[3] = (O)Epr;
+    o[4] = (O)Epi;
 }
 
 }  // namespace efs

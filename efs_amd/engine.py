@@ -72,6 +72,8 @@ SIGNATURES = {
                                C.c_double, C.c_double, _f32p]),
     "efs_farfield_mirror": (C.c_int, [C.c_void_p, C.c_uint32, _f32p, _f32p, C.c_uint32, _f32p, C.c_uint32, _f64p,
                                       C.c_double, C.c_double, C.c_int, C.c_double, C.c_int, C.c_int, _f32p]),
+    "efs_farfield_surface": (C.c_int, [C.c_void_p, C.c_uint32, _f32p, _f64p, _f64p, _f32p, C.c_uint32, _f32p,
+                                       C.c_uint32, _f64p, C.c_double, C.c_double, _f64p]),
     "efs_energy": (C.c_int, [C.c_void_p, C.POINTER(C.c_double)]),
     "efs_profile_enable": (C.c_int, [C.c_void_p, C.c_int]),
     "efs_profile_read": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_uint64)]),
@@ -510,6 +512,22 @@ class HIPEngine:
                                                  float(config["frequency"]), float(config["radius"]),
                                                  int(m["direction"]), float(m.get("position", 0.0)), mtype,
                                                  1 if m.get("imageNormals") else 0, out))
+        return out
+
+    def farfieldSurface(self, point_meta, acc_e, acc_h, theta, phi, center, frequency, radius=1.0):
+        """Far field of caller-supplied surface points (efs_farfield_surface):
+        point_meta 8 floats per point, acc_e / acc_h 6 doubles per point;
+        returns 5 doubles per angle (P_rad, Et re/im, Ep re/im).  Needs
+        initGPU only."""
+        meta = f32(point_meta)
+        P = meta.size // 8
+        ae, ah = f64(acc_e), f64(acc_h)
+        if ae.size != 6 * P or ah.size != 6 * P:
+            raise RuntimeError(f"surface fields need 6 values per point ({6 * P}), got {ae.size} and {ah.size}")
+        th, ph = f32(theta), f32(phi)
+        out = np.empty(5 * len(th) * len(ph), dtype=np.float64)
+        self._check(self.lib.efs_farfield_surface(self.ctx, P, meta, ae, ah, th, len(th), ph, len(ph), f64(center),
+                                                  float(frequency), float(radius), out))
         return out
 
     # -- energy ----------------------------------------------------------------

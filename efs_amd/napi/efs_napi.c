@@ -608,6 +608,29 @@ static napi_value js_farfield(napi_env env, napi_callback_info info) {
                                                mdir, mpos, mtype, image_normals, out));
 }
 
+/* farfieldSurface(h, meta(F32, 8/point), accE(F64, 6/point), accH(F64),
+ *                 theta, phi, center(F64[3]), freq, radius, out(F64, 5/angle)) */
+static napi_value js_farfield_surface(napi_env env, napi_callback_info info) {
+    BEGIN();
+    CTX(0);
+    size_t lm, le, lh, lt, lp, lc, lo;
+    float *meta = F32(1, lm);
+    double *ae = F64(2, le), *ah = F64(3, lh);
+    float *th = F32(4, lt), *ph = F32(5, lp);
+    double *ce = F64(6, lc);
+    double freq = f64_arg(&c, 7), rad = f64_arg(&c, 8);
+    double *out = F64(9, lo);
+    BAIL();
+    const size_t P = lm / 8;
+    if (lm % 8 != 0 || le != 6 * P || lh != 6 * P || lc != 3 || lo != 5 * lt * lp) {
+        throwf(&c, "efs: farfieldSurface needs 8 meta floats and 6 E and H values per point, a center of 3 and "
+                   "5 outputs per angle");
+        return NULL;
+    }
+    return status(&c, ctx, efs_farfield_surface(ctx, (uint32_t)P, meta, ae, ah, th, (uint32_t)lt, ph, (uint32_t)lp,
+                                                ce, freq, rad, out));
+}
+
 static napi_value js_energy(napi_env env, napi_callback_info info) {
     BEGIN();
     CTX(0);
@@ -883,6 +906,7 @@ static napi_value Init(napi_env env, napi_value exports) {
         FN("cfgNF2FF", js_cfg_nf2ff),
         FN("readNF2FF", js_read_nf2ff),
         FN("farfield", js_farfield),
+        FN("farfieldSurface", js_farfield_surface),
         FN("energy", js_energy),
         FN("batchCreate", js_batch_create),
         FN("batchDestroy", js_batch_destroy),

@@ -218,6 +218,22 @@ int efs_farfield_noalloc(efs_ctx *ctx, uint32_t freq_idx, const float *point_met
                          uint32_t n_theta, const float *phi_dev, uint32_t n_phi, const double *center,
                          double frequency, double radius, float *out_dev);
 
+/* computeNF2FF (src/nf2ff.mjs:102-436, the reference's CPU far field over
+ * surface fields it reads back from openEMS dumps) on the GPU: the
+ * radiation integrals of P caller-supplied surface points.  point_meta: 8
+ * floats per point as for efs_farfield (Cartesian position [m], normal axis
+ * 0..2, normal sign, area, 2 unused); acc_e / acc_h: the complex E and H at
+ * each point, 6 doubles per point [6p + 2c + {0: re, 1: im}] (Cartesian
+ * components).  A point of a cylindrical mesh (src/nf2ff.mjs:165-285) is
+ * given as the Cartesian points its rotated normal decomposes into (the
+ * integrals are linear in area * normal); the host wrappers do that.  out
+ * receives 5 doubles per (theta, phi) pair, theta-major: P_rad, Et_re,
+ * Et_im, Ep_re, Ep_im.  Needs a context (efs_create) but not efs_init;
+ * synchronous. */
+int efs_farfield_surface(efs_ctx *ctx, uint32_t P, const float *point_meta, const double *acc_e,
+                         const double *acc_h, const float *theta, uint32_t n_theta, const float *phi,
+                         uint32_t n_phi, const double *center, double frequency, double radius, double *out);
+
 /* NF2FF accumulation form (no counterpart in the reference, which adds each
  * step's term into FP32 accumulators with a plain +=,
  * src/webgpu-engine.mjs:785-798): 1 (default) = compensated, every DFT
