@@ -687,12 +687,18 @@ def calibrate_copy():
 
 
 def stream_reference(reps=20):
-    """Achievable HBM streaming rate on this GPU, for reading the sweeps'
-    real traffic against: out_c = a_c + 0.1 * b_c over FP32 arrays of 64 M
-    floats (k = 1: 2 reads + 1 write, 768 MiB per call; k = 3: 6 reads + 3
-    writes like a sweep's field streams, 2.25 GiB), both larger than the
-    last-level cache, at 1024-8192 workgroups; the best rate is the
-    reference.  Timed with HIP events (efs_amd/csrc/stream_ref.hip)."""
+    """Achievable HBM streaming rates on this GPU, for reading the sweeps'
+    real traffic against (efs_amd/csrc/stream_ref.hip, HIP events, arrays
+    larger than the last-level cache, best over 1024-16384 workgroups):
+    - add: out_c = a_c + 0.1 * b_c over FP32 arrays of 64 M floats (k = 1:
+      2 reads + 1 write; k = 3: 6 reads + 3 writes like a sweep's field
+      streams);
+    - copy: o = a over 256 M floats (1 GiB each way), 4 independent 16-byte
+      loads in flight per thread, plain or non-temporal accesses;
+    - memcpy: hipMemcpyAsync device to device of the same arrays;
+    - read: a sum over the 1 GiB array (read only).
+    "GB/s" is the best read+write rate (add, copy, memcpy); the read-only
+    rate is reported beside it."""
     import ctypes
     path = os.path.join(ROOT, "efs_amd", "libefs_stream_ref.so")
     if not os.path.exists(path):  # an auxiliary figure: the bench line goes on without it
@@ -702,18 +708,40 @@ def stream_reference(reps=20):
     lib.efs_stream_ref.restype = ctypes.c_int
     lib.efs_stream_ref.argtypes = [ctypes.c_int, ctypes.c_uint64, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32,
                                    ctypes.POINTER(ctypes.c_double)]
+    rates = {}
     n = 64 * 1024 * 1024
-    best = None
     for k in (1, 3):
         for grid in (1024, 2048, 4096, 8192):
             ms = ctypes.c_double()
             if lib.efs_stream_ref(0, n, k, grid, reps, ctypes.byref(ms)) != 0:
                 return None
             gbs = 3 * k * 4 * n / (ms.value * 1e-3) / 1e9
-            if best is None or gbs > best["GB/s"]:
-                best = {"GB/s": round(gbs, 1), "streams": f"{2 * k} reads + {k} write{'s' if k > 1 else ''} of 256 MiB FP32 each",
-                        "workgroups": grid, "avg_ms": round(ms.value, 4)}
-    best["kernel"] = "out_c = a_c + 0.1 * b_c, grid-stride, 16-B accesses; best of 2 stream counts x 4 grid sizes"
+            key = f"add_k{k}"
+            if key not in rates or gbs > rates[key]["GB/s"]:
+                rates[key] = {"GB/s": round(gbs, 1), "workgroups": grid, "avg_ms": round(ms.value, 4)}
+    if hasattr(lib, "efs_stream_kind"):
+        lib.efs_stream_kind.restype = ctypes.c_int
+        lib.efs_stream_kind.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_uint64, ctypes.c_uint32, ctypes.c_uint32,
+                                        ctypes.POINTER(ctypes.c_double)]
+        n = 256 * 1024 * 1024
+        for kind, key, nbytes in ((0, "copy", 8 * n), (1, "copy_nontemporal", 8 * n), (2, "read", 4 * n),
+                                  (3, "memcpy", 8 * n)):
+            for grid in ((1024, 2048, 4096, 8192, 16384) if kind != 3 else (1,)):
+                ms = ctypes.c_double()
+                if lib.efs_stream_kind(0, kind, n, grid, reps, ctypes.byref(ms)) != 0:
+                    return None
+                gbs = nbytes / (ms.value * 1e-3) / 1e9
+                if key not in rates or gbs > rates[key]["GB/s"]:
+                    rates[key] = {"GB/s": round(gbs, 1), "avg_ms": round(ms.value, 4)}
+                    if kind != 3:
+                        rates[key]["workgroups"] = grid
+    rw = {k: v for k, v in rates.items() if k != "read"}
+    best_key = max(rw, key=lambda k: rw[k]["GB/s"])
+    best = {"GB/s": rw[best_key]["GB/s"], "best_of": best_key, "rates": rates,
+            "kernel": "best read+write rate of: out_c = a_c + 0.1 * b_c (k = 1, 3), copy (plain, non-temporal), "
+                      "hipMemcpy device to device; arrays larger than the last-level cache"}
+    if "read" in rates:
+        best["read_only_GB/s"] = rates["read"]["GB/s"]
     return best
 
 
