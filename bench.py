@@ -57,6 +57,9 @@ TRAFFIC_FILES = {"fp32": os.path.join(ROOT, "profiles", "fused_sweeps_256cube_st
                  "fp16": os.path.join(ROOT, "profiles", "fused_sweeps_fp16_256cube_stress_traffic.json")}
 
 
+BATCH_TRAFFIC_FILES = {"fp16": os.path.join(ROOT, "profiles", "batched_sweeps_fp16_patch_design_sweep_traffic.json")}
+
+
 def sweep_kernel_symbol(flags, side):
     """Name of the kernel that runs the E or H sweep for the path flags."""
     if "vec4" not in flags:
@@ -277,7 +280,12 @@ def run_batch(args):
     t_build = time.perf_counter()
     b, ws, _ = build_patch_batch(n_inst, storage, K + W + 64, progress=True)
     t_build = time.perf_counter() - t_build
-    shape = b.tune(steps=32)
+    if args.batch_shape:  # a fixed launch shape instead of the tuner (profiling runs)
+        m, tpb, zc, wv, blk = args.batch_shape.split(",")
+        b.setShape(m, int(tpb), int(zc), int(wv), int(blk))
+        shape = b.shape()
+    else:
+        shape = b.tune(steps=32)
     info = b.info()
     cells = info["cells"]
     indexed = [b.coefIndexed(i) for i in range(len(ws))]
@@ -302,6 +310,18 @@ def run_batch(args):
              for c, ix in zip(per_inst, indexed))
     dom_name, dom_ms = ("sweep_h", avg_h) if avg_h >= avg_e else ("sweep_e", avg_e)
     dom_gbs = by / (dom_ms * 1e-3) / 1e9
+    # measured HBM bytes of the dominant sweep kernel (committed rocprofv3
+    # FETCH/WRITE summary of this workload and shape), if there is one
+    prof_shape = shape if not persistent else {"tile_threads": 256, "z_cells": 4, "waves": 4}
+    dom_sym = (f"efs::batch::bsweep<{'true' if dom_name == 'sweep_h' else 'false'}, "
+               f"{'f16' if storage == 'fp16' else 'float'}, {prof_shape['waves']}, {prof_shape['tile_threads']}, "
+               f"{prof_shape['z_cells']}>")
+    traffic, traffic_src = None, None
+    tpath = BATCH_TRAFFIC_FILES.get(storage)
+    if tpath and os.path.exists(tpath) and n_inst == 64:
+        with open(tpath) as f:
+            traffic = json.load(f).get(dom_sym, {}).get("hbm_bytes_per_launch")
+        traffic_src = os.path.relpath(tpath, ROOT) + f" [{dom_sym}]" if traffic else None
     extra = {}
     print("# plain sequential baseline", file=sys.stderr, flush=True)
     plain = plain_sequential(ws, args.plain_steps)
@@ -346,7 +366,8 @@ def run_batch(args):
                    "tuned_shape": shape, "graph_steps": info["graph_steps"], "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "kernel": "batched " + dom_name, "achieved": round(dom_gbs, 1),
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(dom_gbs / HBM_PEAK_GBS, 4),
-                     "traffic": None, "traffic_source": None, "algorithmic_bytes_per_launch": by,
+                     "traffic": round(traffic) if traffic else None, "traffic_source": traffic_src,
+                     "algorithmic_bytes_per_launch": by,
                      "algorithmic_model": ("60 B/cell (FP32 fields and coefficients)" if storage == "fp32" else
                                            f"FP16 fields: {sum(indexed)} of {len(indexed)} instances with 16-bit "
                                            "coefficient indices at 30 B/cell, the others 42 B/cell"),
@@ -767,6 +788,8 @@ def main():
                          "384 M-cell mixed-precision grid with a 4 x 4 port array")
     ap.add_argument("--instances", type=int, default=64, help="batch: number of patch instances")
     ap.add_argument("--plain-steps", type=int, default=200, help="batch: steps per instance of the plain baseline")
+    ap.add_argument("--batch-shape", default=None,
+                    help="batch: fixed launch shape mode,tile_threads,z_cells,waves,blocks instead of the autotuner")
     ap.add_argument("--s11-steps", type=int, default=20000,
                     help="batch: steps of the S11 comparison against the FP64 storage form")
     ap.add_argument("--precision-steps", type=int, default=512,

@@ -791,6 +791,10 @@ void drop_graph(efs_batch* b) {
 int build_graph(efs_batch* b) {
     drop_graph(b);
     if (b->mode == EFS_BATCH_PERSISTENT) return EFS_OK;  // one launch already covers many steps
+    // EFS_BATCH_GRAPH=0: direct launches only (tools that cannot follow
+    // graph replays)
+    if (const char* e = std::getenv("EFS_BATCH_GRAPH"))
+        if (e[0] == '0') return EFS_OK;
     BHIP(b, hipStreamSynchronize(b->stream));
     hipGraph_t g = nullptr;
     BHIP(b, hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
@@ -1458,9 +1462,12 @@ int efs_batch_tune(efs_batch* b, uint32_t steps, uint32_t* blocks_out, uint32_t*
     if (b->zpad == 8)
         for (uint32_t tpb : {256u, 512u})
             for (uint32_t want : {1024u, 2048u, 4096u, 8192u}) cand.push_back({EFS_BATCH_LAUNCH, tpb, 8u, 3u, want});
+    // the persistent form is a candidate unless EFS_BATCH_PERSIST=0 (a
+    // profiler whose tool cannot follow cooperative launches)
+    const char* pe = std::getenv("EFS_BATCH_PERSIST");
     const int mode0 = b->mode;
     b->mode = EFS_BATCH_PERSISTENT;
-    const bool can_persist = persist_grid(b) == EFS_OK;
+    const bool can_persist = !(pe && pe[0] == '0') && persist_grid(b) == EFS_OK;
     b->mode = mode0;
     b->err.clear();
     if (can_persist)

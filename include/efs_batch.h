@@ -74,8 +74,9 @@ int efs_batch_finalize(efs_batch *b, uint32_t sample_interval, uint32_t max_samp
  * -- launch form: column tile width 128/256/512 lanes x occupancy 4/5
  * waves per SIMD x 1024..8192 blocks with 4-cell lanes, and 256/512 lanes x
  * 1024..8192 blocks with 8-cell lanes at 3 waves; persistent
- * form: 1, 2, 4 or 8 blocks per resident workgroup -- and keep the fastest
- * (efs_batch_get_shape reports it).  Changes fields (call before the real run and reset with
+ * form: 1, 2, 4 or 8 blocks per resident workgroup, unless the environment
+ * sets EFS_BATCH_PERSIST=0 -- and keep the fastest (efs_batch_get_shape
+ * reports it).  Changes fields (call before the real run and reset with
  * efs_batch_reset).  *blocks / *waves receive the chosen shape (may be
  * NULL). */
 int efs_batch_tune(efs_batch *b, uint32_t steps, uint32_t *blocks, uint32_t *waves);
