@@ -71,7 +71,9 @@ def sweep_kernel_symbol(flags, side):
         suffix = "m" + cc
     else:
         suffix = cc
-    return f"efs::sweep_{side}4{suffix}" + ("_h" if "f16" in flags else "")
+    # grids with side work run the twin kernel (NAME_s)
+    return (f"efs::sweep_{side}4{suffix}" + ("_h" if "f16" in flags else "") +
+            ("_s" if "side_work" in flags else ""))
 
 
 def measured_traffic(kernel, size, storage="fp32"):

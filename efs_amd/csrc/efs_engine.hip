@@ -53,6 +53,7 @@ namespace {
 std::string g_create_error;
 
 constexpr int kMaxDevices = 64;
+constexpr uint64_t kSideMaxCells = (uint64_t)4 << 20;  // EFS_SIDE_WORK=2: grids up to 4 Mi cells
 std::mutex g_scratch_mu;
 double* g_scratch[kMaxDevices] = {};
 
@@ -183,6 +184,13 @@ struct efs_ctx {
     uint32_t* exc_b = nullptr;
     uint32_t mur_nb = 0, exc_nb = 0;
     bool mur_pre_ahead = false;     // the previous step's k_post_h ran this step's Mur pre pass
+    // point work in extra workgroups of the sweep kernels (SideWork):
+    // side_on from EFS_SIDE_WORK; side_e: the E-sweep side of
+    // the previous step, pending while side_pend (it completes that step's
+    // sample and gather)
+    bool side_on = true;
+    bool side_pend = false, side_pend_gather = false;
+    SideWork side_e{};
     std::vector<uint32_t> h_exc_gidx;   // excitation cells (interface indices)
 
     // ADE (all orders/directions flattened)
@@ -1149,9 +1157,72 @@ void launch_post_h(efs_ctx* c, FV v, FV i, NfCapture cap, float* ring, bool pre)
     hipLaunchKernelGGL(k_post_h<FV>, dim3(q.nb_sample + q.nb_gather + nb_pre), dim3(c->pt), 0, c->stream, v, i, q);
 }
 
+// Side work of the H sweep of step t (SideWork): the E half of the NF2FF
+// sample, the E values of a due gather into `ring` (null: none due) and,
+// with `pre`, the Mur pre pass of step t+1.  Leaves in c->side_e the rest
+// of the sample and gather, for the next E sweep or side_flush.
+SideWork side_h(efs_ctx* c, uint32_t t, float* ring, bool pre) {
+    SideWork q{};
+    q.volt = fv(c, 0);
+    q.curr = fv(c, 1);
+    if (c->nf_P) {
+        if (c->nf_slot == 0) c->nf_ts0 = t + 1;
+        q.pts = c->nf_pts;
+        q.el = c->nf_el;
+        q.g = c->nf_geom;
+        q.row_out = c->nf_buf + (size_t)c->nf_slot * c->nf_rows;
+        q.cap = nf_capture(c);
+        q.nb_sample = nblk(c->nf_P, kSideThreads);
+    }
+    if (ring) {
+        q.gidx = c->gb_idx;
+        q.gout = ring;
+        q.gn = c->gb_n;
+        q.nb_gather = nblk(c->gb_n, kSideThreads);
+    }
+    const uint32_t nb = q.nb_sample + q.nb_gather;
+    c->side_e = q;
+    c->side_e.nb = (nb + 7u) & ~7u;
+    c->side_pend = nb > 0;
+    c->side_pend_gather = ring != nullptr;
+    uint32_t nb_pre = 0;
+    if (pre) {
+        q.vflux = c->pml_vflux;
+        q.coeff = c->mur_coeff;
+        q.normal_src = c->mur_normal_src;
+        q.shifted_src = c->mur_shifted_src;
+        q.saved = c->mur_saved;
+        q.mur_n = c->mur_n;
+        nb_pre = nblk(c->mur_n, kSideThreads);
+    }
+    q.nb = (nb + nb_pre + 7u) & ~7u;
+    return q;
+}
+
+// The pending sample and gather are complete (their E-sweep side was
+// launched): count them, and fold a full K-step block.
+int side_done(efs_ctx* c) {
+    if (!c->side_pend) return EFS_OK;
+    c->side_pend = false;
+    if (c->side_pend_gather) c->gb_count++;
+    if (c->side_e.nb_sample) {
+        c->nf_slot++;
+        if (c->nf_slot == c->nf_K) TRY(nf2ff_fold(c));
+    }
+    return EFS_OK;
+}
+
+// The E-sweep side of the last step of an iterate call, on its own.
+int side_flush(efs_ctx* c) {
+    if (!c->side_pend) return EFS_OK;
+    hipLaunchKernelGGL(k_side_flush, dim3(c->side_e.nb), dim3(kSideThreads), 0, c->stream, c->side_e);
+    CHECK_LAUNCH(c);
+    return side_done(c);
+}
+
 // One step.  next_follows: the same efs_iterate call runs another step
 // right after this one (its Mur pre pass may then run in this step's
-// k_post_h).
+// k_post_h, or in the side work of its H sweep).
 int launch_step(efs_ctx* c, bool next_follows) {
     const uint32_t t = c->numTS;
     ProfScope step_scope(c, EFS_PROF_STEP);
@@ -1186,22 +1257,28 @@ int launch_step(efs_ctx* c, bool next_follows) {
             if (has_pml && c->pml_fused)
                 kern = c->pml_comp ? (x16 ? sweep_e4fxp : c->comp ? sweep_e4fcp : sweep_e4fp)
                                    : (x16 ? sweep_e4fx : c->comp ? sweep_e4fc : sweep_e4f);
+            // the previous step's pending sample and gather ride along
+            const SideWork se = c->side_pend ? c->side_e : SideWork{};
+            if (se.nb) kern = side_twin(kern);
             if (c->half) {
                 auto k16 = has_pml ? (c->pml_comp ? (x16 ? sweep_e4fxp_h : c->comp ? sweep_e4fcp_h : sweep_e4fp_h)
                                                   : (x16 ? sweep_e4fx_h : c->comp ? sweep_e4fc_h : sweep_e4f_h))
                                    : (x16 ? sweep_e4x_h : c->comp ? sweep_e4c_h : sweep_e4_h);
-                hipLaunchKernelGGL(k16, dim3(c->geom.blocks), dim3(kSweepThreads), 0, c->stream, (f16*)c->volt,
-                                   (const f16*)c->curr, cf, c->geom, c->pml, c->pml_coef, c->pml_vflux, c->fscale,
-                                   NfCapture{});
+                if (se.nb) k16 = side_twin(k16);
+                hipLaunchKernelGGL(k16, dim3(se.nb + c->geom.blocks), dim3(kSweepThreads), 0, c->stream,
+                                   (f16*)c->volt, (const f16*)c->curr, cf, c->geom, c->pml, c->pml_coef,
+                                   c->pml_vflux, c->fscale, NfCapture{}, se);
             } else {
-                hipLaunchKernelGGL(kern, dim3(c->geom.blocks), dim3(kSweepThreads), 0, c->stream, c->volt, c->curr,
-                                   cf, c->geom, c->pml, c->pml_coef, c->pml_vflux, c->fscale, NfCapture{});
+                hipLaunchKernelGGL(kern, dim3(se.nb + c->geom.blocks), dim3(kSweepThreads), 0, c->stream, c->volt,
+                                   c->curr, cf, c->geom, c->pml, c->pml_coef, c->pml_vflux, c->fscale, NfCapture{},
+                                   se);
             }
             if (has_pml && !c->pml_fused)
                 hipLaunchKernelGGL(x16 ? sweep_e4ux : c->comp ? sweep_e4uc : sweep_e4u, dim3(c->pml.blocks),
                                    dim3(kSweepThreads), 0,
                                    c->stream, c->volt, c->curr, cf, c->geom, c->pml, c->pml_coef, c->pml_vflux,
-                                   c->fscale, NfCapture{});
+                                   c->fscale, NfCapture{}, SideWork{});
+            TRY(side_done(c));
         } else if (c->dbl) {
             auto kern = has_pml ? sweep_e1<true, double> : sweep_e1<false, double>;
             hipLaunchKernelGGL(kern, dim3(c->geom1.blocks), dim3(kSweepThreads), 0, c->stream, (double*)c->volt,
@@ -1257,6 +1334,12 @@ int launch_step(efs_ctx* c, bool next_follows) {
         if (c->dbl) launch_ade_update(c, fv64(c, 1), 1);
         else launch_ade_update(c, fv(c, 1), 1);
     }
+    // engine hooks of this step: in the H sweep's side work, or after it
+    const bool range_due = c->half && (t + 1) % c->range_interval == 0;
+    const bool gather_due = c->gb_n && (t + 1) % c->gb_interval == 0 && c->gb_count < c->gb_max;
+    const bool pre_ahead = c->fuse_hooks && next_follows && c->mur_n && !range_due;
+    const bool use_side = c->side_on && c->fuse_hooks && c->vec4 && (!has_pml || c->pml_split) && !c->dbl &&
+                          !range_due && (c->nf_P || gather_due || pre_ahead);
     {
         ProfScope s(c, EFS_PROF_SWEEP_H);
         if (c->vec4 && (!has_pml || c->pml_split)) {
@@ -1268,22 +1351,27 @@ int launch_step(efs_ctx* c, bool next_follows) {
             if (has_pml && c->pml_fused)
                 kern = c->pml_comp ? (x16 ? sweep_h4fxp : c->comp ? sweep_h4fcp : sweep_h4fp)
                                    : (x16 ? sweep_h4fx : c->comp ? sweep_h4fc : sweep_h4f);
+            float* ring = gather_due ? c->gb_ring + (size_t)c->gb_count * 2 * c->gb_n : nullptr;
+            const SideWork sh = use_side ? side_h(c, t, ring, pre_ahead) : SideWork{};
+            if (sh.nb) kern = side_twin(kern);
             if (c->half) {
                 auto k16 = has_pml ? (c->pml_comp ? (x16 ? sweep_h4fxp_h : c->comp ? sweep_h4fcp_h : sweep_h4fp_h)
                                                   : (x16 ? sweep_h4fx_h : c->comp ? sweep_h4fc_h : sweep_h4f_h))
                                    : (x16 ? sweep_h4x_h : c->comp ? sweep_h4c_h : sweep_h4_h);
-                hipLaunchKernelGGL(k16, dim3(c->geom.blocks), dim3(kSweepThreads), 0, c->stream, (f16*)c->curr,
-                                   (const f16*)c->volt, cf, c->geom, c->pml, c->pml_coef, c->pml_cflux, c->fscale,
-                                   nf_capture(c));
+                if (sh.nb) k16 = side_twin(k16);
+                hipLaunchKernelGGL(k16, dim3(sh.nb + c->geom.blocks), dim3(kSweepThreads), 0, c->stream,
+                                   (f16*)c->curr, (const f16*)c->volt, cf, c->geom, c->pml, c->pml_coef,
+                                   c->pml_cflux, c->fscale, nf_capture(c), sh);
             } else {
-                hipLaunchKernelGGL(kern, dim3(c->geom.blocks), dim3(kSweepThreads), 0, c->stream, c->curr, c->volt,
-                                   cf, c->geom, c->pml, c->pml_coef, c->pml_cflux, c->fscale, nf_capture(c));
+                hipLaunchKernelGGL(kern, dim3(sh.nb + c->geom.blocks), dim3(kSweepThreads), 0, c->stream, c->curr,
+                                   c->volt, cf, c->geom, c->pml, c->pml_coef, c->pml_cflux, c->fscale,
+                                   nf_capture(c), sh);
             }
             if (has_pml && !c->pml_fused)
                 hipLaunchKernelGGL(x16 ? sweep_h4ux : c->comp ? sweep_h4uc : sweep_h4u, dim3(c->pml.blocks),
                                    dim3(kSweepThreads), 0,
                                    c->stream, c->curr, c->volt, cf, c->geom, c->pml, c->pml_coef, c->pml_cflux,
-                                   c->fscale, NfCapture{});
+                                   c->fscale, NfCapture{}, SideWork{});
         } else if (c->dbl) {
             auto kern = has_pml ? sweep_h1<true, double> : sweep_h1<false, double>;
             hipLaunchKernelGGL(kern, dim3(c->geom1.blocks), dim3(kSweepThreads), 0, c->stream, (double*)c->curr,
@@ -1310,13 +1398,15 @@ int launch_step(efs_ctx* c, bool next_follows) {
     if (c->pad_hi_n)
         hipLaunchKernelGGL(k_pml_pad_sync, dim3(nblk(c->pad_hi_n, c->pt)), dim3(c->pt), 0, c->stream, fv(c, 1),
                            c->pml_cflux, c->pad_hi_g, c->pad_hi_f, c->pad_hi_n);
-    // engine hooks
-    const bool range_due = c->half && (t + 1) % c->range_interval == 0;
-    const bool gather_due = c->gb_n && (t + 1) % c->gb_interval == 0 && c->gb_count < c->gb_max;
+    if (use_side) {  // the hooks ran beside the H sweep
+        c->mur_pre_ahead = pre_ahead;
+        CHECK_LAUNCH(c);
+        c->numTS++;
+        return EFS_OK;
+    }
     // NF2FF sampling, the probe gather and the next step's Mur pre pass in
     // one launch; not when the FP16 range control (which rescales the
     // fields) comes between them
-    const bool pre_ahead = c->fuse_hooks && next_follows && c->mur_n && !range_due;
     const int parts = (c->nf_P ? 1 : 0) + (gather_due ? 1 : 0) + (pre_ahead ? 1 : 0);
     if (c->fuse_hooks && !range_due && parts >= 2) {
         if (c->nf_P && c->nf_slot == 0) c->nf_ts0 = t + 1;
@@ -1489,6 +1579,14 @@ int efs_init(efs_ctx* c, uint32_t nx, uint32_t ny, uint32_t nz, const float* vv,
     c->half = c->storage_req == EFS_STORAGE_F16;
     c->range_interval = (uint32_t)std::max<uint64_t>(1, env_u64("EFS_RANGE_INTERVAL", 32));
     c->fuse_hooks = env_u64("EFS_FUSE_HOOKS", 1) != 0;
+    {
+        // side work: 0 off, 1 on, 2 (default) on for grids of at most
+        // kSideMaxCells cells, whose steps are short enough for the saved
+        // launches to matter
+        const uint64_t m = env_u64("EFS_SIDE_WORK", 2);
+        c->side_on = m == 1 || (m == 2 && c->Np <= kSideMaxCells);
+    }
+    c->side_pend = false;
     if (c->half && !c->vec4) return fail(c, EFS_ERR_ARG, "FP16 field storage needs the 4-wide sweeps (EFS_VEC=4)");
     {
         const float one[6] = {1.f, 1.f, 1.f, 1.f, 1.f, 1.f};
@@ -2056,7 +2154,7 @@ int efs_iterate(efs_ctx* c, uint32_t n) {
     if (n) c->fields_zero = false;
     c->mur_pre_ahead = false;
     for (uint32_t s = 0; s < n; ++s) TRY(launch_step(c, s + 1 < n));
-    return EFS_OK;
+    return side_flush(c);  // no sample or gather stays half done between calls
 }
 
 int efs_sync(efs_ctx* c) {
@@ -2638,7 +2736,9 @@ int efs_path_flags(const efs_ctx* c, uint32_t* flags) {
              (upml4 && c->pml_fused ? EFS_PATH_FUSED : 0u) |
              (upml4 && c->pml_fused && c->pml_comp ? EFS_PATH_UPML_IDX : 0u) | (c->half ? EFS_PATH_F16 : 0u) |
              (nf_capture(c).on ? EFS_PATH_NF2FF_CAPTURE : 0u) | (c->comp16 ? EFS_PATH_COEF_IDX16 : 0u) |
-             (c->dbl ? EFS_PATH_F64 : 0u);
+             (c->dbl ? EFS_PATH_F64 : 0u) |
+             (c->side_on && c->fuse_hooks && c->vec4 && !c->dbl && (!c->pml.n || c->pml_split) ? EFS_PATH_SIDE_WORK
+                                                                                                : 0u);
     return EFS_OK;
 }
 

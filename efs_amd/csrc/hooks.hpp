@@ -128,4 +128,75 @@ __global__ void k_post_h(FV volt, FV curr, PostH<FV> q) {
     if (k < q.mur_n) mur_pre_one(volt, q.vflux, q.coeff, q.normal_src, q.shifted_src, q.saved, k);
 }
 
+// Point work run by extra workgroups of the sweep kernels (FP32 / FP16
+// storage).  Each part reads only arrays that the sweep of its kernel does
+// not write, so it runs beside the sweep, and the step loses the launches
+// (and the cache write-back and invalidate of each kernel boundary) of
+// k_post_h:
+//  - kernel of the H sweep of step t (HSIDE = true): the E half of the
+//    NF2FF sample of step t for the points that read the fields (E is final
+//    after the post-E hooks; the H sweep writes only H), the E values of the
+//    probe gather, and the Mur pre pass of step t+1;
+//  - kernel of the E sweep of step t+1 (or k_side_flush at the end of the
+//    iterate call): the H half of the sample of step t, both halves of
+//    captured points (from the slabs of step t's H sweep), and the H values
+//    of the gather (H is final after step t's post-H hooks; the E sweep
+//    writes only E).
+// The sample and gather rows are those of k_post_h, bit for bit.  The
+// workgroups come first in the grid and their count is a multiple of 8, so
+// the sweep's blocks keep their XCD placement.
+struct SideWork {
+    FieldView volt, curr;
+    // NF2FF sample, workgroups [0, nb_sample)
+    const uint32_t* pts;
+    const float* el;
+    Nf2ffGeom g;
+    float* row_out;
+    NfCapture cap;
+    uint32_t nb_sample;
+    // probe gather (E or H values of slot gout), the next nb_gather
+    const uint32_t* gidx;
+    float* gout;
+    uint32_t gn, nb_gather;
+    // the next step's Mur pre pass (H side only), the rest
+    const float* vflux;
+    const float* coeff;
+    const uint32_t *normal_src, *shifted_src;
+    float* saved;
+    uint32_t mur_n;
+    uint32_t nb;  // workgroups of side work at the start of the grid (0: none)
+};
+
+constexpr uint32_t kSideThreads = 256;  // the sweep kernels' workgroup size
+
+template <bool HSIDE>
+__device__ __forceinline__ void side_work(const SideWork& q, uint32_t b) {
+    if (b < q.nb_sample) {
+        const uint32_t p = xcd_remap(b, q.nb_sample) * kSideThreads + threadIdx.x;
+        nf2ff_sample_point<FieldView, HSIDE ? kSampleE : kSampleRest>(q.volt, q.curr, q.pts, q.el, q.g, q.row_out,
+                                                                       q.cap, p);
+        return;
+    }
+    b -= q.nb_sample;
+    if (b < q.nb_gather) {
+        const uint32_t k = b * kSideThreads + threadIdx.x;
+        if (k < q.gn) {
+            const uint32_t j = q.gidx[k];
+            if (HSIDE)
+                q.gout[2 * k] = q.volt.ld(j);
+            else
+                q.gout[2 * k + 1] = q.curr.ld(j);
+        }
+        return;
+    }
+    b -= q.nb_gather;
+    if (HSIDE) {
+        const uint32_t k = b * kSideThreads + threadIdx.x;
+        if (k < q.mur_n) mur_pre_one(q.volt, q.vflux, q.coeff, q.normal_src, q.shifted_src, q.saved, k);
+    }
+}
+
+// The E-sweep side of the last step of an iterate call, on its own.
+__global__ __launch_bounds__(kSideThreads) void k_side_flush(SideWork q) { side_work<false>(q, blockIdx.x); }
+
 }  // namespace efs

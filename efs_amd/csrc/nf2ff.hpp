@@ -44,11 +44,12 @@ struct Nf2ffGeom {
 // as in the NF2FF accumulate WGSL.  V(c, dx, dy, dz) returns volt component
 // c at (ix+dx, iy+dy, iz+dz); I(n, d) returns curr component n at the point
 // or (d = 1) one cell further along axis n.  One body for both sources of
-// the values keeps their arithmetic identical.
+// the values keeps their arithmetic identical; the E and H halves are
+// separate so that they can run at different places in the step.
 // R: arithmetic type (float; double for FP64 field storage).
-template <class R, class VA, class IA>
-__device__ __forceinline__ void nf_point(uint32_t ix, uint32_t iy, uint32_t iz, const float* __restrict__ el,
-                                         const Nf2ffGeom& g, VA V, IA I, R out[6]) {
+template <class R, class VA>
+__device__ __forceinline__ void nf_point_e(uint32_t ix, uint32_t iy, uint32_t iz, const float* __restrict__ el,
+                                           const Nf2ffGeom& g, VA V, R out[3]) {
     // no contraction: the same bits whichever accessor supplies the values
 #pragma clang fp contract(off)
     R E0 = 0, E1 = 0, E2 = 0;
@@ -57,11 +58,17 @@ __device__ __forceinline__ void nf_point(uint32_t ix, uint32_t iy, uint32_t iz, 
         E1 = (V(1, 0, 0, 0) + V(1, 0, 0, 1) + V(1, 1, 0, 1) + V(1, 1, 0, 0)) / ((R)4 * (R)el[g.el_off[1] + iy]);
         E2 = (V(2, 0, 0, 0) + V(2, 1, 0, 0) + V(2, 1, 1, 0) + V(2, 0, 1, 0)) / ((R)4 * (R)el[g.el_off[2] + iz]);
     }
-    const uint32_t pos[3] = {ix, iy, iz};
-    const uint32_t lim[3] = {g.nx, g.ny, g.nz};
     out[0] = E0;
     out[1] = E1;
     out[2] = E2;
+}
+
+template <class R, class IA>
+__device__ __forceinline__ void nf_point_h(uint32_t ix, uint32_t iy, uint32_t iz, const float* __restrict__ el,
+                                           const Nf2ffGeom& g, IA I, R out[3]) {
+#pragma clang fp contract(off)
+    const uint32_t pos[3] = {ix, iy, iz};
+    const uint32_t lim[3] = {g.nx, g.ny, g.nz};
 #pragma unroll
     for (int n = 0; n < 3; ++n) {
         R h = 0;
@@ -75,13 +82,27 @@ __device__ __forceinline__ void nf_point(uint32_t ix, uint32_t iy, uint32_t iz, 
                 h = h * ((R)1 - dr) + up * dr;
             }
         }
-        out[3 + n] = h;
+        out[n] = h;
     }
+}
+
+template <class R, class VA, class IA>
+__device__ __forceinline__ void nf_point(uint32_t ix, uint32_t iy, uint32_t iz, const float* __restrict__ el,
+                                         const Nf2ffGeom& g, VA V, IA I, R out[6]) {
+    nf_point_e<R>(ix, iy, iz, el, g, V, out);
+    nf_point_h<R>(ix, iy, iz, el, g, I, out + 3);
 }
 
 // One thread per surface point.  Points on a captured z plane (see
 // NfCapture) read the slabs the H sweep wrote; the others read the fields.
-template <class FV>
+// PART: kSampleAll, the whole sample of the point; kSampleE, the E half
+// of a point that reads the fields (captured points: nothing), from the
+// kernel of the H sweep, which does not write E; kSampleRest, the H half
+// of such a point and both halves of a captured one, after the H sweep
+// wrote H and the slabs.  The three give the same rows.
+constexpr int kSampleAll = 0, kSampleE = 1, kSampleRest = 2;
+
+template <class FV, int PART = kSampleAll>
 __device__ __forceinline__ void nf2ff_sample_point(const FV& volt, const FV& curr, const uint32_t* __restrict__ pts,
                                                    const float* __restrict__ el, const Nf2ffGeom& g,
                                                    typename FV::R* __restrict__ row_out, const NfCapture& cap,
@@ -93,7 +114,9 @@ __device__ __forceinline__ void nf2ff_sample_point(const FV& volt, const FV& cur
     int f = -1;
     if (cap.on && ix - cap.x0 < cap.x1 - cap.x0 && iy - cap.y0 < cap.y1 - cap.y0)
         f = iz == cap.z[0] ? 0 : (cap.nf > 1 && iz == cap.z[1] ? 1 : -1);
+    const size_t P = g.P, RE = g.rowsE;
     if (f >= 0) {
+        if (PART == kSampleE) return;
         const float* q = cap.slab + ((size_t)(f * cap.nx_c + (ix - cap.x0)) * cap.ny_c + (iy - cap.y0)) * kCapFloats;
         const size_t rx = (size_t)cap.ny_c * kCapFloats;  // next x row
         auto V = [&](int c, int dx, int dy, int dz) { return (R)q[dx * rx + dy * kCapFloats + 2 * c + dz]; };
@@ -108,9 +131,21 @@ __device__ __forceinline__ void nf2ff_sample_point(const FV& volt, const FV& cur
         const size_t step[3] = {sx, sy, 1};
         auto V = [&](int c, int dx, int dy, int dz) { return (R)volt.ldc(c, b + dx * sx + dy * sy + dz); };
         auto I = [&](int n, int d) { return (R)curr.ldc(n, b + (d ? step[n] : 0)); };
-        nf_point<R>(ix, iy, iz, el, g, V, I, o);
+        if (PART != kSampleRest) nf_point_e<R>(ix, iy, iz, el, g, V, o);
+        if (PART != kSampleE) nf_point_h<R>(ix, iy, iz, el, g, I, o + 3);
+        if (PART == kSampleE) {
+            row_out[p] = o[0];
+            row_out[P + p] = o[1];
+            row_out[2 * P + p] = o[2];
+            return;
+        }
+        if (PART == kSampleRest) {
+            row_out[RE + p] = o[3];
+            row_out[RE + P + p] = o[4];
+            row_out[RE + 2 * P + p] = o[5];
+            return;
+        }
     }
-    const size_t P = g.P, RE = g.rowsE;
     row_out[p] = o[0];
     row_out[P + p] = o[1];
     row_out[2 * P + p] = o[2];

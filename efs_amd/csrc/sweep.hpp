@@ -933,11 +933,12 @@ __device__ __forceinline__ void h4_march(T* __restrict__ curr, const T* __restri
 }
 
 // Interior launch: grid tiling of the (y, z) plane, 1024 cells per workgroup.
+// bid: the sweep block (the workgroup index after any side-work groups).
 template <bool HAS_PML>
 // flags: the block's flags (uniform over the workgroup, set on every path).
 __device__ __forceinline__ bool interior_lane(const SweepGeom& g, const PmlTable& pml, Lane4& w, uint32_t& m,
-                                              uint32_t& flags) {
-    const SweepBlock d = g.tab[xcd_remap(blockIdx.x, g.blocks)];
+                                              uint32_t& flags, uint32_t bid) {
+    const SweepBlock d = g.tab[xcd_remap(bid, g.blocks)];
     flags = d.flags;
     m = 0u;
     const uint32_t c0 = (d.tile * kSweepThreads + threadIdx.x) * 4u;
@@ -972,30 +973,59 @@ __device__ __forceinline__ bool pml_lane(const PmlTable& pml, Lane4& w) {
     return w.x0 < w.x1;
 }
 
+// side: point work of extra workgroups at the start of the grid (SideWork,
+// defined with the point kernels).  Each grid entry point comes in two
+// kernels: NAME, without side work (the code of the sweep alone), and
+// NAME_s, whose first side.nb workgroups run side_work (their extra
+// branch costs the fused FP32 UPML H sweeps a spill, so grids that bring
+// no side work launch NAME).
 #define EFS_SWEEP4_PARAMS(T)                                                                                    \
     T *__restrict__ dst, const T *__restrict__ src, Coef cf, SweepGeom g, PmlTable pml,                        \
-        const float *__restrict__ pcoef, float *__restrict__ pflux, const float *__restrict__ fscale, NfCapture cap
+        const float *__restrict__ pcoef, float *__restrict__ pflux, const float *__restrict__ fscale, NfCapture cap, \
+        SideWork side
+#define EFS_SWEEP4_ARGS dst, src, cf, g, pml, pcoef, pflux, fscale, cap, side
+#define EFS_SWEEP4_SIDE(HF)                      \
+    uint32_t bid = blockIdx.x;                   \
+    if (SIDE) {                                  \
+        if (blockIdx.x < side.nb) {              \
+            side_work<HF>(side, blockIdx.x);     \
+            return;                              \
+        }                                        \
+        bid -= side.nb;                          \
+    }
+// the two kernels of a body NAME##_body<SIDE>
+#define EFS_SWEEP4_KERNELS(NAME, T, WAVES)                                                                         \
+    __global__ __launch_bounds__(kSweepThreads) __attribute__((amdgpu_waves_per_eu(WAVES, WAVES))) void NAME(     \
+        EFS_SWEEP4_PARAMS(T)) {                                                                                    \
+        NAME##_body<false>(EFS_SWEEP4_ARGS);                                                                       \
+    }                                                                                                              \
+    __global__ __launch_bounds__(kSweepThreads) __attribute__((amdgpu_waves_per_eu(WAVES, WAVES))) void NAME##_s( \
+        EFS_SWEEP4_PARAMS(T)) {                                                                                    \
+        NAME##_body<true>(EFS_SWEEP4_ARGS);                                                                        \
+    }
 
 // Interior (4 waves/SIMD) and UPML (3 waves/SIMD) entry points, each with
 // FP32 arrays or table-compressed (suffix c) coefficients; the interior
 // kernels with suffix m skip the UPML regions.  FP32 field storage.
 #define EFS_SWEEP4_INTERIOR(NAME, MARCH, HF, HAS_PML, COMP)                                                       \
-    __global__ __launch_bounds__(kSweepThreads)                                                                    \
-        __attribute__((amdgpu_waves_per_eu(EFS_SWEEP_WAVES, EFS_SWEEP_WAVES))) void NAME(EFS_SWEEP4_PARAMS(float)) { \
-        __shared__ float2 s_tab[COMP == kCoefIdx8 ? 3 * kTab : 1];                                                              \
+    template <bool SIDE>                                                                                           \
+    __device__ __forceinline__ void NAME##_body(EFS_SWEEP4_PARAMS(float)) {                                        \
+        EFS_SWEEP4_SIDE(HF)                                                                                        \
+        __shared__ float2 s_tab[COMP == kCoefIdx8 ? 3 * kTab : 1];                                                 \
         coef_stage<COMP>(cf, s_tab);                                                                               \
         Lane4 w;                                                                                                   \
         uint32_t m, flags;                                                                                         \
-        if (!interior_lane<HAS_PML>(g, pml, w, m, flags)) return;                                                  \
+        if (!interior_lane<HAS_PML>(g, pml, w, m, flags, bid)) return;                                             \
         const Scales sc = load_scales<float>(fscale, HF);                                                          \
         MARCH<kMarchInterior, COMP, false, float>(dst, src, cf, s_tab, nullptr, g, pml, pcoef, pflux, w, m, sc,    \
-                                                  cap);                                                         \
-    }
+                                                  cap);                                                            \
+    }                                                                                                              \
+    EFS_SWEEP4_KERNELS(NAME, float, EFS_SWEEP_WAVES)
 
 #define EFS_SWEEP4_UPML(NAME, MARCH, HF, COMP)                                                                     \
     __global__ __launch_bounds__(kSweepThreads) __attribute__((                                                    \
         amdgpu_waves_per_eu(EFS_SWEEP_WAVES_PML, EFS_SWEEP_WAVES_PML))) void NAME(EFS_SWEEP4_PARAMS(float)) {     \
-        __shared__ float2 s_tab[COMP == kCoefIdx8 ? 3 * kTab : 1];                                                              \
+        __shared__ float2 s_tab[COMP == kCoefIdx8 ? 3 * kTab : 1];                                                 \
         coef_stage<COMP>(cf, s_tab);                                                                               \
         Lane4 w;                                                                                                   \
         if (!pml_lane(pml, w)) return;                                                                             \
@@ -1008,32 +1038,36 @@ __device__ __forceinline__ bool pml_lane(const PmlTable& pml, Lane4& w) {
 // register budget); PCOMP: UPML coefficients as byte indices + LDS table;
 // T: field storage (float or f16).
 #define EFS_SWEEP4_FUSED(NAME, MARCH, HF, COMP, PCOMP, T)                                                          \
-    __global__ __launch_bounds__(kSweepThreads) __attribute__((                                                    \
-        amdgpu_waves_per_eu(EFS_SWEEP_WAVES_PML, EFS_SWEEP_WAVES_PML))) void NAME(EFS_SWEEP4_PARAMS(T)) {         \
-        __shared__ float2 s_tab[COMP == kCoefIdx8 ? 3 * kTab : 1];                                                              \
+    template <bool SIDE>                                                                                           \
+    __device__ __forceinline__ void NAME##_body(EFS_SWEEP4_PARAMS(T)) {                                            \
+        EFS_SWEEP4_SIDE(HF)                                                                                        \
+        __shared__ float2 s_tab[COMP == kCoefIdx8 ? 3 * kTab : 1];                                                 \
         __shared__ float4 s_ptab[PCOMP ? 3 * kTab : 1];                                                            \
         coef_stage<COMP>(cf, s_tab);                                                                               \
         Lane4 w;                                                                                                   \
         uint32_t m, flags;                                                                                         \
-        const bool live = interior_lane<true>(g, pml, w, m, flags);                                                \
+        const bool live = interior_lane<true>(g, pml, w, m, flags, bid);                                           \
         upml_stage<PCOMP>(cf, s_ptab, flags);                                                                      \
         if (!live) return;                                                                                         \
         const Scales sc = load_scales<T>(fscale, HF);                                                              \
         MARCH<kMarchFused, COMP, PCOMP, T>(dst, src, cf, s_tab, s_ptab, g, pml, pcoef, pflux, w, m, sc, cap);      \
-    }
+    }                                                                                                              \
+    EFS_SWEEP4_KERNELS(NAME, T, EFS_SWEEP_WAVES_PML)
 
 // Grid launch without UPML, FP16 field storage (FP32 storage uses the
 // interior kernels above, which are the same code for HAS_PML = false).
 #define EFS_SWEEP4_PLAIN16(NAME, MARCH, HF, COMP)                                                                  \
-    __global__ __launch_bounds__(kSweepThreads)                                                                    \
-        __attribute__((amdgpu_waves_per_eu(EFS_SWEEP_WAVES, EFS_SWEEP_WAVES))) void NAME(EFS_SWEEP4_PARAMS(f16)) { \
-        __shared__ float2 s_tab[COMP == kCoefIdx8 ? 3 * kTab : 1];                                                              \
+    template <bool SIDE>                                                                                           \
+    __device__ __forceinline__ void NAME##_body(EFS_SWEEP4_PARAMS(f16)) {                                          \
+        EFS_SWEEP4_SIDE(HF)                                                                                        \
+        __shared__ float2 s_tab[COMP == kCoefIdx8 ? 3 * kTab : 1];                                                 \
         coef_stage<COMP>(cf, s_tab);                                                                               \
         Lane4 w;                                                                                                   \
         uint32_t m, flags;                                                                                         \
-        if (!interior_lane<false>(g, pml, w, m, flags)) return;                                                    \
+        if (!interior_lane<false>(g, pml, w, m, flags, bid)) return;                                               \
         const Scales sc = load_scales<f16>(fscale, HF);                                                            \
         MARCH<kMarchInterior, COMP, false, f16>(dst, src, cf, s_tab, nullptr, g, pml, pcoef, pflux, w, m, sc, cap); \
-    }
+    }                                                                                                              \
+    EFS_SWEEP4_KERNELS(NAME, f16, EFS_SWEEP_WAVES)
 
 }  // namespace efs

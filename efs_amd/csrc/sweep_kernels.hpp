@@ -2,6 +2,7 @@
 // of the kernel macros of sweep.hpp).  Included by exactly one translation
 // unit (efs_engine.hip): the kernels have external linkage.
 #pragma once
+#include "hooks.hpp"  // SideWork, side_work
 #include "sweep.hpp"
 
 namespace efs {
@@ -55,10 +56,46 @@ EFS_SWEEP4_PLAIN16(sweep_h4x_h, h4_march, true, kCoefIdx16)
 EFS_SWEEP4_FUSED(sweep_h4fx_h, h4_march, true, kCoefIdx16, false, f16)
 EFS_SWEEP4_FUSED(sweep_h4fxp_h, h4_march, true, kCoefIdx16, true, f16)
 
+// Each grid kernel and its side-work twin (NAME_s).
+template <typename T>
+using SweepFn = void (*)(EFS_SWEEP4_PARAMS(T));
+template <typename T>
+struct SweepPair {
+    SweepFn<T> plain, side;
+};
+#define EFS_PAIR(N) {N, N##_s}
+inline const SweepPair<float> kSweepPairs32[] = {
+    EFS_PAIR(sweep_e4),   EFS_PAIR(sweep_e4c),  EFS_PAIR(sweep_e4m),  EFS_PAIR(sweep_e4mc), EFS_PAIR(sweep_e4f),
+    EFS_PAIR(sweep_e4fc), EFS_PAIR(sweep_e4fp), EFS_PAIR(sweep_e4fcp), EFS_PAIR(sweep_e4x), EFS_PAIR(sweep_e4mx),
+    EFS_PAIR(sweep_e4fx), EFS_PAIR(sweep_e4fxp), EFS_PAIR(sweep_h4),  EFS_PAIR(sweep_h4c),  EFS_PAIR(sweep_h4m),
+    EFS_PAIR(sweep_h4mc), EFS_PAIR(sweep_h4f),  EFS_PAIR(sweep_h4fc), EFS_PAIR(sweep_h4fp), EFS_PAIR(sweep_h4fcp),
+    EFS_PAIR(sweep_h4x),  EFS_PAIR(sweep_h4mx), EFS_PAIR(sweep_h4fx), EFS_PAIR(sweep_h4fxp)};
+inline const SweepPair<f16> kSweepPairs16[] = {
+    EFS_PAIR(sweep_e4_h),   EFS_PAIR(sweep_e4c_h),  EFS_PAIR(sweep_e4f_h),  EFS_PAIR(sweep_e4fc_h),
+    EFS_PAIR(sweep_e4fp_h), EFS_PAIR(sweep_e4fcp_h), EFS_PAIR(sweep_e4x_h), EFS_PAIR(sweep_e4fx_h),
+    EFS_PAIR(sweep_e4fxp_h), EFS_PAIR(sweep_h4_h),  EFS_PAIR(sweep_h4c_h),  EFS_PAIR(sweep_h4f_h),
+    EFS_PAIR(sweep_h4fc_h), EFS_PAIR(sweep_h4fp_h), EFS_PAIR(sweep_h4fcp_h), EFS_PAIR(sweep_h4x_h),
+    EFS_PAIR(sweep_h4fx_h), EFS_PAIR(sweep_h4fxp_h)};
+#undef EFS_PAIR
+
+// The twin of k (the kernel to launch when the grid carries side work), or
+// null when k has none.
+template <typename T, size_t N>
+SweepFn<T> side_twin(SweepFn<T> k, const SweepPair<T> (&tab)[N]) {
+    for (const SweepPair<T>& p : tab)
+        if (p.plain == k) return p.side;
+    return nullptr;
+}
+inline SweepFn<float> side_twin(SweepFn<float> k) { return side_twin(k, kSweepPairs32); }
+inline SweepFn<f16> side_twin(SweepFn<f16> k) { return side_twin(k, kSweepPairs16); }
+
 #undef EFS_SWEEP4_INTERIOR
 #undef EFS_SWEEP4_UPML
 #undef EFS_SWEEP4_FUSED
 #undef EFS_SWEEP4_PLAIN16
 #undef EFS_SWEEP4_PARAMS
+#undef EFS_SWEEP4_SIDE
+#undef EFS_SWEEP4_ARGS
+#undef EFS_SWEEP4_KERNELS
 
 }  // namespace efs

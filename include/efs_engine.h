@@ -275,7 +275,9 @@ enum {
     EFS_PATH_F16 = 32,      /* FP16 field storage */
     EFS_PATH_NF2FF_CAPTURE = 64, /* NF2FF z-face points read slabs the H sweep wrote */
     EFS_PATH_COEF_IDX16 = 128,   /* core coefficients as 16-bit indices + cached tables */
-    EFS_PATH_F64 = 256           /* FP64 field storage and arithmetic (1-wide sweeps) */
+    EFS_PATH_F64 = 256,          /* FP64 field storage and arithmetic (1-wide sweeps) */
+    EFS_PATH_SIDE_WORK = 512     /* NF2FF sampling, probe gather and Mur pre pass run in extra
+                                    workgroups of the sweep kernels (EFS_SIDE_WORK) */
 };
 int efs_path_flags(const efs_ctx *ctx, uint32_t *flags);
 

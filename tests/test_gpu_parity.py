@@ -470,6 +470,52 @@ def test_nf2ff_zface_capture(HIPEngine, oracle_mod, monkeypatch, storage):
             assert rel_l2(got, want) <= 2e-5
 
 
+@pytest.mark.parametrize("storage,pml", [("fp32", 4), ("fp16", 4), ("fp32", 0)])
+def test_side_work_same_bits(HIPEngine, monkeypatch, storage, pml):
+    """NF2FF sampling, the buffered probe gather and the Mur pre pass run by
+    extra workgroups of the sweep kernels (the E half of the sample beside
+    the H sweep, the H half beside the next E sweep or in the flush at the
+    end of an iterate call) give the fields, accumulators and gather samples
+    of the separate point launches bit for bit: z-face capture, Mur, FP16
+    range control every 5 steps, gathers every step and every 3rd step,
+    iterate calls of 1 and of several steps."""
+    dims = (30, 28, 40)
+    case = random_case(41, dims, pml_width=pml, mur=True)
+    pts, primal, dual = nf2ff_setup(dims, inset=5)
+    dT = 1.9e-12
+    omegas = 2 * np.pi * np.array([1.0e9, 3.3e9, 7.0e9])
+    cfg = {"surfaceIndices": pts, "omega": list(omegas), "dT": dT, "primalEdgeLens": primal,
+           "dualEdgeLens": dual, "kBlock": 8}
+    N = int(np.prod(dims))
+    probes = np.array([5, 7 * 40 + 3, N + 1234, 2 * N + 999, 17 * 28 * 40 + 11 * 40 + 20], np.uint32)
+    monkeypatch.setenv("EFS_RANGE_INTERVAL", "5")
+    out = {}
+    for mode in ("1", "0"):
+        monkeypatch.setenv("EFS_SIDE_WORK", mode)
+        say("case side work", mode, storage, pml)
+        res = []
+        for interval in (1, 3):
+            g = HIPEngine()
+            assert g.initGPU()
+            g.init(case["dims"], case["coeffs"], storage=storage)
+            configure(g, case)
+            g.configureNF2FFAccumulation(cfg)
+            g.configureProbeGatherBuffered(probes, 64, interval)
+            assert ("side_work" in g.pathFlags()) == (mode == "1")
+            for n in (13, 1, 1, 30, 2):
+                g.iterate(n)
+            f = g.getFields()
+            acc = g.readNF2FFAccumulation()
+            gb = g.readProbeGatherBuffered()
+            res += [np.asarray(f["volt"]), np.asarray(f["curr"]), np.asarray(acc["accumE"]),
+                    np.asarray(acc["accumH"]), np.asarray(gb["data"]), np.array([gb["numSamples"]])]
+            g.destroy()
+        out[mode] = res
+    for a, b in zip(out["1"], out["0"]):
+        assert np.all(np.isfinite(a))
+        np.testing.assert_array_equal(a, b)
+
+
 def test_farfield_matches_fp64_oracle(HIPEngine, oracle_mod):
     dims = (14, 13, 15)
     case = random_case(22, dims, pml_width=2)
