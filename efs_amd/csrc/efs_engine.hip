@@ -183,6 +183,10 @@ struct efs_ctx {
     uint2* mur_b = nullptr;         // deferred Mur entries: (normal cell, entry)
     uint32_t* exc_b = nullptr;
     uint32_t mur_nb = 0, exc_nb = 0;
+    // group form of k_post_e (EFS_POST_GROUP, default 1): the early and
+    // deferred entries done by its first grp_n workgroups, no k_post_e_b
+    uint32_t grp_n = 0;
+    uint32_t *grp_mur = nullptr, *grp_exc = nullptr, *grp_eoff = nullptr;
     bool mur_pre_ahead = false;     // the previous step's k_post_h ran this step's Mur pre pass
     // point work in extra workgroups of the sweep kernels (SideWork):
     // side_on from EFS_SIDE_WORK; side_e: the E-sweep side of
@@ -569,7 +573,9 @@ uint32_t alias_of(const efs_ctx* c, uint32_t gidx) {
 // deferred applies in phase B; an excitation entry on a deferred cell adds
 // in phase B, one on a Mur normal cell is dropped (the apply overwrites
 // it).  Without ADE write-back, "early" is exactly: shifted cell = some
-// normal or excitation cell.
+// normal or excitation cell.  Group form: the early and the deferred Mur
+// entries and the deferred excitation entries go to k_post_e's first
+// workgroup instead (phase A skips them all).
 int plan_post_e(efs_ctx* c, const std::vector<uint8_t>& early) {
     c->post_e_ok = false;
     if (!c->fuse_hooks) return EFS_OK;
@@ -592,6 +598,77 @@ int plan_post_e(efs_ctx* c, const std::vector<uint8_t>& early) {
         if (on_normal || in(sh_early, c->h_exc_gidx[k])) {
             edef[k] = 1;
             if (!on_normal) eb.push_back(k);
+        }
+    }
+    c->grp_n = 0;
+    if (env_u64("EFS_POST_GROUP", 1) != 0 && !(mb.empty() && eb.empty())) {
+        // linked sets: an early entry with the Mur entry and the deferred
+        // excitation entries that write its shifted cell (nodes: Mur
+        // entries, then excitation entries)
+        const uint32_t M = c->mur_n, E = c->exc_n;
+        std::vector<uint32_t> parent(M + E);
+        for (uint32_t a = 0; a < M + E; ++a) parent[a] = a;
+        auto root = [&](uint32_t a) {
+            while (parent[a] != a) a = parent[a] = parent[parent[a]];
+            return a;
+        };
+        std::vector<std::pair<uint32_t, uint32_t>> by_normal(M), by_exc;
+        for (uint32_t k = 0; k < M; ++k) by_normal[k] = {c->h_mur_normal[k], k};
+        std::sort(by_normal.begin(), by_normal.end());
+        for (uint32_t k : eb) by_exc.push_back({c->h_exc_gidx[k], k});
+        std::sort(by_exc.begin(), by_exc.end());
+        std::vector<uint8_t> in_grp(M + E, 0);
+        for (uint32_t k = 0; k < M; ++k) {
+            if (early[k] || mdef[k]) in_grp[k] = 1;
+            if (!early[k]) continue;
+            const uint32_t cell = c->h_mur_shifted[k];
+            auto m = std::lower_bound(by_normal.begin(), by_normal.end(), std::make_pair(cell, 0u));
+            if (m != by_normal.end() && m->first == cell) parent[root(m->second)] = root(k);
+            for (auto e = std::lower_bound(by_exc.begin(), by_exc.end(), std::make_pair(cell, 0u));
+                 e != by_exc.end() && e->first == cell; ++e)
+                parent[root(M + e->second)] = root(k);
+        }
+        for (uint32_t k : eb) in_grp[M + k] = 1;
+        std::vector<std::vector<uint32_t>> sets(M + E);
+        for (uint32_t a = 0; a < M + E; ++a)
+            if (in_grp[a]) sets[root(a)].push_back(a);
+        // pack whole sets into groups of at most kPostThreads Mur entries
+        std::vector<uint32_t> gm, ge, eoff{0};
+        uint32_t fill = 0;  // Mur entries of the open group
+        bool open = false, ok = true;
+        auto close = [&] {
+            gm.resize(gm.size() + (kPostThreads - fill), ~0u);
+            eoff.push_back((uint32_t)ge.size());
+        };
+        for (const auto& st : sets) {
+            if (st.empty()) continue;
+            uint32_t nm = 0;
+            for (uint32_t a : st) nm += a < M;
+            if (nm > (uint32_t)kPostThreads) {
+                ok = false;
+                break;
+            }
+            if (!open || fill + nm > (uint32_t)kPostThreads) {
+                if (open) close();
+                open = true;
+                fill = 0;
+            }
+            for (uint32_t a : st) {
+                if (a < M) gm.push_back(a);
+                else ge.push_back(a - M);
+            }
+            fill += nm;
+        }
+        if (ok && open) {
+            close();
+            c->grp_n = (uint32_t)(gm.size() / kPostThreads);
+            for (uint32_t k = 0; k < M; ++k)
+                if (in_grp[k]) mdef[k] = 1;
+            TRY(dupload(c, c->grp_mur, gm.data(), gm.size()));
+            if (!ge.empty()) TRY(dupload(c, c->grp_exc, ge.data(), ge.size()));
+            TRY(dupload(c, c->grp_eoff, eoff.data(), eoff.size()));
+            mb.clear();
+            eb.clear();
         }
     }
     TRY(dupload(c, c->mur_defer, mdef.data(), mdef.size()));
@@ -766,8 +843,9 @@ void free_all(efs_ctx* c) {
     dfree(c->mur_coeff); dfree(c->mur_saved); dfree(c->mur_normal); dfree(c->mur_shifted);
     dfree(c->mur_early); dfree(c->mur_early_list);
     dfree(c->mur_defer); dfree(c->exc_defer); dfree(c->mur_b); dfree(c->exc_b);
+    dfree(c->grp_mur); dfree(c->grp_exc); dfree(c->grp_eoff);
     c->post_e_ok = false;
-    c->mur_nb = c->exc_nb = 0;
+    c->mur_nb = c->exc_nb = c->grp_n = 0;
     dfree(c->mur_normal_src); dfree(c->mur_shifted_src);
     dfree(c->ade_gidx); dfree(c->ade_src_v); dfree(c->ade_src_i); dfree(c->ade_lor);
     for (auto& p : c->ade_c) dfree(p);
@@ -1012,8 +1090,12 @@ void launch_post_e(efs_ctx* c, FV v, uint32_t t) {
     q.mur_defer = c->mur_defer;
     q.mur_b = c->mur_b;
     q.mur_nb = c->mur_nb;
-    hipLaunchKernelGGL(k_post_e<FV>, dim3(nblk((size_t)c->mur_n + c->exc_n, kPostThreads)), dim3(kPostThreads), 0,
-                       c->stream, v, q);
+    q.grp = c->grp_n;
+    q.grp_mur = c->grp_mur;
+    q.grp_exc = c->grp_exc;
+    q.grp_eoff = c->grp_eoff;
+    hipLaunchKernelGGL(k_post_e<FV>, dim3(q.grp + nblk((size_t)c->mur_n + c->exc_n, kPostThreads)),
+                       dim3(kPostThreads), 0, c->stream, v, q);
     if (c->exc_nb + c->mur_nb)
         hipLaunchKernelGGL(k_post_e_b<FV>, dim3(nblk((size_t)c->exc_nb + c->mur_nb, kPostThreads)),
                            dim3(kPostThreads), 0, c->stream, v, q);

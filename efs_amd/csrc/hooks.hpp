@@ -26,6 +26,17 @@
 // for the deferred Mur entries; the two lists touch disjoint cells.  The
 // host builds the lists; a box of Mur faces defers its edges.
 //
+// Group form (default): the first grp workgroups of k_post_e do the early
+// and the deferred Mur entries and the deferred excitation entries, and
+// there is no phase B launch.  The host links each early entry with the
+// entries that write its shifted cell and packs the linked sets whole into
+// groups of at most kPostThreads Mur entries (one per thread).  A group
+// reads the shifted cells of its Mur entries, waits at a workgroup
+// barrier, then writes their normal cells and adds its excitation entries.
+// Nothing outside a group reads or writes those cells: an entry reading a
+// cell that another entry writes is early, so the writer's cell is
+// deferred and both are in the same group.
+//
 // After the H sweep (k_post_h): NF2FF sampling of the step, the probe
 // gather and the next step's Mur pre pass.  All three only read the
 // fields (the pre pass reads E, which nothing writes between the H sweep
@@ -57,16 +68,32 @@ struct PostE {
     const uint32_t *normal, *shifted;
     R* saved;
     uint32_t mur_n;
-    const uint8_t* mur_defer;   // 1: phase B applies saved[k]
+    const uint8_t* mur_defer;   // 1: phase B (or the group) applies it
     const uint2* mur_b;         // deferred Mur entries: (normal cell, k)
     uint32_t mur_nb;
+    // group form: Mur entry of each thread of group g at grp_mur[g * 256 +
+    // thread] (~0u: none), excitation entries grp_exc[grp_eoff[g] ..
+    // grp_eoff[g + 1])
+    const uint32_t *grp_mur, *grp_exc, *grp_eoff;
+    uint32_t grp;               // workgroups of the group form (0: none)
 };
 
 template <class FV>
 __global__ __launch_bounds__(kPostThreads) void k_post_e(FV volt, PostE<typename FV::R> q) {
     typedef typename FV::R R;
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (blockIdx.x < q.grp) {
+        const uint32_t k = q.grp_mur[blockIdx.x * kPostThreads + threadIdx.x];
+        R s = 0;
+        if (k != ~0u) s = fma1((R)q.coeff[k], volt.ld(q.shifted[k]), q.saved[k]);
+        __syncthreads();  // every read above precedes every write below
+        if (k != ~0u) volt.st(q.normal[k], s);
+        for (uint32_t j = q.grp_eoff[blockIdx.x] + threadIdx.x; j < q.grp_eoff[blockIdx.x + 1]; j += kPostThreads)
+            excitation_one(volt, q.signal, q.len, q.period, q.delay, q.amp, q.gidx, q.grp_exc[j], q.numTS);
+        return;
+    }
+    const uint32_t i = (blockIdx.x - q.grp) * blockDim.x + threadIdx.x;
     if (i < q.mur_n) {
+        if (q.grp && q.mur_defer[i]) return;  // the group's entry (it reads saved[i])
         const R s = fma1((R)q.coeff[i], volt.ld(q.shifted[i]), q.saved[i]);
         if (q.mur_defer[i]) q.saved[i] = s;
         else volt.st(q.normal[i], s);

@@ -6,7 +6,8 @@ The bar is the same bits (fields, buffered gather, NF2FF accumulators):
 the fused launches keep the reference's phase order for every cell that
 two operations share.  The Mur box case puts excitation entries on Mur
 normal cells and on a cell that is both normal and shifted cell, which
-exercises the deferred lists (phase B); iterate calls of 1, 7, 64 and 128
+exercises the deferred lists (phase B, or the first workgroup of k_post_e
+in the group form, EFS_POST_GROUP); iterate calls of 1, 7, 64 and 128
 steps check that the Mur pre pass runs exactly once per step across call
 boundaries.  The fused path is also held to the FP64 oracle (1e-5)."""
 import numpy as np
@@ -82,15 +83,17 @@ def run_case(eng, case, storage="fp32", chunks=CHUNKS):
     return snaps, g["data"].copy(), np.concatenate([acc["accumE"].ravel(), acc["accumH"].ravel()])
 
 
-@pytest.mark.parametrize("storage", ["fp32", "fp16", "fp64"])
-def test_fused_hooks_match_separate_kernels(HIPEngine, monkeypatch, storage):
+@pytest.mark.parametrize("storage,group", [("fp32", "1"), ("fp16", "1"), ("fp64", "1"), ("fp32", "0"),
+                                           ("fp16", "0")])
+def test_fused_hooks_match_separate_kernels(HIPEngine, monkeypatch, storage, group):
     case = mur_box_case()
-    say("case mur box", storage)
+    say("case mur box", storage, group)
     g = HIPEngine()
     assert g.initGPU()
     monkeypatch.setenv("EFS_FUSE_HOOKS", "0")
     ref = run_case(g, case, storage)
     monkeypatch.setenv("EFS_FUSE_HOOKS", "1")
+    monkeypatch.setenv("EFS_POST_GROUP", group)
     out = run_case(g, case, storage)
     for k, ((v0, c0), (v1, c1)) in enumerate(zip(ref[0], out[0])):
         assert np.array_equal(v0, v1) and np.array_equal(c0, c1), f"fields differ after chunk {k}"
