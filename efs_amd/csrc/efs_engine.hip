@@ -171,6 +171,7 @@ struct efs_ctx {
     uint32_t mur_n = 0;
     // entries kept: the last one of each normal cell, ordered by that cell
     std::vector<uint32_t> h_mur_normal, h_mur_shifted;  // interface indices
+    std::vector<float> h_mur_coeff;
     float *mur_coeff = nullptr, *mur_saved = nullptr;
     uint32_t *mur_normal = nullptr, *mur_shifted = nullptr, *mur_normal_src = nullptr, *mur_shifted_src = nullptr;
     uint8_t* mur_early = nullptr;       // post done before the excitation (k_mur_post_some)
@@ -195,6 +196,12 @@ struct efs_ctx {
     bool side_on = true;
     bool side_pend = false, side_pend_gather = false;
     SideWork side_e{};
+    // E-sweep epilogue (sweep_epilogue, EFS_SWEEP_EPILOGUE): Mur and
+    // excitation entries per block-table entry of the 4-wide launch
+    std::vector<SweepBlock> h_tab4;
+    bool ep_ok = false;
+    uint32_t *ep_moff = nullptr, *ep_eoff = nullptr, *ep_exc = nullptr;
+    uint4* ep_mur = nullptr;
     std::vector<uint32_t> h_exc_gidx;   // excitation cells (interface indices)
 
     // ADE (all orders/directions flattened)
@@ -681,6 +688,65 @@ int plan_post_e(efs_ctx* c, const std::vector<uint8_t>& early) {
     return EFS_OK;
 }
 
+// Lists of sweep_epilogue: each Mur entry goes to the block-table entry
+// that holds both its cells, each excitation entry to the one that holds
+// its cell (one on a Mur normal cell is dropped: the apply overwrites it).
+// Off (k_post_e runs) when some entry's cells lie in two blocks or a block
+// gets more Mur entries than its threads hold.
+int plan_epilogue(efs_ctx* c) {
+    c->ep_ok = false;
+    if (env_u64("EFS_SWEEP_EPILOGUE", 1) == 0 || !c->side_on || !c->fuse_hooks || !c->vec4 || c->dbl ||
+        (c->pml.n && !(c->pml_split && c->pml_fused)) || (!c->mur_n && !c->exc_n))
+        return EFS_OK;
+    const uint32_t nx = c->nx, ny = c->ny, nz = c->nz, nzp = c->nzp, tl = c->geom.tile_lanes;
+    const auto& tab = c->h_tab4;
+    const uint32_t T = (uint32_t)(((uint64_t)ny * nzp / 4 + tl - 1) / tl);
+    std::vector<int32_t> blk((size_t)T * nx, -1);  // [tile][x] -> table entry
+    for (size_t e = 0; e < tab.size(); ++e)
+        for (uint32_t x = tab[e].x0; x < tab[e].x1; ++x) blk[(size_t)tab[e].tile * nx + x] = (int32_t)e;
+    auto block_of = [&](uint64_t gidx) -> int32_t {
+        const uint64_t lin = gidx % c->N;
+        const uint32_t x = (uint32_t)(lin / ((uint64_t)ny * nz));
+        const uint32_t rem = (uint32_t)(lin - (uint64_t)x * ny * nz);
+        const uint32_t y = rem / nz, z = rem - y * nz;
+        const uint64_t lane = ((uint64_t)y * nzp + z) / 4;
+        return blk[(size_t)(lane / tl) * nx + x];
+    };
+    std::vector<std::vector<uint32_t>> mur(tab.size()), exc(tab.size());
+    for (uint32_t k = 0; k < c->mur_n; ++k) {
+        const int32_t b = block_of(c->h_mur_normal[k]);
+        if (b < 0 || b != block_of(c->h_mur_shifted[k])) return EFS_OK;
+        mur[b].push_back(k);
+    }
+    std::vector<uint32_t> nor(c->h_mur_normal);
+    std::sort(nor.begin(), nor.end());
+    for (uint32_t k = 0; k < c->exc_n; ++k) {
+        if (std::binary_search(nor.begin(), nor.end(), c->h_exc_gidx[k])) continue;
+        const int32_t b = block_of(c->h_exc_gidx[k]);
+        if (b < 0) return EFS_OK;
+        exc[b].push_back(k);
+    }
+    std::vector<uint32_t> moff{0}, eoff{0}, elist;
+    std::vector<uint4> mlist;
+    for (size_t b = 0; b < tab.size(); ++b) {
+        if (mur[b].size() > (size_t)kEpiPerThread * kSideThreads) return EFS_OK;
+        for (uint32_t k : mur[b]) {
+            uint32_t bits;
+            std::memcpy(&bits, &c->h_mur_coeff[k], sizeof bits);
+            mlist.push_back(make_uint4(tr(c, c->h_mur_shifted[k]), tr(c, c->h_mur_normal[k]), k, bits));
+        }
+        elist.insert(elist.end(), exc[b].begin(), exc[b].end());
+        moff.push_back((uint32_t)mlist.size());
+        eoff.push_back((uint32_t)elist.size());
+    }
+    TRY(dupload(c, c->ep_moff, moff.data(), moff.size()));
+    TRY(dupload(c, c->ep_eoff, eoff.data(), eoff.size()));
+    if (!mlist.empty()) TRY(dupload(c, c->ep_mur, mlist.data(), mlist.size()));
+    if (!elist.empty()) TRY(dupload(c, c->ep_exc, elist.data(), elist.size()));
+    c->ep_ok = true;
+    return EFS_OK;
+}
+
 int rebuild_aliases(efs_ctx* c) {
     if (c->mur_n) {
         std::vector<uint32_t> ns(c->mur_n), ss(c->mur_n);
@@ -707,6 +773,7 @@ int rebuild_aliases(efs_ctx* c) {
         c->mur_ne = (uint32_t)list.size();
         TRY(plan_post_e(c, early));
     }
+    TRY(plan_epilogue(c));
     if (c->ade_n) {
         std::vector<uint32_t> sv(c->ade_n);
         for (uint32_t k = 0; k < c->ade_n; ++k) sv[k] = alias_of(c, c->h_ade_gidx[k]);
@@ -844,6 +911,8 @@ void free_all(efs_ctx* c) {
     dfree(c->mur_early); dfree(c->mur_early_list);
     dfree(c->mur_defer); dfree(c->exc_defer); dfree(c->mur_b); dfree(c->exc_b);
     dfree(c->grp_mur); dfree(c->grp_exc); dfree(c->grp_eoff);
+    dfree(c->ep_moff); dfree(c->ep_mur); dfree(c->ep_eoff); dfree(c->ep_exc);
+    c->ep_ok = false;
     c->post_e_ok = false;
     c->mur_nb = c->exc_nb = c->grp_n = 0;
     dfree(c->mur_normal_src); dfree(c->mur_shifted_src);
@@ -927,7 +996,15 @@ int plan_sweeps(efs_ctx* c) {
     const uint32_t min_planes = (uint32_t)std::max<uint64_t>(1, env_u64("EFS_SWEEP_MIN_PLANES", 1));
     const uint32_t min_chunk = (uint32_t)std::max<uint64_t>(1, env_u64("EFS_SWEEP_MIN_CHUNK", 2));
     for (int v = 0; v < 2; ++v) {
-        const uint64_t tcells = (uint64_t)kSweepThreads * (v == 0 ? 4u : 1u);
+        // 4-wide tiles of grids with side work hold whole rows of lanes when
+        // that leaves at most 10 % of the threads idle, so that a Mur pair
+        // across y (rows y and y+1) stays in one block (sweep_epilogue)
+        uint32_t tl = kSweepThreads;
+        if (v == 0 && c->side_on && nz % 4 == 0) {
+            const uint32_t lpr = nz / 4, r = lpr <= (uint32_t)kSweepThreads ? kSweepThreads / lpr * lpr : 0u;
+            if (r * 10 >= (uint32_t)kSweepThreads * 9) tl = r;
+        }
+        const uint64_t tcells = (uint64_t)tl * (v == 0 ? 4u : 1u);
         const uint32_t T = (uint32_t)((cols + tcells - 1) / tcells);
         // UPML cells of each region in each tile (per plane of the region)
         std::vector<std::vector<double>> cnt(c->pml.n, std::vector<double>(T, 0.0));
@@ -980,6 +1057,12 @@ int plan_sweeps(efs_ctx* c) {
                     const double goal = pre[nx] * j / n;
                     while (x < nx && pre[x + 1] <= goal) ++x;
                     b = (x < nx && goal - pre[x] > pre[x + 1] - goal) ? x + 1 : x;
+                    // grids with side work keep planes 0, 1 and Nx-2, Nx-1
+                    // in one chunk each (Mur pairs across x)
+                    if (c->side_on && nx > 3) {
+                        if (b == 1) b = 2;
+                        if (b == nx - 1) b = nx - 2;
+                    }
                 }
                 if (b > b0) {
                     uint32_t flags = 0;
@@ -998,6 +1081,8 @@ int plan_sweeps(efs_ctx* c) {
         g.cols = (uint32_t)cols;
         g.S = c->S;
         g.blocks = (uint32_t)tab.size();
+        g.tile_lanes = tl;
+        if (v == 0) c->h_tab4 = tab;
         SweepBlock*& dtab = v == 0 ? c->sweep_tab4 : c->sweep_tab1;
         TRY(dupload(c, dtab, tab.data(), tab.size()));
         g.tab = dtab;
@@ -1281,6 +1366,25 @@ SideWork side_h(efs_ctx* c, uint32_t t, float* ring, bool pre) {
     return q;
 }
 
+// The epilogue of the E sweep of step t (sweep_epilogue) into q.
+void add_epilogue(const efs_ctx* c, SideWork& q, uint32_t t) {
+    q.volt = fv(c, 0);
+    q.curr = fv(c, 1);
+    q.ep_on = 1;
+    q.ep_moff = c->ep_moff;
+    q.ep_mur = c->ep_mur;
+    q.ep_eoff = c->ep_eoff;
+    q.ep_exc = c->ep_exc;
+    q.m_saved = c->mur_saved;
+    q.x_signal = c->exc_signal;
+    q.x_len = c->exc_len;
+    q.x_period = c->exc_period;
+    q.numTS = t;
+    q.x_delay = c->exc_delay;
+    q.x_gidx = c->exc_gidx;
+    q.x_amp = c->exc_amp;
+}
+
 // The pending sample and gather are complete (their E-sweep side was
 // launched): count them, and fold a full K-step block.
 int side_done(efs_ctx* c) {
@@ -1328,6 +1432,10 @@ int launch_step(efs_ctx* c, bool next_follows) {
         hipLaunchKernelGGL(k_rlc_pre, dim3(nblk(c->rlc_n, c->pt)), dim3(c->pt), 0, c->stream, c->rlc_type,
                            c->rlc_c, c->rlc_vdn, c->rlc_vil, c->rlc_n);
     }
+    // Mur post + apply and the excitation at the end of the E-sweep blocks
+    const bool use_ep = c->ep_ok && c->vec4 && (!has_pml || (c->pml_split && c->pml_fused)) && !c->tfsf_nv &&
+                        c->pbc_e.empty() &&
+                        !(c->ade_n && c->ade_apply);
     {
         ProfScope s(c, EFS_PROF_SWEEP_E);
         if (c->vec4 && (!has_pml || c->pml_split)) {
@@ -1339,14 +1447,16 @@ int launch_step(efs_ctx* c, bool next_follows) {
             if (has_pml && c->pml_fused)
                 kern = c->pml_comp ? (x16 ? sweep_e4fxp : c->comp ? sweep_e4fcp : sweep_e4fp)
                                    : (x16 ? sweep_e4fx : c->comp ? sweep_e4fc : sweep_e4f);
-            // the previous step's pending sample and gather ride along
-            const SideWork se = c->side_pend ? c->side_e : SideWork{};
-            if (se.nb) kern = side_twin(kern);
+            // the previous step's pending sample and gather ride along, and
+            // the blocks end with the Mur and excitation of their cells
+            SideWork se = c->side_pend ? c->side_e : SideWork{};
+            if (use_ep) add_epilogue(c, se, t);
+            if (se.nb || se.ep_on) kern = side_twin(kern);
             if (c->half) {
                 auto k16 = has_pml ? (c->pml_comp ? (x16 ? sweep_e4fxp_h : c->comp ? sweep_e4fcp_h : sweep_e4fp_h)
                                                   : (x16 ? sweep_e4fx_h : c->comp ? sweep_e4fc_h : sweep_e4f_h))
                                    : (x16 ? sweep_e4x_h : c->comp ? sweep_e4c_h : sweep_e4_h);
-                if (se.nb) k16 = side_twin(k16);
+                if (se.nb || se.ep_on) k16 = side_twin(k16);
                 hipLaunchKernelGGL(k16, dim3(se.nb + c->geom.blocks), dim3(kSweepThreads), 0, c->stream,
                                    (f16*)c->volt, (const f16*)c->curr, cf, c->geom, c->pml, c->pml_coef,
                                    c->pml_vflux, c->fscale, NfCapture{}, se);
@@ -1380,7 +1490,9 @@ int launch_step(efs_ctx* c, bool next_follows) {
     for (const PbcAxis& a : c->pbc_e)  // after TFSF-V, before Mur post (src/webgpu-engine.mjs:2461)
         hipLaunchKernelGGL(k_pbc, dim3(nblk((size_t)a.b0 * a.b1, c->pt)), dim3(c->pt), 0, c->stream, fv(c, 0),
                            c->pbc_imag, a, c->ny, c->nzp, (uint64_t)c->S);
-    if (c->mur_n && c->post_e_ok && !(c->ade_n && c->ade_apply)) {
+    if (use_ep) {
+        // done by the E sweep's blocks
+    } else if (c->mur_n && c->post_e_ok && !(c->ade_n && c->ade_apply)) {
         // Mur post + excitation + Mur apply in one launch
         if (c->dbl) launch_post_e(c, fv64(c, 0), t);
         else launch_post_e(c, fv(c, 0), t);
@@ -2017,6 +2129,7 @@ int efs_cfg_mur(efs_ctx* c, uint32_t n, const float* coeff, const uint32_t* norm
     const uint32_t m = (uint32_t)hn.size();
     c->h_mur_normal = hn;
     c->h_mur_shifted = hs;
+    c->h_mur_coeff = hc;
     TRY(dupload(c, c->mur_coeff, hc.data(), m));
     const std::vector<uint32_t> tn = tr_all(c, hn.data(), m), ts = tr_all(c, hs.data(), m);
     TRY(dupload(c, c->mur_normal, tn.data(), m));
@@ -2820,7 +2933,8 @@ int efs_path_flags(const efs_ctx* c, uint32_t* flags) {
              (nf_capture(c).on ? EFS_PATH_NF2FF_CAPTURE : 0u) | (c->comp16 ? EFS_PATH_COEF_IDX16 : 0u) |
              (c->dbl ? EFS_PATH_F64 : 0u) |
              (c->side_on && c->fuse_hooks && c->vec4 && !c->dbl && (!c->pml.n || c->pml_split) ? EFS_PATH_SIDE_WORK
-                                                                                                : 0u);
+                                                                                                : 0u) |
+             (c->ep_ok ? EFS_PATH_SWEEP_EPILOGUE : 0u);
     return EFS_OK;
 }
 

@@ -192,7 +192,24 @@ struct SideWork {
     float* saved;
     uint32_t mur_n;
     uint32_t nb;  // workgroups of side work at the start of the grid (0: none)
+    // E-sweep epilogue (ep_on): per block-table entry b, Mur entries
+    // ep_mur[ep_moff[b] .. ep_moff[b + 1]) as {shifted cell, normal cell,
+    // entry k, coefficient bits} (one load each) and excitation entries
+    // ep_exc[ep_eoff[b] .. ep_eoff[b + 1]), with the Mur pre values and
+    // the excitation arrays of k_post_e
+    uint32_t ep_on;
+    const uint4* ep_mur;
+    const uint32_t *ep_moff, *ep_eoff, *ep_exc;
+    const float* m_saved;
+    const float* x_signal;
+    uint32_t x_len, x_period, numTS;
+    const uint32_t *x_delay, *x_gidx;
+    const float* x_amp;
 };
+
+// Mur entries one thread of an epilogue holds (the host keeps a block's
+// list within kEpiPerThread * kSideThreads, else it uses k_post_e).
+constexpr int kEpiPerThread = 16;
 
 constexpr uint32_t kSideThreads = 256;  // the sweep kernels' workgroup size
 
@@ -221,6 +238,42 @@ __device__ __forceinline__ void side_work(const SideWork& q, uint32_t b) {
         const uint32_t k = b * kSideThreads + threadIdx.x;
         if (k < q.mur_n) mur_pre_one(q.volt, q.vflux, q.coeff, q.normal_src, q.shifted_src, q.saved, k);
     }
+}
+
+// The end of an E-sweep block when every Mur entry has its normal and its
+// shifted cell in one block (the host checks; else k_post_e runs): the
+// Mur post of the block's entries from its own new E values, then the
+// excitation and the Mur apply, in the reference's order (post of all
+// entries, excitation, apply).  Reads and writes stay inside the block's
+// cells, which no other workgroup of the launch touches, so two workgroup
+// barriers order them: one after the march (its values are written), one
+// between the reads and the writes.  b: the block-table entry.
+__device__ __forceinline__ void sweep_epilogue(const SideWork& q, uint32_t b) {
+    if (!q.ep_on) return;
+    const uint32_t m0 = q.ep_moff[b], m1 = q.ep_moff[b + 1];
+    const uint32_t e0 = q.ep_eoff[b], e1 = q.ep_eoff[b + 1];
+    if (m0 == m1 && e0 == e1) return;  // uniform over the workgroup
+    // the entries and their pre values do not depend on the march: loaded
+    // while the block's other waves finish it
+    uint4 e[kEpiPerThread];
+    float s[kEpiPerThread];
+#pragma unroll
+    for (int u = 0; u < kEpiPerThread; ++u) {
+        const uint32_t j = m0 + u * kSideThreads + threadIdx.x;
+        e[u] = j < m1 ? q.ep_mur[j] : make_uint4(0u, ~0u, 0u, 0u);
+    }
+#pragma unroll
+    for (int u = 0; u < kEpiPerThread; ++u) s[u] = e[u].y != ~0u ? q.m_saved[e[u].z] : 0.f;
+    __syncthreads();  // the block's new E values are written
+#pragma unroll
+    for (int u = 0; u < kEpiPerThread; ++u)
+        if (e[u].y != ~0u) s[u] = fma1(__uint_as_float(e[u].w), q.volt.ld(e[u].x), s[u]);
+    __syncthreads();  // every read above precedes every write below
+#pragma unroll
+    for (int u = 0; u < kEpiPerThread; ++u)
+        if (e[u].y != ~0u) q.volt.st(e[u].y, s[u]);
+    for (uint32_t j = e0 + threadIdx.x; j < e1; j += kSideThreads)
+        excitation_one(q.volt, q.x_signal, q.x_len, q.x_period, q.x_delay, q.x_amp, q.x_gidx, q.ep_exc[j], q.numTS);
 }
 
 // The E-sweep side of the last step of an iterate call, on its own.

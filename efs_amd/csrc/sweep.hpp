@@ -97,7 +97,7 @@ struct SweepGeom {
     uint32_t nx, ny, nz;
     uint32_t cols;        // ny*nz
     uint32_t blocks;      // entries of tab
-    uint32_t pad;
+    uint32_t tile_lanes;  // lanes of a column tile (4-wide launches): 256, or whole rows of lanes
     uint64_t S;           // component stride in floats
     const SweepBlock* tab;
 };
@@ -941,8 +941,8 @@ __device__ __forceinline__ bool interior_lane(const SweepGeom& g, const PmlTable
     const SweepBlock d = g.tab[xcd_remap(bid, g.blocks)];
     flags = d.flags;
     m = 0u;
-    const uint32_t c0 = (d.tile * kSweepThreads + threadIdx.x) * 4u;
-    if (c0 >= g.cols) return false;
+    const uint32_t c0 = (d.tile * g.tile_lanes + threadIdx.x) * 4u;
+    if (threadIdx.x >= g.tile_lanes || c0 >= g.cols) return false;
     w.y = c0 / g.nz;
     w.z0 = c0 - w.y * g.nz;
     w.x0 = d.x0;
@@ -976,9 +976,10 @@ __device__ __forceinline__ bool pml_lane(const PmlTable& pml, Lane4& w) {
 // side: point work of extra workgroups at the start of the grid (SideWork,
 // defined with the point kernels).  Each grid entry point comes in two
 // kernels: NAME, without side work (the code of the sweep alone), and
-// NAME_s, whose first side.nb workgroups run side_work (their extra
-// branch costs the fused FP32 UPML H sweeps a spill, so grids that bring
-// no side work launch NAME).
+// NAME_s, whose first side.nb workgroups run side_work and whose E-sweep
+// blocks end with sweep_epilogue (Mur post and apply and the excitation of
+// the block's own cells; the extra code costs the fused FP32 UPML H sweeps
+// a spill, so grids that bring neither launch NAME).
 #define EFS_SWEEP4_PARAMS(T)                                                                                    \
     T *__restrict__ dst, const T *__restrict__ src, Coef cf, SweepGeom g, PmlTable pml,                        \
         const float *__restrict__ pcoef, float *__restrict__ pflux, const float *__restrict__ fscale, NfCapture cap, \
@@ -1015,10 +1016,12 @@ __device__ __forceinline__ bool pml_lane(const PmlTable& pml, Lane4& w) {
         coef_stage<COMP>(cf, s_tab);                                                                               \
         Lane4 w;                                                                                                   \
         uint32_t m, flags;                                                                                         \
-        if (!interior_lane<HAS_PML>(g, pml, w, m, flags, bid)) return;                                             \
-        const Scales sc = load_scales<float>(fscale, HF);                                                          \
-        MARCH<kMarchInterior, COMP, false, float>(dst, src, cf, s_tab, nullptr, g, pml, pcoef, pflux, w, m, sc,    \
-                                                  cap);                                                            \
+        if (interior_lane<HAS_PML>(g, pml, w, m, flags, bid)) {                                                    \
+            const Scales sc = load_scales<float>(fscale, HF);                                                      \
+            MARCH<kMarchInterior, COMP, false, float>(dst, src, cf, s_tab, nullptr, g, pml, pcoef, pflux, w, m, sc, \
+                                                      cap);                                                        \
+        }                                                                                                          \
+        if (SIDE && !HF) sweep_epilogue(side, xcd_remap(bid, g.blocks));                                           \
     }                                                                                                              \
     EFS_SWEEP4_KERNELS(NAME, float, EFS_SWEEP_WAVES)
 
@@ -1048,9 +1051,11 @@ __device__ __forceinline__ bool pml_lane(const PmlTable& pml, Lane4& w) {
         uint32_t m, flags;                                                                                         \
         const bool live = interior_lane<true>(g, pml, w, m, flags, bid);                                           \
         upml_stage<PCOMP>(cf, s_ptab, flags);                                                                      \
-        if (!live) return;                                                                                         \
-        const Scales sc = load_scales<T>(fscale, HF);                                                              \
-        MARCH<kMarchFused, COMP, PCOMP, T>(dst, src, cf, s_tab, s_ptab, g, pml, pcoef, pflux, w, m, sc, cap);      \
+        if (live) {                                                                                                \
+            const Scales sc = load_scales<T>(fscale, HF);                                                          \
+            MARCH<kMarchFused, COMP, PCOMP, T>(dst, src, cf, s_tab, s_ptab, g, pml, pcoef, pflux, w, m, sc, cap);  \
+        }                                                                                                          \
+        if (SIDE && !HF) sweep_epilogue(side, xcd_remap(bid, g.blocks));                                           \
     }                                                                                                              \
     EFS_SWEEP4_KERNELS(NAME, T, EFS_SWEEP_WAVES_PML)
 
@@ -1064,9 +1069,12 @@ __device__ __forceinline__ bool pml_lane(const PmlTable& pml, Lane4& w) {
         coef_stage<COMP>(cf, s_tab);                                                                               \
         Lane4 w;                                                                                                   \
         uint32_t m, flags;                                                                                         \
-        if (!interior_lane<false>(g, pml, w, m, flags, bid)) return;                                               \
-        const Scales sc = load_scales<f16>(fscale, HF);                                                            \
-        MARCH<kMarchInterior, COMP, false, f16>(dst, src, cf, s_tab, nullptr, g, pml, pcoef, pflux, w, m, sc, cap); \
+        if (interior_lane<false>(g, pml, w, m, flags, bid)) {                                                      \
+            const Scales sc = load_scales<f16>(fscale, HF);                                                        \
+            MARCH<kMarchInterior, COMP, false, f16>(dst, src, cf, s_tab, nullptr, g, pml, pcoef, pflux, w, m, sc,  \
+                                                    cap);                                                          \
+        }                                                                                                          \
+        if (SIDE && !HF) sweep_epilogue(side, xcd_remap(bid, g.blocks));                                           \
     }                                                                                                              \
     EFS_SWEEP4_KERNELS(NAME, f16, EFS_SWEEP_WAVES)
 

@@ -571,11 +571,11 @@ class HIPEngine:
 
     def pathFlags(self):
         """Forms in use: set of 'vec4', 'coef_idx', 'upml4', 'fused', 'upml_idx',
-        'f16', 'nf2ff_capture', 'coef_idx16', 'f64', 'side_work'."""
+        'f16', 'nf2ff_capture', 'coef_idx16', 'f64', 'side_work', 'sweep_epilogue'."""
         f = C.c_uint32(0)
         self._check(self.lib.efs_path_flags(self.ctx, C.byref(f)))
         names = ("vec4", "coef_idx", "upml4", "fused", "upml_idx", "f16", "nf2ff_capture", "coef_idx16", "f64",
-                 "side_work")
+                 "side_work", "sweep_epilogue")
         return {n for b, n in enumerate(names) if f.value & (1 << b)}
 
 

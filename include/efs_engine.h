@@ -276,8 +276,11 @@ enum {
     EFS_PATH_NF2FF_CAPTURE = 64, /* NF2FF z-face points read slabs the H sweep wrote */
     EFS_PATH_COEF_IDX16 = 128,   /* core coefficients as 16-bit indices + cached tables */
     EFS_PATH_F64 = 256,          /* FP64 field storage and arithmetic (1-wide sweeps) */
-    EFS_PATH_SIDE_WORK = 512     /* NF2FF sampling, probe gather and Mur pre pass run in extra
+    EFS_PATH_SIDE_WORK = 512,    /* NF2FF sampling, probe gather and Mur pre pass run in extra
                                     workgroups of the sweep kernels (EFS_SIDE_WORK) */
+    EFS_PATH_SWEEP_EPILOGUE = 1024 /* Mur post/apply and the excitation run at the end of the E-sweep
+                                      blocks (EFS_SWEEP_EPILOGUE; set once an iterate call has
+                                      planned it) */
 };
 int efs_path_flags(const efs_ctx *ctx, uint32_t *flags);
 

@@ -6,8 +6,9 @@ The bar is the same bits (fields, buffered gather, NF2FF accumulators):
 the fused launches keep the reference's phase order for every cell that
 two operations share.  The Mur box case puts excitation entries on Mur
 normal cells and on a cell that is both normal and shifted cell, which
-exercises the deferred lists (phase B, or the first workgroup of k_post_e
-in the group form, EFS_POST_GROUP); iterate calls of 1, 7, 64 and 128
+exercises the deferred lists (phase B, or the first workgroups of k_post_e
+in the group form, EFS_POST_GROUP) and, on the default path for grids this
+small, the end of the E-sweep blocks (EFS_SWEEP_EPILOGUE); iterate calls of 1, 7, 64 and 128
 steps check that the Mur pre pass runs exactly once per step across call
 boundaries.  The fused path is also held to the FP64 oracle (1e-5)."""
 import numpy as np
@@ -83,18 +84,23 @@ def run_case(eng, case, storage="fp32", chunks=CHUNKS):
     return snaps, g["data"].copy(), np.concatenate([acc["accumE"].ravel(), acc["accumH"].ravel()])
 
 
-@pytest.mark.parametrize("storage,group", [("fp32", "1"), ("fp16", "1"), ("fp64", "1"), ("fp32", "0"),
-                                           ("fp16", "0")])
-def test_fused_hooks_match_separate_kernels(HIPEngine, monkeypatch, storage, group):
+@pytest.mark.parametrize("storage,group,epi", [("fp32", "1", "1"), ("fp16", "1", "1"), ("fp64", "1", "1"),
+                                               ("fp32", "1", "0"), ("fp16", "1", "0"), ("fp32", "0", "0"),
+                                               ("fp16", "0", "0")])
+def test_fused_hooks_match_separate_kernels(HIPEngine, monkeypatch, storage, group, epi):
     case = mur_box_case()
-    say("case mur box", storage, group)
+    say("case mur box", storage, group, epi)
     g = HIPEngine()
     assert g.initGPU()
     monkeypatch.setenv("EFS_FUSE_HOOKS", "0")
     ref = run_case(g, case, storage)
     monkeypatch.setenv("EFS_FUSE_HOOKS", "1")
     monkeypatch.setenv("EFS_POST_GROUP", group)
+    monkeypatch.setenv("EFS_SWEEP_EPILOGUE", epi)
     out = run_case(g, case, storage)
+    # the Mur pairs and excitation cells of the box lie inside single sweep
+    # blocks, so the E-sweep epilogue takes them (FP32 / FP16 storage)
+    assert ("sweep_epilogue" in g.pathFlags()) == (epi == "1" and storage != "fp64")
     for k, ((v0, c0), (v1, c1)) in enumerate(zip(ref[0], out[0])):
         assert np.array_equal(v0, v1) and np.array_equal(c0, c1), f"fields differ after chunk {k}"
     assert np.array_equal(ref[1], out[1]), "buffered gather differs"
@@ -148,6 +154,8 @@ def test_patch_fused_hooks_match_separate_kernels(HIPEngine, monkeypatch):
         workloads.configure_patch(g, w, nf2ff=True, max_samples=400)
         for n in (1, 7, 64, 128, 200):
             g.iterate(n)
+        if fuse == "1":  # Mur and the port's excitation at the end of the E-sweep blocks
+            assert {"side_work", "sweep_epilogue"} <= g.pathFlags()
         f = g.getFields()
         acc = g.readNF2FFAccumulation()
         res[fuse] = (f["volt"], f["curr"], g.readProbeGatherBuffered()["data"],
