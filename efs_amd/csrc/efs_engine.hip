@@ -733,7 +733,19 @@ int plan_epilogue(efs_ctx* c) {
         for (uint32_t k : mur[b]) {
             uint32_t bits;
             std::memcpy(&bits, &c->h_mur_coeff[k], sizeof bits);
-            mlist.push_back(make_uint4(tr(c, c->h_mur_shifted[k]), tr(c, c->h_mur_normal[k]), k, bits));
+            // a shifted cell in the block's first two planes is read from
+            // the LDS copy the march keeps of them
+            const uint64_t sg = c->h_mur_shifted[k], lin = sg % c->N;
+            const uint32_t comp = (uint32_t)(sg / c->N);
+            const uint32_t x = (uint32_t)(lin / ((uint64_t)ny * nz));
+            const uint32_t rem = (uint32_t)(lin - (uint64_t)x * ny * nz);
+            const uint32_t y = rem / nz, z = rem - y * nz;
+            const uint64_t lane = ((uint64_t)y * nzp + z) / 4;
+            uint32_t src = tr(c, sg);
+            if (x - tab[b].x0 < kEpLdsPlanes)
+                src = kEpInLds | (((x - tab[b].x0) * 3 + comp) * kEpLdsCells +
+                                  4 * (uint32_t)(lane - (uint64_t)tab[b].tile * tl) + z % 4);
+            mlist.push_back(make_uint4(src, tr(c, c->h_mur_normal[k]), k, bits));
         }
         elist.insert(elist.end(), exc[b].begin(), exc[b].end());
         moff.push_back((uint32_t)mlist.size());

@@ -194,7 +194,9 @@ struct SideWork {
     uint32_t nb;  // workgroups of side work at the start of the grid (0: none)
     // E-sweep epilogue (ep_on): per block-table entry b, Mur entries
     // ep_mur[ep_moff[b] .. ep_moff[b + 1]) as {shifted cell, normal cell,
-    // entry k, coefficient bits} (one load each) and excitation entries
+    // entry k, coefficient bits} (one load each; a shifted cell with
+    // kEpInLds set is an index into the block's LDS copy of its first two
+    // planes) and excitation entries
     // ep_exc[ep_eoff[b] .. ep_eoff[b + 1]), with the Mur pre values and
     // the excitation arrays of k_post_e
     uint32_t ep_on;
@@ -210,6 +212,7 @@ struct SideWork {
 // Mur entries one thread of an epilogue holds (the host keeps a block's
 // list within kEpiPerThread * kSideThreads, else it uses k_post_e).
 constexpr int kEpiPerThread = 16;
+constexpr uint32_t kEpInLds = 0x80000000u;
 
 constexpr uint32_t kSideThreads = 256;  // the sweep kernels' workgroup size
 
@@ -248,7 +251,7 @@ __device__ __forceinline__ void side_work(const SideWork& q, uint32_t b) {
 // cells, which no other workgroup of the launch touches, so two workgroup
 // barriers order them: one after the march (its values are written), one
 // between the reads and the writes.  b: the block-table entry.
-__device__ __forceinline__ void sweep_epilogue(const SideWork& q, uint32_t b) {
+__device__ __forceinline__ void sweep_epilogue(const SideWork& q, uint32_t b, const float* s_ep) {
     if (!q.ep_on) return;
     const uint32_t m0 = q.ep_moff[b], m1 = q.ep_moff[b + 1];
     const uint32_t e0 = q.ep_eoff[b], e1 = q.ep_eoff[b + 1];
@@ -267,7 +270,17 @@ __device__ __forceinline__ void sweep_epilogue(const SideWork& q, uint32_t b) {
     __syncthreads();  // the block's new E values are written
 #pragma unroll
     for (int u = 0; u < kEpiPerThread; ++u)
-        if (e[u].y != ~0u) s[u] = fma1(__uint_as_float(e[u].w), q.volt.ld(e[u].x), s[u]);
+        if (e[u].y != ~0u) {
+            float v;
+            if (e[u].x & kEpInLds) {  // the value as stored, times the component's scale (FieldView::ld)
+                const uint32_t li = e[u].x & ~kEpInLds;
+                v = s_ep[li];
+                if (q.volt.half) v = v * q.volt.scale[(li / kEpLdsCells) % 3];
+            } else {
+                v = q.volt.ld(e[u].x);
+            }
+            s[u] = fma1(__uint_as_float(e[u].w), v, s[u]);
+        }
     __syncthreads();  // every read above precedes every write below
 #pragma unroll
     for (int u = 0; u < kEpiPerThread; ++u)

@@ -692,14 +692,22 @@ __device__ __forceinline__ void upml_decode(const float4* s_ptab, const uint32_t
     }
 }
 
+// Cells of a block's first two planes kept in LDS for sweep_epilogue:
+// [plane][component][4 * lane + cell of the lane], the stored (raw) values.
+constexpr uint32_t kEpLdsPlanes = 2, kEpLdsCells = 4 * kSweepThreads;
+constexpr uint32_t kEpLdsFloats = kEpLdsPlanes * 3 * kEpLdsCells;
+
 // E update of 4 cells over x in [x0, x1); MODE says how UPML cells are
 // treated (column mask m gives the regions the lane's column crosses).
+// ep_lds (null: none): where the first two planes' new values go for the
+// block's epilogue.
 template <int MODE, int COMP, bool PCOMP, typename T>
 __device__ __forceinline__ void e4_march(T* __restrict__ volt, const T* __restrict__ curr, const Coef& cf,
                                          const float2* s_tab, const float4* s_ptab, const SweepGeom& g,
                                          const PmlTable& pml,
                                          const float* __restrict__ pcoef, float* __restrict__ pflux, Lane4 w,
-                                         uint32_t m, const Scales& sc, const NfCapture&, const ZMur& zm = ZMur{}) {
+                                         uint32_t m, const Scales& sc, const NfCapture&, const ZMur& zm = ZMur{},
+                                         float* __restrict__ ep_lds = nullptr) {
     typedef Fs<T> L;
     const size_t S = g.S;
     T* const E[3] = {volt, volt + S, volt + 2 * S};
@@ -740,6 +748,10 @@ __device__ __forceinline__ void e4_march(T* __restrict__ volt, const T* __restri
         }
     }
     PmlCursor u{-1, m ? 0u : 0xffffffffu, nullptr, nullptr, nullptr, 0u, 0u};
+    auto keep = [&](uint32_t x, int n, const v4& out) {
+        if (ep_lds && x - w.x0 < kEpLdsPlanes)
+            *reinterpret_cast<v4*>(ep_lds + ((x - w.x0) * 3 + n) * kEpLdsCells + 4 * threadIdx.x) = L::storedr(out);
+    };
     EFS_MARCH_UNROLL_PRAGMA
     for (uint32_t x = w.x0; x < w.x1; ++x, i += plane) {
         cursor_at<false>(u, pml, m, x, w.y, w.z0, pcoef, pflux, cf.pidx);
@@ -793,14 +805,19 @@ __device__ __forceinline__ void e4_march(T* __restrict__ volt, const T* __restri
                 }
             }
 #pragma unroll
-            for (int n = 0; n < 3; ++n) L::str(E[n] + i, vn[n]);
+            for (int n = 0; n < 3; ++n) {
+                L::str(E[n] + i, vn[n]);
+                keep(x, n, vn[n]);
+            }
         } else {
             upml_decode<PCOMP>(s_ptab, pq, P0, P1, P2);
 #pragma unroll
             for (int n = 0; n < 3; ++n) {
                 const v4 Fn = fma4(a[n], F[n], b[n] * cur[n]);
                 const v4 Vs = V[n] * L::sc(sc.own[n]);
-                L::str(E[n] + i, fma4(P2[n], Fn, fma4(P0[n], Vs, -(P1[n] * F[n]))) * L::sc(sc.own_inv[n]));
+                const v4 out = fma4(P2[n], Fn, fma4(P0[n], Vs, -(P1[n] * F[n]))) * L::sc(sc.own_inv[n]);
+                L::str(E[n] + i, out);
+                keep(x, n, out);
                 st4(u.flux + n * u.cs, Fn);
             }
         }
@@ -814,7 +831,8 @@ __device__ __forceinline__ void h4_march(T* __restrict__ curr, const T* __restri
                                          const float2* s_tab, const float4* s_ptab, const SweepGeom& g,
                                          const PmlTable& pml,
                                          const float* __restrict__ pcoef, float* __restrict__ pflux, Lane4 w,
-                                         uint32_t m, const Scales& sc, const NfCapture& cap) {
+                                         uint32_t m, const Scales& sc, const NfCapture& cap, const ZMur& = ZMur{},
+                                         float* = nullptr) {
     typedef Fs<T> L;
     const size_t S = g.S;
     T* const Ic[3] = {curr, curr + S, curr + 2 * S};
@@ -1013,15 +1031,16 @@ __device__ __forceinline__ bool pml_lane(const PmlTable& pml, Lane4& w) {
     __device__ __forceinline__ void NAME##_body(EFS_SWEEP4_PARAMS(float)) {                                        \
         EFS_SWEEP4_SIDE(HF)                                                                                        \
         __shared__ float2 s_tab[COMP == kCoefIdx8 ? 3 * kTab : 1];                                                 \
+        __shared__ float s_ep[SIDE && !HF ? kEpLdsFloats : 1];                                                     \
         coef_stage<COMP>(cf, s_tab);                                                                               \
         Lane4 w;                                                                                                   \
         uint32_t m, flags;                                                                                         \
         if (interior_lane<HAS_PML>(g, pml, w, m, flags, bid)) {                                                    \
             const Scales sc = load_scales<float>(fscale, HF);                                                      \
             MARCH<kMarchInterior, COMP, false, float>(dst, src, cf, s_tab, nullptr, g, pml, pcoef, pflux, w, m, sc, \
-                                                      cap);                                                        \
+                                                      cap, ZMur{}, SIDE && !HF && side.ep_on ? s_ep : nullptr);    \
         }                                                                                                          \
-        if (SIDE && !HF) sweep_epilogue(side, xcd_remap(bid, g.blocks));                                           \
+        if (SIDE && !HF) sweep_epilogue(side, xcd_remap(bid, g.blocks), s_ep);                                     \
     }                                                                                                              \
     EFS_SWEEP4_KERNELS(NAME, float, EFS_SWEEP_WAVES)
 
@@ -1046,6 +1065,7 @@ __device__ __forceinline__ bool pml_lane(const PmlTable& pml, Lane4& w) {
         EFS_SWEEP4_SIDE(HF)                                                                                        \
         __shared__ float2 s_tab[COMP == kCoefIdx8 ? 3 * kTab : 1];                                                 \
         __shared__ float4 s_ptab[PCOMP ? 3 * kTab : 1];                                                            \
+        __shared__ float s_ep[SIDE && !HF ? kEpLdsFloats : 1];                                                     \
         coef_stage<COMP>(cf, s_tab);                                                                               \
         Lane4 w;                                                                                                   \
         uint32_t m, flags;                                                                                         \
@@ -1053,9 +1073,10 @@ __device__ __forceinline__ bool pml_lane(const PmlTable& pml, Lane4& w) {
         upml_stage<PCOMP>(cf, s_ptab, flags);                                                                      \
         if (live) {                                                                                                \
             const Scales sc = load_scales<T>(fscale, HF);                                                          \
-            MARCH<kMarchFused, COMP, PCOMP, T>(dst, src, cf, s_tab, s_ptab, g, pml, pcoef, pflux, w, m, sc, cap);  \
+            MARCH<kMarchFused, COMP, PCOMP, T>(dst, src, cf, s_tab, s_ptab, g, pml, pcoef, pflux, w, m, sc, cap,   \
+                                               ZMur{}, SIDE && !HF && side.ep_on ? s_ep : nullptr);                \
         }                                                                                                          \
-        if (SIDE && !HF) sweep_epilogue(side, xcd_remap(bid, g.blocks));                                           \
+        if (SIDE && !HF) sweep_epilogue(side, xcd_remap(bid, g.blocks), s_ep);                                     \
     }                                                                                                              \
     EFS_SWEEP4_KERNELS(NAME, T, EFS_SWEEP_WAVES_PML)
 
@@ -1066,15 +1087,16 @@ __device__ __forceinline__ bool pml_lane(const PmlTable& pml, Lane4& w) {
     __device__ __forceinline__ void NAME##_body(EFS_SWEEP4_PARAMS(f16)) {                                          \
         EFS_SWEEP4_SIDE(HF)                                                                                        \
         __shared__ float2 s_tab[COMP == kCoefIdx8 ? 3 * kTab : 1];                                                 \
+        __shared__ float s_ep[SIDE && !HF ? kEpLdsFloats : 1];                                                     \
         coef_stage<COMP>(cf, s_tab);                                                                               \
         Lane4 w;                                                                                                   \
         uint32_t m, flags;                                                                                         \
         if (interior_lane<false>(g, pml, w, m, flags, bid)) {                                                      \
             const Scales sc = load_scales<f16>(fscale, HF);                                                        \
             MARCH<kMarchInterior, COMP, false, f16>(dst, src, cf, s_tab, nullptr, g, pml, pcoef, pflux, w, m, sc,  \
-                                                    cap);                                                          \
+                                                    cap, ZMur{}, SIDE && !HF && side.ep_on ? s_ep : nullptr);      \
         }                                                                                                          \
-        if (SIDE && !HF) sweep_epilogue(side, xcd_remap(bid, g.blocks));                                           \
+        if (SIDE && !HF) sweep_epilogue(side, xcd_remap(bid, g.blocks), s_ep);                                     \
     }                                                                                                              \
     EFS_SWEEP4_KERNELS(NAME, f16, EFS_SWEEP_WAVES)
 
