@@ -1055,12 +1055,24 @@ int plan_sweeps(efs_ctx* c) {
         const double target = total / nblocks;
         std::vector<SweepBlock> tab;
         std::vector<double> pre(nx + 1);
-        for (uint32_t t = 0; t < T; ++t) {
+        auto chunks_of = [&](uint32_t t) {
             const double* ct = &cost[(size_t)t * nx];
             pre[0] = 0.0;
             for (uint32_t x = 0; x < nx; ++x) pre[x + 1] = pre[x] + ct[x];
-            const uint32_t n = (uint32_t)std::max(1.0, std::min((double)std::max(1u, nx / min_chunk),
-                                                                std::floor(pre[nx] / target + 0.5)));
+            return (uint32_t)std::max(1.0, std::min((double)std::max(1u, nx / min_chunk),
+                                                    std::floor(pre[nx] / target + 0.5)));
+        };
+        // the last tile of a plane holds fewer lanes: equal bytes would give
+        // it longer chunks, but a march's time grows with its planes when
+        // the grid is too small to load HBM (its block would end last), so
+        // it gets at least the chunk count of the full tiles
+        const bool partial = cols % tcells != 0;
+        uint32_t n_full = 1;
+        if (partial)
+            for (uint32_t t = 0; t + 1 < T; ++t) n_full = std::max(n_full, chunks_of(t));
+        for (uint32_t t = 0; t < T; ++t) {
+            uint32_t n = chunks_of(t);
+            if (partial && t + 1 == T) n = std::max(n, std::min(n_full, std::max(1u, nx / min_chunk)));
             // boundary j at the plane whose prefix cost is nearest j/n of the tile's total
             uint32_t x = 0, b0 = 0;
             for (uint32_t j = 1; j <= n; ++j) {
@@ -1118,6 +1130,14 @@ inline size_t rw(const efs_ctx* c) { return c->dbl ? 2 : 1; }
 
 template <class FV>
 typename FV::R* rp(float* p) { return reinterpret_cast<typename FV::R*>(p); }
+
+// The E sweeps end with Mur post + apply and the excitation of their own
+// cells (sweep_epilogue) in the steps of this configuration.
+bool epilogue_on(const efs_ctx* c) {
+    const bool has_pml = c->pml.n > 0;
+    return c->ep_ok && c->vec4 && (!has_pml || (c->pml_split && c->pml_fused)) && !c->tfsf_nv &&
+           c->pbc_e.empty() && !(c->ade_n && c->ade_apply);
+}
 
 // The point-kernel launches that the FP64 form shares, for view type FV.
 template <class FV>
@@ -1445,9 +1465,7 @@ int launch_step(efs_ctx* c, bool next_follows) {
                            c->rlc_c, c->rlc_vdn, c->rlc_vil, c->rlc_n);
     }
     // Mur post + apply and the excitation at the end of the E-sweep blocks
-    const bool use_ep = c->ep_ok && c->vec4 && (!has_pml || (c->pml_split && c->pml_fused)) && !c->tfsf_nv &&
-                        c->pbc_e.empty() &&
-                        !(c->ade_n && c->ade_apply);
+    const bool use_ep = epilogue_on(c);
     {
         ProfScope s(c, EFS_PROF_SWEEP_E);
         if (c->vec4 && (!has_pml || c->pml_split)) {
