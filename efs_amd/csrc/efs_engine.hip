@@ -701,16 +701,28 @@ int plan_epilogue(efs_ctx* c) {
     const uint32_t nx = c->nx, ny = c->ny, nz = c->nz, nzp = c->nzp, tl = c->geom.tile_lanes;
     const auto& tab = c->h_tab4;
     const uint32_t T = (uint32_t)(((uint64_t)ny * nzp / 4 + tl - 1) / tl);
-    std::vector<int32_t> blk((size_t)T * nx, -1);  // [tile][x] -> table entry
+    // [tile][x] -> table entries: the block of the tile's first lanes, and
+    // the one from lane `split` of the tile on when a block takes part of it
+    std::vector<int32_t> blk((size_t)T * nx, -1), blk2((size_t)T * nx, -1);
+    std::vector<uint32_t> split((size_t)T * nx, 0);
     for (size_t e = 0; e < tab.size(); ++e)
-        for (uint32_t x = tab[e].x0; x < tab[e].x1; ++x) blk[(size_t)tab[e].tile * nx + x] = (int32_t)e;
+        for (uint32_t x = tab[e].x0; x < tab[e].x1; ++x) {
+            const size_t i = (size_t)tab[e].tile * nx + x;
+            if (block_lane_first(tab[e].flags) == 0) {
+                blk[i] = (int32_t)e;
+            } else {
+                blk2[i] = (int32_t)e;
+                split[i] = block_lane_first(tab[e].flags);
+            }
+        }
     auto block_of = [&](uint64_t gidx) -> int32_t {
         const uint64_t lin = gidx % c->N;
         const uint32_t x = (uint32_t)(lin / ((uint64_t)ny * nz));
         const uint32_t rem = (uint32_t)(lin - (uint64_t)x * ny * nz);
         const uint32_t y = rem / nz, z = rem - y * nz;
         const uint64_t lane = ((uint64_t)y * nzp + z) / 4;
-        return blk[(size_t)(lane / tl) * nx + x];
+        const size_t i = (size_t)(lane / tl) * nx + x;
+        return blk2[i] >= 0 && lane % tl >= split[i] ? blk2[i] : blk[i];
     };
     std::vector<std::vector<uint32_t>> mur(tab.size()), exc(tab.size());
     for (uint32_t k = 0; k < c->mur_n; ++k) {
@@ -744,7 +756,8 @@ int plan_epilogue(efs_ctx* c) {
             uint32_t src = tr(c, sg);
             if (x - tab[b].x0 < kEpLdsPlanes)
                 src = kEpInLds | (((x - tab[b].x0) * 3 + comp) * kEpLdsCells +
-                                  4 * (uint32_t)(lane - (uint64_t)tab[b].tile * tl) + z % 4);
+                                  4 * (uint32_t)(lane - (uint64_t)tab[b].tile * tl - block_lane_first(tab[b].flags)) +
+                                  z % 4);
             mlist.push_back(make_uint4(src, tr(c, c->h_mur_normal[k]), k, bits));
         }
         elist.insert(elist.end(), exc[b].begin(), exc[b].end());
@@ -1096,6 +1109,31 @@ int plan_sweeps(efs_ctx* c) {
                     b0 = b;
                 }
             }
+        }
+        // with side work the blocks of the x-face chunks end with the Mur of
+        // a whole face plane (sweep_epilogue) and finish last: each takes
+        // half of its tile's rows (the rows 0, 1 and Ny-2, Ny-1 of the y-face
+        // Mur pairs stay together)
+        const uint32_t lpr = nz / 4;
+        if (v == 0 && c->side_on && nx > 3 && nz % 4 == 0 && tl % lpr == 0 && env_u64("EFS_SPLIT_XFACE", 1)) {
+            const uint32_t lanes = (uint32_t)(cols / 4);
+            std::vector<SweepBlock> out;
+            for (const SweepBlock& b : tab) {
+                const uint32_t tl_t = std::min(tl, lanes - b.tile * tl), r0 = b.tile * tl / lpr;
+                const uint32_t rows = tl_t / lpr;
+                uint32_t rs = r0 + rows / 2;
+                if (rs == 1) rs = 2;
+                if (rs == ny - 1) rs = ny - 2;
+                if ((b.x0 != 0 && b.x1 != nx) || b.flags != (b.flags & kBlockUpml) || rows < 2 || rs <= r0 ||
+                    rs >= r0 + rows) {
+                    out.push_back(b);
+                    continue;
+                }
+                const uint32_t l1 = (rs - r0) * lpr;
+                out.push_back(SweepBlock{b.tile, b.x0, b.x1, b.flags | block_lanes(0, l1)});
+                out.push_back(SweepBlock{b.tile, b.x0, b.x1, b.flags | block_lanes(l1, tl_t - l1)});
+            }
+            tab.swap(out);
         }
         std::stable_sort(tab.begin(), tab.end(), [](const SweepBlock& a, const SweepBlock& b) {
             return a.x0 + a.x1 != b.x0 + b.x1 ? a.x0 + a.x1 < b.x0 + b.x1 : a.tile < b.tile;

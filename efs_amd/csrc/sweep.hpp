@@ -82,11 +82,19 @@ struct PmlTable {
 
 // One workgroup of a grid launch: a column tile (256 lanes over
 // contiguous (y, z) columns) and the x planes [x0, x1) it marches over.
+// A block may take part of its tile (4-wide launches): the lanes
+// [first, first + count) of it, with count in flag bits 4-15 (0: the whole
+// tile) and first in bits 16-31.
 struct SweepBlock {
     uint32_t tile, x0, x1;
-    uint32_t flags;  // kBlockUpml: the block contains UPML cells
+    uint32_t flags;  // kBlockUpml: the block contains UPML cells; its lanes of the tile
 };
 constexpr uint32_t kBlockUpml = 1u;
+__host__ __device__ constexpr uint32_t block_lane_first(uint32_t flags) { return flags >> 16; }
+__host__ __device__ constexpr uint32_t block_lane_count(uint32_t flags) { return (flags >> 4) & 0xfffu; }
+__host__ __device__ constexpr uint32_t block_lanes(uint32_t first, uint32_t count) {
+    return (first << 16) | (count << 4);
+}
 
 // Grid launches read their work from a host-built block table: each column
 // tile's x range is cut into chunks of equal bytes moved (UPML cells move
@@ -959,8 +967,9 @@ __device__ __forceinline__ bool interior_lane(const SweepGeom& g, const PmlTable
     const SweepBlock d = g.tab[xcd_remap(bid, g.blocks)];
     flags = d.flags;
     m = 0u;
-    const uint32_t c0 = (d.tile * g.tile_lanes + threadIdx.x) * 4u;
-    if (threadIdx.x >= g.tile_lanes || c0 >= g.cols) return false;
+    const uint32_t n = block_lane_count(d.flags);
+    const uint32_t c0 = (d.tile * g.tile_lanes + block_lane_first(d.flags) + threadIdx.x) * 4u;
+    if (threadIdx.x >= (n ? n : g.tile_lanes) || c0 >= g.cols) return false;
     w.y = c0 / g.nz;
     w.z0 = c0 - w.y * g.nz;
     w.x0 = d.x0;
