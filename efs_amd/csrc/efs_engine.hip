@@ -1009,8 +1009,9 @@ bool box_overlap(const PmlRegion& a, const PmlRegion& b) {
 // stage the UPML coefficient tables).
 // The launch gets ~2048 workgroups (8 per CU on 256 CUs) but no chunk
 // shorter than 2 planes (a chunk re-reads one halo plane): a 256^3 grid
-// gets chunks of 8 planes on average, a grid of a few hundred thousand
-// cells 2-plane chunks (there the march is latency-bound, not HBM-bound).
+// gets chunks of 8 planes on average.  Grids small enough for side work
+// (a few hundred thousand cells) get 1-plane chunks: there the march is
+// latency-bound, not HBM-bound, and a shorter march ends sooner.
 int plan_sweeps(efs_ctx* c) {
     const uint32_t nx = c->nx, ny = c->ny, nz = c->nzp;  // device rows
     const uint64_t cols = (uint64_t)ny * nz;
@@ -1019,7 +1020,10 @@ int plan_sweeps(efs_ctx* c) {
     const double upml_b = upml_in_grid ? core_b + (c->pml_comp ? 27.0 : 60.0) : 2.0;
     const uint64_t want = env_u64("EFS_SWEEP_BLOCKS", 2048);
     const uint32_t min_planes = (uint32_t)std::max<uint64_t>(1, env_u64("EFS_SWEEP_MIN_PLANES", 1));
-    const uint32_t min_chunk = (uint32_t)std::max<uint64_t>(1, env_u64("EFS_SWEEP_MIN_CHUNK", 2));
+    // grids small enough for side work march 1-plane chunks (their blocks
+    // are latency-bound; the x-face chunks keep 2 planes for the Mur pairs)
+    const uint32_t min_chunk =
+        (uint32_t)std::max<uint64_t>(1, env_u64("EFS_SWEEP_MIN_CHUNK", c->side_on ? 1 : 2));
     for (int v = 0; v < 2; ++v) {
         // 4-wide tiles of grids with side work hold whole rows of lanes when
         // that leaves at most 10 % of the threads idle, so that a Mur pair
