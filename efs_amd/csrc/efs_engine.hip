@@ -701,20 +701,18 @@ int plan_epilogue(efs_ctx* c) {
     const uint32_t nx = c->nx, ny = c->ny, nz = c->nz, nzp = c->nzp, tl = c->geom.tile_lanes;
     const auto& tab = c->h_tab4;
     const uint32_t T = (uint32_t)(((uint64_t)ny * nzp / 4 + tl - 1) / tl);
-    // [tile][x] -> table entries: the block of the tile's first lanes, and
-    // the one from lane `split` of the tile on when a block takes part of it
-    std::vector<int32_t> blk((size_t)T * nx, -1), blk2((size_t)T * nx, -1);
-    std::vector<uint32_t> split((size_t)T * nx, 0);
+    // [tile][x] -> the table entry of the block holding the tile's lanes
+    // there; a block that takes part of its tile is listed in sub[tile][x]
+    // with its first lane
+    std::vector<int32_t> blk((size_t)T * nx, -1);
+    std::vector<std::vector<std::pair<uint32_t, int32_t>>> sub((size_t)T * nx);
     for (size_t e = 0; e < tab.size(); ++e)
         for (uint32_t x = tab[e].x0; x < tab[e].x1; ++x) {
             const size_t i = (size_t)tab[e].tile * nx + x;
-            if (block_lane_first(tab[e].flags) == 0) {
-                blk[i] = (int32_t)e;
-            } else {
-                blk2[i] = (int32_t)e;
-                split[i] = block_lane_first(tab[e].flags);
-            }
+            if (block_lane_count(tab[e].flags) == 0) blk[i] = (int32_t)e;
+            else sub[i].push_back({block_lane_first(tab[e].flags), (int32_t)e});
         }
+    for (auto& s : sub) std::sort(s.begin(), s.end());
     auto block_of = [&](uint64_t gidx) -> int32_t {
         const uint64_t lin = gidx % c->N;
         const uint32_t x = (uint32_t)(lin / ((uint64_t)ny * nz));
@@ -722,7 +720,10 @@ int plan_epilogue(efs_ctx* c) {
         const uint32_t y = rem / nz, z = rem - y * nz;
         const uint64_t lane = ((uint64_t)y * nzp + z) / 4;
         const size_t i = (size_t)(lane / tl) * nx + x;
-        return blk2[i] >= 0 && lane % tl >= split[i] ? blk2[i] : blk[i];
+        int32_t b = blk[i];
+        for (const auto& s : sub[i])
+            if (lane % tl >= s.first) b = s.second;
+        return b;
     };
     std::vector<std::vector<uint32_t>> mur(tab.size()), exc(tab.size());
     for (uint32_t k = 0; k < c->mur_n; ++k) {
@@ -1115,27 +1116,36 @@ int plan_sweeps(efs_ctx* c) {
             }
         }
         // with side work the blocks of the x-face chunks end with the Mur of
-        // a whole face plane (sweep_epilogue) and finish last: each takes
-        // half of its tile's rows (the rows 0, 1 and Ny-2, Ny-1 of the y-face
-        // Mur pairs stay together)
+        // a whole face plane (sweep_epilogue) and finish last: they are cut
+        // into EFS_XFACE_PARTS blocks of whole rows of their tile (the rows
+        // 0, 1 and Ny-2, Ny-1 of the y-face Mur pairs stay together)
         const uint32_t lpr = nz / 4;
-        if (v == 0 && c->side_on && nx > 3 && nz % 4 == 0 && tl % lpr == 0 && env_u64("EFS_SPLIT_XFACE", 1)) {
+        const uint32_t parts = (uint32_t)std::min<uint64_t>(16, env_u64("EFS_XFACE_PARTS", 2));
+        if (v == 0 && c->side_on && nx > 3 && nz % 4 == 0 && tl % lpr == 0 && parts > 1) {
             const uint32_t lanes = (uint32_t)(cols / 4);
             std::vector<SweepBlock> out;
             for (const SweepBlock& b : tab) {
                 const uint32_t tl_t = std::min(tl, lanes - b.tile * tl), r0 = b.tile * tl / lpr;
                 const uint32_t rows = tl_t / lpr;
-                uint32_t rs = r0 + rows / 2;
-                if (rs == 1) rs = 2;
-                if (rs == ny - 1) rs = ny - 2;
-                if ((b.x0 != 0 && b.x1 != nx) || b.flags != (b.flags & kBlockUpml) || rows < 2 || rs <= r0 ||
-                    rs >= r0 + rows) {
+                if ((b.x0 != 0 && b.x1 != nx) || b.flags != (b.flags & kBlockUpml) || rows < 2) {
                     out.push_back(b);
                     continue;
                 }
-                const uint32_t l1 = (rs - r0) * lpr;
-                out.push_back(SweepBlock{b.tile, b.x0, b.x1, b.flags | block_lanes(0, l1)});
-                out.push_back(SweepBlock{b.tile, b.x0, b.x1, b.flags | block_lanes(l1, tl_t - l1)});
+                // row boundaries r0 = s_0 < s_1 < ... < s_m = r0 + rows
+                std::vector<uint32_t> cut{r0};
+                for (uint32_t j = 1; j < parts; ++j) {
+                    uint32_t rs = r0 + (uint32_t)((uint64_t)rows * j / parts);
+                    if (rs == 1) rs = 2;
+                    if (rs == ny - 1) rs = ny - 2;
+                    if (rs > cut.back() && rs < r0 + rows) cut.push_back(rs);
+                }
+                cut.push_back(r0 + rows);
+                const uint32_t last = tl_t - (cut[cut.size() - 2] - r0) * lpr;  // the partial row of a last tile
+                for (size_t j = 0; j + 1 < cut.size(); ++j) {
+                    const uint32_t l0 = (cut[j] - r0) * lpr;
+                    const uint32_t n = j + 2 == cut.size() ? last : (cut[j + 1] - cut[j]) * lpr;
+                    out.push_back(SweepBlock{b.tile, b.x0, b.x1, b.flags | (cut.size() > 2 ? block_lanes(l0, n) : 0u)});
+                }
             }
             tab.swap(out);
         }
