@@ -486,6 +486,7 @@ def run_dispersive(args):
     if not g.initGPU():
         raise RuntimeError("no HIP device")
     workloads.configure_dispersive(g, w, storage=args.storage)
+    blocks_scale = g.tuneSweeps(args.tune_reps)
     g.sync()
     t0 = time.perf_counter()
     g.iterate(K)
@@ -559,7 +560,7 @@ def run_dispersive(args):
                                "write-back, TFSF plane wave along +x (Gaussian f0 6 GHz, fc 5 GHz), NF2FF at 3/5/7/9 "
                                "GHz, bistatic RCS cut at 64 angles; whole run timed from zero fields",
                    "grid": w["gridSize"], "dT": w["dT"], "path": sorted(flags), "parallelism": "single GPU",
-                   "ade_cells": [o["numCells"] for o in w["ade"]["orders"]],
+                   "sweep_blocks_scale": blocks_scale, "ade_cells": [o["numCells"] for o in w["ade"]["orders"]],
                    "tfsf_points": [len(w["tfsf"]["voltagePoints"]["field_idx"]),
                                    len(w["tfsf"]["currentPoints"]["field_idx"])]},
         "roofline": {"bound": "hbm", "kernel": dom_name, "achieved": round(b / (dom_ms * 1e-3) / 1e9, 1),
@@ -613,6 +614,7 @@ def run_large(args):
         if not g.initGPU():
             raise RuntimeError("no HIP device")
         slices = workloads.configure_large(g, w, storage=storage, max_samples=K)
+        scale = g.tuneSweeps(args.tune_reps)
         g.sync()
         say(storage, "configured", f"{time.perf_counter() - t_b:.1f} s")
         t0 = time.perf_counter()
@@ -636,7 +638,7 @@ def run_large(args):
         flags = g.pathFlags()
         g.destroy()
         return {"value": n * K / elapsed / 1e6, "elapsed": elapsed, "s11": s11, "dmax": dmax, "kern": kern,
-                "flags": flags}
+                "flags": flags, "blocks_scale": scale}
 
     main = run(args.storage, True)
     extra = {"results": {"Dmax_dBi_at_f0": round(main["dmax"], 3),
@@ -673,7 +675,8 @@ def run_large(args):
                                "boxes (150 dielectric, 50 PEC plates), 4 x 4 array of 50-ohm lumped ports (Gaussian f0 "
                                "10 GHz, fc 8 GHz), NF2FF six-face box at 32 frequencies (K = 64); whole run timed from "
                                "zero fields", "grid": w["gridSize"], "dT": w["dT"], "path": sorted(main["flags"]),
-                   "nf2ff_points": w["nf2ff"]["numPoints"], "parallelism": "single GPU"},
+                   "nf2ff_points": w["nf2ff"]["numPoints"], "sweep_blocks_scale": main["blocks_scale"],
+                   "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "kernel": dom_name, "achieved": round(b / (dom_ms * 1e-3) / 1e9, 1),
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(b / (dom_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
                      "traffic": None, "traffic_source": None, "algorithmic_bytes_per_launch": b,
@@ -801,6 +804,8 @@ def main():
                          "0 skips it)")
     ap.add_argument("--no-compare", action="store_true",
                     help="skip the comparison runs (FP32 speed and FP64 error) of --storage fp16 / batch")
+    ap.add_argument("--tune-reps", type=int, default=5,
+                    help="sweep pairs the sweep-shape tuner times per candidate before the run (0: planner's shape)")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -829,6 +834,7 @@ def main():
     if not eng.initGPU():
         raise RuntimeError("no HIP device")
     workloads.configure_engine(eng, w, storage=args.storage)
+    blocks_scale = eng.tuneSweeps(args.tune_reps)
 
     K, E = args.steps, args.energy_interval
     kb = int(w["nf2ff"]["kBlock"])
@@ -914,7 +920,7 @@ def main():
                          f"10 PEC plates, 50-ohm lumped port, Gaussian f0=5 GHz fc=4 GHz, energy every {E} steps, "
                          f"NF2FF 6-face box {w['nf2ff']['numPoints']} points x 16 frequencies (1-9 GHz)"),
             "grid": w["gridSize"], "pml_cells": w["n_pml_cells"], "dT": w["dT"], "seed": w["seed"],
-            "nf2ff_k_block": w["nf2ff"]["kBlock"], "parallelism": "single GPU",
+            "nf2ff_k_block": w["nf2ff"]["kBlock"], "sweep_blocks_scale": blocks_scale, "parallelism": "single GPU",
         },
         "roofline": {
             "bound": "hbm",

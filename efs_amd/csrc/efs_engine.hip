@@ -199,6 +199,9 @@ struct efs_ctx {
     // E-sweep epilogue (sweep_epilogue, EFS_SWEEP_EPILOGUE): Mur and
     // excitation entries per block-table entry of the 4-wide launch
     std::vector<SweepBlock> h_tab4;
+    double blocks_scale = 1.0;   // factor on the 4-wide launches' workgroup target (efs_tune_sweeps)
+    uint64_t field_floats = 0;   // floats of the volt and curr arrays at the start of the pool
+    uint64_t flux_floats = 0;    // floats of each UPML flux array
     bool ep_ok = false;
     uint32_t *ep_moff = nullptr, *ep_eoff = nullptr, *ep_exc = nullptr;
     uint4* ep_mur = nullptr;
@@ -1010,7 +1013,12 @@ bool box_overlap(const PmlRegion& a, const PmlRegion& b) {
 // stage the UPML coefficient tables).
 // The launch gets ~2048 workgroups (8 per CU on 256 CUs) but no chunk
 // shorter than 2 planes (a chunk re-reads one halo plane): a 256^3 grid
-// gets chunks of 8 planes on average.  Grids small enough for side work
+// gets chunks of 8 planes on average.  No chunk is longer than 48 planes
+// (EFS_SWEEP_MAX_CHUNK) on average: the workgroups of one XCD march side by
+// side over neighbouring tiles and read each other's edge rows from its
+// L2, and over long chunks they drift apart and those reads go to HBM
+// (800 x 800 x 600 in FP16: 2048 chunks of 182 planes 65.6k MCells/s,
+// chunks of ~46 planes 73.8k).  Grids small enough for side work
 // (a few hundred thousand cells) get 1-plane chunks: there the march is
 // latency-bound, not HBM-bound, and a shorter march ends sooner.
 int plan_sweeps(efs_ctx* c) {
@@ -1020,6 +1028,7 @@ int plan_sweeps(efs_ctx* c) {
     const double core_b = c->comp ? 39.0 : c->comp16 ? 42.0 : 60.0;
     const double upml_b = upml_in_grid ? core_b + (c->pml_comp ? 27.0 : 60.0) : 2.0;
     const uint64_t want = env_u64("EFS_SWEEP_BLOCKS", 2048);
+    const uint64_t max_chunk = std::max<uint64_t>(1, env_u64("EFS_SWEEP_MAX_CHUNK", 48));
     const uint32_t min_planes = (uint32_t)std::max<uint64_t>(1, env_u64("EFS_SWEEP_MIN_PLANES", 1));
     // grids small enough for side work march 1-plane chunks (their blocks
     // are latency-bound; the x-face chunks keep 2 planes for the Mur pairs)
@@ -1068,8 +1077,10 @@ int plan_sweeps(efs_ctx* c) {
         // small grids have few column tiles: cut them into more, shorter
         // chunks (down to min_planes planes, default 1) so that the launch
         // still fills the GPU
+        uint64_t want_v = std::max<uint64_t>(want, ((uint64_t)T * nx + max_chunk - 1) / max_chunk);
+        if (v == 0) want_v = std::max<uint64_t>(1, (uint64_t)std::llround((double)want_v * c->blocks_scale));
         const double nblocks =
-            (double)std::max<uint64_t>(1, std::min<uint64_t>(want, (uint64_t)T * std::max(1u, nx / min_planes)));
+            (double)std::max<uint64_t>(1, std::min<uint64_t>(want_v, (uint64_t)T * std::max(1u, nx / min_planes)));
         const double target = total / nblocks;
         std::vector<SweepBlock> tab;
         std::vector<double> pre(nx + 1);
@@ -1824,6 +1835,7 @@ int efs_init(efs_ctx* c, uint32_t nx, uint32_t ny, uint32_t nz, const float* vv,
     HIP_TRY(c, hipMemset(c->pool, 0, (2 * farr + 4 * arr) * sizeof(float)));
     c->volt = c->pool;
     c->curr = c->pool + farr;
+    c->field_floats = 2 * farr;
     c->vv = c->pool + 2 * farr;
     c->vi = c->vv + arr;
     c->ii = c->vv + 2 * arr;
@@ -2118,8 +2130,10 @@ int efs_cfg_pml(efs_ctx* c, uint32_t nreg, const uint32_t* start, const uint32_t
         TRY(dupload(c, c->pml_coef, coef.data(), coef.size()));
         TRY(dzero(c, c->pml_vflux, total * rw(c)));
         TRY(dzero(c, c->pml_cflux, total * rw(c)));
+        c->flux_floats = total * rw(c);
     } else {
         dfree(c->pml_coef); dfree(c->pml_vflux); dfree(c->pml_cflux);
+        c->flux_floats = 0;
     }
     c->pml_tz0 = tz0;
     c->pml_tz1 = tz1;
@@ -2432,6 +2446,119 @@ int efs_iterate(efs_ctx* c, uint32_t n) {
     c->mur_pre_ahead = false;
     for (uint32_t s = 0; s < n; ++s) TRY(launch_step(c, s + 1 < n));
     return side_flush(c);  // no sample or gather stays half done between calls
+}
+
+// The E and H sweeps of one step and nothing else (no point work, no
+// NF2FF capture), on the 4-wide fused path: what efs_tune_sweeps times.
+int launch_sweep_pair(efs_ctx* c) {
+    const bool has_pml = c->pml.n > 0, x16 = c->comp16;
+    const Coef ce{c->vv, c->vi, c->cidx_e, c->comp16 ? c->ctab16_e : c->ctab_e, c->pml_idx, c->pml_tab_e,
+                  c->cidx16_e, c->tabn_e};
+    const Coef ch{c->ii, c->iv, c->cidx_h, c->comp16 ? c->ctab16_h : c->ctab_h, c->pml_idx, c->pml_tab_h,
+                  c->cidx16_h, c->tabn_h};
+    const dim3 grid(c->geom.blocks), blk(kSweepThreads);
+    if (c->half) {
+        auto ke = has_pml ? (c->pml_comp ? (x16 ? sweep_e4fxp_h : c->comp ? sweep_e4fcp_h : sweep_e4fp_h)
+                                         : (x16 ? sweep_e4fx_h : c->comp ? sweep_e4fc_h : sweep_e4f_h))
+                          : (x16 ? sweep_e4x_h : c->comp ? sweep_e4c_h : sweep_e4_h);
+        auto kh = has_pml ? (c->pml_comp ? (x16 ? sweep_h4fxp_h : c->comp ? sweep_h4fcp_h : sweep_h4fp_h)
+                                         : (x16 ? sweep_h4fx_h : c->comp ? sweep_h4fc_h : sweep_h4f_h))
+                          : (x16 ? sweep_h4x_h : c->comp ? sweep_h4c_h : sweep_h4_h);
+        hipLaunchKernelGGL(ke, grid, blk, 0, c->stream, (f16*)c->volt, (const f16*)c->curr, ce, c->geom, c->pml,
+                           c->pml_coef, c->pml_vflux, c->fscale, NfCapture{}, SideWork{});
+        hipLaunchKernelGGL(kh, grid, blk, 0, c->stream, (f16*)c->curr, (const f16*)c->volt, ch, c->geom, c->pml,
+                           c->pml_coef, c->pml_cflux, c->fscale, NfCapture{}, SideWork{});
+    } else {
+        auto ke = has_pml ? (c->pml_comp ? (x16 ? sweep_e4fxp : c->comp ? sweep_e4fcp : sweep_e4fp)
+                                         : (x16 ? sweep_e4fx : c->comp ? sweep_e4fc : sweep_e4f))
+                          : (x16 ? sweep_e4x : c->comp ? sweep_e4c : sweep_e4);
+        auto kh = has_pml ? (c->pml_comp ? (x16 ? sweep_h4fxp : c->comp ? sweep_h4fcp : sweep_h4fp)
+                                         : (x16 ? sweep_h4fx : c->comp ? sweep_h4fc : sweep_h4f))
+                          : (x16 ? sweep_h4x : c->comp ? sweep_h4c : sweep_h4);
+        hipLaunchKernelGGL(ke, grid, blk, 0, c->stream, c->volt, c->curr, ce, c->geom, c->pml, c->pml_coef,
+                           c->pml_vflux, c->fscale, NfCapture{}, SideWork{});
+        hipLaunchKernelGGL(kh, grid, blk, 0, c->stream, c->curr, c->volt, ch, c->geom, c->pml, c->pml_coef,
+                           c->pml_cflux, c->fscale, NfCapture{}, SideWork{});
+    }
+    CHECK_LAUNCH(c);
+    return EFS_OK;
+}
+
+// Sweep-shape tuner.  Candidates: the planner's workgroup target times
+// 1/2, 1, 2, 3 and 4; each is timed over `reps` E + H sweep pairs (after one
+// untimed pair) with HIP events on the engine's stream, and the fastest is
+// kept (the planner's own target unless another is at least 1 % faster).
+// The sweeps run on the live arrays, so the fields and UPML flux are copied
+// aside first and back after: the simulation state is unchanged.  Grids with
+// side work (small grids, latency-bound) and the other paths keep the
+// planner's target.
+int tune_sweeps(efs_ctx* c, uint32_t reps, double* scale_out) {
+    const bool has_pml = c->pml.n > 0;
+    const bool eligible = c->vec4 && !c->dbl && !c->side_on && (!has_pml || (c->pml_split && c->pml_fused));
+    if (!eligible || reps == 0) {
+        if (scale_out) *scale_out = c->blocks_scale;
+        return EFS_OK;
+    }
+    HIP_TRY(c, hipSetDevice(c->device));
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    float *fsave = nullptr, *vsave = nullptr, *csave = nullptr;
+    hipEvent_t ev[2] = {nullptr, nullptr};
+    auto release = [&]() {
+        dfree(fsave); dfree(vsave); dfree(csave);
+        for (hipEvent_t& e : ev)
+            if (e) hipEventDestroy(e);
+    };
+    auto run = [&]() -> int {
+        TRY(dalloc(c, fsave, c->field_floats));
+        HIP_TRY(c, hipMemcpyAsync(fsave, c->pool, c->field_floats * sizeof(float), hipMemcpyDeviceToDevice, c->stream));
+        if (c->flux_floats) {
+            TRY(dalloc(c, vsave, c->flux_floats));
+            TRY(dalloc(c, csave, c->flux_floats));
+            HIP_TRY(c, hipMemcpyAsync(vsave, c->pml_vflux, c->flux_floats * sizeof(float), hipMemcpyDeviceToDevice,
+                                      c->stream));
+            HIP_TRY(c, hipMemcpyAsync(csave, c->pml_cflux, c->flux_floats * sizeof(float), hipMemcpyDeviceToDevice,
+                                      c->stream));
+        }
+        for (hipEvent_t& e : ev) HIP_TRY(c, hipEventCreate(&e));
+        const double cand[] = {1.0, 0.5, 2.0, 3.0, 4.0};
+        double best = 1.0, best_ms = 0.0;
+        for (double s : cand) {
+            c->blocks_scale = s;
+            TRY(plan_sweeps(c));
+            TRY(launch_sweep_pair(c));
+            HIP_TRY(c, hipEventRecord(ev[0], c->stream));
+            for (uint32_t r = 0; r < reps; ++r) TRY(launch_sweep_pair(c));
+            HIP_TRY(c, hipEventRecord(ev[1], c->stream));
+            HIP_TRY(c, hipEventSynchronize(ev[1]));
+            float ms = 0.f;
+            HIP_TRY(c, hipEventElapsedTime(&ms, ev[0], ev[1]));
+            if (s == 1.0 || ms < 0.99 * best_ms) {
+                best = s;
+                best_ms = ms;
+            }
+        }
+        c->blocks_scale = best;
+        TRY(plan_sweeps(c));
+        c->alias_dirty = true;  // lists built on the block table are rebuilt before the next step
+        HIP_TRY(c, hipMemcpyAsync(c->pool, fsave, c->field_floats * sizeof(float), hipMemcpyDeviceToDevice, c->stream));
+        if (c->flux_floats) {
+            HIP_TRY(c, hipMemcpyAsync(c->pml_vflux, vsave, c->flux_floats * sizeof(float), hipMemcpyDeviceToDevice,
+                                      c->stream));
+            HIP_TRY(c, hipMemcpyAsync(c->pml_cflux, csave, c->flux_floats * sizeof(float), hipMemcpyDeviceToDevice,
+                                      c->stream));
+        }
+        HIP_TRY(c, hipStreamSynchronize(c->stream));
+        if (scale_out) *scale_out = best;
+        return EFS_OK;
+    };
+    const int rc = run();
+    release();
+    return rc;
+}
+
+int efs_tune_sweeps(efs_ctx* c, uint32_t reps, double* scale_out) {
+    TRY(need_init(c));
+    return tune_sweeps(c, reps, scale_out);
 }
 
 int efs_sync(efs_ctx* c) {

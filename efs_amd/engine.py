@@ -78,6 +78,7 @@ SIGNATURES = {
     "efs_profile_enable": (C.c_int, [C.c_void_p, C.c_int]),
     "efs_profile_read": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_uint64)]),
     "efs_path_flags": (C.c_int, [C.c_void_p, C.POINTER(C.c_uint32)]),
+    "efs_tune_sweeps": (C.c_int, [C.c_void_p, C.c_uint32, C.POINTER(C.c_double)]),
     "efs_get_sizes": (C.c_int, [C.c_void_p, C.POINTER(EfsSizes)]),
     "efs_set_nf2ff_compensated": (C.c_int, [C.c_void_p, C.c_int]),
     "efs_farfield_noalloc": (C.c_int, [C.c_void_p, C.c_uint32, C.c_void_p, C.c_void_p, C.c_uint32, C.c_void_p,
@@ -568,6 +569,14 @@ class HIPEngine:
         out = np.zeros(6, np.float32)
         self._check(self.lib.efs_field_scales(self.ctx, out))
         return out
+
+    def tuneSweeps(self, reps=5):
+        """Pick the sweep workgroup target by timing (efs_tune_sweeps); the
+        fields are left as they were.  Returns the chosen factor on the
+        planner's target (1.0 where the grid's path is not tuned)."""
+        s = C.c_double(0.0)
+        self._check(self.lib.efs_tune_sweeps(self.ctx, int(reps), C.byref(s)))
+        return s.value
 
     def pathFlags(self):
         """Forms in use: set of 'vec4', 'coef_idx', 'upml4', 'fused', 'upml_idx',

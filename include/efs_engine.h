@@ -284,6 +284,17 @@ enum {
 };
 int efs_path_flags(const efs_ctx *ctx, uint32_t *flags);
 
+/* Sweep-shape tuner (no counterpart in the reference, whose compute passes
+ * have fixed workgroup sizes and dispatch shapes, src/webgpu-engine.mjs:
+ * 7-122, 2407-2504).  Times the E + H sweeps of the configured grid under
+ * the block planner's workgroup target times 1/2, 1, 2, 3 and 4 (reps
+ * timed sweep pairs each) and keeps the fastest for the following steps;
+ * *scale (may be null) receives the chosen factor.  The fields and UPML
+ * flux are restored afterwards, so results do not change.  Grids small
+ * enough for side work, FP64 storage and the split UPML form keep the
+ * planner's target (factor 1).  Call after init and configurePML. */
+int efs_tune_sweeps(efs_ctx *ctx, uint32_t reps, double *scale);
+
 /* Field storage (no counterpart in the reference, whose fields are FP32):
  * EFS_STORAGE_F32 (default); EFS_STORAGE_F16 -- volt and curr held as FP16
  * with a power-of-two scale per component, updated in FP32 registers, the
