@@ -198,6 +198,14 @@ export class HIPEngine {
         addon.iterate(this._h, numSteps >>> 0);
     }
 
+    // Not in the reference: picks the sweeps' workgroup target by timing
+    // (efs_tune_sweeps) and leaves the fields as they were; returns the
+    // chosen factor on the planner's target.
+    tuneSweeps(reps) {
+        this._need();
+        return addon.tuneSweeps(this._h, reps === undefined ? 5 : reps >>> 0);
+    }
+
     async getFields() {
         this._need();
         const volt = new Float32Array(3 * this.totalCells);

@@ -435,6 +435,17 @@ static napi_value js_sync(napi_env env, napi_callback_info info) {
     return status(&c, ctx, efs_sync(ctx));
 }
 
+/* tuneSweeps(h, reps) -> the chosen factor on the planner's workgroup target */
+static napi_value js_tune_sweeps(napi_env env, napi_callback_info info) {
+    BEGIN();
+    CTX(0);
+    uint32_t reps = u32_arg(&c, 1);
+    BAIL();
+    double scale = 1.0;
+    if (!status(&c, ctx, efs_tune_sweeps(ctx, reps, &scale))) return NULL;
+    return num(env, scale);
+}
+
 static napi_value js_num_ts(napi_env env, napi_callback_info info) {
     BEGIN();
     CTX(0);
@@ -895,6 +906,7 @@ static napi_value Init(napi_env env, napi_value exports) {
         FN("ssRead", js_ss_read),
         FN("iterate", js_iterate),
         FN("sync", js_sync),
+        FN("tuneSweeps", js_tune_sweeps),
         FN("numTS", js_num_ts),
         FN("setNumTS", js_set_num_ts),
         FN("getFields", js_get_fields),
