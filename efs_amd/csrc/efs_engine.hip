@@ -215,6 +215,15 @@ struct efs_ctx {
     bool ade_dirty = false;
     bool ade_i_live = false;  // some current-side ADE coefficient is non-zero
     std::vector<uint32_t> h_ti_idx;  // TFSF current points (interface indices)
+    // point work of TFSF and ADE in the launches of other hooks
+    // (plan_point_fusion, redone when a list changes)
+    bool fuse_dirty = true;
+    bool post_v_fuse = false;  // TFSF-V and the ADE write-back touch disjoint E cells
+    bool post_i_fuse = false;  // no TFSF-I cell is read by the NF2FF sampling or the buffered gather
+    bool ade_ahead = false;    // this step's E-side ADE update ran in the previous step's hooks launch
+    bool ade_fuse = false;     // the E-side ADE update may run in the previous step's hooks launch
+    std::vector<uint32_t> h_nf_pts;  // NF2FF points (ix, iy, iz), as configured
+    std::vector<uint32_t> h_gb_idx;  // buffered gather cells (interface indices)
     bool cap_clash = false;          // an H-side hook writes inside the NF2FF capture rows
     bool cap_dirty = true;
     uint32_t ade_n = 0;
@@ -983,6 +992,7 @@ void reset_extensions(efs_ctx* c) {
     c->tfsf_nv = c->tfsf_ni = 0;
     c->h_ti_idx.clear();
     c->cap_dirty = true;
+    c->fuse_dirty = true;
     c->rlc_n = 0;
     c->g_n = 0;
     c->gb_n = 0;
@@ -1370,6 +1380,7 @@ int plan_nf_capture(efs_ctx* c, const std::vector<uint32_t>& pts, uint32_t P) {
     k.slab = c->nf_slab;
     c->nf_cap = k;
     c->cap_dirty = true;
+    c->fuse_dirty = true;
     return EFS_OK;
 }
 
@@ -1390,7 +1401,8 @@ void launch_range_control(efs_ctx* c, uint32_t t) {
 // (null: none due) and, with `pre`, the next step's Mur pre pass, in one
 // launch (k_post_h).
 template <class FV>
-void launch_post_h(efs_ctx* c, FV v, FV i, NfCapture cap, float* ring, bool pre) {
+void launch_post_h(efs_ctx* c, FV v, FV i, NfCapture cap, float* ring, bool pre, bool tfsf_i, bool ade_next,
+                   uint32_t t) {
     PostH<FV> q{};
     if (c->nf_P) {
         q.pts = c->nf_pts;
@@ -1416,7 +1428,96 @@ void launch_post_h(efs_ctx* c, FV v, FV i, NfCapture cap, float* ring, bool pre)
         q.mur_n = c->mur_n;
         nb_pre = nblk(c->mur_n, c->pt);
     }
-    hipLaunchKernelGGL(k_post_h<FV>, dim3(q.nb_sample + q.nb_gather + nb_pre), dim3(c->pt), 0, c->stream, v, i, q);
+    q.nb_pre = nb_pre;
+    if (tfsf_i) {
+        q.t_signal = c->tfsf_signal;
+        q.t_len = c->tfsf_len;
+        q.t_period = c->tfsf_period;
+        q.t_numTS = t;
+        q.t_idx = c->ti_idx;
+        q.t_dint = c->ti_dint;
+        q.t_dfrac = c->ti_dfrac;
+        q.t_amp = c->ti_amp;
+        q.t_n = c->tfsf_ni;
+        q.nb_tfsf = nblk(c->tfsf_ni, c->pt);
+    }
+    uint32_t nb_ade = 0;
+    if (ade_next) {
+        q.a_flux = rp<FV>(c->pml_vflux);
+        q.a_src = c->ade_src_v;
+        q.a_lor = c->ade_lor;
+        q.a_int = c->ade_c[0];
+        q.a_ext = c->ade_c[1];
+        q.a_cl = c->ade_c[2];
+        q.a_A = rp<FV>(c->ade_va);
+        q.a_L = rp<FV>(c->ade_vl);
+        q.a_n = c->ade_n;
+        nb_ade = nblk(c->ade_n, c->pt);
+    }
+    hipLaunchKernelGGL(k_post_h<FV>, dim3(q.nb_sample + q.nb_gather + nb_pre + q.nb_tfsf + nb_ade), dim3(c->pt), 0,
+                       c->stream, v, i, q);
+}
+
+// TFSF on the E side and the ADE write-back in one launch (k_post_v).
+template <class FV>
+void launch_post_v(efs_ctx* c, FV v, uint32_t t) {
+    PostV<FV> q{};
+    q.signal = c->tfsf_signal;
+    q.len = c->tfsf_len;
+    q.period = c->tfsf_period;
+    q.numTS = t;
+    q.t_idx = c->tv_idx;
+    q.t_dint = c->tv_dint;
+    q.t_dfrac = c->tv_dfrac;
+    q.t_amp = c->tv_amp;
+    q.t_n = c->tfsf_nv;
+    q.nb_tfsf = nblk(c->tfsf_nv, c->pt);
+    q.a_gidx = c->ade_gidx;
+    q.a_A = rp<FV>(c->ade_va);
+    q.a_n = c->ade_n;
+    hipLaunchKernelGGL(k_post_v<FV>, dim3(q.nb_tfsf + nblk(c->ade_n, c->pt)), dim3(c->pt), 0, c->stream, v, q);
+}
+
+// Which TFSF and ADE point work may share the launches of other hooks:
+// TFSF-V with the ADE write-back when no E cell is in both lists (their
+// adds then commute exactly); TFSF-I with the NF2FF sampling and the
+// buffered gather when none of the H cells those read is a TFSF-I cell.
+void plan_point_fusion(efs_ctx* c) {
+    c->post_v_fuse = c->post_i_fuse = c->ade_fuse = false;
+    c->fuse_dirty = false;
+    if (!c->fuse_hooks || env_u64("EFS_FUSE_POINTS", 1) == 0) return;
+    c->ade_fuse = true;
+    // cell sets as bitmaps over the 3N interface indices (linear-time
+    // checks: the first iterate call after a change pays for them)
+    std::vector<bool> mark;
+    auto fill = [&](const std::vector<uint32_t>& v) {
+        mark.assign(3 * c->N, false);
+        for (uint32_t g : v)
+            if (g < 3 * c->N) mark[g] = true;
+    };
+    if (c->tfsf_nv && c->ade_n) {
+        fill(c->h_tv_idx);
+        bool both = false;
+        for (size_t k = 0; k < c->h_ade_gidx.size() && !both; ++k)
+            both = c->h_ade_gidx[k] < 3 * c->N && mark[c->h_ade_gidx[k]];
+        c->post_v_fuse = !both;
+    }
+    if (c->tfsf_ni) {
+        fill(c->h_ti_idx);
+        auto hit = [&](uint64_t g) { return g < 3 * c->N && mark[g]; };
+        bool clash = false;
+        const uint64_t sx = (uint64_t)c->ny * c->nz, sy = c->nz;
+        // the H values of a sampled point: component n at the point and one
+        // cell further along axis n
+        for (size_t p = 0; c->nf_P && !clash && p < c->h_nf_pts.size() / 3; ++p) {
+            const uint64_t b = c->h_nf_pts[3 * p] * sx + c->h_nf_pts[3 * p + 1] * sy + c->h_nf_pts[3 * p + 2];
+            const uint64_t step[3] = {sx, sy, 1};
+            for (uint32_t n = 0; n < 3 && !clash; ++n)
+                clash = hit(n * c->N + b) || hit(n * c->N + b + step[n]);
+        }
+        for (size_t k = 0; c->gb_n && !clash && k < c->h_gb_idx.size(); ++k) clash = hit(c->h_gb_idx[k]);
+        c->post_i_fuse = !clash;
+    }
 }
 
 // Side work of the H sweep of step t (SideWork): the E half of the NF2FF
@@ -1514,10 +1615,11 @@ int launch_step(efs_ctx* c, bool next_follows) {
                            c->ss_gidx, c->ss_n, c->ss_period, c->ss_current, c->ss_e, c->ss_e + c->ss_n);
         c->ss_current++;
     }
-    if (c->ade_n) {
+    if (c->ade_n && !c->ade_ahead) {
         if (c->dbl) launch_ade_update(c, fv64(c, 0), 0);
         else launch_ade_update(c, fv(c, 0), 0);
     }
+    c->ade_ahead = false;
     if (c->mur_n && !c->mur_pre_ahead) {
         if (c->dbl) launch_mur_pre(c, fv64(c, 0));
         else launch_mur_pre(c, fv(c, 0));
@@ -1575,8 +1677,14 @@ int launch_step(efs_ctx* c, bool next_follows) {
                                c->vv, c->vi, c->geom1, c->pml, c->pml_coef, c->pml_vflux);
         }
     }
-    // post-voltage / apply-to-voltages
-    if (c->tfsf_nv) {
+    // post-voltage / apply-to-voltages; TFSF and the ADE write-back share
+    // a launch when they are the only work here and touch disjoint cells
+    const bool fuse_v = c->post_v_fuse && c->tfsf_nv && c->ade_n && c->ade_apply && c->pbc_e.empty() &&
+                        !c->mur_n && !c->exc_n && !c->rlc_n;
+    if (fuse_v) {
+        if (c->dbl) launch_post_v(c, fv64(c, 0), t);
+        else launch_post_v(c, fv(c, 0), t);
+    } else if (c->tfsf_nv) {
         if (c->dbl) launch_tfsf(c, fv64(c, 0), 0, t);
         else launch_tfsf(c, fv(c, 0), 0, t);
     }
@@ -1596,7 +1704,7 @@ int launch_step(efs_ctx* c, bool next_follows) {
                 else launch_mur_post_some(c, fv(c, 0));
             }
         }
-        if (c->ade_n && c->ade_apply) {
+        if (c->ade_n && c->ade_apply && !fuse_v) {
             if (c->dbl) launch_ade_apply(c, fv64(c, 0), 0);
             else launch_ade_apply(c, fv(c, 0), 0);
         }
@@ -1670,8 +1778,17 @@ int launch_step(efs_ctx* c, bool next_follows) {
                                c->ii, c->iv, c->geom1, c->pml, c->pml_coef, c->pml_cflux);
         }
     }
-    // post-current
-    if (c->tfsf_ni) {
+    // post-current: TFSF here, or in the hooks launch below (fuse_i)
+    const bool hooks_fusable = !use_side && c->fuse_hooks && !range_due;
+    bool fuse_i = hooks_fusable && c->tfsf_ni && c->post_i_fuse && c->pbc_h.empty() &&
+                  !(c->ade_n && c->ade_apply && c->ade_i_live) && !c->pad_hi_n;
+    // the next step's E-side ADE update reads E only, which nothing after
+    // this step's write-back changes before that step's E sweep
+    bool ade_next = hooks_fusable && next_follows && c->ade_n && c->ade_fuse;
+    const int parts = (c->nf_P ? 1 : 0) + (gather_due ? 1 : 0) + (pre_ahead ? 1 : 0) + (fuse_i ? 1 : 0) +
+                      (ade_next ? 1 : 0);
+    if (parts < 2) fuse_i = ade_next = false;
+    if (c->tfsf_ni && !fuse_i) {
         if (c->dbl) launch_tfsf(c, fv64(c, 1), 1, t);
         else launch_tfsf(c, fv(c, 1), 1, t);
     }
@@ -1691,17 +1808,18 @@ int launch_step(efs_ctx* c, bool next_follows) {
         c->numTS++;
         return EFS_OK;
     }
-    // NF2FF sampling, the probe gather and the next step's Mur pre pass in
-    // one launch; not when the FP16 range control (which rescales the
-    // fields) comes between them
-    const int parts = (c->nf_P ? 1 : 0) + (gather_due ? 1 : 0) + (pre_ahead ? 1 : 0);
+    // NF2FF sampling, the probe gather, the next step's Mur pre pass, TFSF
+    // of the H side and the next step's E-side ADE update in one launch;
+    // not when the FP16 range control (which rescales the fields) comes
+    // between them
     if (c->fuse_hooks && !range_due && parts >= 2) {
         if (c->nf_P && c->nf_slot == 0) c->nf_ts0 = t + 1;
         float* ring = gather_due ? c->gb_ring + (size_t)c->gb_count * 2 * c->gb_n : nullptr;
-        if (c->dbl) launch_post_h(c, fv64(c, 0), fv64(c, 1), NfCapture{}, ring, pre_ahead);
-        else launch_post_h(c, fv(c, 0), fv(c, 1), nf_capture(c), ring, pre_ahead);
+        if (c->dbl) launch_post_h(c, fv64(c, 0), fv64(c, 1), NfCapture{}, ring, pre_ahead, fuse_i, ade_next, t);
+        else launch_post_h(c, fv(c, 0), fv(c, 1), nf_capture(c), ring, pre_ahead, fuse_i, ade_next, t);
         if (gather_due) c->gb_count++;
         c->mur_pre_ahead = pre_ahead;
+        c->ade_ahead = ade_next;
         if (c->nf_P) {
             c->nf_slot++;
             if (c->nf_slot == c->nf_K) TRY(nf2ff_fold(c));
@@ -2247,6 +2365,7 @@ int efs_cfg_ade_clear(efs_ctx* c) {
     c->ade_dirty = false;
     c->alias_dirty = true;
     c->cap_dirty = true;
+    c->fuse_dirty = true;
     return EFS_OK;
 }
 
@@ -2276,6 +2395,7 @@ int efs_cfg_ade_add(efs_ctx* c, uint32_t has_lorentz, uint32_t dir, uint32_t nce
     }
     c->ade_dirty = true;
     c->cap_dirty = true;
+    c->fuse_dirty = true;
     return EFS_OK;
 }
 
@@ -2285,6 +2405,7 @@ int efs_set_ade_apply(efs_ctx* c, int apply) {
     c->ade_apply = apply != 0;
     c->alias_dirty = true;
     c->cap_dirty = true;
+    c->fuse_dirty = true;
     return EFS_OK;
 }
 
@@ -2326,6 +2447,7 @@ int efs_cfg_tfsf(efs_ctx* c, const float* signal, uint32_t len, uint32_t period,
     c->h_ti_idx.assign(i_idx, i_idx + ni);
     c->h_tv_idx.assign(v_idx, v_idx + nv);
     c->cap_dirty = true;
+    c->fuse_dirty = true;
     if (nv || ni) TRY(source_scales(c, signal, len, v_amp, nv, i_amp, ni));
     return EFS_OK;
 }
@@ -2444,6 +2566,8 @@ int efs_iterate(efs_ctx* c, uint32_t n) {
     }
     if (n) c->fields_zero = false;
     c->mur_pre_ahead = false;
+    c->ade_ahead = false;
+    if (c->fuse_dirty) plan_point_fusion(c);
     for (uint32_t s = 0; s < n; ++s) TRY(launch_step(c, s + 1 < n));
     return side_flush(c);  // no sample or gather stays half done between calls
 }
@@ -2736,6 +2860,8 @@ int efs_cfg_gather_buffered(efs_ctx* c, const uint32_t* idx, uint32_t n, uint32_
     HIP_TRY(c, hipStreamSynchronize(c->stream));
     const std::vector<uint32_t> t = tr_all(c, idx, n);
     TRY(dupload(c, c->gb_idx, t.data(), n));
+    c->h_gb_idx.assign(idx, idx + n);
+    c->fuse_dirty = true;
     TRY(dzero(c, c->gb_ring, (size_t)max_samples * 2 * n));
     c->gb_n = n;
     c->gb_max = max_samples;
@@ -2799,6 +2925,8 @@ int efs_cfg_nf2ff(efs_ctx* c, const uint32_t* points, uint32_t P, const double* 
     for (uint32_t q = 0; q < P; ++q)
         for (int a = 0; a < 3; ++a) sorted[3 * (size_t)q + a] = points[3 * (size_t)perm[q] + a];
     TRY(dupload(c, c->nf_pts, sorted.data(), sorted.size()));
+    c->h_nf_pts = sorted;
+    c->fuse_dirty = true;
     TRY(dupload(c, c->nf_perm_d, perm.data(), perm.size()));
     c->nf_perm = std::move(perm);
     TRY(plan_nf_capture(c, sorted, P));
@@ -3143,7 +3271,8 @@ int efs_path_flags(const efs_ctx* c, uint32_t* flags) {
              (c->dbl ? EFS_PATH_F64 : 0u) |
              (c->side_on && c->fuse_hooks && c->vec4 && !c->dbl && (!c->pml.n || c->pml_split) ? EFS_PATH_SIDE_WORK
                                                                                                 : 0u) |
-             (c->ep_ok ? EFS_PATH_SWEEP_EPILOGUE : 0u);
+             (c->ep_ok ? EFS_PATH_SWEEP_EPILOGUE : 0u) |
+             (c->post_v_fuse ? EFS_PATH_FUSE_TFSF_ADE : 0u) | (c->post_i_fuse ? EFS_PATH_FUSE_TFSF_HOOKS : 0u);
     return EFS_OK;
 }
 

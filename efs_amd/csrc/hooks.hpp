@@ -128,12 +128,28 @@ struct PostH {
     const uint32_t* gidx;
     float* gout;
     uint32_t gn, nb_gather;
-    // the next step's Mur pre pass, the rest
+    // the next step's Mur pre pass, the next nb_pre workgroups
     const R* vflux;
     const float* coeff;
     const uint32_t *normal_src, *shifted_src;
     R* saved;
-    uint32_t mur_n;
+    uint32_t mur_n, nb_pre;
+    // TFSF on the H side (k_tfsf's arguments), the next nb_tfsf workgroups:
+    // H cells that the sampling and the gather of this launch do not read
+    // (the host checks)
+    const float* t_signal;
+    uint32_t t_len, t_period, t_numTS;
+    const uint32_t *t_idx, *t_dint;
+    const float *t_dfrac, *t_amp;
+    uint32_t t_n, nb_tfsf;
+    // the next step's ADE update of the E side (k_ade_update's arguments),
+    // the rest: it reads E, which nothing in this launch writes
+    const R* a_flux;
+    const uint32_t* a_src;
+    const uint8_t* a_lor;
+    const float *a_int, *a_ext, *a_cl;
+    R *a_A, *a_L;
+    uint32_t a_n;
 };
 
 template <class FV>
@@ -151,8 +167,50 @@ __global__ void k_post_h(FV volt, FV curr, PostH<FV> q) {
         return;
     }
     b -= q.nb_gather;
+    if (b < q.nb_pre) {
+        const uint32_t k = b * blockDim.x + threadIdx.x;
+        if (k < q.mur_n) mur_pre_one(volt, q.vflux, q.coeff, q.normal_src, q.shifted_src, q.saved, k);
+        return;
+    }
+    b -= q.nb_pre;
+    if (b < q.nb_tfsf) {
+        const uint32_t k = b * blockDim.x + threadIdx.x;
+        if (k < q.t_n)
+            tfsf_one(curr, q.t_signal, q.t_len, q.t_period, q.t_idx, q.t_dint, q.t_dfrac, q.t_amp, q.t_numTS, k);
+        return;
+    }
+    b -= q.nb_tfsf;
     const uint32_t k = b * blockDim.x + threadIdx.x;
-    if (k < q.mur_n) mur_pre_one(volt, q.vflux, q.coeff, q.normal_src, q.shifted_src, q.saved, k);
+    if (k < q.a_n) ade_update_one(volt, q.a_flux, q.a_src, q.a_lor, q.a_int, q.a_ext, q.a_cl, q.a_A, q.a_L, k);
+}
+
+// After the E sweep, when the step's only voltage-side work is TFSF and
+// the ADE write-back and their cells are disjoint (the host checks): both
+// in one launch, the TFSF entries in the first nb_tfsf workgroups.
+template <class FV>
+struct PostV {
+    typedef typename FV::R R;
+    const float* signal;
+    uint32_t len, period, numTS;
+    const uint32_t *t_idx, *t_dint;
+    const float *t_dfrac, *t_amp;
+    uint32_t t_n, nb_tfsf;
+    const uint32_t* a_gidx;
+    const R* a_A;
+    uint32_t a_n;
+};
+
+template <class FV>
+__global__ void k_post_v(FV volt, PostV<FV> q) {
+    uint32_t b = blockIdx.x;
+    if (b < q.nb_tfsf) {
+        const uint32_t k = b * blockDim.x + threadIdx.x;
+        if (k < q.t_n) tfsf_one(volt, q.signal, q.len, q.period, q.t_idx, q.t_dint, q.t_dfrac, q.t_amp, q.numTS, k);
+        return;
+    }
+    b -= q.nb_tfsf;
+    const uint32_t k = b * blockDim.x + threadIdx.x;
+    if (k < q.a_n) volt.add(q.a_gidx[k], -q.a_A[k]);
 }
 
 // Point work run by extra workgroups of the sweep kernels (FP32 / FP16

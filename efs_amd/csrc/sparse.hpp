@@ -108,14 +108,12 @@ __global__ void k_mur_finish(FV volt, const float* __restrict__ coeff, const uin
 // The state A, Lr and the flux are in the arithmetic type R (double in the
 // FP64 storage form); the coefficients stay the FP32 values given.
 template <class FV>
-__global__ void k_ade_update(FV field, const typename FV::R* __restrict__ flux,
-                             const uint32_t* __restrict__ src, const uint8_t* __restrict__ lorentz,
-                             const float* __restrict__ c_int, const float* __restrict__ c_ext,
-                             const float* __restrict__ c_lor, typename FV::R* __restrict__ A,
-                             typename FV::R* __restrict__ Lr, uint32_t n) {
+__device__ __forceinline__ void ade_update_one(const FV& field, const typename FV::R* __restrict__ flux,
+                                               const uint32_t* __restrict__ src, const uint8_t* __restrict__ lorentz,
+                                               const float* __restrict__ c_int, const float* __restrict__ c_ext,
+                                               const float* __restrict__ c_lor, typename FV::R* __restrict__ A,
+                                               typename FV::R* __restrict__ Lr, uint32_t k) {
     typedef typename FV::R R;
-    const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
-    if (k >= n) return;
     const R V = read_alias(field, flux, src[k]);
     if (lorentz[k]) {
         const R l = Lr[k] + (R)c_lor[k] * A[k];
@@ -124,6 +122,17 @@ __global__ void k_ade_update(FV field, const typename FV::R* __restrict__ flux,
     } else {
         A[k] = (R)c_int[k] * A[k] + (R)c_ext[k] * V;
     }
+}
+
+template <class FV>
+__global__ void k_ade_update(FV field, const typename FV::R* __restrict__ flux,
+                             const uint32_t* __restrict__ src, const uint8_t* __restrict__ lorentz,
+                             const float* __restrict__ c_int, const float* __restrict__ c_ext,
+                             const float* __restrict__ c_lor, typename FV::R* __restrict__ A,
+                             typename FV::R* __restrict__ Lr, uint32_t n) {
+    const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    ade_update_one(field, flux, src, lorentz, c_int, c_ext, c_lor, A, Lr, k);
 }
 
 // openEMS write-back of the ADE state (optional): field -= A.
@@ -139,13 +148,11 @@ __global__ void k_ade_apply(FV field, const uint32_t* __restrict__ gidx, const t
 // between signal[d] and signal[d+1]; skipped before the delay and at or
 // past the last sample.
 template <class FV>
-__global__ void k_tfsf(FV field, const float* __restrict__ signal, uint32_t len,
-                       uint32_t period, const uint32_t* __restrict__ gidx, const uint32_t* __restrict__ dint,
-                       const float* __restrict__ dfrac, const float* __restrict__ amp, uint32_t n,
-                       uint32_t numTS) {
+__device__ __forceinline__ void tfsf_one(const FV& field, const float* __restrict__ signal, uint32_t len,
+                                         uint32_t period, const uint32_t* __restrict__ gidx,
+                                         const uint32_t* __restrict__ dint, const float* __restrict__ dfrac,
+                                         const float* __restrict__ amp, uint32_t numTS, uint32_t k) {
     typedef typename FV::R R;
-    const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
-    if (k >= n) return;
     int64_t d = (int64_t)numTS - (int64_t)dint[k];
     if (d < 0) return;
     if (period > 0) d %= (int64_t)period;
@@ -153,6 +160,16 @@ __global__ void k_tfsf(FV field, const float* __restrict__ signal, uint32_t len,
     const R delta = dfrac[k];
     const R sig = ((R)1 - delta) * (R)signal[d] + delta * (R)signal[d + 1];
     field.add(gidx[k], (R)amp[k] * sig);
+}
+
+template <class FV>
+__global__ void k_tfsf(FV field, const float* __restrict__ signal, uint32_t len,
+                       uint32_t period, const uint32_t* __restrict__ gidx, const uint32_t* __restrict__ dint,
+                       const float* __restrict__ dfrac, const float* __restrict__ amp, uint32_t n,
+                       uint32_t numTS) {
+    const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    tfsf_one(field, signal, len, period, gidx, dint, dfrac, amp, numTS, k);
 }
 
 // Lumped RLC, CPU-engine order: history shift + parallel inductor current

@@ -580,11 +580,12 @@ class HIPEngine:
 
     def pathFlags(self):
         """Forms in use: set of 'vec4', 'coef_idx', 'upml4', 'fused', 'upml_idx',
-        'f16', 'nf2ff_capture', 'coef_idx16', 'f64', 'side_work', 'sweep_epilogue'."""
+        'f16', 'nf2ff_capture', 'coef_idx16', 'f64', 'side_work', 'sweep_epilogue', 'fuse_tfsf_ade',
+        'fuse_tfsf_hooks'."""
         f = C.c_uint32(0)
         self._check(self.lib.efs_path_flags(self.ctx, C.byref(f)))
         names = ("vec4", "coef_idx", "upml4", "fused", "upml_idx", "f16", "nf2ff_capture", "coef_idx16", "f64",
-                 "side_work", "sweep_epilogue")
+                 "side_work", "sweep_epilogue", "fuse_tfsf_ade", "fuse_tfsf_hooks")
         return {n for b, n in enumerate(names) if f.value & (1 << b)}
 
 

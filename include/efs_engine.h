@@ -278,9 +278,15 @@ enum {
     EFS_PATH_F64 = 256,          /* FP64 field storage and arithmetic (1-wide sweeps) */
     EFS_PATH_SIDE_WORK = 512,    /* NF2FF sampling, probe gather and Mur pre pass run in extra
                                     workgroups of the sweep kernels (EFS_SIDE_WORK) */
-    EFS_PATH_SWEEP_EPILOGUE = 1024 /* Mur post/apply and the excitation run at the end of the E-sweep
-                                      blocks (EFS_SWEEP_EPILOGUE; set once an iterate call has
-                                      planned it) */
+    EFS_PATH_SWEEP_EPILOGUE = 1024, /* Mur post/apply and the excitation run at the end of the E-sweep
+                                       blocks (EFS_SWEEP_EPILOGUE; set once an iterate call has
+                                       planned it) */
+    EFS_PATH_FUSE_TFSF_ADE = 2048,  /* TFSF of the E side and the ADE write-back share a launch
+                                       (disjoint cells; EFS_FUSE_POINTS; set once an iterate call
+                                       has planned it) */
+    EFS_PATH_FUSE_TFSF_HOOKS = 4096 /* TFSF of the H side may run in the launch of the NF2FF
+                                       sampling and gather (cells they do not read), beside the next
+                                       step's E-side ADE update */
 };
 int efs_path_flags(const efs_ctx *ctx, uint32_t *flags);
 

@@ -35,6 +35,52 @@ def HIPEngine():
     return engine.HIPEngine
 
 
+def run_dispersive_chunks(HIPEngine, w, chunks, fused):
+    from efs_amd import workloads
+    g = HIPEngine()
+    assert g.initGPU()
+    workloads.configure_dispersive(g, w)
+    for n in chunks:
+        g.iterate(n)
+    flags = g.pathFlags()
+    assert ("fuse_tfsf_ade" in flags) == fused and ("fuse_tfsf_hooks" in flags) == fused, flags
+    assert "side_work" not in flags
+    f = g.getFields()
+    acc = g.readNF2FFAccumulation()
+    g.destroy()
+    return f, acc
+
+
+def test_fused_tfsf_ade_launches_are_bit_identical(HIPEngine, oracle_mod, monkeypatch):
+    """TFSF-E with the ADE write-back in one launch, TFSF-H with the NF2FF
+    sampling and the next step's ADE update in another (the path of grids
+    without side work, forced here on 40^3), against the separate kernels
+    (EFS_FUSE_POINTS=0): the same bits, over iterate calls of 1, 7, 64
+    and 128 steps (the ADE update that runs ahead must not be repeated or
+    lost at a call boundary); and within 1e-5 of the FP32 oracle."""
+    from efs_amd import workloads
+    w = workloads.dispersive_scatter(40, pml=6)
+    chunks = (1, 7, 64, 128)
+    monkeypatch.setenv("EFS_SIDE_WORK", "0")
+    monkeypatch.setenv("EFS_FUSE_POINTS", "1")
+    fa, aa = run_dispersive_chunks(HIPEngine, w, chunks, True)
+    monkeypatch.setenv("EFS_FUSE_POINTS", "0")
+    fb, ab = run_dispersive_chunks(HIPEngine, w, chunks, False)
+    for k in ("volt", "curr"):
+        np.testing.assert_array_equal(fa[k], fb[k])
+    for k in ("accumE", "accumH"):
+        np.testing.assert_array_equal(np.asarray(aa[k]), np.asarray(ab[k]))
+    o = oracle_mod.Oracle(w["gridSize"], w["coefficients"], fp64=False)
+    workloads.configure_dispersive(o, w)
+    o.iterate(sum(chunks))
+    ref = o.getFields()
+    assert np.abs(ref["volt"]).max() > 1e-5
+    for k in ("volt", "curr"):
+        e = rel_l2(fa[k], ref[k])
+        say(f"fused dispersive 40^3 vs FP32 oracle: {k} {e:.2e}")
+        assert e <= 1e-5, k
+
+
 def test_dispersive_matches_oracle(HIPEngine, oracle_mod):
     from efs_amd import workloads
     w = workloads.dispersive_scatter(48, pml=6)
