@@ -33,6 +33,13 @@
 #include <string.h>
 #ifdef _OPENMP
 #include <omp.h>
+
+/* Loops over fewer cells than this run on one thread: on small grids a
+ * thread team per loop costs more than the loop (each cell's result is the
+ * same either way). */
+#ifndef OMP_MIN_CELLS
+#define OMP_MIN_CELLS 65536
+#endif
 #endif
 
 #if ORC_F64
@@ -256,7 +263,7 @@ void PFX(update_voltages)(orc *e) {
     const long Nx = e->nx, Ny = e->ny, Nz = e->nz;
     real *volt = e->volt;
     const real *curr = e->curr, *vv = e->vv, *vi = e->vi;
-#pragma omp parallel for schedule(static)
+#pragma omp parallel for schedule(static) if (Nx * Ny * Nz >= OMP_MIN_CELLS)
     for (long x = 0; x < Nx; ++x) {
         const long sX = x > 0 ? 1 : 0;
         for (long y = 0; y < Ny; ++y) {
@@ -289,7 +296,7 @@ void PFX(update_currents)(orc *e) {
     const long Nx = e->nx, Ny = e->ny, Nz = e->nz;
     real *curr = e->curr;
     const real *volt = e->volt, *ii = e->ii, *iv = e->iv;
-#pragma omp parallel for schedule(static)
+#pragma omp parallel for schedule(static) if (Nx * Ny * Nz >= OMP_MIN_CELLS)
     for (long x = 0; x < Nx - 1; ++x) {
         for (long y = 0; y < Ny - 1; ++y) {
             for (long z = 0; z < Nz - 1; ++z) {
@@ -376,7 +383,7 @@ void PFX(pml_step)(orc *e, int mode) {
         pml_region *R = &e->pml[r];
         const long nx = R->num[0], ny = R->num[1], nz = R->num[2];
         const size_t cs = (size_t)nx * ny * nz;
-#pragma omp parallel for schedule(static)
+#pragma omp parallel for schedule(static) if (cs >= OMP_MIN_CELLS)
         for (long lx = 0; lx < nx; ++lx) {
             for (long ly = 0; ly < ny; ++ly) {
                 for (long lz = 0; lz < nz; ++lz) {
@@ -703,7 +710,7 @@ void PFX(nf2ff_accumulate)(orc *e, uint32_t ts) {
     if (e->nf_window == 1 && e->nf_maxTS > 1) w = 0.5 * (1.0 - cos(2.0 * M_PI * ts / (double)(e->nf_maxTS - 1)));
     const double wdT = w * e->nf_wnorm * e->nf_dT;
     const size_t P6 = 6 * (size_t)e->nf_P;
-#pragma omp parallel for schedule(static)
+#pragma omp parallel for schedule(static) if (e->nf_P >= OMP_MIN_CELLS / 16)
     for (long p = 0; p < (long)e->nf_P; ++p) {
         double E[3], H[3];
         PFX(nf2ff_interp)(e, e->nf_points[3 * p], e->nf_points[3 * p + 1], e->nf_points[3 * p + 2], E, H);
@@ -840,7 +847,7 @@ double PFX(energy)(const orc *e) {
     const long Nx = e->nx, Ny = e->ny, Nz = e->nz;
     double total = 0;
     double *plane = (double *)xcalloc(Nx > 1 ? Nx - 1 : 1, sizeof(double));
-#pragma omp parallel for schedule(static)
+#pragma omp parallel for schedule(static) if (Nx * Ny * Nz >= OMP_MIN_CELLS)
     for (long x = 0; x < Nx - 1; ++x) {
         double s = 0;
         for (long y = 0; y < Ny - 1; ++y)
