@@ -296,7 +296,8 @@ int efs_path_flags(const efs_ctx *ctx, uint32_t *flags);
  * the block planner's workgroup target times 1/2, 1, 2, 3 and 4 (reps
  * timed sweep pairs each) and keeps the fastest for the following steps;
  * *scale (may be null) receives the chosen factor.  The fields and UPML
- * flux are restored afterwards, so results do not change.  Grids small
+ * flux are copied aside for the call (device memory of their size, freed
+ * before it returns) and restored afterwards, so results do not change.  Grids small
  * enough for side work, FP64 storage and the split UPML form keep the
  * planner's target (factor 1).  Call after init and configurePML. */
 int efs_tune_sweeps(efs_ctx *ctx, uint32_t reps, double *scale);
