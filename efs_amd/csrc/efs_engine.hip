@@ -2627,6 +2627,8 @@ int tune_sweeps(efs_ctx* c, uint32_t reps, double* scale_out) {
     HIP_TRY(c, hipStreamSynchronize(c->stream));
     float *fsave = nullptr, *vsave = nullptr, *csave = nullptr;
     hipEvent_t ev[2] = {nullptr, nullptr};
+    const double scale0 = c->blocks_scale;
+    bool swept = false;  // the live arrays have been stepped (a failure must put them back)
     auto release = [&]() {
         dfree(fsave); dfree(vsave); dfree(csave);
         for (hipEvent_t& e : ev)
@@ -2649,6 +2651,7 @@ int tune_sweeps(efs_ctx* c, uint32_t reps, double* scale_out) {
         for (double s : cand) {
             c->blocks_scale = s;
             TRY(plan_sweeps(c));
+            swept = true;
             TRY(launch_sweep_pair(c));
             HIP_TRY(c, hipEventRecord(ev[0], c->stream));
             for (uint32_t r = 0; r < reps; ++r) TRY(launch_sweep_pair(c));
@@ -2672,10 +2675,21 @@ int tune_sweeps(efs_ctx* c, uint32_t reps, double* scale_out) {
                                       c->stream));
         }
         HIP_TRY(c, hipStreamSynchronize(c->stream));
+        swept = false;
         if (scale_out) *scale_out = best;
         return EFS_OK;
     };
     const int rc = run();
+    if (rc != EFS_OK && swept) {
+        // back to the planner's shape and the saved state (best effort: the
+        // error that got here is the one reported)
+        c->blocks_scale = scale0;
+        plan_sweeps(c);
+        hipStreamSynchronize(c->stream);
+        if (fsave) hipMemcpy(c->pool, fsave, c->field_floats * sizeof(float), hipMemcpyDeviceToDevice);
+        if (vsave) hipMemcpy(c->pml_vflux, vsave, c->flux_floats * sizeof(float), hipMemcpyDeviceToDevice);
+        if (csave) hipMemcpy(c->pml_cflux, csave, c->flux_floats * sizeof(float), hipMemcpyDeviceToDevice);
+    }
     release();
     return rc;
 }
